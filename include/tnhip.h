@@ -1,0 +1,184 @@
+/*
+ * tnhip.h -- C ABI of libtnhip.so, the MI355X (gfx950) effective-Hamiltonian
+ * hot path of TenNetLib.jl (two-site DMRG / TDVP on MPS).
+ *
+ * Every entry point returns an int status: TN_OK (0) or a negative TN_E_*
+ * code; tn_last_error() returns the message of the last failure on a context
+ * (or of the last failure of the calling thread when ctx == NULL).
+ *
+ * Ownership: the library owns every device buffer it allocates for
+ * environments, MPS tensors and workspaces.  Host pointers are borrowed for
+ * the duration of a call.  Device pointers handed in (phi, v, out) are
+ * caller-owned buffers from tn_alloc()/tn_free().
+ *
+ * Threading: one tn_ctx = one HIP stream; a context and the environments
+ * created on it are not thread-safe.  Separate contexts may run concurrently.
+ *
+ * Index conventions (row-major, contiguous):
+ *   MPS site      A_k[a, s, b]          (chi_{k-1}, d_k, chi_k)
+ *   MPO tensor    W_k[wl, s', s, wr]    (w_{k-1}, d_k, d_k, w_k), s' = bra
+ *   two-site phi  phi[a, s, t, b]       (chi_{p-1}, d_p, d_{p+1}, chi_{p+1})
+ *   one-site phi  phi[a, s, b]
+ * Sites and bonds are 1-based as in the reference.
+ *
+ * Reference interfaces replaced (TenNetLib.jl 0.2.2 unless marked upstream):
+ *   StateEnvs(psi, H::MPO)         src/mps/state_envs.jl:54-60
+ *   position!(sysenv, pos)         src/mps/state_envs.jl:364-367
+ *   set_nsite!/nsite               src/mps/state_envs.jl:341,352-355
+ *   product(sysenv, v)/sysenv(v)   src/mps/state_envs.jl:376-378
+ *   ProjMPO makeL!/makeR!/product  upstream ITensorMPS; mirror src/mps/projmps2.jl:77-173
+ *   eig_solver / exp_solver        src/base/solver.jl:23-43, 66-88
+ *   replacebond!                   upstream ITensorMPS, called src/mps/update_site.jl:64-76
+ *   _update_two_site!              src/mps/update_site.jl:27-90
+ *   fullsweep!                     src/mps/sweep.jl:94-196
+ */
+#ifndef TNHIP_H
+#define TNHIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define TN_OK 0
+#define TN_E_INVALID (-1)      /* bad argument / parameter validation       */
+#define TN_E_SHAPE (-2)        /* index or order mismatch (projmps2.jl:208-222) */
+#define TN_E_POSITION (-3)     /* wrong position / orthocenter (update_site.jl:39-41) */
+#define TN_E_NOMEM (-4)        /* device allocation failed                  */
+#define TN_E_HIP (-5)          /* HIP runtime error                         */
+#define TN_E_NOTCONV (-6)      /* solver not converged when asked (solver.jl:38-41) */
+#define TN_E_UNSUPPORTED (-7)  /* feature not built                         */
+#define TN_E_NUMERIC (-8)      /* NaN/Inf detected                           */
+
+/* ---- element types and matvec precision ----------------------------- */
+#define TN_F64 0   /* real float64 storage                                  */
+#define TN_C128 1  /* complex float64 storage (interleaved re, im)          */
+
+#define TN_PREC_FP64 0    /* FP64 MFMA GEMM chain                          */
+#define TN_PREC_FP32 1    /* FP32 MFMA GEMM chain, FP64 in/out             */
+#define TN_PREC_BF16X3 2  /* bf16 hi/lo split, 3 MFMA products, FP32 accum   */
+
+typedef struct tn_ctx tn_ctx;
+typedef struct tn_env tn_env;
+
+typedef struct tn_krylov_info {
+  int converged;    /* number of converged Ritz pairs (eig) / 1 if done (exp) */
+  int numops;       /* operator applications                                  */
+  int numiter;      /* restarts + 1                                           */
+  double residual;  /* residual norm of the returned Ritz pair / error est.  */
+} tn_krylov_info;
+
+/* Keyword arguments of update_position!/fullsweep! (update_site.jl:235-245,
+ * solver.jl:27-35, 72-79).  Fill with tn_update_params_default(). */
+typedef struct tn_update_params {
+  int solver;          /* 0 = eig_solver, 1 = exp_solver                     */
+  double time_step_re; /* exp_solver time step (already halved by tdvpsweep!) */
+  double time_step_im;
+  int normalize;       /* normalize = true                                    */
+  long maxdim;         /* typemax(Int)                                        */
+  long mindim;         /* 1                                                   */
+  double cutoff;       /* Float64_threshold() = 1e-15                         */
+  int reverse_step;    /* TDVP one-site backward step                         */
+  int krylovdim;       /* 5 (eig) / 30 (exp)                                  */
+  int maxiter;         /* 2 (eig) / 100 (exp)                                 */
+  double tol;          /* 1e-14 (eig) / 1e-12 (exp)                           */
+  int check_convergence;
+  int precision;       /* TN_PREC_*: precision of the Krylov matvecs          */
+  int refine_fp64;     /* energy from one FP64 matvec of the converged vector */
+} tn_update_params;
+
+typedef struct tn_update_result {
+  double energy;       /* Ritz value, or Rayleigh quotient (update_site.jl:51-57) */
+  double energy_im;
+  double truncerr;     /* spec.truncerr                                        */
+  long nkept;          /* new bond dimension                                   */
+  tn_krylov_info info;
+  double* eigs;        /* in: caller buffer (or NULL) for spec.eigs = kept P   */
+  long eigs_cap;       /* in: capacity of eigs                                 */
+} tn_update_result;
+
+/* ---- context and device memory --------------------------------------- */
+int tn_ctx_create(int device, tn_ctx** out);
+int tn_ctx_destroy(tn_ctx* ctx);
+const char* tn_last_error(const tn_ctx* ctx);
+int tn_ctx_synchronize(tn_ctx* ctx);
+int tn_alloc(tn_ctx* ctx, size_t bytes, void** dptr);
+int tn_free(tn_ctx* ctx, void* dptr);
+int tn_memcpy_h2d(tn_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int tn_memcpy_d2h(tn_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+/* per-phase device time accumulated on the context's stream (ms):
+ * [0] matvec, [1] env update, [2] Krylov vector ops, [3] SVD/replacebond */
+int tn_ctx_timers(tn_ctx* ctx, double* ms4, int reset);
+/* turn per-phase event timing on (1) or off (0); off by default */
+int tn_ctx_enable_timing(tn_ctx* ctx, int on);
+void tn_update_params_default(tn_update_params* p, int solver);
+
+/* ---- StateEnvs{ProjMPO}: MPS + cached environments -------------------- */
+/* N sites, physical dims d[0..N-1], MPO bond dims w[0..N] (w[0] = w[N] = 1),
+ * element type TN_F64 / TN_C128.  StateEnvs(psi, H) state_envs.jl:54-60. */
+int tn_env_create(tn_ctx* ctx, int N, const int* d, const int* w, int dtype, tn_env** out);
+int tn_env_destroy(tn_env* env);
+/* W_host: real (w_{k-1}, d, d, w_k); k is 1-based. */
+int tn_env_set_mpo(tn_env* env, int k, const double* W_host);
+/* A_host: (chil, d_k, chir) of the env's dtype; invalidates cached envs that
+ * contain site k (ProjMPO lpos/rpos semantics). */
+int tn_env_set_site(tn_env* env, int k, const void* A_host, long chil, long chir);
+int tn_env_get_site_dims(tn_env* env, int k, long* chil, long* chir);
+int tn_env_get_site(tn_env* env, int k, void* A_host);
+/* orthocenter bookkeeping of the MPS (0 = unknown / not orthogonal) */
+int tn_env_set_orthocenter(tn_env* env, int k);
+int tn_env_orthocenter(tn_env* env, int* k);
+/* set_nsite!/nsite (state_envs.jl:341,352) */
+int tn_env_set_nsite(tn_env* env, int nsite);
+int tn_env_nsite(tn_env* env, int* nsite);
+/* position!(sysenv, pos) -> makeL!(pos-1), makeR!(pos+nsite) */
+int tn_env_position(tn_env* env, int pos);
+/* drop cached environments (lpos = 0, rpos = N+1) */
+int tn_env_reset(tn_env* env);
+/* shape of the projected-operator domain at the current position:
+ * dims[0..3] = (chil, d_p, d_{p+1} or 1, chir); *n = element count */
+int tn_env_phi_dims(tn_env* env, long* dims4, long* n);
+/* product(sysenv, v) at the current position and nsite; v, out on device */
+int tn_env_matvec(tn_env* env, const void* v_dev, void* out_dev, int precision);
+/* phi = A_pos * A_{pos+1} (update_site.jl:46) into phi_dev */
+int tn_env_contract_phi(tn_env* env, int pos, void* phi_dev);
+/* orthogonalize!(psi, site): exact (untruncated) gauge move on the device */
+int tn_env_orthogonalize(tn_env* env, int site);
+
+/* ---- Krylov solvers (solver.jl) --------------------------------------- */
+int tn_krylov_eig(tn_env* env, void* phi_dev, int krylovdim, int maxiter, double tol,
+                  int precision, double* eig, tn_krylov_info* info);
+int tn_krylov_expm(tn_env* env, void* phi_dev, double t_re, double t_im, int krylovdim,
+                   int maxiter, double tol, int precision, tn_krylov_info* info);
+/* <phi|H|phi> from one FP64 matvec (real part in e[0], imaginary in e[1]) */
+int tn_rayleigh_fp64(tn_env* env, const void* phi_dev, double* e2);
+
+/* ---- bond replacement and truncated SVD -------------------------------- */
+/* replacebond!(psi, b, phi; maxdim, mindim, cutoff, ortho, normalize):
+ * eigs_out receives the kept weights P = s^2 (capacity eigs_cap). */
+int tn_replacebond(tn_env* env, int b, const void* phi_dev, int ortho_left, long maxdim,
+                   long mindim, double cutoff, int normalize, double* truncerr,
+                   double* eigs_out, long eigs_cap, long* nkept);
+/* Standalone truncated SVD of a real m x n row-major device matrix:
+ * M = U S Vt with U (m x k) and Vt (k x n) row-major on the device, S on the
+ * host (capacity min(m,n)).  Truncation rule of NDTensors truncate!!. */
+int tn_svd_trunc(tn_ctx* ctx, const double* M_dev, long m, long n, long maxdim, long mindim,
+                 double cutoff, double* U_dev, double* S_host, double* Vt_dev, long* nkept,
+                 double* truncerr);
+
+/* ---- fused updates ---------------------------------------------------- */
+/* update_position!(sysenv, solver, pos, 2, ortho; kwargs...) */
+int tn_update_two_site(tn_env* env, int pos, int ortho_left, const tn_update_params* p,
+                       tn_update_result* r);
+/* fullsweep!(sysenv, solver, 2, swdata; kwargs...): energy of the last
+ * update, max truncation error, max bond dim; lasteigs of the mid bond
+ * N/2 (right half-sweep) in mid_eigs (capacity cap, count in *nmid). */
+int tn_fullsweep(tn_env* env, const tn_update_params* p, double* energy, double* maxtruncerr,
+                 long* maxchi, double* mid_eigs, long cap, long* nmid);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TNHIP_H */

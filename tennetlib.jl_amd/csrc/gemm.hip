@@ -1,0 +1,585 @@
+// MFMA GEMM kernels for gfx950 (CDNA4), row-major, batched, optional
+// transposes.  Three arithmetic forms:
+//   F64    : v_mfma_f64_16x16x4_f64, FP64 accumulate
+//   F32    : v_mfma_f32_32x32x2_f32, FP32 accumulate
+//   BF16X3 : each FP32 operand split as a = hi + lo (hi = bf16(a), lo = bf16(a - hi));
+//            C += hi*hi + hi*lo + lo*hi with v_mfma_f32_32x32x16_bf16, FP32 accumulate
+//            (split-precision scheme of Henry, Tang & Heinecke 2019).
+//   Z64    : complex FP64 (interleaved), four real f64 MFMAs per complex product.
+// Tiles are staged global -> registers -> LDS with one tile of register
+// prefetch; a workgroup is 4 wavefronts (2 x 2) of 64 lanes.
+#include "tn_internal.h"
+
+namespace tn {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+struct GArgs {
+  long M, N, K;
+  const void* A; long lda; long sA; int tA; int cA;
+  const void* B; long ldb; long sB; int tB; int cB;
+  void* C; long ldc; long sC;
+  double alpha, beta;
+};
+
+// XCD-aware tile order: consecutive workgroup ids are dealt round-robin to
+// the 8 XCDs, so remap the linear id so that each XCD gets a contiguous run
+// of output tiles (shared A/B panels stay in that XCD's L2).
+__device__ __forceinline__ void tile_coords(int ntm, int ntn, int& tm, int& tn) {
+  const int nt = ntm * ntn;
+  int id = blockIdx.x;
+  if (nt >= 64 && (nt % 8) == 0) {
+    const int per = nt / 8;
+    id = (id % 8) * per + (id / 8);
+  }
+  // group rows of tiles (GROUP_M = 8) for L2 reuse of B panels
+  const int GM = 8;
+  const int group = id / (GM * ntn);
+  const int first_m = group * GM;
+  const int gsz = min(ntm - first_m, GM);
+  const int in_group = id % (GM * ntn);
+  tm = first_m + (in_group % gsz);
+  tn = in_group / gsz;
+}
+
+// ------------------------------------------------------------------------
+// FP64 / FP32 kernel: LDS tiles As[BK][BM+PAD], Bs[BK][BN+PAD]
+// ------------------------------------------------------------------------
+template <typename T, int BM, int BN, int BK>
+struct RealTile {
+  static constexpr int NT = 256;
+  static constexpr int PAD = (sizeof(T) == 8) ? 2 : 4;
+  static constexpr int AE = BM * BK / NT;  // A elements per thread
+  static constexpr int BE = BK * BN / NT;
+};
+
+template <typename T>
+__device__ __forceinline__ T ldg(const T* p) { return *p; }
+
+template <typename T, int BM, int BN, int BK>
+__device__ __forceinline__ void load_tiles(const GArgs& a, const T* __restrict__ A,
+                                           const T* __restrict__ B, long m0, long n0, long k0,
+                                           T (&ra)[RealTile<T, BM, BN, BK>::AE],
+                                           T (&rb)[RealTile<T, BM, BN, BK>::BE]) {
+  using TT = RealTile<T, BM, BN, BK>;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < TT::AE; ++i) {
+    const int idx = tid + i * TT::NT;
+    int mm, kk;
+    if (!a.tA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+    const long m = m0 + mm, k = k0 + kk;
+    T v = T(0);
+    if (m < a.M && k < a.K) v = a.tA ? A[k * a.lda + m] : A[m * a.lda + k];
+    ra[i] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < TT::BE; ++i) {
+    const int idx = tid + i * TT::NT;
+    int nn, kk;
+    if (!a.tB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
+    const long n = n0 + nn, k = k0 + kk;
+    T v = T(0);
+    if (n < a.N && k < a.K) v = a.tB ? B[n * a.ldb + k] : B[k * a.ldb + n];
+    rb[i] = v;
+  }
+}
+
+template <typename T, int BM, int BN, int BK>
+__device__ __forceinline__ void store_tiles(const GArgs& a, T* As, T* Bs,
+                                            const T (&ra)[RealTile<T, BM, BN, BK>::AE],
+                                            const T (&rb)[RealTile<T, BM, BN, BK>::BE]) {
+  using TT = RealTile<T, BM, BN, BK>;
+  constexpr int LA = BM + TT::PAD, LB = BN + TT::PAD;
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < TT::AE; ++i) {
+    const int idx = tid + i * TT::NT;
+    int mm, kk;
+    if (!a.tA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+    As[kk * LA + mm] = ra[i];
+  }
+#pragma unroll
+  for (int i = 0; i < TT::BE; ++i) {
+    const int idx = tid + i * TT::NT;
+    int nn, kk;
+    if (!a.tB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
+    Bs[kk * LB + nn] = rb[i];
+  }
+}
+
+// FP64: wave tile (BM/2) x (BN/2) made of 16x16 MFMA blocks
+template <int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_f64_kernel(GArgs a) {
+  using T = double;
+  using TT = RealTile<T, BM, BN, BK>;
+  constexpr int LA = BM + TT::PAD, LB = BN + TT::PAD;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  __shared__ T As[BK * LA];
+  __shared__ T Bs[BK * LB];
+
+  const int ntm = (int)((a.M + BM - 1) / BM), ntn = (int)((a.N + BN - 1) / BN);
+  int tm, tn;
+  tile_coords(ntm, ntn, tm, tn);
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const long bz = blockIdx.z;
+  const T* A = reinterpret_cast<const T*>(a.A) + bz * a.sA;
+  const T* B = reinterpret_cast<const T*>(a.B) + bz * a.sB;
+  T* C = reinterpret_cast<T*>(a.C) + bz * a.sC;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+
+  T ra[TT::AE], rb[TT::BE];
+  load_tiles<T, BM, BN, BK>(a, A, B, m0, n0, 0, ra, rb);
+  for (long k0 = 0; k0 < a.K; k0 += BK) {
+    __syncthreads();
+    store_tiles<T, BM, BN, BK>(a, As, Bs, ra, rb);
+    __syncthreads();
+    if (k0 + BK < a.K) load_tiles<T, BM, BN, BK>(a, A, B, m0, n0, k0 + BK, ra, rb);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      double fa[FM], fb[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) fa[i] = As[kr * LA + wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[j] = Bs[kr * LB + wn * WTN + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each block
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long m = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const long n = n0 + wn * WTN + j * 16 + (lane & 15);
+        if (m < a.M && n < a.N) {
+          double v = a.alpha * acc[i][j][r];
+          if (a.beta != 0.0) v += a.beta * C[m * a.ldc + n];
+          C[m * a.ldc + n] = v;
+        }
+      }
+}
+
+// FP32: wave tile (BM/2) x (BN/2) made of 32x32 MFMA blocks
+template <int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_f32_kernel(GArgs a) {
+  using T = float;
+  using TT = RealTile<T, BM, BN, BK>;
+  constexpr int LA = BM + TT::PAD, LB = BN + TT::PAD;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int FM = WTM / 32, FN = WTN / 32;
+  __shared__ T As[BK * LA];
+  __shared__ T Bs[BK * LB];
+
+  const int ntm = (int)((a.M + BM - 1) / BM), ntn = (int)((a.N + BN - 1) / BN);
+  int tm, tn;
+  tile_coords(ntm, ntn, tm, tn);
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const long bz = blockIdx.z;
+  const T* A = reinterpret_cast<const T*>(a.A) + bz * a.sA;
+  const T* B = reinterpret_cast<const T*>(a.B) + bz * a.sB;
+  T* C = reinterpret_cast<T*>(a.C) + bz * a.sC;
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  f16v acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f16v{};
+
+  T ra[TT::AE], rb[TT::BE];
+  load_tiles<T, BM, BN, BK>(a, A, B, m0, n0, 0, ra, rb);
+  for (long k0 = 0; k0 < a.K; k0 += BK) {
+    __syncthreads();
+    store_tiles<T, BM, BN, BK>(a, As, Bs, ra, rb);
+    __syncthreads();
+    if (k0 + BK < a.K) load_tiles<T, BM, BN, BK>(a, A, B, m0, n0, k0 + BK, ra, rb);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 2) {
+      const int kr = kk + (lane >> 5);
+      float fa[FM], fb[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) fa[i] = As[kr * LA + wm * WTM + i * 32 + (lane & 31)];
+#pragma unroll
+      for (int j = 0; j < FN; ++j) fb[j] = Bs[kr * LB + wn * WTN + j * 32 + (lane & 31)];
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  // epilogue: register r of lane -> row 8*(r/4) + 4*(lane/32) + r%4, column lane%32
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const long m = m0 + wm * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long n = n0 + wn * WTN + j * 32 + (lane & 31);
+        if (m < a.M && n < a.N) {
+          float v = (float)a.alpha * acc[i][j][r];
+          if (a.beta != 0.0) v += (float)a.beta * C[m * a.ldc + n];
+          C[m * a.ldc + n] = v;
+        }
+      }
+}
+
+// ------------------------------------------------------------------------
+// BF16x3: LDS tiles hold bf16 hi/lo parts with k contiguous per row:
+// Ah[BM][BK+8], Al[BM][BK+8], Bh[BN][BK+8], Bl[BN][BK+8].
+// ------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) {
+  return __uint_as_float(((unsigned int)h) << 16);
+}
+
+template <int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_bf16x3_kernel(GArgs a) {
+  constexpr int NT = 256;
+  constexpr int LK = BK + 8;  // row pitch in bf16 elements (16-byte aligned)
+  constexpr int AE = BM * BK / NT, BE = BK * BN / NT;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int FM = WTM / 32, FN = WTN / 32;
+  __shared__ __attribute__((aligned(16))) unsigned short Ah[BM * LK];
+  __shared__ __attribute__((aligned(16))) unsigned short Al[BM * LK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bh[BN * LK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bl[BN * LK];
+
+  const int ntm = (int)((a.M + BM - 1) / BM), ntn = (int)((a.N + BN - 1) / BN);
+  int tm, tn;
+  tile_coords(ntm, ntn, tm, tn);
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const long bz = blockIdx.z;
+  const float* A = reinterpret_cast<const float*>(a.A) + bz * a.sA;
+  const float* B = reinterpret_cast<const float*>(a.B) + bz * a.sB;
+  float* C = reinterpret_cast<float*>(a.C) + bz * a.sC;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  f16v acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = f16v{};
+
+  float ra[AE], rb[BE];
+  auto gload = [&](long k0) {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) {
+      const int idx = tid + i * NT;
+      int mm, kk;
+      if (!a.tA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+      const long m = m0 + mm, k = k0 + kk;
+      float v = 0.f;
+      if (m < a.M && k < a.K) v = a.tA ? A[k * a.lda + m] : A[m * a.lda + k];
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      const int idx = tid + i * NT;
+      int nn, kk;
+      if (!a.tB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
+      const long n = n0 + nn, k = k0 + kk;
+      float v = 0.f;
+      if (n < a.N && k < a.K) v = a.tB ? B[n * a.ldb + k] : B[k * a.ldb + n];
+      rb[i] = v;
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) {
+      const int idx = tid + i * NT;
+      int mm, kk;
+      if (!a.tA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+      const unsigned short h = f2bf_rne(ra[i]);
+      Ah[mm * LK + kk] = h;
+      Al[mm * LK + kk] = f2bf_rne(ra[i] - bf2f(h));
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      const int idx = tid + i * NT;
+      int nn, kk;
+      if (!a.tB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
+      const unsigned short h = f2bf_rne(rb[i]);
+      Bh[nn * LK + kk] = h;
+      Bl[nn * LK + kk] = f2bf_rne(rb[i] - bf2f(h));
+    }
+  };
+
+  gload(0);
+  for (long k0 = 0; k0 < a.K; k0 += BK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + BK < a.K) gload(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      const int kc = kk + 8 * (lane >> 5);
+      bf8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        const int row = wm * WTM + i * 32 + (lane & 31);
+        us8 h = *reinterpret_cast<const us8*>(&Ah[row * LK + kc]);
+        us8 l = *reinterpret_cast<const us8*>(&Al[row * LK + kc]);
+        ah[i] = __builtin_bit_cast(bf8, h);
+        al[i] = __builtin_bit_cast(bf8, l);
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        const int col = wn * WTN + j * 32 + (lane & 31);
+        us8 h = *reinterpret_cast<const us8*>(&Bh[col * LK + kc]);
+        us8 l = *reinterpret_cast<const us8*>(&Bl[col * LK + kc]);
+        bh[j] = __builtin_bit_cast(bf8, h);
+        bl[j] = __builtin_bit_cast(bf8, l);
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const long m = m0 + wm * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long n = n0 + wn * WTN + j * 32 + (lane & 31);
+        if (m < a.M && n < a.N) {
+          float v = (float)a.alpha * acc[i][j][r];
+          if (a.beta != 0.0) v += (float)a.beta * C[m * a.ldc + n];
+          C[m * a.ldc + n] = v;
+        }
+      }
+}
+
+// ------------------------------------------------------------------------
+// Complex FP64 (interleaved re/im).  LDS holds re and im planes; each complex
+// multiply-accumulate is four f64 MFMAs into re/im accumulators.
+// ------------------------------------------------------------------------
+template <int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_z64_kernel(GArgs a) {
+  constexpr int NT = 256;
+  constexpr int LA = BM + 2, LB = BN + 2;
+  constexpr int AE = BM * BK / NT, BE = BK * BN / NT;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  __shared__ double Ar[BK * LA], Ai[BK * LA];
+  __shared__ double Br[BK * LB], Bi[BK * LB];
+
+  const int ntm = (int)((a.M + BM - 1) / BM), ntn = (int)((a.N + BN - 1) / BN);
+  int tm, tn;
+  tile_coords(ntm, ntn, tm, tn);
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const long bz = blockIdx.z;
+  const double2* A = reinterpret_cast<const double2*>(a.A) + bz * a.sA;
+  const double2* B = reinterpret_cast<const double2*>(a.B) + bz * a.sB;
+  double2* C = reinterpret_cast<double2*>(a.C) + bz * a.sC;
+  const double sa = a.cA ? -1.0 : 1.0, sb = a.cB ? -1.0 : 1.0;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  d4 accr[FM][FN], acci[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) { accr[i][j] = d4{0, 0, 0, 0}; acci[i][j] = d4{0, 0, 0, 0}; }
+
+  double2 ra[AE], rb[BE];
+  auto gload = [&](long k0) {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) {
+      const int idx = tid + i * NT;
+      int mm, kk;
+      if (!a.tA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+      const long m = m0 + mm, k = k0 + kk;
+      double2 v = make_double2(0.0, 0.0);
+      if (m < a.M && k < a.K) v = a.tA ? A[k * a.lda + m] : A[m * a.lda + k];
+      ra[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      const int idx = tid + i * NT;
+      int nn, kk;
+      if (!a.tB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
+      const long n = n0 + nn, k = k0 + kk;
+      double2 v = make_double2(0.0, 0.0);
+      if (n < a.N && k < a.K) v = a.tB ? B[n * a.ldb + k] : B[k * a.ldb + n];
+      rb[i] = v;
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < AE; ++i) {
+      const int idx = tid + i * NT;
+      int mm, kk;
+      if (!a.tA) { kk = idx % BK; mm = idx / BK; } else { mm = idx % BM; kk = idx / BM; }
+      Ar[kk * LA + mm] = ra[i].x;
+      Ai[kk * LA + mm] = sa * ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < BE; ++i) {
+      const int idx = tid + i * NT;
+      int nn, kk;
+      if (!a.tB) { nn = idx % BN; kk = idx / BN; } else { kk = idx % BK; nn = idx / BK; }
+      Br[kk * LB + nn] = rb[i].x;
+      Bi[kk * LB + nn] = sb * rb[i].y;
+    }
+  };
+
+  gload(0);
+  for (long k0 = 0; k0 < a.K; k0 += BK) {
+    __syncthreads();
+    sstore();
+    __syncthreads();
+    if (k0 + BK < a.K) gload(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      double far_[FM], fai[FM], fbr[FN], fbi[FN], fbin[FN];
+#pragma unroll
+      for (int i = 0; i < FM; ++i) {
+        far_[i] = Ar[kr * LA + wm * WTM + i * 16 + (lane & 15)];
+        fai[i] = Ai[kr * LA + wm * WTM + i * 16 + (lane & 15)];
+      }
+#pragma unroll
+      for (int j = 0; j < FN; ++j) {
+        fbr[j] = Br[kr * LB + wn * WTN + j * 16 + (lane & 15)];
+        fbi[j] = Bi[kr * LB + wn * WTN + j * 16 + (lane & 15)];
+        fbin[j] = -fbi[j];
+      }
+#pragma unroll
+      for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(far_[i], fbr[j], accr[i][j], 0, 0, 0);
+          accr[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fai[i], fbin[j], accr[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(far_[i], fbi[j], acci[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fai[i], fbr[j], acci[i][j], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long m = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const long n = n0 + wn * WTN + j * 16 + (lane & 15);
+        if (m < a.M && n < a.N) {
+          double vr = a.alpha * accr[i][j][r];
+          double vi = a.alpha * acci[i][j][r];
+          if (a.beta != 0.0) {
+            vr += a.beta * C[m * a.ldc + n].x;
+            vi += a.beta * C[m * a.ldc + n].y;
+          }
+          C[m * a.ldc + n] = make_double2(vr, vi);
+        }
+      }
+}
+
+template <int BM, int BN>
+dim3 grid_for(const GemmDesc& g) {
+  const long ntm = (g.M + BM - 1) / BM, ntn = (g.N + BN - 1) / BN;
+  return dim3((unsigned)(ntm * ntn), 1, (unsigned)g.batch);
+}
+
+GArgs to_args(const GemmDesc& g) {
+  GArgs a;
+  a.M = g.M; a.N = g.N; a.K = g.K;
+  a.A = g.A; a.lda = g.lda; a.sA = g.sA; a.tA = g.tA; a.cA = g.cA;
+  a.B = g.B; a.ldb = g.ldb; a.sB = g.sB; a.tB = g.tB; a.cB = g.cB;
+  a.C = g.C; a.ldc = g.ldc; a.sC = g.sC;
+  a.alpha = g.alpha; a.beta = g.beta;
+  return a;
+}
+
+}  // namespace
+
+void gemm(tn_ctx* c, Prec p, const GemmDesc& g) {
+  TN_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.batch >= 1, TN_E_INVALID, "gemm: bad sizes");
+  if (g.M == 0 || g.N == 0) return;
+  TN_REQUIRE(g.A && g.B && g.C, TN_E_INVALID, "gemm: null operand");
+  TN_REQUIRE(g.batch <= 65535, TN_E_INVALID, "gemm: batch too large");
+  GArgs a = to_args(g);
+  const bool small = (g.M * g.N <= 256L * 256L);
+  switch (p) {
+    case Prec::F64:
+      if (small) {
+        dim3 gr = grid_for<64, 64>(g);
+        hipLaunchKernelGGL((gemm_f64_kernel<64, 64, 16>), gr, dim3(256), 0, c->stream, a);
+      } else {
+        dim3 gr = grid_for<128, 128>(g);
+        hipLaunchKernelGGL((gemm_f64_kernel<128, 128, 16>), gr, dim3(256), 0, c->stream, a);
+      }
+      break;
+    case Prec::F32:
+      if (small) {
+        dim3 gr = grid_for<64, 64>(g);
+        hipLaunchKernelGGL((gemm_f32_kernel<64, 64, 16>), gr, dim3(256), 0, c->stream, a);
+      } else {
+        dim3 gr = grid_for<128, 128>(g);
+        hipLaunchKernelGGL((gemm_f32_kernel<128, 128, 16>), gr, dim3(256), 0, c->stream, a);
+      }
+      break;
+    case Prec::BF16X3:
+      if (small) {
+        dim3 gr = grid_for<64, 64>(g);
+        hipLaunchKernelGGL((gemm_bf16x3_kernel<64, 64, 32>), gr, dim3(256), 0, c->stream, a);
+      } else {
+        dim3 gr = grid_for<128, 128>(g);
+        hipLaunchKernelGGL((gemm_bf16x3_kernel<128, 128, 32>), gr, dim3(256), 0, c->stream, a);
+      }
+      break;
+    case Prec::Z64: {
+      dim3 gr = grid_for<64, 64>(g);
+      hipLaunchKernelGGL((gemm_z64_kernel<64, 64, 8>), gr, dim3(256), 0, c->stream, a);
+      break;
+    }
+  }
+  TN_HIP(hipGetLastError());
+}
+
+}  // namespace tn

@@ -1,0 +1,392 @@
+// Memory-bound kernels of the hot path: the small MPO contraction ("W step")
+// between the two big GEMMs of a matvec / environment update, precision
+// conversions, the FP64 Krylov vector operations (fixed-order two-stage
+// reductions, no atomics, so results are reproducible run to run) and the
+// gather / transpose kernels used by the truncated SVD.
+#include "tn_internal.h"
+
+#include <cmath>
+
+namespace tn {
+namespace {
+
+constexpr int RED_BLOCKS = 1024;  // first-stage reduction blocks (4 per CU)
+constexpr int RED_THREADS = 256;
+constexpr int MAX_DOT = 32;       // basis vectors per multi-dot pass
+
+// ---------------------------------------------------------------------------
+// W step: out[g', A, S', B] = sum_nz val * in[g, A, S, B]
+// block = 64 B-columns x 4 row groups; the (g S) x 64 input slab of one A is
+// staged in LDS, each wavefront walks a quarter of the CSR output rows (rows
+// are wave-uniform, so the CSR entries come through the scalar cache).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void wstep_kernel(int G, int Gp, int D, int Dp, long nA, long nB,
+                                                    const T* __restrict__ in, long in_sg, long in_sA,
+                                                    long in_sS, T* __restrict__ out, long out_sg,
+                                                    long out_sA, long out_sS,
+                                                    const int* __restrict__ rowptr,
+                                                    const int* __restrict__ colidx,
+                                                    const double* __restrict__ vals) {
+  extern __shared__ unsigned char smem_raw[];
+  T* slab = reinterpret_cast<T*>(smem_raw);  // [K][64]
+  const int K = G * D;
+  const int tb = threadIdx.x & 63;
+  const int ty = threadIdx.x >> 6;
+  const long A = blockIdx.y;
+  const long B = (long)blockIdx.x * 64 + tb;
+  const bool valid = B < nB;
+  for (int i = ty; i < K; i += 4) {
+    const int g = i / D, S = i % D;
+    T v = T(0);
+    if (valid) v = in[g * in_sg + A * in_sA + S * in_sS + B];
+    slab[i * 64 + tb] = v;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int rows = Gp * Dp;
+  for (int o = ty; o < rows; o += 4) {
+    const int b = rowptr[o], e = rowptr[o + 1];
+    T acc = T(0);
+    for (int z = b; z < e; ++z) acc += (T)vals[z] * slab[colidx[z] * 64 + tb];
+    const int gp = o / Dp, Sp = o % Dp;
+    out[gp * out_sg + A * out_sA + Sp * out_sS + B] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// conversions / fills
+// ---------------------------------------------------------------------------
+__global__ void cvt_d2f_kernel(const double* __restrict__ x, float* __restrict__ y, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+__global__ void cvt_f2d_kernel(const float* __restrict__ x, double* __restrict__ y, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = (double)x[i];
+}
+__global__ void fill_kernel(double* __restrict__ x, double v, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = v;
+}
+__global__ void scaled_copy_kernel(double* __restrict__ y, const double* __restrict__ x, double a, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
+
+// ---------------------------------------------------------------------------
+// Krylov: multi-dot (first stage + fixed-order second stage)
+// part[v * nblk + blk] = partial sum of <basis_v, x> over the block's slice.
+// Complex: data are interleaved (re, im); <u, x> = sum conj(u) x -> two parts.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(RED_THREADS) void multidot_stage1(const double* __restrict__ basis, long ld,
+                                                               int count, const double* __restrict__ x,
+                                                               long n, double* __restrict__ part) {
+  __shared__ double red[RED_THREADS / 64][2 * MAX_DOT];
+  const int nblk = gridDim.x;
+  double acc[2 * MAX_DOT];
+#pragma unroll
+  for (int v = 0; v < 2 * MAX_DOT; ++v) acc[v] = 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (!CPLX) {
+      const double xi = x[i];
+#pragma unroll
+      for (int v = 0; v < MAX_DOT; ++v)
+        if (v < count) acc[v] = fma(basis[v * ld + i], xi, acc[v]);
+    } else {
+      const double xr = x[2 * i], xi = x[2 * i + 1];
+#pragma unroll
+      for (int v = 0; v < MAX_DOT; ++v)
+        if (v < count) {
+          const double ur = basis[v * ld + 2 * i], ui = basis[v * ld + 2 * i + 1];
+          acc[2 * v] = fma(ur, xr, fma(ui, xi, acc[2 * v]));
+          acc[2 * v + 1] = fma(ur, xi, fma(-ui, xr, acc[2 * v + 1]));
+        }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nacc = CPLX ? 2 * count : count;
+#pragma unroll
+  for (int v = 0; v < 2 * MAX_DOT; ++v) {
+    if (v < nacc) {
+      double s = wave_sum(acc[v]);
+      if (lane == 0) red[wave][v] = s;
+    }
+  }
+  __syncthreads();
+  for (int v = threadIdx.x; v < nacc; v += blockDim.x) {
+    double s = 0.0;
+    for (int w = 0; w < RED_THREADS / 64; ++w) s += red[w][v];
+    part[(long)v * nblk + blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void multidot_stage2(const double* __restrict__ part, int nblk,
+                                                       int nacc, double* __restrict__ out) {
+  __shared__ double red[4];
+  const int v = blockIdx.x;
+  if (v >= nacc) return;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) s += part[(long)v * nblk + i];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[v] = red[0] + red[1] + red[2] + red[3];
+}
+
+// x -= sum_v coef[v] * basis_v (real or complex coefficients)
+template <bool CPLX>
+__global__ __launch_bounds__(256) void multi_axpy_kernel(const double* __restrict__ basis, long ld, int count,
+                                                         const double* __restrict__ coef, double* __restrict__ x,
+                                                         long n, int sub) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (!CPLX) {
+      double acc = 0.0;
+      for (int v = 0; v < count; ++v) acc = fma(coef[v], basis[v * ld + i], acc);
+      x[i] = sub ? x[i] - acc : acc;
+    } else {
+      double ar = 0.0, ai = 0.0;
+      for (int v = 0; v < count; ++v) {
+        const double cr = coef[2 * v], ci = coef[2 * v + 1];
+        const double br = basis[v * ld + 2 * i], bi = basis[v * ld + 2 * i + 1];
+        ar = fma(cr, br, fma(-ci, bi, ar));
+        ai = fma(cr, bi, fma(ci, br, ai));
+      }
+      if (sub) { x[2 * i] -= ar; x[2 * i + 1] -= ai; }
+      else { x[2 * i] = ar; x[2 * i + 1] = ai; }
+    }
+  }
+}
+
+// row sums of squares: one block per row
+__global__ __launch_bounds__(256) void row_norms2_kernel(const double* __restrict__ X, long L, long ld,
+                                                         double* __restrict__ out) {
+  __shared__ double red[4];
+  const long r = blockIdx.x;
+  double s = 0.0;
+  for (long j = threadIdx.x; j < L; j += blockDim.x) {
+    const double v = X[r * ld + j];
+    s = fma(v, v, s);
+  }
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[r] = red[0] + red[1] + red[2] + red[3];
+}
+
+// out (C x R) = in^T, in is R x C; 32x32 LDS tiles
+__global__ __launch_bounds__(256) void transpose_kernel(const double* __restrict__ in, long R, long C,
+                                                        double* __restrict__ out) {
+  __shared__ double t[32][33];
+  const long r0 = (long)blockIdx.y * 32, c0 = (long)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const long r = r0 + k, cc = c0 + tx;
+    t[k][tx] = (r < R && cc < C) ? in[r * C + cc] : 0.0;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const long cc = c0 + k, r = r0 + tx;
+    if (cc < C && r < R) out[cc * R + r] = t[tx][k];
+  }
+}
+
+__global__ __launch_bounds__(256) void gather_rows_kernel(const double* __restrict__ src, long ld,
+                                                          const int* __restrict__ perm,
+                                                          const double* __restrict__ scale, long L,
+                                                          double* __restrict__ out) {
+  const long k = blockIdx.y;
+  const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < L) out[k * L + j] = src[(long)perm[k] * ld + j] * scale[k];
+}
+
+// out[j][k] = src[perm[k]][j] * scale[k]; 32x32 tiles through LDS
+__global__ __launch_bounds__(256) void gather_rows_T_kernel(const double* __restrict__ src, long ld,
+                                                            const int* __restrict__ perm,
+                                                            const double* __restrict__ scale, long kcount,
+                                                            long L, double* __restrict__ out) {
+  __shared__ double t[32][33];
+  const long k0 = (long)blockIdx.y * 32, j0 = (long)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int a = ty; a < 32; a += 8) {
+    const long k = k0 + a, j = j0 + tx;
+    t[a][tx] = (k < kcount && j < L) ? src[(long)perm[k] * ld + j] * scale[k] : 0.0;
+  }
+  __syncthreads();
+  for (int a = ty; a < 32; a += 8) {
+    const long j = j0 + a, k = k0 + tx;
+    if (j < L && k < kcount) out[j * kcount + k] = t[tx][a];
+  }
+}
+
+inline int grid1d(long n, int threads = 256) {
+  long g = (n + threads - 1) / threads;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>& dense) {
+  std::vector<int> rp(rows + 1, 0), ci;
+  std::vector<double> v;
+  for (int r = 0; r < rows; ++r) {
+    for (int c = 0; c < cols; ++c) {
+      const double x = dense[(size_t)r * cols + c];
+      if (x != 0.0) { ci.push_back(c); v.push_back(x); }
+    }
+    rp[r + 1] = (int)ci.size();
+  }
+  op.rows = rows;
+  op.cols = cols;
+  op.nnz = (int)ci.size();
+  op.rowptr.ensure(rp.size() * sizeof(int));
+  op.colidx.ensure(std::max<size_t>(1, ci.size()) * sizeof(int));
+  op.vals.ensure(std::max<size_t>(1, v.size()) * sizeof(double));
+  TN_HIP(hipMemcpy(op.rowptr.p, rp.data(), rp.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!ci.empty()) {
+    TN_HIP(hipMemcpy(op.colidx.p, ci.data(), ci.size() * sizeof(int), hipMemcpyHostToDevice));
+    TN_HIP(hipMemcpy(op.vals.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+}
+
+void wstep(tn_ctx* c, bool fp32, const WStepDesc& w) {
+  TN_REQUIRE(w.op && w.op->rows == w.Gp * w.Dp && w.op->cols == w.G * w.D, TN_E_SHAPE,
+             "wstep: operator shape mismatch");
+  if (w.nA == 0 || w.nB == 0) return;
+  TN_REQUIRE(w.nA <= 2147483647L, TN_E_INVALID, "wstep: nA too large");
+  const int K = w.G * w.D;
+  const size_t shm = (size_t)K * 64 * (fp32 ? sizeof(float) : sizeof(double));
+  TN_REQUIRE(shm <= 64 * 1024, TN_E_UNSUPPORTED, "wstep: MPO slab exceeds 64 KiB of LDS");
+  dim3 grid((unsigned)((w.nB + 63) / 64), (unsigned)w.nA);
+  TN_REQUIRE(w.nA <= 65535L, TN_E_INVALID, "wstep: nA exceeds grid.y");
+  if (fp32)
+    hipLaunchKernelGGL(wstep_kernel<float>, grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
+                       w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg,
+                       w.out_sA, w.out_sS, w.op->rowptr.as<int>(), w.op->colidx.as<int>(),
+                       w.op->vals.as<double>());
+  else
+    hipLaunchKernelGGL(wstep_kernel<double>, grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
+                       w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg,
+                       w.out_sA, w.out_sS, w.op->rowptr.as<int>(), w.op->colidx.as<int>(),
+                       w.op->vals.as<double>());
+  TN_HIP(hipGetLastError());
+}
+
+void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(cvt_d2f_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, x, y, n);
+  TN_HIP(hipGetLastError());
+}
+void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(cvt_f2d_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, x, y, n);
+  TN_HIP(hipGetLastError());
+}
+void fill_f64(tn_ctx* c, double* x, double v, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, x, v, n);
+  TN_HIP(hipGetLastError());
+}
+void copy_f64(tn_ctx* c, double* dst, const double* src, long n) {
+  if (n <= 0) return;
+  TN_HIP(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+}
+void scale_f64(tn_ctx* c, double* x, double s, long n) { scaled_copy(c, x, x, s, n); }
+void scaled_copy(tn_ctx* c, double* y, const double* x, double a, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(scaled_copy_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, y, x, a, n);
+  TN_HIP(hipGetLastError());
+}
+
+void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double* x, long n, bool cplx,
+               double* out_dev) {
+  TN_REQUIRE(count >= 1, TN_E_INVALID, "multi_dot: count");
+  const int nblk = (int)std::min<long>(RED_BLOCKS, std::max<long>(1, (n + RED_THREADS - 1) / RED_THREADS));
+  double* part = (double*)c->workspace(WS_KRY_PART, (size_t)2 * MAX_DOT * nblk * sizeof(double));
+  for (int v0 = 0; v0 < count; v0 += MAX_DOT) {
+    const int cnt = std::min(MAX_DOT, count - v0);
+    const int nacc = cplx ? 2 * cnt : cnt;
+    if (cplx)
+      hipLaunchKernelGGL(multidot_stage1<true>, dim3(nblk), dim3(RED_THREADS), 0, c->stream, basis + v0 * ld,
+                         ld, cnt, x, n, part);
+    else
+      hipLaunchKernelGGL(multidot_stage1<false>, dim3(nblk), dim3(RED_THREADS), 0, c->stream, basis + v0 * ld,
+                         ld, cnt, x, n, part);
+    TN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(multidot_stage2, dim3(nacc), dim3(256), 0, c->stream, part, nblk, nacc,
+                       out_dev + (cplx ? 2 * v0 : v0));
+    TN_HIP(hipGetLastError());
+  }
+}
+
+void multi_axpy_sub(tn_ctx* c, const double* basis, long ld, int count, const double* coef_dev, double* x,
+                    long n, bool cplx) {
+  if (n <= 0 || count <= 0) return;
+  if (cplx)
+    hipLaunchKernelGGL(multi_axpy_kernel<true>, dim3(grid1d(n)), dim3(256), 0, c->stream, basis, ld, count,
+                       coef_dev, x, n, 1);
+  else
+    hipLaunchKernelGGL(multi_axpy_kernel<false>, dim3(grid1d(n)), dim3(256), 0, c->stream, basis, ld, count,
+                       coef_dev, x, n, 1);
+  TN_HIP(hipGetLastError());
+}
+
+void lincomb(tn_ctx* c, const double* basis, long ld, int count, const double* coef_dev, double* out, long n,
+             bool cplx) {
+  if (n <= 0) return;
+  if (cplx)
+    hipLaunchKernelGGL(multi_axpy_kernel<true>, dim3(grid1d(n)), dim3(256), 0, c->stream, basis, ld, count,
+                       coef_dev, out, n, 0);
+  else
+    hipLaunchKernelGGL(multi_axpy_kernel<false>, dim3(grid1d(n)), dim3(256), 0, c->stream, basis, ld, count,
+                       coef_dev, out, n, 0);
+  TN_HIP(hipGetLastError());
+}
+
+void row_norms2(tn_ctx* c, const double* X, long rows, long L, long ld, double* out) {
+  if (rows <= 0) return;
+  TN_REQUIRE(rows <= 2147483647L, TN_E_INVALID, "row_norms2: rows");
+  hipLaunchKernelGGL(row_norms2_kernel, dim3((unsigned)rows), dim3(256), 0, c->stream, X, L, ld, out);
+  TN_HIP(hipGetLastError());
+}
+
+void transpose_f64(tn_ctx* c, const double* in, long R, long C, double* out) {
+  if (R <= 0 || C <= 0) return;
+  dim3 grid((unsigned)((C + 31) / 32), (unsigned)((R + 31) / 32));
+  TN_REQUIRE((R + 31) / 32 <= 65535, TN_E_INVALID, "transpose: too many rows");
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, c->stream, in, R, C, out);
+  TN_HIP(hipGetLastError());
+}
+
+void gather_rows(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale, long kcount,
+                 long L, double* out) {
+  if (kcount <= 0 || L <= 0) return;
+  TN_REQUIRE(kcount <= 65535, TN_E_INVALID, "gather_rows: kcount");
+  dim3 grid((unsigned)((L + 255) / 256), (unsigned)kcount);
+  hipLaunchKernelGGL(gather_rows_kernel, grid, dim3(256), 0, c->stream, src, ld_src, perm, scale, L, out);
+  TN_HIP(hipGetLastError());
+}
+
+void gather_rows_T(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
+                   long kcount, long L, double* out) {
+  if (kcount <= 0 || L <= 0) return;
+  dim3 grid((unsigned)((L + 31) / 32), (unsigned)((kcount + 31) / 32));
+  TN_REQUIRE((kcount + 31) / 32 <= 65535, TN_E_INVALID, "gather_rows_T: kcount");
+  hipLaunchKernelGGL(gather_rows_T_kernel, grid, dim3(256), 0, c->stream, src, ld_src, perm, scale, kcount, L,
+                     out);
+  TN_HIP(hipGetLastError());
+}
+
+}  // namespace tn

@@ -1,0 +1,212 @@
+// Internal declarations shared by the libtnhip sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tnhip.h"
+
+namespace tn {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define TN_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      throw ::tn::Error(TN_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TN_REQUIRE(cond, code, msg)                 \
+  do {                                              \
+    if (!(cond)) throw ::tn::Error((code), (msg));  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// device buffers
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // grow-only; contents are NOT preserved
+  void ensure(size_t nbytes) {
+    if (nbytes <= bytes && p) return;
+    release();
+    size_t nb = nbytes ? nbytes : 256;
+    hipError_t e = hipMalloc(&p, nb);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(TN_E_NOMEM, "hipMalloc(" + std::to_string(nb) + ") failed: " + hipGetErrorString(e));
+    }
+    bytes = nb;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+enum Phase { PH_MATVEC = 0, PH_ENV = 1, PH_KRYLOV = 2, PH_SVD = 3, PH_COUNT = 4 };
+
+}  // namespace tn
+
+struct tn_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // named workspace slots (grow-only)
+  std::vector<tn::DevBuf> ws;
+  // host-pinned staging for small scalars read back by the Krylov loop
+  double* pinned = nullptr;
+  size_t pinned_n = 0;
+  // per-phase timing
+  bool timing = false;
+  double phase_ms[tn::PH_COUNT] = {0, 0, 0, 0};
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
+  std::vector<int> ev_phase;
+  size_t ev_used = 0;
+  const double* ones = nullptr;  // device scalar 1.0 (edge environments)
+  tn::DevBuf ones_buf;
+
+  void* workspace(int slot, size_t bytes) {
+    if ((int)ws.size() <= slot) ws.resize(slot + 1);
+    ws[slot].ensure(bytes);
+    return ws[slot].p;
+  }
+};
+
+namespace tn {
+
+// workspace slot ids (each slot is an independent grow-only buffer)
+enum WsSlot {
+  WS_T1 = 0, WS_T2, WS_PSI32, WS_OUT32, WS_KRY_BASIS, WS_KRY_W, WS_KRY_PART, WS_KRY_COEF,
+  WS_SVD_X, WS_SVD_Q, WS_SVD_GRAM, WS_SVD_J, WS_SVD_CNT, WS_SVD_PAIRS, WS_SVD_S, WS_SVD_PERM,
+  WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP, WS_SLOTS
+};
+
+// scoped phase timer (HIP events on the context stream)
+struct PhaseTimer {
+  tn_ctx* c;
+  int ph;
+  size_t idx;
+  PhaseTimer(tn_ctx* ctx, int phase);
+  ~PhaseTimer();
+};
+void flush_timers(tn_ctx* c);
+
+// ---------------------------------------------------------------------------
+// GEMM (MFMA) -- gemm.hip
+// C[b] = alpha * op(A[b]) * op(B[b]) + beta * C[b], row-major, batch over b.
+// op(A) is M x K: element (m,k) = A[m*lda + k] (tA = 0) or A[k*lda + m] (tA = 1).
+// op(B) is K x N: element (k,n) = B[k*ldb + n] (tB = 0) or B[n*ldb + k] (tB = 1).
+// ---------------------------------------------------------------------------
+enum class Prec { F64 = 0, F32 = 1, BF16X3 = 2, Z64 = 3 };
+
+struct GemmDesc {
+  long M = 0, N = 0, K = 0;
+  const void* A = nullptr; long lda = 0; long sA = 0; int tA = 0; int cA = 0;
+  const void* B = nullptr; long ldb = 0; long sB = 0; int tB = 0; int cB = 0;
+  void* C = nullptr; long ldc = 0; long sC = 0;
+  int batch = 1;
+  double alpha = 1.0, beta = 0.0;
+};
+void gemm(tn_ctx* c, Prec p, const GemmDesc& g);
+
+// ---------------------------------------------------------------------------
+// small-MPO-operator contraction ("W step") -- tensor_ops.hip
+// out[g', A, S', B] = sum_{g,S} Op[(g' S'), (g S)] * in[g, A, S, B]
+// Op is given as CSR over output rows o = g'*Dp + S' with columns i = g*D + S.
+// Element type: double or float; complex data is handled by the caller as
+// interleaved real pairs (B doubled, strides doubled).
+// ---------------------------------------------------------------------------
+struct SparseOp {
+  int rows = 0, cols = 0, nnz = 0;
+  DevBuf rowptr, colidx, vals;  // int, int, double
+};
+struct WStepDesc {
+  int G = 0, Gp = 0, D = 0, Dp = 0;
+  long nA = 0, nB = 0;
+  const void* in = nullptr; long in_sg = 0, in_sA = 0, in_sS = 0;
+  void* out = nullptr; long out_sg = 0, out_sA = 0, out_sS = 0;
+  const SparseOp* op = nullptr;
+};
+void wstep(tn_ctx* c, bool fp32, const WStepDesc& w);
+void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>& dense);
+
+// ---------------------------------------------------------------------------
+// elementwise / Krylov vector kernels -- tensor_ops.hip
+// ---------------------------------------------------------------------------
+void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n);
+void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n);
+void fill_f64(tn_ctx* c, double* x, double v, long n);
+void copy_f64(tn_ctx* c, double* dst, const double* src, long n);
+// out[i] = <basis_i, x> for i < count (basis vectors contiguous with stride ld);
+// cplx: complex128 data of n complex elements, out holds (re, im) pairs.
+void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double* x, long n,
+               bool cplx, double* out_dev);
+// x -= sum_i coef[i] * basis_i ; coef on device (real or complex pairs)
+void multi_axpy_sub(tn_ctx* c, const double* basis, long ld, int count, const double* coef_dev,
+                    double* x, long n, bool cplx);
+// out = sum_i coef[i] * basis_i ; coef on device
+void lincomb(tn_ctx* c, const double* basis, long ld, int count, const double* coef_dev,
+             double* out, long n, bool cplx);
+// x *= s (s real, on host)
+void scale_f64(tn_ctx* c, double* x, double s, long n);
+// y = a*x (a real), n doubles
+void scaled_copy(tn_ctx* c, double* y, const double* x, double a, long n);
+// row sum of squares of a (rows x L) row-major matrix into out[rows]
+void row_norms2(tn_ctx* c, const double* X, long rows, long L, long ld, double* out);
+// out (rows x cols) = in^T  (in: cols x rows), doubles
+void transpose_f64(tn_ctx* c, const double* in, long in_rows, long in_cols, double* out);
+// out[k][j] = src[perm[k]][j] * scale[k]   (k < kcount, j < L)
+void gather_rows(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
+                 long kcount, long L, double* out);
+// out[j][k] = src[perm[k]][j] * scale[k]   (transposed gather)
+void gather_rows_T(tn_ctx* c, const double* src, long ld_src, const int* perm,
+                   const double* scale, long kcount, long L, double* out);
+
+// ---------------------------------------------------------------------------
+// truncated SVD (one-sided block Jacobi) -- svd.hip
+// ---------------------------------------------------------------------------
+struct SvdOut {
+  long p = 0, q = 0;        // p = min(m,n) vectors of length q = max(m,n)
+  bool vec_cols = true;     // true: vectors are the columns of M (n <= m)
+  double* X = nullptr;      // p x q: rotated vectors (rows), x_i = s_i * (u_i or v_i)
+  double* Qt = nullptr;     // p x p: accumulated rotation, rows = v_i (cols case) or u_i
+  std::vector<double> s;    // singular values (unsorted, length p)
+  int sweeps = 0;
+};
+void svd_jacobi(tn_ctx* c, const double* M, long m, long n, SvdOut& out);
+
+// truncation rule of NDTensors truncate!!
+struct TruncRule {
+  long n;
+  double truncerr;
+  double docut;
+};
+TruncRule truncate_weights(const std::vector<double>& P_desc, long maxdim, long mindim,
+                           double cutoff);
+
+}  // namespace tn

@@ -1,0 +1,1432 @@
+// C ABI of libtnhip: StateEnvs{ProjMPO} on one MI355X.
+//
+// Storage in HBM (row-major, one buffer per tensor, FP64 or complex128):
+//   site k        A_k[a, s, b]               (chi_{k-1}, d_k, chi_k)
+//   left env k    Lt_k[w, a'(bra), a(ket)]   (w_k, chi_k, chi_k)       sites 1..k
+//   right env k   Rt_k[w, b(ket), b'(bra)]   (w_{k-1}, chi_{k-1}, ...) sites k..N
+// With these layouts every large contraction is a plain (batched) GEMM:
+//   matvec (two-site, pos p, D = d_p d_{p+1}):
+//     T1[(w a'), (S b)]   = Lt[(w a'), a] * phi[a, (S b)]            GEMM  M=w chil, N=D chir, K=chil
+//     T2[a', S', w2, b]   = sum_{w,S} Op2[(w2 S'), (w S)] T1          W step (sparse MPO pair)
+//     out[(a' S'), b']    = T2[(a' S'), (w2 b)] * Rt[(w2 b), b']      GEMM  M=chil D, N=chir, K=w chir
+//   left env update (site k):  X = Lt*A ; Y = W-step ; Lt'[w'] = A^H * Y[w']   (batched over w')
+//   right env update (site k): X[w2] = B*Rt[w2] ; Y = W-step ; Rt'[w] = Y[w] * B^H (batched over w)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <numeric>
+
+#include "tn_internal.h"
+
+using namespace tn;
+using cplx = std::complex<double>;
+
+namespace {
+thread_local std::string g_thread_error;
+
+int set_error(tn_ctx* c, int code, const std::string& msg) {
+  g_thread_error = msg;
+  if (c) c->last_error = msg;
+  return code;
+}
+
+template <class F>
+int guarded(tn_ctx* c, F&& f) {
+  try {
+    f();
+    return TN_OK;
+  } catch (const tn::Error& e) {
+    return set_error(c, e.code, e.what());
+  } catch (const std::bad_alloc&) {
+    return set_error(c, TN_E_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return set_error(c, TN_E_INVALID, e.what());
+  }
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// phase timers
+// ---------------------------------------------------------------------------
+namespace tn {
+PhaseTimer::PhaseTimer(tn_ctx* ctx, int phase) : c(ctx), ph(phase), idx((size_t)-1) {
+  if (!c->timing) return;
+  if (c->ev_used >= 2048) flush_timers(c);
+  if (c->ev_used >= c->ev_pool.size()) {
+    hipEvent_t a, b;
+    TN_HIP(hipEventCreate(&a));
+    TN_HIP(hipEventCreate(&b));
+    c->ev_pool.push_back({a, b});
+    c->ev_phase.push_back(0);
+  }
+  idx = c->ev_used++;
+  c->ev_phase[idx] = ph;
+  TN_HIP(hipEventRecord(c->ev_pool[idx].first, c->stream));
+}
+PhaseTimer::~PhaseTimer() {
+  if (idx != (size_t)-1) (void)hipEventRecord(c->ev_pool[idx].second, c->stream);
+}
+void flush_timers(tn_ctx* c) {
+  if (c->ev_used == 0) return;
+  TN_HIP(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < c->ev_used; ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev_pool[i].first, c->ev_pool[i].second) == hipSuccess)
+      c->phase_ms[c->ev_phase[i]] += ms;
+  }
+  c->ev_used = 0;
+}
+}  // namespace tn
+
+// ---------------------------------------------------------------------------
+// environment object
+// ---------------------------------------------------------------------------
+struct SiteT {
+  DevBuf buf;
+  long chil = 0, chir = 0;
+};
+struct EnvT {
+  DevBuf buf;
+  long chi = 0;
+  int w = 0;
+};
+
+struct tn_env {
+  tn_ctx* ctx = nullptr;
+  int N = 0;
+  std::vector<int> d, w;  // d[k-1] for site k; w[k] for MPO bond k (0..N)
+  int dtype = TN_F64;
+  size_t esz = 8;
+  std::vector<std::vector<double>> Wh;  // host MPO copies, site k at [k-1]
+  std::vector<char> W_set;
+  std::vector<SiteT> A;                 // site k at [k-1]
+  std::vector<EnvT> LR;                 // 0..N+1
+  int lpos = 0, rpos = 0;
+  int nsite = 2;
+  int ortho = 0;
+  std::vector<SparseOp> opL, opR, op2;
+  std::vector<char> opL_ok, opR_ok, op2_ok;
+  // FP32 shadows of the environments used by mixed-precision matvecs
+  DevBuf L32, R32;
+  long shadow_key_l = -1, shadow_key_r = -1;
+  long env_version = 0;
+  std::vector<long> LR_version;
+
+  bool cplx() const { return dtype == TN_C128; }
+};
+
+namespace {
+
+tn_ctx* C(tn_env* e) { return e ? e->ctx : nullptr; }
+
+void check_env(tn_env* e) { TN_REQUIRE(e != nullptr, TN_E_INVALID, "null environment"); }
+
+void require_mpo(tn_env* e) {
+  for (int k = 0; k < e->N; ++k)
+    TN_REQUIRE(e->W_set[k], TN_E_INVALID, "MPO tensor of site " + std::to_string(k + 1) + " not set");
+}
+void require_sites(tn_env* e) {
+  for (int k = 0; k < e->N; ++k)
+    TN_REQUIRE(e->A[k].buf.p != nullptr, TN_E_INVALID, "MPS tensor of site " + std::to_string(k + 1) + " not set");
+}
+
+// edge environment: a 1x1x1 tensor holding 1
+void set_edge(tn_env* e, int k) {
+  EnvT& E = e->LR[k];
+  E.buf.ensure(16);
+  E.chi = 1;
+  E.w = 1;
+  double one[2] = {1.0, 0.0};
+  TN_HIP(hipMemcpyAsync(E.buf.p, one, e->esz, hipMemcpyHostToDevice, e->ctx->stream));
+  TN_HIP(hipStreamSynchronize(e->ctx->stream));
+}
+
+// sparse W-step operators built from the host MPO copies
+const SparseOp& op_left(tn_env* e, int k) {  // Op[(w' s'), (w s)] = W_k[w, s', s, w']
+  if (!e->opL_ok[k - 1]) {
+    const int wl = e->w[k - 1], wr = e->w[k], d = e->d[k - 1];
+    const auto& W = e->Wh[k - 1];
+    std::vector<double> M((size_t)wr * d * wl * d, 0.0);
+    for (int a = 0; a < wl; ++a)
+      for (int sp = 0; sp < d; ++sp)
+        for (int s = 0; s < d; ++s)
+          for (int b = 0; b < wr; ++b)
+            M[(size_t)(b * d + sp) * (wl * d) + (a * d + s)] = W[((size_t)(a * d + sp) * d + s) * wr + b];
+    build_sparse_op(e->opL[k - 1], wr * d, wl * d, M);
+    e->opL_ok[k - 1] = 1;
+  }
+  return e->opL[k - 1];
+}
+const SparseOp& op_right(tn_env* e, int k) {  // Op[(w t'), (w2 t)] = W_k[w, t', t, w2]
+  if (!e->opR_ok[k - 1]) {
+    const int wl = e->w[k - 1], wr = e->w[k], d = e->d[k - 1];
+    const auto& W = e->Wh[k - 1];
+    std::vector<double> M((size_t)wl * d * wr * d, 0.0);
+    for (int a = 0; a < wl; ++a)
+      for (int tp = 0; tp < d; ++tp)
+        for (int t = 0; t < d; ++t)
+          for (int b = 0; b < wr; ++b)
+            M[(size_t)(a * d + tp) * (wr * d) + (b * d + t)] = W[((size_t)(a * d + tp) * d + t) * wr + b];
+    build_sparse_op(e->opR[k - 1], wl * d, wr * d, M);
+    e->opR_ok[k - 1] = 1;
+  }
+  return e->opR[k - 1];
+}
+// fused pair (p, p+1): Op2[(w2, s', t'), (w, s, t)] = sum_w1 W_p[w,s',s,w1] W_{p+1}[w1,t',t,w2]
+const SparseOp& op_two(tn_env* e, int p) {
+  if (!e->op2_ok[p - 1]) {
+    const int w0 = e->w[p - 1], w1 = e->w[p], w2 = e->w[p + 1];
+    const int d1 = e->d[p - 1], d2 = e->d[p];
+    const int D = d1 * d2;
+    const auto& Wa = e->Wh[p - 1];
+    const auto& Wb = e->Wh[p];
+    std::vector<double> M((size_t)w2 * D * w0 * D, 0.0);
+    for (int a = 0; a < w0; ++a)
+      for (int sp = 0; sp < d1; ++sp)
+        for (int s = 0; s < d1; ++s)
+          for (int m = 0; m < w1; ++m) {
+            const double x = Wa[((size_t)(a * d1 + sp) * d1 + s) * w1 + m];
+            if (x == 0.0) continue;
+            for (int tp = 0; tp < d2; ++tp)
+              for (int t = 0; t < d2; ++t)
+                for (int b = 0; b < w2; ++b) {
+                  const double y = Wb[((size_t)(m * d2 + tp) * d2 + t) * w2 + b];
+                  if (y == 0.0) continue;
+                  const size_t row = (size_t)b * D + (sp * d2 + tp);
+                  const size_t col = (size_t)a * D + (s * d2 + t);
+                  M[row * (w0 * D) + col] += x * y;
+                }
+          }
+    build_sparse_op(e->op2[p - 1], w2 * D, w0 * D, M);
+    e->op2_ok[p - 1] = 1;
+  }
+  return e->op2[p - 1];
+}
+
+Prec gemm_prec_full(tn_env* e) { return e->cplx() ? Prec::Z64 : Prec::F64; }
+
+// environment updates -------------------------------------------------------
+void update_left(tn_env* e, int k) {  // LR[k] from LR[k-1] and site k
+  PhaseTimer pt(e->ctx, PH_ENV);
+  tn_ctx* c = e->ctx;
+  const EnvT& Lin = e->LR[k - 1];
+  const SiteT& S = e->A[k - 1];
+  const long chil = S.chil, chir = S.chir;
+  const int d = e->d[k - 1], wl = e->w[k - 1], wr = e->w[k];
+  TN_REQUIRE(Lin.chi == chil && Lin.w == wl, TN_E_SHAPE,
+             "left environment " + std::to_string(k - 1) + " does not match site " + std::to_string(k));
+  const size_t es = e->esz;
+  void* X = c->workspace(WS_ENV_X, (size_t)wl * chil * d * chir * es);
+  void* Y = c->workspace(WS_ENV_Y, (size_t)wr * chil * d * chir * es);
+  const Prec P = gemm_prec_full(e);
+  GemmDesc g1;
+  g1.M = (long)wl * chil; g1.N = (long)d * chir; g1.K = chil;
+  g1.A = Lin.buf.p; g1.lda = chil;
+  g1.B = S.buf.p; g1.ldb = (long)d * chir;
+  g1.C = X; g1.ldc = (long)d * chir;
+  gemm(c, P, g1);
+  const int m2 = e->cplx() ? 2 : 1;
+  WStepDesc ws;
+  ws.G = wl; ws.Gp = wr; ws.D = d; ws.Dp = d;
+  ws.nA = chil; ws.nB = chir * m2;
+  ws.in = X; ws.in_sg = chil * d * chir * m2; ws.in_sA = d * chir * m2; ws.in_sS = chir * m2;
+  ws.out = Y; ws.out_sg = chil * d * chir * m2; ws.out_sA = d * chir * m2; ws.out_sS = chir * m2;
+  ws.op = &op_left(e, k);
+  wstep(c, false, ws);
+  EnvT& Lo = e->LR[k];
+  Lo.buf.ensure((size_t)wr * chir * chir * es);
+  Lo.chi = chir;
+  Lo.w = wr;
+  GemmDesc g3;
+  g3.M = chir; g3.N = chir; g3.K = chil * d;
+  g3.A = S.buf.p; g3.lda = chir; g3.tA = 1; g3.sA = 0; g3.cA = e->cplx();
+  g3.B = Y; g3.ldb = chir; g3.sB = chil * d * chir;
+  g3.C = Lo.buf.p; g3.ldc = chir; g3.sC = chir * chir;
+  g3.batch = wr;
+  gemm(c, P, g3);
+  e->LR_version[k] = ++e->env_version;
+}
+
+void update_right(tn_env* e, int k) {  // LR[k] from LR[k+1] and site k
+  PhaseTimer pt(e->ctx, PH_ENV);
+  tn_ctx* c = e->ctx;
+  const EnvT& Rin = e->LR[k + 1];
+  const SiteT& S = e->A[k - 1];
+  const long chil = S.chil, chir = S.chir;
+  const int d = e->d[k - 1], wl = e->w[k - 1], wr = e->w[k];
+  TN_REQUIRE(Rin.chi == chir && Rin.w == wr, TN_E_SHAPE,
+             "right environment " + std::to_string(k + 1) + " does not match site " + std::to_string(k));
+  const size_t es = e->esz;
+  void* X = c->workspace(WS_ENV_X, (size_t)wr * chil * d * chir * es);
+  void* Y = c->workspace(WS_ENV_Y, (size_t)wl * chil * d * chir * es);
+  const Prec P = gemm_prec_full(e);
+  GemmDesc g1;
+  g1.M = chil * d; g1.N = chir; g1.K = chir;
+  g1.A = S.buf.p; g1.lda = chir; g1.sA = 0;
+  g1.B = Rin.buf.p; g1.ldb = chir; g1.sB = chir * chir;
+  g1.C = X; g1.ldc = chir; g1.sC = chil * d * chir;
+  g1.batch = wr;
+  gemm(c, P, g1);
+  const int m2 = e->cplx() ? 2 : 1;
+  WStepDesc ws;
+  ws.G = wr; ws.Gp = wl; ws.D = d; ws.Dp = d;
+  ws.nA = chil; ws.nB = chir * m2;
+  ws.in = X; ws.in_sg = chil * d * chir * m2; ws.in_sA = d * chir * m2; ws.in_sS = chir * m2;
+  ws.out = Y; ws.out_sg = chil * d * chir * m2; ws.out_sA = d * chir * m2; ws.out_sS = chir * m2;
+  ws.op = &op_right(e, k);
+  wstep(c, false, ws);
+  EnvT& Ro = e->LR[k];
+  Ro.buf.ensure((size_t)wl * chil * chil * es);
+  Ro.chi = chil;
+  Ro.w = wl;
+  GemmDesc g3;
+  g3.M = chil; g3.N = chil; g3.K = d * chir;
+  g3.A = Y; g3.lda = d * chir; g3.sA = chil * d * chir;
+  g3.B = S.buf.p; g3.ldb = d * chir; g3.tB = 1; g3.sB = 0; g3.cB = e->cplx();
+  g3.C = Ro.buf.p; g3.ldc = chil; g3.sC = chil * chil;
+  g3.batch = wl;
+  gemm(c, P, g3);
+  e->LR_version[k] = ++e->env_version;
+}
+
+// ProjMPO makeL!/makeR!/position! (mirror src/mps/projmps2.jl:77-149)
+void makeL(tn_env* e, int k) {
+  int ll = e->lpos;
+  if (ll >= k) { e->lpos = k; return; }
+  ll = std::max(ll, 0);
+  while (ll < k) {
+    update_left(e, ll + 1);
+    ++ll;
+  }
+  e->lpos = k;
+}
+void makeR(tn_env* e, int k) {
+  int rl = e->rpos;
+  if (rl <= k) { e->rpos = k; return; }
+  rl = std::min(rl, e->N + 1);
+  while (rl > k) {
+    update_right(e, rl - 1);
+    --rl;
+  }
+  e->rpos = k;
+}
+void position(tn_env* e, int pos) {
+  TN_REQUIRE(pos >= 1 && pos + e->nsite - 1 <= e->N, TN_E_POSITION,
+             "position " + std::to_string(pos) + " out of range for nsite=" + std::to_string(e->nsite));
+  require_mpo(e);
+  require_sites(e);
+  makeL(e, pos - 1);
+  makeR(e, pos + e->nsite);
+}
+
+void invalidate_site(tn_env* e, int k) {
+  if (k <= e->lpos) e->lpos = k - 1;
+  if (k >= e->rpos) e->rpos = k + 1;
+}
+
+// shape of the operator domain at the current position
+struct Dom {
+  int pos;
+  long chil, chir;
+  int d1, d2;
+  long n;  // elements
+};
+Dom domain(tn_env* e) {
+  const int pos = e->lpos + 1;
+  TN_REQUIRE(e->rpos == pos + e->nsite, TN_E_POSITION,
+             "environment not positioned: call position! first (lpos=" + std::to_string(e->lpos) +
+                 ", rpos=" + std::to_string(e->rpos) + ", nsite=" + std::to_string(e->nsite) + ")");
+  Dom D;
+  D.pos = pos;
+  if (e->nsite == 2) {
+    D.chil = e->A[pos - 1].chil;
+    D.chir = e->A[pos].chir;
+    D.d1 = e->d[pos - 1];
+    D.d2 = e->d[pos];
+  } else if (e->nsite == 1) {
+    D.chil = e->A[pos - 1].chil;
+    D.chir = e->A[pos - 1].chir;
+    D.d1 = e->d[pos - 1];
+    D.d2 = 1;
+  } else {
+    throw Error(TN_E_UNSUPPORTED, "nsite must be 1 or 2");
+  }
+  D.n = D.chil * D.d1 * D.d2 * D.chir;
+  return D;
+}
+
+// FP32 shadow of an environment (mixed-precision matvecs)
+const float* shadow(tn_env* e, int k, bool left) {
+  DevBuf& B = left ? e->L32 : e->R32;
+  long& key = left ? e->shadow_key_l : e->shadow_key_r;
+  const EnvT& E = e->LR[k];
+  const long want = e->LR_version[k] * 4096 + k;
+  const long n = (long)E.w * E.chi * E.chi;
+  if (key != want) {
+    B.ensure((size_t)n * sizeof(float));
+    cvt_f64_to_f32(e->ctx, E.buf.as<double>(), B.as<float>(), n);
+    key = want;
+  }
+  return B.as<float>();
+}
+
+// product(sysenv, v) (ProjMPO.product; mirror src/mps/projmps2.jl:159-173)
+void matvec(tn_env* e, const void* v, void* out, int prec) {
+  PhaseTimer pt(e->ctx, PH_MATVEC);
+  tn_ctx* c = e->ctx;
+  const Dom D = domain(e);
+  const int pos = D.pos;
+  const EnvT& L = e->LR[pos - 1];
+  const EnvT& R = e->LR[pos + e->nsite];
+  const long chil = D.chil, chir = D.chir;
+  const int DD = D.d1 * D.d2;
+  const int w0 = e->w[pos - 1];
+  const int w2 = e->w[pos - 1 + e->nsite];
+  TN_REQUIRE(L.chi == chil && L.w == w0 && R.chi == chir && R.w == w2, TN_E_SHAPE,
+             "The order of the ProjMPO-ITensor product P*v is not equal to the order of the ITensor v: "
+             "environment and wave-function shapes disagree");
+  const SparseOp& op = (e->nsite == 2) ? op_two(e, pos) : op_left(e, pos);
+  const bool cx = e->cplx();
+  if (cx && prec != TN_PREC_FP64)
+    throw Error(TN_E_UNSUPPORTED, "mixed-precision matvec is implemented for real tensors only");
+  Prec P;
+  bool f32 = false;
+  if (prec == TN_PREC_FP64) P = cx ? Prec::Z64 : Prec::F64;
+  else if (prec == TN_PREC_FP32) { P = Prec::F32; f32 = true; }
+  else if (prec == TN_PREC_BF16X3) { P = Prec::BF16X3; f32 = true; }
+  else throw Error(TN_E_INVALID, "unknown precision");
+  const size_t es = f32 ? 4 : e->esz;
+  const int m2 = cx ? 2 : 1;
+  void* T1 = c->workspace(WS_T1, (size_t)w0 * chil * DD * chir * es);
+  void* T2 = c->workspace(WS_T2, (size_t)w2 * chil * DD * chir * es);
+  const void* vin = v;
+  const void* Lp = L.buf.p;
+  const void* Rp = R.buf.p;
+  void* outp = out;
+  if (f32) {
+    float* v32 = (float*)c->workspace(WS_PSI32, (size_t)D.n * 4);
+    cvt_f64_to_f32(c, (const double*)v, v32, D.n);
+    vin = v32;
+    Lp = shadow(e, pos - 1, true);
+    Rp = shadow(e, pos + e->nsite, false);
+    outp = c->workspace(WS_OUT32, (size_t)D.n * 4);
+  }
+  GemmDesc g1;
+  g1.M = (long)w0 * chil; g1.N = DD * chir; g1.K = chil;
+  g1.A = Lp; g1.lda = chil;
+  g1.B = vin; g1.ldb = DD * chir;
+  g1.C = T1; g1.ldc = DD * chir;
+  gemm(c, P, g1);
+  WStepDesc ws;
+  ws.G = w0; ws.Gp = w2; ws.D = DD; ws.Dp = DD;
+  ws.nA = chil; ws.nB = chir * m2;
+  ws.in = T1; ws.in_sg = chil * DD * chir * m2; ws.in_sA = DD * chir * m2; ws.in_sS = chir * m2;
+  ws.out = T2; ws.out_sg = chir * m2; ws.out_sA = (long)DD * w2 * chir * m2; ws.out_sS = (long)w2 * chir * m2;
+  ws.op = &op;
+  wstep(c, f32, ws);
+  GemmDesc g3;
+  g3.M = chil * DD; g3.N = chir; g3.K = (long)w2 * chir;
+  g3.A = T2; g3.lda = (long)w2 * chir;
+  g3.B = Rp; g3.ldb = chir;
+  g3.C = outp; g3.ldc = chir;
+  gemm(c, P, g3);
+  if (f32) cvt_f32_to_f64(c, (const float*)outp, (double*)out, D.n);
+}
+
+// ---------------------------------------------------------------------------
+// small dense symmetric eigensolver (cyclic Jacobi, FP64, ascending order)
+// ---------------------------------------------------------------------------
+void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<double>& V) {
+  V.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) V[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) (i == j ? diag : off) += A[(size_t)i * n + j] * A[(size_t)i * n + j];
+    if (off <= 1e-32 * diag || off == 0.0) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = A[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double app = A[(size_t)p * n + p], aqq = A[(size_t)q * n + q];
+        const double zeta = (aqq - app) / (2.0 * apq);
+        const double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / std::sqrt(1.0 + t * t), sn = cs * t;
+        for (int k = 0; k < n; ++k) {  // columns p, q of A
+          const double akp = A[(size_t)k * n + p], akq = A[(size_t)k * n + q];
+          A[(size_t)k * n + p] = cs * akp - sn * akq;
+          A[(size_t)k * n + q] = sn * akp + cs * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          const double apk = A[(size_t)p * n + k], aqk = A[(size_t)q * n + k];
+          A[(size_t)p * n + k] = cs * apk - sn * aqk;
+          A[(size_t)q * n + k] = sn * apk + cs * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double vkp = V[(size_t)k * n + p], vkq = V[(size_t)k * n + q];
+          V[(size_t)k * n + p] = cs * vkp - sn * vkq;
+          V[(size_t)k * n + q] = sn * vkp + cs * vkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(),
+                   [&](int a, int b) { return A[(size_t)a * n + a] < A[(size_t)b * n + b]; });
+  w.resize(n);
+  std::vector<double> Vs((size_t)n * n);
+  for (int j = 0; j < n; ++j) {
+    w[j] = A[(size_t)idx[j] * n + idx[j]];
+    for (int i = 0; i < n; ++i) Vs[(size_t)i * n + j] = V[(size_t)i * n + idx[j]];
+  }
+  V.swap(Vs);
+}
+
+// ---------------------------------------------------------------------------
+// Krylov machinery on device vectors
+// ---------------------------------------------------------------------------
+struct Krylov {
+  tn_env* e;
+  tn_ctx* c;
+  bool cx;
+  long n;       // complex or real elements
+  long nd;      // doubles per vector
+  int slots;
+  double* base; // slots x nd
+  double* w;    // nd
+  double* coef; // device scratch
+  Krylov(tn_env* env, long nelem, int nslots) : e(env), c(env->ctx), cx(env->cplx()), n(nelem) {
+    nd = cx ? 2 * n : n;
+    slots = nslots;
+    base = (double*)c->workspace(WS_KRY_BASIS, (size_t)slots * nd * sizeof(double));
+    w = (double*)c->workspace(WS_KRY_W, (size_t)nd * sizeof(double));
+    coef = (double*)c->workspace(WS_KRY_COEF, 1024 * sizeof(double));
+  }
+  double* V(int i) { return base + (long)i * nd; }
+  // <x, x>
+  double norm2(const double* x) {
+    multi_dot(c, x, nd, 1, x, n, cx, coef);
+    double h[2];
+    TN_HIP(hipMemcpyAsync(h, coef, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+    return h[0];
+  }
+  // w <- w - V_{0..K-1} (V^H w), twice; returns the total coefficients
+  // (complex pairs when cx) and |w|^2.
+  void orthogonalize(int K, std::vector<cplx>& h, double& nrm2) {
+    const int str = cx ? 2 : 1;
+    // coefficient scratch: pass 1 at [0, 2K), pass 2 at [128, 128+2K),
+    // |w|^2 at 256 (K <= 61 basis vectors, complex pairs fit in 128)
+    multi_dot(c, base, nd, K, w, n, cx, coef);
+    multi_axpy_sub(c, base, nd, K, coef, w, n, cx);
+    multi_dot(c, base, nd, K, w, n, cx, coef + 128);
+    multi_axpy_sub(c, base, nd, K, coef + 128, w, n, cx);
+    multi_dot(c, w, nd, 1, w, n, cx, coef + 256);
+    double hb[258];
+    TN_HIP(hipMemcpyAsync(hb, coef, 258 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+    h.resize(K);
+    for (int i = 0; i < K; ++i) {
+      if (cx) h[i] = cplx(hb[2 * i] + hb[128 + 2 * i], hb[2 * i + 1] + hb[128 + 2 * i + 1]);
+      else h[i] = cplx(hb[i] + hb[128 + i], 0.0);
+    }
+    (void)str;
+    nrm2 = hb[256];
+  }
+  // out = sum_i coefs[i] V_i
+  void combine(int K, const std::vector<cplx>& co, double* out) {
+    std::vector<double> hc(cx ? 2 * K : K);
+    for (int i = 0; i < K; ++i) {
+      if (cx) { hc[2 * i] = co[i].real(); hc[2 * i + 1] = co[i].imag(); }
+      else hc[i] = co[i].real();
+    }
+    double* dc = coef + 512;  // [512, 512 + 2K)
+    TN_HIP(hipMemcpyAsync(dc, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    lincomb(c, base, nd, K, dc, out, n, cx);
+  }
+};
+
+void apply_op(tn_env* e, const double* x, double* y, int prec) { matvec(e, x, y, prec); }
+
+// eigsolve(A, x0, 1, :SR; krylovdim, maxiter, tol, eager=false) (src/base/solver.jl:36)
+double krylov_eig(tn_env* e, double* phi, int kdim, int maxiter, double tol, int prec, tn_krylov_info* info) {
+  TN_REQUIRE(kdim >= 1 && kdim <= 60 && maxiter >= 1, TN_E_INVALID, "eig_solver: bad krylovdim/maxiter");
+  const Dom D = domain(e);
+  Krylov K(e, D.n, 2 * kdim + 2);
+  tn_krylov_info inf{0, 0, 1, NAN};
+  // v0 = x0 / |x0|
+  double b0;
+  {
+    PhaseTimer pt(e->ctx, PH_KRYLOV);
+    b0 = std::sqrt(K.norm2(phi));
+    TN_REQUIRE(std::isfinite(b0) && b0 > 0.0, TN_E_NUMERIC, "eig_solver: zero or non-finite start vector");
+    scaled_copy(K.c, K.V(0), phi, 1.0 / b0, K.nd);
+  }
+  std::vector<double> Hm((size_t)(kdim + 1) * (kdim + 1), 0.0);
+  auto H = [&](int i, int j) -> double& { return Hm[(size_t)i * (kdim + 1) + j]; };
+  int Kn = 1;
+  std::vector<cplx> h;
+  double r2;
+  apply_op(e, K.V(0), K.w, prec);
+  inf.numops++;
+  {
+    PhaseTimer pt(e->ctx, PH_KRYLOV);
+    K.orthogonalize(1, h, r2);
+  }
+  H(0, 0) = h[0].real();
+  double beta = std::sqrt(std::max(0.0, r2));
+  std::vector<double> theta, U;
+  while (true) {
+    if (Kn == kdim || beta <= tol) {
+      std::vector<double> Hs((size_t)Kn * Kn);
+      for (int i = 0; i < Kn; ++i)
+        for (int j = 0; j < Kn; ++j) Hs[(size_t)i * Kn + j] = 0.5 * (H(i, j) + H(j, i));
+      sym_eig(Kn, Hs, theta, U);
+      std::vector<double> res(Kn);
+      for (int i = 0; i < Kn; ++i) res[i] = std::fabs(beta * U[(size_t)(Kn - 1) * Kn + i]);
+      int conv = 0;
+      while (conv < Kn && res[conv] <= tol) ++conv;
+      inf.converged = conv;
+      inf.residual = res[0];
+      if (conv >= 1 || beta <= tol || inf.numiter >= maxiter) {
+        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        std::vector<cplx> co(Kn);
+        for (int i = 0; i < Kn; ++i) co[i] = U[(size_t)i * Kn + 0];
+        K.combine(Kn, co, phi);
+        const double ny = std::sqrt(K.norm2(phi));
+        scale_f64(K.c, phi, 1.0 / ny, K.nd);
+        if (info) *info = inf;
+        return theta[0];
+      }
+      // thick restart: keep (3*kdim + 2*conv)/5 Ritz vectors + residual direction
+      int keep = (3 * kdim + 2 * conv) / 5;
+      keep = std::max(1, std::min(keep, Kn - 1));
+      {
+        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        for (int j = 0; j < keep; ++j) {
+          std::vector<cplx> co(Kn);
+          for (int i = 0; i < Kn; ++i) co[i] = U[(size_t)i * Kn + j];
+          K.combine(Kn, co, K.V(kdim + 1 + j));
+        }
+        for (int j = 0; j < keep; ++j) copy_f64(K.c, K.V(j), K.V(kdim + 1 + j), K.nd);
+        scaled_copy(K.c, K.V(keep), K.w, 1.0 / beta, K.nd);
+      }
+      std::fill(Hm.begin(), Hm.end(), 0.0);
+      for (int j = 0; j < keep; ++j) {
+        H(j, j) = theta[j];
+        H(keep, j) = H(j, keep) = beta * U[(size_t)(Kn - 1) * Kn + j];
+      }
+      Kn = keep + 1;
+      inf.numiter++;
+    } else {
+      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      scaled_copy(K.c, K.V(Kn), K.w, 1.0 / beta, K.nd);
+      H(Kn, Kn - 1) = H(Kn - 1, Kn) = beta;
+      ++Kn;
+    }
+    apply_op(e, K.V(Kn - 1), K.w, prec);
+    inf.numops++;
+    {
+      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      K.orthogonalize(Kn, h, r2);
+    }
+    H(Kn - 1, Kn - 1) = h[Kn - 1].real();
+    beta = std::sqrt(std::max(0.0, r2));
+    TN_REQUIRE(std::isfinite(beta), TN_E_NUMERIC, "eig_solver: non-finite residual");
+  }
+}
+
+// exp(t A) x0 by a Lanczos exponential integrator (src/base/solver.jl:80)
+void krylov_expm(tn_env* e, double* phi, cplx t, int kdim, int maxiter, double tol, int prec,
+                 tn_krylov_info* info) {
+  TN_REQUIRE(kdim >= 1 && kdim <= 60, TN_E_INVALID, "exp_solver: bad krylovdim");
+  TN_REQUIRE(e->cplx() || t.imag() == 0.0, TN_E_INVALID,
+             "exp_solver: complex time step needs complex (TN_C128) tensors");
+  const Dom D = domain(e);
+  Krylov K(e, D.n, kdim + 1);
+  tn_krylov_info inf{0, 0, 0, 0.0};
+  double remaining = 1.0;  // fraction of t still to go
+  double last = 1.0;       // last accepted sub-step
+  std::vector<cplx> h;
+  double r2;
+  while (remaining > 0.0 && inf.numiter < maxiter) {
+    inf.numiter++;
+    double nx;
+    {
+      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      nx = std::sqrt(K.norm2(phi));
+    }
+    if (nx == 0.0) break;
+    {
+      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      scaled_copy(K.c, K.V(0), phi, 1.0 / nx, K.nd);
+    }
+    std::vector<double> Hm((size_t)(kdim + 1) * (kdim + 1), 0.0);
+    auto H = [&](int i, int j) -> double& { return Hm[(size_t)i * (kdim + 1) + j]; };
+    int Kn = 0;
+    while (true) {
+      apply_op(e, K.V(Kn), K.w, prec);
+      inf.numops++;
+      {
+        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        K.orthogonalize(Kn + 1, h, r2);
+      }
+      H(Kn, Kn) = h[Kn].real();
+      ++Kn;
+      const double beta = std::sqrt(std::max(0.0, r2));
+      std::vector<double> Hs((size_t)Kn * Kn), th, U;
+      double hmax = 1.0;
+      for (int i = 0; i < Kn; ++i)
+        for (int j = 0; j < Kn; ++j) {
+          Hs[(size_t)i * Kn + j] = H(i, j);
+          hmax = std::max(hmax, std::fabs(H(i, j)));
+        }
+      sym_eig(Kn, Hs, th, U);
+      auto expv = [&](cplx tau) {
+        std::vector<cplx> y(Kn, 0.0);
+        for (int j = 0; j < Kn; ++j) {
+          const cplx f = std::exp(tau * th[j]) * U[(size_t)0 * Kn + j];
+          for (int i = 0; i < Kn; ++i) y[i] += U[(size_t)i * Kn + j] * f;
+        }
+        return y;
+      };
+      const cplx tau = remaining * t;
+      std::vector<cplx> y = expv(tau);
+      const double err = std::abs(beta * y[Kn - 1]) * nx;
+      inf.residual = err;
+      if (err <= tol * remaining || beta <= 1e-14 * hmax) {
+        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        for (auto& v : y) v *= nx;
+        K.combine(Kn, y, phi);
+        remaining = 0.0;
+        break;
+      }
+      if (Kn == kdim) {
+        // largest sub-step (halving from twice the last accepted one) whose
+        // error estimate meets the per-unit-time tolerance
+        double frac = std::min(remaining, 2.0 * last);
+        std::vector<cplx> yy;
+        while (true) {
+          yy = expv(frac * t);
+          if (std::abs(beta * yy[Kn - 1]) * nx <= tol * frac || frac <= 1e-12) break;
+          frac *= 0.5;
+        }
+        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        for (auto& v : yy) v *= nx;
+        K.combine(Kn, yy, phi);
+        remaining -= frac;
+        last = frac;
+        break;
+      }
+      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      scaled_copy(K.c, K.V(Kn), K.w, 1.0 / beta, K.nd);
+      H(Kn, Kn - 1) = H(Kn - 1, Kn) = beta;
+    }
+  }
+  inf.converged = remaining == 0.0 ? 1 : 0;
+  if (info) *info = inf;
+}
+
+// <phi|H|phi> with one FP64 matvec
+cplx rayleigh(tn_env* e, const double* phi) {
+  const Dom D = domain(e);
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const long nd = cx ? 2 * D.n : D.n;
+  double* y = (double*)c->workspace(WS_KRY_TMP, (size_t)nd * sizeof(double));
+  matvec(e, phi, y, TN_PREC_FP64);
+  double* co = (double*)c->workspace(WS_MISC, 64 * sizeof(double));
+  multi_dot(c, phi, nd, 1, y, D.n, cx, co);
+  multi_dot(c, phi, nd, 1, phi, D.n, cx, co + 4);
+  double h[8];
+  TN_HIP(hipMemcpyAsync(h, co, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipStreamSynchronize(c->stream));
+  return cplx(h[0], cx ? h[1] : 0.0) / h[4];
+}
+
+// phi = A_pos * A_{pos+1}
+void contract_phi(tn_env* e, int pos, void* phi) {
+  TN_REQUIRE(pos >= 1 && pos < e->N, TN_E_POSITION, "contract_phi: bond out of range");
+  const SiteT& a = e->A[pos - 1];
+  const SiteT& b = e->A[pos];
+  TN_REQUIRE(a.chir == b.chil, TN_E_SHAPE, "contract_phi: bond dimensions disagree");
+  GemmDesc g;
+  g.M = a.chil * e->d[pos - 1]; g.N = e->d[pos] * b.chir; g.K = a.chir;
+  g.A = a.buf.p; g.lda = a.chir;
+  g.B = b.buf.p; g.ldb = e->d[pos] * b.chir;
+  g.C = phi; g.ldc = e->d[pos] * b.chir;
+  gemm(e->ctx, gemm_prec_full(e), g);
+}
+
+// replacebond!(psi, b, phi; maxdim, mindim, cutoff, ortho, normalize)
+void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
+                 bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept) {
+  PhaseTimer pt(e->ctx, PH_SVD);
+  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "replacebond!: complex SVD is not built");
+  TN_REQUIRE(b >= 1 && b < e->N, TN_E_POSITION, "replacebond!: bond out of range");
+  tn_ctx* c = e->ctx;
+  const long chil = e->A[b - 1].chil, chir = e->A[b].chir;
+  const int d1 = e->d[b - 1], d2 = e->d[b];
+  const long m = chil * d1, n = (long)d2 * chir;
+  SvdOut sv;
+  svd_jacobi(c, phi, m, n, sv);
+  const long p = sv.p;
+  std::vector<int> order(p);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
+  std::vector<double> P(p);
+  for (long i = 0; i < p; ++i) P[i] = sv.s[order[i]] * sv.s[order[i]];
+  const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
+  const long k = tr.n;
+  double nrm = 0.0;
+  for (long i = 0; i < k; ++i) nrm += P[i];
+  nrm = std::sqrt(nrm);
+  const double inv_norm = (normalize && nrm > 0.0) ? 1.0 / nrm : 1.0;
+  std::vector<double> sc_inv_s(k), sc_s_n(k), sc_one(k, 1.0), sc_invn(k, inv_norm), sc_invs_n(k);
+  for (long i = 0; i < k; ++i) {
+    const double s = sv.s[order[i]];
+    sc_inv_s[i] = s > 0.0 ? 1.0 / s : 0.0;
+    sc_s_n[i] = s * inv_norm;
+  }
+  int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
+  double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * k * sizeof(double));
+  TN_HIP(hipMemcpyAsync(dperm, order.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const long ldq = ((p + 63) / 64) * 64;
+  SiteT& Sa = e->A[b - 1];
+  SiteT& Sb = e->A[b];
+  Sa.buf.ensure((size_t)m * k * sizeof(double));
+  Sb.buf.ensure((size_t)k * n * sizeof(double));
+  // scale vectors for the two outputs
+  const std::vector<double>* scA;
+  const std::vector<double>* scB;
+  const double* srcA;
+  long ldA;
+  const double* srcB;
+  long ldB;
+  if (sv.vec_cols) {  // X rows = s_i u_i^T (length m), Qt rows = v_i^T (length n)
+    srcA = sv.X; ldA = sv.q; srcB = sv.Qt; ldB = ldq;
+    if (ortho_left) { scA = &sc_inv_s; scB = &sc_s_n; }
+    else { scA = &sc_invn; scB = &sc_one; }
+  } else {  // X rows = s_i v_i^T (length n), Qt rows = u_i^T (length m)
+    srcA = sv.Qt; ldA = ldq; srcB = sv.X; ldB = sv.q;
+    if (ortho_left) { scA = &sc_one; scB = &sc_invn; }
+    else { scA = &sc_s_n; scB = &sc_inv_s; }
+  }
+  TN_HIP(hipMemcpyAsync(dsc, scA->data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  TN_HIP(hipMemcpyAsync(dsc + k, scB->data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  gather_rows_T(c, srcA, ldA, dperm, dsc, k, m, Sa.buf.as<double>());  // A_b (m x k)
+  gather_rows(c, srcB, ldB, dperm, dsc + k, k, n, Sb.buf.as<double>());  // A_{b+1} (k x n)
+  TN_HIP(hipStreamSynchronize(c->stream));  // host scale vectors go out of scope
+  Sa.chir = k;
+  Sb.chil = k;
+  invalidate_site(e, b);
+  invalidate_site(e, b + 1);
+  e->ortho = ortho_left ? b + 1 : b;
+  if (truncerr_out) *truncerr_out = tr.truncerr;
+  if (eigs_out) eigs_out->assign(P.begin(), P.begin() + k);
+  if (nkept) *nkept = k;
+}
+
+// orthogonalize!(psi, site): exact gauge move with untruncated SVDs
+void orthogonalize(tn_env* e, int site) {
+  TN_REQUIRE(site >= 1 && site <= e->N, TN_E_POSITION, "orthogonalize!: site out of range");
+  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "orthogonalize!: complex SVD is not built");
+  require_sites(e);
+  tn_ctx* c = e->ctx;
+  // right part: sites N..site+1 become right-orthonormal
+  for (int k = e->N; k > site; --k) {
+    SiteT& S = e->A[k - 1];
+    SiteT& P = e->A[k - 2];
+    const long m = S.chil, n = (long)e->d[k - 1] * S.chir;
+    SvdOut sv;
+    svd_jacobi(c, S.buf.as<double>(), m, n, sv);
+    const long p = sv.p;
+    const long ldq = ((p + 63) / 64) * 64;
+    std::vector<int> order(p);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
+    std::vector<double> sc1(p, 1.0), scs(p), scinv(p);
+    for (long i = 0; i < p; ++i) {
+      const double s = sv.s[order[i]];
+      scs[i] = s;
+      scinv[i] = s > 0 ? 1.0 / s : 0.0;
+    }
+    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)p * sizeof(int));
+    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * p * sizeof(double));
+    TN_HIP(hipMemcpyAsync(dperm, order.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // new A_k = V^T (p x n); US (m x p)
+    double* US = (double*)c->workspace(WS_PHI, (size_t)m * p * sizeof(double));
+    DevBuf newA;
+    newA.ensure((size_t)p * n * sizeof(double));
+    if (sv.vec_cols) {  // X rows = s u^T (len m), Qt rows = v^T (len n)
+      TN_HIP(hipMemcpyAsync(dsc, sc1.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      gather_rows(c, sv.Qt, ldq, dperm, dsc, p, n, newA.as<double>());
+      gather_rows_T(c, sv.X, sv.q, dperm, dsc, p, m, US);
+    } else {  // X rows = s v^T (len n), Qt rows = u^T (len m)
+      TN_HIP(hipMemcpyAsync(dsc, scinv.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      TN_HIP(hipMemcpyAsync(dsc + p, scs.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      gather_rows(c, sv.X, sv.q, dperm, dsc, p, n, newA.as<double>());
+      gather_rows_T(c, sv.Qt, ldq, dperm, dsc + p, p, m, US);
+    }
+    // A_{k-1} <- A_{k-1} * US
+    const long pm = P.chil * e->d[k - 2];
+    DevBuf newP;
+    newP.ensure((size_t)pm * p * sizeof(double));
+    GemmDesc g;
+    g.M = pm; g.N = p; g.K = m;
+    g.A = P.buf.p; g.lda = m;
+    g.B = US; g.ldb = p;
+    g.C = newP.p; g.ldc = p;
+    gemm(c, Prec::F64, g);
+    TN_HIP(hipStreamSynchronize(c->stream));
+    S.buf = std::move(newA);
+    S.chil = p;
+    P.buf = std::move(newP);
+    P.chir = p;
+    invalidate_site(e, k);
+    invalidate_site(e, k - 1);
+  }
+  // left part: sites 1..site-1 become left-orthonormal
+  for (int k = 1; k < site; ++k) {
+    SiteT& S = e->A[k - 1];
+    SiteT& Nx = e->A[k];
+    const long m = S.chil * e->d[k - 1], n = S.chir;
+    SvdOut sv;
+    svd_jacobi(c, S.buf.as<double>(), m, n, sv);
+    const long p = sv.p;
+    const long ldq = ((p + 63) / 64) * 64;
+    std::vector<int> order(p);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
+    std::vector<double> sc1(p, 1.0), scs(p), scinv(p);
+    for (long i = 0; i < p; ++i) {
+      const double s = sv.s[order[i]];
+      scs[i] = s;
+      scinv[i] = s > 0 ? 1.0 / s : 0.0;
+    }
+    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)p * sizeof(int));
+    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * p * sizeof(double));
+    TN_HIP(hipMemcpyAsync(dperm, order.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    double* SV = (double*)c->workspace(WS_PHI, (size_t)p * n * sizeof(double));
+    DevBuf newA;
+    newA.ensure((size_t)m * p * sizeof(double));
+    if (sv.vec_cols) {  // X rows = s u^T (len m), Qt rows = v^T (len n)
+      TN_HIP(hipMemcpyAsync(dsc, scinv.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      TN_HIP(hipMemcpyAsync(dsc + p, scs.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      gather_rows_T(c, sv.X, sv.q, dperm, dsc, p, m, newA.as<double>());
+      gather_rows(c, sv.Qt, ldq, dperm, dsc + p, p, n, SV);
+    } else {
+      TN_HIP(hipMemcpyAsync(dsc, sc1.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      gather_rows_T(c, sv.Qt, ldq, dperm, dsc, p, m, newA.as<double>());
+      gather_rows(c, sv.X, sv.q, dperm, dsc, p, n, SV);
+    }
+    const long nn = (long)e->d[k] * Nx.chir;
+    DevBuf newN;
+    newN.ensure((size_t)p * nn * sizeof(double));
+    GemmDesc g;
+    g.M = p; g.N = nn; g.K = n;
+    g.A = SV; g.lda = n;
+    g.B = Nx.buf.p; g.ldb = nn;
+    g.C = newN.p; g.ldc = nn;
+    gemm(c, Prec::F64, g);
+    TN_HIP(hipStreamSynchronize(c->stream));
+    S.buf = std::move(newA);
+    S.chir = p;
+    Nx.buf = std::move(newN);
+    Nx.chil = p;
+    invalidate_site(e, k);
+    invalidate_site(e, k + 1);
+  }
+  e->ortho = site;
+}
+
+// _update_two_site! (src/mps/update_site.jl:27-90)
+void update_two_site(tn_env* e, int pos, bool ortho_left, const tn_update_params* p, tn_update_result* r,
+                     std::vector<double>* eigs_out = nullptr) {
+  TN_REQUIRE(p != nullptr, TN_E_INVALID, "null params");
+  TN_REQUIRE(pos > 0 && pos < e->N, TN_E_POSITION, "update_position!: bond out of range");
+  TN_REQUIRE(!(p->solver == 0 && p->reverse_step), TN_E_INVALID,
+             "update_position!: reverse_step needs a time step (exp_solver)");
+  TN_REQUIRE((ortho_left && e->ortho == pos) || (!ortho_left && e->ortho == pos + 1), TN_E_POSITION,
+             "update_position!: orthocenter is " + std::to_string(e->ortho) + ", expected " +
+                 std::to_string(ortho_left ? pos : pos + 1));
+  tn_ctx* c = e->ctx;
+  e->nsite = 2;
+  const long chil = e->A[pos - 1].chil, chir = e->A[pos].chir;
+  const long n = chil * e->d[pos - 1] * e->d[pos] * chir;
+  const long nd = e->cplx() ? 2 * n : n;
+  double* phi = (double*)c->workspace(WS_PHI, (size_t)nd * sizeof(double));
+  contract_phi(e, pos, phi);
+  position(e, pos);
+  tn_update_result res{};
+  if (r) { res.eigs = r->eigs; res.eigs_cap = r->eigs_cap; }
+  cplx energy(NAN, 0.0);
+  const cplx t(p->time_step_re, p->time_step_im);
+  if (p->solver == 0) {
+    const double th = krylov_eig(e, phi, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
+    if (p->check_convergence && res.info.converged < 1)
+      throw Error(TN_E_NOTCONV, "`eig_solver()` not converged !!");
+    energy = th;
+    if (p->refine_fp64 && p->precision != TN_PREC_FP64) energy = rayleigh(e, phi);
+  } else {
+    krylov_expm(e, phi, t, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
+    if (p->check_convergence && res.info.converged < 1)
+      throw Error(TN_E_NOTCONV, "`eig_solver()` not converged !!");
+  }
+  if (p->normalize) {
+    Krylov K(e, n, 1);
+    const double nr = std::sqrt(K.norm2(phi));
+    if (nr > 0) scale_f64(c, phi, 1.0 / nr, nd);
+  }
+  if (std::isnan(energy.real())) energy = rayleigh(e, phi);
+  double terr = 0.0;
+  long nk = 0;
+  std::vector<double> eigs;
+  replacebond(e, pos, phi, ortho_left, p->maxdim, p->mindim, p->cutoff, p->normalize != 0, &terr, &eigs, &nk);
+  const bool done = (pos == 1 && !ortho_left) || (pos == e->N - 1 && ortho_left);
+  if (p->reverse_step && !done) {
+    const int pos1 = ortho_left ? pos + 1 : pos;
+    SiteT& S = e->A[pos1 - 1];
+    const long n1 = S.chil * e->d[pos1 - 1] * S.chir;
+    const long nd1 = e->cplx() ? 2 * n1 : n1;
+    double* phi0 = (double*)c->workspace(WS_PHI, (size_t)nd1 * sizeof(double));
+    copy_f64(c, phi0, S.buf.as<double>(), nd1);
+    e->nsite = 1;
+    position(e, pos1);
+    krylov_expm(e, phi0, -t, p->krylovdim, p->maxiter, p->tol, p->precision, nullptr);
+    if (p->normalize) {
+      Krylov K(e, n1, 1);
+      const double nr = std::sqrt(K.norm2(phi0));
+      if (nr > 0) scale_f64(c, phi0, 1.0 / nr, nd1);
+    }
+    energy = rayleigh(e, phi0);
+    copy_f64(c, S.buf.as<double>(), phi0, nd1);
+    invalidate_site(e, pos1);
+  }
+  res.energy = energy.real();
+  res.energy_im = energy.imag();
+  res.truncerr = terr;
+  res.nkept = nk;
+  if (res.eigs)
+    for (long i = 0; i < std::min<long>(res.eigs_cap, (long)eigs.size()); ++i) res.eigs[i] = eigs[i];
+  if (r) *r = res;
+  if (eigs_out) eigs_out->swap(eigs);
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int tn_ctx_create(int device, tn_ctx** out) {
+  return guarded(nullptr, [&] {
+    TN_REQUIRE(out != nullptr, TN_E_INVALID, "null output pointer");
+    int ndev = 0;
+    TN_HIP(hipGetDeviceCount(&ndev));
+    TN_REQUIRE(device >= 0 && device < ndev, TN_E_INVALID, "no such device");
+    TN_HIP(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    TN_HIP(hipGetDeviceProperties(&prop, device));
+    TN_REQUIRE(std::string(prop.gcnArchName).rfind("gfx950", 0) == 0, TN_E_UNSUPPORTED,
+               std::string("libtnhip is built for gfx950 only, device is ") + prop.gcnArchName);
+    auto* c = new tn_ctx();
+    c->device = device;
+    TN_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->ws.resize(WS_SLOTS);
+    c->ones_buf.ensure(16);
+    double one[2] = {1.0, 0.0};
+    TN_HIP(hipMemcpy(c->ones_buf.p, one, 16, hipMemcpyHostToDevice));
+    c->ones = c->ones_buf.as<double>();
+    const char* tv = std::getenv("TNHIP_TIMING");
+    c->timing = tv && tv[0] == '1';
+    *out = c;
+  });
+}
+
+int tn_ctx_destroy(tn_ctx* c) {
+  if (!c) return TN_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& pr : c->ev_pool) {
+    (void)hipEventDestroy(pr.first);
+    (void)hipEventDestroy(pr.second);
+  }
+  c->ws.clear();
+  c->ones_buf.release();
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TN_OK;
+}
+
+const char* tn_last_error(const tn_ctx* c) { return c ? c->last_error.c_str() : g_thread_error.c_str(); }
+
+int tn_ctx_synchronize(tn_ctx* c) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    TN_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+int tn_alloc(tn_ctx* c, size_t bytes, void** dptr) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c && dptr, TN_E_INVALID, "null argument");
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 16);
+    if (e != hipSuccess) throw Error(TN_E_NOMEM, std::string("tn_alloc: ") + hipGetErrorString(e));
+  });
+}
+int tn_free(tn_ctx* c, void* dptr) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    if (dptr) {
+      TN_HIP(hipStreamSynchronize(c->stream));
+      TN_HIP(hipFree(dptr));
+    }
+  });
+}
+int tn_memcpy_h2d(tn_ctx* c, void* dst, const void* src, size_t bytes) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    if (!bytes) return;
+    TN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+int tn_memcpy_d2h(tn_ctx* c, void* dst, const void* src, size_t bytes) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    if (!bytes) return;
+    TN_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+int tn_ctx_timers(tn_ctx* c, double* ms4, int reset) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    flush_timers(c);
+    if (ms4)
+      for (int i = 0; i < PH_COUNT; ++i) ms4[i] = c->phase_ms[i];
+    if (reset)
+      for (int i = 0; i < PH_COUNT; ++i) c->phase_ms[i] = 0.0;
+  });
+}
+int tn_ctx_enable_timing(tn_ctx* c, int on) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    flush_timers(c);
+    c->timing = on != 0;
+  });
+}
+
+void tn_update_params_default(tn_update_params* p, int solver) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->solver = solver;
+  p->normalize = 1;
+  p->maxdim = 9223372036854775807L;
+  p->mindim = 1;
+  p->cutoff = 1e-15;
+  p->reverse_step = solver == 1 ? 1 : 0;
+  p->krylovdim = solver == 1 ? 30 : 5;
+  p->maxiter = solver == 1 ? 100 : 2;
+  p->tol = solver == 1 ? 1e-12 : 1e-14;
+  p->check_convergence = 0;
+  p->precision = TN_PREC_FP64;
+  p->refine_fp64 = 1;
+}
+
+int tn_env_create(tn_ctx* c, int N, const int* d, const int* w, int dtype, tn_env** out) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c && d && w && out, TN_E_INVALID, "null argument");
+    TN_REQUIRE(N >= 2, TN_E_INVALID, "StateEnvs: need at least 2 sites");
+    TN_REQUIRE(dtype == TN_F64 || dtype == TN_C128, TN_E_INVALID, "StateEnvs: unknown dtype");
+    TN_REQUIRE(w[0] == 1 && w[N] == 1, TN_E_SHAPE, "StateEnvs: MPO edge bond dimensions must be 1");
+    for (int k = 0; k < N; ++k) TN_REQUIRE(d[k] >= 1 && w[k] >= 1, TN_E_INVALID, "StateEnvs: bad dims");
+    auto* e = new tn_env();
+    e->ctx = c;
+    e->N = N;
+    e->d.assign(d, d + N);
+    e->w.assign(w, w + N + 1);
+    e->dtype = dtype;
+    e->esz = dtype == TN_C128 ? 16 : 8;
+    e->Wh.resize(N);
+    e->W_set.assign(N, 0);
+    e->A.resize(N);
+    e->LR.resize(N + 2);
+    e->LR_version.assign(N + 2, 0);
+    e->lpos = 0;
+    e->rpos = N + 1;
+    e->opL.resize(N);
+    e->opR.resize(N);
+    e->op2.resize(N);
+    e->opL_ok.assign(N, 0);
+    e->opR_ok.assign(N, 0);
+    e->op2_ok.assign(N, 0);
+    set_edge(e, 0);
+    set_edge(e, N + 1);
+    *out = e;
+  });
+}
+
+int tn_env_destroy(tn_env* e) {
+  if (!e) return TN_OK;
+  (void)hipStreamSynchronize(e->ctx->stream);
+  delete e;
+  return TN_OK;
+}
+
+int tn_env_set_mpo(tn_env* e, int k, const double* W) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(k >= 1 && k <= e->N && W, TN_E_INVALID, "set_mpo: bad site");
+    const size_t n = (size_t)e->w[k - 1] * e->d[k - 1] * e->d[k - 1] * e->w[k];
+    e->Wh[k - 1].assign(W, W + n);
+    e->W_set[k - 1] = 1;
+    e->opL_ok[k - 1] = e->opR_ok[k - 1] = 0;
+    if (k >= 2) e->op2_ok[k - 2] = 0;
+    e->op2_ok[k - 1] = 0;
+    e->lpos = std::min(e->lpos, k - 1);
+    e->rpos = std::max(e->rpos, k + 1);
+  });
+}
+
+int tn_env_set_site(tn_env* e, int k, const void* A, long chil, long chir) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(k >= 1 && k <= e->N && A, TN_E_INVALID, "set_site: bad site");
+    TN_REQUIRE(chil >= 1 && chir >= 1, TN_E_SHAPE, "set_site: bad bond dimensions");
+    TN_REQUIRE((k > 1 || chil == 1) && (k < e->N || chir == 1), TN_E_SHAPE,
+               "set_site: edge bond dimensions must be 1");
+    SiteT& S = e->A[k - 1];
+    const size_t bytes = (size_t)chil * e->d[k - 1] * chir * e->esz;
+    S.buf.ensure(bytes);
+    TN_HIP(hipMemcpyAsync(S.buf.p, A, bytes, hipMemcpyHostToDevice, e->ctx->stream));
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+    S.chil = chil;
+    S.chir = chir;
+    invalidate_site(e, k);
+    e->ortho = 0;
+  });
+}
+
+int tn_env_get_site_dims(tn_env* e, int k, long* chil, long* chir) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(k >= 1 && k <= e->N, TN_E_INVALID, "get_site_dims: bad site");
+    if (chil) *chil = e->A[k - 1].chil;
+    if (chir) *chir = e->A[k - 1].chir;
+  });
+}
+
+int tn_env_get_site(tn_env* e, int k, void* A) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(k >= 1 && k <= e->N && A, TN_E_INVALID, "get_site: bad site");
+    const SiteT& S = e->A[k - 1];
+    TN_REQUIRE(S.buf.p, TN_E_INVALID, "get_site: site not set");
+    const size_t bytes = (size_t)S.chil * e->d[k - 1] * S.chir * e->esz;
+    TN_HIP(hipMemcpyAsync(A, S.buf.p, bytes, hipMemcpyDeviceToHost, e->ctx->stream));
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+  });
+}
+
+int tn_env_set_orthocenter(tn_env* e, int k) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(k >= 0 && k <= e->N, TN_E_INVALID, "set_orthocenter: bad site");
+    e->ortho = k;
+  });
+}
+int tn_env_orthocenter(tn_env* e, int* k) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(k, TN_E_INVALID, "null output");
+    *k = e->ortho;
+  });
+}
+int tn_env_set_nsite(tn_env* e, int nsite) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(nsite == 1 || nsite == 2, TN_E_UNSUPPORTED, "set_nsite!: only nsite 1 and 2 are built");
+    e->nsite = nsite;
+  });
+}
+int tn_env_nsite(tn_env* e, int* nsite) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(nsite, TN_E_INVALID, "null output");
+    *nsite = e->nsite;
+  });
+}
+int tn_env_position(tn_env* e, int pos) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    position(e, pos);
+  });
+}
+int tn_env_reset(tn_env* e) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    e->lpos = 0;
+    e->rpos = e->N + 1;
+  });
+}
+int tn_env_phi_dims(tn_env* e, long* dims4, long* n) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    const Dom D = domain(e);
+    if (dims4) { dims4[0] = D.chil; dims4[1] = D.d1; dims4[2] = D.d2; dims4[3] = D.chir; }
+    if (n) *n = D.n;
+  });
+}
+int tn_env_matvec(tn_env* e, const void* v, void* out, int precision) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(v && out, TN_E_INVALID, "matvec: null vector");
+    matvec(e, v, out, precision);
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+  });
+}
+int tn_env_contract_phi(tn_env* e, int pos, void* phi) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(phi, TN_E_INVALID, "contract_phi: null output");
+    contract_phi(e, pos, phi);
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+  });
+}
+int tn_env_orthogonalize(tn_env* e, int site) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    orthogonalize(e, site);
+  });
+}
+
+int tn_krylov_eig(tn_env* e, void* phi, int kdim, int maxiter, double tol, int precision, double* eig,
+                  tn_krylov_info* info) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(phi, TN_E_INVALID, "eig_solver: null vector");
+    const double th = krylov_eig(e, (double*)phi, kdim, maxiter, tol, precision, info);
+    if (eig) *eig = th;
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+  });
+}
+int tn_krylov_expm(tn_env* e, void* phi, double t_re, double t_im, int kdim, int maxiter, double tol,
+                   int precision, tn_krylov_info* info) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(phi, TN_E_INVALID, "exp_solver: null vector");
+    krylov_expm(e, (double*)phi, cplx(t_re, t_im), kdim, maxiter, tol, precision, info);
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+  });
+}
+int tn_rayleigh_fp64(tn_env* e, const void* phi, double* e2) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(phi && e2, TN_E_INVALID, "rayleigh: null argument");
+    const cplx r = rayleigh(e, (const double*)phi);
+    e2[0] = r.real();
+    e2[1] = r.imag();
+  });
+}
+
+int tn_replacebond(tn_env* e, int b, const void* phi, int ortho_left, long maxdim, long mindim, double cutoff,
+                   int normalize, double* truncerr, double* eigs_out, long eigs_cap, long* nkept) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(phi, TN_E_INVALID, "replacebond!: null phi");
+    std::vector<double> eigs;
+    replacebond(e, b, (const double*)phi, ortho_left != 0, maxdim, mindim, cutoff, normalize != 0, truncerr,
+                &eigs, nkept);
+    if (eigs_out)
+      for (long i = 0; i < std::min<long>(eigs_cap, (long)eigs.size()); ++i) eigs_out[i] = eigs[i];
+  });
+}
+
+int tn_svd_trunc(tn_ctx* c, const double* M, long m, long n, long maxdim, long mindim, double cutoff, double* U,
+                 double* S, double* Vt, long* nkept, double* truncerr) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c && M, TN_E_INVALID, "svd: null argument");
+    PhaseTimer pt(c, PH_SVD);
+    SvdOut sv;
+    svd_jacobi(c, M, m, n, sv);
+    const long p = sv.p;
+    std::vector<int> order(p);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
+    std::vector<double> P(p);
+    for (long i = 0; i < p; ++i) P[i] = sv.s[order[i]] * sv.s[order[i]];
+    const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
+    const long k = tr.n;
+    const long ldq = ((p + 63) / 64) * 64;
+    std::vector<double> one(k, 1.0), inv(k);
+    for (long i = 0; i < k; ++i) {
+      const double s = sv.s[order[i]];
+      inv[i] = s > 0 ? 1.0 / s : 0.0;
+      if (S) S[i] = s;
+    }
+    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
+    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * k * sizeof(double));
+    TN_HIP(hipMemcpyAsync(dperm, order.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(dsc, one.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(dsc + k, inv.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (sv.vec_cols) {  // X rows = s u^T, Qt rows = v^T
+      if (U) gather_rows_T(c, sv.X, sv.q, dperm, dsc + k, k, m, U);
+      if (Vt) gather_rows(c, sv.Qt, ldq, dperm, dsc, k, n, Vt);
+    } else {  // X rows = s v^T, Qt rows = u^T
+      if (U) gather_rows_T(c, sv.Qt, ldq, dperm, dsc, k, m, U);
+      if (Vt) gather_rows(c, sv.X, sv.q, dperm, dsc + k, k, n, Vt);
+    }
+    TN_HIP(hipStreamSynchronize(c->stream));
+    if (nkept) *nkept = k;
+    if (truncerr) *truncerr = tr.truncerr;
+  });
+}
+
+int tn_update_two_site(tn_env* e, int pos, int ortho_left, const tn_update_params* p, tn_update_result* r) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    update_two_site(e, pos, ortho_left != 0, p, r);
+  });
+}
+
+int tn_fullsweep(tn_env* e, const tn_update_params* p, double* energy, double* maxtruncerr, long* maxchi,
+                 double* mid_eigs, long cap, long* nmid) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(p, TN_E_INVALID, "null params");
+    if (e->ortho != 1) orthogonalize(e, 1);
+    const int N = e->N;
+    double en = NAN, mte = 0.0;
+    std::vector<double> mid;
+    tn_update_result r;
+    for (int b = 1; b <= N - 1; ++b) {
+      update_two_site(e, b, true, p, &r);
+      en = r.energy;
+      mte = std::max(mte, r.truncerr);
+    }
+    for (int b = N - 1; b >= 1; --b) {
+      std::vector<double> eigs;
+      update_two_site(e, b, false, p, &r, &eigs);
+      en = r.energy;
+      mte = std::max(mte, r.truncerr);
+      if (b == N / 2) mid.swap(eigs);  // lasteigs[N/2] (sweep.jl:141,158)
+    }
+    long mc = 0;
+    for (int k = 0; k < N; ++k) mc = std::max(mc, e->A[k].chir);
+    if (energy) *energy = en;
+    if (maxtruncerr) *maxtruncerr = mte;
+    if (maxchi) *maxchi = mc;
+    if (nmid) *nmid = (long)mid.size();
+    if (mid_eigs)
+      for (long i = 0; i < std::min<long>(cap, (long)mid.size()); ++i) mid_eigs[i] = mid[i];
+  });
+}
+
+}  // extern "C"

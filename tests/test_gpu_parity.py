@@ -1,0 +1,269 @@
+"""GPU parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances: FP64 forms to 1e-12 relative (different summation order only);
+FP32-MFMA matvec 2e-6 and BF16x3 1e-4 relative to |H psi| (their input
+rounding); energies of the mixed forms after FP64 refinement within
+|dE/N| <= 1e-9 of the FP64 form.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import tn_oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def T():
+    import tennetlib_amd as T
+    T.default_context()
+    return T
+
+
+def rand_mps(N, d, chis, seed, cplx=False):
+    rng = np.random.default_rng(seed)
+    dims = [1] + list(chis) + [1]
+    A = []
+    for k in range(N):
+        a = rng.standard_normal((dims[k], d, dims[k + 1]))
+        if cplx:
+            a = a + 1j * rng.standard_normal(a.shape)
+        A.append(a / np.sqrt(dims[k] * d))
+    return A
+
+
+CASES = [
+    # (model, N, chis, positions)
+    ("xxz", 8, [2, 4, 8, 16, 8, 4, 2], [1, 4, 7]),
+    ("xxz", 6, [2, 4, 65, 4, 2], [2, 3]),
+    ("xxz", 6, [2, 4, 70, 129, 2], [3, 4]),
+    ("hubbard", 5, [4, 16, 37, 4], [1, 2, 4]),
+]
+
+
+def model(name, N):
+    if name == "xxz":
+        return O.xxz_mpo(N, 1.0, 0.7, np.linspace(-0.4, 0.6, N))
+    return O.hubbard_mpo(N, 1.0, 4.0)
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("nsite", [2, 1])
+def test_matvec_fp64_matches_oracle(T, case, nsite):
+    name, N, chis, positions = CASES[case]
+    Ws = model(name, N)
+    d = Ws[0].shape[1]
+    A = rand_mps(N, d, chis, seed=case)
+    env = T.StateEnvs(A, Ws)
+    ref = O.Envs(Ws)
+    ref.nsite = nsite
+    T.set_nsite_(env, nsite)
+    rng = np.random.default_rng(100 + case)
+    for p in positions:
+        if p + nsite - 1 > N:
+            continue
+        print(f"matvec case={case} nsite={nsite} pos={p}", flush=True)
+        T.position_(env, p)
+        ref.position(A, p)
+        dims, n = env.phi_dims()
+        shape = (dims[0], dims[1], dims[2], dims[3]) if nsite == 2 else (dims[0], dims[1], dims[3])
+        psi = rng.standard_normal(shape)
+        want = ref.product(psi)
+        got = T.product(env, psi).reshape(want.shape)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), (p, nsite)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("bf16x3", 1e-4)])
+def test_matvec_mixed_precision(T, prec, tol):
+    for case in (0, 2, 3):
+        name, N, chis, positions = CASES[case]
+        Ws = model(name, N)
+        d = Ws[0].shape[1]
+        A = rand_mps(N, d, chis, seed=case)
+        env = T.StateEnvs(A, Ws)
+        ref = O.Envs(Ws)
+        for p in positions:
+            if p + 1 > N:
+                continue
+            print(f"mixed {prec} case={case} pos={p}", flush=True)
+            T.position_(env, p)
+            ref.position(A, p)
+            dims, n = env.phi_dims()
+            psi = np.random.default_rng(p).standard_normal(dims)
+            want = ref.product(psi)
+            got = T.product(env, psi, precision=prec).reshape(want.shape)
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            assert err <= tol, (prec, case, p, err)
+            assert err > 1e-12  # really computed in reduced precision
+
+
+def test_matvec_complex_matches_oracle(T):
+    N = 6
+    Ws = O.xxz_mpo(N, 1.0, 0.5)
+    A = rand_mps(N, 2, [2, 4, 33, 4, 2], seed=7, cplx=True)
+    env = T.StateEnvs(A, Ws)
+    ref = O.Envs(Ws, dtype=np.complex128)
+    rng = np.random.default_rng(8)
+    for nsite, positions in ((2, [1, 3, 5]), (1, [2, 6])):
+        T.set_nsite_(env, nsite)
+        ref.nsite = nsite
+        for p in positions:
+            print(f"complex nsite={nsite} pos={p}", flush=True)
+            T.position_(env, p)
+            ref.position(A, p)
+            dims, n = env.phi_dims()
+            shape = dims if nsite == 2 else (dims[0], dims[1], dims[3])
+            psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+            want = ref.product(psi)
+            got = T.product(env, psi).reshape(want.shape)
+            assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def _svd_gpu(T, M, maxdim=10 ** 9, mindim=1, cutoff=-1.0):
+    import ctypes as C
+    ctx = T.default_context()
+    m, n = M.shape
+    dM = T.DeviceArray.from_numpy(ctx, M)
+    p = min(m, n)
+    dU = T.DeviceArray(ctx, (m, p))
+    dV = T.DeviceArray(ctx, (p, n))
+    S = (C.c_double * p)()
+    k = C.c_long()
+    te = C.c_double()
+    ctx.check(ctx.lib.tn_svd_trunc(ctx.h, dM.ptr, m, n, maxdim, mindim, cutoff, dU.ptr, S, dV.ptr,
+                                   C.byref(k), C.byref(te)))
+    k = k.value
+    U = dU.numpy().reshape(-1)[: m * k].reshape(m, k)
+    V = dV.numpy().reshape(-1)[: k * n].reshape(k, n)
+    return U, np.array(S[:k]), V, te.value
+
+
+@pytest.mark.parametrize("m,n,grade", [(1, 1, 0.0), (1, 7, 0.0), (7, 1, 0.0), (30, 20, 0.5), (20, 30, 0.5),
+                                       (64, 64, 0.1), (200, 130, 0.2), (129, 257, 0.05), (300, 300, 0.03)])
+def test_svd_matches_lapack(T, m, n, grade):
+    print(f"svd {m}x{n}", flush=True)
+    rng = np.random.default_rng(m * 1000 + n)
+    p = min(m, n)
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, p)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, p)))
+    s = np.exp(-grade * np.arange(p))
+    M = (Q1 * s) @ Q2.T
+    U, S, V, _ = _svd_gpu(T, M)
+    Sref = np.linalg.svd(M, compute_uv=False)
+    assert len(S) == p
+    # high relative accuracy of the small singular values
+    assert np.all(np.abs(S - Sref) <= 1e-12 * Sref + 1e-15 * Sref[0])
+    assert np.abs(U.T @ U - np.eye(p)).max() < 1e-12
+    assert np.abs(V @ V.T - np.eye(p)).max() < 1e-12
+    assert np.abs((U * S) @ V - M).max() < 1e-12 * Sref[0]
+
+
+def test_svd_truncation_rule(T):
+    rng = np.random.default_rng(5)
+    m, n = 90, 60
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.exp(-0.4 * np.arange(n))
+    M = (Q1 * s) @ Q2.T
+    for maxdim, cutoff in [(10, 0.0), (100, 1e-8), (100, 1e-12), (5, 1e-3), (100, -1.0)]:
+        U, S, V, te = _svd_gpu(T, M, maxdim=maxdim, cutoff=cutoff)
+        ref = O.truncate_weights(s ** 2, maxdim, 1, cutoff)
+        assert len(S) == ref.n, (maxdim, cutoff)
+        assert abs(te - ref.truncerr) <= 1e-14 + 1e-10 * ref.truncerr
+
+
+def test_krylov_eig_matches_oracle(T):
+    import ctypes as C
+    N = 8
+    Ws = O.heisenberg_mpo(N)
+    A = rand_mps(N, 2, [2, 4, 8, 16, 8, 4, 2], seed=3)
+    env = T.StateEnvs(A, Ws)
+    ref = O.Envs(Ws)
+    ctx = env.ctx
+    for p, kd in [(3, 3), (4, 5), (1, 3)]:
+        T.position_(env, p)
+        ref.position(A, p)
+        dims, n = env.phi_dims()
+        x0 = np.random.default_rng(p).standard_normal(dims)
+        th_ref, y_ref, info_ref = O.eigsolve_lanczos(ref.product, x0, krylovdim=kd, maxiter=2, tol=1e-14)
+        dx = T.DeviceArray.from_numpy(ctx, x0)
+        th = C.c_double()
+        info = T._lib.KrylovInfo()
+        ctx.check(ctx.lib.tn_krylov_eig(env.h, dx.ptr, kd, 2, 1e-14, 0, C.byref(th), C.byref(info)))
+        y = dx.numpy()
+        assert info.numops == info_ref.numops
+        assert abs(th.value - th_ref) < 1e-12 * max(1.0, abs(th_ref))
+        assert abs(abs(np.vdot(y, y_ref)) - 1) < 1e-9
+
+
+def test_dmrg_c1_reproduces_golden(T):
+    g = json.loads((GOLDEN / "c1_heisenberg_n20.json").read_text())
+    env = T.StateEnvs(O.random_product_mps(20, 2, seed=1), O.heisenberg_mpo(20))
+    params = T.make_dmrg_params(maxdim=g["maxdim"], nsweeps=[1] * 5, cutoff=g["cutoff"])
+    sw = T.dmrg_(env, params, 2, solver_krylovdim=3, solver_maxiter=2, outputlevel=0)
+    assert sw.maxchi == g["maxchi"]
+    assert np.allclose(sw.energy, g["energy"], rtol=0, atol=1e-10)
+    assert np.allclose(sw.entropy, g["entropy"], rtol=0, atol=1e-9)
+    e0 = -8.68247333439896  # ED, N=20
+    assert e0 - 1e-12 <= sw.energy[-1] < e0 + 1e-6
+
+
+def test_dmrg_n12_reaches_ed(T):
+    env = T.StateEnvs(O.neel_mps(12), O.heisenberg_mpo(12))
+    params = T.make_dmrg_params(maxdim=[64], nsweeps=[4], cutoff=1e-14)
+    sw = T.dmrg_(env, params, 2, outputlevel=0)
+    assert abs(sw.energy[-1] - (-5.14209063284054)) < 1e-10
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_dmrg_mixed_precision_energy(T, prec):
+    N = 20
+    runs = {}
+    for pr in ("fp64", prec):
+        env = T.StateEnvs(O.random_product_mps(N, 2, seed=1), O.heisenberg_mpo(N))
+        params = T.make_dmrg_params(maxdim=[16, 32, 64, 64, 64, 64], nsweeps=[1] * 6, cutoff=1e-10)
+        sw = T.dmrg_(env, params, 2, solver_krylovdim=3, solver_maxiter=2, outputlevel=0, precision=pr)
+        runs[pr] = sw.energy[-1]
+    assert abs(runs[prec] - runs["fp64"]) / N <= 1e-9, runs
+
+
+def test_c_fullsweep_matches_python_sweep(T):
+    import ctypes as C
+    N = 14
+    envs = [T.StateEnvs(O.random_product_mps(N, 2, seed=2), O.heisenberg_mpo(N)) for _ in range(2)]
+    sw = T.SweepData()
+    kw = dict(maxdim=32, cutoff=1e-12, solver_krylovdim=3, solver_maxiter=2, outputlevel=0)
+    e_py = []
+    for _ in range(3):
+        T.fullsweep_(envs[0], T.eig_solver, 2, sw, **kw)
+    p = T.core._params(T.eig_solver, kw)
+    ctx = envs[1].ctx
+    for _ in range(3):
+        e = C.c_double(); mt = C.c_double(); mc = C.c_long(); nm = C.c_long()
+        buf = (C.c_double * 4096)()
+        ctx.check(ctx.lib.tn_fullsweep(envs[1].h, C.byref(p), C.byref(e), C.byref(mt), C.byref(mc), buf, 4096,
+                                       C.byref(nm)))
+        e_py.append(e.value)
+    assert abs(e_py[-1] - sw.energy[-1]) < 1e-12
+    mid = np.array(buf[: nm.value])
+    assert np.allclose(mid, sw.lasteigs[N // 2 - 1], atol=1e-14)
+
+
+def test_error_paths(T):
+    N = 6
+    env = T.StateEnvs(O.neel_mps(N), O.heisenberg_mpo(N))
+    with pytest.raises(T.TNHIPError) as ei:
+        T.position_(env, N)  # nsite=2 at the last site
+    assert ei.value.code == T._lib.TN_E_POSITION
+    T.position_(env, 2)
+    with pytest.raises(T.TNHIPError) as ei:
+        T.product(env, np.zeros(5))
+    assert ei.value.code == T._lib.TN_E_SHAPE
+    with pytest.raises(T.TNHIPError) as ei:
+        T.update_position_(env, T.eig_solver, 3, 2, "left")  # orthocenter unknown
+    assert ei.value.code == T._lib.TN_E_POSITION

@@ -1,0 +1,379 @@
+"""Pins of the FP64 CPU oracle (oracle/tn_oracle.py) -- CPU only.
+
+Each restated function is checked against something that fixes it
+independently: explicit Kronecker-product Hamiltonians and exact
+diagonalisation, the P^T H P projection of the dense Hamiltonian, explicit
+Krylov-space constructions, scipy.linalg.expm, closed forms (free fermions,
+hand-computed truncation cases) and the committed golden fixtures.
+"""
+
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sps
+import scipy.sparse.linalg as spla
+
+from oracle import tn_oracle as O
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+# --------------------------------------------------------------------------
+# independent Hamiltonian constructions (Kronecker products)
+# --------------------------------------------------------------------------
+SZ = np.diag([0.5, -0.5])
+SP = np.array([[0.0, 1.0], [0.0, 0.0]])
+SM = SP.T
+
+
+def _site_op(op, i, N, d=2, sparse=False):
+    mats = [sps.identity(d, format="csr") if sparse else np.eye(d) for _ in range(N)]
+    mats[i] = sps.csr_matrix(op) if sparse else op
+    out = mats[0]
+    for m in mats[1:]:
+        out = sps.kron(out, m, format="csr") if sparse else np.kron(out, m)
+    return out
+
+
+def xxz_dense(N, J=1.0, Delta=1.0, h=None, sparse=False):
+    D = 2 ** N
+    H = sps.csr_matrix((D, D)) if sparse else np.zeros((D, D))
+    for i in range(N - 1):
+        H = H + 0.5 * J * (_site_op(SP, i, N, sparse=sparse) @ _site_op(SM, i + 1, N, sparse=sparse)
+                           + _site_op(SM, i, N, sparse=sparse) @ _site_op(SP, i + 1, N, sparse=sparse))
+        H = H + J * Delta * _site_op(SZ, i, N, sparse=sparse) @ _site_op(SZ, i + 1, N, sparse=sparse)
+    if h is not None:
+        for i in range(N):
+            H = H + h[i] * _site_op(SZ, i, N, sparse=sparse)
+    return H
+
+
+def test_xxz_mpo_matches_kronecker_construction():
+    h = [0.3, -0.7, 1.1, 0.0, 0.25, -0.4]
+    for N, J, De, hh in [(2, 1.0, 1.0, None), (6, 1.0, 1.0, None), (6, 0.8, 0.5, h), (5, 1.3, 0.0, None)]:
+        Hm = O.mpo_to_dense(O.xxz_mpo(N, J, De, hh[:N] if hh else None))
+        He = xxz_dense(N, J, De, hh[:N] if hh else None)
+        assert np.abs(Hm - He).max() < 1e-14
+
+
+def test_heisenberg_ed_pins():
+    # OBC Heisenberg ground energies (exact diagonalisation of the Kronecker
+    # construction; N=12 is the reference test size, test/test_MPS_DMRG.jl)
+    H6 = xxz_dense(6)
+    assert abs(np.linalg.eigvalsh(H6)[0] - (-2.4935771338879276)) < 1e-12
+    H12 = xxz_dense(12, sparse=True)
+    e = spla.eigsh(H12, k=1, which="SA", tol=1e-13)[0][0]
+    assert abs(e - (-5.14209063284054)) < 1e-10
+
+
+def _cdag_c_dense(N):
+    """Jordan-Wigner fermion operators on a spinful chain, site ordering
+    (site, spin) with up before down inside a site -- built from scratch."""
+    Z = np.diag([1.0, -1.0])
+    a = np.array([[0.0, 1.0], [0.0, 0.0]])  # annihilates an occupied mode (|1> = index 1)
+    modes = 2 * N
+    ops = []
+    for m in range(modes):
+        mats = [Z] * m + [a] + [np.eye(2)] * (modes - m - 1)
+        out = mats[0]
+        for x in mats[1:]:
+            out = np.kron(out, x)
+        ops.append(out)
+    return ops
+
+
+def test_hubbard_mpo_matches_second_quantisation():
+    N, t, U = 3, 1.0, 4.0
+    Hm = O.mpo_to_dense(O.hubbard_mpo(N, t, U))
+    c = _cdag_c_dense(N)
+    dim = 4 ** N
+    # map MPO basis (per site |0>,|up>,|dn>,|updn>) to mode occupations (n_up, n_dn)
+    occ = {0: (0, 0), 1: (1, 0), 2: (0, 1), 3: (1, 1)}
+    He = np.zeros((dim, dim))
+    for i in range(N - 1):
+        for s in range(2):
+            a, b = 2 * i + s, 2 * (i + 1) + s
+            He += -t * (c[a].T @ c[b] + c[b].T @ c[a])
+    for i in range(N):
+        nu = c[2 * i].T @ c[2 * i]
+        nd = c[2 * i + 1].T @ c[2 * i + 1]
+        He += U * nu @ nd
+    # spectra agree (the two bases differ by fermionic sign conventions only)
+    assert np.allclose(np.linalg.eigvalsh(Hm), np.linalg.eigvalsh(He), atol=1e-12)
+    # U = 0: free fermions, E0 = 2 * sum of negative single-particle levels
+    H0 = O.mpo_to_dense(O.hubbard_mpo(4, 1.0, 0.0))
+    eps = -2 * np.cos(np.pi * np.arange(1, 5) / 5)
+    assert abs(np.linalg.eigvalsh(H0)[0] - 2 * eps[eps < 0].sum()) < 1e-12
+
+
+# --------------------------------------------------------------------------
+# environments and projected products against P^T H P
+# --------------------------------------------------------------------------
+def _random_mps(N, d, chis, seed):
+    rng = np.random.default_rng(seed)
+    dims = [1] + list(chis) + [1]
+    return [rng.standard_normal((dims[k], d, dims[k + 1])) for k in range(N)]
+
+
+def _mixed_canonical(A, p, nsite):
+    """Left-orthonormalise sites < p, right-orthonormalise sites > p+nsite-1."""
+    A = [a.copy() for a in A]
+    N = len(A)
+    for k in range(p - 1):
+        a, s, b = A[k].shape
+        Q, R = np.linalg.qr(A[k].reshape(a * s, b))
+        A[k] = Q.reshape(a, s, Q.shape[1])
+        A[k + 1] = np.tensordot(R, A[k + 1], axes=([1], [0]))
+    for k in range(N - 1, p + nsite - 2, -1):
+        a, s, b = A[k].shape
+        Q, R = np.linalg.qr(A[k].reshape(a, s * b).T)
+        A[k] = Q.T.reshape(Q.shape[1], s, b)
+        A[k - 1] = np.tensordot(A[k - 1], R.T, axes=([2], [0]))
+    return A
+
+
+def _left_iso(A, p):
+    """Matrix (d^(p-1) x chi) of the left block (sites 1..p-1)."""
+    v = np.ones((1, 1))
+    for k in range(p - 1):
+        v = np.tensordot(v, A[k], axes=([1], [0])).reshape(-1, A[k].shape[2])
+    return v
+
+
+def _right_iso(A, q):
+    """Matrix (chi x d^(N-q)) of the right block (sites q+1..N)."""
+    v = np.ones((1, 1))
+    for k in range(len(A) - 1, q - 1, -1):
+        v = np.tensordot(A[k], v, axes=([2], [0])).reshape(A[k].shape[0], -1)
+    return v
+
+
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_matvec2_equals_projected_hamiltonian(p):
+    N, d = 6, 2
+    h = [0.2, -0.5, 0.9, 0.1, -0.3, 0.7]
+    Ws = O.xxz_mpo(N, 1.0, 0.7, h)
+    A = _mixed_canonical(_random_mps(N, d, [2, 4, 5, 4, 2], seed=p), p, 2)
+    env = O.Envs(Ws)
+    env.nsite = 2
+    env.position(A, p)
+    Lm, Rm = _left_iso(A, p), _right_iso(A, p + 1)
+    H = xxz_dense(N, 1.0, 0.7, h)
+    P = np.kron(np.kron(Lm, np.eye(d * d)), Rm.T)  # (2^N) x (chil d d chir)
+    Heff = P.T @ H @ P
+    rng = np.random.default_rng(10 + p)
+    psi = rng.standard_normal((Lm.shape[1], d, d, Rm.shape[0]))
+    got = env.product(psi).reshape(-1)
+    assert np.abs(got - Heff @ psi.reshape(-1)).max() < 1e-12
+    # hermiticity of the projected operator
+    x = rng.standard_normal(psi.shape)
+    assert abs(np.vdot(x, env.product(psi)) - np.vdot(env.product(x), psi)) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 4, 6])
+def test_matvec1_equals_projected_hamiltonian(p):
+    N, d = 6, 2
+    Ws = O.heisenberg_mpo(N)
+    A = _mixed_canonical(_random_mps(N, d, [2, 3, 5, 3, 2], seed=20 + p), p, 1)
+    env = O.Envs(Ws)
+    env.nsite = 1
+    env.position(A, p)
+    Lm, Rm = _left_iso(A, p), _right_iso(A, p)
+    P = np.kron(np.kron(Lm, np.eye(d)), Rm.T)
+    Heff = P.T @ xxz_dense(N) @ P
+    psi = np.random.default_rng(p).standard_normal((Lm.shape[1], d, Rm.shape[0]))
+    assert np.abs(env.product(psi).reshape(-1) - Heff @ psi.reshape(-1)).max() < 1e-12
+
+
+# --------------------------------------------------------------------------
+# Krylov solvers
+# --------------------------------------------------------------------------
+def _rand_sym(n, seed):
+    r = np.random.default_rng(seed)
+    M = r.standard_normal((n, n))
+    return 0.5 * (M + M.T)
+
+
+def test_lanczos_full_space_is_exact():
+    A = _rand_sym(40, 1)
+    x0 = np.random.default_rng(2).standard_normal(40)
+    th, y, info = O.eigsolve_lanczos(lambda v: A @ v, x0, krylovdim=40, maxiter=1, tol=1e-14)
+    w, V = np.linalg.eigh(A)
+    assert abs(th - w[0]) < 1e-12
+    assert abs(abs(np.dot(y, V[:, 0])) - 1) < 1e-10
+
+
+def _ritz(A, Q):
+    Q, _ = np.linalg.qr(Q)
+    w, U = np.linalg.eigh(Q.T @ A @ Q)
+    return w, Q @ U
+
+
+def test_lanczos_restart_keep1_matches_explicit_krylov():
+    """krylovdim=3, maxiter=2: keep = (3*3+0)//5 = 1 Ritz vector, so the
+    result is the lowest Ritz pair of K_3(A, y1) with y1 the lowest Ritz
+    vector of K_3(A, x0)."""
+    A = _rand_sym(30, 3)
+    x0 = np.random.default_rng(4).standard_normal(30)
+    th, y, info = O.eigsolve_lanczos(lambda v: A @ v, x0, krylovdim=3, maxiter=2, tol=1e-14)
+    _, R1 = _ritz(A, np.stack([x0, A @ x0, A @ A @ x0], 1))
+    y1 = R1[:, 0]
+    w2, R2 = _ritz(A, np.stack([y1, A @ y1, A @ A @ y1], 1))
+    assert info.numops == 5
+    assert abs(th - w2[0]) < 1e-12
+    assert abs(abs(np.dot(y, R2[:, 0])) - 1) < 1e-9
+
+
+def test_lanczos_thick_restart_keep3_matches_explicit_space():
+    """krylovdim=5: keep = 3 Ritz vectors + the residual direction r, then two
+    expansions: the space is span{y1, y2, y3, r, A r}."""
+    A = _rand_sym(40, 5)
+    x0 = np.random.default_rng(6).standard_normal(40)
+    th, y, info = O.eigsolve_lanczos(lambda v: A @ v, x0, krylovdim=5, maxiter=2, tol=1e-14)
+    K = np.stack([np.linalg.matrix_power(A, j) @ x0 for j in range(5)], 1)
+    Q, _ = np.linalg.qr(K)
+    w, U = np.linalg.eigh(Q.T @ A @ Q)
+    Y = Q @ U[:, :3]
+    r = A @ Q[:, 4]
+    r = r - Q @ (Q.T @ r)
+    r = r - Q @ (Q.T @ r)
+    S = np.column_stack([Y, r, A @ r])
+    w2, _ = _ritz(A, S)
+    assert info.numops == 7
+    assert abs(th - w2[0]) < 1e-11
+
+
+def test_expm_matches_scipy():
+    A = _rand_sym(60, 7) / 3.0
+    x0 = np.random.default_rng(8).standard_normal(60)
+    for t, kd in [(-0.3j, 30), (0.2, 30), (-0.4j, 8)]:
+        y, info = O.exponentiate_lanczos(lambda v: A @ v, t, x0.astype(complex), krylovdim=kd, maxiter=200,
+                                         tol=1e-12)
+        ref = sla.expm(t * A) @ x0
+        assert np.abs(y - ref).max() < 1e-10 * np.linalg.norm(x0), (t, kd)
+        assert info.converged == 1
+    # too small a Krylov space for the step within maxiter: reported as not converged
+    _, info = O.exponentiate_lanczos(lambda v: A @ v, -0.4j, x0.astype(complex), krylovdim=6, maxiter=20,
+                                     tol=1e-12)
+    assert info.converged == 0
+
+
+# --------------------------------------------------------------------------
+# truncation rule (hand-computed cases)
+# --------------------------------------------------------------------------
+def test_truncation_rule_cases():
+    P = np.array([0.5, 0.3, 0.15, 0.05])
+    r = O.truncate_weights(P, maxdim=2, mindim=1, cutoff=0.0)
+    assert r.n == 2 and abs(r.truncerr - 0.2) < 1e-15
+    r = O.truncate_weights(P, maxdim=10, mindim=1, cutoff=0.06)
+    assert r.n == 3 and abs(r.truncerr - 0.05) < 1e-15
+    r = O.truncate_weights(P, maxdim=10, mindim=3, cutoff=0.5)
+    assert r.n == 3
+    r = O.truncate_weights(P, maxdim=10, mindim=1, cutoff=-1.0)
+    assert r.n == 4 and r.truncerr == 0.0
+    # relative cutoff scales with sum(P)
+    r = O.truncate_weights(10 * P, maxdim=10, mindim=1, cutoff=0.06)
+    assert r.n == 3 and abs(r.truncerr - 0.05) < 1e-15
+    # docut midpoint and near-degeneracy bump
+    r = O.truncate_weights(P, maxdim=2, mindim=1, cutoff=0.0)
+    assert abs(r.docut - 0.225) < 1e-15
+    Pd = np.array([0.5, 0.25, 0.25 - 1e-6, 0.0])
+    r = O.truncate_weights(Pd, maxdim=2, mindim=1, cutoff=0.0)
+    assert r.n == 2 and abs(r.docut - ((0.25 + 0.25 - 1e-6) / 2 + 1e-3 * 0.25)) < 1e-15
+    # single value and negative tail
+    r = O.truncate_weights(np.array([0.7]), 5, 1, 1e-3)
+    assert r.n == 1 and r.docut == 0.35
+    r = O.truncate_weights(np.array([0.6, 0.4, -1e-18]), 5, 1, 1e-20)
+    assert r.n == 2
+
+
+def test_svd_truncate_error_definition():
+    rng = np.random.default_rng(9)
+    M = rng.standard_normal((30, 20)) * np.exp(-0.5 * np.arange(20))[None, :]
+    U, S, Vh, spec = O.svd_truncate(M, maxdim=8, mindim=1, cutoff=1e-12)
+    R = M - (U * S) @ Vh
+    assert abs(np.linalg.norm(R) ** 2 / np.linalg.norm(M) ** 2 - spec.truncerr) < 1e-12
+    assert len(S) == 8
+
+
+# --------------------------------------------------------------------------
+# end-to-end DMRG / TDVP pins
+# --------------------------------------------------------------------------
+def test_dmrg_n12_reaches_ed():
+    env = O.StateEnvs(O.neel_mps(12), O.heisenberg_mpo(12))
+    p = O.DMRGParams.make([64], [4], cutoff=1e-14)
+    sw = O.dmrg(env, p, krylovdim=5, maxiter=2)
+    assert abs(sw.energy[-1] - (-5.14209063284054)) < 1e-10
+    # the reported energy is the Ritz value of the last update; the state's
+    # own energy agrees
+    assert abs(O.mps_energy(env.psi, O.heisenberg_mpo(12)) - sw.energy[-1]) < 1e-10
+
+
+def test_golden_c1_heisenberg_n20():
+    """Config-1 DMRG (N=20, random product start seed 1, maxdim
+    16,32,64,64,64, cutoff 1e-10, krylovdim 3) reproduces the committed
+    fixture and lies above the ED ground energy within its truncation error."""
+    g = json.loads((GOLDEN / "c1_heisenberg_n20.json").read_text())
+    env = O.StateEnvs(O.random_product_mps(20, 2, seed=1), O.heisenberg_mpo(20))
+    p = O.DMRGParams.make(g["maxdim"], [1] * 5, cutoff=g["cutoff"])
+    sw = O.dmrg(env, p, krylovdim=3, maxiter=2)
+    assert sw.maxchi == g["maxchi"]
+    assert np.allclose(sw.energy, g["energy"], rtol=0, atol=1e-10)
+    e0 = -8.68247333439896
+    assert e0 - 1e-12 <= sw.energy[-1] < e0 + 1e-6
+
+
+def test_golden_matvec_fixture():
+    z = np.load(GOLDEN / "matvec_n8.npz")
+    N = int(z["N"])
+    A = [z[f"A{k}"] for k in range(N)]
+    env = O.Envs(O.xxz_mpo(N, 1.0, float(z["Delta"]), z["h"]))
+    env.nsite = 2
+    p = int(z["pos"])
+    env.position(A, p)
+    assert np.abs(env.product(z["psi"]) - z["Hpsi"]).max() < 1e-12
+
+
+def _sz_expect(A):
+    """<Sz_i> of a (not necessarily normalised) MPS by transfer matrices."""
+    N = len(A)
+    out = []
+    for i in range(N):
+        L = np.ones((1, 1), dtype=complex)
+        nrm = np.ones((1, 1), dtype=complex)
+        for k in range(N):
+            op = SZ if k == i else np.eye(2)
+            L = np.einsum("ab,asc,st,btd->cd", L, A[k], op, A[k].conj())
+            nrm = np.einsum("ab,asc,bsd->cd", nrm, A[k], A[k].conj())
+        out.append((L[0, 0] / nrm[0, 0]).real)
+    return np.array(out)
+
+
+def test_tdvp_xx_chain_matches_free_fermions():
+    """XX chain (Delta = 0) from Neel is free fermions: <n_i(t)> =
+    sum_{j occupied} |exp(-i h t)_{ij}|^2 with h_ij = J/2 (hopping)."""
+    N, dt, steps = 8, 0.05, 10
+    Ws = O.xxz_mpo(N, 1.0, 0.0)
+    # Neel state zero-padded to the full bond dimensions min(2^k, 2^(N-k));
+    # with cutoff = -1 nothing is truncated, every tangent-space projector is
+    # the identity and two-site TDVP is exact up to the Krylov tolerance.
+    chis = [1] + [min(2 ** k, 2 ** (N - k)) for k in range(1, N)] + [1]
+    A = []
+    for k in range(N):
+        t = np.zeros((chis[k], 2, chis[k + 1]), dtype=complex)
+        t[0, k % 2, 0] = 1.0
+        A.append(t)
+    env = O.StateEnvs(A, Ws)
+    sw = O.SweepData()
+    for _ in range(steps):
+        O.tdvpsweep(env, sw, -1j * dt, maxdim=64, cutoff=-1.0, krylovdim=30, maxiter=100, tol=1e-12)
+    assert [a.shape[2] for a in env.psi[:-1]] == chis[1:-1]
+    hsp = 0.5 * (np.eye(N, k=1) + np.eye(N, k=-1))
+    Ut = sla.expm(-1j * hsp * dt * steps)
+    occ = [i for i in range(N) if i % 2 == 0]  # |up> = particle on even sites
+    n = (np.abs(Ut[:, occ]) ** 2).sum(1)
+    assert np.abs(_sz_expect(env.psi) - (n - 0.5)).max() < 1e-8
+    # <H> conserved (Neel energy of the XX chain is 0)
+    assert abs(sw.energy[-1]) < 1e-8
