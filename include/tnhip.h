@@ -113,6 +113,9 @@ int tn_memcpy_d2h(tn_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes
 int tn_ctx_timers(tn_ctx* ctx, double* ms4, int reset);
 /* turn per-phase event timing on (1) or off (0); off by default */
 int tn_ctx_enable_timing(tn_ctx* ctx, int on);
+/* accumulated device ms and event count of one phase: 0 matvec, 1 env update,
+ * 2 Krylov vector ops, 3 SVD/replacebond, 4 each GEMM launch of a matvec */
+int tn_ctx_phase_stats(tn_ctx* ctx, int phase, double* ms, long* count);
 void tn_update_params_default(tn_update_params* p, int solver);
 
 /* ---- StateEnvs{ProjMPO}: MPS + cached environments -------------------- */

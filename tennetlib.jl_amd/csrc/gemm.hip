@@ -164,14 +164,15 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(GArgs a) {
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
   }
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each block
+  // epilogue: register r of lane holds row lane/16 + 4r, column lane%16 of
+  // each 16x16 block (v_mfma_f64_16x16x4f64 output layout)
 #pragma unroll
   for (int i = 0; i < FM; ++i)
 #pragma unroll
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long m = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const long m = m0 + wm * WTM + i * 16 + (lane >> 4) + 4 * r;
         const long n = n0 + wn * WTN + j * 16 + (lane & 15);
         if (m < a.M && n < a.N) {
           double v = a.alpha * acc[i][j][r];
@@ -506,7 +507,7 @@ __global__ __launch_bounds__(256) void gemm_z64_kernel(GArgs a) {
     for (int j = 0; j < FN; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const long m = m0 + wm * WTM + i * 16 + 4 * (lane >> 4) + r;
+        const long m = m0 + wm * WTM + i * 16 + (lane >> 4) + 4 * r;
         const long n = n0 + wn * WTN + j * 16 + (lane & 15);
         if (m < a.M && n < a.N) {
           double vr = a.alpha * accr[i][j][r];

@@ -79,7 +79,7 @@ __global__ __launch_bounds__(256) void jac_gram_kernel(const double* __restrict_
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        const int row = wm * 32 + i * 16 + (lane >> 4) + 4 * r;  // f64 MFMA output layout
         const int col = wn * 32 + j * 16 + (lane & 15);
         out[row * PR + col] = acc[i][j][r];
       }
@@ -163,8 +163,46 @@ __global__ __launch_bounds__(256) void jac_inner_kernel(const double* __restrict
     ++total;
   }
   double* J = Jout + (long)pr * PR * PR;
-  for (int idx = tid; idx < PR * PR; idx += 256) J[idx] = Js[idx / PR][idx % PR];
+  if (total > 0) {
+    // J is a product of a few hundred plane rotations and is orthogonal only
+    // to ~1e-14; one Newton-Schulz step J <- J (3I - J^T J) / 2 squares that
+    // error, so repeated application keeps the vectors' norms exact.
+    for (int idx = tid; idx < PR * PR; idx += 256) {
+      const int i = idx / PR, j = idx % PR;
+      double s = 0.0;
+      for (int k = 0; k < PR; ++k) s = fma(Js[k][i], Js[k][j], s);
+      Gs[i][j] = (i == j ? 1.5 : 0.0) - 0.5 * s;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < PR * PR; idx += 256) {
+      const int r = idx / PR, cc = idx % PR;
+      double s = 0.0;
+      for (int k = 0; k < PR; ++k) s = fma(Js[r][k], Gs[k][cc], s);
+      J[idx] = s;
+    }
+  } else {
+    for (int idx = tid; idx < PR * PR; idx += 256) J[idx] = Js[idx / PR][idx % PR];
+  }
   if (tid == 0) rotcount[pr] = total;
+}
+
+// rows of Q (unit vectors up to rounding) rescaled to unit norm; grid (rows)
+__global__ __launch_bounds__(256) void normalize_rows_kernel(double* __restrict__ Q, long L, long ld) {
+  __shared__ double red[4];
+  __shared__ double inv;
+  const long r = blockIdx.x;
+  double s = 0.0;
+  for (long j = threadIdx.x; j < L; j += blockDim.x) s = fma(Q[r * ld + j], Q[r * ld + j], s);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double t = red[0] + red[1] + red[2] + red[3];
+    inv = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
+  }
+  __syncthreads();
+  for (long j = threadIdx.x; j < L; j += blockDim.x) Q[r * ld + j] *= inv;
 }
 
 // X_pair <- J^T X_pair (and Qt_pair) in place; grid (npairs, tilesX + tilesQ)
@@ -217,7 +255,7 @@ __global__ __launch_bounds__(256) void jac_apply_kernel(double* __restrict__ X, 
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int rr = wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        const int rr = wm * 32 + i * 16 + (lane >> 4) + 4 * r;  // f64 MFMA output layout
         const int cc = wn * 32 + j * 16 + (lane & 15);
         const long row = (rr < BR) ? ((long)bi * BR + rr) : ((long)bj * BR + rr - BR);
         const long col = c0 + cc;
@@ -315,6 +353,8 @@ void svd_jacobi(tn_ctx* c, const double* M, long m, long n, SvdOut& out) {
     if (tot == 0) break;
   }
   out.sweeps = sweep + 1;
+  hipLaunchKernelGGL(normalize_rows_kernel, dim3((unsigned)p), dim3(256), 0, c->stream, Q, p, ppad);
+  TN_HIP(hipGetLastError());
   double* s2 = (double*)c->workspace(WS_SVD_S, (size_t)p * sizeof(double));
   row_norms2(c, X, p, q, q, s2);
   out.s.resize(p);

@@ -68,7 +68,9 @@ struct DevBuf {
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
-enum Phase { PH_MATVEC = 0, PH_ENV = 1, PH_KRYLOV = 2, PH_SVD = 3, PH_COUNT = 4 };
+// timed phases; PH_MV_GEMM brackets each GEMM launch of a matvec on its own
+// (nested inside PH_MATVEC), the others do not nest
+enum Phase { PH_MATVEC = 0, PH_ENV = 1, PH_KRYLOV = 2, PH_SVD = 3, PH_MV_GEMM = 4, PH_COUNT = 5 };
 
 }  // namespace tn
 
@@ -83,7 +85,8 @@ struct tn_ctx {
   size_t pinned_n = 0;
   // per-phase timing
   bool timing = false;
-  double phase_ms[tn::PH_COUNT] = {0, 0, 0, 0};
+  double phase_ms[tn::PH_COUNT] = {0, 0, 0, 0, 0};
+  long phase_n[tn::PH_COUNT] = {0, 0, 0, 0, 0};
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   std::vector<int> ev_phase;
   size_t ev_used = 0;

@@ -77,6 +77,7 @@ void flush_timers(tn_ctx* c) {
     float ms = 0.f;
     if (hipEventElapsedTime(&ms, c->ev_pool[i].first, c->ev_pool[i].second) == hipSuccess)
       c->phase_ms[c->ev_phase[i]] += ms;
+    c->phase_n[c->ev_phase[i]] += 1;
   }
   c->ev_used = 0;
 }
@@ -420,7 +421,10 @@ void matvec(tn_env* e, const void* v, void* out, int prec) {
   g1.A = Lp; g1.lda = chil;
   g1.B = vin; g1.ldb = DD * chir;
   g1.C = T1; g1.ldc = DD * chir;
-  gemm(c, P, g1);
+  {
+    PhaseTimer pg(c, PH_MV_GEMM);
+    gemm(c, P, g1);
+  }
   WStepDesc ws;
   ws.G = w0; ws.Gp = w2; ws.D = DD; ws.Dp = DD;
   ws.nA = chil; ws.nB = chir * m2;
@@ -433,7 +437,10 @@ void matvec(tn_env* e, const void* v, void* out, int prec) {
   g3.A = T2; g3.lda = (long)w2 * chir;
   g3.B = Rp; g3.ldb = chir;
   g3.C = outp; g3.ldc = chir;
-  gemm(c, P, g3);
+  {
+    PhaseTimer pg(c, PH_MV_GEMM);
+    gemm(c, P, g3);
+  }
   if (f32) cvt_f32_to_f64(c, (const float*)outp, (double*)out, D.n);
 }
 
@@ -1110,9 +1117,9 @@ int tn_ctx_timers(tn_ctx* c, double* ms4, int reset) {
     TN_REQUIRE(c, TN_E_INVALID, "null context");
     flush_timers(c);
     if (ms4)
-      for (int i = 0; i < PH_COUNT; ++i) ms4[i] = c->phase_ms[i];
+      for (int i = 0; i < 4; ++i) ms4[i] = c->phase_ms[i];
     if (reset)
-      for (int i = 0; i < PH_COUNT; ++i) c->phase_ms[i] = 0.0;
+      for (int i = 0; i < PH_COUNT; ++i) { c->phase_ms[i] = 0.0; c->phase_n[i] = 0; }
   });
 }
 int tn_ctx_enable_timing(tn_ctx* c, int on) {
@@ -1120,6 +1127,16 @@ int tn_ctx_enable_timing(tn_ctx* c, int on) {
     TN_REQUIRE(c, TN_E_INVALID, "null context");
     flush_timers(c);
     c->timing = on != 0;
+  });
+}
+
+int tn_ctx_phase_stats(tn_ctx* c, int phase, double* ms, long* count) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    TN_REQUIRE(phase >= 0 && phase < PH_COUNT, TN_E_INVALID, "phase_stats: unknown phase");
+    flush_timers(c);
+    if (ms) *ms = c->phase_ms[phase];
+    if (count) *count = c->phase_n[phase];
   });
 }
 
@@ -1405,7 +1422,7 @@ int tn_fullsweep(tn_env* e, const tn_update_params* p, double* energy, double* m
     const int N = e->N;
     double en = NAN, mte = 0.0;
     std::vector<double> mid;
-    tn_update_result r;
+    tn_update_result r{};
     for (int b = 1; b <= N - 1; ++b) {
       update_two_site(e, b, true, p, &r);
       en = r.energy;

@@ -19,9 +19,11 @@ numpy only carries data across the boundary.
 
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import math
 import time
+import weakref
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence, Union
 
@@ -44,6 +46,7 @@ class Context:
         h = C.c_void_p()
         L.check(self.lib.tn_ctx_create(device, C.byref(h)))
         self.h = h
+        _live_contexts.add(self)
 
     def close(self):
         if self.h:
@@ -70,6 +73,25 @@ class Context:
 
 _default_ctx: Optional[Context] = None
 
+# Live handles, released in dependency order at interpreter exit: device
+# buffers and environments before the context (stream) they were made on.
+_live_children: "weakref.WeakSet" = weakref.WeakSet()
+_live_contexts: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    for obj in list(_live_children):
+        try:
+            obj.free() if isinstance(obj, DeviceArray) else obj.close()
+        except Exception:
+            pass
+    for ctx in list(_live_contexts):
+        try:
+            ctx.close()
+        except Exception:
+            pass
+
 
 def default_context() -> Context:
     global _default_ctx
@@ -89,6 +111,7 @@ class DeviceArray:
         p = C.c_void_p()
         ctx.check(ctx.lib.tn_alloc(ctx.h, max(self.nbytes, 16), C.byref(p)))
         self.ptr = p
+        _live_children.add(self)
 
     @classmethod
     def from_numpy(cls, ctx: Context, a: np.ndarray) -> "DeviceArray":
@@ -104,9 +127,9 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             self.ctx.lib.tn_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         try:
@@ -148,6 +171,7 @@ class StateEnvs:
         wi = (C.c_int * (N + 1))(*w)
         self.ctx.check(self.ctx.lib.tn_env_create(self.ctx.h, N, di, wi, dt, C.byref(h)))
         self.h = h
+        _live_children.add(self)
         self.N = N
         self.d = d
         self.w = w
@@ -158,11 +182,14 @@ class StateEnvs:
             self.set_site(k, A)
         self.H = [np.array(W) for W in H]
 
+    def close(self):
+        if self.h and self.ctx.h:
+            self.ctx.lib.tn_env_destroy(self.h)
+        self.h = None
+
     def __del__(self):
         try:
-            if self.h:
-                self.ctx.lib.tn_env_destroy(self.h)
-                self.h = None
+            self.close()
         except Exception:
             pass
 
