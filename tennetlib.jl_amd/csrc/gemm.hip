@@ -544,6 +544,10 @@ void gemm(tn_ctx* c, Prec p, const GemmDesc& g) {
   if (g.M == 0 || g.N == 0) return;
   TN_REQUIRE(g.A && g.B && g.C, TN_E_INVALID, "gemm: null operand");
   TN_REQUIRE(g.batch <= 65535, TN_E_INVALID, "gemm: batch too large");
+  if (gemm_fast_ok(p, g)) {
+    gemm_fast(c, p, g);
+    return;
+  }
   GArgs a = to_args(g);
   const bool small = (g.M * g.N <= 256L * 256L);
   switch (p) {

@@ -1,0 +1,518 @@
+// Double-buffered MFMA GEMMs for the large contractions of the hot path.
+//
+//  * gemm_fast<T, TA, TB>: FP64 (v_mfma_f64_16x16x4_f64) or FP32
+//    (v_mfma_f32_32x32x2_f32), 128 x 128 tiles, 4 wavefronts of 64 x 64.
+//    Each operand is staged with 16-byte loads along its contiguous
+//    dimension into an LDS tile that keeps the global orientation
+//    (A: [m][k] if k is contiguous, [k][m] if m is; B: [k][n] or [n][k]),
+//    so staging never transposes; the MFMA fragment reads adapt instead.
+//    Two LDS stages: the next tile is fetched into registers while the
+//    current one feeds the MFMAs, then written to the other stage; one
+//    barrier per K tile.
+//  * gemm_bf16x3_nt: C = A * B^T with A (M x K) and B (N x K) given as bf16
+//    hi/lo pairs (k contiguous, pre-split by split_bf16_*).  Each 32x32x16
+//    step issues lo*hi + hi*lo + hi*hi with FP32 accumulation.
+// Row strides must be multiples of the 16-byte vector width and bases
+// 16-byte aligned; the dispatcher in gemm.hip checks this and otherwise
+// uses the generic kernels.
+#include "tn_internal.h"
+
+#include <algorithm>
+
+namespace tn {
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
+
+struct FArgs {
+  long M, N, K;
+  const void* A; long lda; long sA;
+  const void* B; long ldb; long sB;
+  void* C; long ldc; long sC;
+  double alpha, beta;
+};
+
+__device__ __forceinline__ void tile_coords2(int ntm, int ntn, int& tm, int& tn) {
+  const int nt = ntm * ntn;
+  int id = blockIdx.x;
+  if (nt >= 64 && (nt % 8) == 0) {  // contiguous tile runs per XCD
+    const int per = nt / 8;
+    id = (id % 8) * per + (id / 8);
+  }
+  const int GM = 8;
+  const int group = id / (GM * ntn);
+  const int first_m = group * GM;
+  const int gsz = min(ntm - first_m, GM);
+  const int in_group = id % (GM * ntn);
+  tm = first_m + (in_group % gsz);
+  tn = in_group / gsz;
+}
+
+template <typename T> struct Vec;
+template <> struct Vec<double> { typedef double2 type; static constexpr int n = 2; };
+template <> struct Vec<float> { typedef float4 type; static constexpr int n = 4; };
+
+template <typename T>
+__device__ __forceinline__ typename Vec<T>::type vzero() {
+  typename Vec<T>::type v;
+  T* p = reinterpret_cast<T*>(&v);
+#pragma unroll
+  for (int i = 0; i < Vec<T>::n; ++i) p[i] = T(0);
+  return v;
+}
+
+// load VN consecutive elements starting at (r, c) of a row-major matrix with
+// `rows` x `cols` valid entries; out-of-range elements are zero
+template <typename T>
+__device__ __forceinline__ typename Vec<T>::type vload(const T* __restrict__ base, long ld, long r, long c,
+                                                       long rows, long cols) {
+  constexpr int VN = Vec<T>::n;
+  typedef typename Vec<T>::type V;
+  if (r >= rows || c >= cols) return vzero<T>();
+  const T* p = base + r * ld + c;
+  if (c + VN <= cols) return *reinterpret_cast<const V*>(p);
+  V v = vzero<T>();
+  T* q = reinterpret_cast<T*>(&v);
+  for (int i = 0; i < VN; ++i)
+    if (c + i < cols) q[i] = p[i];
+  return v;
+}
+
+template <typename T, int TA, int TB>
+struct FastCfg {
+  static constexpr int BM = 128, BN = 128;
+  static constexpr int BK = (sizeof(T) == 8) ? 16 : 32;
+  static constexpr int VN = Vec<T>::n;
+  static constexpr int PAD = VN;  // keeps 16-byte alignment of every LDS row
+  // A stage: [BM][BK+PAD] if TA == 0 else [BK][BM+PAD]
+  static constexpr int A_ROWS = TA ? BK : BM, A_COLS = TA ? BM : BK, A_LD = A_COLS + PAD;
+  static constexpr int B_ROWS = TB ? BN : BK, B_COLS = TB ? BK : BN, B_LD = B_COLS + PAD;
+  static constexpr int A_VECS = A_ROWS * A_COLS / VN / 256;  // per thread
+  static constexpr int B_VECS = B_ROWS * B_COLS / VN / 256;
+  static constexpr int A_ELEMS = A_ROWS * A_LD, B_ELEMS = B_ROWS * B_LD;
+};
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
+  using CF = FastCfg<T, TA, TB>;
+  constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, VN = CF::VN;
+  typedef typename Vec<T>::type V;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  T* As = reinterpret_cast<T*>(smem);                 // 2 stages
+  T* Bs = As + 2 * CF::A_ELEMS;
+
+  const int ntm = (int)((a.M + BM - 1) / BM), ntn = (int)((a.N + BN - 1) / BN);
+  int tm, tn;
+  tile_coords2(ntm, ntn, tm, tn);
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const long bz = blockIdx.z;
+  const T* A = reinterpret_cast<const T*>(a.A) + bz * a.sA;
+  const T* B = reinterpret_cast<const T*>(a.B) + bz * a.sB;
+  T* C = reinterpret_cast<T*>(a.C) + bz * a.sC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  V ra[CF::A_VECS], rb[CF::B_VECS];
+  auto gload = [&](long k0) {
+#pragma unroll
+    for (int i = 0; i < CF::A_VECS; ++i) {
+      const int idx = tid + i * 256;
+      const int rr = idx / (CF::A_COLS / VN), cc = (idx % (CF::A_COLS / VN)) * VN;
+      if (!TA) ra[i] = vload<T>(A, a.lda, m0 + rr, k0 + cc, a.M, a.K);   // A[m][k]
+      else ra[i] = vload<T>(A, a.lda, k0 + rr, m0 + cc, a.K, a.M);       // A^T stored [k][m]
+    }
+#pragma unroll
+    for (int i = 0; i < CF::B_VECS; ++i) {
+      const int idx = tid + i * 256;
+      const int rr = idx / (CF::B_COLS / VN), cc = (idx % (CF::B_COLS / VN)) * VN;
+      if (!TB) rb[i] = vload<T>(B, a.ldb, k0 + rr, n0 + cc, a.K, a.N);   // B[k][n]
+      else rb[i] = vload<T>(B, a.ldb, n0 + rr, k0 + cc, a.N, a.K);       // B^T stored [n][k]
+    }
+  };
+  auto sstore = [&](int st) {
+    T* as = As + st * CF::A_ELEMS;
+    T* bs = Bs + st * CF::B_ELEMS;
+#pragma unroll
+    for (int i = 0; i < CF::A_VECS; ++i) {
+      const int idx = tid + i * 256;
+      const int rr = idx / (CF::A_COLS / VN), cc = (idx % (CF::A_COLS / VN)) * VN;
+      *reinterpret_cast<V*>(&as[rr * CF::A_LD + cc]) = ra[i];
+    }
+#pragma unroll
+    for (int i = 0; i < CF::B_VECS; ++i) {
+      const int idx = tid + i * 256;
+      const int rr = idx / (CF::B_COLS / VN), cc = (idx % (CF::B_COLS / VN)) * VN;
+      *reinterpret_cast<V*>(&bs[rr * CF::B_LD + cc]) = rb[i];
+    }
+  };
+  // element (m, k) of the A stage and (k, n) of the B stage
+  auto Ael = [&](const T* as, int m, int k) -> T { return TA ? as[k * CF::A_LD + m] : as[m * CF::A_LD + k]; };
+  auto Bel = [&](const T* bs, int k, int n) -> T { return TB ? bs[n * CF::B_LD + k] : bs[k * CF::B_LD + n]; };
+
+  const long nk = (a.K + BK - 1) / BK;
+  if constexpr (sizeof(T) == 8) {
+    constexpr int FM = 4, FN = 4;  // 64x64 wave tile of 16x16 blocks
+    d4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (long t = 0; t < nk; ++t) {
+      if (t + 1 < nk) gload((t + 1) * BK);
+      const T* as = As + (t & 1) * CF::A_ELEMS;
+      const T* bs = Bs + (t & 1) * CF::B_ELEMS;
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        double fa[FM], fb[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) fa[i] = Ael(as, wm * 64 + i * 16 + (lane & 15), kr);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[j] = Bel(bs, kr, wn * 64 + j * 16 + (lane & 15));
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+      }
+      if (t + 1 < nk) sstore((t + 1) & 1);
+      __syncthreads();
+    }
+    // v_mfma_f64_16x16x4f64 output: register r of lane -> row lane/16 + 4r, column lane%16
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long m = m0 + wm * 64 + i * 16 + (lane >> 4) + 4 * r;
+          const long n = n0 + wn * 64 + j * 16 + (lane & 15);
+          if (m < a.M && n < a.N) {
+            double v = a.alpha * acc[i][j][r];
+            if (a.beta != 0.0) v += a.beta * C[m * a.ldc + n];
+            C[m * a.ldc + n] = (T)v;
+          }
+        }
+  } else {
+    constexpr int FM = 2, FN = 2;  // 64x64 wave tile of 32x32 blocks
+    f16v acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j) acc[i][j] = f16v{};
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (long t = 0; t < nk; ++t) {
+      if (t + 1 < nk) gload((t + 1) * BK);
+      const T* as = As + (t & 1) * CF::A_ELEMS;
+      const T* bs = Bs + (t & 1) * CF::B_ELEMS;
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 2) {
+        const int kr = kk + (lane >> 5);
+        float fa[FM], fb[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) fa[i] = Ael(as, wm * 64 + i * 32 + (lane & 31), kr);
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[j] = Bel(bs, kr, wn * 64 + j * 32 + (lane & 31));
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
+      }
+      if (t + 1 < nk) sstore((t + 1) & 1);
+      __syncthreads();
+    }
+    // register r of lane -> row 8*(r/4) + 4*(lane/32) + r%4, column lane%32
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+      for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const long m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const long n = n0 + wn * 64 + j * 32 + (lane & 31);
+          if (m < a.M && n < a.N) {
+            float v = (float)a.alpha * acc[i][j][r];
+            if (a.beta != 0.0) v += (float)a.beta * C[m * a.ldc + n];
+            C[m * a.ldc + n] = v;
+          }
+        }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3, C (float) = A * B^T, A = (Ah + Al) M x K, B = (Bh + Bl) N x K
+// ---------------------------------------------------------------------------
+struct SArgs {
+  long M, N, K;
+  const unsigned short* Ah; const unsigned short* Al; long lda;
+  const unsigned short* Bh; const unsigned short* Bl; long ldb;
+  float* C; long ldc;
+};
+
+constexpr int SBM = 128, SBN = 128, SBK = 64, SLD = SBK + 8;  // bf16 elements per LDS row
+constexpr int S_STAGE = (2 * SBM + 2 * SBN) * SLD;            // hi+lo of A and B, bf16 elements
+
+__device__ __forceinline__ uint4 bload(const unsigned short* __restrict__ base, long ld, long r, long c, long rows,
+                                       long cols) {
+  if (r >= rows || c >= cols) return make_uint4(0, 0, 0, 0);
+  uint4 v = *reinterpret_cast<const uint4*>(base + r * ld + c);
+  if (c + 8 > cols) {  // zero the k tail of the vector (rows are padded, the load is in bounds)
+    unsigned short* e = reinterpret_cast<unsigned short*>(&v);
+    for (int i = 0; i < 8; ++i)
+      if (c + i >= cols) e[i] = 0;
+  }
+  return v;
+}
+
+__global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(SArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  unsigned short* S = reinterpret_cast<unsigned short*>(smem);
+  const int ntm = (int)((a.M + SBM - 1) / SBM), ntn = (int)((a.N + SBN - 1) / SBN);
+  int tm, tn;
+  tile_coords2(ntm, ntn, tm, tn);
+  const long m0 = (long)tm * SBM, n0 = (long)tn * SBN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  constexpr int VPR = SBK / 8;                  // 16-byte vectors per row
+  constexpr int NV = SBM * VPR / 256;           // per thread, per operand part
+  uint4 rah[NV], ral[NV], rbh[NV], rbl[NV];
+  auto gload = [&](long k0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = tid + i * 256;
+      const int rr = idx / VPR, cc = (idx % VPR) * 8;
+      rah[i] = bload(a.Ah, a.lda, m0 + rr, k0 + cc, a.M, a.K);
+      ral[i] = bload(a.Al, a.lda, m0 + rr, k0 + cc, a.M, a.K);
+      rbh[i] = bload(a.Bh, a.ldb, n0 + rr, k0 + cc, a.N, a.K);
+      rbl[i] = bload(a.Bl, a.ldb, n0 + rr, k0 + cc, a.N, a.K);
+    }
+  };
+  auto sstore = [&](int st) {
+    unsigned short* base = S + st * S_STAGE;
+    unsigned short* Ah = base;
+    unsigned short* Al = Ah + SBM * SLD;
+    unsigned short* Bh = Al + SBM * SLD;
+    unsigned short* Bl = Bh + SBN * SLD;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int idx = tid + i * 256;
+      const int rr = idx / VPR, cc = (idx % VPR) * 8;
+      *reinterpret_cast<uint4*>(&Ah[rr * SLD + cc]) = rah[i];
+      *reinterpret_cast<uint4*>(&Al[rr * SLD + cc]) = ral[i];
+      *reinterpret_cast<uint4*>(&Bh[rr * SLD + cc]) = rbh[i];
+      *reinterpret_cast<uint4*>(&Bl[rr * SLD + cc]) = rbl[i];
+    }
+  };
+  f16v acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f16v{};
+  const long nk = (a.K + SBK - 1) / SBK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (long t = 0; t < nk; ++t) {
+    if (t + 1 < nk) gload((t + 1) * SBK);
+    const unsigned short* Ah = S + (t & 1) * S_STAGE;
+    const unsigned short* Al = Ah + SBM * SLD;
+    const unsigned short* Bh = Al + SBM * SLD;
+    const unsigned short* Bl = Bh + SBN * SLD;
+#pragma unroll
+    for (int kk = 0; kk < SBK; kk += 16) {
+      const int kc = kk + 8 * (lane >> 5);
+      bf8 ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int row = wm * 64 + i * 32 + (lane & 31);
+        ah[i] = __builtin_bit_cast(bf8, *reinterpret_cast<const us8*>(&Ah[row * SLD + kc]));
+        al[i] = __builtin_bit_cast(bf8, *reinterpret_cast<const us8*>(&Al[row * SLD + kc]));
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = wn * 64 + j * 32 + (lane & 31);
+        bh[j] = __builtin_bit_cast(bf8, *reinterpret_cast<const us8*>(&Bh[col * SLD + kc]));
+        bl[j] = __builtin_bit_cast(bf8, *reinterpret_cast<const us8*>(&Bl[col * SLD + kc]));
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    if (t + 1 < nk) sstore((t + 1) & 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const long m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long n = n0 + wn * 64 + j * 32 + (lane & 31);
+        if (m < a.M && n < a.N) a.C[m * a.ldc + n] = acc[i][j][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// bf16 hi/lo split: hi = rne(x), lo = rne(x - hi); from FP64 or FP32 sources
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned short f2bf(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f_(unsigned short h) { return __uint_as_float(((unsigned int)h) << 16); }
+
+template <typename T>
+__device__ __forceinline__ void split1(T x, unsigned short& h, unsigned short& l) {
+  h = f2bf((float)x);
+  l = f2bf((float)(x - (T)bf2f_(h)));
+}
+
+// dst[r][c] (ld_dst) = split(src[r][c]); columns cols..ld_dst-1 zeroed
+template <typename T>
+__global__ __launch_bounds__(256) void split_rows_kernel(const T* __restrict__ src, long rows, long cols,
+                                                         long lds, unsigned short* __restrict__ hi,
+                                                         unsigned short* __restrict__ lo, long ldd) {
+  const long total = rows * ldd;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / ldd, c = i % ldd;
+    unsigned short h = 0, l = 0;
+    if (c < cols) split1<T>(src[r * lds + c], h, l);
+    hi[i] = h;
+    lo[i] = l;
+  }
+}
+
+// dst[c][r] (ld_dst) = split(src[r][c]) for r < rows, c < cols; 32x32 tiles
+template <typename T>
+__global__ __launch_bounds__(256) void split_transpose_kernel(const T* __restrict__ src, long rows, long cols,
+                                                              long lds, unsigned short* __restrict__ hi,
+                                                              unsigned short* __restrict__ lo, long ldd) {
+  __shared__ T t[32][33];
+  const long r0 = (long)blockIdx.y * 32, c0 = (long)blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const long r = r0 + k, c = c0 + tx;
+    t[k][tx] = (r < rows && c < cols) ? src[r * lds + c] : T(0);
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const long c = c0 + k, r = r0 + tx;  // output row c, column r
+    if (c < cols && r < ldd) {
+      unsigned short h = 0, l = 0;
+      if (r < rows) split1<T>(t[tx][k], h, l);
+      hi[c * ldd + r] = h;
+      lo[c * ldd + r] = l;
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Raise the dynamic-LDS limit of a kernel above the 64 KiB default (once).
+static void allow_lds(const void* fn, size_t bytes) {
+  if (bytes <= 64 * 1024) return;
+  TN_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+bool gemm_fast_ok(Prec p, const GemmDesc& g) {
+  if (p != Prec::F64 && p != Prec::F32) return false;
+  if (g.cA || g.cB) return false;
+  if (g.M < 128 || g.N < 128 || g.K < 16) return false;
+  const int vn = (p == Prec::F64) ? 2 : 4;
+  const size_t es = (p == Prec::F64) ? 8 : 4;
+  auto al = [&](const void* ptr, long ld, long s) {
+    return ((uintptr_t)ptr % 16 == 0) && (ld % vn == 0) && (s % vn == 0);
+  };
+  (void)es;
+  return al(g.A, g.lda, g.sA) && al(g.B, g.ldb, g.sB);
+}
+
+void gemm_fast(tn_ctx* c, Prec p, const GemmDesc& g) {
+  FArgs a;
+  a.M = g.M; a.N = g.N; a.K = g.K;
+  a.A = g.A; a.lda = g.lda; a.sA = g.sA;
+  a.B = g.B; a.ldb = g.ldb; a.sB = g.sB;
+  a.C = g.C; a.ldc = g.ldc; a.sC = g.sC;
+  a.alpha = g.alpha; a.beta = g.beta;
+  const long ntm = (g.M + 127) / 128, ntn = (g.N + 127) / 128;
+  dim3 grid((unsigned)(ntm * ntn), 1, (unsigned)g.batch);
+#define TN_LAUNCH_FAST(T, TA, TB)                                                                    \
+  do {                                                                                               \
+    using CF = FastCfg<T, TA, TB>;                                                                   \
+    const size_t shm = 2 * (size_t)(CF::A_ELEMS + CF::B_ELEMS) * sizeof(T);                          \
+    allow_lds((const void*)gemm_fast_kernel<T, TA, TB>, shm);                                        \
+    hipLaunchKernelGGL((gemm_fast_kernel<T, TA, TB>), grid, dim3(256), shm, c->stream, a);           \
+  } while (0)
+  if (p == Prec::F64) {
+    if (!g.tA && !g.tB) TN_LAUNCH_FAST(double, 0, 0);
+    else if (g.tA && !g.tB) TN_LAUNCH_FAST(double, 1, 0);
+    else if (!g.tA && g.tB) TN_LAUNCH_FAST(double, 0, 1);
+    else TN_LAUNCH_FAST(double, 1, 1);
+  } else {
+    if (!g.tA && !g.tB) TN_LAUNCH_FAST(float, 0, 0);
+    else if (g.tA && !g.tB) TN_LAUNCH_FAST(float, 1, 0);
+    else if (!g.tA && g.tB) TN_LAUNCH_FAST(float, 0, 1);
+    else TN_LAUNCH_FAST(float, 1, 1);
+  }
+#undef TN_LAUNCH_FAST
+  TN_HIP(hipGetLastError());
+}
+
+void gemm_bf16x3_nt(tn_ctx* c, long M, long N, long K, const unsigned short* Ah, const unsigned short* Al,
+                    long lda, const unsigned short* Bh, const unsigned short* Bl, long ldb, float* C, long ldc) {
+  if (M == 0 || N == 0) return;
+  TN_REQUIRE(lda % 8 == 0 && ldb % 8 == 0 && lda >= ((K + 7) / 8) * 8 && ldb >= ((K + 7) / 8) * 8,
+             TN_E_INVALID, "gemm_bf16x3_nt: rows must be padded to 8 bf16");
+  SArgs a{M, N, K, Ah, Al, lda, Bh, Bl, ldb, C, ldc};
+  const long ntm = (M + SBM - 1) / SBM, ntn = (N + SBN - 1) / SBN;
+  const size_t shm = 2 * (size_t)S_STAGE * sizeof(unsigned short);
+  allow_lds((const void*)gemm_bf16x3_nt_kernel, shm);
+  hipLaunchKernelGGL(gemm_bf16x3_nt_kernel, dim3((unsigned)(ntm * ntn)), dim3(256), shm, c->stream, a);
+  TN_HIP(hipGetLastError());
+}
+
+void split_bf16(tn_ctx* c, const void* src, bool src_f64, long rows, long cols, long lds, bool transpose,
+                unsigned short* hi, unsigned short* lo, long ldd) {
+  if (rows == 0 || cols == 0) return;
+  if (!transpose) {
+    const long total = rows * ldd;
+    const int g = (int)std::min<long>(8192, (total + 255) / 256);
+    if (src_f64)
+      hipLaunchKernelGGL(split_rows_kernel<double>, dim3(g), dim3(256), 0, c->stream, (const double*)src, rows,
+                         cols, lds, hi, lo, ldd);
+    else
+      hipLaunchKernelGGL(split_rows_kernel<float>, dim3(g), dim3(256), 0, c->stream, (const float*)src, rows,
+                         cols, lds, hi, lo, ldd);
+  } else {
+    // output is cols x ldd with ldd >= rows
+    const long orows = std::max(rows, ldd);
+    dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((orows + 31) / 32));
+    TN_REQUIRE((orows + 31) / 32 <= 65535, TN_E_INVALID, "split_bf16: too many rows");
+    if (src_f64)
+      hipLaunchKernelGGL(split_transpose_kernel<double>, grid, dim3(256), 0, c->stream, (const double*)src, rows,
+                         cols, lds, hi, lo, ldd);
+    else
+      hipLaunchKernelGGL(split_transpose_kernel<float>, grid, dim3(256), 0, c->stream, (const float*)src, rows,
+                         cols, lds, hi, lo, ldd);
+  }
+  TN_HIP(hipGetLastError());
+}
+
+}  // namespace tn

@@ -1,25 +1,29 @@
-// Truncated SVD for replacebond!: one-sided (Hestenes) block Jacobi in FP64.
+// Truncated factorisation for replacebond!: symmetric eigen-decomposition of
+// the two-site density matrix on the device, in FP64.
 //
-// The p = min(m, n) vectors of length q = max(m, n) (columns of M when
-// n <= m, rows of M otherwise) are stored as rows of X (p x q).  They are
-// split into blocks of 32 rows; a sweep visits every pair of blocks once in a
-// round-robin tournament (nb - 1 rounds of nb/2 independent pairs).  For each
-// pair of 64 rows:
-//   1. the Gram matrix G = X_pair X_pair^T is formed afresh (FP64 MFMA,
-//      split over q and summed in a fixed order),
-//   2. G is diagonalised by parallel-order cyclic Jacobi in LDS with the
-//      relative threshold |g_ij| > tol * sqrt(g_ii g_jj) (Demmel & Veselic:
-//      on a nearly-diagonal scaled Gram this keeps high relative accuracy),
-//      accumulating the rotation J (64 x 64),
-//   3. X_pair <- J^T X_pair and Qt_pair <- J^T Qt_pair in place (FP64 MFMA).
-// Sweeps stop when a whole sweep performs no rotation.  The singular values
-// are the row norms of X; the rows of Qt are the other singular vectors.
-// Small singular values keep high relative accuracy, which the truncation
-// rule (a cumulative sum of discarded weights against cutoff) relies on.
+// For phi (m x n) the Gram matrix G = phi phi^T (m x m, "left") or
+// G = phi^T phi (n x n, "right") is formed with one FP64 MFMA GEMM and
+// diagonalised by two-sided block Jacobi:
+//   * indices are grouped in blocks of 32; a sweep visits every pair of
+//     blocks once in a round-robin tournament (nb - 1 rounds of nb/2
+//     disjoint pairs);
+//   * per pair, the 64 x 64 principal sub-matrix is diagonalised in LDS by
+//     parallel-order cyclic Jacobi (rotation of (i, j) when |g_ij| exceeds
+//     max(tol_rel * sqrt(g_ii g_jj), tol_abs)), accumulating J;
+//   * G <- J^T G J (rows, then columns of the pair) and V^T <- J^T V^T with
+//     FP64 MFMA tiles.
+// Sweeps stop when a sweep performs no rotation.  The eigenvalues are the
+// weights P = s^2 with absolute accuracy ~eps * sum(P) (the accuracy of a
+// density-matrix eigensolver), and the eigenvectors are orthonormal to
+// rounding, so the orthonormal MPS factor is taken from them directly and
+// the other factor is the exact product V^T phi (or phi V): no division by
+// small singular values anywhere.
 #include "tn_internal.h"
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 
 namespace tn {
@@ -27,96 +31,48 @@ namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-constexpr int BR = 32;      // rows per block
-constexpr int PR = 2 * BR;  // rows per pair
+constexpr int BR = 32;      // indices per block
+constexpr int PR = 2 * BR;  // indices per pair
 constexpr int LDP = PR + 1;
-constexpr int GBK = 32;     // k-chunk of the Gram kernel
 
-// partial Gram of one pair over one k-range; grid (npairs, ksplit)
-__global__ __launch_bounds__(256) void jac_gram_kernel(const double* __restrict__ X, long q, long ldx,
-                                                       const int* __restrict__ pairs, long kc,
-                                                       double* __restrict__ part) {
-  __shared__ double Xs[GBK][PR + 2];
-  const int pr = blockIdx.x, ks = blockIdx.y, nks = gridDim.y;
-  const int bi = pairs[2 * pr], bj = pairs[2 * pr + 1];
-  const long kbeg = (long)ks * kc, kend = min(q, kbeg + kc);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
-  d4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = d4{0, 0, 0, 0};
-  for (long k0 = kbeg; k0 < kend; k0 += GBK) {
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < (PR * GBK) / 256; ++i) {
-      const int idx = tid + i * 256;
-      const int kk = idx % GBK, rr = idx / GBK;
-      const long row = (rr < BR) ? ((long)bi * BR + rr) : ((long)bj * BR + rr - BR);
-      const long k = k0 + kk;
-      Xs[kk][rr] = (k < kend) ? X[row * ldx + k] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < GBK; kk += 4) {
-      const int kr = kk + (lane >> 4);
-      double fa[2], fb[2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) fa[i] = Xs[kr][wm * 32 + i * 16 + (lane & 15)];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) fb[j] = Xs[kr][wn * 32 + j * 16 + (lane & 15)];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
-    }
-  }
-  double* out = part + ((long)pr * nks + ks) * PR * PR;
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = wm * 32 + i * 16 + (lane >> 4) + 4 * r;  // f64 MFMA output layout
-        const int col = wn * 32 + j * 16 + (lane & 15);
-        out[row * PR + col] = acc[i][j][r];
-      }
+__device__ __forceinline__ long pair_index(int bi, int bj, int r) {
+  return (r < BR) ? ((long)bi * BR + r) : ((long)bj * BR + r - BR);
 }
 
-// Jacobi diagonalisation of one 64 x 64 pair Gram; grid (npairs)
-__global__ __launch_bounds__(256) void jac_inner_kernel(const double* __restrict__ part, int nks, double tol,
-                                                        int max_inner, double* __restrict__ Jout,
+// Jacobi diagonalisation of the 64 x 64 principal sub-matrix of G for each
+// pair of a round; grid (npairs), 256 threads.  Thread t owns pair t/8 of a
+// step and 8 columns (rows) of it, so every update loop has 8 independent
+// LDS read-modify-writes in flight.
+__global__ __launch_bounds__(256) void jac_inner_kernel(const double* __restrict__ G, long ldg,
+                                                        const int* __restrict__ pairs, double tol_rel,
+                                                        double tol_abs, int max_inner, double* __restrict__ Jout,
                                                         int* __restrict__ rotcount) {
   __shared__ double Gs[PR][LDP];
   __shared__ double Js[PR][LDP];
   __shared__ double cs[PR / 2], sn[PR / 2];
   __shared__ int pi_[PR / 2], pj_[PR / 2], fl[PR / 2];
-  __shared__ int any;
   const int pr = blockIdx.x, tid = threadIdx.x;
-  const double* src = part + (long)pr * nks * PR * PR;
+  const int bi = pairs[2 * pr], bj = pairs[2 * pr + 1];
   for (int idx = tid; idx < PR * PR; idx += 256) {
-    double s = 0.0;
-    for (int k = 0; k < nks; ++k) s += src[(long)k * PR * PR + idx];
-    Gs[idx / PR][idx % PR] = s;
-    Js[idx / PR][idx % PR] = (idx / PR == idx % PR) ? 1.0 : 0.0;
+    const int r = idx / PR, cc = idx % PR;
+    Gs[r][cc] = G[pair_index(bi, bj, r) * ldg + pair_index(bi, bj, cc)];
+    Js[r][cc] = (r == cc) ? 1.0 : 0.0;
   }
-  int total = 0;
   __syncthreads();
+  int total = 0;
+  const int p = tid >> 3, sub = tid & 7;  // pair slot and 8-wide column group
   for (int sweep = 0; sweep < max_inner; ++sweep) {
-    if (tid == 0) any = 0;
-    __syncthreads();
+    int nsweep = 0;
     for (int step = 0; step < PR - 1; ++step) {
+      int f = 0;
       if (tid < PR / 2) {
         // circle method: position 0 fixed, the others rotate by `step`
         const int pa = tid, pb = PR - 1 - tid;
         const int i = (pa == 0) ? 0 : 1 + ((pa - 1 + step) % (PR - 1));
         const int j = 1 + ((pb - 1 + step) % (PR - 1));
         const double a = Gs[i][i], b = Gs[j][j], g = Gs[i][j];
-        int f = 0;
         double c = 1.0, s = 0.0;
-        if (a > 0.0 && b > 0.0 && fabs(g) > tol * sqrt(a) * sqrt(b)) {
+        if (fabs(g) > fmax(tol_rel * sqrt(fabs(a)) * sqrt(fabs(b)), tol_abs)) {
           const double zeta = (b - a) / (2.0 * g);
           double t;
           if (fabs(zeta) > 1e150) t = 0.5 / zeta;
@@ -124,29 +80,30 @@ __global__ __launch_bounds__(256) void jac_inner_kernel(const double* __restrict
           c = 1.0 / sqrt(1.0 + t * t);
           s = c * t;
           f = 1;
-          any = 1;
         }
         pi_[tid] = i; pj_[tid] = j; cs[tid] = c; sn[tid] = s; fl[tid] = f;
       }
-      __syncthreads();
-      // rows: row_i' = c row_i - s row_j ; row_j' = s row_i + c row_j
-      for (int idx = tid; idx < (PR / 2) * PR; idx += 256) {
-        const int p = idx / PR, col = idx % PR;
-        if (fl[p]) {
-          const int i = pi_[p], j = pj_[p];
-          const double c = cs[p], s = sn[p];
+      const int nrot = __syncthreads_count(f);
+      if (nrot == 0) continue;  // uniform: no rotation in this step
+      nsweep += nrot;
+      if (fl[p]) {  // rows: row_i' = c row_i - s row_j ; row_j' = s row_i + c row_j
+        const int i = pi_[p], j = pj_[p];
+        const double c = cs[p], s = sn[p];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int col = sub * 8 + u;
           const double gi = Gs[i][col], gj = Gs[j][col];
           Gs[i][col] = c * gi - s * gj;
           Gs[j][col] = s * gi + c * gj;
         }
       }
       __syncthreads();
-      // columns of G and J
-      for (int idx = tid; idx < (PR / 2) * PR; idx += 256) {
-        const int p = idx / PR, row = idx % PR;
-        if (fl[p]) {
-          const int i = pi_[p], j = pj_[p];
-          const double c = cs[p], s = sn[p];
+      if (fl[p]) {  // columns of G and J
+        const int i = pi_[p], j = pj_[p];
+        const double c = cs[p], s = sn[p];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int row = sub * 8 + u;
           const double gi = Gs[row][i], gj = Gs[row][j];
           Gs[row][i] = c * gi - s * gj;
           Gs[row][j] = s * gi + c * gj;
@@ -157,78 +114,44 @@ __global__ __launch_bounds__(256) void jac_inner_kernel(const double* __restrict
       }
       __syncthreads();
     }
-    const int a_ = any;
-    __syncthreads();
-    if (!a_) break;
+    if (nsweep == 0) break;
     ++total;
   }
   double* J = Jout + (long)pr * PR * PR;
-  if (total > 0) {
-    // J is a product of a few hundred plane rotations and is orthogonal only
-    // to ~1e-14; one Newton-Schulz step J <- J (3I - J^T J) / 2 squares that
-    // error, so repeated application keeps the vectors' norms exact.
-    for (int idx = tid; idx < PR * PR; idx += 256) {
-      const int i = idx / PR, j = idx % PR;
-      double s = 0.0;
-      for (int k = 0; k < PR; ++k) s = fma(Js[k][i], Js[k][j], s);
-      Gs[i][j] = (i == j ? 1.5 : 0.0) - 0.5 * s;
-    }
-    __syncthreads();
-    for (int idx = tid; idx < PR * PR; idx += 256) {
-      const int r = idx / PR, cc = idx % PR;
-      double s = 0.0;
-      for (int k = 0; k < PR; ++k) s = fma(Js[r][k], Gs[k][cc], s);
-      J[idx] = s;
-    }
-  } else {
-    for (int idx = tid; idx < PR * PR; idx += 256) J[idx] = Js[idx / PR][idx % PR];
-  }
+  for (int idx = tid; idx < PR * PR; idx += 256) J[idx] = Js[idx / PR][idx % PR];
   if (tid == 0) rotcount[pr] = total;
 }
 
-// rows of Q (unit vectors up to rounding) rescaled to unit norm; grid (rows)
-__global__ __launch_bounds__(256) void normalize_rows_kernel(double* __restrict__ Q, long L, long ld) {
-  __shared__ double red[4];
-  __shared__ double inv;
-  const long r = blockIdx.x;
-  double s = 0.0;
-  for (long j = threadIdx.x; j < L; j += blockDim.x) s = fma(Q[r * ld + j], Q[r * ld + j], s);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const double t = red[0] + red[1] + red[2] + red[3];
-    inv = t > 0.0 ? 1.0 / sqrt(t) : 0.0;
-  }
-  __syncthreads();
-  for (long j = threadIdx.x; j < L; j += blockDim.x) Q[r * ld + j] *= inv;
-}
-
-// X_pair <- J^T X_pair (and Qt_pair) in place; grid (npairs, tilesX + tilesQ)
-__global__ __launch_bounds__(256) void jac_apply_kernel(double* __restrict__ X, long q, long ldx,
-                                                        double* __restrict__ Q, long qq, long ldq,
+// Row update: rows of each pair of A (ld, L columns) <- J^T rows.  Used for
+// G (rows phase) and V^T.  grid (npairs, tiles), 64-column tiles.
+// Column update (COLS): columns of each pair of G <- columns * J, 64-row tiles.
+template <bool COLS>
+__global__ __launch_bounds__(256) void jac_apply_kernel(double* __restrict__ A, long L, long ld,
                                                         const int* __restrict__ pairs,
                                                         const double* __restrict__ Jall,
-                                                        const int* __restrict__ rotcount, int tilesX) {
-  __shared__ double Jt[PR][PR + 2];   // Jt[k][m] = J[k][m]  (A^T operand, k-major)
-  __shared__ double Ys[PR][PR + 2];   // Ys[k][n]
+                                                        const int* __restrict__ rotcount) {
+  __shared__ double Xs[PR][PR + 2];   // A operand, [m][k]
+  __shared__ double Ys[PR][PR + 2];   // B operand, [k][n]
   const int pr = blockIdx.x;
   if (rotcount[pr] == 0) return;
   const int bi = pairs[2 * pr], bj = pairs[2 * pr + 1];
-  double* base;
-  long L, ld, c0;
-  if ((int)blockIdx.y < tilesX) { base = X; L = q; ld = ldx; c0 = (long)blockIdx.y * PR; }
-  else { base = Q; L = qq; ld = ldq; c0 = (long)(blockIdx.y - tilesX) * PR; }
+  const long t0 = (long)blockIdx.y * PR;  // first free index of the tile
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const double* J = Jall + (long)pr * PR * PR;
   for (int idx = tid; idx < PR * PR; idx += 256) {
     const int r = idx / PR, cc = idx % PR;
-    Jt[r][cc] = J[idx];
-    const long row = (r < BR) ? ((long)bi * BR + r) : ((long)bj * BR + r - BR);
-    const long col = c0 + cc;
-    Ys[r][cc] = (col < L) ? base[row * ld + col] : 0.0;
+    if (!COLS) {
+      // out[rr][col] = sum_k J[k][rr] * A[pair k][col]: Xs = J^T, Ys = A rows
+      Xs[r][cc] = J[cc * PR + r];
+      const long col = t0 + cc;
+      Ys[r][cc] = (col < L) ? A[pair_index(bi, bj, r) * ld + col] : 0.0;
+    } else {
+      // out[row][cc] = sum_k A[row][pair k] * J[k][cc]: Xs = A tile, Ys = J
+      const long row = t0 + r;
+      Xs[r][cc] = (row < L) ? A[row * ld + pair_index(bi, bj, cc)] : 0.0;
+      Ys[r][cc] = J[r * PR + cc];
+    }
   }
   __syncthreads();
   d4 acc[2][2];
@@ -241,7 +164,7 @@ __global__ __launch_bounds__(256) void jac_apply_kernel(double* __restrict__ X, 
     const int kr = kk + (lane >> 4);
     double fa[2], fb[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) fa[i] = Jt[kr][wm * 32 + i * 16 + (lane & 15)];
+    for (int i = 0; i < 2; ++i) fa[i] = Xs[wm * 32 + i * 16 + (lane & 15)][kr];
 #pragma unroll
     for (int j = 0; j < 2; ++j) fb[j] = Ys[kr][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
@@ -257,9 +180,13 @@ __global__ __launch_bounds__(256) void jac_apply_kernel(double* __restrict__ X, 
       for (int r = 0; r < 4; ++r) {
         const int rr = wm * 32 + i * 16 + (lane >> 4) + 4 * r;  // f64 MFMA output layout
         const int cc = wn * 32 + j * 16 + (lane & 15);
-        const long row = (rr < BR) ? ((long)bi * BR + rr) : ((long)bj * BR + rr - BR);
-        const long col = c0 + cc;
-        if (col < L) base[row * ld + col] = acc[i][j][r];
+        if (!COLS) {
+          const long col = t0 + cc;
+          if (col < L) A[pair_index(bi, bj, rr) * ld + col] = acc[i][j][r];
+        } else {
+          const long row = t0 + rr;
+          if (row < L) A[row * ld + pair_index(bi, bj, cc)] = acc[i][j][r];
+        }
       }
 }
 
@@ -267,6 +194,11 @@ __global__ void set_identity_kernel(double* __restrict__ Q, long n) {
   const long total = n * n;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
     Q[i] = (i / n == i % n) ? 1.0 : 0.0;
+}
+
+__global__ void diag_kernel(const double* __restrict__ G, long ld, long n, double* __restrict__ d) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    d[i] = G[i * ld + i];
 }
 
 struct Schedule {
@@ -292,58 +224,59 @@ Schedule make_schedule(long nb) {
 
 }  // namespace
 
-void svd_jacobi(tn_ctx* c, const double* M, long m, long n, SvdOut& out) {
-  TN_REQUIRE(m > 0 && n > 0, TN_E_INVALID, "svd: empty matrix");
-  const long p = std::min(m, n), q = std::max(m, n);
+void sym_eig_jacobi(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w,
+                    int* sweeps_out) {
+  // G (p x p, leading dimension ldg >= ppad, rows/cols p..ppad-1 zero) is
+  // overwritten by its (nearly) diagonal form; Vt (ppad x ppad) receives the
+  // eigenvectors as rows.
   const long ppad = ((p + PR - 1) / PR) * PR;
   const long nb = ppad / BR;
-  out.p = p;
-  out.q = q;
-  out.vec_cols = (n <= m);
-  double* X = (double*)c->workspace(WS_SVD_X, (size_t)ppad * q * sizeof(double));
-  double* Q = (double*)c->workspace(WS_SVD_Q, (size_t)ppad * ppad * sizeof(double));
-  out.X = X;
-  out.Qt = Q;
-  // vectors as rows of X
-  if (out.vec_cols) transpose_f64(c, M, m, n, X);  // X = M^T (n x m)
-  else copy_f64(c, X, M, p * q);                    // X = M   (m x n)
-  if (ppad > p) fill_f64(c, X + p * q, 0.0, (ppad - p) * q);
   {
-    long tot = ppad * ppad;
-    int g = (int)std::min<long>(4096, (tot + 255) / 256);
-    hipLaunchKernelGGL(set_identity_kernel, dim3(g), dim3(256), 0, c->stream, Q, ppad);
+    const long tot = ppad * ppad;
+    const int g = (int)std::min<long>(4096, (tot + 255) / 256);
+    TN_REQUIRE(ldv == ppad, TN_E_INVALID, "sym_eig: V leading dimension");
+    hipLaunchKernelGGL(set_identity_kernel, dim3(g), dim3(256), 0, c->stream, Vt, ppad);
     TN_HIP(hipGetLastError());
   }
+  double* d = (double*)c->workspace(WS_SVD_S, (size_t)ppad * sizeof(double));
+  hipLaunchKernelGGL(diag_kernel, dim3((unsigned)std::min<long>(1024, (p + 255) / 256)), dim3(256), 0, c->stream, G,
+                     ldg, p, d);
+  TN_HIP(hipGetLastError());
+  std::vector<double> d0(p);
+  TN_HIP(hipMemcpyAsync(d0.data(), d, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipStreamSynchronize(c->stream));
+  double trace = 0.0;
+  for (double v : d0) trace += std::fabs(v);
   const Schedule sch = make_schedule(nb);
-  const long np = nb / 2;
-  const long nrounds = nb - 1;
+  const long np = nb / 2, nrounds = nb - 1;
   int* dpairs = (int*)c->workspace(WS_SVD_PAIRS, sch.pairs.size() * sizeof(int));
   TN_HIP(hipMemcpyAsync(dpairs, sch.pairs.data(), sch.pairs.size() * sizeof(int), hipMemcpyHostToDevice,
                         c->stream));
-  // k-split of the Gram: aim for ~512 workgroups per round
-  long ksplit = std::max<long>(1, std::min<long>((512 + np - 1) / np, (q + 127) / 128));
-  long kc = (q + ksplit - 1) / ksplit;
-  kc = ((kc + GBK - 1) / GBK) * GBK;
-  ksplit = (q + kc - 1) / kc;
-  double* part = (double*)c->workspace(WS_SVD_GRAM, (size_t)np * ksplit * PR * PR * sizeof(double));
   double* J = (double*)c->workspace(WS_SVD_J, (size_t)np * PR * PR * sizeof(double));
   int* rot = (int*)c->workspace(WS_SVD_CNT, (size_t)nrounds * np * sizeof(int));
-  const double tol = 2.220446049250313e-16 * std::max(1.0, std::sqrt((double)q));
-  const int tilesX = (int)((q + PR - 1) / PR), tilesQ = (int)((ppad + PR - 1) / PR);
+  // rotate (i, j) while |g_ij| > max(tol_rel sqrt(g_ii g_jj), tol_abs): the
+  // absolute floor eps * trace(G) is the accuracy a density-matrix
+  // eigensolver reaches; below it a rotation only moves rounding noise.
+  const double eps = 2.220446049250313e-16;
+  const double tol_rel = 64.0 * eps, tol_abs = eps * trace;
+  const int tiles = (int)(ppad / PR);
   std::vector<int> hrot((size_t)nrounds * np);
-  const int max_sweeps = 60;
+  const int max_sweeps = 40;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     for (long r = 0; r < nrounds; ++r) {
-      const int* pr = dpairs + r * np * 2;
-      hipLaunchKernelGGL(jac_gram_kernel, dim3((unsigned)np, (unsigned)ksplit), dim3(256), 0, c->stream, X, q,
-                         q, pr, kc, part);
+      const int* prs = dpairs + r * np * 2;
+      hipLaunchKernelGGL(jac_inner_kernel, dim3((unsigned)np), dim3(256), 0, c->stream, G, ldg, prs, tol_rel,
+                         tol_abs, 3, J, rot + r * np);
       TN_HIP(hipGetLastError());
-      hipLaunchKernelGGL(jac_inner_kernel, dim3((unsigned)np), dim3(256), 0, c->stream, part, (int)ksplit, tol,
-                         4, J, rot + r * np);
+      hipLaunchKernelGGL((jac_apply_kernel<false>), dim3((unsigned)np, (unsigned)tiles), dim3(256), 0, c->stream,
+                         G, ppad, ldg, prs, J, rot + r * np);
       TN_HIP(hipGetLastError());
-      hipLaunchKernelGGL(jac_apply_kernel, dim3((unsigned)np, (unsigned)(tilesX + tilesQ)), dim3(256), 0,
-                         c->stream, X, q, q, Q, ppad, ppad, pr, J, rot + r * np, tilesX);
+      hipLaunchKernelGGL((jac_apply_kernel<true>), dim3((unsigned)np, (unsigned)tiles), dim3(256), 0, c->stream,
+                         G, ppad, ldg, prs, J, rot + r * np);
+      TN_HIP(hipGetLastError());
+      hipLaunchKernelGGL((jac_apply_kernel<false>), dim3((unsigned)np, (unsigned)tiles), dim3(256), 0, c->stream,
+                         Vt, ppad, ldv, prs, J, rot + r * np);
       TN_HIP(hipGetLastError());
     }
     TN_HIP(hipMemcpyAsync(hrot.data(), rot, hrot.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -352,15 +285,37 @@ void svd_jacobi(tn_ctx* c, const double* M, long m, long n, SvdOut& out) {
     for (int v : hrot) tot += v;
     if (tot == 0) break;
   }
-  out.sweeps = sweep + 1;
-  hipLaunchKernelGGL(normalize_rows_kernel, dim3((unsigned)p), dim3(256), 0, c->stream, Q, p, ppad);
+  if (sweeps_out) *sweeps_out = sweep + 1;
+  static const bool dbg = std::getenv("TNHIP_DEBUG_SVD") != nullptr;
+  if (dbg) std::fprintf(stderr, "[tnhip eig] %ld: %d sweeps, %ld rounds/sweep\n", p, sweep + 1, nrounds);
+  hipLaunchKernelGGL(diag_kernel, dim3((unsigned)std::min<long>(1024, (p + 255) / 256)), dim3(256), 0, c->stream, G,
+                     ldg, p, d);
   TN_HIP(hipGetLastError());
-  double* s2 = (double*)c->workspace(WS_SVD_S, (size_t)p * sizeof(double));
-  row_norms2(c, X, p, q, q, s2);
-  out.s.resize(p);
-  TN_HIP(hipMemcpyAsync(out.s.data(), s2, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  w.resize(p);
+  TN_HIP(hipMemcpyAsync(w.data(), d, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TN_HIP(hipStreamSynchronize(c->stream));
-  for (auto& v : out.s) v = std::sqrt(std::max(0.0, v));
+}
+
+void gram_eig(tn_ctx* c, const double* M, long m, long n, bool left, GramEig& out) {
+  TN_REQUIRE(m > 0 && n > 0, TN_E_INVALID, "svd: empty matrix");
+  const long p = left ? m : n;
+  const long ppad = ((p + PR - 1) / PR) * PR;
+  double* G = (double*)c->workspace(WS_SVD_X, (size_t)ppad * ppad * sizeof(double));
+  double* Vt = (double*)c->workspace(WS_SVD_Q, (size_t)ppad * ppad * sizeof(double));
+  fill_f64(c, G, 0.0, ppad * ppad);
+  GemmDesc g;
+  g.M = p; g.N = p; g.C = G; g.ldc = ppad;
+  if (left) {  // G = M M^T
+    g.K = n; g.A = M; g.lda = n; g.B = M; g.ldb = n; g.tB = 1;
+  } else {     // G = M^T M
+    g.K = m; g.A = M; g.lda = n; g.tA = 1; g.B = M; g.ldb = n;
+  }
+  gemm(c, Prec::F64, g);
+  out.p = p;
+  out.ppad = ppad;
+  out.left = left;
+  out.Vt = Vt;
+  sym_eig_jacobi(c, G, p, ppad, Vt, ppad, out.w, &out.sweeps);
 }
 
 TruncRule truncate_weights(const std::vector<double>& Pin, long maxdim, long mindim, double cutoff) {

@@ -20,14 +20,23 @@ constexpr int MAX_DOT = 32;       // basis vectors per multi-dot pass
 // staged in LDS, each wavefront walks a quarter of the CSR output rows (rows
 // are wave-uniform, so the CSR entries come through the scalar cache).
 // ---------------------------------------------------------------------------
-template <typename T>
+__device__ __forceinline__ unsigned short f2bf_rne(float x) {
+  unsigned int u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+
+// SPLIT: T is float and the result is stored as bf16 hi/lo pairs
+template <typename T, bool SPLIT>
 __global__ __launch_bounds__(256) void wstep_kernel(int G, int Gp, int D, int Dp, long nA, long nB,
                                                     const T* __restrict__ in, long in_sg, long in_sA,
                                                     long in_sS, T* __restrict__ out, long out_sg,
                                                     long out_sA, long out_sS,
                                                     const int* __restrict__ rowptr,
                                                     const int* __restrict__ colidx,
-                                                    const double* __restrict__ vals) {
+                                                    const double* __restrict__ vals,
+                                                    unsigned short* __restrict__ ohi,
+                                                    unsigned short* __restrict__ olo) {
   extern __shared__ unsigned char smem_raw[];
   T* slab = reinterpret_cast<T*>(smem_raw);  // [K][64]
   const int K = G * D;
@@ -50,7 +59,14 @@ __global__ __launch_bounds__(256) void wstep_kernel(int G, int Gp, int D, int Dp
     T acc = T(0);
     for (int z = b; z < e; ++z) acc += (T)vals[z] * slab[colidx[z] * 64 + tb];
     const int gp = o / Dp, Sp = o % Dp;
-    out[gp * out_sg + A * out_sA + Sp * out_sS + B] = acc;
+    const long off = gp * out_sg + A * out_sA + Sp * out_sS + B;
+    if constexpr (SPLIT) {
+      const unsigned short h = f2bf_rne((float)acc);
+      ohi[off] = h;
+      olo[off] = f2bf_rne((float)acc - __uint_as_float(((unsigned int)h) << 16));
+    } else {
+      out[off] = acc;
+    }
   }
 }
 
@@ -261,27 +277,38 @@ void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>
   }
 }
 
-void wstep(tn_ctx* c, bool fp32, const WStepDesc& w) {
+static void wstep_launch(tn_ctx* c, bool fp32, bool split, const WStepDesc& w, unsigned short* hi,
+                         unsigned short* lo) {
   TN_REQUIRE(w.op && w.op->rows == w.Gp * w.Dp && w.op->cols == w.G * w.D, TN_E_SHAPE,
              "wstep: operator shape mismatch");
   if (w.nA == 0 || w.nB == 0) return;
-  TN_REQUIRE(w.nA <= 2147483647L, TN_E_INVALID, "wstep: nA too large");
+  TN_REQUIRE(w.nA <= 65535L, TN_E_INVALID, "wstep: nA exceeds grid.y");
   const int K = w.G * w.D;
   const size_t shm = (size_t)K * 64 * (fp32 ? sizeof(float) : sizeof(double));
   TN_REQUIRE(shm <= 64 * 1024, TN_E_UNSUPPORTED, "wstep: MPO slab exceeds 64 KiB of LDS");
   dim3 grid((unsigned)((w.nB + 63) / 64), (unsigned)w.nA);
-  TN_REQUIRE(w.nA <= 65535L, TN_E_INVALID, "wstep: nA exceeds grid.y");
-  if (fp32)
-    hipLaunchKernelGGL(wstep_kernel<float>, grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
-                       w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg,
-                       w.out_sA, w.out_sS, w.op->rowptr.as<int>(), w.op->colidx.as<int>(),
-                       w.op->vals.as<double>());
+  const int* rp = w.op->rowptr.as<int>();
+  const int* ci = w.op->colidx.as<int>();
+  const double* vv = w.op->vals.as<double>();
+  if (split)
+    hipLaunchKernelGGL((wstep_kernel<float, true>), grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
+                       w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)nullptr, w.out_sg, w.out_sA,
+                       w.out_sS, rp, ci, vv, hi, lo);
+  else if (fp32)
+    hipLaunchKernelGGL((wstep_kernel<float, false>), grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
+                       w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg, w.out_sA,
+                       w.out_sS, rp, ci, vv, (unsigned short*)nullptr, (unsigned short*)nullptr);
   else
-    hipLaunchKernelGGL(wstep_kernel<double>, grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
-                       w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg,
-                       w.out_sA, w.out_sS, w.op->rowptr.as<int>(), w.op->colidx.as<int>(),
-                       w.op->vals.as<double>());
+    hipLaunchKernelGGL((wstep_kernel<double, false>), grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
+                       w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg, w.out_sA,
+                       w.out_sS, rp, ci, vv, (unsigned short*)nullptr, (unsigned short*)nullptr);
   TN_HIP(hipGetLastError());
+}
+
+void wstep(tn_ctx* c, bool fp32, const WStepDesc& w) { wstep_launch(c, fp32, false, w, nullptr, nullptr); }
+
+void wstep_split(tn_ctx* c, const WStepDesc& w, unsigned short* hi, unsigned short* lo) {
+  wstep_launch(c, true, true, w, hi, lo);
 }
 
 void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n) {

@@ -90,6 +90,7 @@ struct tn_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   std::vector<int> ev_phase;
   size_t ev_used = 0;
+  int open_timers = 0;
   const double* ones = nullptr;  // device scalar 1.0 (edge environments)
   tn::DevBuf ones_buf;
 
@@ -136,6 +137,19 @@ struct GemmDesc {
   double alpha = 1.0, beta = 0.0;
 };
 void gemm(tn_ctx* c, Prec p, const GemmDesc& g);
+// double-buffered FP64/FP32 kernels (gemm_fast.hip); gemm() uses them when
+// gemm_fast_ok() holds (shapes >= one tile, 16-byte aligned operand rows)
+bool gemm_fast_ok(Prec p, const GemmDesc& g);
+void gemm_fast(tn_ctx* c, Prec p, const GemmDesc& g);
+// C (M x N, float) = A * B^T with A (M x K) and B (N x K) as bf16 hi/lo
+// pairs, rows padded to multiples of 8 elements
+void gemm_bf16x3_nt(tn_ctx* c, long M, long N, long K, const unsigned short* Ah, const unsigned short* Al,
+                    long lda, const unsigned short* Bh, const unsigned short* Bl, long ldb, float* C, long ldc);
+// bf16 hi/lo split of a rows x cols matrix (row stride lds, FP64 or FP32);
+// transpose = false: dst is rows x ldd; true: dst is cols x ldd (ldd >= rows);
+// padding columns are zero
+void split_bf16(tn_ctx* c, const void* src, bool src_f64, long rows, long cols, long lds, bool transpose,
+                unsigned short* hi, unsigned short* lo, long ldd);
 
 // ---------------------------------------------------------------------------
 // small-MPO-operator contraction ("W step") -- tensor_ops.hip
@@ -156,6 +170,9 @@ struct WStepDesc {
   const SparseOp* op = nullptr;
 };
 void wstep(tn_ctx* c, bool fp32, const WStepDesc& w);
+// W step from FP32 input writing the result as bf16 hi/lo pairs at the
+// positions w.out_* would address (w.out is ignored)
+void wstep_split(tn_ctx* c, const WStepDesc& w, unsigned short* hi, unsigned short* lo);
 void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>& dense);
 
 // ---------------------------------------------------------------------------
@@ -191,17 +208,20 @@ void gather_rows_T(tn_ctx* c, const double* src, long ld_src, const int* perm,
                    const double* scale, long kcount, long L, double* out);
 
 // ---------------------------------------------------------------------------
-// truncated SVD (one-sided block Jacobi) -- svd.hip
+// density-matrix eigen-decomposition (two-sided block Jacobi) -- svd.hip
 // ---------------------------------------------------------------------------
-struct SvdOut {
-  long p = 0, q = 0;        // p = min(m,n) vectors of length q = max(m,n)
-  bool vec_cols = true;     // true: vectors are the columns of M (n <= m)
-  double* X = nullptr;      // p x q: rotated vectors (rows), x_i = s_i * (u_i or v_i)
-  double* Qt = nullptr;     // p x p: accumulated rotation, rows = v_i (cols case) or u_i
-  std::vector<double> s;    // singular values (unsorted, length p)
+struct GramEig {
+  long p = 0, ppad = 0;     // Gram size (m if left, n otherwise), padded to 64
+  bool left = true;         // G = M M^T (left) or M^T M (right)
+  double* Vt = nullptr;     // ppad x ppad workspace: eigenvectors as rows (first p columns)
+  std::vector<double> w;    // eigenvalues (unsorted, length p) = weights s^2
   int sweeps = 0;
 };
-void svd_jacobi(tn_ctx* c, const double* M, long m, long n, SvdOut& out);
+void gram_eig(tn_ctx* c, const double* M, long m, long n, bool left, GramEig& out);
+// eigen-decomposition of a symmetric p x p device matrix G (ld ldg, padding
+// rows/columns zero), eigenvectors as rows of Vt (ld = padded size)
+void sym_eig_jacobi(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w,
+                    int* sweeps_out);
 
 // truncation rule of NDTensors truncate!!
 struct TruncRule {

@@ -55,7 +55,9 @@ int guarded(tn_ctx* c, F&& f) {
 namespace tn {
 PhaseTimer::PhaseTimer(tn_ctx* ctx, int phase) : c(ctx), ph(phase), idx((size_t)-1) {
   if (!c->timing) return;
-  if (c->ev_used >= 2048) flush_timers(c);
+  // flush only when no timer is open: an enclosing timer's stop event is
+  // not recorded yet and must keep its slot
+  if (c->ev_used >= 4096 && c->open_timers == 0) flush_timers(c);
   if (c->ev_used >= c->ev_pool.size()) {
     hipEvent_t a, b;
     TN_HIP(hipEventCreate(&a));
@@ -65,19 +67,25 @@ PhaseTimer::PhaseTimer(tn_ctx* ctx, int phase) : c(ctx), ph(phase), idx((size_t)
   }
   idx = c->ev_used++;
   c->ev_phase[idx] = ph;
+  c->open_timers++;
   TN_HIP(hipEventRecord(c->ev_pool[idx].first, c->stream));
 }
 PhaseTimer::~PhaseTimer() {
-  if (idx != (size_t)-1) (void)hipEventRecord(c->ev_pool[idx].second, c->stream);
+  if (idx == (size_t)-1) return;
+  c->open_timers--;
+  if (hipEventRecord(c->ev_pool[idx].second, c->stream) != hipSuccess) (void)hipGetLastError();
 }
 void flush_timers(tn_ctx* c) {
   if (c->ev_used == 0) return;
   TN_HIP(hipStreamSynchronize(c->stream));
   for (size_t i = 0; i < c->ev_used; ++i) {
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c->ev_pool[i].first, c->ev_pool[i].second) == hipSuccess)
+    if (hipEventElapsedTime(&ms, c->ev_pool[i].first, c->ev_pool[i].second) == hipSuccess) {
       c->phase_ms[c->ev_phase[i]] += ms;
-    c->phase_n[c->ev_phase[i]] += 1;
+      c->phase_n[c->ev_phase[i]] += 1;
+    } else {
+      (void)hipGetLastError();  // do not leave a sticky error for the next launch check
+    }
   }
   c->ev_used = 0;
 }
@@ -114,6 +122,9 @@ struct tn_env {
   // FP32 shadows of the environments used by mixed-precision matvecs
   DevBuf L32, R32;
   long shadow_key_l = -1, shadow_key_r = -1;
+  // bf16 hi/lo shadows (BF16x3 matvec)
+  DevBuf Lh, Ll, Rh, Rl;
+  long split_key_l = -1, split_key_r = -1;
   long env_version = 0;
   std::vector<long> LR_version;
 
@@ -375,6 +386,39 @@ const float* shadow(tn_env* e, int k, bool left) {
   return B.as<float>();
 }
 
+inline long pad8(long x) { return ((x + 7) / 8) * 8; }
+
+// bf16 hi/lo shadows of the environments for the BF16x3 matvec:
+//   left  Lt[(w a'), a]   -> rows w*chi, k = a contiguous, ld = pad8(chi)
+//   right Rt[(w2 b), b']  -> transposed: rows b', k = (w2 b) contiguous,
+//                            ld = pad8(w2 chi)
+struct SplitView {
+  const unsigned short* hi;
+  const unsigned short* lo;
+  long ld;
+};
+SplitView split_shadow(tn_env* e, int k, bool left) {
+  DevBuf& H = left ? e->Lh : e->Rh;
+  DevBuf& Lo = left ? e->Ll : e->Rl;
+  long& key = left ? e->split_key_l : e->split_key_r;
+  const EnvT& E = e->LR[k];
+  const long want = e->LR_version[k] * 4096 + k;
+  long ld, rows;
+  if (left) { rows = (long)E.w * E.chi; ld = pad8(E.chi); }
+  else { rows = E.chi; ld = pad8((long)E.w * E.chi); }
+  if (key != want) {
+    H.ensure((size_t)rows * ld * 2);
+    Lo.ensure((size_t)rows * ld * 2);
+    if (left)
+      split_bf16(e->ctx, E.buf.p, true, rows, E.chi, E.chi, false, H.as<unsigned short>(), Lo.as<unsigned short>(), ld);
+    else  // source [(w2 b)][b'] is (w2 chi) x chi; transposed to chi x (w2 chi)
+      split_bf16(e->ctx, E.buf.p, true, (long)E.w * E.chi, E.chi, E.chi, true, H.as<unsigned short>(),
+                 Lo.as<unsigned short>(), ld);
+    key = want;
+  }
+  return SplitView{H.as<unsigned short>(), Lo.as<unsigned short>(), ld};
+}
+
 // product(sysenv, v) (ProjMPO.product; mirror src/mps/projmps2.jl:159-173)
 void matvec(tn_env* e, const void* v, void* out, int prec) {
   PhaseTimer pt(e->ctx, PH_MATVEC);
@@ -394,11 +438,45 @@ void matvec(tn_env* e, const void* v, void* out, int prec) {
   const bool cx = e->cplx();
   if (cx && prec != TN_PREC_FP64)
     throw Error(TN_E_UNSUPPORTED, "mixed-precision matvec is implemented for real tensors only");
+  if (prec == TN_PREC_BF16X3) {
+    // T1 = Lt * psi              (bf16x3, A = Lt split, B^T = psi^T split)
+    // T2 = W-step(T1) written as bf16 hi/lo rows (a' S') x (w2 b)
+    // out = T2 * Rt              (bf16x3, B^T = Rt^T split)
+    const SplitView Ls = split_shadow(e, pos - 1, true);
+    const SplitView Rs = split_shadow(e, pos + e->nsite, false);
+    const long ldp = pad8(chil);
+    const long N1 = (long)DD * chir;
+    unsigned short* ph = (unsigned short*)c->workspace(WS_PSI32, (size_t)N1 * ldp * 2 * 2);
+    unsigned short* pl = ph + N1 * ldp;
+    split_bf16(c, v, true, chil, N1, N1, true, ph, pl, ldp);  // psi^T: N1 x chil
+    float* T1 = (float*)c->workspace(WS_T1, (size_t)w0 * chil * N1 * 4);
+    {
+      PhaseTimer pg(c, PH_MV_GEMM);
+      gemm_bf16x3_nt(c, (long)w0 * chil, N1, chil, Ls.hi, Ls.lo, Ls.ld, ph, pl, ldp, T1, N1);
+    }
+    const long K3 = (long)w2 * chir, ld3 = pad8(K3);
+    const long M3 = chil * DD;
+    unsigned short* th = (unsigned short*)c->workspace(WS_T2, (size_t)M3 * ld3 * 2 * 2);
+    unsigned short* tl = th + M3 * ld3;
+    WStepDesc ws;
+    ws.G = w0; ws.Gp = w2; ws.D = DD; ws.Dp = DD;
+    ws.nA = chil; ws.nB = chir;
+    ws.in = T1; ws.in_sg = chil * N1; ws.in_sA = N1; ws.in_sS = chir;
+    ws.out = nullptr; ws.out_sg = chir; ws.out_sA = (long)DD * ld3; ws.out_sS = ld3;
+    ws.op = &op;
+    wstep_split(c, ws, th, tl);
+    float* o32 = (float*)c->workspace(WS_OUT32, (size_t)D.n * 4);
+    {
+      PhaseTimer pg(c, PH_MV_GEMM);
+      gemm_bf16x3_nt(c, M3, chir, K3, th, tl, ld3, Rs.hi, Rs.lo, Rs.ld, o32, chir);
+    }
+    cvt_f32_to_f64(c, o32, (double*)out, D.n);
+    return;
+  }
   Prec P;
   bool f32 = false;
   if (prec == TN_PREC_FP64) P = cx ? Prec::Z64 : Prec::F64;
   else if (prec == TN_PREC_FP32) { P = Prec::F32; f32 = true; }
-  else if (prec == TN_PREC_BF16X3) { P = Prec::BF16X3; f32 = true; }
   else throw Error(TN_E_INVALID, "unknown precision");
   const size_t es = f32 ? 4 : e->esz;
   const int m2 = cx ? 2 : 1;
@@ -769,65 +847,72 @@ void contract_phi(tn_env* e, int pos, void* phi) {
   gemm(e->ctx, gemm_prec_full(e), g);
 }
 
+// order of the p eigenvalues, largest first (stable), and the weights P of
+// the first r = rank-bound of them (negative rounding residue clipped)
+void sorted_weights(const std::vector<double>& w, long r, std::vector<int>& order, std::vector<double>& P) {
+  const long p = (long)w.size();
+  order.resize(p);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return w[x] > w[y]; });
+  P.resize(r);
+  for (long i = 0; i < r; ++i) P[i] = std::max(0.0, w[order[i]]);
+}
+
 // replacebond!(psi, b, phi; maxdim, mindim, cutoff, ortho, normalize)
+// ortho = "left":  A_b = top-k eigenvectors of phi phi^T, A_{b+1} = A_b^T phi / |.|
+// ortho = "right": A_{b+1} = top-k eigenvectors of phi^T phi (rows), A_b = phi A_{b+1}^T / |.|
+// Same factors as U, S V^T (left) or U S, V^T (right) of the truncated SVD.
 void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
                  bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept) {
   PhaseTimer pt(e->ctx, PH_SVD);
-  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "replacebond!: complex SVD is not built");
+  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "replacebond!: complex density-matrix factorisation is not built");
   TN_REQUIRE(b >= 1 && b < e->N, TN_E_POSITION, "replacebond!: bond out of range");
   tn_ctx* c = e->ctx;
   const long chil = e->A[b - 1].chil, chir = e->A[b].chir;
   const int d1 = e->d[b - 1], d2 = e->d[b];
   const long m = chil * d1, n = (long)d2 * chir;
-  SvdOut sv;
-  svd_jacobi(c, phi, m, n, sv);
-  const long p = sv.p;
-  std::vector<int> order(p);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
-  std::vector<double> P(p);
-  for (long i = 0; i < p; ++i) P[i] = sv.s[order[i]] * sv.s[order[i]];
+  GramEig ge;
+  gram_eig(c, phi, m, n, ortho_left, ge);
+  std::vector<int> order;
+  std::vector<double> P;
+  sorted_weights(ge.w, std::min(m, n), order, P);
   const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
   const long k = tr.n;
   double nrm = 0.0;
   for (long i = 0; i < k; ++i) nrm += P[i];
   nrm = std::sqrt(nrm);
   const double inv_norm = (normalize && nrm > 0.0) ? 1.0 / nrm : 1.0;
-  std::vector<double> sc_inv_s(k), sc_s_n(k), sc_one(k, 1.0), sc_invn(k, inv_norm), sc_invs_n(k);
-  for (long i = 0; i < k; ++i) {
-    const double s = sv.s[order[i]];
-    sc_inv_s[i] = s > 0.0 ? 1.0 / s : 0.0;
-    sc_s_n[i] = s * inv_norm;
-  }
   int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
-  double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * k * sizeof(double));
+  double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
+  std::vector<double> ones(k, 1.0);
   TN_HIP(hipMemcpyAsync(dperm, order.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  const long ldq = ((p + 63) / 64) * 64;
+  TN_HIP(hipMemcpyAsync(dsc, ones.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
   SiteT& Sa = e->A[b - 1];
   SiteT& Sb = e->A[b];
   Sa.buf.ensure((size_t)m * k * sizeof(double));
   Sb.buf.ensure((size_t)k * n * sizeof(double));
-  // scale vectors for the two outputs
-  const std::vector<double>* scA;
-  const std::vector<double>* scB;
-  const double* srcA;
-  long ldA;
-  const double* srcB;
-  long ldB;
-  if (sv.vec_cols) {  // X rows = s_i u_i^T (length m), Qt rows = v_i^T (length n)
-    srcA = sv.X; ldA = sv.q; srcB = sv.Qt; ldB = ldq;
-    if (ortho_left) { scA = &sc_inv_s; scB = &sc_s_n; }
-    else { scA = &sc_invn; scB = &sc_one; }
-  } else {  // X rows = s_i v_i^T (length n), Qt rows = u_i^T (length m)
-    srcA = sv.Qt; ldA = ldq; srcB = sv.X; ldB = sv.q;
-    if (ortho_left) { scA = &sc_one; scB = &sc_invn; }
-    else { scA = &sc_s_n; scB = &sc_inv_s; }
+  if (ortho_left) {
+    gather_rows_T(c, ge.Vt, ge.ppad, dperm, dsc, k, m, Sa.buf.as<double>());  // A_b = V_k (m x k)
+    double* Vk = (double*)c->workspace(WS_SVD_GRAM, (size_t)k * m * sizeof(double));
+    gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, m, Vk);                    // V_k^T (k x m)
+    GemmDesc g;
+    g.M = k; g.N = n; g.K = m;
+    g.A = Vk; g.lda = m;
+    g.B = phi; g.ldb = n;
+    g.C = Sb.buf.p; g.ldc = n;
+    g.alpha = inv_norm;
+    gemm(c, Prec::F64, g);                                                   // A_{b+1} = V_k^T phi
+  } else {
+    gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, n, Sb.buf.as<double>());   // A_{b+1} = V_k^T (k x n)
+    GemmDesc g;
+    g.M = m; g.N = k; g.K = n;
+    g.A = phi; g.lda = n;
+    g.B = Sb.buf.p; g.ldb = n; g.tB = 1;
+    g.C = Sa.buf.p; g.ldc = k;
+    g.alpha = inv_norm;
+    gemm(c, Prec::F64, g);                                                   // A_b = phi V_k
   }
-  TN_HIP(hipMemcpyAsync(dsc, scA->data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  TN_HIP(hipMemcpyAsync(dsc + k, scB->data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  gather_rows_T(c, srcA, ldA, dperm, dsc, k, m, Sa.buf.as<double>());  // A_b (m x k)
-  gather_rows(c, srcB, ldB, dperm, dsc + k, k, n, Sb.buf.as<double>());  // A_{b+1} (k x n)
-  TN_HIP(hipStreamSynchronize(c->stream));  // host scale vectors go out of scope
+  TN_HIP(hipStreamSynchronize(c->stream));  // host vectors above go out of scope
   Sa.chir = k;
   Sb.chil = k;
   invalidate_site(e, b);
@@ -838,62 +923,60 @@ void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxd
   if (nkept) *nkept = k;
 }
 
-// orthogonalize!(psi, site): exact gauge move with untruncated SVDs
+// orthogonalize!(psi, site): exact gauge move.  Each step keeps the full row
+// (column) space of the site matrix: the top min(m, n) eigenvectors of its
+// Gram matrix become the orthonormal factor, the exact remainder moves on.
 void orthogonalize(tn_env* e, int site) {
   TN_REQUIRE(site >= 1 && site <= e->N, TN_E_POSITION, "orthogonalize!: site out of range");
-  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "orthogonalize!: complex SVD is not built");
+  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "orthogonalize!: complex density-matrix factorisation is not built");
   require_sites(e);
   tn_ctx* c = e->ctx;
+  auto upload_perm = [&](const std::vector<int>& order, long r, int*& dperm, double*& dsc) {
+    dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)r * sizeof(int));
+    dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)r * sizeof(double));
+    std::vector<double> ones(r, 1.0);
+    TN_HIP(hipMemcpyAsync(dperm, order.data(), r * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(dsc, ones.data(), r * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+  };
   // right part: sites N..site+1 become right-orthonormal
   for (int k = e->N; k > site; --k) {
     SiteT& S = e->A[k - 1];
-    SiteT& P = e->A[k - 2];
+    SiteT& Pv = e->A[k - 2];
     const long m = S.chil, n = (long)e->d[k - 1] * S.chir;
-    SvdOut sv;
-    svd_jacobi(c, S.buf.as<double>(), m, n, sv);
-    const long p = sv.p;
-    const long ldq = ((p + 63) / 64) * 64;
-    std::vector<int> order(p);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
-    std::vector<double> sc1(p, 1.0), scs(p), scinv(p);
-    for (long i = 0; i < p; ++i) {
-      const double s = sv.s[order[i]];
-      scs[i] = s;
-      scinv[i] = s > 0 ? 1.0 / s : 0.0;
-    }
-    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)p * sizeof(int));
-    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * p * sizeof(double));
-    TN_HIP(hipMemcpyAsync(dperm, order.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    // new A_k = V^T (p x n); US (m x p)
-    double* US = (double*)c->workspace(WS_PHI, (size_t)m * p * sizeof(double));
+    GramEig ge;
+    gram_eig(c, S.buf.as<double>(), m, n, false, ge);
+    std::vector<int> order;
+    std::vector<double> P;
+    const long r = std::min(m, n);
+    sorted_weights(ge.w, r, order, P);
+    int* dperm;
+    double* dsc;
+    upload_perm(order, r, dperm, dsc);
     DevBuf newA;
-    newA.ensure((size_t)p * n * sizeof(double));
-    if (sv.vec_cols) {  // X rows = s u^T (len m), Qt rows = v^T (len n)
-      TN_HIP(hipMemcpyAsync(dsc, sc1.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      gather_rows(c, sv.Qt, ldq, dperm, dsc, p, n, newA.as<double>());
-      gather_rows_T(c, sv.X, sv.q, dperm, dsc, p, m, US);
-    } else {  // X rows = s v^T (len n), Qt rows = u^T (len m)
-      TN_HIP(hipMemcpyAsync(dsc, scinv.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      TN_HIP(hipMemcpyAsync(dsc + p, scs.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      gather_rows(c, sv.X, sv.q, dperm, dsc, p, n, newA.as<double>());
-      gather_rows_T(c, sv.Qt, ldq, dperm, dsc + p, p, m, US);
-    }
-    // A_{k-1} <- A_{k-1} * US
-    const long pm = P.chil * e->d[k - 2];
+    newA.ensure((size_t)r * n * sizeof(double));
+    gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, r, n, newA.as<double>());  // V_r^T (r x n)
+    double* SV = (double*)c->workspace(WS_PHI, (size_t)m * r * sizeof(double));
+    GemmDesc g1;
+    g1.M = m; g1.N = r; g1.K = n;
+    g1.A = S.buf.p; g1.lda = n;
+    g1.B = newA.p; g1.ldb = n; g1.tB = 1;
+    g1.C = SV; g1.ldc = r;
+    gemm(c, Prec::F64, g1);                                              // S V_r (m x r)
+    const long pm = Pv.chil * e->d[k - 2];
     DevBuf newP;
-    newP.ensure((size_t)pm * p * sizeof(double));
-    GemmDesc g;
-    g.M = pm; g.N = p; g.K = m;
-    g.A = P.buf.p; g.lda = m;
-    g.B = US; g.ldb = p;
-    g.C = newP.p; g.ldc = p;
-    gemm(c, Prec::F64, g);
+    newP.ensure((size_t)pm * r * sizeof(double));
+    GemmDesc g2;
+    g2.M = pm; g2.N = r; g2.K = m;
+    g2.A = Pv.buf.p; g2.lda = m;
+    g2.B = SV; g2.ldb = r;
+    g2.C = newP.p; g2.ldc = r;
+    gemm(c, Prec::F64, g2);                                              // A_{k-1} S V_r
     TN_HIP(hipStreamSynchronize(c->stream));
     S.buf = std::move(newA);
-    S.chil = p;
-    P.buf = std::move(newP);
-    P.chir = p;
+    S.chil = r;
+    Pv.buf = std::move(newP);
+    Pv.chir = r;
     invalidate_site(e, k);
     invalidate_site(e, k - 1);
   }
@@ -902,54 +985,45 @@ void orthogonalize(tn_env* e, int site) {
     SiteT& S = e->A[k - 1];
     SiteT& Nx = e->A[k];
     const long m = S.chil * e->d[k - 1], n = S.chir;
-    SvdOut sv;
-    svd_jacobi(c, S.buf.as<double>(), m, n, sv);
-    const long p = sv.p;
-    const long ldq = ((p + 63) / 64) * 64;
-    std::vector<int> order(p);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
-    std::vector<double> sc1(p, 1.0), scs(p), scinv(p);
-    for (long i = 0; i < p; ++i) {
-      const double s = sv.s[order[i]];
-      scs[i] = s;
-      scinv[i] = s > 0 ? 1.0 / s : 0.0;
-    }
-    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)p * sizeof(int));
-    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * p * sizeof(double));
-    TN_HIP(hipMemcpyAsync(dperm, order.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    double* SV = (double*)c->workspace(WS_PHI, (size_t)p * n * sizeof(double));
+    GramEig ge;
+    gram_eig(c, S.buf.as<double>(), m, n, true, ge);
+    std::vector<int> order;
+    std::vector<double> P;
+    const long r = std::min(m, n);
+    sorted_weights(ge.w, r, order, P);
+    int* dperm;
+    double* dsc;
+    upload_perm(order, r, dperm, dsc);
     DevBuf newA;
-    newA.ensure((size_t)m * p * sizeof(double));
-    if (sv.vec_cols) {  // X rows = s u^T (len m), Qt rows = v^T (len n)
-      TN_HIP(hipMemcpyAsync(dsc, scinv.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      TN_HIP(hipMemcpyAsync(dsc + p, scs.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      gather_rows_T(c, sv.X, sv.q, dperm, dsc, p, m, newA.as<double>());
-      gather_rows(c, sv.Qt, ldq, dperm, dsc + p, p, n, SV);
-    } else {
-      TN_HIP(hipMemcpyAsync(dsc, sc1.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      gather_rows_T(c, sv.Qt, ldq, dperm, dsc, p, m, newA.as<double>());
-      gather_rows(c, sv.X, sv.q, dperm, dsc, p, n, SV);
-    }
+    newA.ensure((size_t)m * r * sizeof(double));
+    gather_rows_T(c, ge.Vt, ge.ppad, dperm, dsc, r, m, newA.as<double>());  // U_r (m x r)
+    double* US = (double*)c->workspace(WS_PHI, (size_t)r * n * sizeof(double));
+    GemmDesc g1;
+    g1.M = r; g1.N = n; g1.K = m;
+    g1.A = newA.p; g1.lda = r; g1.tA = 1;
+    g1.B = S.buf.p; g1.ldb = n;
+    g1.C = US; g1.ldc = n;
+    gemm(c, Prec::F64, g1);                                              // U_r^T S (r x n)
     const long nn = (long)e->d[k] * Nx.chir;
     DevBuf newN;
-    newN.ensure((size_t)p * nn * sizeof(double));
-    GemmDesc g;
-    g.M = p; g.N = nn; g.K = n;
-    g.A = SV; g.lda = n;
-    g.B = Nx.buf.p; g.ldb = nn;
-    g.C = newN.p; g.ldc = nn;
-    gemm(c, Prec::F64, g);
+    newN.ensure((size_t)r * nn * sizeof(double));
+    GemmDesc g2;
+    g2.M = r; g2.N = nn; g2.K = n;
+    g2.A = US; g2.lda = n;
+    g2.B = Nx.buf.p; g2.ldb = nn;
+    g2.C = newN.p; g2.ldc = nn;
+    gemm(c, Prec::F64, g2);
     TN_HIP(hipStreamSynchronize(c->stream));
     S.buf = std::move(newA);
-    S.chir = p;
+    S.chir = r;
     Nx.buf = std::move(newN);
-    Nx.chil = p;
+    Nx.chil = r;
     invalidate_site(e, k);
     invalidate_site(e, k + 1);
   }
   e->ortho = site;
 }
+
 
 // _update_two_site! (src/mps/update_site.jl:27-90)
 void update_two_site(tn_env* e, int pos, bool ortho_left, const tn_update_params* p, tn_update_result* r,
@@ -1371,40 +1445,73 @@ int tn_svd_trunc(tn_ctx* c, const double* M, long m, long n, long maxdim, long m
   return guarded(c, [&] {
     TN_REQUIRE(c && M, TN_E_INVALID, "svd: null argument");
     PhaseTimer pt(c, PH_SVD);
-    SvdOut sv;
-    svd_jacobi(c, M, m, n, sv);
-    const long p = sv.p;
-    std::vector<int> order(p);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sv.s[x] > sv.s[y]; });
-    std::vector<double> P(p);
-    for (long i = 0; i < p; ++i) P[i] = sv.s[order[i]] * sv.s[order[i]];
+    // eigen-decomposition of the smaller Gram matrix; that side's singular
+    // vectors are its (orthonormal) eigenvectors, the other side's are
+    // M v / s (or u^T M / s)
+    const bool left = m <= n;
+    GramEig ge;
+    gram_eig(c, M, m, n, left, ge);
+    std::vector<int> order;
+    std::vector<double> P;
+    sorted_weights(ge.w, std::min(m, n), order, P);
     const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
     const long k = tr.n;
-    const long ldq = ((p + 63) / 64) * 64;
     std::vector<double> one(k, 1.0), inv(k);
     for (long i = 0; i < k; ++i) {
-      const double s = sv.s[order[i]];
+      const double s = std::sqrt(P[i]);
       inv[i] = s > 0 ? 1.0 / s : 0.0;
       if (S) S[i] = s;
     }
     int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
-    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)2 * k * sizeof(double));
+    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
     TN_HIP(hipMemcpyAsync(dperm, order.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
     TN_HIP(hipMemcpyAsync(dsc, one.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    TN_HIP(hipMemcpyAsync(dsc + k, inv.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if (sv.vec_cols) {  // X rows = s u^T, Qt rows = v^T
-      if (U) gather_rows_T(c, sv.X, sv.q, dperm, dsc + k, k, m, U);
-      if (Vt) gather_rows(c, sv.Qt, ldq, dperm, dsc, k, n, Vt);
-    } else {  // X rows = s v^T, Qt rows = u^T
-      if (U) gather_rows_T(c, sv.Qt, ldq, dperm, dsc, k, m, U);
-      if (Vt) gather_rows(c, sv.X, sv.q, dperm, dsc + k, k, n, Vt);
+    double* W = (double*)c->workspace(WS_SVD_GRAM, (size_t)k * std::max(m, n) * sizeof(double));
+    if (left) {
+      gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, m, W);  // U_k^T (k x m)
+      if (U) gather_rows_T(c, ge.Vt, ge.ppad, dperm, dsc, k, m, U);
+      if (Vt) {
+        GemmDesc g;
+        g.M = k; g.N = n; g.K = m;
+        g.A = W; g.lda = m;
+        g.B = M; g.ldb = n;
+        g.C = Vt; g.ldc = n;
+        gemm(c, Prec::F64, g);  // U_k^T M = S V^T
+      }
+    } else {
+      gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, n, W);  // V_k^T (k x n)
+      if (Vt) copy_f64(c, Vt, W, k * n);
+      if (U) {
+        GemmDesc g;
+        g.M = m; g.N = k; g.K = n;
+        g.A = M; g.lda = n;
+        g.B = W; g.ldb = n; g.tB = 1;
+        g.C = U; g.ldc = k;
+        gemm(c, Prec::F64, g);  // M V_k = U S
+      }
+    }
+    // divide the non-orthonormal factor by s
+    double* dinv = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
+    TN_HIP(hipMemcpyAsync(dinv, inv.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<int> ident(k);
+    std::iota(ident.begin(), ident.end(), 0);
+    TN_HIP(hipMemcpyAsync(dperm, ident.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (left && Vt) {
+      copy_f64(c, W, Vt, k * n);
+      gather_rows(c, W, n, dperm, dinv, k, n, Vt);
+    }
+    if (!left && U) {
+      // U (m x k) columns scaled: go through the transpose
+      double* T = (double*)c->workspace(WS_MISC, (size_t)m * k * sizeof(double));
+      transpose_f64(c, U, m, k, T);                // T = U^T (k x m)
+      gather_rows_T(c, T, m, dperm, dinv, k, m, U);  // U[j][i] = T[i][j] / s_i
     }
     TN_HIP(hipStreamSynchronize(c->stream));
     if (nkept) *nkept = k;
     if (truncerr) *truncerr = tr.truncerr;
   });
 }
+
 
 int tn_update_two_site(tn_env* e, int pos, int ortho_left, const tn_update_params* p, tn_update_result* r) {
   return guarded(C(e), [&] {
