@@ -190,6 +190,192 @@ __global__ __launch_bounds__(256) void jac_apply_kernel(double* __restrict__ A, 
       }
 }
 
+// ---------------------------------------------------------------------------
+// Complex Hermitian variant (interleaved re/im).  For g_ij = |g| e^{i alpha}
+// the unitary R = diag(1, e^{-i alpha}) R_real zeroes the pair:
+//   R[:, i] = (c, -s e^{-i alpha}),  R[:, j] = (s, c e^{-i alpha}),
+// rows:    X_i <- c X_i - s e^{i alpha} X_j,  X_j <- s X_i + c e^{i alpha} X_j   (R^H X)
+// columns: X_i <- c X_i - s e^{-i alpha} X_j, X_j <- s X_i + c e^{-i alpha} X_j  (X R)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void jac_inner_z_kernel(const double2* __restrict__ G, long ldg,
+                                                          const int* __restrict__ pairs, double tol_rel,
+                                                          double tol_abs, int max_inner,
+                                                          double2* __restrict__ Jout, int* __restrict__ rotcount) {
+  __shared__ double Gr[PR][LDP], Gi[PR][LDP];
+  __shared__ double Jr[PR][LDP], Ji[PR][LDP];
+  __shared__ double cs[PR / 2], sn[PR / 2], er[PR / 2], ei[PR / 2];  // e^{i alpha}
+  __shared__ int pi_[PR / 2], pj_[PR / 2], fl[PR / 2];
+  const int pr = blockIdx.x, tid = threadIdx.x;
+  const int bi = pairs[2 * pr], bj = pairs[2 * pr + 1];
+  for (int idx = tid; idx < PR * PR; idx += 256) {
+    const int r = idx / PR, cc = idx % PR;
+    const double2 v = G[pair_index(bi, bj, r) * ldg + pair_index(bi, bj, cc)];
+    Gr[r][cc] = v.x;
+    Gi[r][cc] = v.y;
+    Jr[r][cc] = (r == cc) ? 1.0 : 0.0;
+    Ji[r][cc] = 0.0;
+  }
+  __syncthreads();
+  int total = 0;
+  const int p = tid >> 3, sub = tid & 7;
+  for (int sweep = 0; sweep < max_inner; ++sweep) {
+    int nsweep = 0;
+    for (int step = 0; step < PR - 1; ++step) {
+      int f = 0;
+      if (tid < PR / 2) {
+        const int pa = tid, pb = PR - 1 - tid;
+        const int i = (pa == 0) ? 0 : 1 + ((pa - 1 + step) % (PR - 1));
+        const int j = 1 + ((pb - 1 + step) % (PR - 1));
+        const double a = Gr[i][i], b = Gr[j][j];
+        const double gr = Gr[i][j], gi = Gi[i][j];
+        const double ag = sqrt(gr * gr + gi * gi);
+        double c = 1.0, s = 0.0, ur = 1.0, ui = 0.0;
+        if (ag > fmax(tol_rel * sqrt(fabs(a)) * sqrt(fabs(b)), tol_abs)) {
+          const double zeta = (b - a) / (2.0 * ag);
+          double t;
+          if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+          else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = c * t;
+          ur = gr / ag;
+          ui = gi / ag;
+          f = 1;
+        }
+        pi_[tid] = i; pj_[tid] = j; cs[tid] = c; sn[tid] = s; er[tid] = ur; ei[tid] = ui; fl[tid] = f;
+      }
+      const int nrot = __syncthreads_count(f);
+      if (nrot == 0) continue;
+      nsweep += nrot;
+      if (fl[p]) {  // rows (R^H G): uses e^{i alpha}
+        const int i = pi_[p], j = pj_[p];
+        const double c = cs[p], s = sn[p], ur = er[p], ui = ei[p];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int col = sub * 8 + u;
+          const double xr = Gr[i][col], xi = Gi[i][col], yr = Gr[j][col], yi = Gi[j][col];
+          const double wr = ur * yr - ui * yi, wi = ur * yi + ui * yr;  // e^{i alpha} X_j
+          Gr[i][col] = c * xr - s * wr;
+          Gi[i][col] = c * xi - s * wi;
+          Gr[j][col] = s * xr + c * wr;
+          Gi[j][col] = s * xi + c * wi;
+        }
+      }
+      __syncthreads();
+      if (fl[p]) {  // columns (G R) and J R: uses e^{-i alpha}
+        const int i = pi_[p], j = pj_[p];
+        const double c = cs[p], s = sn[p], ur = er[p], ui = -ei[p];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int row = sub * 8 + u;
+          {
+            const double xr = Gr[row][i], xi = Gi[row][i], yr = Gr[row][j], yi = Gi[row][j];
+            const double wr = ur * yr - ui * yi, wi = ur * yi + ui * yr;
+            Gr[row][i] = c * xr - s * wr;
+            Gi[row][i] = c * xi - s * wi;
+            Gr[row][j] = s * xr + c * wr;
+            Gi[row][j] = s * xi + c * wi;
+          }
+          {
+            const double xr = Jr[row][i], xi = Ji[row][i], yr = Jr[row][j], yi = Ji[row][j];
+            const double wr = ur * yr - ui * yi, wi = ur * yi + ui * yr;
+            Jr[row][i] = c * xr - s * wr;
+            Ji[row][i] = c * xi - s * wi;
+            Jr[row][j] = s * xr + c * wr;
+            Ji[row][j] = s * xi + c * wi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+    if (nsweep == 0) break;
+    ++total;
+  }
+  double2* J = Jout + (long)pr * PR * PR;
+  for (int idx = tid; idx < PR * PR; idx += 256) J[idx] = make_double2(Jr[idx / PR][idx % PR], Ji[idx / PR][idx % PR]);
+  if (tid == 0) rotcount[pr] = total;
+}
+
+// rows mode: pair rows of A <- J^H rows; COLS: pair columns of A <- columns J.
+// Scalar complex FMAs; thread t owns row t/4 and 16 columns of the 64x64 tile.
+template <bool COLS>
+__global__ __launch_bounds__(256) void jac_apply_z_kernel(double2* __restrict__ A, long L, long ld,
+                                                          const int* __restrict__ pairs,
+                                                          const double2* __restrict__ Jall,
+                                                          const int* __restrict__ rotcount) {
+  __shared__ double Xr[PR][LDP], Xi[PR][LDP];  // tile of A (row-mode: [pair k][col]; col-mode: [row][pair k])
+  __shared__ double Kr[PR][LDP], Ki[PR][LDP];  // J
+  const int pr = blockIdx.x;
+  if (rotcount[pr] == 0) return;
+  const int bi = pairs[2 * pr], bj = pairs[2 * pr + 1];
+  const long t0 = (long)blockIdx.y * PR;
+  const int tid = threadIdx.x;
+  const double2* J = Jall + (long)pr * PR * PR;
+  for (int idx = tid; idx < PR * PR; idx += 256) {
+    const int r = idx / PR, cc = idx % PR;
+    const double2 jv = J[idx];
+    Kr[r][cc] = jv.x;
+    Ki[r][cc] = jv.y;
+    double2 v = make_double2(0.0, 0.0);
+    if (!COLS) {
+      const long col = t0 + cc;
+      if (col < L) v = A[pair_index(bi, bj, r) * ld + col];
+    } else {
+      const long row = t0 + r;
+      if (row < L) v = A[row * ld + pair_index(bi, bj, cc)];
+    }
+    Xr[r][cc] = v.x;
+    Xi[r][cc] = v.y;
+  }
+  __syncthreads();
+  const int rr = tid >> 2, c0 = (tid & 3) * 16;
+  double orr[16], oi[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) { orr[u] = 0.0; oi[u] = 0.0; }
+  for (int k = 0; k < PR; ++k) {
+    if (!COLS) {
+      // out[rr][col] = sum_k conj(J[k][rr]) X[k][col]
+      const double jr = Kr[k][rr], ji = -Ki[k][rr];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const double xr = Xr[k][c0 + u], xi = Xi[k][c0 + u];
+        orr[u] = fma(jr, xr, fma(-ji, xi, orr[u]));
+        oi[u] = fma(jr, xi, fma(ji, xr, oi[u]));
+      }
+    } else {
+      // out[rr][cc] = sum_k X[rr][k] J[k][cc]
+      const double xr = Xr[rr][k], xi = Xi[rr][k];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const double jr = Kr[k][c0 + u], ji = Ki[k][c0 + u];
+        orr[u] = fma(xr, jr, fma(-xi, ji, orr[u]));
+        oi[u] = fma(xr, ji, fma(xi, jr, oi[u]));
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int cc = c0 + u;
+    if (!COLS) {
+      const long col = t0 + cc;
+      if (col < L) A[pair_index(bi, bj, rr) * ld + col] = make_double2(orr[u], oi[u]);
+    } else {
+      const long row = t0 + rr;
+      if (row < L) A[row * ld + pair_index(bi, bj, cc)] = make_double2(orr[u], oi[u]);
+    }
+  }
+}
+
+__global__ void set_identity_z_kernel(double2* __restrict__ Q, long n) {
+  const long total = n * n;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
+    Q[i] = make_double2((i / n == i % n) ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void diag_z_kernel(const double2* __restrict__ G, long ld, long n, double* __restrict__ d) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    d[i] = G[i * ld + i].x;
+}
+
 __global__ void set_identity_kernel(double* __restrict__ Q, long n) {
   const long total = n * n;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < total; i += (long)gridDim.x * blockDim.x)
@@ -224,24 +410,30 @@ Schedule make_schedule(long nb) {
 
 }  // namespace
 
-void sym_eig_jacobi(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w,
+void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bool cplx, std::vector<double>& w,
                     int* sweeps_out) {
-  // G (p x p, leading dimension ldg >= ppad, rows/cols p..ppad-1 zero) is
-  // overwritten by its (nearly) diagonal form; Vt (ppad x ppad) receives the
-  // eigenvectors as rows.
+  // G (p x p symmetric / Hermitian, leading dimension ldg >= ppad, padding
+  // rows/columns zero) is overwritten by its (nearly) diagonal form; Vt
+  // (ppad x ppad) receives V^T (real) or V^H (complex): eigenvectors as rows
+  // (conjugated in the complex case).
   const long ppad = ((p + PR - 1) / PR) * PR;
   const long nb = ppad / BR;
+  const unsigned gd = (unsigned)std::min<long>(1024, (p + 255) / 256);
   {
     const long tot = ppad * ppad;
     const int g = (int)std::min<long>(4096, (tot + 255) / 256);
     TN_REQUIRE(ldv == ppad, TN_E_INVALID, "sym_eig: V leading dimension");
-    hipLaunchKernelGGL(set_identity_kernel, dim3(g), dim3(256), 0, c->stream, Vt, ppad);
+    if (cplx) hipLaunchKernelGGL(set_identity_z_kernel, dim3(g), dim3(256), 0, c->stream, (double2*)Vt, ppad);
+    else hipLaunchKernelGGL(set_identity_kernel, dim3(g), dim3(256), 0, c->stream, (double*)Vt, ppad);
     TN_HIP(hipGetLastError());
   }
   double* d = (double*)c->workspace(WS_SVD_S, (size_t)ppad * sizeof(double));
-  hipLaunchKernelGGL(diag_kernel, dim3((unsigned)std::min<long>(1024, (p + 255) / 256)), dim3(256), 0, c->stream, G,
-                     ldg, p, d);
-  TN_HIP(hipGetLastError());
+  auto diag = [&]() {
+    if (cplx) hipLaunchKernelGGL(diag_z_kernel, dim3(gd), dim3(256), 0, c->stream, (const double2*)G, ldg, p, d);
+    else hipLaunchKernelGGL(diag_kernel, dim3(gd), dim3(256), 0, c->stream, (const double*)G, ldg, p, d);
+    TN_HIP(hipGetLastError());
+  };
+  diag();
   std::vector<double> d0(p);
   TN_HIP(hipMemcpyAsync(d0.data(), d, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TN_HIP(hipStreamSynchronize(c->stream));
@@ -252,7 +444,7 @@ void sym_eig_jacobi(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv
   int* dpairs = (int*)c->workspace(WS_SVD_PAIRS, sch.pairs.size() * sizeof(int));
   TN_HIP(hipMemcpyAsync(dpairs, sch.pairs.data(), sch.pairs.size() * sizeof(int), hipMemcpyHostToDevice,
                         c->stream));
-  double* J = (double*)c->workspace(WS_SVD_J, (size_t)np * PR * PR * sizeof(double));
+  void* J = c->workspace(WS_SVD_J, (size_t)np * PR * PR * (cplx ? 16 : 8));
   int* rot = (int*)c->workspace(WS_SVD_CNT, (size_t)nrounds * np * sizeof(int));
   // rotate (i, j) while |g_ij| > max(tol_rel sqrt(g_ii g_jj), tol_abs): the
   // absolute floor eps * trace(G) is the accuracy a density-matrix
@@ -260,23 +452,35 @@ void sym_eig_jacobi(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv
   const double eps = 2.220446049250313e-16;
   const double tol_rel = 64.0 * eps, tol_abs = eps * trace;
   const int tiles = (int)(ppad / PR);
+  const dim3 ga((unsigned)np, (unsigned)tiles);
   std::vector<int> hrot((size_t)nrounds * np);
   const int max_sweeps = 40;
   int sweep = 0;
   for (; sweep < max_sweeps; ++sweep) {
     for (long r = 0; r < nrounds; ++r) {
       const int* prs = dpairs + r * np * 2;
-      hipLaunchKernelGGL(jac_inner_kernel, dim3((unsigned)np), dim3(256), 0, c->stream, G, ldg, prs, tol_rel,
-                         tol_abs, 3, J, rot + r * np);
-      TN_HIP(hipGetLastError());
-      hipLaunchKernelGGL((jac_apply_kernel<false>), dim3((unsigned)np, (unsigned)tiles), dim3(256), 0, c->stream,
-                         G, ppad, ldg, prs, J, rot + r * np);
-      TN_HIP(hipGetLastError());
-      hipLaunchKernelGGL((jac_apply_kernel<true>), dim3((unsigned)np, (unsigned)tiles), dim3(256), 0, c->stream,
-                         G, ppad, ldg, prs, J, rot + r * np);
-      TN_HIP(hipGetLastError());
-      hipLaunchKernelGGL((jac_apply_kernel<false>), dim3((unsigned)np, (unsigned)tiles), dim3(256), 0, c->stream,
-                         Vt, ppad, ldv, prs, J, rot + r * np);
+      int* rc = rot + r * np;
+      if (!cplx) {
+        double* Gd = (double*)G;
+        double* Jd = (double*)J;
+        hipLaunchKernelGGL(jac_inner_kernel, dim3((unsigned)np), dim3(256), 0, c->stream, Gd, ldg, prs, tol_rel,
+                           tol_abs, 3, Jd, rc);
+        TN_HIP(hipGetLastError());
+        hipLaunchKernelGGL((jac_apply_kernel<false>), ga, dim3(256), 0, c->stream, Gd, ppad, ldg, prs, Jd, rc);
+        hipLaunchKernelGGL((jac_apply_kernel<true>), ga, dim3(256), 0, c->stream, Gd, ppad, ldg, prs, Jd, rc);
+        hipLaunchKernelGGL((jac_apply_kernel<false>), ga, dim3(256), 0, c->stream, (double*)Vt, ppad, ldv, prs, Jd,
+                           rc);
+      } else {
+        double2* Gz = (double2*)G;
+        double2* Jz = (double2*)J;
+        hipLaunchKernelGGL(jac_inner_z_kernel, dim3((unsigned)np), dim3(256), 0, c->stream, Gz, ldg, prs, tol_rel,
+                           tol_abs, 3, Jz, rc);
+        TN_HIP(hipGetLastError());
+        hipLaunchKernelGGL((jac_apply_z_kernel<false>), ga, dim3(256), 0, c->stream, Gz, ppad, ldg, prs, Jz, rc);
+        hipLaunchKernelGGL((jac_apply_z_kernel<true>), ga, dim3(256), 0, c->stream, Gz, ppad, ldg, prs, Jz, rc);
+        hipLaunchKernelGGL((jac_apply_z_kernel<false>), ga, dim3(256), 0, c->stream, (double2*)Vt, ppad, ldv, prs,
+                           Jz, rc);
+      }
       TN_HIP(hipGetLastError());
     }
     TN_HIP(hipMemcpyAsync(hrot.data(), rot, hrot.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -287,35 +491,36 @@ void sym_eig_jacobi(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv
   }
   if (sweeps_out) *sweeps_out = sweep + 1;
   static const bool dbg = std::getenv("TNHIP_DEBUG_SVD") != nullptr;
-  if (dbg) std::fprintf(stderr, "[tnhip eig] %ld: %d sweeps, %ld rounds/sweep\n", p, sweep + 1, nrounds);
-  hipLaunchKernelGGL(diag_kernel, dim3((unsigned)std::min<long>(1024, (p + 255) / 256)), dim3(256), 0, c->stream, G,
-                     ldg, p, d);
-  TN_HIP(hipGetLastError());
+  if (dbg) std::fprintf(stderr, "[tnhip eig] %ld%s: %d sweeps, %ld rounds/sweep\n", p, cplx ? " (complex)" : "",
+                        sweep + 1, nrounds);
+  diag();
   w.resize(p);
   TN_HIP(hipMemcpyAsync(w.data(), d, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TN_HIP(hipStreamSynchronize(c->stream));
 }
 
-void gram_eig(tn_ctx* c, const double* M, long m, long n, bool left, GramEig& out) {
+void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out) {
   TN_REQUIRE(m > 0 && n > 0, TN_E_INVALID, "svd: empty matrix");
   const long p = left ? m : n;
   const long ppad = ((p + PR - 1) / PR) * PR;
-  double* G = (double*)c->workspace(WS_SVD_X, (size_t)ppad * ppad * sizeof(double));
-  double* Vt = (double*)c->workspace(WS_SVD_Q, (size_t)ppad * ppad * sizeof(double));
-  fill_f64(c, G, 0.0, ppad * ppad);
+  const size_t es = cplx ? 16 : 8;
+  void* G = c->workspace(WS_SVD_X, (size_t)ppad * ppad * es);
+  void* Vt = c->workspace(WS_SVD_Q, (size_t)ppad * ppad * es);
+  fill_f64(c, (double*)G, 0.0, ppad * ppad * (cplx ? 2 : 1));
   GemmDesc g;
   g.M = p; g.N = p; g.C = G; g.ldc = ppad;
-  if (left) {  // G = M M^T
-    g.K = n; g.A = M; g.lda = n; g.B = M; g.ldb = n; g.tB = 1;
-  } else {     // G = M^T M
-    g.K = m; g.A = M; g.lda = n; g.tA = 1; g.B = M; g.ldb = n;
+  if (left) {  // G = M M^T  (M M^H)
+    g.K = n; g.A = M; g.lda = n; g.B = M; g.ldb = n; g.tB = 1; g.cB = cplx;
+  } else {     // G = M^T M  (M^H M)
+    g.K = m; g.A = M; g.lda = n; g.tA = 1; g.cA = cplx; g.B = M; g.ldb = n;
   }
-  gemm(c, Prec::F64, g);
+  gemm(c, cplx ? Prec::Z64 : Prec::F64, g);
   out.p = p;
   out.ppad = ppad;
   out.left = left;
+  out.cplx = cplx;
   out.Vt = Vt;
-  sym_eig_jacobi(c, G, p, ppad, Vt, ppad, out.w, &out.sweeps);
+  sym_eig_jacobi(c, G, p, ppad, Vt, ppad, cplx, out.w, &out.sweeps);
 }
 
 TruncRule truncate_weights(const std::vector<double>& Pin, long maxdim, long mindim, double cutoff) {

@@ -244,6 +244,30 @@ __global__ __launch_bounds__(256) void gather_rows_T_kernel(const double* __rest
   }
 }
 
+// complex: out[j][k] = conj?(src[perm[k]][j]) * scale[k]; 16x16 complex tiles
+__global__ __launch_bounds__(256) void gather_rows_T_z_kernel(const double2* __restrict__ src, long ld,
+                                                              const int* __restrict__ perm,
+                                                              const double* __restrict__ scale, long kcount,
+                                                              long L, double2* __restrict__ out, int conj) {
+  __shared__ double2 t[16][17];
+  const long k0 = (long)blockIdx.y * 16, j0 = (long)blockIdx.x * 16;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16
+  {
+    const long k = k0 + ty, j = j0 + tx;
+    double2 v = make_double2(0.0, 0.0);
+    if (k < kcount && j < L) {
+      v = src[(long)perm[k] * ld + j];
+      if (conj) v.y = -v.y;
+      v.x *= scale[k];
+      v.y *= scale[k];
+    }
+    t[ty][tx] = v;
+  }
+  __syncthreads();
+  const long j = j0 + ty, k = k0 + tx;
+  if (j < L && k < kcount) out[j * kcount + k] = t[tx][ty];
+}
+
 inline int grid1d(long n, int threads = 256) {
   long g = (n + threads - 1) / threads;
   if (g > 4096) g = 4096;
@@ -403,6 +427,16 @@ void gather_rows(tn_ctx* c, const double* src, long ld_src, const int* perm, con
   TN_REQUIRE(kcount <= 65535, TN_E_INVALID, "gather_rows: kcount");
   dim3 grid((unsigned)((L + 255) / 256), (unsigned)kcount);
   hipLaunchKernelGGL(gather_rows_kernel, grid, dim3(256), 0, c->stream, src, ld_src, perm, scale, L, out);
+  TN_HIP(hipGetLastError());
+}
+
+void gather_rows_T_z(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
+                     long kcount, long L, double* out, bool conj) {
+  if (kcount <= 0 || L <= 0) return;
+  dim3 grid((unsigned)((L + 15) / 16), (unsigned)((kcount + 15) / 16));
+  TN_REQUIRE((kcount + 15) / 16 <= 65535, TN_E_INVALID, "gather_rows_T_z: kcount");
+  hipLaunchKernelGGL(gather_rows_T_z_kernel, grid, dim3(256), 0, c->stream, (const double2*)src, ld_src, perm,
+                     scale, kcount, L, (double2*)out, (int)conj);
   TN_HIP(hipGetLastError());
 }
 

@@ -212,16 +212,20 @@ void gather_rows_T(tn_ctx* c, const double* src, long ld_src, const int* perm,
 // ---------------------------------------------------------------------------
 struct GramEig {
   long p = 0, ppad = 0;     // Gram size (m if left, n otherwise), padded to 64
-  bool left = true;         // G = M M^T (left) or M^T M (right)
-  double* Vt = nullptr;     // ppad x ppad workspace: eigenvectors as rows (first p columns)
+  bool left = true;         // G = M M^H (left) or M^H M (right)
+  bool cplx = false;        // complex128 data (interleaved)
+  void* Vt = nullptr;       // ppad x ppad workspace: V^T (real) / V^H (complex), first p columns
   std::vector<double> w;    // eigenvalues (unsorted, length p) = weights s^2
   int sweeps = 0;
 };
-void gram_eig(tn_ctx* c, const double* M, long m, long n, bool left, GramEig& out);
-// eigen-decomposition of a symmetric p x p device matrix G (ld ldg, padding
-// rows/columns zero), eigenvectors as rows of Vt (ld = padded size)
-void sym_eig_jacobi(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w,
+void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out);
+// eigen-decomposition of a symmetric / Hermitian p x p device matrix G (ld
+// ldg, padding rows/columns zero); Vt (ld = padded size) receives V^T / V^H
+void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bool cplx, std::vector<double>& w,
                     int* sweeps_out);
+// complex: out[j][k] = conj?(src[perm[k]][j]) * scale[k]  (src rows of complex)
+void gather_rows_T_z(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
+                     long kcount, long L, double* out, bool conj);
 
 // truncation rule of NDTensors truncate!!
 struct TruncRule {

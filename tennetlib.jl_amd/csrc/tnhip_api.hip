@@ -865,14 +865,15 @@ void sorted_weights(const std::vector<double>& w, long r, std::vector<int>& orde
 void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
                  bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept) {
   PhaseTimer pt(e->ctx, PH_SVD);
-  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "replacebond!: complex density-matrix factorisation is not built");
   TN_REQUIRE(b >= 1 && b < e->N, TN_E_POSITION, "replacebond!: bond out of range");
   tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const int w2 = cx ? 2 : 1;  // doubles per element
   const long chil = e->A[b - 1].chil, chir = e->A[b].chir;
   const int d1 = e->d[b - 1], d2 = e->d[b];
   const long m = chil * d1, n = (long)d2 * chir;
   GramEig ge;
-  gram_eig(c, phi, m, n, ortho_left, ge);
+  gram_eig(c, phi, m, n, ortho_left, cx, ge);
   std::vector<int> order;
   std::vector<double> P;
   sorted_weights(ge.w, std::min(m, n), order, P);
@@ -889,28 +890,33 @@ void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxd
   TN_HIP(hipMemcpyAsync(dsc, ones.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
   SiteT& Sa = e->A[b - 1];
   SiteT& Sb = e->A[b];
-  Sa.buf.ensure((size_t)m * k * sizeof(double));
-  Sb.buf.ensure((size_t)k * n * sizeof(double));
+  Sa.buf.ensure((size_t)m * k * e->esz);
+  Sb.buf.ensure((size_t)k * n * e->esz);
+  const double* Vt = (const double*)ge.Vt;  // V^T (real) or V^H (complex) rows
+  const Prec P64 = gemm_prec_full(e);
   if (ortho_left) {
-    gather_rows_T(c, ge.Vt, ge.ppad, dperm, dsc, k, m, Sa.buf.as<double>());  // A_b = V_k (m x k)
-    double* Vk = (double*)c->workspace(WS_SVD_GRAM, (size_t)k * m * sizeof(double));
-    gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, m, Vk);                    // V_k^T (k x m)
+    // A_b = V_k (m x k), A_{b+1} = V_k^H phi / |.|
+    if (cx) gather_rows_T_z(c, Vt, ge.ppad, dperm, dsc, k, m, Sa.buf.as<double>(), true);
+    else gather_rows_T(c, Vt, ge.ppad, dperm, dsc, k, m, Sa.buf.as<double>());
+    double* Vk = (double*)c->workspace(WS_SVD_GRAM, (size_t)k * m * e->esz);
+    gather_rows(c, Vt, ge.ppad * w2, dperm, dsc, k, m * w2, Vk);
     GemmDesc g;
     g.M = k; g.N = n; g.K = m;
     g.A = Vk; g.lda = m;
     g.B = phi; g.ldb = n;
     g.C = Sb.buf.p; g.ldc = n;
     g.alpha = inv_norm;
-    gemm(c, Prec::F64, g);                                                   // A_{b+1} = V_k^T phi
+    gemm(c, P64, g);
   } else {
-    gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, n, Sb.buf.as<double>());   // A_{b+1} = V_k^T (k x n)
+    // A_{b+1} = V_k^H (k x n), A_b = phi V_k / |.|
+    gather_rows(c, Vt, ge.ppad * w2, dperm, dsc, k, n * w2, Sb.buf.as<double>());
     GemmDesc g;
     g.M = m; g.N = k; g.K = n;
     g.A = phi; g.lda = n;
-    g.B = Sb.buf.p; g.ldb = n; g.tB = 1;
+    g.B = Sb.buf.p; g.ldb = n; g.tB = 1; g.cB = cx;
     g.C = Sa.buf.p; g.ldc = k;
     g.alpha = inv_norm;
-    gemm(c, Prec::F64, g);                                                   // A_b = phi V_k
+    gemm(c, P64, g);
   }
   TN_HIP(hipStreamSynchronize(c->stream));  // host vectors above go out of scope
   Sa.chir = k;
@@ -928,9 +934,12 @@ void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxd
 // Gram matrix become the orthonormal factor, the exact remainder moves on.
 void orthogonalize(tn_env* e, int site) {
   TN_REQUIRE(site >= 1 && site <= e->N, TN_E_POSITION, "orthogonalize!: site out of range");
-  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "orthogonalize!: complex density-matrix factorisation is not built");
   require_sites(e);
   tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const int w2 = cx ? 2 : 1;
+  const size_t es = e->esz;
+  const Prec P64 = gemm_prec_full(e);
   auto upload_perm = [&](const std::vector<int>& order, long r, int*& dperm, double*& dsc) {
     dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)r * sizeof(int));
     dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)r * sizeof(double));
@@ -945,7 +954,7 @@ void orthogonalize(tn_env* e, int site) {
     SiteT& Pv = e->A[k - 2];
     const long m = S.chil, n = (long)e->d[k - 1] * S.chir;
     GramEig ge;
-    gram_eig(c, S.buf.as<double>(), m, n, false, ge);
+    gram_eig(c, S.buf.p, m, n, false, cx, ge);
     std::vector<int> order;
     std::vector<double> P;
     const long r = std::min(m, n);
@@ -954,24 +963,24 @@ void orthogonalize(tn_env* e, int site) {
     double* dsc;
     upload_perm(order, r, dperm, dsc);
     DevBuf newA;
-    newA.ensure((size_t)r * n * sizeof(double));
-    gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, r, n, newA.as<double>());  // V_r^T (r x n)
-    double* SV = (double*)c->workspace(WS_PHI, (size_t)m * r * sizeof(double));
+    newA.ensure((size_t)r * n * es);
+    gather_rows(c, (const double*)ge.Vt, ge.ppad * w2, dperm, dsc, r, n * w2, newA.as<double>());  // V_r^H
+    double* SV = (double*)c->workspace(WS_PHI, (size_t)m * r * es);
     GemmDesc g1;
     g1.M = m; g1.N = r; g1.K = n;
     g1.A = S.buf.p; g1.lda = n;
-    g1.B = newA.p; g1.ldb = n; g1.tB = 1;
+    g1.B = newA.p; g1.ldb = n; g1.tB = 1; g1.cB = cx;
     g1.C = SV; g1.ldc = r;
-    gemm(c, Prec::F64, g1);                                              // S V_r (m x r)
+    gemm(c, P64, g1);                                                    // S V_r (m x r)
     const long pm = Pv.chil * e->d[k - 2];
     DevBuf newP;
-    newP.ensure((size_t)pm * r * sizeof(double));
+    newP.ensure((size_t)pm * r * es);
     GemmDesc g2;
     g2.M = pm; g2.N = r; g2.K = m;
     g2.A = Pv.buf.p; g2.lda = m;
     g2.B = SV; g2.ldb = r;
     g2.C = newP.p; g2.ldc = r;
-    gemm(c, Prec::F64, g2);                                              // A_{k-1} S V_r
+    gemm(c, P64, g2);                                                    // A_{k-1} S V_r
     TN_HIP(hipStreamSynchronize(c->stream));
     S.buf = std::move(newA);
     S.chil = r;
@@ -986,7 +995,7 @@ void orthogonalize(tn_env* e, int site) {
     SiteT& Nx = e->A[k];
     const long m = S.chil * e->d[k - 1], n = S.chir;
     GramEig ge;
-    gram_eig(c, S.buf.as<double>(), m, n, true, ge);
+    gram_eig(c, S.buf.p, m, n, true, cx, ge);
     std::vector<int> order;
     std::vector<double> P;
     const long r = std::min(m, n);
@@ -995,24 +1004,25 @@ void orthogonalize(tn_env* e, int site) {
     double* dsc;
     upload_perm(order, r, dperm, dsc);
     DevBuf newA;
-    newA.ensure((size_t)m * r * sizeof(double));
-    gather_rows_T(c, ge.Vt, ge.ppad, dperm, dsc, r, m, newA.as<double>());  // U_r (m x r)
-    double* US = (double*)c->workspace(WS_PHI, (size_t)r * n * sizeof(double));
+    newA.ensure((size_t)m * r * es);
+    if (cx) gather_rows_T_z(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, r, m, newA.as<double>(), true);
+    else gather_rows_T(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, r, m, newA.as<double>());  // U_r
+    double* US = (double*)c->workspace(WS_PHI, (size_t)r * n * es);
     GemmDesc g1;
     g1.M = r; g1.N = n; g1.K = m;
-    g1.A = newA.p; g1.lda = r; g1.tA = 1;
+    g1.A = newA.p; g1.lda = r; g1.tA = 1; g1.cA = cx;
     g1.B = S.buf.p; g1.ldb = n;
     g1.C = US; g1.ldc = n;
-    gemm(c, Prec::F64, g1);                                              // U_r^T S (r x n)
+    gemm(c, P64, g1);                                                    // U_r^H S (r x n)
     const long nn = (long)e->d[k] * Nx.chir;
     DevBuf newN;
-    newN.ensure((size_t)r * nn * sizeof(double));
+    newN.ensure((size_t)r * nn * es);
     GemmDesc g2;
     g2.M = r; g2.N = nn; g2.K = n;
     g2.A = US; g2.lda = n;
     g2.B = Nx.buf.p; g2.ldb = nn;
     g2.C = newN.p; g2.ldc = nn;
-    gemm(c, Prec::F64, g2);
+    gemm(c, P64, g2);
     TN_HIP(hipStreamSynchronize(c->stream));
     S.buf = std::move(newA);
     S.chir = r;
@@ -1450,7 +1460,7 @@ int tn_svd_trunc(tn_ctx* c, const double* M, long m, long n, long maxdim, long m
     // M v / s (or u^T M / s)
     const bool left = m <= n;
     GramEig ge;
-    gram_eig(c, M, m, n, left, ge);
+    gram_eig(c, M, m, n, left, false, ge);
     std::vector<int> order;
     std::vector<double> P;
     sorted_weights(ge.w, std::min(m, n), order, P);
@@ -1468,8 +1478,8 @@ int tn_svd_trunc(tn_ctx* c, const double* M, long m, long n, long maxdim, long m
     TN_HIP(hipMemcpyAsync(dsc, one.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
     double* W = (double*)c->workspace(WS_SVD_GRAM, (size_t)k * std::max(m, n) * sizeof(double));
     if (left) {
-      gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, m, W);  // U_k^T (k x m)
-      if (U) gather_rows_T(c, ge.Vt, ge.ppad, dperm, dsc, k, m, U);
+      gather_rows(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, m, W);  // U_k^T (k x m)
+      if (U) gather_rows_T(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, m, U);
       if (Vt) {
         GemmDesc g;
         g.M = k; g.N = n; g.K = m;
@@ -1479,7 +1489,7 @@ int tn_svd_trunc(tn_ctx* c, const double* M, long m, long n, long maxdim, long m
         gemm(c, Prec::F64, g);  // U_k^T M = S V^T
       }
     } else {
-      gather_rows(c, ge.Vt, ge.ppad, dperm, dsc, k, n, W);  // V_k^T (k x n)
+      gather_rows(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, n, W);  // V_k^T (k x n)
       if (Vt) copy_f64(c, Vt, W, k * n);
       if (U) {
         GemmDesc g;

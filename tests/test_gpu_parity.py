@@ -275,3 +275,54 @@ def test_error_paths(T):
     with pytest.raises(T.TNHIPError) as ei:
         T.update_position_(env, T.eig_solver, 3, 2, "left")  # orthocenter unknown
     assert ei.value.code == T._lib.TN_E_POSITION
+
+
+def _sz_profile(A):
+    from tests.test_oracle import _sz_expect
+    return _sz_expect(A)
+
+
+def _padded_neel(N):
+    chis = [1] + [min(2 ** k, 2 ** (N - k)) for k in range(1, N)] + [1]
+    A = []
+    for k in range(N):
+        t = np.zeros((chis[k], 2, chis[k + 1]), dtype=complex)
+        t[0, k % 2, 0] = 1.0
+        A.append(t)
+    return A, chis
+
+
+def test_tdvp_exact_xx_chain_free_fermions(T):
+    """Two-site TDVP on the GPU at full bond dimension (cutoff -1) is exact:
+    <Sz_i(t)> of the XX chain from Neel follows the free-fermion closed form."""
+    import scipy.linalg as sla
+    N, dt, steps = 8, 0.05, 6
+    A, chis = _padded_neel(N)
+    eng = T.TDVPEngine(A, O.xxz_mpo(N, 1.0, 0.0))
+    for _ in range(steps):
+        T.tdvpsweep_(eng, -1j * dt, 2, maxdim=64, cutoff=-1.0, outputlevel=0)
+    hsp = 0.5 * (np.eye(N, k=1) + np.eye(N, k=-1))
+    Ut = sla.expm(-1j * hsp * dt * steps)
+    n = (np.abs(Ut[:, [i for i in range(N) if i % 2 == 0]]) ** 2).sum(1)
+    assert np.abs(_sz_profile(eng.sysenv.psi) - (n - 0.5)).max() < 1e-8
+    assert abs(eng.abstime - dt * steps) < 1e-14
+    assert abs(eng.swdata.energy[-1]) < 1e-8  # <H> of the XX chain from Neel is 0 and conserved
+
+
+def test_tdvp_truncated_xxz_matches_oracle(T):
+    """XXZ (Delta = 0.5) from Neel with truncation: GPU and oracle TDVP agree
+    on the energy (conserved, -(N-1) Delta / 4) and the Sz profile."""
+    N, dt, steps = 12, 0.05, 4
+    Ws = O.xxz_mpo(N, 1.0, 0.5)
+    A = [a.astype(complex) for a in O.neel_mps(N)]
+    eng = T.TDVPEngine(A, Ws)
+    ref = O.StateEnvs(A, Ws)
+    sw = O.SweepData()
+    for _ in range(steps):
+        T.tdvpsweep_(eng, -1j * dt, 2, maxdim=32, cutoff=1e-12, outputlevel=0)
+        O.tdvpsweep(ref, sw, -1j * dt, maxdim=32, cutoff=1e-12)
+    e_neel = -(N - 1) * 0.5 / 4
+    assert abs(eng.swdata.energy[-1] - sw.energy[-1]) < 1e-8
+    assert abs(eng.swdata.energy[-1] - e_neel) < 1e-3
+    assert np.abs(_sz_profile(eng.sysenv.psi) - _sz_profile(ref.psi)).max() < 1e-7
+    assert eng.swdata.maxchi == sw.maxchi
