@@ -154,13 +154,14 @@ def main():
     ap.add_argument("--chi", type=int, default=1024)
     ap.add_argument("--mv-chi", type=int, default=4096)
     ap.add_argument("--mv-iters", type=int, default=5)
-    ap.add_argument("--prec", default="bf16x3", choices=["fp32", "bf16x3"])
+    ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3"])
     ap.add_argument("--cutoff", type=float, default=1e-12)
     ap.add_argument("--krylovdim", type=int, default=5)
     ap.add_argument("--maxiter", type=int, default=2)
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one MI355X); multi-GPU sharding of the "
@@ -171,7 +172,10 @@ def main():
     T.default_context()
 
     # 1. matvec microbenchmark at chi = mv_chi
-    mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", args.prec])
+    mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", "bf16x3"])
+    if args.mv_only:
+        print(json.dumps({"matvec": mv, "chi": args.mv_chi}), flush=True)
+        return
 
     # 2. DMRG: warmup ramp, then timed sweeps in the mixed and FP64 forms
     N, chi = args.N, args.chi

@@ -18,6 +18,7 @@
 #include "tn_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 
 namespace tn {
 namespace {
@@ -55,30 +56,33 @@ template <typename T> struct Vec;
 template <> struct Vec<double> { typedef double2 type; static constexpr int n = 2; };
 template <> struct Vec<float> { typedef float4 type; static constexpr int n = 4; };
 
-template <typename T>
-__device__ __forceinline__ typename Vec<T>::type vzero() {
-  typename Vec<T>::type v;
-  T* p = reinterpret_cast<T*>(&v);
-#pragma unroll
-  for (int i = 0; i < Vec<T>::n; ++i) p[i] = T(0);
-  return v;
-}
+// ---------------------------------------------------------------------------
+// Tile loads through buffer resources.  A resource covers the rows of one
+// tile of a row-major operand, from its first row to the last valid element
+// ((rows_valid - 1) * ld + cols elements); the hardware returns zeros for any
+// offset at or beyond that extent, and vectors past the last column get an
+// offset far beyond it.  Tile edges therefore need no branches, the loads of
+// the next tile stay in flight while the MFMAs run, and no load touches
+// memory outside the operand.  Every row's valid width (cols) is a multiple
+// of the 16-byte vector, so a vector is either fully valid or fully outside.
+// ---------------------------------------------------------------------------
+typedef __amdgpu_buffer_rsrc_t Rsrc;
+constexpr unsigned OOB = 0x80000000u;  // offset past every extent (< 2^31 bytes)
 
-// load VN consecutive elements starting at (r, c) of a row-major matrix with
-// `rows` x `cols` valid entries; out-of-range elements are zero
-template <typename T>
-__device__ __forceinline__ typename Vec<T>::type vload(const T* __restrict__ base, long ld, long r, long c,
-                                                       long rows, long cols) {
-  constexpr int VN = Vec<T>::n;
-  typedef typename Vec<T>::type V;
-  if (r >= rows || c >= cols) return vzero<T>();
-  const T* p = base + r * ld + c;
-  if (c + VN <= cols) return *reinterpret_cast<const V*>(p);
-  V v = vzero<T>();
-  T* q = reinterpret_cast<T*>(&v);
-  for (int i = 0; i < VN; ++i)
-    if (c + i < cols) q[i] = p[i];
-  return v;
+__device__ __forceinline__ Rsrc tile_rsrc(const void* base, long r0, long rows, long tile_rows, long ld, long cols,
+                                          int esz) {
+  const long rv = min(tile_rows, rows - r0);
+  const long bytes = rv > 0 ? ((rv - 1) * ld + cols) * esz : 0;
+  return __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<void*>(reinterpret_cast<const void*>(reinterpret_cast<const char*>(base) + r0 * ld * esz)),
+      (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ uint4 ld16(Rsrc r, unsigned off) {
+  return __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+// byte offset of the vector at (tile row rr, column c) or OOB past the columns
+__device__ __forceinline__ unsigned voff(int rr, long c, long ld, long cols, int esz) {
+  return c < cols ? (unsigned)((rr * ld + c) * esz) : OOB;
 }
 
 template <typename T, int TA, int TB>
@@ -99,7 +103,6 @@ template <typename T, int TA, int TB>
 __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
   using CF = FastCfg<T, TA, TB>;
   constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, VN = CF::VN;
-  typedef typename Vec<T>::type V;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   T* As = reinterpret_cast<T*>(smem);                 // 2 stages
   T* Bs = As + 2 * CF::A_ELEMS;
@@ -115,21 +118,24 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
 
-  V ra[CF::A_VECS], rb[CF::B_VECS];
+  constexpr int ES = (int)sizeof(T);
+  uint4 ra[CF::A_VECS], rb[CF::B_VECS];
+  // tile k0 of A: rows m (TA = 0, k contiguous) or rows k (TA = 1, m contiguous);
+  // a k0 past the end loads zeros (the unconditional prefetch of the last step)
   auto gload = [&](long k0) {
+    const Rsrc ra_r = TA ? tile_rsrc(A, k0, a.K, BK, a.lda, a.M, ES) : tile_rsrc(A, m0, a.M, BM, a.lda, a.K, ES);
+    const Rsrc rb_r = TB ? tile_rsrc(B, n0, a.N, BN, a.ldb, a.K, ES) : tile_rsrc(B, k0, a.K, BK, a.ldb, a.N, ES);
 #pragma unroll
     for (int i = 0; i < CF::A_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::A_COLS / VN), cc = (idx % (CF::A_COLS / VN)) * VN;
-      if (!TA) ra[i] = vload<T>(A, a.lda, m0 + rr, k0 + cc, a.M, a.K);   // A[m][k]
-      else ra[i] = vload<T>(A, a.lda, k0 + rr, m0 + cc, a.K, a.M);       // A^T stored [k][m]
+      ra[i] = ld16(ra_r, TA ? voff(rr, m0 + cc, a.lda, a.M, ES) : voff(rr, k0 + cc, a.lda, a.K, ES));
     }
 #pragma unroll
     for (int i = 0; i < CF::B_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::B_COLS / VN), cc = (idx % (CF::B_COLS / VN)) * VN;
-      if (!TB) rb[i] = vload<T>(B, a.ldb, k0 + rr, n0 + cc, a.K, a.N);   // B[k][n]
-      else rb[i] = vload<T>(B, a.ldb, n0 + rr, k0 + cc, a.N, a.K);       // B^T stored [n][k]
+      rb[i] = ld16(rb_r, TB ? voff(rr, k0 + cc, a.ldb, a.K, ES) : voff(rr, n0 + cc, a.ldb, a.N, ES));
     }
   };
   auto sstore = [&](int st) {
@@ -139,13 +145,13 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
     for (int i = 0; i < CF::A_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::A_COLS / VN), cc = (idx % (CF::A_COLS / VN)) * VN;
-      *reinterpret_cast<V*>(&as[rr * CF::A_LD + cc]) = ra[i];
+      *reinterpret_cast<uint4*>(&as[rr * CF::A_LD + cc]) = ra[i];
     }
 #pragma unroll
     for (int i = 0; i < CF::B_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::B_COLS / VN), cc = (idx % (CF::B_COLS / VN)) * VN;
-      *reinterpret_cast<V*>(&bs[rr * CF::B_LD + cc]) = rb[i];
+      *reinterpret_cast<uint4*>(&bs[rr * CF::B_LD + cc]) = rb[i];
     }
   };
   // element (m, k) of the A stage and (k, n) of the B stage
@@ -164,7 +170,8 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
     sstore(0);
     __syncthreads();
     for (long t = 0; t < nk; ++t) {
-      if (t + 1 < nk) gload((t + 1) * BK);
+      gload((t + 1) * BK);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
       const T* as = As + (t & 1) * CF::A_ELEMS;
       const T* bs = Bs + (t & 1) * CF::B_ELEMS;
 #pragma unroll
@@ -181,7 +188,8 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
           for (int j = 0; j < FN; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
       }
-      if (t + 1 < nk) sstore((t + 1) & 1);
+      __builtin_amdgcn_sched_barrier(0);
+      sstore((t + 1) & 1);
       __syncthreads();
     }
     // v_mfma_f64_16x16x4f64 output: register r of lane -> row lane/16 + 4r, column lane%16
@@ -210,7 +218,8 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
     sstore(0);
     __syncthreads();
     for (long t = 0; t < nk; ++t) {
-      if (t + 1 < nk) gload((t + 1) * BK);
+      gload((t + 1) * BK);
+      __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
       const T* as = As + (t & 1) * CF::A_ELEMS;
       const T* bs = Bs + (t & 1) * CF::B_ELEMS;
 #pragma unroll
@@ -227,7 +236,8 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
           for (int j = 0; j < FN; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
       }
-      if (t + 1 < nk) sstore((t + 1) & 1);
+      __builtin_amdgcn_sched_barrier(0);
+      sstore((t + 1) & 1);
       __syncthreads();
     }
     // register r of lane -> row 8*(r/4) + 4*(lane/32) + r%4, column lane%32
@@ -258,22 +268,20 @@ struct SArgs {
   float* C; long ldc;
 };
 
-constexpr int SBM = 128, SBN = 128, SBK = 64, SLD = SBK + 8;  // bf16 elements per LDS row
-constexpr int S_STAGE = (2 * SBM + 2 * SBN) * SLD;            // hi+lo of A and B, bf16 elements
+// 128 x 128 output tile, 4 wavefronts of 64 x 64 (2 x 2 blocks of 32 x 32);
+// K tile SBK = 32 (two 80 KiB stages: two workgroups per CU) or 64 (one).
+constexpr int SBM = 128, SBN = 128;
+template <int SBK> struct SCfg {
+  static constexpr int SLD = SBK + 8;                      // bf16 per LDS row (16 B pad)
+  static constexpr int STAGE = (2 * SBM + 2 * SBN) * SLD;  // hi+lo of A and B, bf16 elements
+  static constexpr int VPR = SBK / 8;                      // 16-byte vectors per row
+  static constexpr int NV = SBM * VPR / 256;               // per thread, per operand part
+};
 
-__device__ __forceinline__ uint4 bload(const unsigned short* __restrict__ base, long ld, long r, long c, long rows,
-                                       long cols) {
-  if (r >= rows || c >= cols) return make_uint4(0, 0, 0, 0);
-  uint4 v = *reinterpret_cast<const uint4*>(base + r * ld + c);
-  if (c + 8 > cols) {  // zero the k tail of the vector (rows are padded, the load is in bounds)
-    unsigned short* e = reinterpret_cast<unsigned short*>(&v);
-    for (int i = 0; i < 8; ++i)
-      if (c + i >= cols) e[i] = 0;
-  }
-  return v;
-}
-
+template <int SBK>
 __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(SArgs a) {
+  using CF = SCfg<SBK>;
+  constexpr int SLD = CF::SLD, VPR = CF::VPR, NV = CF::NV;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   unsigned short* S = reinterpret_cast<unsigned short*>(smem);
   const int ntm = (int)((a.M + SBM - 1) / SBM), ntn = (int)((a.N + SBN - 1) / SBN);
@@ -282,23 +290,27 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(SArgs a) {
   const long m0 = (long)tm * SBM, n0 = (long)tn * SBN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  constexpr int VPR = SBK / 8;                  // 16-byte vectors per row
-  constexpr int NV = SBM * VPR / 256;           // per thread, per operand part
+  // rows are zero-padded to Kp (a multiple of 8): vectors below Kp are valid
+  const long Kp = (a.K + 7) & ~7L;
+  const Rsrc rah_r = tile_rsrc(a.Ah, m0, a.M, SBM, a.lda, Kp, 2);
+  const Rsrc ral_r = tile_rsrc(a.Al, m0, a.M, SBM, a.lda, Kp, 2);
+  const Rsrc rbh_r = tile_rsrc(a.Bh, n0, a.N, SBN, a.ldb, Kp, 2);
+  const Rsrc rbl_r = tile_rsrc(a.Bl, n0, a.N, SBN, a.ldb, Kp, 2);
   uint4 rah[NV], ral[NV], rbh[NV], rbl[NV];
   auto gload = [&](long k0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / VPR, cc = (idx % VPR) * 8;
-      rah[i] = bload(a.Ah, a.lda, m0 + rr, k0 + cc, a.M, a.K);
-      ral[i] = bload(a.Al, a.lda, m0 + rr, k0 + cc, a.M, a.K);
-      rbh[i] = bload(a.Bh, a.ldb, n0 + rr, k0 + cc, a.N, a.K);
-      rbl[i] = bload(a.Bl, a.ldb, n0 + rr, k0 + cc, a.N, a.K);
+      const unsigned oa = voff(rr, k0 + cc, a.lda, Kp, 2), ob = voff(rr, k0 + cc, a.ldb, Kp, 2);
+      rah[i] = ld16(rah_r, oa);
+      ral[i] = ld16(ral_r, oa);
+      rbh[i] = ld16(rbh_r, ob);
+      rbl[i] = ld16(rbl_r, ob);
     }
   };
   auto sstore = [&](int st) {
-    unsigned short* base = S + st * S_STAGE;
-    unsigned short* Ah = base;
+    unsigned short* Ah = S + st * CF::STAGE;
     unsigned short* Al = Ah + SBM * SLD;
     unsigned short* Bh = Al + SBM * SLD;
     unsigned short* Bl = Bh + SBN * SLD;
@@ -322,8 +334,9 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(SArgs a) {
   sstore(0);
   __syncthreads();
   for (long t = 0; t < nk; ++t) {
-    if (t + 1 < nk) gload((t + 1) * SBK);
-    const unsigned short* Ah = S + (t & 1) * S_STAGE;
+    gload((t + 1) * SBK);  // past the last tile this loads zeros and touches no memory
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch ahead of the MFMAs
+    const unsigned short* Ah = S + (t & 1) * CF::STAGE;
     const unsigned short* Al = Ah + SBM * SLD;
     const unsigned short* Bh = Al + SBM * SLD;
     const unsigned short* Bl = Bh + SBN * SLD;
@@ -352,7 +365,8 @@ __global__ __launch_bounds__(256) void gemm_bf16x3_nt_kernel(SArgs a) {
           acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
         }
     }
-    if (t + 1 < nk) sstore((t + 1) & 1);
+    __builtin_amdgcn_sched_barrier(0);
+    sstore((t + 1) & 1);
     __syncthreads();
   }
 #pragma unroll
@@ -437,11 +451,13 @@ bool gemm_fast_ok(Prec p, const GemmDesc& g) {
   if (g.M < 128 || g.N < 128 || g.K < 16) return false;
   const int vn = (p == Prec::F64) ? 2 : 4;
   const size_t es = (p == Prec::F64) ? 8 : 4;
-  auto al = [&](const void* ptr, long ld, long s) {
-    return ((uintptr_t)ptr % 16 == 0) && (ld % vn == 0) && (s % vn == 0);
+  // 16-byte aligned rows whose valid width is whole vectors; a 128-row tile
+  // must span less than 2^31 bytes (32-bit buffer offsets)
+  auto al = [&](const void* ptr, long ld, long s, long cols) {
+    return ((uintptr_t)ptr % 16 == 0) && (ld % vn == 0) && (s % vn == 0) && (cols % vn == 0) &&
+           (size_t)ld * es * 130 < ((size_t)1 << 31);
   };
-  (void)es;
-  return al(g.A, g.lda, g.sA) && al(g.B, g.ldb, g.sB);
+  return al(g.A, g.lda, g.sA, g.tA ? g.M : g.K) && al(g.B, g.ldb, g.sB, g.tB ? g.K : g.N);
 }
 
 void gemm_fast(tn_ctx* c, Prec p, const GemmDesc& g) {
@@ -480,11 +496,24 @@ void gemm_bf16x3_nt(tn_ctx* c, long M, long N, long K, const unsigned short* Ah,
   if (M == 0 || N == 0) return;
   TN_REQUIRE(lda % 8 == 0 && ldb % 8 == 0 && lda >= ((K + 7) / 8) * 8 && ldb >= ((K + 7) / 8) * 8,
              TN_E_INVALID, "gemm_bf16x3_nt: rows must be padded to 8 bf16");
+  TN_REQUIRE((size_t)std::max(lda, ldb) * 2 * 130 < ((size_t)1 << 31), TN_E_INVALID,
+             "gemm_bf16x3_nt: row stride too large for 32-bit tile offsets");
   SArgs a{M, N, K, Ah, Al, lda, Bh, Bl, ldb, C, ldc};
   const long ntm = (M + SBM - 1) / SBM, ntn = (N + SBN - 1) / SBN;
-  const size_t shm = 2 * (size_t)S_STAGE * sizeof(unsigned short);
-  allow_lds((const void*)gemm_bf16x3_nt_kernel, shm);
-  hipLaunchKernelGGL(gemm_bf16x3_nt_kernel, dim3((unsigned)(ntm * ntn)), dim3(256), shm, c->stream, a);
+  // K tile: 32 by default; TNHIP_BF16_BK=64 selects the one-workgroup-per-CU form
+  static const int bk = [] {
+    const char* e = std::getenv("TNHIP_BF16_BK");
+    return (e && std::atoi(e) == 64) ? 64 : 32;
+  }();
+  if (bk == 64) {
+    const size_t shm = 2 * (size_t)SCfg<64>::STAGE * sizeof(unsigned short);
+    allow_lds((const void*)gemm_bf16x3_nt_kernel<64>, shm);
+    hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<64>, dim3((unsigned)(ntm * ntn)), dim3(256), shm, c->stream, a);
+  } else {
+    const size_t shm = 2 * (size_t)SCfg<32>::STAGE * sizeof(unsigned short);
+    allow_lds((const void*)gemm_bf16x3_nt_kernel<32>, shm);
+    hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<32>, dim3((unsigned)(ntm * ntn)), dim3(256), shm, c->stream, a);
+  }
   TN_HIP(hipGetLastError());
 }
 
