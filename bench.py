@@ -162,11 +162,26 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
+    ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (one MI355X); multi-GPU sharding of the "
               "environments is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_ref:
+        # calibration for the FETCH_SIZE / WRITE_SIZE counters: a plain device
+        # copy of a known size (reads and writes copy_ref MiB per launch)
+        import torch
+        n = args.copy_ref * (1 << 20) // 8
+        src = torch.rand(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_ref_mib": args.copy_ref, "launches": 5}), flush=True)
+        return
 
     import tennetlib_amd as T
     T.default_context()
