@@ -1,0 +1,801 @@
+// Symmetric eigen-decomposition of the density matrix for p > 64, in FP64:
+//
+//  1. Householder tridiagonalisation G = H T H^T on the device, one column
+//     per step: trd_reflect_kernel forms the reflector from row j (every
+//     workgroup redundantly, so no extra launch) and y = tau G22 v;
+//     trd_update_kernel applies G22 -= v w^T + w v^T.  The reflectors are
+//     kept below the diagonal of G.
+//  2. Divide and conquer on T (Cuppen's tearing, bottom-up): leaves of at
+//     most 64 rows are diagonalised by parallel Jacobi in LDS (one
+//     workgroup per leaf); each merge of two neighbouring blocks solves
+//     D + rho z z^T: deflation (small |rho z_i|, or a Givens rotation for
+//     nearly equal d_i) on the host, the secular equation for every
+//     remaining root on the device (one thread per root, bracketed Newton
+//     in tau = lambda - d_origin measured from the nearer pole), the
+//     Gu-Eisenstat z-hat, the eigenvectors z-hat / (d - lambda), and the
+//     block update Q <- Q U as an FP64 MFMA GEMM.  The host keeps the
+//     eigenvalues and only reads z and the roots once per level.
+//  3. Back-transformation Q <- H Q with blocks of 64 reflectors in the
+//     compact WY form I - V T V^T (three GEMMs per block).
+// Accuracy: eigenvalues to ~eps * ||G||, orthonormal eigenvectors; the
+// same absolute accuracy as the block-Jacobi solver used for p <= 64.
+#include "tn_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <numeric>
+
+namespace tn {
+namespace {
+
+constexpr double EPS = 2.220446049250313e-16;
+constexpr int HR = 16;    // rows of y per workgroup in the reflector step
+constexpr int LEAF = 64;  // leaf size of the divide and conquer
+constexpr int NBW = 64;   // reflectors per compact-WY block
+
+// sum over the 256 threads of a workgroup; every thread gets the same value
+__device__ double block_sum(double v, double* red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// ---------------------------------------------------------------------------
+// 1. tridiagonalisation
+// ---------------------------------------------------------------------------
+// Step j (trailing block G22 = G[j+1:, j+1:], m = p - j - 1 rows).  Grid:
+// ceil(m / HR) workgroups, dynamic LDS m doubles.
+__global__ __launch_bounds__(256) void trd_reflect_kernel(double* __restrict__ A, long ld, int p, int j,
+                                                          double* __restrict__ y, double* __restrict__ part,
+                                                          double* __restrict__ dv, double* __restrict__ ev,
+                                                          double* __restrict__ tauv) {
+  extern __shared__ double vs[];
+  __shared__ double red[4];
+  const int m = p - j - 1, tid = threadIdx.x;
+  const double* x = A + (long)j * ld + (j + 1);  // row j = column j (symmetric)
+  double ss = 0.0;
+  for (int c = tid; c < m; c += 256) {
+    const double t = x[c];
+    vs[c] = t;
+    if (c > 0) ss += t * t;
+  }
+  ss = block_sum(ss, red);
+  const double x0 = vs[0];
+  double tau = 0.0, alpha = x0, scale = 0.0;
+  if (ss > 0.0) {
+    const double nrm = sqrt(x0 * x0 + ss);
+    alpha = (x0 >= 0.0) ? -nrm : nrm;
+    tau = (alpha - x0) / alpha;
+    scale = 1.0 / (x0 - alpha);
+  }
+  __syncthreads();
+  for (int c = tid; c < m; c += 256) vs[c] = (c == 0) ? 1.0 : vs[c] * scale;
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    // v below the diagonal of column j (never read again as matrix data)
+    for (int c = tid; c < m; c += 256) A[(long)(j + 1 + c) * ld + j] = vs[c];
+    if (tid == 0) {
+      dv[j] = A[(long)j * ld + j];
+      ev[j] = alpha;
+      tauv[j] = tau;
+    }
+  }
+  if (tau == 0.0) {  // uniform over the workgroup
+    if (tid == 0) part[blockIdx.x] = 0.0;
+    return;
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r0 = blockIdx.x * HR;
+  double dot = 0.0;
+  for (int rr = wave; rr < HR; rr += 4) {
+    const int r = r0 + rr;
+    if (r >= m) break;
+    const double* row = A + (long)(j + 1 + r) * ld + (j + 1);
+    double s = 0.0;
+    for (int c = lane; c < m; c += 64) s += row[c] * vs[c];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) {
+      const double yr = tau * s;
+      y[r] = yr;
+      dot += yr * vs[r];
+    }
+  }
+  dot = block_sum(dot, red);
+  if (tid == 0) part[blockIdx.x] = dot;
+}
+
+// G22 -= v w^T + w v^T with w = y - (tau/2)(y^T v) v; 64 x 64 tiles.
+__global__ __launch_bounds__(256) void trd_update_kernel(double* __restrict__ A, long ld, int p, int j,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ part, int nparts,
+                                                         const double* __restrict__ tauv) {
+  const double tau = tauv[j];
+  if (tau == 0.0) return;
+  __shared__ double red[4];
+  __shared__ double vr[64], wr[64], vc[64], wc[64];
+  const int m = p - j - 1, tid = threadIdx.x;
+  double s = 0.0;
+  for (int i = tid; i < nparts; i += 256) s += part[i];
+  s = block_sum(s, red);
+  const double hs = 0.5 * tau * s;
+  const int tr0 = blockIdx.y * 64, tc0 = blockIdx.x * 64;
+  if (tid < 64) {
+    const int r = tr0 + tid;
+    if (r < m) {
+      const double v = A[(long)(j + 1 + r) * ld + j];
+      vr[tid] = v;
+      wr[tid] = y[r] - hs * v;
+    } else {
+      vr[tid] = 0.0;
+      wr[tid] = 0.0;
+    }
+  } else if (tid < 128) {
+    const int c = tc0 + tid - 64;
+    if (c < m) {
+      const double v = A[(long)(j + 1 + c) * ld + j];
+      vc[tid - 64] = v;
+      wc[tid - 64] = y[c] - hs * v;
+    } else {
+      vc[tid - 64] = 0.0;
+      wc[tid - 64] = 0.0;
+    }
+  }
+  __syncthreads();
+  const int cc = tid & 63;
+  const int c = tc0 + cc;
+  if (c >= m) return;
+  for (int rr = tid >> 6; rr < 64; rr += 4) {
+    const int r = tr0 + rr;
+    if (r >= m) break;
+    double* a = A + (long)(j + 1 + r) * ld + (j + 1 + c);
+    *a -= vr[rr] * wc[cc] + wr[rr] * vc[cc];
+  }
+}
+
+// last 2 x 2 corner and the explicit reflector matrix V (p x p, column j =
+// v_j with zeros above row j + 1; columns j >= p - 2 are zero)
+__global__ void trd_tail_kernel(const double* __restrict__ A, long ld, int p, double* dv, double* ev) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (p >= 2) {
+    dv[p - 2] = A[(long)(p - 2) * ld + (p - 2)];
+    ev[p - 2] = A[(long)(p - 2) * ld + (p - 1)];
+  }
+  dv[p - 1] = A[(long)(p - 1) * ld + (p - 1)];
+}
+__global__ __launch_bounds__(256) void trd_extract_v_kernel(const double* __restrict__ A, long ld, int p,
+                                                            double* __restrict__ V) {
+  const long tot = (long)p * p;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
+    const int r = (int)(i / p), c = (int)(i % p);
+    V[i] = (c < p - 2 && r > c) ? A[(long)r * ld + c] : 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. divide and conquer
+// ---------------------------------------------------------------------------
+// Leaf Jacobi: block [a, a + n) of the torn tridiagonal (n <= 64), one
+// workgroup per leaf; eigenvectors into Q[a:a+n, a:a+n] (columns), values
+// into lam[a:a+n].
+__global__ __launch_bounds__(256) void dc_leaf_kernel(const double* __restrict__ d, const double* __restrict__ e,
+                                                      const int* __restrict__ leaf_a, const int* __restrict__ leaf_n,
+                                                      double tol_abs, double* __restrict__ Q, long ldq,
+                                                      double* __restrict__ lam) {
+  constexpr int P = LEAF, LD = LEAF + 1;
+  __shared__ double Gs[P][LD];
+  __shared__ double Js[P][LD];
+  __shared__ double cs[P / 2], sn[P / 2];
+  __shared__ int pi_[P / 2], pj_[P / 2], fl[P / 2];
+  const int a = leaf_a[blockIdx.x], n = leaf_n[blockIdx.x], tid = threadIdx.x;
+  for (int idx = tid; idx < P * P; idx += 256) {
+    const int r = idx / P, c = idx % P;
+    double g = 0.0;
+    if (r < n && c < n) {
+      if (r == c) g = d[a + r];
+      else if (c == r + 1) g = e[a + r];
+      else if (r == c + 1) g = e[a + c];
+    }
+    Gs[r][c] = g;
+    Js[r][c] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const double tol_rel = 64.0 * EPS;
+  const int pp = tid >> 3, sub = tid & 7;
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int nsweep = 0;
+    for (int step = 0; step < P - 1; ++step) {
+      int f = 0;
+      if (tid < P / 2) {
+        const int pa = tid, pb = P - 1 - tid;
+        const int i = (pa == 0) ? 0 : 1 + ((pa - 1 + step) % (P - 1));
+        const int jj = 1 + ((pb - 1 + step) % (P - 1));
+        const double x = Gs[i][i], z = Gs[jj][jj], g = Gs[i][jj];
+        double c = 1.0, s = 0.0;
+        if (fabs(g) > fmax(tol_rel * sqrt(fabs(x)) * sqrt(fabs(z)), tol_abs)) {
+          const double zeta = (z - x) / (2.0 * g);
+          double t;
+          if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+          else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+          c = 1.0 / sqrt(1.0 + t * t);
+          s = c * t;
+          f = 1;
+        }
+        pi_[tid] = i; pj_[tid] = jj; cs[tid] = c; sn[tid] = s; fl[tid] = f;
+      }
+      const int nrot = __syncthreads_count(f);
+      if (nrot == 0) continue;
+      nsweep += nrot;
+      if (fl[pp]) {
+        const int i = pi_[pp], jj = pj_[pp];
+        const double c = cs[pp], s = sn[pp];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int col = sub * 8 + u;
+          const double gi = Gs[i][col], gj = Gs[jj][col];
+          Gs[i][col] = c * gi - s * gj;
+          Gs[jj][col] = s * gi + c * gj;
+        }
+      }
+      __syncthreads();
+      if (fl[pp]) {
+        const int i = pi_[pp], jj = pj_[pp];
+        const double c = cs[pp], s = sn[pp];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int row = sub * 8 + u;
+          const double gi = Gs[row][i], gj = Gs[row][jj];
+          Gs[row][i] = c * gi - s * gj;
+          Gs[row][jj] = s * gi + c * gj;
+          const double ji = Js[row][i], jv = Js[row][jj];
+          Js[row][i] = c * ji - s * jv;
+          Js[row][jj] = s * ji + c * jv;
+        }
+      }
+      __syncthreads();
+    }
+    if (nsweep == 0) break;
+  }
+  // padding indices never rotate with real ones (zero coupling), so the
+  // first n columns of J restricted to the first n rows are the eigenvectors
+  for (int idx = tid; idx < n * n; idx += 256) {
+    const int r = idx / n, c = idx % n;
+    Q[(long)(a + r) * ldq + (a + c)] = Js[r][c];
+  }
+  if (tid < n) lam[a + tid] = Gs[tid][tid];
+}
+
+// Zero the two off-diagonal sub-blocks of each merge block, so that every
+// column of the block is explicit over all n rows.
+struct MergeDev {
+  int a, n1, n;    // block [a, a + n), split at a + n1
+  int k;           // non-deflated roots
+  int koff;        // offset into the packed root arrays
+  int roff, nrot;  // deflation rotations
+  long uoff;       // offset of this merge's U^T (k x k) in the packed buffer
+  double rho;      // > 0
+};
+
+__global__ __launch_bounds__(256) void dc_zero_off_kernel(double* __restrict__ Q, long ldq,
+                                                          const MergeDev* __restrict__ md) {
+  const MergeDev m = md[blockIdx.y];
+  const long tot = (long)m.n * m.n;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
+    const int r = (int)(i / m.n), c = (int)(i % m.n);
+    if ((r < m.n1) != (c < m.n1)) Q[(long)(m.a + r) * ldq + (m.a + c)] = 0.0;
+  }
+}
+
+struct Rot {
+  int ca, cb;  // block-local columns
+  double c, s;
+};
+// deflation rotations, in order, on the rows of each merge block
+__global__ __launch_bounds__(256) void dc_givens_kernel(double* __restrict__ Q, long ldq,
+                                                        const MergeDev* __restrict__ md,
+                                                        const Rot* __restrict__ rots) {
+  const MergeDev m = md[blockIdx.y];
+  const int r = blockIdx.x * 256 + threadIdx.x;
+  if (r >= m.n || m.nrot == 0) return;
+  double* row = Q + (long)(m.a + r) * ldq + m.a;
+  for (int t = 0; t < m.nrot; ++t) {
+    const Rot R = rots[m.roff + t];
+    const double x = row[R.ca], y = row[R.cb];
+    row[R.ca] = R.c * x + R.s * y;
+    row[R.cb] = R.c * y - R.s * x;
+  }
+}
+
+// secular equation 1 + rho sum_j z_j^2 / (d_j - lambda) = 0 (rho > 0, d
+// increasing) for root i of its merge: returns the origin pole and tau
+__global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restrict__ md,
+                                                         const int* __restrict__ root_merge,
+                                                         const double* __restrict__ dK,
+                                                         const double* __restrict__ zK, int nroots,
+                                                         int* __restrict__ org, double* __restrict__ tau) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= nroots) return;
+  const MergeDev m = md[root_merge[g]];
+  const int i = g - m.koff, k = m.k;
+  const double* dd = dK + m.koff;
+  const double* zz = zK + m.koff;
+  const double rho = m.rho;
+  int o;
+  double lo, hi;
+  if (i < k - 1) {
+    const double mid = 0.5 * (dd[i] + dd[i + 1]);
+    double f = 1.0;
+    for (int q = 0; q < k; ++q) f += rho * zz[q] * zz[q] / (dd[q] - mid);
+    if (f >= 0.0) {
+      o = i; lo = 0.0; hi = mid - dd[i];
+    } else {
+      o = i + 1; lo = mid - dd[i + 1]; hi = 0.0;
+    }
+  } else {
+    o = i;
+    lo = 0.0;
+    double z2 = 0.0;
+    for (int q = 0; q < k; ++q) z2 += zz[q] * zz[q];
+    hi = rho * z2;
+    for (int it = 0; it < 64; ++it) {  // make sure the bracket holds the root
+      double gh = 1.0;
+      for (int q = 0; q < k; ++q) gh += rho * zz[q] * zz[q] / ((dd[q] - dd[o]) - hi);
+      if (gh >= 0.0) break;
+      hi *= 2.0;
+    }
+  }
+  const double dO = dd[o];
+  double x = 0.5 * (lo + hi);
+  for (int it = 0; it < 400; ++it) {
+    double gv = 1.0, dg = 0.0;
+    for (int q = 0; q < k; ++q) {
+      const double den = (dd[q] - dO) - x;
+      const double t = zz[q] / den;
+      gv += rho * zz[q] * t;
+      dg += rho * t * t;
+    }
+    if (gv < 0.0) lo = x;
+    else hi = x;
+    double xn = x - gv / dg;
+    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    const bool done = fabs(xn - x) <= 2.0 * EPS * fabs(xn) || (hi - lo) <= 2.0 * EPS * fmax(fabs(lo), fabs(hi));
+    x = xn;
+    if (done) break;
+  }
+  org[g] = o;
+  tau[g] = x;
+}
+
+// Gu-Eisenstat: z-hat_j so that the computed roots are exact eigenvalues
+__global__ __launch_bounds__(256) void dc_zhat_kernel(const MergeDev* __restrict__ md,
+                                                      const int* __restrict__ root_merge,
+                                                      const double* __restrict__ dK,
+                                                      const double* __restrict__ zK,
+                                                      const int* __restrict__ org, const double* __restrict__ tau,
+                                                      int nroots, double* __restrict__ zh) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= nroots) return;
+  const MergeDev m = md[root_merge[g]];
+  const int j = g - m.koff, k = m.k;
+  const double* dd = dK + m.koff;
+  const int* oo = org + m.koff;
+  const double* tt = tau + m.koff;
+  const double dj = dd[j];
+  // lambda_i - d_j = (d_o(i) - d_j) + tau_i
+  double prod = ((dd[oo[k - 1]] - dj) + tt[k - 1]) / m.rho;
+  for (int i = 0; i < j; ++i) prod *= ((dd[oo[i]] - dj) + tt[i]) / (dd[i] - dj);
+  for (int i = j; i < k - 1; ++i) prod *= ((dd[oo[i]] - dj) + tt[i]) / (dd[i + 1] - dj);
+  zh[g] = copysign(sqrt(fmax(prod, 0.0)), zK[m.koff + j]);
+}
+
+// row i of U^T: u_i[j] = zh_j / (d_j - lambda_i), normalised; one workgroup per root
+__global__ __launch_bounds__(256) void dc_evec_kernel(const MergeDev* __restrict__ md,
+                                                      const int* __restrict__ root_merge,
+                                                      const double* __restrict__ dK,
+                                                      const double* __restrict__ zh,
+                                                      const int* __restrict__ org, const double* __restrict__ tau,
+                                                      double* __restrict__ Ut) {
+  __shared__ double red[4];
+  const int g = blockIdx.x;
+  const MergeDev m = md[root_merge[g]];
+  const int i = g - m.koff, k = m.k;
+  const double* dd = dK + m.koff;
+  const double* zz = zh + m.koff;
+  const double dO = dd[org[g]], t = tau[g];
+  double* u = Ut + m.uoff + (long)i * k;
+  double ss = 0.0;
+  for (int j = threadIdx.x; j < k; j += 256) {
+    const double v = zz[j] / ((dd[j] - dO) - t);
+    u[j] = v;
+    ss += v * v;
+  }
+  ss = block_sum(ss, red);
+  const double inv = 1.0 / sqrt(ss);
+  for (int j = threadIdx.x; j < k; j += 256) u[j] *= inv;
+}
+
+// new column order of a merge block: tmp[:, t] = Q[:, a + src[t]] (n x n, ld n)
+__global__ __launch_bounds__(256) void dc_gather_cols_kernel(const double* __restrict__ Q, long ldq, int a, int n,
+                                                             const int* __restrict__ src, double* __restrict__ tmp) {
+  const long tot = (long)n * n;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
+    const int r = (int)(i / n), t = (int)(i % n);
+    tmp[i] = Q[(long)(a + r) * ldq + (a + src[t])];
+  }
+}
+// Q[a + r][a + t] = tmp[r][t] for t in [t0, n)
+__global__ __launch_bounds__(256) void dc_put_cols_kernel(double* __restrict__ Q, long ldq, int a, int n, int t0,
+                                                          const double* __restrict__ tmp) {
+  const int w = n - t0;
+  const long tot = (long)n * w;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
+    const int r = (int)(i / w), t = t0 + (int)(i % w);
+    Q[(long)(a + r) * ldq + (a + t)] = tmp[(long)r * n + t];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. back-transformation helpers
+// ---------------------------------------------------------------------------
+// T factor of the compact WY form (forward, columnwise) from S = V^T V:
+// T[i][i] = tau_i, T[0:i, i] = T[0:i, 0:i] (-tau_i S[0:i, i]); nb <= 64
+__global__ __launch_bounds__(64) void wy_tfactor_kernel(const double* __restrict__ S, const double* __restrict__ tauv,
+                                                        int j0, int nb, double* __restrict__ T) {
+  __shared__ double Ts[NBW][NBW + 1];
+  __shared__ double sv[NBW];
+  const int l = threadIdx.x;
+  for (int c = 0; c < NBW; ++c) Ts[l][c] = 0.0;
+  __syncthreads();
+  for (int i = 0; i < nb; ++i) {
+    const double ti = tauv[j0 + i];
+    if (l < i) sv[l] = -ti * S[l * NBW + i];
+    __syncthreads();
+    if (l < i) {
+      double acc = 0.0;
+      for (int q = l; q < i; ++q) acc += Ts[l][q] * sv[q];
+      Ts[l][i] = acc;
+    }
+    if (l == i) Ts[i][i] = ti;
+    __syncthreads();
+  }
+  for (int c = 0; c < NBW; ++c) T[l * NBW + c] = Ts[l][c];
+}
+
+unsigned grid_for(long tot, int cap = 4096) { return (unsigned)std::max<long>(1, std::min<long>(cap, (tot + 255) / 256)); }
+
+}  // namespace
+
+// host deflation of one merge (block-local indices); see the file comment
+struct HostMerge {
+  int a, n1, n;
+  double sgn;                  // eigenvalues of the solved problem are sgn * lambda
+  std::vector<int> K;          // non-deflated columns, by increasing solved d
+  std::vector<double> dK, zK;  // solved problem (rho > 0)
+  std::vector<int> defl;       // deflated columns
+  std::vector<double> ddefl;   // their eigenvalues (true sign)
+  std::vector<Rot> rots;
+  double rho;
+};
+
+static void deflate(HostMerge& hm, std::vector<double> D, std::vector<double> z, double beta) {
+  const int n = hm.n;
+  double rho = 2.0 * beta;
+  hm.sgn = 1.0;
+  if (rho < 0.0) {
+    hm.sgn = -1.0;
+    rho = -rho;
+    for (double& v : D) v = -v;
+  }
+  hm.rho = rho;
+  std::vector<int> perm(n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(), [&](int x, int y) { return D[x] < D[y]; });
+  double dmax = 0.0, zmax = 0.0;
+  for (int i = 0; i < n; ++i) {
+    dmax = std::max(dmax, std::fabs(D[i]));
+    zmax = std::max(zmax, std::fabs(z[i]));
+  }
+  const double tol = 8.0 * EPS * std::max(dmax, rho * zmax);
+  std::vector<char> dfl(n, 0);
+  int pj = -1;
+  for (int c : perm) {
+    if (rho * std::fabs(z[c]) <= tol) {
+      dfl[c] = 1;
+      continue;
+    }
+    if (pj < 0) {
+      pj = c;
+      continue;
+    }
+    const double s = z[pj], cc = z[c];
+    const double t = std::hypot(cc, s);
+    const double cth = cc / t, sth = -s / t;
+    if (std::fabs((D[c] - D[pj]) * cth * sth) <= tol) {
+      hm.rots.push_back(Rot{pj, c, cth, sth});
+      const double dp = D[pj] * cth * cth + D[c] * sth * sth;
+      const double dj = D[pj] * sth * sth + D[c] * cth * cth;
+      D[pj] = dp;
+      D[c] = dj;
+      z[c] = t;
+      z[pj] = 0.0;
+      dfl[pj] = 1;
+    }
+    pj = c;
+  }
+  for (int c = 0; c < n; ++c) {
+    if (dfl[c]) {
+      hm.defl.push_back(c);
+      hm.ddefl.push_back(hm.sgn * D[c]);
+    } else {
+      hm.K.push_back(c);
+    }
+  }
+  std::stable_sort(hm.K.begin(), hm.K.end(), [&](int x, int y) { return D[x] < D[y]; });
+  for (int c : hm.K) {
+    hm.dK.push_back(D[c]);
+    hm.zK.push_back(z[c]);
+  }
+}
+
+void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w) {
+  TN_REQUIRE(p >= 3 && p <= 16384, TN_E_INVALID, "sym_eig_dc: size out of range");
+  const int P = (int)p;
+  // ---- 1. tridiagonalisation
+  double* vecs = (double*)c->workspace(WS_DC_VEC, (size_t)(6 * p + 64) * sizeof(double));
+  double* dv = vecs;
+  double* ev = dv + p;
+  double* tauv = ev + p;
+  double* y = tauv + p;
+  double* part = y + p;  // <= ceil(p / HR) entries
+  const size_t lds_bytes = (size_t)p * sizeof(double);
+  if (lds_bytes > 64 * 1024)
+    TN_HIP(hipFuncSetAttribute((const void*)trd_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds_bytes));
+  for (int j = 0; j < P - 2; ++j) {
+    const int m = P - j - 1;
+    const int nparts = (m + HR - 1) / HR;
+    hipLaunchKernelGGL(trd_reflect_kernel, dim3(nparts), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg,
+                       P, j, y, part, dv, ev, tauv);
+    const unsigned tiles = (unsigned)((m + 63) / 64);
+    hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j, y, part,
+                       nparts, tauv);
+  }
+  hipLaunchKernelGGL(trd_tail_kernel, dim3(1), dim3(64), 0, c->stream, G, ldg, P, dv, ev);
+  TN_HIP(hipGetLastError());
+  std::vector<double> d(p), e(p - 1), taus(p);
+  TN_HIP(hipMemcpyAsync(d.data(), dv, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipMemcpyAsync(e.data(), ev, (p - 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipStreamSynchronize(c->stream));
+
+  // ---- 2. divide and conquer on (d, e)
+  double tnorm = 0.0;
+  for (long i = 0; i < p; ++i) tnorm = std::max(tnorm, std::fabs(d[i]));
+  for (long i = 0; i + 1 < p; ++i) tnorm = std::max(tnorm, 2.0 * std::fabs(e[i]));
+  struct Blk { int a, n; };
+  std::vector<Blk> blocks;
+  for (int a = 0; a < P; a += LEAF) blocks.push_back(Blk{a, std::min(LEAF, P - a)});
+  std::vector<double> dt = d;  // torn diagonal
+  for (size_t b = 1; b < blocks.size(); ++b) {
+    const int k = blocks[b].a;
+    dt[k - 1] -= e[k - 1];
+    dt[k] -= e[k - 1];
+  }
+  double* Q = (double*)c->workspace(WS_DC_Q, (size_t)p * p * sizeof(double));
+  const long ldq = p;
+  double* small = (double*)c->workspace(WS_DC_SMALL, (size_t)(8 * p + 64) * sizeof(double));
+  double* d_dt = small;
+  double* d_e = d_dt + p;
+  double* d_lam = d_e + p;
+  int* d_la = (int*)(d_lam + p);
+  int* d_ln = d_la + p;
+  std::vector<int> la, ln;  // alive until the synchronisation below
+  {
+    for (auto& bk : blocks) { la.push_back(bk.a); ln.push_back(bk.n); }
+    TN_HIP(hipMemcpyAsync(d_dt, dt.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(d_e, e.data(), (p - 1) * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(d_la, la.data(), la.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(d_ln, ln.data(), ln.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(dc_leaf_kernel, dim3((unsigned)blocks.size()), dim3(256), 0, c->stream, d_dt, d_e, d_la,
+                       d_ln, EPS * tnorm, Q, ldq, d_lam);
+    TN_HIP(hipGetLastError());
+  }
+  std::vector<double> D(p);  // eigenvalues per block, block-local column order
+  TN_HIP(hipMemcpyAsync(D.data(), d_lam, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipStreamSynchronize(c->stream));
+
+  double* tmp = (double*)c->workspace(WS_DC_TMP, (size_t)p * p * sizeof(double));
+  double* Utb = (double*)c->workspace(WS_DC_UT, (size_t)p * p * sizeof(double));
+  // packed per-level device arrays: dK, zK, zh, tau (p doubles each), org,
+  // root_merge, src (p ints each), merge descriptors and rotations
+  char* lv = (char*)c->workspace(WS_DC_LEVEL, (size_t)p * (4 * 8 + 3 * 4) + (size_t)p * sizeof(MergeDev) +
+                                                  (size_t)p * sizeof(Rot) + 1024);
+  double* d_dK = (double*)lv;
+  double* d_zK = d_dK + p;
+  double* d_zh = d_zK + p;
+  double* d_tau = d_zh + p;
+  int* d_org = (int*)(d_tau + p);
+  int* d_rm = d_org + p;
+  int* d_src = d_rm + p;
+  MergeDev* d_md = (MergeDev*)(((uintptr_t)(d_src + p) + 15) & ~(uintptr_t)15);
+  Rot* d_rot = (Rot*)(((uintptr_t)(d_md + p) + 15) & ~(uintptr_t)15);
+
+  while (blocks.size() > 1) {
+    std::vector<Blk> nb;
+    std::vector<HostMerge> hms;
+    // z of every merge: last row of the left block, first row of the right block
+    std::vector<double> zrow(p);
+    for (size_t i = 0; i + 1 < blocks.size(); i += 2) {
+      const Blk b1 = blocks[i], b2 = blocks[i + 1];
+      TN_HIP(hipMemcpyAsync(zrow.data() + b1.a, Q + (long)(b1.a + b1.n - 1) * ldq + b1.a, b1.n * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+      TN_HIP(hipMemcpyAsync(zrow.data() + b2.a, Q + (long)b2.a * ldq + b2.a, b2.n * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+    }
+    TN_HIP(hipStreamSynchronize(c->stream));
+    std::vector<MergeDev> mds;
+    std::vector<Rot> allrot;
+    std::vector<double> hdK, hzK;
+    std::vector<int> hrm, hsrc;
+    long uoff = 0;
+    for (size_t i = 0; i + 1 < blocks.size(); i += 2) {
+      const Blk b1 = blocks[i], b2 = blocks[i + 1];
+      HostMerge hm;
+      hm.a = b1.a;
+      hm.n1 = b1.n;
+      hm.n = b1.n + b2.n;
+      std::vector<double> Dm(D.begin() + hm.a, D.begin() + hm.a + hm.n);
+      std::vector<double> zm(zrow.begin() + hm.a, zrow.begin() + hm.a + hm.n);
+      const double is2 = 1.0 / std::sqrt(2.0);
+      for (double& v : zm) v *= is2;
+      deflate(hm, Dm, zm, e[b2.a - 1]);
+      MergeDev md;
+      md.a = hm.a; md.n1 = hm.n1; md.n = hm.n;
+      md.k = (int)hm.K.size();
+      md.koff = (int)hdK.size();
+      md.roff = (int)allrot.size();
+      md.nrot = (int)hm.rots.size();
+      md.uoff = uoff;
+      md.rho = hm.rho;
+      uoff += (long)md.k * md.k;
+      for (int t = 0; t < md.k; ++t) hrm.push_back((int)mds.size());
+      hdK.insert(hdK.end(), hm.dK.begin(), hm.dK.end());
+      hzK.insert(hzK.end(), hm.zK.begin(), hm.zK.end());
+      allrot.insert(allrot.end(), hm.rots.begin(), hm.rots.end());
+      mds.push_back(md);
+      hms.push_back(std::move(hm));
+      nb.push_back(Blk{b1.a, b1.n + b2.n});
+    }
+    if (blocks.size() % 2) nb.push_back(blocks.back());
+    const int nroots = (int)hdK.size();
+    const int nm = (int)mds.size();
+    TN_HIP(hipMemcpyAsync(d_md, mds.data(), nm * sizeof(MergeDev), hipMemcpyHostToDevice, c->stream));
+    if (!allrot.empty())
+      TN_HIP(hipMemcpyAsync(d_rot, allrot.data(), allrot.size() * sizeof(Rot), hipMemcpyHostToDevice, c->stream));
+    if (nroots > 0) {
+      TN_HIP(hipMemcpyAsync(d_dK, hdK.data(), nroots * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      TN_HIP(hipMemcpyAsync(d_zK, hzK.data(), nroots * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      TN_HIP(hipMemcpyAsync(d_rm, hrm.data(), nroots * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
+    int maxn = 0;
+    for (auto& m : mds) maxn = std::max(maxn, m.n);
+    // explicit zeros off the two diagonal sub-blocks, then the deflation rotations
+    hipLaunchKernelGGL(dc_zero_off_kernel, dim3(grid_for((long)maxn * maxn, 1024), nm), dim3(256), 0, c->stream, Q,
+                       ldq, d_md);
+    if (!allrot.empty())
+      hipLaunchKernelGGL(dc_givens_kernel, dim3((maxn + 255) / 256, nm), dim3(256), 0, c->stream, Q, ldq, d_md,
+                         d_rot);
+    if (nroots > 0) {
+      const unsigned gr = (unsigned)((nroots + 255) / 256);
+      hipLaunchKernelGGL(dc_secular_kernel, dim3(gr), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zK, nroots, d_org,
+                         d_tau);
+      hipLaunchKernelGGL(dc_zhat_kernel, dim3(gr), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zK, d_org, d_tau,
+                         nroots, d_zh);
+      hipLaunchKernelGGL(dc_evec_kernel, dim3((unsigned)nroots), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zh,
+                         d_org, d_tau, Utb);
+    }
+    TN_HIP(hipGetLastError());
+    // per merge: new column order [roots..., deflated...]; Q_blk[:, :k] = Q_K U
+    std::vector<std::vector<int>> srcs(nm);  // alive until the level's synchronisation
+    for (int mi = 0; mi < nm; ++mi) {
+      const HostMerge& hm = hms[mi];
+      const MergeDev& md = mds[mi];
+      std::vector<int>& src = srcs[mi];
+      src = hm.K;
+      src.insert(src.end(), hm.defl.begin(), hm.defl.end());
+      int* d_s = d_src + hm.a;
+      TN_HIP(hipMemcpyAsync(d_s, src.data(), hm.n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(dc_gather_cols_kernel, dim3(grid_for((long)hm.n * hm.n)), dim3(256), 0, c->stream, Q, ldq,
+                         hm.a, hm.n, d_s, tmp);
+      if (md.k > 0) {
+        GemmDesc g;
+        g.M = hm.n; g.N = md.k; g.K = md.k;
+        g.A = tmp; g.lda = hm.n;
+        g.B = Utb + md.uoff; g.ldb = md.k; g.tB = 1;
+        g.C = Q + (long)hm.a * ldq + hm.a; g.ldc = ldq;
+        gemm(c, Prec::F64, g);
+      }
+      if (md.k < hm.n)
+        hipLaunchKernelGGL(dc_put_cols_kernel, dim3(grid_for((long)hm.n * (hm.n - md.k))), dim3(256), 0, c->stream,
+                           Q, ldq, hm.a, hm.n, md.k, tmp);
+      TN_HIP(hipGetLastError());
+    }
+    // eigenvalues of the merged blocks (host), in the new column order
+    std::vector<int> horg(nroots);
+    std::vector<double> htau(nroots);
+    if (nroots > 0) {
+      TN_HIP(hipMemcpyAsync(horg.data(), d_org, nroots * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      TN_HIP(hipMemcpyAsync(htau.data(), d_tau, nroots * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    TN_HIP(hipStreamSynchronize(c->stream));  // also keeps the host src vectors alive until used
+    for (int mi = 0; mi < nm; ++mi) {
+      const HostMerge& hm = hms[mi];
+      const MergeDev& md = mds[mi];
+      for (int t = 0; t < md.k; ++t)
+        D[hm.a + t] = hm.sgn * (hm.dK[horg[md.koff + t]] + htau[md.koff + t]);
+      for (size_t t = 0; t < hm.defl.size(); ++t) D[hm.a + md.k + t] = hm.ddefl[t];
+    }
+    blocks.swap(nb);
+  }
+
+  // ---- 3. back-transformation Q <- H Q, H = H_0 ... H_{p-3}
+  double* V = tmp;  // explicit reflectors (p x p)
+  hipLaunchKernelGGL(trd_extract_v_kernel, dim3(grid_for(p * p)), dim3(256), 0, c->stream, G, ldg, P, V);
+  double* W = (double*)c->workspace(WS_DC_W, (size_t)(2 * NBW * p + 2 * NBW * NBW) * sizeof(double));
+  double* W2 = W + (long)NBW * p;
+  double* S = W2 + (long)NBW * p;
+  double* Tf = S + NBW * NBW;
+  const int nrefl = P - 2;
+  for (int j0 = ((nrefl - 1) / NBW) * NBW; j0 >= 0; j0 -= NBW) {
+    const int nbk = std::min(NBW, nrefl - j0);
+    const long rows = p - j0 - 1;  // rows touched by the block
+    const double* Vs = V + (long)(j0 + 1) * p + j0;
+    fill_f64(c, S, 0.0, NBW * NBW);
+    {  // S = Vs^T Vs (nbk x nbk, ld NBW)
+      GemmDesc g;
+      g.M = nbk; g.N = nbk; g.K = rows;
+      g.A = Vs; g.lda = p; g.tA = 1;
+      g.B = Vs; g.ldb = p;
+      g.C = S; g.ldc = NBW;
+      gemm(c, Prec::F64, g);
+    }
+    hipLaunchKernelGGL(wy_tfactor_kernel, dim3(1), dim3(64), 0, c->stream, S, tauv, j0, nbk, Tf);
+    {  // W = Vs^T Q[j0+1:, :]
+      GemmDesc g;
+      g.M = nbk; g.N = p; g.K = rows;
+      g.A = Vs; g.lda = p; g.tA = 1;
+      g.B = Q + (long)(j0 + 1) * ldq; g.ldb = ldq;
+      g.C = W; g.ldc = p;
+      gemm(c, Prec::F64, g);
+    }
+    {  // W2 = T W
+      GemmDesc g;
+      g.M = nbk; g.N = p; g.K = nbk;
+      g.A = Tf; g.lda = NBW;
+      g.B = W; g.ldb = p;
+      g.C = W2; g.ldc = p;
+      gemm(c, Prec::F64, g);
+    }
+    {  // Q[j0+1:, :] -= Vs W2
+      GemmDesc g;
+      g.M = rows; g.N = p; g.K = nbk;
+      g.A = Vs; g.lda = p;
+      g.B = W2; g.ldb = p;
+      g.C = Q + (long)(j0 + 1) * ldq; g.ldc = ldq;
+      g.alpha = -1.0; g.beta = 1.0;
+      gemm(c, Prec::F64, g);
+    }
+    TN_HIP(hipGetLastError());
+  }
+  // eigenvectors as rows: Vt[i][r] = Q[r][i]
+  transpose_f64(c, Q, p, p, tmp);
+  TN_HIP(hipMemcpy2DAsync(Vt, ldv * sizeof(double), tmp, p * sizeof(double), p * sizeof(double), p,
+                          hipMemcpyDeviceToDevice, c->stream));
+  w.assign(D.begin(), D.end());
+  TN_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace tn

@@ -520,7 +520,16 @@ void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, Gr
   out.left = left;
   out.cplx = cplx;
   out.Vt = Vt;
-  sym_eig_jacobi(c, G, p, ppad, Vt, ppad, cplx, out.w, &out.sweeps);
+  static const long dc_min = [] {  // smallest p for the divide-and-conquer path
+    const char* e = std::getenv("TNHIP_EIG_DC_MIN");
+    return e ? std::atol(e) : 65L;
+  }();
+  if (!cplx && p >= std::max(3L, dc_min)) {
+    sym_eig_dc(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
+    out.sweeps = 0;
+  } else {
+    sym_eig_jacobi(c, G, p, ppad, Vt, ppad, cplx, out.w, &out.sweeps);
+  }
 }
 
 TruncRule truncate_weights(const std::vector<double>& Pin, long maxdim, long mindim, double cutoff) {

@@ -107,7 +107,8 @@ namespace tn {
 enum WsSlot {
   WS_T1 = 0, WS_T2, WS_PSI32, WS_OUT32, WS_KRY_BASIS, WS_KRY_W, WS_KRY_PART, WS_KRY_COEF,
   WS_SVD_X, WS_SVD_Q, WS_SVD_GRAM, WS_SVD_J, WS_SVD_CNT, WS_SVD_PAIRS, WS_SVD_S, WS_SVD_PERM,
-  WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP, WS_SLOTS
+  WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP,
+  WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_SLOTS
 };
 
 // scoped phase timer (HIP events on the context stream)
@@ -223,6 +224,10 @@ void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, Gr
 // ldg, padding rows/columns zero); Vt (ld = padded size) receives V^T / V^H
 void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bool cplx, std::vector<double>& w,
                     int* sweeps_out);
+// real symmetric p x p (3 <= p <= 16384) by Householder tridiagonalisation and
+// divide and conquer (eig_dc.hip); same contract as sym_eig_jacobi (G is
+// overwritten, Vt rows are the eigenvectors, w their eigenvalues)
+void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w);
 // complex: out[j][k] = conj?(src[perm[k]][j]) * scale[k]  (src rows of complex)
 void gather_rows_T_z(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
                      long kcount, long L, double* out, bool conj);

@@ -1466,56 +1466,49 @@ int tn_svd_trunc(tn_ctx* c, const double* M, long m, long n, long maxdim, long m
     sorted_weights(ge.w, std::min(m, n), order, P);
     const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
     const long k = tr.n;
-    std::vector<double> one(k, 1.0), inv(k);
-    for (long i = 0; i < k; ++i) {
-      const double s = std::sqrt(P[i]);
-      inv[i] = s > 0 ? 1.0 / s : 0.0;
-      if (S) S[i] = s;
-    }
+    std::vector<double> one(k, 1.0);
     int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
     double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
     TN_HIP(hipMemcpyAsync(dperm, order.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
     TN_HIP(hipMemcpyAsync(dsc, one.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const long L = left ? n : m;  // length of the non-orthonormal singular vectors
     double* W = (double*)c->workspace(WS_SVD_GRAM, (size_t)k * std::max(m, n) * sizeof(double));
-    if (left) {
-      gather_rows(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, m, W);  // U_k^T (k x m)
-      if (U) gather_rows_T(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, m, U);
-      if (Vt) {
-        GemmDesc g;
-        g.M = k; g.N = n; g.K = m;
-        g.A = W; g.lda = m;
-        g.B = M; g.ldb = n;
-        g.C = Vt; g.ldc = n;
-        gemm(c, Prec::F64, g);  // U_k^T M = S V^T
+    double* F = (double*)c->workspace(WS_MISC, (size_t)k * L * sizeof(double));
+    // W = top-k eigenvectors as rows; F (k x L) = W M (left) or W M^T (right)
+    // holds s_i times the other side's singular vectors
+    gather_rows(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, left ? m : n, W);
+    if (left && U) gather_rows_T(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, m, U);
+    if (!left && Vt) copy_f64(c, Vt, W, k * n);
+    {
+      GemmDesc g;
+      g.M = k; g.N = L;
+      if (left) {
+        g.K = m; g.A = W; g.lda = m; g.B = M; g.ldb = n;
+      } else {
+        g.K = n; g.A = W; g.lda = n; g.B = M; g.ldb = n; g.tB = 1;
       }
-    } else {
-      gather_rows(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, n, W);  // V_k^T (k x n)
-      if (Vt) copy_f64(c, Vt, W, k * n);
-      if (U) {
-        GemmDesc g;
-        g.M = m; g.N = k; g.K = n;
-        g.A = M; g.lda = n;
-        g.B = W; g.ldb = n; g.tB = 1;
-        g.C = U; g.ldc = k;
-        gemm(c, Prec::F64, g);  // M V_k = U S
-      }
+      g.C = F; g.ldc = L;
+      gemm(c, Prec::F64, g);
     }
-    // divide the non-orthonormal factor by s
+    // s_i = |F_i|: U S V^T = M V V^T (or U U^T M) holds exactly, whatever the
+    // rounding of weights near eps * sum(P)
+    double* dn2 = (double*)c->workspace(WS_SVD_S, (size_t)k * sizeof(double));
+    row_norms2(c, F, k, L, L, dn2);
+    std::vector<double> n2(k), inv(k);
+    TN_HIP(hipMemcpyAsync(n2.data(), dn2, k * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+    for (long i = 0; i < k; ++i) {
+      const double sv = std::sqrt(n2[i]);
+      inv[i] = sv > 0 ? 1.0 / sv : 0.0;
+      if (S) S[i] = sv;
+    }
     double* dinv = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
     TN_HIP(hipMemcpyAsync(dinv, inv.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
     std::vector<int> ident(k);
     std::iota(ident.begin(), ident.end(), 0);
     TN_HIP(hipMemcpyAsync(dperm, ident.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    if (left && Vt) {
-      copy_f64(c, W, Vt, k * n);
-      gather_rows(c, W, n, dperm, dinv, k, n, Vt);
-    }
-    if (!left && U) {
-      // U (m x k) columns scaled: go through the transpose
-      double* T = (double*)c->workspace(WS_MISC, (size_t)m * k * sizeof(double));
-      transpose_f64(c, U, m, k, T);                // T = U^T (k x m)
-      gather_rows_T(c, T, m, dperm, dinv, k, m, U);  // U[j][i] = T[i][j] / s_i
-    }
+    if (left && Vt) gather_rows(c, F, n, dperm, dinv, k, n, Vt);     // Vt[i][j] = F[i][j] / s_i
+    if (!left && U) gather_rows_T(c, F, m, dperm, dinv, k, m, U);   // U[j][i] = F[i][j] / s_i
     TN_HIP(hipStreamSynchronize(c->stream));
     if (nkept) *nkept = k;
     if (truncerr) *truncerr = tr.truncerr;

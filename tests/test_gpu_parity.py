@@ -171,6 +171,44 @@ def test_svd_matches_lapack(T, m, n, grade):
     assert np.abs((U * S) @ V - M).max() < 1e-12 * Sref[0]
 
 
+def _spectrum(case, p, rng):
+    if case == "multiplets":  # exact 3- and 5-fold degeneracies (SU(2)-like)
+        v = np.exp(-0.35 * np.arange(p))
+        for a in range(0, p - 5, 8):
+            v[a:a + 3] = v[a]
+            v[a + 3:a + 8] = v[a + 3]
+        return np.sort(v)[::-1]
+    if case == "lowrank":  # rank 40: a large exactly-zero cluster
+        return np.concatenate([np.exp(-0.2 * np.arange(40)), np.zeros(p - 40)])
+    if case == "graded":  # P from 1 down to 1e-28
+        return np.exp(-np.arange(p) * (32.0 / p))
+    return rng.uniform(0.1, 1.0, p)
+
+
+@pytest.mark.parametrize("case,m,n", [("multiplets", 150, 170), ("lowrank", 160, 150), ("graded", 1000, 1040),
+                                      ("uniform", 65, 80), ("graded", 130, 97)])
+def test_density_eigensolver_spectra(T, case, m, n):
+    """Divide-and-conquer range (Gram size > 64): weights to eps * sum(P),
+    orthonormal Gram-side factor, exact reconstruction, for degenerate,
+    rank-deficient and graded spectra."""
+    print(f"eig {case} {m}x{n}", flush=True)
+    rng = np.random.default_rng(m + n)
+    p = min(m, n)
+    Q1, _ = np.linalg.qr(rng.standard_normal((m, p)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((n, p)))
+    s = _spectrum(case, p, rng)
+    M = (Q1 * s) @ Q2.T
+    U, S, V, _ = _svd_gpu(T, M)
+    P, Pref = np.sort(S ** 2)[::-1], np.sort(s ** 2)[::-1]
+    assert len(S) == p
+    assert np.abs(P - Pref).max() <= 2e-14 * Pref.sum()
+    if m <= n:
+        assert np.abs(U.T @ U - np.eye(p)).max() < 1e-12
+    else:
+        assert np.abs(V @ V.T - np.eye(p)).max() < 1e-12
+    assert np.abs((U * S) @ V - M).max() < 1e-12 * s.max()
+
+
 def test_svd_truncation_rule(T):
     rng = np.random.default_rng(5)
     m, n = 90, 60
