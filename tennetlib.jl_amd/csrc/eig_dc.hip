@@ -6,8 +6,8 @@
 //     trd_update_kernel applies G22 -= v w^T + w v^T.  The reflectors are
 //     kept below the diagonal of G.
 //  2. Divide and conquer on T (Cuppen's tearing, bottom-up): leaves of at
-//     most 64 rows are diagonalised by parallel Jacobi in LDS (one
-//     workgroup per leaf); each merge of two neighbouring blocks solves
+//     most 32 rows are diagonalised by parallel Jacobi in LDS (one
+//     wavefront per leaf); each merge of two neighbouring blocks solves
 //     D + rho z z^T: deflation (small |rho z_i|, or a Givens rotation for
 //     nearly equal d_i) on the host, the secular equation for every
 //     remaining root on the device (one thread per root, bracketed Newton
@@ -25,14 +25,15 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <numeric>
 
 namespace tn {
 namespace {
 
 constexpr double EPS = 2.220446049250313e-16;
-constexpr int HR = 16;    // rows of y per workgroup in the reflector step
-constexpr int LEAF = 64;  // leaf size of the divide and conquer
+constexpr int HR = 4;     // rows of y per workgroup in the reflector step (one per wave)
+constexpr int LEAF = 32;  // leaf size of the divide and conquer
 constexpr int NBW = 64;   // reflectors per compact-WY block
 
 // sum over the 256 threads of a workgroup; every thread gets the same value
@@ -53,7 +54,7 @@ __device__ double block_sum(double v, double* red) {
 __global__ __launch_bounds__(256) void trd_reflect_kernel(double* __restrict__ A, long ld, int p, int j,
                                                           double* __restrict__ y, double* __restrict__ part,
                                                           double* __restrict__ dv, double* __restrict__ ev,
-                                                          double* __restrict__ tauv) {
+                                                          double* __restrict__ tauv, double* __restrict__ vcont) {
   extern __shared__ double vs[];
   __shared__ double red[4];
   const int m = p - j - 1, tid = threadIdx.x;
@@ -78,7 +79,10 @@ __global__ __launch_bounds__(256) void trd_reflect_kernel(double* __restrict__ A
   __syncthreads();
   if (blockIdx.x == 0) {
     // v below the diagonal of column j (never read again as matrix data)
-    for (int c = tid; c < m; c += 256) A[(long)(j + 1 + c) * ld + j] = vs[c];
+    for (int c = tid; c < m; c += 256) {
+      A[(long)(j + 1 + c) * ld + j] = vs[c];
+      if (vcont) vcont[c] = vs[c];
+    }
     if (tid == 0) {
       dv[j] = A[(long)j * ld + j];
       ev[j] = alpha;
@@ -157,6 +161,303 @@ __global__ __launch_bounds__(256) void trd_update_kernel(double* __restrict__ A,
   }
 }
 
+// ---- fused step: one launch per column.  Launch j applies the rank-2
+// update of step j to the rows it owns (rows >= j + 2; row j + 1 is never
+// written again) and, at the same time, forms reflector j + 1 and
+// y_{j+1} = tau_{j+1} G22^{(j+1)} v_{j+1} from the old rows:
+//   new row j+1 = old row j+1 - v_j[j+1] w_j - w_j[j+1] v_j   (every workgroup)
+//   y_{j+1}[r] = tau (old_r . v' - v_j[r] (w_j . v') - w_j[r] (v_j . v'))
+// so each row is read once and written once per column, by one workgroup.
+// Vectors are indexed from row j + 1 (v_j, w_j) and from row j + 2 (v').
+// Dynamic LDS: 3 (p - j - 1) doubles.
+__global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, long ld, int p, int j,
+                                                       const double* __restrict__ vj_in,
+                                                       const double* __restrict__ yj,
+                                                       const double* __restrict__ partj, int npj,
+                                                       double* __restrict__ vn_out, double* __restrict__ yn,
+                                                       double* __restrict__ partn, double* __restrict__ dv,
+                                                       double* __restrict__ ev, double* __restrict__ tauv) {
+  extern __shared__ double lds[];
+  __shared__ double red[4];
+  const int mj = p - j - 1, mn = mj - 1;
+  double* vj = lds;
+  double* wj = vj + mj;
+  double* vn = wj + mj;
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const double tj = tauv[j];
+  double s = 0.0;
+  if (tj != 0.0)
+    for (int i = tid; i < npj; i += 256) s += partj[i];
+  s = block_sum(s, red);
+  const double hs = 0.5 * tj * s;
+  for (int c = tid; c < mj; c += 256) {
+    const double v = vj_in[c];
+    vj[c] = v;
+    wj[c] = (tj != 0.0) ? yj[c] - hs * v : 0.0;
+  }
+  __syncthreads();
+  // new row j+1, columns j+2.. (index t), and its reflector
+  const double* orow = A + (long)(j + 1) * ld + (j + 2);
+  const double v0 = vj[0], w0 = wj[0];
+  double ss = 0.0;
+  for (int t = tid; t < mn; t += 256) {
+    const double x = orow[t] - v0 * wj[t + 1] - w0 * vj[t + 1];
+    vn[t] = x;
+    if (t > 0) ss += x * x;
+  }
+  ss = block_sum(ss, red);
+  const double x0 = vn[0];
+  double tau = 0.0, alpha = x0, scale = 0.0;
+  if (ss > 0.0) {
+    const double nrm = sqrt(x0 * x0 + ss);
+    alpha = (x0 >= 0.0) ? -nrm : nrm;
+    tau = (alpha - x0) / alpha;
+    scale = 1.0 / (x0 - alpha);
+  }
+  __syncthreads();
+  double pa = 0.0, pb = 0.0;
+  for (int t = tid; t < mn; t += 256) {
+    const double v = (t == 0) ? 1.0 : vn[t] * scale;
+    vn[t] = v;
+    pa += wj[t + 1] * v;
+    pb += vj[t + 1] * v;
+  }
+  const double a = block_sum(pa, red);
+  const double b = block_sum(pb, red);
+  if (g == 0) {
+    for (int t = tid; t < mn; t += 256) {
+      A[(long)(j + 2 + t) * ld + (j + 1)] = vn[t];
+      vn_out[t] = vn[t];
+    }
+    if (tid == 0) {
+      dv[j + 1] = A[(long)(j + 1) * ld + (j + 1)] - 2.0 * v0 * w0;
+      ev[j + 1] = alpha;
+      tauv[j + 1] = tau;
+    }
+  }
+  // own rows: update of step j and the rows of y_{j+1}
+  double dot = 0.0;
+  for (int rr = g * 4 + wave; rr < mn; rr += G * 4) {
+    double* row = A + (long)(j + 2 + rr) * ld + (j + 2);
+    const double vr = vj[rr + 1], wr = wj[rr + 1];
+    double acc = 0.0;
+    for (int t = lane; t < mn; t += 64) {
+      const double old = row[t];
+      acc += old * vn[t];
+      if (tj != 0.0) row[t] = old - vr * wj[t + 1] - wr * vj[t + 1];
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+    if (lane == 0) {
+      const double yv = tau * (acc - vr * a - wr * b);
+      yn[rr] = yv;
+      dot += yv * vn[rr];
+    }
+  }
+  dot = block_sum(dot, red);
+  if (tid == 0) partn[g] = dot;
+}
+
+// ---- the same reduction as one cooperative launch (grid co-resident), two
+// grid barriers per column instead of two launches
+//
+// Grid barrier: one counter in device memory, raised with agent-scope
+// release atomics and polled with acquire loads; the spin is bounded and a
+// timeout raises *abort so that every workgroup leaves the kernel.
+__device__ bool grid_barrier(unsigned* ctr, unsigned target, int* abort_flag) {
+  __shared__ int ok;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ok = 1;
+    __threadfence();
+    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    long spins = 0;
+    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1L << 22)) {
+        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        ok = 0;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __threadfence();
+  }
+  __syncthreads();
+  return ok != 0;
+}
+
+// dynamic LDS: 2 m0 doubles (v and w of the current column), m0 = p - 1
+__global__ __launch_bounds__(256) void trd_coop_kernel(double* __restrict__ A, long ld, int p,
+                                                       double* __restrict__ y, double* __restrict__ part,
+                                                       double* __restrict__ dv, double* __restrict__ ev,
+                                                       double* __restrict__ tauv, unsigned* __restrict__ bar,
+                                                       int* __restrict__ abort_flag) {
+  extern __shared__ double lds[];
+  __shared__ double red[4];
+  double* vs = lds;
+  double* ws = lds + (p - 1);
+  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  unsigned target = 0;
+  for (int j = 0; j < p - 2; ++j) {
+    const int m = p - j - 1;
+    // phase A: reflector from row j (every workgroup), rows of y = tau G22 v
+    const double* x = A + (long)j * ld + (j + 1);
+    double ss = 0.0;
+    for (int c = tid; c < m; c += 256) {
+      const double t = x[c];
+      vs[c] = t;
+      if (c > 0) ss += t * t;
+    }
+    ss = block_sum(ss, red);
+    const double x0 = vs[0];
+    double tau = 0.0, alpha = x0, scale = 0.0;
+    if (ss > 0.0) {
+      const double nrm = sqrt(x0 * x0 + ss);
+      alpha = (x0 >= 0.0) ? -nrm : nrm;
+      tau = (alpha - x0) / alpha;
+      scale = 1.0 / (x0 - alpha);
+    }
+    __syncthreads();
+    for (int c = tid; c < m; c += 256) vs[c] = (c == 0) ? 1.0 : vs[c] * scale;
+    __syncthreads();
+    if (g == 0) {
+      for (int c = tid; c < m; c += 256) A[(long)(j + 1 + c) * ld + j] = vs[c];
+      if (tid == 0) {
+        dv[j] = A[(long)j * ld + j];
+        ev[j] = alpha;
+        tauv[j] = tau;
+      }
+    }
+    double dot = 0.0;
+    if (tau != 0.0) {
+      for (int r = g * 4 + wave; r < m; r += G * 4) {
+        const double* row = A + (long)(j + 1 + r) * ld + (j + 1);
+        double s = 0.0;
+        for (int c = lane; c < m; c += 64) s += row[c] * vs[c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+        if (lane == 0) {
+          const double yr = tau * s;
+          y[r] = yr;
+          dot += yr * vs[r];
+        }
+      }
+    }
+    dot = block_sum(dot, red);
+    if (tid == 0) part[g] = dot;
+    target += G;
+    if (!grid_barrier(bar, target, abort_flag)) return;
+    // phase B: G22 -= v w^T + w v^T, w = y - (tau/2)(y^T v) v
+    if (tau != 0.0) {
+      double s = 0.0;
+      for (int i = tid; i < G; i += 256) s += part[i];
+      s = block_sum(s, red);
+      const double hs = 0.5 * tau * s;
+      for (int c = tid; c < m; c += 256) ws[c] = y[c] - hs * vs[c];
+      __syncthreads();
+      const int T = (m + 63) / 64;
+      for (int t = g; t < T * T; t += G) {
+        const int tr0 = (t / T) * 64, tc0 = (t % T) * 64;
+        const int c = tc0 + (tid & 63);
+        if (c < m) {
+          const double vc = vs[c], wc = ws[c];
+          for (int rr = tid >> 6; rr < 64; rr += 4) {
+            const int r = tr0 + rr;
+            if (r >= m) break;
+            double* a = A + (long)(j + 1 + r) * ld + (j + 1 + c);
+            *a -= vs[r] * wc + ws[r] * vc;
+          }
+        }
+      }
+    }
+    target += G;
+    if (!grid_barrier(bar, target, abort_flag)) return;
+  }
+  if (g == 0 && tid == 0) {
+    dv[p - 2] = A[(long)(p - 2) * ld + (p - 2)];
+    ev[p - 2] = A[(long)(p - 2) * ld + (p - 1)];
+    dv[p - 1] = A[(long)(p - 1) * ld + (p - 1)];
+  }
+}
+
+// ---- the last columns in LDS: the trailing block A[j0:, j0:] (m0 = p - j0
+// <= TAIL rows) is reduced by one workgroup without leaving LDS; reflectors,
+// d, e and tau are written exactly as by the per-column kernels.
+constexpr int TAIL = 128;
+__global__ __launch_bounds__(256) void trd_tail_lds_kernel(double* __restrict__ A, long ld, int p, int j0,
+                                                           double* __restrict__ dv, double* __restrict__ ev,
+                                                           double* __restrict__ tauv) {
+  extern __shared__ double lds[];
+  __shared__ double red[4];
+  const int m0 = p - j0, L = m0 + 1;
+  double* M = lds;               // m0 x L
+  double* vs = M + (long)m0 * L; // m0
+  double* ys = vs + m0;          // m0
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int i = tid; i < m0 * m0; i += 256) {
+    const int r = i / m0, c = i % m0;
+    M[r * L + c] = A[(long)(j0 + r) * ld + (j0 + c)];
+  }
+  __syncthreads();
+  for (int jj = 0; jj < m0 - 2; ++jj) {
+    const int j = j0 + jj, m = m0 - jj - 1;
+    const double* x = M + jj * L + (jj + 1);
+    double ss = 0.0;
+    for (int c = tid; c < m; c += 256) {
+      const double t = x[c];
+      vs[c] = t;
+      if (c > 0) ss += t * t;
+    }
+    ss = block_sum(ss, red);
+    const double x0 = vs[0];
+    double tau = 0.0, alpha = x0, scale = 0.0;
+    if (ss > 0.0) {
+      const double nrm = sqrt(x0 * x0 + ss);
+      alpha = (x0 >= 0.0) ? -nrm : nrm;
+      tau = (alpha - x0) / alpha;
+      scale = 1.0 / (x0 - alpha);
+    }
+    __syncthreads();
+    for (int c = tid; c < m; c += 256) {
+      const double v = (c == 0) ? 1.0 : vs[c] * scale;
+      vs[c] = v;
+      A[(long)(j + 1 + c) * ld + j] = v;
+    }
+    if (tid == 0) {
+      dv[j] = M[jj * L + jj];
+      ev[j] = alpha;
+      tauv[j] = tau;
+    }
+    __syncthreads();
+    if (tau == 0.0) continue;  // uniform
+    // y = tau M22 v, one wave per row
+    double dot = 0.0;
+    for (int r = wave; r < m; r += 4) {
+      const double* row = M + (jj + 1 + r) * L + (jj + 1);
+      double sacc = 0.0;
+      for (int c = lane; c < m; c += 64) sacc += row[c] * vs[c];
+      for (int o = 32; o > 0; o >>= 1) sacc += __shfl_down(sacc, o, 64);
+      if (lane == 0) {
+        const double yr = tau * sacc;
+        ys[r] = yr;
+        dot += yr * vs[r];
+      }
+    }
+    dot = block_sum(dot, red);
+    const double hs = 0.5 * tau * dot;
+    for (int c = tid; c < m; c += 256) ys[c] = ys[c] - hs * vs[c];  // w
+    __syncthreads();
+    for (int i = tid; i < m * m; i += 256) {
+      const int r = i / m, c = i % m;
+      M[(jj + 1 + r) * L + (jj + 1 + c)] -= vs[r] * ys[c] + ys[r] * vs[c];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    dv[p - 2] = M[(m0 - 2) * L + (m0 - 2)];
+    ev[p - 2] = M[(m0 - 2) * L + (m0 - 1)];
+    dv[p - 1] = M[(m0 - 1) * L + (m0 - 1)];
+  }
+}
+
 // last 2 x 2 corner and the explicit reflector matrix V (p x p, column j =
 // v_j with zeros above row j + 1; columns j >= p - 2 are zero)
 __global__ void trd_tail_kernel(const double* __restrict__ A, long ld, int p, double* dv, double* ev) {
@@ -179,20 +480,21 @@ __global__ __launch_bounds__(256) void trd_extract_v_kernel(const double* __rest
 // ---------------------------------------------------------------------------
 // 2. divide and conquer
 // ---------------------------------------------------------------------------
-// Leaf Jacobi: block [a, a + n) of the torn tridiagonal (n <= 64), one
-// workgroup per leaf; eigenvectors into Q[a:a+n, a:a+n] (columns), values
-// into lam[a:a+n].
-__global__ __launch_bounds__(256) void dc_leaf_kernel(const double* __restrict__ d, const double* __restrict__ e,
-                                                      const int* __restrict__ leaf_a, const int* __restrict__ leaf_n,
-                                                      double tol_abs, double* __restrict__ Q, long ldq,
-                                                      double* __restrict__ lam) {
+// Leaf Jacobi: block [a, a + n) of the torn tridiagonal (n <= LEAF = 32),
+// one wavefront per leaf; eigenvectors into Q[a:a+n, a:a+n] (columns),
+// values into lam[a:a+n].  Lane t owns rotation pair t/4 of a step and 8 of
+// its columns (rows).
+__global__ __launch_bounds__(64) void dc_leaf_kernel(const double* __restrict__ d, const double* __restrict__ e,
+                                                     const int* __restrict__ leaf_a, const int* __restrict__ leaf_n,
+                                                     double tol_abs, double* __restrict__ Q, long ldq,
+                                                     double* __restrict__ lam) {
   constexpr int P = LEAF, LD = LEAF + 1;
   __shared__ double Gs[P][LD];
   __shared__ double Js[P][LD];
   __shared__ double cs[P / 2], sn[P / 2];
   __shared__ int pi_[P / 2], pj_[P / 2], fl[P / 2];
   const int a = leaf_a[blockIdx.x], n = leaf_n[blockIdx.x], tid = threadIdx.x;
-  for (int idx = tid; idx < P * P; idx += 256) {
+  for (int idx = tid; idx < P * P; idx += 64) {
     const int r = idx / P, c = idx % P;
     double g = 0.0;
     if (r < n && c < n) {
@@ -205,7 +507,7 @@ __global__ __launch_bounds__(256) void dc_leaf_kernel(const double* __restrict__
   }
   __syncthreads();
   const double tol_rel = 64.0 * EPS;
-  const int pp = tid >> 3, sub = tid & 7;
+  const int pp = tid >> 2, sub = tid & 3;
   for (int sweep = 0; sweep < 60; ++sweep) {
     int nsweep = 0;
     for (int step = 0; step < P - 1; ++step) {
@@ -262,7 +564,7 @@ __global__ __launch_bounds__(256) void dc_leaf_kernel(const double* __restrict__
   }
   // padding indices never rotate with real ones (zero coupling), so the
   // first n columns of J restricted to the first n rows are the eigenvectors
-  for (int idx = tid; idx < n * n; idx += 256) {
+  for (int idx = tid; idx < n * n; idx += 64) {
     const int r = idx / n, c = idx % n;
     Q[(long)(a + r) * ldq + (a + c)] = Js[r][c];
   }
@@ -310,8 +612,13 @@ __global__ __launch_bounds__(256) void dc_givens_kernel(double* __restrict__ Q, 
   }
 }
 
-// secular equation 1 + rho sum_j z_j^2 / (d_j - lambda) = 0 (rho > 0, d
-// increasing) for root i of its merge: returns the origin pole and tau
+// secular equation 1 + rho sum_q z_q^2 / (d_q - lambda) = 0 (rho > 0, d
+// increasing) for root i of its merge: returns the origin pole o (the pole of
+// the interval (d_i, d_{i+1}) nearer to the root) and tau = lambda - d_o.
+// Iteration: Li's "middle way" -- the poles on the origin side of the root
+// and those on the far side are each replaced by one rational term at their
+// nearest pole matching value and slope, and the resulting quadratic is
+// solved for tau; bisection of the sign bracket guards every step.
 __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restrict__ md,
                                                          const int* __restrict__ root_merge,
                                                          const double* __restrict__ dK,
@@ -325,19 +632,20 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
   const double* zz = zK + m.koff;
   const double rho = m.rho;
   int o;
-  double lo, hi;
+  double lo, hi, df;  // bracket for tau, far pole relative to the origin (0: none)
   if (i < k - 1) {
     const double mid = 0.5 * (dd[i] + dd[i + 1]);
     double f = 1.0;
     for (int q = 0; q < k; ++q) f += rho * zz[q] * zz[q] / (dd[q] - mid);
     if (f >= 0.0) {
-      o = i; lo = 0.0; hi = mid - dd[i];
+      o = i; lo = 0.0; hi = mid - dd[i]; df = dd[i + 1] - dd[i];
     } else {
-      o = i + 1; lo = mid - dd[i + 1]; hi = 0.0;
+      o = i + 1; lo = mid - dd[i + 1]; hi = 0.0; df = dd[i] - dd[i + 1];
     }
   } else {
     o = i;
     lo = 0.0;
+    df = 0.0;
     double z2 = 0.0;
     for (int q = 0; q < k; ++q) z2 += zz[q] * zz[q];
     hi = rho * z2;
@@ -349,25 +657,55 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
     }
   }
   const double dO = dd[o];
-  double x = 0.5 * (lo + hi);
-  for (int it = 0; it < 400; ++it) {
-    double gv = 1.0, dg = 0.0;
+  // poles q with (q <= i) == (o == i) are on the origin side
+  const bool left_origin = (o == i);
+  double t = 0.5 * (lo + hi);
+  for (int it = 0; it < 100; ++it) {
+    double so = 0.0, dso = 0.0, sf = 0.0, dsf = 0.0;
     for (int q = 0; q < k; ++q) {
-      const double den = (dd[q] - dO) - x;
-      const double t = zz[q] / den;
-      gv += rho * zz[q] * t;
-      dg += rho * t * t;
+      const double den = (dd[q] - dO) - t;
+      const double w = rho * zz[q] * zz[q];
+      const double r = w / den;
+      if ((q <= i) == left_origin) { so += r; dso += r / den; }
+      else { sf += r; dsf += r / den; }
     }
-    if (gv < 0.0) lo = x;
-    else hi = x;
-    double xn = x - gv / dg;
-    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-    const bool done = fabs(xn - x) <= 2.0 * EPS * fabs(xn) || (hi - lo) <= 2.0 * EPS * fmax(fabs(lo), fabs(hi));
-    x = xn;
+    const double gv = 1.0 + so + sf;
+    if (gv < 0.0) lo = t;
+    else hi = t;
+    if (gv == 0.0) break;
+    // model: c + bo / (0 - x) + bf / (df - x), matching value and slope at t
+    const double bo = dso * t * t;
+    const double ao = so - bo / (-t);
+    double bf = 0.0, af = 0.0;
+    if (df != 0.0) {
+      bf = dsf * (df - t) * (df - t);
+      af = sf - bf / (df - t);
+    } else {
+      af = sf;
+    }
+    const double cc = 1.0 + ao + af;
+    // c x^2 - (c df + bo + bf) x + bo df = 0  (df = 0: c x - bo = 0)
+    double tn;
+    if (df == 0.0) {
+      tn = (cc > 0.0) ? bo / cc : 0.5 * (lo + hi);
+    } else {
+      const double A = cc, B = -(cc * df + bo + bf), Cq = bo * df;
+      if (A == 0.0) {
+        tn = -Cq / B;
+      } else {
+        const double disc = fmax(B * B - 4.0 * A * Cq, 0.0);
+        const double qq = -0.5 * (B + copysign(sqrt(disc), B));
+        const double t1 = qq / A, t2 = (qq != 0.0) ? Cq / qq : t1;
+        tn = (t2 > lo && t2 < hi) ? t2 : t1;
+      }
+    }
+    if (!(tn > lo && tn < hi)) tn = 0.5 * (lo + hi);
+    const bool done = fabs(tn - t) <= 4.0 * EPS * fabs(tn) || (hi - lo) <= 4.0 * EPS * fmax(fabs(lo), fabs(hi));
+    t = tn;
     if (done) break;
   }
   org[g] = o;
-  tau[g] = x;
+  tau[g] = t;
 }
 
 // Gu-Eisenstat: z-hat_j so that the computed roots are exact eigenvalues
@@ -418,6 +756,16 @@ __global__ __launch_bounds__(256) void dc_evec_kernel(const MergeDev* __restrict
   for (int j = threadIdx.x; j < k; j += 256) u[j] *= inv;
 }
 
+// z of each merge: last row of its left block, first row of its right block
+__global__ __launch_bounds__(256) void dc_zgather_kernel(const double* __restrict__ Q, long ldq,
+                                                         const int* __restrict__ zl, double* __restrict__ z) {
+  const int a = zl[3 * blockIdx.y], n1 = zl[3 * blockIdx.y + 1], n = zl[3 * blockIdx.y + 2];
+  for (int t = threadIdx.x; t < n; t += 256) {
+    const long r = (t < n1) ? (a + n1 - 1) : (a + n1);
+    z[a + t] = Q[r * ldq + (a + t)];
+  }
+}
+
 // new column order of a merge block: tmp[:, t] = Q[:, a + src[t]] (n x n, ld n)
 __global__ __launch_bounds__(256) void dc_gather_cols_kernel(const double* __restrict__ Q, long ldq, int a, int n,
                                                              const int* __restrict__ src, double* __restrict__ tmp) {
@@ -446,13 +794,17 @@ __global__ __launch_bounds__(256) void dc_put_cols_kernel(double* __restrict__ Q
 __global__ __launch_bounds__(64) void wy_tfactor_kernel(const double* __restrict__ S, const double* __restrict__ tauv,
                                                         int j0, int nb, double* __restrict__ T) {
   __shared__ double Ts[NBW][NBW + 1];
+  __shared__ double Ss[NBW][NBW + 1];
   __shared__ double sv[NBW];
   const int l = threadIdx.x;
-  for (int c = 0; c < NBW; ++c) Ts[l][c] = 0.0;
+  for (int c = 0; c < NBW; ++c) {
+    Ts[l][c] = 0.0;
+    Ss[l][c] = S[l * NBW + c];
+  }
   __syncthreads();
   for (int i = 0; i < nb; ++i) {
     const double ti = tauv[j0 + i];
-    if (l < i) sv[l] = -ti * S[l * NBW + i];
+    if (l < i) sv[l] = -ti * Ss[l][i];
     __syncthreads();
     if (l < i) {
       double acc = 0.0;
@@ -544,33 +896,128 @@ static void deflate(HostMerge& hm, std::vector<double> D, std::vector<double> z,
 void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w) {
   TN_REQUIRE(p >= 3 && p <= 16384, TN_E_INVALID, "sym_eig_dc: size out of range");
   const int P = (int)p;
+  double* Gm = G;
+  // TNHIP_DEBUG_EIG: stage times (each stage synchronised) on stderr
+  static const bool dbg = std::getenv("TNHIP_DEBUG_EIG") != nullptr;
+  auto t_last = std::chrono::steady_clock::now();
+  auto stage = [&](const char* name) {
+    if (!dbg) return;
+    TN_HIP(hipStreamSynchronize(c->stream));
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "[tnhip dc] p=%ld %s %.3f ms\n", p, name,
+                 std::chrono::duration<double, std::milli>(now - t_last).count());
+    t_last = now;
+  };
   // ---- 1. tridiagonalisation
-  double* vecs = (double*)c->workspace(WS_DC_VEC, (size_t)(6 * p + 64) * sizeof(double));
+  double* vecs = (double*)c->workspace(WS_DC_VEC, (size_t)(12 * p + 2048) * sizeof(double));
   double* dv = vecs;
   double* ev = dv + p;
   double* tauv = ev + p;
   double* y = tauv + p;
   double* part = y + p;  // <= ceil(p / HR) entries
-  const size_t lds_bytes = (size_t)p * sizeof(double);
-  if (lds_bytes > 64 * 1024)
-    TN_HIP(hipFuncSetAttribute((const void*)trd_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)lds_bytes));
-  for (int j = 0; j < P - 2; ++j) {
-    const int m = P - j - 1;
-    const int nparts = (m + HR - 1) / HR;
-    hipLaunchKernelGGL(trd_reflect_kernel, dim3(nparts), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg,
-                       P, j, y, part, dv, ev, tauv);
-    const unsigned tiles = (unsigned)((m + 63) / 64);
-    hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j, y, part,
-                       nparts, tauv);
+  // one cooperative launch when the grid fits co-resident, else two launches per column
+  static const bool coop_env = [] {
+    const char* e = std::getenv("TNHIP_TRD_COOP");  // measured slower than per-column launches
+    return e && std::atoi(e) == 1;
+  }();
+  bool done = false;
+  if (coop_env && p <= 8192) {
+    int dev = 0, coop = 0, ncu = 0;
+    TN_HIP(hipGetDevice(&dev));
+    TN_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    TN_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t shm = 2 * (size_t)(p - 1) * sizeof(double);
+    if (shm > 64 * 1024)
+      TN_HIP(hipFuncSetAttribute((const void*)trd_coop_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)shm));
+    int per_cu = 0;
+    TN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)trd_coop_kernel, 256, shm));
+    const long want = std::min<long>(256, std::max<long>(8, (p - 1 + 7) / 8));
+    const long G = std::min<long>(want, (long)per_cu * ncu);
+    if (coop && G >= 1 && per_cu >= 1) {
+      unsigned* bar = (unsigned*)(vecs + 6 * p + 256);  // past the partial sums (<= 2p + 256)
+      int* abort_flag = (int*)(bar + 2);
+      fill_f64(c, (double*)bar, 0.0, 2);
+      int P_ = P;
+      long ld_ = ldg;
+      double* A_ = Gm;
+      void* args[] = {&A_, &ld_, &P_, &y, &part, &dv, &ev, &tauv, &bar, &abort_flag};
+      TN_HIP(hipLaunchCooperativeKernel((const void*)trd_coop_kernel, dim3((unsigned)G), dim3(256), args, shm,
+                                        c->stream));
+      int hab[4] = {0, 0, 0, 0};
+      TN_HIP(hipMemcpyAsync(hab, bar, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      TN_HIP(hipStreamSynchronize(c->stream));
+      TN_REQUIRE(hab[2] == 0, TN_E_HIP, "sym_eig_dc: grid barrier timed out in the tridiagonalisation");
+      done = true;
+    }
   }
-  hipLaunchKernelGGL(trd_tail_kernel, dim3(1), dim3(64), 0, c->stream, G, ldg, P, dv, ev);
+  if (!done) {
+    const size_t lds_bytes = (size_t)p * sizeof(double);
+    if (lds_bytes > 64 * 1024)
+      TN_HIP(hipFuncSetAttribute((const void*)trd_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds_bytes));
+    const int j0 = std::max(0, P - TAIL);  // columns j0.. are reduced in LDS
+    if (j0 >= 1 && P <= 6000) {
+      // fused path: reflector 0, then one launch per column, then the update of step j0 - 1
+      double* vA = vecs + 6 * p + 512;
+      double* vB = vA + p;
+      double* yB = vB + p;
+      double* pB = yB + p;  // >= 512 + p / 4 entries
+      const size_t step_lds_max = 3 * (size_t)(P - 1) * sizeof(double);
+      static size_t step_attr = 0;
+      if (step_lds_max > 64 * 1024 && step_lds_max > step_attr) {
+        TN_HIP(hipFuncSetAttribute((const void*)trd_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)step_lds_max));
+        step_attr = step_lds_max;
+      }
+      const int m = P - 1;
+      const int np0 = (m + HR - 1) / HR;
+      hipLaunchKernelGGL(trd_reflect_kernel, dim3(np0), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg, P,
+                         0, y, part, dv, ev, tauv, vA);
+      double *cv = vA, *cy = y, *cp = part, *nv = vB, *ny = yB, *npart = pB;
+      int cnp = np0;
+      for (int j = 0; j + 2 <= j0; ++j) {
+        const int mj = P - j - 1, mn = mj - 1;
+        const int nwg = std::min(256, (mn + 3) / 4);
+        hipLaunchKernelGGL(trd_step_kernel, dim3(nwg), dim3(256), 3 * (size_t)mj * sizeof(double), c->stream, G, ldg,
+                           P, j, cv, cy, cp, cnp, nv, ny, npart, dv, ev, tauv);
+        std::swap(cv, nv);
+        std::swap(cy, ny);
+        std::swap(cp, npart);
+        cnp = nwg;
+      }
+      const int mlast = P - j0;
+      const unsigned tiles = (unsigned)((mlast + 63) / 64);
+      hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j0 - 1, cy, cp,
+                         cnp, tauv);
+    } else {
+      for (int j = 0; j < j0; ++j) {
+        const int m = P - j - 1;
+        const int nparts = (m + HR - 1) / HR;
+        hipLaunchKernelGGL(trd_reflect_kernel, dim3(nparts), dim3(256), (size_t)m * sizeof(double), c->stream, G,
+                           ldg, P, j, y, part, dv, ev, tauv, (double*)nullptr);
+        const unsigned tiles = (unsigned)((m + 63) / 64);
+        hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j, y, part,
+                           nparts, tauv);
+      }
+    }
+    const int m0 = P - j0;
+    const size_t tshm = ((size_t)m0 * (m0 + 1) + 2 * (size_t)m0) * sizeof(double);
+    static bool tail_attr = false;
+    if (!tail_attr) {
+      TN_HIP(hipFuncSetAttribute((const void*)trd_tail_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(((size_t)TAIL * (TAIL + 1) + 2 * TAIL) * sizeof(double))));
+      tail_attr = true;
+    }
+    hipLaunchKernelGGL(trd_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, j0, dv, ev, tauv);
+  }
   TN_HIP(hipGetLastError());
   std::vector<double> d(p), e(p - 1), taus(p);
   TN_HIP(hipMemcpyAsync(d.data(), dv, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TN_HIP(hipMemcpyAsync(e.data(), ev, (p - 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TN_HIP(hipStreamSynchronize(c->stream));
 
+  stage("tridiagonal");
   // ---- 2. divide and conquer on (d, e)
   double tnorm = 0.0;
   for (long i = 0; i < p; ++i) tnorm = std::max(tnorm, std::fabs(d[i]));
@@ -599,19 +1046,20 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     TN_HIP(hipMemcpyAsync(d_e, e.data(), (p - 1) * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TN_HIP(hipMemcpyAsync(d_la, la.data(), la.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     TN_HIP(hipMemcpyAsync(d_ln, ln.data(), ln.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(dc_leaf_kernel, dim3((unsigned)blocks.size()), dim3(256), 0, c->stream, d_dt, d_e, d_la,
+    hipLaunchKernelGGL(dc_leaf_kernel, dim3((unsigned)blocks.size()), dim3(64), 0, c->stream, d_dt, d_e, d_la,
                        d_ln, EPS * tnorm, Q, ldq, d_lam);
     TN_HIP(hipGetLastError());
   }
   std::vector<double> D(p);  // eigenvalues per block, block-local column order
   TN_HIP(hipMemcpyAsync(D.data(), d_lam, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  stage("leaves");
   TN_HIP(hipStreamSynchronize(c->stream));
 
   double* tmp = (double*)c->workspace(WS_DC_TMP, (size_t)p * p * sizeof(double));
   double* Utb = (double*)c->workspace(WS_DC_UT, (size_t)p * p * sizeof(double));
   // packed per-level device arrays: dK, zK, zh, tau (p doubles each), org,
   // root_merge, src (p ints each), merge descriptors and rotations
-  char* lv = (char*)c->workspace(WS_DC_LEVEL, (size_t)p * (4 * 8 + 3 * 4) + (size_t)p * sizeof(MergeDev) +
+  char* lv = (char*)c->workspace(WS_DC_LEVEL, (size_t)p * (5 * 8 + 6 * 4) + (size_t)p * sizeof(MergeDev) +
                                                   (size_t)p * sizeof(Rot) + 1024);
   double* d_dK = (double*)lv;
   double* d_zK = d_dK + p;
@@ -622,19 +1070,24 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   int* d_src = d_rm + p;
   MergeDev* d_md = (MergeDev*)(((uintptr_t)(d_src + p) + 15) & ~(uintptr_t)15);
   Rot* d_rot = (Rot*)(((uintptr_t)(d_md + p) + 15) & ~(uintptr_t)15);
+  double* d_zrow = (double*)(((uintptr_t)(d_rot + p) + 15) & ~(uintptr_t)15);
+  int* d_zl = (int*)(d_zrow + p);  // 3 ints per merge
 
   while (blocks.size() > 1) {
     std::vector<Blk> nb;
     std::vector<HostMerge> hms;
     // z of every merge: last row of the left block, first row of the right block
     std::vector<double> zrow(p);
+    std::vector<int> zl;  // (a, n1, n) per merge
     for (size_t i = 0; i + 1 < blocks.size(); i += 2) {
-      const Blk b1 = blocks[i], b2 = blocks[i + 1];
-      TN_HIP(hipMemcpyAsync(zrow.data() + b1.a, Q + (long)(b1.a + b1.n - 1) * ldq + b1.a, b1.n * sizeof(double),
-                            hipMemcpyDeviceToHost, c->stream));
-      TN_HIP(hipMemcpyAsync(zrow.data() + b2.a, Q + (long)b2.a * ldq + b2.a, b2.n * sizeof(double),
-                            hipMemcpyDeviceToHost, c->stream));
+      zl.push_back(blocks[i].a);
+      zl.push_back(blocks[i].n);
+      zl.push_back(blocks[i].n + blocks[i + 1].n);
     }
+    const int nzm = (int)zl.size() / 3;
+    TN_HIP(hipMemcpyAsync(d_zl, zl.data(), zl.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(dc_zgather_kernel, dim3(1, nzm), dim3(256), 0, c->stream, Q, ldq, d_zl, d_zrow);
+    TN_HIP(hipMemcpyAsync(zrow.data(), d_zrow, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TN_HIP(hipStreamSynchronize(c->stream));
     std::vector<MergeDev> mds;
     std::vector<Rot> allrot;
@@ -699,15 +1152,17 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     }
     TN_HIP(hipGetLastError());
     // per merge: new column order [roots..., deflated...]; Q_blk[:, :k] = Q_K U
-    std::vector<std::vector<int>> srcs(nm);  // alive until the level's synchronisation
+    std::vector<int> srcall(p, 0);  // new column order of every merge block, alive until the sync below
+    for (int mi = 0; mi < nm; ++mi) {
+      const HostMerge& hm = hms[mi];
+      std::copy(hm.K.begin(), hm.K.end(), srcall.begin() + hm.a);
+      std::copy(hm.defl.begin(), hm.defl.end(), srcall.begin() + hm.a + hm.K.size());
+    }
+    TN_HIP(hipMemcpyAsync(d_src, srcall.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
     for (int mi = 0; mi < nm; ++mi) {
       const HostMerge& hm = hms[mi];
       const MergeDev& md = mds[mi];
-      std::vector<int>& src = srcs[mi];
-      src = hm.K;
-      src.insert(src.end(), hm.defl.begin(), hm.defl.end());
       int* d_s = d_src + hm.a;
-      TN_HIP(hipMemcpyAsync(d_s, src.data(), hm.n * sizeof(int), hipMemcpyHostToDevice, c->stream));
       hipLaunchKernelGGL(dc_gather_cols_kernel, dim3(grid_for((long)hm.n * hm.n)), dim3(256), 0, c->stream, Q, ldq,
                          hm.a, hm.n, d_s, tmp);
       if (md.k > 0) {
@@ -739,6 +1194,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
       for (size_t t = 0; t < hm.defl.size(); ++t) D[hm.a + md.k + t] = hm.ddefl[t];
     }
     blocks.swap(nb);
+    stage("merge level");
   }
 
   // ---- 3. back-transformation Q <- H Q, H = H_0 ... H_{p-3}
@@ -794,6 +1250,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   transpose_f64(c, Q, p, p, tmp);
   TN_HIP(hipMemcpy2DAsync(Vt, ldv * sizeof(double), tmp, p * sizeof(double), p * sizeof(double), p,
                           hipMemcpyDeviceToDevice, c->stream));
+  stage("back-transform");
   w.assign(D.begin(), D.end());
   TN_HIP(hipStreamSynchronize(c->stream));
 }
