@@ -5,7 +5,8 @@
 Workload (Heisenberg chain, open boundaries, d=2, MPO bond dim w=5):
   * warmup: W two-site DMRG sweeps from the Neel state with maxdim ramped
     64, 128, ... up to --chi (cutoff 1e-12, eig_solver defaults krylovdim=5,
-    maxiter=2), mixed precision;
+    maxiter=2), mixed precision; W = 7 plus the timed sweep gives the
+    8-sweep ramp 64..1024 of the Heisenberg N = 100 workload;
   * timed: K sweeps at maxdim --chi in the mixed form (matvec --prec, FP64
     Lanczos / SVD / final Rayleigh quotient) -> `value` (seconds per sweep);
   * the same K sweeps from the same snapshot in the plain FP64 GPU form,
@@ -13,9 +14,8 @@ Workload (Heisenberg chain, open boundaries, d=2, MPO bond dim w=5):
   * matvec microbenchmark at chi=4096 (dense, d=2, w=5) in FP64, FP32 and
     the mixed form: per-GEMM-launch HIP-event timings on the library's own
     stream give the roofline of the dominant kernel (matvec GEMM).
-  * cpu_baseline: the FP64 numpy oracle (OpenBLAS, all host threads) on one
-    bond's work at the final shape (7 matvecs + 1 env update + 1 SVD of the
-    (2 chi x 2 chi) two-site matrix), extrapolated to a sweep of 2(N-1) bonds.
+  * cpu_baseline: the FP64 numpy oracle (OpenBLAS, all host threads) running
+    the same timed sweep from the same snapshot.
 Prints one JSON line.
 """
 
@@ -121,35 +121,33 @@ def sweeps(T, psi, N, chi, nsweeps, prec, cutoff, kd, mi):
     return env, sw, dt, tm
 
 
-def cpu_sample(chi, reps):
-    """FP64 oracle on one bond's work at the final shape (random tensors)."""
+def cpu_sweep(snap, N, chi, cutoff, kd, mi):
+    """The FP64 numpy oracle running the same timed sweep (same snapshot,
+    maxdim, cutoff and Krylov settings) on the host cores."""
     from oracle import tn_oracle as O
     try:
         from threadpoolctl import threadpool_info
         cores = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
     except Exception:
         cores = os.cpu_count() or 1
-    rng = np.random.default_rng(3)
-    d, w = 2, 5
-    W = O.heisenberg_mpo(4)[1]
-    L = rng.standard_normal((chi, w, chi))
-    R = rng.standard_normal((chi, w, chi))
-    A = rng.standard_normal((chi, d, chi)) / chi
-    psi = rng.standard_normal((chi, d, d, chi))
+    env = O.StateEnvs([np.array(a) for a in snap], O.heisenberg_mpo(N))
+    sw = O.SweepData()
     t0 = time.perf_counter()
-    for _ in range(reps):
-        for _ in range(7):
-            O.matvec2(L, W, W, R, psi)
-        O.update_left_env(L, A, W)
-        np.linalg.svd(psi.reshape(chi * d, d * chi), full_matrices=False)
-    return (time.perf_counter() - t0) / reps, cores
+    O.fullsweep(env, sw, maxdim=chi, cutoff=cutoff, krylovdim=kd, maxiter=mi)
+    return time.perf_counter() - t0, cores, sw.energy[-1]
+
+
+# HBM bytes per launch of the FP32 matvec GEMM at chi = 4096 (mean of its two
+# launches), from rocprofv3 FETCH_SIZE / WRITE_SIZE scaled by a plain copy:
+# profiles/matvec_gemm_chi4096_pmc_traffic.md
+GEMM_TRAFFIC = {("fp32", 4096): 17.2e9}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=7)
     ap.add_argument("--N", type=int, default=100)
     ap.add_argument("--chi", type=int, default=1024)
     ap.add_argument("--mv-chi", type=int, default=4096)
@@ -158,7 +156,6 @@ def main():
     ap.add_argument("--cutoff", type=float, default=1e-12)
     ap.add_argument("--krylovdim", type=int, default=5)
     ap.add_argument("--maxiter", type=int, default=2)
-    ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
@@ -229,12 +226,12 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        t_bond, cores = cpu_sample(chi, args.cpu_reps)
-        cpu = {"value": t_bond * 2 * (N - 1), "unit": "s/sweep", "cores": cores, "kind": "port",
-               "sample": f"numpy FP64 oracle, one bond's work at chi={chi} (7 two-site matvecs + 1 env "
-                         f"update + 1 SVD of {2 * chi}x{2 * chi}) x{args.cpu_reps}, random tensors, "
-                         f"extrapolated x{2 * (N - 1)} bonds"}
-        log(f"cpu baseline: {t_bond:.2f} s/bond on {cores} threads")
+        t_cpu, cores, e_cpu = cpu_sweep(snap, N, chi, args.cutoff, args.krylovdim, args.maxiter)
+        cpu = {"value": round(t_cpu, 3), "unit": "s/sweep", "cores": cores, "kind": "port",
+               "sample": f"numpy FP64 oracle (restatement, not the Julia reference), the same sweep: "
+                         f"{2 * (N - 1)} two-site updates from the same snapshot at maxdim {chi}, "
+                         f"cutoff {args.cutoff:g}; energy {e_cpu:.12f}"}
+        log(f"cpu baseline: {t_cpu:.2f} s/sweep on {cores} threads, E={e_cpu:.14f}")
 
     pk = PEAK["fp32"]
     g = mv[args.prec]
@@ -249,7 +246,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.prec} matvec, fp64 Krylov/SVD/refine",
+        "dtype": f"{args.prec} matvec (MFMA), fp64 Krylov / factorisation / energy refinement",
         "data": "synthetic (Neel start, Heisenberg MPO built in-tree)",
         "config": {"workload": f"two-site DMRG sweep, Heisenberg OBC N={N}, d=2, w=5, maxdim {chi} "
                                f"(ramp {ramp}), cutoff {args.cutoff:g}, krylovdim {args.krylovdim}, "
@@ -262,7 +259,8 @@ def main():
         "matvec_chi4096": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv.items()},
         "matvec_pct_fp32_peak": round(100 * mv[args.prec]["matvec_tflops"] / pk, 2),
         "roofline": {"bound": "mfma", "achieved": round(g["gemm_tflops"], 2), "peak": pk, "unit": "TFLOP/s",
-                     "frac": round(g["gemm_tflops"] / pk, 4), "traffic": None,
+                     "frac": round(g["gemm_tflops"] / pk, 4),
+                     "traffic": GEMM_TRAFFIC.get((args.prec, args.mv_chi)),
                      "kernel": f"matvec GEMM ({args.prec}), M=5*{args.mv_chi}, N=4*{args.mv_chi}, "
                                f"K={args.mv_chi}; {F_gemm:.3e} algorithmic flop/launch"},
         "cpu_baseline": cpu,
