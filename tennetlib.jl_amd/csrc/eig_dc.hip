@@ -16,7 +16,8 @@
 //     block update Q <- Q U as an FP64 MFMA GEMM.  The host keeps the
 //     eigenvalues and only reads z and the roots once per level.
 //  3. Back-transformation Q <- H Q with blocks of 64 reflectors in the
-//     compact WY form I - V T V^T (three GEMMs per block).
+//     compact WY form I - V T V^T: three GEMMs per block and a triangular
+//     solve with T^{-1} = diag(1/tau) + striu(V^T V) in place of T.
 // Accuracy: eigenvalues to ~eps * ||G||, orthonormal eigenvectors; the
 // same absolute accuracy as the block-Jacobi solver used for p <= 64.
 #include "tn_internal.h"
@@ -184,6 +185,27 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
   double* wj = vj + mj;
   double* vn = wj + mj;
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // issue the loads of row j+1 and of this wave's first own row up front, so
+  // their latency overlaps the vector set-up (rows up to 6144 / 1024 long)
+  constexpr int OV = 24, RV = 16;
+  const double* orow = A + (long)(j + 1) * ld + (j + 2);
+  double ov[OV];
+#pragma unroll
+  for (int u = 0; u < OV; ++u) {
+    const int t = tid + 256 * u;
+    ov[u] = (t < mn) ? orow[t] : 0.0;
+  }
+  const int rr0 = g * 4 + wave;
+  const bool pre = (mn <= 64 * RV) && (rr0 < mn);
+  double rv[RV];
+  {
+    const double* row0 = A + (long)(j + 2 + (pre ? rr0 : 0)) * ld + (j + 2);
+#pragma unroll
+    for (int u = 0; u < RV; ++u) {
+      const int t = lane + 64 * u;
+      rv[u] = (pre && t < mn) ? row0[t] : 0.0;
+    }
+  }
   const double tj = tauv[j];
   double s = 0.0;
   if (tj != 0.0)
@@ -197,13 +219,16 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
   }
   __syncthreads();
   // new row j+1, columns j+2.. (index t), and its reflector
-  const double* orow = A + (long)(j + 1) * ld + (j + 2);
   const double v0 = vj[0], w0 = wj[0];
   double ss = 0.0;
-  for (int t = tid; t < mn; t += 256) {
-    const double x = orow[t] - v0 * wj[t + 1] - w0 * vj[t + 1];
-    vn[t] = x;
-    if (t > 0) ss += x * x;
+#pragma unroll
+  for (int u = 0; u < OV; ++u) {
+    const int t = tid + 256 * u;
+    if (t < mn) {
+      const double x = ov[u] - v0 * wj[t + 1] - w0 * vj[t + 1];
+      vn[t] = x;
+      if (t > 0) ss += x * x;
+    }
   }
   ss = block_sum(ss, red);
   const double x0 = vn[0];
@@ -237,14 +262,26 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
   }
   // own rows: update of step j and the rows of y_{j+1}
   double dot = 0.0;
-  for (int rr = g * 4 + wave; rr < mn; rr += G * 4) {
+  for (int rr = rr0; rr < mn; rr += G * 4) {
     double* row = A + (long)(j + 2 + rr) * ld + (j + 2);
     const double vr = vj[rr + 1], wr = wj[rr + 1];
     double acc = 0.0;
-    for (int t = lane; t < mn; t += 64) {
-      const double old = row[t];
-      acc += old * vn[t];
-      if (tj != 0.0) row[t] = old - vr * wj[t + 1] - wr * vj[t + 1];
+    if (pre && rr == rr0) {
+#pragma unroll
+      for (int u = 0; u < RV; ++u) {
+        const int t = lane + 64 * u;
+        if (t < mn) {
+          const double old = rv[u];
+          acc += old * vn[t];
+          if (tj != 0.0) row[t] = old - vr * wj[t + 1] - wr * vj[t + 1];
+        }
+      }
+    } else {
+      for (int t = lane; t < mn; t += 64) {
+        const double old = row[t];
+        acc += old * vn[t];
+        if (tj != 0.0) row[t] = old - vr * wj[t + 1] - wr * vj[t + 1];
+      }
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
     if (lane == 0) {
@@ -392,10 +429,8 @@ __global__ __launch_bounds__(256) void trd_tail_lds_kernel(double* __restrict__ 
   double* vs = M + (long)m0 * L; // m0
   double* ys = vs + m0;          // m0
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int i = tid; i < m0 * m0; i += 256) {
-    const int r = i / m0, c = i % m0;
-    M[r * L + c] = A[(long)(j0 + r) * ld + (j0 + c)];
-  }
+  for (int r = wave; r < m0; r += 4)
+    for (int c = lane; c < m0; c += 64) M[r * L + c] = A[(long)(j0 + r) * ld + (j0 + c)];
   __syncthreads();
   for (int jj = 0; jj < m0 - 2; ++jj) {
     const int j = j0 + jj, m = m0 - jj - 1;
@@ -428,26 +463,23 @@ __global__ __launch_bounds__(256) void trd_tail_lds_kernel(double* __restrict__ 
     }
     __syncthreads();
     if (tau == 0.0) continue;  // uniform
-    // y = tau M22 v, one wave per row
+    // y = tau M22 v, one thread per row (odd row stride: no bank conflicts)
     double dot = 0.0;
-    for (int r = wave; r < m; r += 4) {
-      const double* row = M + (jj + 1 + r) * L + (jj + 1);
+    if (tid < m) {
+      const double* row = M + (jj + 1 + tid) * L + (jj + 1);
       double sacc = 0.0;
-      for (int c = lane; c < m; c += 64) sacc += row[c] * vs[c];
-      for (int o = 32; o > 0; o >>= 1) sacc += __shfl_down(sacc, o, 64);
-      if (lane == 0) {
-        const double yr = tau * sacc;
-        ys[r] = yr;
-        dot += yr * vs[r];
-      }
+      for (int c = 0; c < m; ++c) sacc += row[c] * vs[c];
+      const double yr = tau * sacc;
+      ys[tid] = yr;
+      dot = yr * vs[tid];
     }
     dot = block_sum(dot, red);
     const double hs = 0.5 * tau * dot;
     for (int c = tid; c < m; c += 256) ys[c] = ys[c] - hs * vs[c];  // w
     __syncthreads();
-    for (int i = tid; i < m * m; i += 256) {
-      const int r = i / m, c = i % m;
-      M[(jj + 1 + r) * L + (jj + 1 + c)] -= vs[r] * ys[c] + ys[r] * vs[c];
+    for (int c = lane; c < m; c += 64) {
+      const double vc = vs[c], wc = ys[c];
+      for (int r = wave; r < m; r += 4) M[(jj + 1 + r) * L + (jj + 1 + c)] -= vs[r] * wc + ys[r] * vc;
     }
     __syncthreads();
   }
@@ -469,11 +501,11 @@ __global__ void trd_tail_kernel(const double* __restrict__ A, long ld, int p, do
   dv[p - 1] = A[(long)(p - 1) * ld + (p - 1)];
 }
 __global__ __launch_bounds__(256) void trd_extract_v_kernel(const double* __restrict__ A, long ld, int p,
-                                                            double* __restrict__ V) {
+                                                            const double* __restrict__ tauv, double* __restrict__ V) {
   const long tot = (long)p * p;
   for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
     const int r = (int)(i / p), c = (int)(i % p);
-    V[i] = (c < p - 2 && r > c) ? A[(long)r * ld + c] : 0.0;
+    V[i] = (c < p - 2 && r > c && tauv[c] != 0.0) ? A[(long)r * ld + c] : 0.0;
   }
 }
 
@@ -661,18 +693,19 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
   const bool left_origin = (o == i);
   double t = 0.5 * (lo + hi);
   for (int it = 0; it < 100; ++it) {
-    double so = 0.0, dso = 0.0, sf = 0.0, dsf = 0.0;
+    double so = 0.0, dso = 0.0, sf = 0.0, dsf = 0.0, sabs = 0.0;
     for (int q = 0; q < k; ++q) {
-      const double den = (dd[q] - dO) - t;
-      const double w = rho * zz[q] * zz[q];
-      const double r = w / den;
-      if ((q <= i) == left_origin) { so += r; dso += r / den; }
-      else { sf += r; dsf += r / den; }
+      const double inv = 1.0 / ((dd[q] - dO) - t);
+      const double r = rho * zz[q] * zz[q] * inv;
+      sabs += fabs(r);
+      if ((q <= i) == left_origin) { so += r; dso += r * inv; }
+      else { sf += r; dsf += r * inv; }
     }
     const double gv = 1.0 + so + sf;
+    // converged once g is within its own rounding error
+    if (fabs(gv) <= 8.0 * EPS * (1.0 + sabs)) break;
     if (gv < 0.0) lo = t;
     else hi = t;
-    if (gv == 0.0) break;
     // model: c + bo / (0 - x) + bf / (df - x), matching value and slope at t
     const double bo = dso * t * t;
     const double ao = so - bo / (-t);
@@ -789,32 +822,38 @@ __global__ __launch_bounds__(256) void dc_put_cols_kernel(double* __restrict__ Q
 // ---------------------------------------------------------------------------
 // 3. back-transformation helpers
 // ---------------------------------------------------------------------------
-// T factor of the compact WY form (forward, columnwise) from S = V^T V:
-// T[i][i] = tau_i, T[0:i, i] = T[0:i, 0:i] (-tau_i S[0:i, i]); nb <= 64
-__global__ __launch_bounds__(64) void wy_tfactor_kernel(const double* __restrict__ S, const double* __restrict__ tauv,
-                                                        int j0, int nb, double* __restrict__ T) {
-  __shared__ double Ts[NBW][NBW + 1];
+// W <- T W for the compact WY form, through T^{-1} = diag(1/tau) + striu(S),
+// S = V^T V (Joffrain et al.): back-substitution, one thread per column of
+// W (nb x n, ld n).  A reflector with tau = 0 has a zero column in V.
+__global__ __launch_bounds__(256) void wy_tsolve_kernel(const double* __restrict__ S, const double* __restrict__ tauv,
+                                                        int j0, int nb, double* __restrict__ W, long n) {
   __shared__ double Ss[NBW][NBW + 1];
-  __shared__ double sv[NBW];
-  const int l = threadIdx.x;
-  for (int c = 0; c < NBW; ++c) {
-    Ts[l][c] = 0.0;
-    Ss[l][c] = S[l * NBW + c];
+  __shared__ double dinv[NBW];
+  for (int idx = threadIdx.x; idx < NBW * NBW; idx += 256) Ss[idx / NBW][idx % NBW] = S[idx];
+  if (threadIdx.x < NBW) {
+    const double t = (threadIdx.x < nb) ? tauv[j0 + threadIdx.x] : 0.0;
+    dinv[threadIdx.x] = (t != 0.0) ? t : 0.0;  // T_ii = tau_i (T^{-1}_ii = 1/tau_i)
   }
   __syncthreads();
-  for (int i = 0; i < nb; ++i) {
-    const double ti = tauv[j0 + i];
-    if (l < i) sv[l] = -ti * Ss[l][i];
-    __syncthreads();
-    if (l < i) {
-      double acc = 0.0;
-      for (int q = l; q < i; ++q) acc += Ts[l][q] * sv[q];
-      Ts[l][i] = acc;
+  const long col = blockIdx.x * 256L + threadIdx.x;
+  if (col >= n) return;
+  double x[NBW];
+#pragma unroll
+  for (int i = 0; i < NBW; ++i) x[i] = (i < nb) ? W[(long)i * n + col] : 0.0;
+  // solve (D^{-1} + U) y = x from the bottom: y_i = tau_i (x_i - sum_{j>i} S_ij y_j)
+#pragma unroll
+  for (int i = NBW - 1; i >= 0; --i) {
+    if (i < nb) {
+      double acc = x[i];
+#pragma unroll
+      for (int j = i + 1; j < NBW; ++j)
+        if (j < nb) acc -= Ss[i][j] * x[j];
+      x[i] = dinv[i] * acc;
     }
-    if (l == i) Ts[i][i] = ti;
-    __syncthreads();
   }
-  for (int c = 0; c < NBW; ++c) T[l * NBW + c] = Ts[l][c];
+#pragma unroll
+  for (int i = 0; i < NBW; ++i)
+    if (i < nb) W[(long)i * n + col] = x[i];
 }
 
 unsigned grid_for(long tot, int cap = 4096) { return (unsigned)std::max<long>(1, std::min<long>(cap, (tot + 255) / 256)); }
@@ -1012,10 +1051,15 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     hipLaunchKernelGGL(trd_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, j0, dv, ev, tauv);
   }
   TN_HIP(hipGetLastError());
-  std::vector<double> d(p), e(p - 1), taus(p);
-  TN_HIP(hipMemcpyAsync(d.data(), dv, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  TN_HIP(hipMemcpyAsync(e.data(), ev, (p - 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  // pinned host staging: [0, lvb) mirrors the per-level device tables (same
+  // offsets), followed by 8p doubles of scratch; every transfer below goes
+  // through it, one copy per direction per step
+  const size_t lvb = (size_t)p * (5 * 8 + 6 * 4) + (size_t)p * sizeof(MergeDev) + (size_t)p * sizeof(Rot) + 1024;
+  char* hs = (char*)c->host_stage(lvb + 8 * (size_t)p * sizeof(double) + 256);
+  double* hx = (double*)(hs + ((lvb + 255) & ~(size_t)255));
+  TN_HIP(hipMemcpyAsync(hx, dv, (2 * p - 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));  // d, e
   TN_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> d(hx, hx + p), e(hx + p, hx + 2 * p - 1);
 
   stage("tridiagonal");
   // ---- 2. divide and conquer on (d, e)
@@ -1039,28 +1083,30 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   double* d_lam = d_e + p;
   int* d_la = (int*)(d_lam + p);
   int* d_ln = d_la + p;
-  std::vector<int> la, ln;  // alive until the synchronisation below
   {
-    for (auto& bk : blocks) { la.push_back(bk.a); ln.push_back(bk.n); }
-    TN_HIP(hipMemcpyAsync(d_dt, dt.data(), p * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    TN_HIP(hipMemcpyAsync(d_e, e.data(), (p - 1) * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    TN_HIP(hipMemcpyAsync(d_la, la.data(), la.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    TN_HIP(hipMemcpyAsync(d_ln, ln.data(), ln.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // mirror of the `small` layout: dt, e, lam (p doubles each), leaf starts and sizes (p ints each)
+    std::copy(dt.begin(), dt.end(), hx);
+    std::copy(e.begin(), e.end(), hx + p);
+    int* hla = (int*)(hx + 3 * p);
+    int* hln = hla + p;
+    for (size_t b = 0; b < blocks.size(); ++b) { hla[b] = blocks[b].a; hln[b] = blocks[b].n; }
+    TN_HIP(hipMemcpyAsync(small, hx, (size_t)3 * p * sizeof(double) + 2 * (size_t)p * sizeof(int),
+                          hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(dc_leaf_kernel, dim3((unsigned)blocks.size()), dim3(64), 0, c->stream, d_dt, d_e, d_la,
                        d_ln, EPS * tnorm, Q, ldq, d_lam);
     TN_HIP(hipGetLastError());
   }
-  std::vector<double> D(p);  // eigenvalues per block, block-local column order
-  TN_HIP(hipMemcpyAsync(D.data(), d_lam, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipMemcpyAsync(hx + 2 * p, d_lam, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   stage("leaves");
   TN_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> D(hx + 2 * p, hx + 3 * p);  // eigenvalues per block, block-local column order
 
   double* tmp = (double*)c->workspace(WS_DC_TMP, (size_t)p * p * sizeof(double));
   double* Utb = (double*)c->workspace(WS_DC_UT, (size_t)p * p * sizeof(double));
   // packed per-level device arrays: dK, zK, zh, tau (p doubles each), org,
   // root_merge, src (p ints each), merge descriptors and rotations
-  char* lv = (char*)c->workspace(WS_DC_LEVEL, (size_t)p * (5 * 8 + 6 * 4) + (size_t)p * sizeof(MergeDev) +
-                                                  (size_t)p * sizeof(Rot) + 1024);
+  char* lv = (char*)c->workspace(WS_DC_LEVEL, lvb);
+  auto H = [&](const void* dev) { return hs + ((const char*)dev - lv); };  // host mirror of a level table
   double* d_dK = (double*)lv;
   double* d_zK = d_dK + p;
   double* d_zh = d_zK + p;
@@ -1077,18 +1123,18 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     std::vector<Blk> nb;
     std::vector<HostMerge> hms;
     // z of every merge: last row of the left block, first row of the right block
-    std::vector<double> zrow(p);
-    std::vector<int> zl;  // (a, n1, n) per merge
-    for (size_t i = 0; i + 1 < blocks.size(); i += 2) {
-      zl.push_back(blocks[i].a);
-      zl.push_back(blocks[i].n);
-      zl.push_back(blocks[i].n + blocks[i + 1].n);
+    int* hzl = (int*)H(d_zl);  // (a, n1, n) per merge
+    int nzm = 0;
+    for (size_t i = 0; i + 1 < blocks.size(); i += 2, ++nzm) {
+      hzl[3 * nzm] = blocks[i].a;
+      hzl[3 * nzm + 1] = blocks[i].n;
+      hzl[3 * nzm + 2] = blocks[i].n + blocks[i + 1].n;
     }
-    const int nzm = (int)zl.size() / 3;
-    TN_HIP(hipMemcpyAsync(d_zl, zl.data(), zl.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(d_zl, hzl, 3 * nzm * sizeof(int), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(dc_zgather_kernel, dim3(1, nzm), dim3(256), 0, c->stream, Q, ldq, d_zl, d_zrow);
-    TN_HIP(hipMemcpyAsync(zrow.data(), d_zrow, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipMemcpyAsync(H(d_zrow), d_zrow, p * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TN_HIP(hipStreamSynchronize(c->stream));
+    const double* zrow = (const double*)H(d_zrow);
     std::vector<MergeDev> mds;
     std::vector<Rot> allrot;
     std::vector<double> hdK, hzK;
@@ -1101,7 +1147,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
       hm.n1 = b1.n;
       hm.n = b1.n + b2.n;
       std::vector<double> Dm(D.begin() + hm.a, D.begin() + hm.a + hm.n);
-      std::vector<double> zm(zrow.begin() + hm.a, zrow.begin() + hm.a + hm.n);
+      std::vector<double> zm(zrow + hm.a, zrow + hm.a + hm.n);
       const double is2 = 1.0 / std::sqrt(2.0);
       for (double& v : zm) v *= is2;
       deflate(hm, Dm, zm, e[b2.a - 1]);
@@ -1125,14 +1171,21 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     if (blocks.size() % 2) nb.push_back(blocks.back());
     const int nroots = (int)hdK.size();
     const int nm = (int)mds.size();
-    TN_HIP(hipMemcpyAsync(d_md, mds.data(), nm * sizeof(MergeDev), hipMemcpyHostToDevice, c->stream));
-    if (!allrot.empty())
-      TN_HIP(hipMemcpyAsync(d_rot, allrot.data(), allrot.size() * sizeof(Rot), hipMemcpyHostToDevice, c->stream));
-    if (nroots > 0) {
-      TN_HIP(hipMemcpyAsync(d_dK, hdK.data(), nroots * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      TN_HIP(hipMemcpyAsync(d_zK, hzK.data(), nroots * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      TN_HIP(hipMemcpyAsync(d_rm, hrm.data(), nroots * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // all tables of the level in one upload through the pinned mirror
+    std::copy(mds.begin(), mds.end(), (MergeDev*)H(d_md));
+    std::copy(allrot.begin(), allrot.end(), (Rot*)H(d_rot));
+    std::copy(hdK.begin(), hdK.end(), (double*)H(d_dK));
+    std::copy(hzK.begin(), hzK.end(), (double*)H(d_zK));
+    std::copy(hrm.begin(), hrm.end(), (int*)H(d_rm));
+    {
+      int* hsrc = (int*)H(d_src);  // new column order of every merge block: [roots..., deflated...]
+      for (int mi = 0; mi < nm; ++mi) {
+        const HostMerge& hm = hms[mi];
+        std::copy(hm.K.begin(), hm.K.end(), hsrc + hm.a);
+        std::copy(hm.defl.begin(), hm.defl.end(), hsrc + hm.a + hm.K.size());
+      }
     }
+    TN_HIP(hipMemcpyAsync(lv, hs, (size_t)((char*)d_zrow - lv), hipMemcpyHostToDevice, c->stream));
     int maxn = 0;
     for (auto& m : mds) maxn = std::max(maxn, m.n);
     // explicit zeros off the two diagonal sub-blocks, then the deflation rotations
@@ -1151,14 +1204,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
                          d_org, d_tau, Utb);
     }
     TN_HIP(hipGetLastError());
-    // per merge: new column order [roots..., deflated...]; Q_blk[:, :k] = Q_K U
-    std::vector<int> srcall(p, 0);  // new column order of every merge block, alive until the sync below
-    for (int mi = 0; mi < nm; ++mi) {
-      const HostMerge& hm = hms[mi];
-      std::copy(hm.K.begin(), hm.K.end(), srcall.begin() + hm.a);
-      std::copy(hm.defl.begin(), hm.defl.end(), srcall.begin() + hm.a + hm.K.size());
-    }
-    TN_HIP(hipMemcpyAsync(d_src, srcall.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    // per merge: Q_blk[:, :k] = Q_K U, deflated columns after the roots
     for (int mi = 0; mi < nm; ++mi) {
       const HostMerge& hm = hms[mi];
       const MergeDev& md = mds[mi];
@@ -1179,13 +1225,13 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
       TN_HIP(hipGetLastError());
     }
     // eigenvalues of the merged blocks (host), in the new column order
-    std::vector<int> horg(nroots);
-    std::vector<double> htau(nroots);
-    if (nroots > 0) {
-      TN_HIP(hipMemcpyAsync(horg.data(), d_org, nroots * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      TN_HIP(hipMemcpyAsync(htau.data(), d_tau, nroots * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    }
-    TN_HIP(hipStreamSynchronize(c->stream));  // also keeps the host src vectors alive until used
+    // tau (p doubles) and org (p ints) are adjacent: one download
+    if (nroots > 0)
+      TN_HIP(hipMemcpyAsync(H(d_tau), d_tau, (size_t)p * sizeof(double) + (size_t)nroots * sizeof(int),
+                            hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+    const int* horg = (const int*)H(d_org);
+    const double* htau = (const double*)H(d_tau);
     for (int mi = 0; mi < nm; ++mi) {
       const HostMerge& hm = hms[mi];
       const MergeDev& md = mds[mi];
@@ -1199,11 +1245,9 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
 
   // ---- 3. back-transformation Q <- H Q, H = H_0 ... H_{p-3}
   double* V = tmp;  // explicit reflectors (p x p)
-  hipLaunchKernelGGL(trd_extract_v_kernel, dim3(grid_for(p * p)), dim3(256), 0, c->stream, G, ldg, P, V);
-  double* W = (double*)c->workspace(WS_DC_W, (size_t)(2 * NBW * p + 2 * NBW * NBW) * sizeof(double));
-  double* W2 = W + (long)NBW * p;
-  double* S = W2 + (long)NBW * p;
-  double* Tf = S + NBW * NBW;
+  hipLaunchKernelGGL(trd_extract_v_kernel, dim3(grid_for(p * p)), dim3(256), 0, c->stream, G, ldg, P, tauv, V);
+  double* W = (double*)c->workspace(WS_DC_W, (size_t)(NBW * p + NBW * NBW) * sizeof(double));
+  double* S = W + (long)NBW * p;
   const int nrefl = P - 2;
   for (int j0 = ((nrefl - 1) / NBW) * NBW; j0 >= 0; j0 -= NBW) {
     const int nbk = std::min(NBW, nrefl - j0);
@@ -1218,7 +1262,6 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
       g.C = S; g.ldc = NBW;
       gemm(c, Prec::F64, g);
     }
-    hipLaunchKernelGGL(wy_tfactor_kernel, dim3(1), dim3(64), 0, c->stream, S, tauv, j0, nbk, Tf);
     {  // W = Vs^T Q[j0+1:, :]
       GemmDesc g;
       g.M = nbk; g.N = p; g.K = rows;
@@ -1227,19 +1270,14 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
       g.C = W; g.ldc = p;
       gemm(c, Prec::F64, g);
     }
-    {  // W2 = T W
-      GemmDesc g;
-      g.M = nbk; g.N = p; g.K = nbk;
-      g.A = Tf; g.lda = NBW;
-      g.B = W; g.ldb = p;
-      g.C = W2; g.ldc = p;
-      gemm(c, Prec::F64, g);
-    }
-    {  // Q[j0+1:, :] -= Vs W2
+    // W <- T W
+    hipLaunchKernelGGL(wy_tsolve_kernel, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, c->stream, S, tauv, j0, nbk,
+                       W, p);
+    {  // Q[j0+1:, :] -= Vs W
       GemmDesc g;
       g.M = rows; g.N = p; g.K = nbk;
       g.A = Vs; g.lda = p;
-      g.B = W2; g.ldb = p;
+      g.B = W; g.ldb = p;
       g.C = Q + (long)(j0 + 1) * ldq; g.ldc = ldq;
       g.alpha = -1.0; g.beta = 1.0;
       gemm(c, Prec::F64, g);

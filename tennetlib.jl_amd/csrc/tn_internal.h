@@ -80,9 +80,28 @@ struct tn_ctx {
   std::string last_error;
   // named workspace slots (grow-only)
   std::vector<tn::DevBuf> ws;
-  // host-pinned staging for small scalars read back by the Krylov loop
+  // host-pinned staging: `pinned` (PINNED_N doubles) for the Krylov loop's
+  // scalars, `hstage` (grow-only) for the eigensolver's per-level tables
+  static constexpr size_t PINNED_N = 4096;
   double* pinned = nullptr;
-  size_t pinned_n = 0;
+  void* hstage = nullptr;
+  size_t hstage_bytes = 0;
+  void* host_stage(size_t bytes) {
+    if (hstage && bytes <= hstage_bytes) return hstage;
+    if (hstage) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipHostFree(hstage);
+      hstage = nullptr;
+      hstage_bytes = 0;
+    }
+    const size_t nb = bytes > ((size_t)1 << 20) ? bytes : ((size_t)1 << 20);
+    if (hipHostMalloc(&hstage, nb, hipHostMallocDefault) != hipSuccess) {
+      hstage = nullptr;
+      throw tn::Error(TN_E_NOMEM, "hipHostMalloc failed for the staging buffer");
+    }
+    hstage_bytes = nb;
+    return hstage;
+  }
   // per-phase timing
   bool timing = false;
   double phase_ms[tn::PH_COUNT] = {0, 0, 0, 0, 0};

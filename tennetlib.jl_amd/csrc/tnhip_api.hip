@@ -595,7 +595,7 @@ struct Krylov {
   // <x, x>
   double norm2(const double* x) {
     multi_dot(c, x, nd, 1, x, n, cx, coef);
-    double h[2];
+    double* h = c->pinned;
     TN_HIP(hipMemcpyAsync(h, coef, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TN_HIP(hipStreamSynchronize(c->stream));
     return h[0];
@@ -611,7 +611,7 @@ struct Krylov {
     multi_dot(c, base, nd, K, w, n, cx, coef + 128);
     multi_axpy_sub(c, base, nd, K, coef + 128, w, n, cx);
     multi_dot(c, w, nd, 1, w, n, cx, coef + 256);
-    double hb[258];
+    double* hb = c->pinned;
     TN_HIP(hipMemcpyAsync(hb, coef, 258 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TN_HIP(hipStreamSynchronize(c->stream));
     h.resize(K);
@@ -624,13 +624,16 @@ struct Krylov {
   }
   // out = sum_i coefs[i] V_i
   void combine(int K, const std::vector<cplx>& co, double* out) {
-    std::vector<double> hc(cx ? 2 * K : K);
+    // the pinned upload area may still feed an earlier copy: drain the stream first
+    TN_HIP(hipStreamSynchronize(c->stream));
+    double* hc = c->pinned + 2048;
+    const int nc = cx ? 2 * K : K;
     for (int i = 0; i < K; ++i) {
       if (cx) { hc[2 * i] = co[i].real(); hc[2 * i + 1] = co[i].imag(); }
       else hc[i] = co[i].real();
     }
     double* dc = coef + 512;  // [512, 512 + 2K)
-    TN_HIP(hipMemcpyAsync(dc, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(dc, hc, nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
     lincomb(c, base, nd, K, dc, out, n, cx);
   }
 };
@@ -827,7 +830,7 @@ cplx rayleigh(tn_env* e, const double* phi) {
   double* co = (double*)c->workspace(WS_MISC, 64 * sizeof(double));
   multi_dot(c, phi, nd, 1, y, D.n, cx, co);
   multi_dot(c, phi, nd, 1, phi, D.n, cx, co + 4);
-  double h[8];
+  double* h = c->pinned;
   TN_HIP(hipMemcpyAsync(h, co, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   TN_HIP(hipStreamSynchronize(c->stream));
   return cplx(h[0], cx ? h[1] : 0.0) / h[4];
@@ -1134,6 +1137,7 @@ int tn_ctx_create(int device, tn_ctx** out) {
     double one[2] = {1.0, 0.0};
     TN_HIP(hipMemcpy(c->ones_buf.p, one, 16, hipMemcpyHostToDevice));
     c->ones = c->ones_buf.as<double>();
+    TN_HIP(hipHostMalloc((void**)&c->pinned, tn_ctx::PINNED_N * sizeof(double), hipHostMallocDefault));
     const char* tv = std::getenv("TNHIP_TIMING");
     c->timing = tv && tv[0] == '1';
     *out = c;
@@ -1150,6 +1154,8 @@ int tn_ctx_destroy(tn_ctx* c) {
   }
   c->ws.clear();
   c->ones_buf.release();
+  if (c->pinned) (void)hipHostFree(c->pinned);
+  if (c->hstage) (void)hipHostFree(c->hstage);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return TN_OK;
