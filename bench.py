@@ -54,8 +54,10 @@ def neel(N):
     return T.neel_state(N)
 
 
-def matvec_flops(chil, chir, d, w):
-    return 2.0 * chil * chir * d * d * w * (chil + chir) + 4.0 * chil * chir * d ** 3 * w * w
+def matvec_flops(chil, chir, d, wl, wr=None):
+    """Two-site matvec: the two GEMMs (L and R sides) plus the W steps."""
+    wr = wl if wr is None else wr
+    return 2.0 * chil * chir * d * d * (wl * chil + wr * chir) + 4.0 * chil * chir * d ** 3 * wl * wr
 
 
 def phase(ctx, ph):
@@ -64,20 +66,34 @@ def phase(ctx, ph):
     return ms.value, n.value
 
 
-def matvec_microbench(T, chi, iters, precs):
-    """Two-site matvec at (chi, 2, 2, chi) with w=5 environments of shape
-    (chi, 5, chi); L/R come from random (seed 7, U[-1,1]) edge tensors."""
+def wide_mpo4():
+    """Four bulk sites (35..38) of the J1-J2 Ly = 6 x Lx = 12 cylinder MPO
+    (inner bonds w = 20), outer bonds closed with fixed random vectors."""
+    import tennetlib_amd as T
+    Ws = T.j1j2_cylinder_mpo(6, 12, 1.0, 0.5)[35:39]
+    rng = np.random.default_rng(11)
+    Ws[0] = np.tensordot(rng.uniform(-1, 1, Ws[0].shape[0]), Ws[0], axes=([0], [0]))[None]
+    Ws[3] = np.tensordot(Ws[3], rng.uniform(-1, 1, Ws[3].shape[3]), axes=([3], [0]))[..., None]
+    return [np.ascontiguousarray(w) for w in Ws]
+
+
+def matvec_microbench(T, chi, iters, precs, Ws=None):
+    """Two-site matvec at (chi, 2, 2, chi) at position 2 of a 4-site MPO
+    (default: Heisenberg, w = 5); L/R come from random (seed 7, U[-1,1])
+    edge tensors."""
     ctx = T.default_context()
     rng = np.random.default_rng(7)
     N, d = 4, 2
     A = [rng.uniform(-1, 1, (1, d, chi)), rng.uniform(-1, 1, (chi, d, chi)) / chi,
          rng.uniform(-1, 1, (chi, d, chi)) / chi, rng.uniform(-1, 1, (chi, d, 1))]
-    env = T.StateEnvs(A, heis_mpo(N), ctx=ctx)
+    Ws = heis_mpo(N) if Ws is None else Ws
+    wl, wr = Ws[1].shape[0], Ws[2].shape[3]  # environment bonds at position 2
+    env = T.StateEnvs(A, Ws, ctx=ctx)
     T.position_(env, 2)
     psi = T.DeviceArray.from_numpy(ctx, rng.uniform(-1, 1, (chi, d, d, chi)))
     out = T.DeviceArray(ctx, (chi, d, d, chi))
-    F = matvec_flops(chi, chi, d, 5)
-    Fgemm = 2.0 * 5 * d * d * chi ** 3  # one of the two big GEMM launches
+    F = matvec_flops(chi, chi, d, wl, wr)
+    Fgemm = 2.0 * 0.5 * (wl + wr) * d * d * chi ** 3  # mean of the two big GEMM launches
     res = {}
     for prec in precs:
         code = T._lib.PRECISIONS[prec]
@@ -95,7 +111,7 @@ def matvec_microbench(T, chi, iters, precs):
         mv_avg = mv_ms / max(mv_n, 1)
         res[prec] = dict(matvec_ms=mv_avg, wall_ms=wall * 1e3, matvec_tflops=F / (mv_avg * 1e-3) / 1e12,
                          gemm_launch_ms=g_avg, gemm_tflops=Fgemm / (g_avg * 1e-3) / 1e12)
-        log(f"matvec chi={chi} {prec}: {mv_avg:.2f} ms/matvec ({res[prec]['matvec_tflops']:.1f} TF/s), "
+        log(f"matvec chi={chi} w={wl},{wr} {prec}: {mv_avg:.2f} ms/matvec ({res[prec]['matvec_tflops']:.1f} TF/s), "
             f"GEMM launch {g_avg:.2f} ms ({res[prec]['gemm_tflops']:.1f} TF/s)")
     psi.free()
     out.free()
@@ -159,6 +175,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
+    ap.add_argument("--mv-wide", type=int, default=1, help="also time the w = 20 (J1-J2 cylinder) matvec")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -185,8 +202,13 @@ def main():
 
     # 1. matvec microbenchmark at chi = mv_chi
     mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", "bf16x3"])
+    mv_wide = None
+    if args.mv_wide:
+        # the wide-MPO matvec (J1-J2 cylinder bulk, w = 20) at the same chi
+        mv_wide, F_wide, _ = matvec_microbench(T, args.mv_chi, max(2, args.mv_iters // 2), ["fp64", "fp32"],
+                                               Ws=wide_mpo4())
     if args.mv_only:
-        print(json.dumps({"matvec": mv, "chi": args.mv_chi}), flush=True)
+        print(json.dumps({"matvec": mv, "matvec_wide": mv_wide, "chi": args.mv_chi}), flush=True)
         return
 
     # 2. DMRG: warmup ramp, then timed sweeps in the mixed and FP64 forms
@@ -258,6 +280,8 @@ def main():
         "energy_mixed": E_m,
         "matvec_chi4096": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv.items()},
         "matvec_pct_fp32_peak": round(100 * mv[args.prec]["matvec_tflops"] / pk, 2),
+        "matvec_chi4096_w20": ({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv_wide.items()}
+                               if mv_wide else None),
         "roofline": {"bound": "mfma", "achieved": round(g["gemm_tflops"], 2), "peak": pk, "unit": "TFLOP/s",
                      "frac": round(g["gemm_tflops"] / pk, 4),
                      "traffic": GEMM_TRAFFIC.get((args.prec, args.mv_chi)),

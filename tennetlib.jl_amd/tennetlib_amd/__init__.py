@@ -10,4 +10,5 @@ from .core import (Context, DeviceArray, DMRGParams, StateEnvs, SweepData, TDVPE
                    default_context, dmrg, dmrg2, dmrg_, eig_solver, exp_solver, fullsweep_, getpsi,
                    make_dmrg_params, nsite, orthogonalize_, position_, product, set_nsite_, tdvpsweep_,
                    update_position_)
-from .models import heisenberg_mpo, neel_state, product_state, xxz_mpo  # noqa: F401
+from .models import (cylinder_couplings, heisenberg_mpo, j1j2_cylinder_mpo, neel_state, product_state,  # noqa: F401
+                     spin_coupling_mpo, xxz_mpo)

@@ -56,3 +56,115 @@ def neel_state(N: int, d: int = 2, dtype=np.float64) -> List[np.ndarray]:
         v[k % 2] = 1.0
         vs.append(v)
     return product_state(vs, dtype)
+
+
+def spin_coupling_mpo(N: int, couplings, h: Optional[Sequence[float]] = None, compress: bool = True,
+                      tol: float = 1e-13) -> List[np.ndarray]:
+    """H = sum_{(i,j)} J_ij S_i . S_j + sum_i h_i Sz_i for arbitrary pairs
+    (0-based sites, i != j), as an MPO.
+
+    Finite-state machine with one channel per (operator, source site) for
+    every site that still has partners beyond the bond: states are
+    0 = done, 1..3|A_k| = (S+, S-, Sz) emitted at a source i <= k, last =
+    start; W = 2 + 3|A_k|.  This plays the role of the reference's OpSum ->
+    MPO input (e.g. its J1-J2 cylinder models).  With `compress`, each bond
+    is then reduced by an SVD sweep pair (relative tolerance `tol` on the
+    singular values of the bond matrices), which keeps H exact to
+    ~tol * |H| and gives the minimal bond dimension of the sum."""
+    I = np.eye(2)
+    Sz = np.diag([0.5, -0.5])
+    Sp = np.array([[0.0, 1.0], [0.0, 0.0]])
+    Sm = Sp.T.copy()
+    J = {}
+    for (i, j), v in dict(couplings).items():
+        if i == j:
+            raise ValueError("self coupling")
+        a, b = (i, j) if i < j else (j, i)
+        J[(a, b)] = J.get((a, b), 0.0) + float(v)
+    h = np.zeros(N) if h is None else np.asarray(h, dtype=np.float64)
+    partners = [[] for _ in range(N)]  # partners j > i
+    for (a, b), v in J.items():
+        if v != 0.0:
+            partners[a].append(b)
+    # active sources across bond k (between k and k+1): i <= k with a partner > k
+    active = [[i for i in range(k + 1) if any(j > k for j in partners[i])] for k in range(N)]
+    # emitted operator at the source and the partner's operator with its weight
+    chans = [(Sp, Sm, 0.5), (Sm, Sp, 0.5), (Sz, Sz, 1.0)]
+    Ws = []
+    for k in range(N):
+        left = active[k - 1] if k > 0 else []
+        right = active[k] if k < N - 1 else []
+        wl, wr = 2 + 3 * len(left), 2 + 3 * len(right)
+        W = np.zeros((wl, 2, 2, wr))
+        sl, sr = wl - 1, wr - 1  # start states
+        W[0, :, :, 0] = I
+        W[sl, :, :, sr] = I
+        W[sl, :, :, 0] = h[k] * Sz
+        ridx = {i: n for n, i in enumerate(right)}
+        lidx = {i: n for n, i in enumerate(left)}
+        if k in ridx:
+            for c, (src, _, _) in enumerate(chans):
+                W[sl, :, :, 1 + 3 * ridx[k] + c] = src
+        for i, n in lidx.items():
+            for c, (_, dst, wt) in enumerate(chans):
+                if i in ridx:
+                    W[1 + 3 * n + c, :, :, 1 + 3 * ridx[i] + c] = I
+                v = J.get((i, k), 0.0)
+                if v != 0.0:
+                    W[1 + 3 * n + c, :, :, 0] = wt * v * dst
+        if k == 0:
+            W = W[sl:sl + 1]
+        if k == N - 1:
+            W = W[..., 0:1]
+        Ws.append(np.ascontiguousarray(W))
+    if compress and N > 1:
+        Ws = _compress_mpo(Ws, tol)
+    return Ws
+
+
+def _compress_mpo(Ws: List[np.ndarray], tol: float) -> List[np.ndarray]:
+    """Left-to-right QR then right-to-left truncated SVD of the MPO bonds."""
+    Ws = [w.copy() for w in Ws]
+    N = len(Ws)
+    for k in range(N - 1):  # left-orthogonalise with QR
+        wl, d1, d2, wr = Ws[k].shape
+        Q, R = np.linalg.qr(Ws[k].reshape(wl * d1 * d2, wr))
+        Ws[k] = Q.reshape(wl, d1, d2, -1)
+        Ws[k + 1] = np.tensordot(R, Ws[k + 1], axes=([1], [0]))
+    for k in range(N - 1, 0, -1):  # right-to-left truncated SVD
+        wl, d1, d2, wr = Ws[k].shape
+        U, s, Vt = np.linalg.svd(Ws[k].reshape(wl, d1 * d2 * wr), full_matrices=False)
+        keep = max(1, int(np.sum(s > tol * s[0])))
+        Ws[k] = Vt[:keep].reshape(keep, d1, d2, wr)
+        Ws[k - 1] = np.tensordot(Ws[k - 1], U[:, :keep] * s[:keep], axes=([3], [0]))
+    return [np.ascontiguousarray(w) for w in Ws]
+
+
+def cylinder_couplings(Ly: int, Lx: int, J1: float = 1.0, J2: float = 0.0, snake: bool = True) -> dict:
+    """J1-J2 couplings on an Ly x Lx square-lattice cylinder, periodic in y
+    (Ly >= 3), open in x; sites numbered column by column, reversing the
+    y order in every other column when `snake`."""
+    def idx(x, y):
+        y %= Ly
+        return x * Ly + (y if (not snake or x % 2 == 0) else Ly - 1 - y)
+    J = {}
+
+    def add(a, b, v):
+        key = (min(a, b), max(a, b))
+        J[key] = J.get(key, 0.0) + v
+    for x in range(Lx):
+        for y in range(Ly):
+            add(idx(x, y), idx(x, y + 1), J1)  # vertical (periodic)
+            if x + 1 < Lx:
+                add(idx(x, y), idx(x + 1, y), J1)  # horizontal
+                if J2 != 0.0:
+                    add(idx(x, y), idx(x + 1, y + 1), J2)
+                    add(idx(x, y), idx(x + 1, y - 1), J2)
+    return J
+
+
+def j1j2_cylinder_mpo(Ly: int, Lx: int, J1: float = 1.0, J2: float = 0.5, snake: bool = True,
+                      compress: bool = True) -> List[np.ndarray]:
+    """The J1-J2 Heisenberg cylinder of the wide-MPO benchmark (Ly = 6, Lx = 12,
+    J2 = 0.5 there)."""
+    return spin_coupling_mpo(Ly * Lx, cylinder_couplings(Ly, Lx, J1, J2, snake), compress=compress)

@@ -367,3 +367,35 @@ def test_tdvp_truncated_xxz_matches_oracle(T):
     assert abs(eng.swdata.energy[-1] - e_neel) < 1e-3
     assert np.abs(_sz_profile(eng.sysenv.psi) - _sz_profile(ref.psi)).max() < 1e-7
     assert eng.swdata.maxchi == sw.maxchi
+
+
+def _sparse_ed(N, J):
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    sz = sp.csr_matrix(np.diag([0.5, -0.5]))
+    spm = sp.csr_matrix(np.array([[0.0, 1.0], [0.0, 0.0]]))
+    smm = spm.T.tocsr()
+
+    def site(op, i):
+        return sp.kron(sp.kron(sp.identity(2 ** i), op), sp.identity(2 ** (N - i - 1)), format="csr")
+    H = sp.csr_matrix((2 ** N, 2 ** N))
+    for (i, j), v in J.items():
+        H = H + v * (site(sz, i) @ site(sz, j) + 0.5 * (site(spm, i) @ site(smm, j) + site(smm, i) @ site(spm, j)))
+    return spl.eigsh(H, k=1, which="SA")[0][0]
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-9)])
+def test_dmrg_j1j2_cylinder_reaches_ed(T, prec, tol):
+    """Wide MPO (w up to 14): J1-J2 Heisenberg (J2 = 0.5) on a 4 x 4 cylinder,
+    periodic in y, snake order.  Two-site DMRG to full bond dimension reaches
+    the sparse exact-diagonalisation ground energy (FP64 and FP32-MFMA forms)."""
+    Ly, Lx = 4, 4
+    N = Ly * Lx
+    e0 = _sparse_ed(N, T.cylinder_couplings(Ly, Lx, 1.0, 0.5))
+    Ws = T.j1j2_cylinder_mpo(Ly, Lx, 1.0, 0.5)
+    assert max(w.shape[3] for w in Ws[:-1]) == 14
+    env = T.StateEnvs(T.neel_state(N), Ws)
+    params = T.make_dmrg_params(maxdim=[32, 64, 128, 256, 256, 256, 256], nsweeps=[1] * 7, cutoff=1e-12)
+    sw = T.dmrg_(env, params, 2, outputlevel=0, precision=prec)
+    assert sw.maxchi[-1] == 256
+    assert abs(sw.energy[-1] - e0) / N < tol, (sw.energy, e0)

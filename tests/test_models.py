@@ -1,0 +1,64 @@
+"""CPU pins of the host-side model inputs (MPO builders): each MPO, contracted
+to a dense matrix, equals the Hamiltonian built independently from Kronecker
+products of the same couplings."""
+
+import numpy as np
+import pytest
+
+from oracle import tn_oracle as O
+from tennetlib_amd import models as M
+
+SZ = np.diag([0.5, -0.5])
+SP = np.array([[0.0, 1.0], [0.0, 0.0]])
+SM = SP.T.copy()
+
+
+def _site(op, i, N):
+    out = np.ones((1, 1))
+    for k in range(N):
+        out = np.kron(out, op if k == i else np.eye(2))
+    return out
+
+
+def _dense_from_couplings(N, J, h=None):
+    H = np.zeros((2 ** N, 2 ** N))
+    for (i, j), v in J.items():
+        H += v * (_site(SZ, i, N) @ _site(SZ, j, N)
+                  + 0.5 * (_site(SP, i, N) @ _site(SM, j, N) + _site(SM, i, N) @ _site(SP, j, N)))
+    if h is not None:
+        for i, v in enumerate(h):
+            H += v * _site(SZ, i, N)
+    return H
+
+
+@pytest.mark.parametrize("compress", [False, True])
+def test_coupling_mpo_random_graph(compress):
+    rng = np.random.default_rng(3)
+    N = 8
+    J = {}
+    for _ in range(12):
+        i, j = rng.choice(N, 2, replace=False)
+        J[(int(min(i, j)), int(max(i, j)))] = float(rng.normal())
+    h = rng.normal(size=N)
+    Ws = M.spin_coupling_mpo(N, J, h=h, compress=compress)
+    assert np.abs(O.mpo_to_dense(Ws) - _dense_from_couplings(N, J, h)).max() < 1e-12
+
+
+@pytest.mark.parametrize("snake", [True, False])
+def test_j1j2_cylinder_mpo(snake):
+    Ly, Lx = 3, 3
+    J = M.cylinder_couplings(Ly, Lx, 1.0, 0.5, snake)
+    # 9 sites: 9 vertical (periodic), 6 horizontal, 12 diagonal bonds
+    assert len(J) == 27
+    Hd = _dense_from_couplings(Ly * Lx, J)
+    for comp in (False, True):
+        Ws = M.j1j2_cylinder_mpo(Ly, Lx, 1.0, 0.5, snake=snake, compress=comp)
+        assert np.abs(O.mpo_to_dense(Ws) - Hd).max() < 1e-12
+
+
+def test_chain_builder_matches_fsm_heisenberg():
+    N = 8
+    J = {(i, i + 1): 1.0 for i in range(N - 1)}
+    Ws = M.spin_coupling_mpo(N, J)
+    assert max(w.shape[3] for w in Ws[:-1]) == 5  # compressed to the FSM bond dimension
+    assert np.abs(O.mpo_to_dense(Ws) - O.mpo_to_dense(O.heisenberg_mpo(N))).max() < 1e-12
