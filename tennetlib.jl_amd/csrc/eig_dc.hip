@@ -510,6 +510,287 @@ __global__ __launch_bounds__(256) void trd_extract_v_kernel(const double* __rest
 }
 
 // ---------------------------------------------------------------------------
+// 1z. Hermitian tridiagonalisation (complex128, interleaved) to a REAL
+// symmetric tridiagonal: reflectors H = I - tau v v^H with v[0] = 1 chosen so
+// that H^H x = beta e_1 with beta real (the zlarfg convention); the update of
+// step j is G22 <- H^H G22 H = G22 - v w^H - w v^H with y = tau G22 v,
+// alpha = -(tau/2) (y^H v) (real) and w = y + alpha v.  Reflectors are kept
+// below the diagonal; p - 1 of them (the last one only removes a phase).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 zadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 zsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 zscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__device__ __forceinline__ double2 zinv(double2 a) {
+  const double d = a.x * a.x + a.y * a.y;
+  return make_double2(a.x / d, -a.y / d);
+}
+__device__ double2 block_sum_z(double2 v, double* red) {
+  const double re = block_sum(v.x, red);
+  const double im = block_sum(v.y, red);
+  return make_double2(re, im);
+}
+// reflector from x (length m, x[0] = alpha): tau, beta and the scale of x[1:]
+struct ZRefl {
+  double2 tau, scale;
+  double beta;
+};
+__device__ ZRefl zlarfg(double2 alpha, double xnorm2) {
+  ZRefl r;
+  if (xnorm2 == 0.0 && alpha.y == 0.0) {
+    r.tau = make_double2(0.0, 0.0);
+    r.beta = alpha.x;
+    r.scale = make_double2(0.0, 0.0);
+    return r;
+  }
+  const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xnorm2);
+  r.beta = (alpha.x >= 0.0) ? -nrm : nrm;
+  r.tau = make_double2((r.beta - alpha.x) / r.beta, -alpha.y / r.beta);
+  r.scale = zinv(make_double2(alpha.x - r.beta, alpha.y));
+  return r;
+}
+
+// step j: x = column j below the diagonal = conj(row j right of it)
+__global__ __launch_bounds__(256) void trdz_reflect_kernel(double2* __restrict__ A, long ld, int p, int j,
+                                                           double2* __restrict__ y, double2* __restrict__ part,
+                                                           double* __restrict__ dv, double* __restrict__ ev,
+                                                           double2* __restrict__ tauv) {
+  extern __shared__ double2 zv[];
+  __shared__ double red[4];
+  const int m = p - j - 1, tid = threadIdx.x;
+  const double2* xr = A + (long)j * ld + (j + 1);
+  double ss = 0.0;
+  for (int c = tid; c < m; c += 256) {
+    const double2 t = zconj(xr[c]);
+    zv[c] = t;
+    if (c > 0) ss += t.x * t.x + t.y * t.y;
+  }
+  ss = block_sum(ss, red);
+  const double2 alpha = zv[0];
+  const ZRefl R = zlarfg(alpha, ss);
+  __syncthreads();
+  for (int c = tid; c < m; c += 256) zv[c] = (c == 0) ? make_double2(1.0, 0.0) : zmul(zv[c], R.scale);
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int c = tid; c < m; c += 256) A[(long)(j + 1 + c) * ld + j] = zv[c];
+    if (tid == 0) {
+      dv[j] = A[(long)j * ld + j].x;
+      ev[j] = R.beta;
+      tauv[j] = R.tau;
+    }
+  }
+  if (R.tau.x == 0.0 && R.tau.y == 0.0) {
+    if (tid == 0) part[blockIdx.x] = make_double2(0.0, 0.0);
+    return;
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  const int r0 = blockIdx.x * HR;
+  double2 dot = make_double2(0.0, 0.0);
+  for (int rr = wave; rr < HR; rr += 4) {
+    const int r = r0 + rr;
+    if (r >= m) break;
+    const double2* row = A + (long)(j + 1 + r) * ld + (j + 1);
+    double sre = 0.0, sim = 0.0;
+    for (int c = lane; c < m; c += 64) {
+      const double2 t = zmul(row[c], zv[c]);
+      sre += t.x;
+      sim += t.y;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      sre += __shfl_down(sre, o, 64);
+      sim += __shfl_down(sim, o, 64);
+    }
+    if (lane == 0) {
+      const double2 yr = zmul(R.tau, make_double2(sre, sim));
+      y[r] = yr;
+      dot = zadd(dot, zmul(zconj(yr), zv[r]));  // y^H v
+    }
+  }
+  dot = block_sum_z(dot, red);
+  if (tid == 0) part[blockIdx.x] = dot;
+}
+
+__global__ __launch_bounds__(256) void trdz_update_kernel(double2* __restrict__ A, long ld, int p, int j,
+                                                          const double2* __restrict__ y,
+                                                          const double2* __restrict__ part, int nparts,
+                                                          const double2* __restrict__ tauv) {
+  const double2 tau = tauv[j];
+  if (tau.x == 0.0 && tau.y == 0.0) return;
+  __shared__ double red[4];
+  __shared__ double2 vr[64], wr[64], vc[64], wc[64];
+  const int m = p - j - 1, tid = threadIdx.x;
+  double2 s = make_double2(0.0, 0.0);
+  for (int i = tid; i < nparts; i += 256) s = zadd(s, part[i]);
+  s = block_sum_z(s, red);
+  const double2 al = zscale(zmul(tau, s), -0.5);  // alpha (real up to rounding)
+  const int tr0 = blockIdx.y * 64, tc0 = blockIdx.x * 64;
+  if (tid < 64) {
+    const int r = tr0 + tid;
+    if (r < m) {
+      const double2 v = A[(long)(j + 1 + r) * ld + j];
+      vr[tid] = v;
+      wr[tid] = zadd(y[r], zmul(al, v));
+    } else {
+      vr[tid] = make_double2(0.0, 0.0);
+      wr[tid] = make_double2(0.0, 0.0);
+    }
+  } else if (tid < 128) {
+    const int c = tc0 + tid - 64;
+    if (c < m) {
+      const double2 v = A[(long)(j + 1 + c) * ld + j];
+      vc[tid - 64] = v;
+      wc[tid - 64] = zadd(y[c], zmul(al, v));
+    } else {
+      vc[tid - 64] = make_double2(0.0, 0.0);
+      wc[tid - 64] = make_double2(0.0, 0.0);
+    }
+  }
+  __syncthreads();
+  const int cc = tid & 63;
+  const int c = tc0 + cc;
+  if (c >= m) return;
+  const double2 cwc = zconj(wc[cc]), cvc = zconj(vc[cc]);
+  for (int rr = tid >> 6; rr < 64; rr += 4) {
+    const int r = tr0 + rr;
+    if (r >= m) break;
+    double2* a = A + (long)(j + 1 + r) * ld + (j + 1 + c);
+    *a = zsub(*a, zadd(zmul(vr[rr], cwc), zmul(wr[rr], cvc)));
+  }
+}
+
+// the last columns in LDS (m0 = p - j0 <= TAILZ), including the final
+// one-element reflector that makes e_{p-2} real
+constexpr int TAILZ = 90;
+__global__ __launch_bounds__(256) void trdz_tail_lds_kernel(double2* __restrict__ A, long ld, int p, int j0,
+                                                            double* __restrict__ dv, double* __restrict__ ev,
+                                                            double2* __restrict__ tauv) {
+  extern __shared__ double2 zl[];
+  __shared__ double red[4];
+  const int m0 = p - j0, L = m0 + 1;
+  double2* M = zl;                // m0 x L
+  double2* vs = M + (long)m0 * L; // m0
+  double2* ys = vs + m0;          // m0
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int r = wave; r < m0; r += 4)
+    for (int c = lane; c < m0; c += 64) M[r * L + c] = A[(long)(j0 + r) * ld + (j0 + c)];
+  __syncthreads();
+  for (int jj = 0; jj < m0 - 1; ++jj) {
+    const int j = j0 + jj, m = m0 - jj - 1;
+    double ss = 0.0;
+    for (int c = tid; c < m; c += 256) {
+      const double2 t = zconj(M[jj * L + (jj + 1 + c)]);
+      vs[c] = t;
+      if (c > 0) ss += t.x * t.x + t.y * t.y;
+    }
+    ss = block_sum(ss, red);
+    const ZRefl R = zlarfg(vs[0], ss);
+    __syncthreads();
+    for (int c = tid; c < m; c += 256) {
+      const double2 v = (c == 0) ? make_double2(1.0, 0.0) : zmul(vs[c], R.scale);
+      vs[c] = v;
+      A[(long)(j + 1 + c) * ld + j] = v;
+    }
+    if (tid == 0) {
+      dv[j] = M[jj * L + jj].x;
+      ev[j] = R.beta;
+      tauv[j] = R.tau;
+    }
+    __syncthreads();
+    if (R.tau.x == 0.0 && R.tau.y == 0.0) continue;  // uniform
+    double2 dot = make_double2(0.0, 0.0);
+    if (tid < m) {
+      const double2* row = M + (jj + 1 + tid) * L + (jj + 1);
+      double sre = 0.0, sim = 0.0;
+      for (int c = 0; c < m; ++c) {
+        const double2 t = zmul(row[c], vs[c]);
+        sre += t.x;
+        sim += t.y;
+      }
+      const double2 yr = zmul(R.tau, make_double2(sre, sim));
+      ys[tid] = yr;
+      dot = zmul(zconj(yr), vs[tid]);
+    }
+    dot = block_sum_z(dot, red);
+    const double2 al = zscale(zmul(R.tau, dot), -0.5);
+    for (int c = tid; c < m; c += 256) ys[c] = zadd(ys[c], zmul(al, vs[c]));  // w
+    __syncthreads();
+    for (int c = lane; c < m; c += 64) {
+      const double2 cvc = zconj(vs[c]), cwc = zconj(ys[c]);
+      for (int r = wave; r < m; r += 4) {
+        double2* a = M + (jj + 1 + r) * L + (jj + 1 + c);
+        *a = zsub(*a, zadd(zmul(vs[r], cwc), zmul(ys[r], cvc)));
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) dv[p - 1] = M[(m0 - 1) * L + (m0 - 1)].x;
+}
+
+// explicit complex reflectors (p x p): column j = v_j below row j, zero elsewhere
+__global__ __launch_bounds__(256) void trdz_extract_v_kernel(const double2* __restrict__ A, long ld, int p,
+                                                             const double2* __restrict__ tauv,
+                                                             double2* __restrict__ V) {
+  const long tot = (long)p * p;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
+    const int r = (int)(i / p), c = (int)(i % p);
+    const bool live = c < p - 1 && r > c && (tauv[c].x != 0.0 || tauv[c].y != 0.0);
+    V[i] = live ? A[(long)r * ld + c] : make_double2(0.0, 0.0);
+  }
+}
+
+// W <- T W (complex), T^{-1} = diag(1/tau) + striu(S), S = V^H V; one thread per column
+__global__ __launch_bounds__(256) void wyz_tsolve_kernel(const double2* __restrict__ S, const double2* __restrict__ tauv,
+                                                         int j0, int nb, double2* __restrict__ W, long n) {
+  constexpr int NB = 32;  // reflectors per complex block
+  __shared__ double2 Ss[NB][NB + 1];
+  __shared__ double2 tt[NB];
+  for (int idx = threadIdx.x; idx < NB * NB; idx += 256) Ss[idx / NB][idx % NB] = S[idx];
+  if (threadIdx.x < NB) tt[threadIdx.x] = (threadIdx.x < nb) ? tauv[j0 + threadIdx.x] : make_double2(0.0, 0.0);
+  __syncthreads();
+  const long col = blockIdx.x * 256L + threadIdx.x;
+  if (col >= n) return;
+  double2 x[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i) x[i] = (i < nb) ? W[(long)i * n + col] : make_double2(0.0, 0.0);
+#pragma unroll
+  for (int i = NB - 1; i >= 0; --i) {
+    if (i < nb) {
+      double2 acc = x[i];
+#pragma unroll
+      for (int jx = i + 1; jx < NB; ++jx)
+        if (jx < nb) acc = zsub(acc, zmul(Ss[i][jx], x[jx]));
+      x[i] = zmul(tt[i], acc);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    if (i < nb) W[(long)i * n + col] = x[i];
+}
+
+// real p x p (ld p) -> complex (ld p), and Vt[i][r] = conj(E[r][i]) (ld ldv)
+__global__ __launch_bounds__(256) void real_to_z_kernel(const double* __restrict__ Q, long n, double2* __restrict__ E) {
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += (long)gridDim.x * 256) E[i] = make_double2(Q[i], 0.0);
+}
+__global__ __launch_bounds__(256) void adjoint_z_kernel(const double2* __restrict__ E, int p, double2* __restrict__ Vt,
+                                                        long ldv) {
+  __shared__ double2 t[32][33];
+  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    t[k][tx] = (r < p && c < p) ? E[(long)r * p + c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int i = c0 + k, r = r0 + tx;  // Vt[i][r] = conj(E[r][i])
+    if (i < p && r < p) Vt[(long)i * ldv + r] = zconj(t[tx][k]);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // 2. divide and conquer
 // ---------------------------------------------------------------------------
 // Leaf Jacobi: block [a, a + n) of the torn tridiagonal (n <= LEAF = 32),
@@ -932,136 +1213,36 @@ static void deflate(HostMerge& hm, std::vector<double> D, std::vector<double> z,
   }
 }
 
-void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w) {
-  TN_REQUIRE(p >= 3 && p <= 16384, TN_E_INVALID, "sym_eig_dc: size out of range");
-  const int P = (int)p;
-  double* Gm = G;
-  // TNHIP_DEBUG_EIG: stage times (each stage synchronised) on stderr
-  static const bool dbg = std::getenv("TNHIP_DEBUG_EIG") != nullptr;
-  auto t_last = std::chrono::steady_clock::now();
-  auto stage = [&](const char* name) {
-    if (!dbg) return;
+// TNHIP_DEBUG_EIG: stage times (each stage synchronised) on stderr
+struct EigStages {
+  tn_ctx* c;
+  long p;
+  bool on;
+  std::chrono::steady_clock::time_point t;
+  EigStages(tn_ctx* ctx, long n) : c(ctx), p(n), on(std::getenv("TNHIP_DEBUG_EIG") != nullptr),
+                                   t(std::chrono::steady_clock::now()) {}
+  void operator()(const char* name) {
+    if (!on) return;
     TN_HIP(hipStreamSynchronize(c->stream));
     const auto now = std::chrono::steady_clock::now();
     std::fprintf(stderr, "[tnhip dc] p=%ld %s %.3f ms\n", p, name,
-                 std::chrono::duration<double, std::milli>(now - t_last).count());
-    t_last = now;
-  };
-  // ---- 1. tridiagonalisation
-  double* vecs = (double*)c->workspace(WS_DC_VEC, (size_t)(12 * p + 2048) * sizeof(double));
-  double* dv = vecs;
-  double* ev = dv + p;
-  double* tauv = ev + p;
-  double* y = tauv + p;
-  double* part = y + p;  // <= ceil(p / HR) entries
-  // one cooperative launch when the grid fits co-resident, else two launches per column
-  static const bool coop_env = [] {
-    const char* e = std::getenv("TNHIP_TRD_COOP");  // measured slower than per-column launches
-    return e && std::atoi(e) == 1;
-  }();
-  bool done = false;
-  if (coop_env && p <= 8192) {
-    int dev = 0, coop = 0, ncu = 0;
-    TN_HIP(hipGetDevice(&dev));
-    TN_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-    TN_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t shm = 2 * (size_t)(p - 1) * sizeof(double);
-    if (shm > 64 * 1024)
-      TN_HIP(hipFuncSetAttribute((const void*)trd_coop_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shm));
-    int per_cu = 0;
-    TN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)trd_coop_kernel, 256, shm));
-    const long want = std::min<long>(256, std::max<long>(8, (p - 1 + 7) / 8));
-    const long G = std::min<long>(want, (long)per_cu * ncu);
-    if (coop && G >= 1 && per_cu >= 1) {
-      unsigned* bar = (unsigned*)(vecs + 6 * p + 256);  // past the partial sums (<= 2p + 256)
-      int* abort_flag = (int*)(bar + 2);
-      fill_f64(c, (double*)bar, 0.0, 2);
-      int P_ = P;
-      long ld_ = ldg;
-      double* A_ = Gm;
-      void* args[] = {&A_, &ld_, &P_, &y, &part, &dv, &ev, &tauv, &bar, &abort_flag};
-      TN_HIP(hipLaunchCooperativeKernel((const void*)trd_coop_kernel, dim3((unsigned)G), dim3(256), args, shm,
-                                        c->stream));
-      int hab[4] = {0, 0, 0, 0};
-      TN_HIP(hipMemcpyAsync(hab, bar, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      TN_HIP(hipStreamSynchronize(c->stream));
-      TN_REQUIRE(hab[2] == 0, TN_E_HIP, "sym_eig_dc: grid barrier timed out in the tridiagonalisation");
-      done = true;
-    }
+                 std::chrono::duration<double, std::milli>(now - t).count());
+    t = now;
   }
-  if (!done) {
-    const size_t lds_bytes = (size_t)p * sizeof(double);
-    if (lds_bytes > 64 * 1024)
-      TN_HIP(hipFuncSetAttribute((const void*)trd_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds_bytes));
-    const int j0 = std::max(0, P - TAIL);  // columns j0.. are reduced in LDS
-    if (j0 >= 1 && P <= 6000) {
-      // fused path: reflector 0, then one launch per column, then the update of step j0 - 1
-      double* vA = vecs + 6 * p + 512;
-      double* vB = vA + p;
-      double* yB = vB + p;
-      double* pB = yB + p;  // >= 512 + p / 4 entries
-      const size_t step_lds_max = 3 * (size_t)(P - 1) * sizeof(double);
-      static size_t step_attr = 0;
-      if (step_lds_max > 64 * 1024 && step_lds_max > step_attr) {
-        TN_HIP(hipFuncSetAttribute((const void*)trd_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)step_lds_max));
-        step_attr = step_lds_max;
-      }
-      const int m = P - 1;
-      const int np0 = (m + HR - 1) / HR;
-      hipLaunchKernelGGL(trd_reflect_kernel, dim3(np0), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg, P,
-                         0, y, part, dv, ev, tauv, vA);
-      double *cv = vA, *cy = y, *cp = part, *nv = vB, *ny = yB, *npart = pB;
-      int cnp = np0;
-      for (int j = 0; j + 2 <= j0; ++j) {
-        const int mj = P - j - 1, mn = mj - 1;
-        const int nwg = std::min(256, (mn + 3) / 4);
-        hipLaunchKernelGGL(trd_step_kernel, dim3(nwg), dim3(256), 3 * (size_t)mj * sizeof(double), c->stream, G, ldg,
-                           P, j, cv, cy, cp, cnp, nv, ny, npart, dv, ev, tauv);
-        std::swap(cv, nv);
-        std::swap(cy, ny);
-        std::swap(cp, npart);
-        cnp = nwg;
-      }
-      const int mlast = P - j0;
-      const unsigned tiles = (unsigned)((mlast + 63) / 64);
-      hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j0 - 1, cy, cp,
-                         cnp, tauv);
-    } else {
-      for (int j = 0; j < j0; ++j) {
-        const int m = P - j - 1;
-        const int nparts = (m + HR - 1) / HR;
-        hipLaunchKernelGGL(trd_reflect_kernel, dim3(nparts), dim3(256), (size_t)m * sizeof(double), c->stream, G,
-                           ldg, P, j, y, part, dv, ev, tauv, (double*)nullptr);
-        const unsigned tiles = (unsigned)((m + 63) / 64);
-        hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j, y, part,
-                           nparts, tauv);
-      }
-    }
-    const int m0 = P - j0;
-    const size_t tshm = ((size_t)m0 * (m0 + 1) + 2 * (size_t)m0) * sizeof(double);
-    static bool tail_attr = false;
-    if (!tail_attr) {
-      TN_HIP(hipFuncSetAttribute((const void*)trd_tail_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(((size_t)TAIL * (TAIL + 1) + 2 * TAIL) * sizeof(double))));
-      tail_attr = true;
-    }
-    hipLaunchKernelGGL(trd_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, j0, dv, ev, tauv);
-  }
-  TN_HIP(hipGetLastError());
-  // pinned host staging: [0, lvb) mirrors the per-level device tables (same
-  // offsets), followed by 8p doubles of scratch; every transfer below goes
-  // through it, one copy per direction per step
-  const size_t lvb = (size_t)p * (5 * 8 + 6 * 4) + (size_t)p * sizeof(MergeDev) + (size_t)p * sizeof(Rot) + 1024;
+};
+
+static size_t level_bytes(long p) {
+  return (size_t)p * (5 * 8 + 6 * 4) + (size_t)p * sizeof(MergeDev) + (size_t)p * sizeof(Rot) + 1024;
+}
+
+// Divide and conquer on the symmetric tridiagonal (d, e), p >= 2: Q (p x p,
+// ld p) receives the eigenvectors as columns; returns their eigenvalues.
+static std::vector<double> dc_tridiag(tn_ctx* c, long p, const std::vector<double>& d, const std::vector<double>& e,
+                                      double* Q, EigStages& stage) {
+  const int P = (int)p;
+  const size_t lvb = level_bytes(p);
   char* hs = (char*)c->host_stage(lvb + 8 * (size_t)p * sizeof(double) + 256);
   double* hx = (double*)(hs + ((lvb + 255) & ~(size_t)255));
-  TN_HIP(hipMemcpyAsync(hx, dv, (2 * p - 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));  // d, e
-  TN_HIP(hipStreamSynchronize(c->stream));
-  std::vector<double> d(hx, hx + p), e(hx + p, hx + 2 * p - 1);
-
-  stage("tridiagonal");
   // ---- 2. divide and conquer on (d, e)
   double tnorm = 0.0;
   for (long i = 0; i < p; ++i) tnorm = std::max(tnorm, std::fabs(d[i]));
@@ -1075,7 +1256,6 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     dt[k - 1] -= e[k - 1];
     dt[k] -= e[k - 1];
   }
-  double* Q = (double*)c->workspace(WS_DC_Q, (size_t)p * p * sizeof(double));
   const long ldq = p;
   double* small = (double*)c->workspace(WS_DC_SMALL, (size_t)(8 * p + 64) * sizeof(double));
   double* d_dt = small;
@@ -1243,6 +1423,129 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     stage("merge level");
   }
 
+  return D;
+}
+
+void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w) {
+  TN_REQUIRE(p >= 3 && p <= 16384, TN_E_INVALID, "sym_eig_dc: size out of range");
+  const int P = (int)p;
+  double* Gm = G;
+  EigStages stage(c, p);
+  // ---- 1. tridiagonalisation
+  double* vecs = (double*)c->workspace(WS_DC_VEC, (size_t)(12 * p + 2048) * sizeof(double));
+  double* dv = vecs;
+  double* ev = dv + p;
+  double* tauv = ev + p;
+  double* y = tauv + p;
+  double* part = y + p;  // <= ceil(p / HR) entries
+  // one cooperative launch when the grid fits co-resident, else two launches per column
+  static const bool coop_env = [] {
+    const char* e = std::getenv("TNHIP_TRD_COOP");  // measured slower than per-column launches
+    return e && std::atoi(e) == 1;
+  }();
+  bool done = false;
+  if (coop_env && p <= 8192) {
+    int dev = 0, coop = 0, ncu = 0;
+    TN_HIP(hipGetDevice(&dev));
+    TN_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    TN_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const size_t shm = 2 * (size_t)(p - 1) * sizeof(double);
+    if (shm > 64 * 1024)
+      TN_HIP(hipFuncSetAttribute((const void*)trd_coop_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)shm));
+    int per_cu = 0;
+    TN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)trd_coop_kernel, 256, shm));
+    const long want = std::min<long>(256, std::max<long>(8, (p - 1 + 7) / 8));
+    const long G = std::min<long>(want, (long)per_cu * ncu);
+    if (coop && G >= 1 && per_cu >= 1) {
+      unsigned* bar = (unsigned*)(vecs + 6 * p + 256);  // past the partial sums (<= 2p + 256)
+      int* abort_flag = (int*)(bar + 2);
+      fill_f64(c, (double*)bar, 0.0, 2);
+      int P_ = P;
+      long ld_ = ldg;
+      double* A_ = Gm;
+      void* args[] = {&A_, &ld_, &P_, &y, &part, &dv, &ev, &tauv, &bar, &abort_flag};
+      TN_HIP(hipLaunchCooperativeKernel((const void*)trd_coop_kernel, dim3((unsigned)G), dim3(256), args, shm,
+                                        c->stream));
+      int hab[4] = {0, 0, 0, 0};
+      TN_HIP(hipMemcpyAsync(hab, bar, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+      TN_HIP(hipStreamSynchronize(c->stream));
+      TN_REQUIRE(hab[2] == 0, TN_E_HIP, "sym_eig_dc: grid barrier timed out in the tridiagonalisation");
+      done = true;
+    }
+  }
+  if (!done) {
+    const size_t lds_bytes = (size_t)p * sizeof(double);
+    if (lds_bytes > 64 * 1024)
+      TN_HIP(hipFuncSetAttribute((const void*)trd_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds_bytes));
+    const int j0 = std::max(0, P - TAIL);  // columns j0.. are reduced in LDS
+    if (j0 >= 1 && P <= 6000) {
+      // fused path: reflector 0, then one launch per column, then the update of step j0 - 1
+      double* vA = vecs + 6 * p + 512;
+      double* vB = vA + p;
+      double* yB = vB + p;
+      double* pB = yB + p;  // >= 512 + p / 4 entries
+      const size_t step_lds_max = 3 * (size_t)(P - 1) * sizeof(double);
+      static size_t step_attr = 0;
+      if (step_lds_max > 64 * 1024 && step_lds_max > step_attr) {
+        TN_HIP(hipFuncSetAttribute((const void*)trd_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)step_lds_max));
+        step_attr = step_lds_max;
+      }
+      const int m = P - 1;
+      const int np0 = (m + HR - 1) / HR;
+      hipLaunchKernelGGL(trd_reflect_kernel, dim3(np0), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg, P,
+                         0, y, part, dv, ev, tauv, vA);
+      double *cv = vA, *cy = y, *cp = part, *nv = vB, *ny = yB, *npart = pB;
+      int cnp = np0;
+      for (int j = 0; j + 2 <= j0; ++j) {
+        const int mj = P - j - 1, mn = mj - 1;
+        const int nwg = std::min(256, (mn + 3) / 4);
+        hipLaunchKernelGGL(trd_step_kernel, dim3(nwg), dim3(256), 3 * (size_t)mj * sizeof(double), c->stream, G, ldg,
+                           P, j, cv, cy, cp, cnp, nv, ny, npart, dv, ev, tauv);
+        std::swap(cv, nv);
+        std::swap(cy, ny);
+        std::swap(cp, npart);
+        cnp = nwg;
+      }
+      const int mlast = P - j0;
+      const unsigned tiles = (unsigned)((mlast + 63) / 64);
+      hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j0 - 1, cy, cp,
+                         cnp, tauv);
+    } else {
+      for (int j = 0; j < j0; ++j) {
+        const int m = P - j - 1;
+        const int nparts = (m + HR - 1) / HR;
+        hipLaunchKernelGGL(trd_reflect_kernel, dim3(nparts), dim3(256), (size_t)m * sizeof(double), c->stream, G,
+                           ldg, P, j, y, part, dv, ev, tauv, (double*)nullptr);
+        const unsigned tiles = (unsigned)((m + 63) / 64);
+        hipLaunchKernelGGL(trd_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j, y, part,
+                           nparts, tauv);
+      }
+    }
+    const int m0 = P - j0;
+    const size_t tshm = ((size_t)m0 * (m0 + 1) + 2 * (size_t)m0) * sizeof(double);
+    static bool tail_attr = false;
+    if (!tail_attr) {
+      TN_HIP(hipFuncSetAttribute((const void*)trd_tail_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(((size_t)TAIL * (TAIL + 1) + 2 * TAIL) * sizeof(double))));
+      tail_attr = true;
+    }
+    hipLaunchKernelGGL(trd_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, j0, dv, ev, tauv);
+  }
+  TN_HIP(hipGetLastError());
+  // d, e to the host through the pinned staging buffer
+  double* hx = (double*)c->host_stage(2 * (size_t)p * sizeof(double));
+  TN_HIP(hipMemcpyAsync(hx, dv, (2 * p - 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));  // d, e
+  TN_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> d(hx, hx + p), e(hx + p, hx + 2 * p - 1);
+  stage("tridiagonal");
+  double* Q = (double*)c->workspace(WS_DC_Q, (size_t)p * p * sizeof(double));
+  const long ldq = p;
+  std::vector<double> D = dc_tridiag(c, p, d, e, Q, stage);
+  double* tmp = (double*)c->workspace(WS_DC_TMP, (size_t)p * p * sizeof(double));
+
   // ---- 3. back-transformation Q <- H Q, H = H_0 ... H_{p-3}
   double* V = tmp;  // explicit reflectors (p x p)
   hipLaunchKernelGGL(trd_extract_v_kernel, dim3(grid_for(p * p)), dim3(256), 0, c->stream, G, ldg, P, tauv, V);
@@ -1288,6 +1591,111 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   transpose_f64(c, Q, p, p, tmp);
   TN_HIP(hipMemcpy2DAsync(Vt, ldv * sizeof(double), tmp, p * sizeof(double), p * sizeof(double), p,
                           hipMemcpyDeviceToDevice, c->stream));
+  stage("back-transform");
+  w.assign(D.begin(), D.end());
+  TN_HIP(hipStreamSynchronize(c->stream));
+}
+
+
+void sym_eig_dc_z(tn_ctx* c, double* Gd, long p, long ldg, double* Vtd, long ldv, std::vector<double>& w) {
+  TN_REQUIRE(p >= 2 && p <= 16384, TN_E_INVALID, "sym_eig_dc_z: size out of range");
+  double2* G = reinterpret_cast<double2*>(Gd);
+  double2* Vt = reinterpret_cast<double2*>(Vtd);
+  const int P = (int)p;
+  EigStages stage(c, p);
+  // ---- 1. Hermitian -> real tridiagonal
+  double* vecs = (double*)c->workspace(WS_DC_VEC, (size_t)(12 * p + 2048) * sizeof(double));
+  double* dv = vecs;
+  double* ev = dv + p;
+  double2* tauz = reinterpret_cast<double2*>(ev + p);  // p
+  double2* yz = tauz + p;                              // p
+  double2* partz = yz + p;                             // <= ceil(p / HR)
+  const int j0 = std::max(0, P - TAILZ);
+  {
+    const size_t lds_max = (size_t)p * sizeof(double2);
+    static size_t attr = 0;
+    if (lds_max > 64 * 1024 && lds_max > attr) {
+      TN_HIP(hipFuncSetAttribute((const void*)trdz_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds_max));
+      attr = lds_max;
+    }
+  }
+  for (int j = 0; j < j0; ++j) {
+    const int m = P - j - 1;
+    const int nparts = (m + HR - 1) / HR;
+    hipLaunchKernelGGL(trdz_reflect_kernel, dim3(nparts), dim3(256), (size_t)m * sizeof(double2), c->stream, G, ldg,
+                       P, j, yz, partz, dv, ev, tauz);
+    const unsigned tiles = (unsigned)((m + 63) / 64);
+    hipLaunchKernelGGL(trdz_update_kernel, dim3(tiles, tiles), dim3(256), 0, c->stream, G, ldg, P, j, yz, partz,
+                       nparts, tauz);
+  }
+  {
+    const int m0 = P - j0;
+    const size_t tshm = ((size_t)m0 * (m0 + 1) + 2 * (size_t)m0) * sizeof(double2);
+    static bool tail_attr = false;
+    if (!tail_attr) {
+      TN_HIP(hipFuncSetAttribute((const void*)trdz_tail_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(((size_t)TAILZ * (TAILZ + 1) + 2 * TAILZ) * sizeof(double2))));
+      tail_attr = true;
+    }
+    hipLaunchKernelGGL(trdz_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, j0, dv, ev, tauz);
+  }
+  TN_HIP(hipGetLastError());
+  double* hx = (double*)c->host_stage(2 * (size_t)p * sizeof(double));
+  TN_HIP(hipMemcpyAsync(hx, dv, (2 * p - 1) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipStreamSynchronize(c->stream));
+  std::vector<double> d(hx, hx + p), e(hx + p, hx + 2 * p - 1);
+  stage("tridiagonal");
+  // ---- 2. real divide and conquer
+  double* Q = (double*)c->workspace(WS_DC_Q, (size_t)p * p * sizeof(double));
+  std::vector<double> D = dc_tridiag(c, p, d, e, Q, stage);
+  // ---- 3. E = H_0 ... H_{p-2} Q (complex), blocks of 32 reflectors in compact WY form
+  double2* V = (double2*)c->workspace(WS_DC_TMP, (size_t)p * p * sizeof(double2));
+  double2* E = (double2*)c->workspace(WS_DC_UT, (size_t)p * p * sizeof(double2));
+  hipLaunchKernelGGL(real_to_z_kernel, dim3(grid_for(p * p)), dim3(256), 0, c->stream, Q, p * p, E);
+  hipLaunchKernelGGL(trdz_extract_v_kernel, dim3(grid_for(p * p)), dim3(256), 0, c->stream, G, ldg, P, tauz, V);
+  constexpr int NBZ = 32;
+  double2* W = (double2*)c->workspace(WS_DC_W, (size_t)(NBZ * p + NBZ * NBZ) * sizeof(double2));
+  double2* S = W + (long)NBZ * p;
+  const int nrefl = P - 1;
+  for (int jb = ((nrefl - 1) / NBZ) * NBZ; jb >= 0; jb -= NBZ) {
+    const int nbk = std::min(NBZ, nrefl - jb);
+    const long rows = p - jb - 1;
+    const double2* Vs = V + (long)(jb + 1) * p + jb;
+    fill_f64(c, (double*)S, 0.0, 2 * NBZ * NBZ);
+    {  // S = Vs^H Vs
+      GemmDesc g;
+      g.M = nbk; g.N = nbk; g.K = rows;
+      g.A = Vs; g.lda = p; g.tA = 1; g.cA = 1;
+      g.B = Vs; g.ldb = p;
+      g.C = S; g.ldc = NBZ;
+      gemm(c, Prec::Z64, g);
+    }
+    {  // W = Vs^H E[jb+1:, :]
+      GemmDesc g;
+      g.M = nbk; g.N = p; g.K = rows;
+      g.A = Vs; g.lda = p; g.tA = 1; g.cA = 1;
+      g.B = E + (long)(jb + 1) * p; g.ldb = p;
+      g.C = W; g.ldc = p;
+      gemm(c, Prec::Z64, g);
+    }
+    hipLaunchKernelGGL(wyz_tsolve_kernel, dim3((unsigned)((p + 255) / 256)), dim3(256), 0, c->stream, S, tauz, jb,
+                       nbk, W, p);
+    {  // E[jb+1:, :] -= Vs W
+      GemmDesc g;
+      g.M = rows; g.N = p; g.K = nbk;
+      g.A = Vs; g.lda = p;
+      g.B = W; g.ldb = p;
+      g.C = E + (long)(jb + 1) * p; g.ldc = p;
+      g.alpha = -1.0; g.beta = 1.0;
+      gemm(c, Prec::Z64, g);
+    }
+    TN_HIP(hipGetLastError());
+  }
+  // eigenvectors as conjugated rows: Vt = E^H
+  hipLaunchKernelGGL(adjoint_z_kernel, dim3((unsigned)((p + 31) / 32), (unsigned)((p + 31) / 32)), dim3(256), 0,
+                     c->stream, E, P, Vt, ldv);
+  TN_HIP(hipGetLastError());
   stage("back-transform");
   w.assign(D.begin(), D.end());
   TN_HIP(hipStreamSynchronize(c->stream));

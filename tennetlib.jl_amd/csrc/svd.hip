@@ -527,6 +527,9 @@ void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, Gr
   if (!cplx && p >= std::max(3L, dc_min)) {
     sym_eig_dc(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
     out.sweeps = 0;
+  } else if (cplx && p >= std::max(3L, dc_min)) {
+    sym_eig_dc_z(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
+    out.sweeps = 0;
   } else {
     sym_eig_jacobi(c, G, p, ppad, Vt, ppad, cplx, out.w, &out.sweeps);
   }

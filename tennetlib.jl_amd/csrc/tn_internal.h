@@ -247,6 +247,10 @@ void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bo
 // divide and conquer (eig_dc.hip); same contract as sym_eig_jacobi (G is
 // overwritten, Vt rows are the eigenvectors, w their eigenvalues)
 void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w);
+// Hermitian p x p complex128 (interleaved, 2 <= p <= 16384): complex Householder
+// reduction to a real tridiagonal, the same divide and conquer, complex
+// back-transformation; Vt rows receive the conjugated eigenvectors
+void sym_eig_dc_z(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w);
 // complex: out[j][k] = conj?(src[perm[k]][j]) * scale[k]  (src rows of complex)
 void gather_rows_T_z(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
                      long kcount, long L, double* out, bool conj);

@@ -399,3 +399,23 @@ def test_dmrg_j1j2_cylinder_reaches_ed(T, prec, tol):
     sw = T.dmrg_(env, params, 2, outputlevel=0, precision=prec)
     assert sw.maxchi[-1] == 256
     assert abs(sw.energy[-1] - e0) / N < tol, (sw.energy, e0)
+
+
+def test_tdvp_xxz_chi64_complex_divide_and_conquer(T):
+    """TDVP from a random complex MPS with bond dimensions up to 48: the
+    complex density matrices (sizes up to 96) go through the Hermitian
+    divide-and-conquer solver.  GPU and oracle agree on the (conserved)
+    energy and the Sz profile."""
+    N, dt, steps = 14, 0.05, 3
+    Ws = O.xxz_mpo(N, 1.0, 0.5)
+    A = rand_mps(N, 2, [2, 4, 8, 16, 32, 48, 48, 48, 32, 16, 8, 4, 2], seed=5, cplx=True)
+    eng = T.TDVPEngine(A, Ws)
+    ref = O.StateEnvs([a.copy() for a in A], Ws)
+    sw = O.SweepData()
+    for _ in range(steps):
+        T.tdvpsweep_(eng, -1j * dt, 2, maxdim=64, cutoff=1e-12, outputlevel=0)
+        O.tdvpsweep(ref, sw, -1j * dt, maxdim=64, cutoff=1e-12)
+    assert max(a.shape[2] for a in eng.sysenv.psi) > 32  # the divide-and-conquer range was exercised
+    assert abs(eng.swdata.energy[-1] - sw.energy[-1]) < 1e-8
+    assert abs(eng.swdata.energy[-1] - eng.swdata.energy[0]) < 1e-6
+    assert np.abs(_sz_profile(eng.sysenv.psi) - _sz_profile(ref.psi)).max() < 1e-7
