@@ -137,6 +137,26 @@ def sweeps(T, psi, N, chi, nsweeps, prec, cutoff, kd, mi):
     return env, sw, dt, tm
 
 
+def tdvp_run(T, N, steps, dt, maxdim, cutoff):
+    """Two-site TDVP of the XXZ chain (Delta = 0.5) from the Neel state,
+    time_step = -i dt per tdvpsweep!; returns s/step, energy drift, max bond."""
+    env = T.TDVPEngine([a.astype(complex) for a in T.neel_state(N)], T.xxz_mpo(N, 1.0, 0.5))
+    ctx = env.sysenv.ctx
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        T.tdvpsweep_(env, -1j * dt, 2, maxdim=maxdim, cutoff=cutoff, outputlevel=0)
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    el = time.perf_counter() - t0
+    e_neel = -(N - 1) * 0.5 / 4
+    out = {"workload": f"TDVP XXZ N={N} Delta=0.5 from Neel, dt={dt}, {steps} tdvpsweep! calls, maxdim {maxdim}, "
+                       f"cutoff {cutoff:g}, complex128",
+           "s_per_step": round(el / steps, 4), "steps": steps,
+           "energy_drift": abs(env.swdata.energy[-1] - e_neel), "maxchi": env.swdata.maxchi[-1]}
+    env.sysenv.close()
+    return out
+
+
 def cpu_sweep(snap, N, chi, cutoff, kd, mi):
     """The FP64 numpy oracle running the same timed sweep (same snapshot,
     maxdim, cutoff and Krylov settings) on the host cores."""
@@ -176,6 +196,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
     ap.add_argument("--mv-wide", type=int, default=1, help="also time the w = 20 (J1-J2 cylinder) matvec")
+    ap.add_argument("--tdvp-steps", type=int, default=40, help="TDVP steps of the XXZ N=100 run (0: skip)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -246,6 +267,11 @@ def main():
                     phase_ms={k: round(v, 2) for k, v in tm_d.items()})
         env_d.close()
 
+    tdvp = None
+    if args.tdvp_steps > 0:
+        tdvp = tdvp_run(T, 100, args.tdvp_steps, 0.05, 512, 1e-12)
+        log(f"tdvp: {tdvp}")
+
     cpu = None
     if not args.skip_cpu:
         t_cpu, cores, e_cpu = cpu_sweep(snap, N, chi, args.cutoff, args.krylovdim, args.maxiter)
@@ -287,6 +313,7 @@ def main():
                      "traffic": GEMM_TRAFFIC.get((args.prec, args.mv_chi)),
                      "kernel": f"matvec GEMM ({args.prec}), M=5*{args.mv_chi}, N=4*{args.mv_chi}, "
                                f"K={args.mv_chi}; {F_gemm:.3e} algorithmic flop/launch"},
+        "tdvp_xxz_n100": tdvp,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -140,6 +140,15 @@ int tn_env_nsite(tn_env* env, int* nsite);
 int tn_env_position(tn_env* env, int pos);
 /* drop cached environments (lpos = 0, rpos = N+1) */
 int tn_env_reset(tn_env* env);
+/* Penalty projectors for excited states: StateEnvs(psi, H, Ms; weight)
+ * (state_envs.jl:83-106) builds ProjMPO_MPS2 (projmpo_mps2.jl:37-101), whose
+ * product is H v + weight * sum_k m_k (m_k^H v) with m_k the projection of
+ * the MPS M_k at the current position (ProjMPS2, projmps2.jl:34-224).
+ * set_projectors allocates nproj projectors (weight > 0, else TN_E_INVALID with
+ * the reference's message); set_projector_site uploads site k (1-based) of
+ * projector j (0-based), shape (chil, d_k, chir) of the env's dtype. */
+int tn_env_set_projectors(tn_env* env, int nproj, double weight);
+int tn_env_set_projector_site(tn_env* env, int j, int k, const void* A_host, long chil, long chir);
 /* shape of the projected-operator domain at the current position:
  * dims[0..3] = (chil, d_p, d_{p+1} or 1, chir); *n = element count */
 int tn_env_phi_dims(tn_env* env, long* dims4, long* n);

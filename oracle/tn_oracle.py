@@ -325,7 +325,7 @@ class Envs:
     makeL!(k) LR[k] holds sites 1..k; after makeR!(k) LR[k] holds sites k..N.
     Sites are 1-based as in the reference."""
 
-    def __init__(self, Ws: List[np.ndarray], dtype=np.float64):
+    def __init__(self, Ws: List[np.ndarray], dtype=np.float64, Ms=None, weight: float = 0.0):
         self.W = Ws
         self.N = len(Ws)
         self.LR: List[Optional[np.ndarray]] = [None] * (self.N + 2)
@@ -333,6 +333,10 @@ class Envs:
         self.rpos = self.N + 1
         self.nsite = 2
         self.dtype = dtype
+        # penalty projectors (ProjMPO_MPS2, src/mps/projmpo_mps2.jl:94-101)
+        self.Ms = Ms or []
+        self.weight = weight
+        self.m: List[np.ndarray] = []
 
     def lproj(self):
         return edge_env(self.dtype) if self.lpos <= 0 else self.LR[self.lpos]
@@ -370,15 +374,47 @@ class Envs:
         """position!(P, psi, pos) = makeL!(pos-1); makeR!(pos+nsite)."""
         self.makeL(A, pos - 1)
         self.makeR(A, pos + self.nsite)
+        self.m = [projector_local(A, M, pos, self.nsite) for M in self.Ms]
 
     def product(self, v):
         pos = self.lpos + 1
         L, R = self.lproj(), self.rproj()
         if self.nsite == 2:
-            return matvec2(L, self.W[pos - 1], self.W[pos], R, v)
-        if self.nsite == 1:
-            return matvec1(L, self.W[pos - 1], R, v)
-        raise ValueError("nsite must be 1 or 2")
+            out = matvec2(L, self.W[pos - 1], self.W[pos], R, v)
+        elif self.nsite == 1:
+            out = matvec1(L, self.W[pos - 1], R, v)
+        else:
+            raise ValueError("nsite must be 1 or 2")
+        for m in self.m:
+            out = out + self.weight * np.vdot(m, v) * m
+        return out
+
+
+def overlap_left(A, M, k):
+    """L[b, b'] = <M|psi> over sites 1..k (psi ket, M bra; projmps2.jl:77-92)."""
+    L = np.ones((1, 1), dtype=np.result_type(A[0], M[0]))
+    for j in range(k):
+        L = np.einsum("ab,asc,bsd->cd", L, A[j], M[j].conj())
+    return L
+
+
+def overlap_right(A, M, k):
+    """R[a, a'] over sites k..N (projmps2.jl:108-123)."""
+    N = len(A)
+    R = np.ones((1, 1), dtype=np.result_type(A[0], M[0]))
+    for j in range(N - 1, k - 2, -1):
+        R = np.einsum("asc,bsd,cd->ab", A[j], M[j].conj(), R)
+    return R
+
+
+def projector_local(A, M, pos, nsite):
+    """Projection m of M onto the local space at pos (1-based), such that
+    ProjMPS2 v = m (m^H v) (proj_mps and product, projmps2.jl:158-224)."""
+    L = overlap_left(A, M, pos - 1)
+    R = overlap_right(A, M, pos + nsite)
+    if nsite == 2:
+        return np.einsum("ab,bsx,xtd,cd->astc", L.conj(), M[pos - 1], M[pos], R.conj())
+    return np.einsum("ab,bsd,cd->asc", L.conj(), M[pos - 1], R.conj())
 
 
 # ---------------------------------------------------------------------------
@@ -684,9 +720,12 @@ class DMRGParams:
 class StateEnvs:
     """{psi::MPS, PH} container (src/mps/state_envs.jl:18-60)."""
 
-    def __init__(self, psi: List[np.ndarray], Ws: List[np.ndarray]):
+    def __init__(self, psi: List[np.ndarray], Ws: List[np.ndarray], Ms=None, weight: Optional[float] = None):
+        """StateEnvs(psi, H) or StateEnvs(psi, H, Ms; weight) (state_envs.jl:54-106)."""
+        if Ms is not None and (weight is None or weight <= 0.0):
+            raise ValueError(f"`weight` parameter should be > 0.0 (value passed was `weight={weight}`)")
         self.psi = [a.copy() for a in psi]
-        self.PH = Envs(Ws, dtype=psi[0].dtype)
+        self.PH = Envs(Ws, dtype=psi[0].dtype, Ms=Ms, weight=weight or 0.0)
         self.ortho_center: Optional[int] = None  # 1-based when known
 
     def __len__(self):
@@ -697,7 +736,7 @@ def _ensure_ortho_first(env: StateEnvs):
     if env.ortho_center != 1:
         env.psi = right_orthogonalize(env.psi)
         env.ortho_center = 1
-        env.PH = Envs(env.PH.W, dtype=env.psi[0].dtype)
+        env.PH = Envs(env.PH.W, dtype=env.psi[0].dtype, Ms=env.PH.Ms, weight=env.PH.weight)
 
 
 def update_two_site(env: StateEnvs, pos: int, ortho: str, *, solver: str = "eig",

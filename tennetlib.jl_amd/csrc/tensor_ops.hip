@@ -382,6 +382,33 @@ void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double*
   }
 }
 
+namespace {
+// y += scale * coef[0] * x (coef on the device; complex pair when cplx)
+__global__ __launch_bounds__(256) void add_scaled_dev_kernel(double* __restrict__ y, const double* __restrict__ x,
+                                                             const double* __restrict__ coef, double scale, long n,
+                                                             int cplx) {
+  const double cr = scale * coef[0], ci = cplx ? scale * coef[1] : 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (!cplx) {
+      y[i] = fma(cr, x[i], y[i]);
+    } else {
+      const double xr = x[2 * i], xi = x[2 * i + 1];
+      y[2 * i] += cr * xr - ci * xi;
+      y[2 * i + 1] += cr * xi + ci * xr;
+    }
+  }
+}
+
+}  // namespace
+
+void add_scaled_dev(tn_ctx* c, double* y, const double* x, const double* coef_dev, double scale, long n, bool cplx) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(add_scaled_dev_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, y, x, coef_dev, scale, n,
+                     cplx ? 1 : 0);
+  TN_HIP(hipGetLastError());
+}
+
 void multi_axpy_sub(tn_ctx* c, const double* basis, long ld, int count, const double* coef_dev, double* x,
                     long n, bool cplx) {
   if (n <= 0 || count <= 0) return;

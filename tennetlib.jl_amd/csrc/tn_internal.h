@@ -127,7 +127,8 @@ enum WsSlot {
   WS_T1 = 0, WS_T2, WS_PSI32, WS_OUT32, WS_KRY_BASIS, WS_KRY_W, WS_KRY_PART, WS_KRY_COEF,
   WS_SVD_X, WS_SVD_Q, WS_SVD_GRAM, WS_SVD_J, WS_SVD_CNT, WS_SVD_PAIRS, WS_SVD_S, WS_SVD_PERM,
   WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP,
-  WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_SLOTS
+  WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_PROJ_X, WS_PROJ_Y, WS_PROJ_C,
+  WS_SLOTS
 };
 
 // scoped phase timer (HIP events on the context stream)
@@ -218,6 +219,8 @@ void scale_f64(tn_ctx* c, double* x, double s, long n);
 void scaled_copy(tn_ctx* c, double* y, const double* x, double a, long n);
 // row sum of squares of a (rows x L) row-major matrix into out[rows]
 void row_norms2(tn_ctx* c, const double* X, long rows, long L, long ld, double* out);
+// y += scale * coef_dev[0] * x (complex coefficient pair when cplx)
+void add_scaled_dev(tn_ctx* c, double* y, const double* x, const double* coef_dev, double scale, long n, bool cplx);
 // out (rows x cols) = in^T  (in: cols x rows), doubles
 void transpose_f64(tn_ctx* c, const double* in, long in_rows, long in_cols, double* out);
 // out[k][j] = src[perm[k]][j] * scale[k]   (k < kcount, j < L)

@@ -104,6 +104,20 @@ struct EnvT {
   int w = 0;
 };
 
+// ProjMPS2 (src/mps/projmps2.jl:34-224): an MPS M held constant, its overlap
+// environments with psi and the local projection m of M at the current
+// position; the projected operator adds weight * m (m^H v) to H v
+// (ProjMPO_MPS2, src/mps/projmpo_mps2.jl:94-101).
+struct ProjT {
+  std::vector<SiteT> M;   // site k at [k-1]
+  std::vector<EnvT> LR;   // LR[k]: overlap environment (chi_psi x chi_M) of sites 1..k (left) or k..N (right)
+  std::vector<long> chiM; // bond of M of the environment LR[k]
+  int lpos = 0, rpos = 0;
+  DevBuf m;               // local projection at position m_pos for m_nsite sites
+  bool m_ok = false;
+  int m_pos = 0, m_nsite = 0;
+};
+
 struct tn_env {
   tn_ctx* ctx = nullptr;
   int N = 0;
@@ -127,6 +141,9 @@ struct tn_env {
   long split_key_l = -1, split_key_r = -1;
   long env_version = 0;
   std::vector<long> LR_version;
+  // penalty projectors (excited states): H v + weight sum_k m_k (m_k^H v)
+  std::vector<ProjT> proj;
+  double proj_weight = 0.0;
 
   bool cplx() const { return dtype == TN_C128; }
 };
@@ -326,6 +343,7 @@ void makeR(tn_env* e, int k) {
   }
   e->rpos = k;
 }
+void proj_position(tn_env* e, int pos);
 void position(tn_env* e, int pos) {
   TN_REQUIRE(pos >= 1 && pos + e->nsite - 1 <= e->N, TN_E_POSITION,
              "position " + std::to_string(pos) + " out of range for nsite=" + std::to_string(e->nsite));
@@ -333,11 +351,98 @@ void position(tn_env* e, int pos) {
   require_sites(e);
   makeL(e, pos - 1);
   makeR(e, pos + e->nsite);
+  if (!e->proj.empty()) proj_position(e, pos);
+}
+
+// overlap environments of a projector (mirror projmps2.jl:77-149):
+//   L_k[b, b'] = sum L_{k-1}[a, a'] psi_k[a, s, b] conj(M_k[a', s, b'])
+//   R_k[a, a'] = sum psi_k[a, s, b] conj(M_k[a', s, b']) R_{k+1}[b, b']
+void proj_edge(tn_env* e, ProjT& P, int k) {
+  EnvT& E = P.LR[k];
+  E.buf.ensure(16);
+  TN_HIP(hipMemcpyAsync(E.buf.p, e->ctx->ones, e->esz, hipMemcpyDeviceToDevice, e->ctx->stream));
+  E.chi = 1;
+  P.chiM[k] = 1;
+}
+void proj_update_left(tn_env* e, ProjT& P, int k) {
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const Prec pr = cx ? Prec::Z64 : Prec::F64;
+  if (k == 1) proj_edge(e, P, 0);
+  const EnvT& L = P.LR[k - 1];
+  const SiteT& A = e->A[k - 1];
+  const SiteT& M = P.M[k - 1];
+  const int d = e->d[k - 1];
+  TN_REQUIRE(L.chi == A.chil && P.chiM[k - 1] == M.chil, TN_E_SHAPE, "projector: bond dimensions disagree");
+  double* X = (double*)c->workspace(WS_PROJ_X, (size_t)A.chil * d * M.chir * e->esz);
+  GemmDesc g1;  // X[a, (s b')] = L[a, a'] conj(M)[a', (s b')]
+  g1.M = A.chil; g1.N = (long)d * M.chir; g1.K = M.chil;
+  g1.A = L.buf.p; g1.lda = M.chil;
+  g1.B = M.buf.p; g1.ldb = (long)d * M.chir; g1.cB = cx;
+  g1.C = X; g1.ldc = (long)d * M.chir;
+  gemm(c, pr, g1);
+  EnvT& Ln = P.LR[k];
+  Ln.buf.ensure((size_t)A.chir * M.chir * e->esz);
+  GemmDesc g2;  // L'[b, b'] = psi[(a s), b]^T X[(a s), b']
+  g2.M = A.chir; g2.N = M.chir; g2.K = A.chil * d;
+  g2.A = A.buf.p; g2.lda = A.chir; g2.tA = 1;
+  g2.B = X; g2.ldb = M.chir;
+  g2.C = Ln.buf.p; g2.ldc = M.chir;
+  gemm(c, pr, g2);
+  Ln.chi = A.chir;
+  P.chiM[k] = M.chir;
+}
+void proj_update_right(tn_env* e, ProjT& P, int k) {
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const Prec pr = cx ? Prec::Z64 : Prec::F64;
+  if (k == e->N) proj_edge(e, P, e->N + 1);
+  const EnvT& R = P.LR[k + 1];
+  const SiteT& A = e->A[k - 1];
+  const SiteT& M = P.M[k - 1];
+  const int d = e->d[k - 1];
+  TN_REQUIRE(R.chi == A.chir && P.chiM[k + 1] == M.chir, TN_E_SHAPE, "projector: bond dimensions disagree");
+  double* X = (double*)c->workspace(WS_PROJ_X, (size_t)M.chil * d * A.chir * e->esz);
+  GemmDesc g1;  // X[(a' s), b] = conj(M)[(a' s), b'] R[b, b']^T
+  g1.M = M.chil * d; g1.N = A.chir; g1.K = M.chir;
+  g1.A = M.buf.p; g1.lda = M.chir; g1.cA = cx;
+  g1.B = R.buf.p; g1.ldb = M.chir; g1.tB = 1;
+  g1.C = X; g1.ldc = A.chir;
+  gemm(c, pr, g1);
+  EnvT& Rn = P.LR[k];
+  Rn.buf.ensure((size_t)A.chil * M.chil * e->esz);
+  GemmDesc g2;  // R'[a, a'] = psi[a, (s b)] X[a', (s b)]^T
+  g2.M = A.chil; g2.N = M.chil; g2.K = (long)d * A.chir;
+  g2.A = A.buf.p; g2.lda = (long)d * A.chir;
+  g2.B = X; g2.ldb = (long)d * A.chir; g2.tB = 1;
+  g2.C = Rn.buf.p; g2.ldc = M.chil;
+  gemm(c, pr, g2);
+  Rn.chi = A.chil;
+  P.chiM[k] = M.chil;
+}
+void proj_position(tn_env* e, int pos) {
+  for (ProjT& P : e->proj) {
+    const int kl = pos - 1, kr = pos + e->nsite;
+    if (P.lpos >= kl) P.lpos = kl;
+    for (int ll = std::max(P.lpos, 0); ll < kl; ++ll) proj_update_left(e, P, ll + 1);
+    if (kl == 0) proj_edge(e, P, 0);
+    P.lpos = kl;
+    if (P.rpos <= kr) P.rpos = kr;
+    for (int rl = std::min(P.rpos, e->N + 1); rl > kr; --rl) proj_update_right(e, P, rl - 1);
+    if (kr == e->N + 1) proj_edge(e, P, e->N + 1);
+    P.rpos = kr;
+    P.m_ok = false;
+  }
 }
 
 void invalidate_site(tn_env* e, int k) {
   if (k <= e->lpos) e->lpos = k - 1;
   if (k >= e->rpos) e->rpos = k + 1;
+  for (ProjT& P : e->proj) {
+    if (k <= P.lpos) P.lpos = k - 1;
+    if (k >= P.rpos) P.rpos = k + 1;
+    P.m_ok = false;
+  }
 }
 
 // shape of the operator domain at the current position
@@ -420,7 +525,7 @@ SplitView split_shadow(tn_env* e, int k, bool left) {
 }
 
 // product(sysenv, v) (ProjMPO.product; mirror src/mps/projmps2.jl:159-173)
-void matvec(tn_env* e, const void* v, void* out, int prec) {
+void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   PhaseTimer pt(e->ctx, PH_MATVEC);
   tn_ctx* c = e->ctx;
   const Dom D = domain(e);
@@ -637,6 +742,67 @@ struct Krylov {
     lincomb(c, base, nd, K, dc, out, n, cx);
   }
 };
+
+// local projection of a projector at the current position (proj_mps,
+// projmps2.jl:177-194), conjugated: m[a, S, b] = sum conj(L[a, a']) M[a', S, b'] conj(R[b, b'])
+void proj_local(tn_env* e, ProjT& P, const Dom& D) {
+  if (P.m_ok && P.m_pos == D.pos && P.m_nsite == e->nsite) return;
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const Prec pr = cx ? Prec::Z64 : Prec::F64;
+  const int pos = D.pos;
+  const EnvT& L = P.LR[pos - 1];
+  const EnvT& R = P.LR[pos + e->nsite];
+  const long cml = P.chiM[pos - 1], cmr = P.chiM[pos + e->nsite];
+  const int DD = (e->nsite == 2) ? D.d1 * D.d2 : D.d1;
+  TN_REQUIRE(L.chi == D.chil && R.chi == D.chir, TN_E_SHAPE, "projector: environment shapes disagree");
+  const void* Mphi = P.M[pos - 1].buf.p;
+  if (e->nsite == 2) {
+    const SiteT& M1 = P.M[pos - 1];
+    const SiteT& M2 = P.M[pos];
+    double* Y = (double*)c->workspace(WS_PROJ_Y, (size_t)cml * DD * cmr * e->esz);
+    GemmDesc g;
+    g.M = cml * D.d1; g.N = (long)D.d2 * cmr; g.K = M1.chir;
+    g.A = M1.buf.p; g.lda = M1.chir;
+    g.B = M2.buf.p; g.ldb = (long)D.d2 * cmr;
+    g.C = Y; g.ldc = (long)D.d2 * cmr;
+    gemm(c, pr, g);
+    Mphi = Y;
+  }
+  double* X = (double*)c->workspace(WS_PROJ_X, (size_t)D.chil * DD * cmr * e->esz);
+  GemmDesc g1;  // X[a, (S b')] = conj(L)[a, a'] Mphi[a', (S b')]
+  g1.M = D.chil; g1.N = (long)DD * cmr; g1.K = cml;
+  g1.A = L.buf.p; g1.lda = cml; g1.cA = cx;
+  g1.B = Mphi; g1.ldb = (long)DD * cmr;
+  g1.C = X; g1.ldc = (long)DD * cmr;
+  gemm(c, pr, g1);
+  P.m.ensure((size_t)D.n * e->esz);
+  GemmDesc g2;  // m[(a S), b] = X[(a S), b'] conj(R)[b, b']^T
+  g2.M = D.chil * DD; g2.N = D.chir; g2.K = cmr;
+  g2.A = X; g2.lda = cmr;
+  g2.B = R.buf.p; g2.ldb = cmr; g2.tB = 1; g2.cB = cx;
+  g2.C = P.m.p; g2.ldc = D.chir;
+  gemm(c, pr, g2);
+  P.m_ok = true;
+  P.m_pos = D.pos;
+  P.m_nsite = e->nsite;
+}
+
+// product(ProjMPO_MPS2, v) = PH(v) + weight sum_k m_k (m_k^H v) (projmpo_mps2.jl:94-101)
+void matvec(tn_env* e, const void* v, void* out, int prec) {
+  matvec_mpo(e, v, out, prec);
+  if (e->proj.empty()) return;
+  tn_ctx* c = e->ctx;
+  const Dom D = domain(e);
+  const bool cx = e->cplx();
+  const long nd = cx ? 2 * D.n : D.n;
+  double* coef = (double*)c->workspace(WS_PROJ_C, 64 * sizeof(double));
+  for (ProjT& P : e->proj) {
+    proj_local(e, P, D);
+    multi_dot(c, P.m.as<double>(), nd, 1, (const double*)v, D.n, cx, coef);
+    add_scaled_dev(c, (double*)out, P.m.as<double>(), coef, e->proj_weight, D.n, cx);
+  }
+}
 
 void apply_op(tn_env* e, const double* x, double* y, int prec) { matvec(e, x, y, prec); }
 
@@ -1376,9 +1542,54 @@ int tn_env_position(tn_env* e, int pos) {
     position(e, pos);
   });
 }
+int tn_env_set_projectors(tn_env* e, int nproj, double weight) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(nproj >= 0, TN_E_INVALID, "set_projectors: negative count");
+    TN_REQUIRE(nproj == 0 || weight > 0.0, TN_E_INVALID,
+               "`weight` parameter should be > 0.0 (value passed was `weight=" + std::to_string(weight) + "`)");
+    e->proj.clear();
+    e->proj.resize(nproj);
+    for (ProjT& P : e->proj) {
+      P.M.resize(e->N);
+      P.LR.resize(e->N + 2);
+      P.chiM.assign(e->N + 2, 0);
+      P.lpos = 0;
+      P.rpos = e->N + 1;
+    }
+    e->proj_weight = weight;
+  });
+}
+
+int tn_env_set_projector_site(tn_env* e, int j, int k, const void* A, long chil, long chir) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(j >= 0 && j < (int)e->proj.size(), TN_E_INVALID, "set_projector_site: bad projector");
+    TN_REQUIRE(k >= 1 && k <= e->N && A, TN_E_INVALID, "set_projector_site: bad site");
+    TN_REQUIRE(chil >= 1 && chir >= 1 && (k > 1 || chil == 1) && (k < e->N || chir == 1), TN_E_SHAPE,
+               "set_projector_site: bad bond dimensions");
+    ProjT& P = e->proj[j];
+    SiteT& S = P.M[k - 1];
+    const size_t bytes = (size_t)chil * e->d[k - 1] * chir * e->esz;
+    S.buf.ensure(bytes);
+    TN_HIP(hipMemcpyAsync(S.buf.p, A, bytes, hipMemcpyHostToDevice, e->ctx->stream));
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+    S.chil = chil;
+    S.chir = chir;
+    if (k <= P.lpos) P.lpos = k - 1;
+    if (k >= P.rpos) P.rpos = k + 1;
+    P.m_ok = false;
+  });
+}
+
 int tn_env_reset(tn_env* e) {
   return guarded(C(e), [&] {
     check_env(e);
+    for (ProjT& P : e->proj) {
+      P.lpos = 0;
+      P.rpos = e->N + 1;
+      P.m_ok = false;
+    }
     e->lpos = 0;
     e->rpos = e->N + 1;
   });

@@ -91,6 +91,8 @@ SIGNATURES = {
     "tn_env_nsite": [_P, _PI],
     "tn_env_position": [_P, _I],
     "tn_env_reset": [_P],
+    "tn_env_set_projectors": [_P, _I, _D],
+    "tn_env_set_projector_site": [_P, _I, _I, _P, _L, _L],
     "tn_env_phi_dims": [_P, _PL, _PL],
     "tn_env_matvec": [_P, _P, _P, _I],
     "tn_env_contract_phi": [_P, _I, _P],

@@ -149,10 +149,18 @@ def _precision_code(p) -> int:
 
 class StateEnvs:
     """Holds the MPS ``psi`` and its projected operator ``PH`` on the GPU
-    (src/mps/state_envs.jl:18-60).  ``StateEnvs(psi, H)`` copies psi."""
+    (src/mps/state_envs.jl:18-60).  ``StateEnvs(psi, H)`` copies psi.
+    ``StateEnvs(psi, H, Ms, weight=w)`` adds the penalty projectors of
+    previously found states (ProjMPO_MPS2, state_envs.jl:83-106), for
+    excited states."""
 
-    def __init__(self, psi: Sequence[np.ndarray], H: Sequence[np.ndarray], ctx: Optional[Context] = None,
-                 dtype=None):
+    def __init__(self, psi: Sequence[np.ndarray], H: Sequence[np.ndarray], Ms=None, weight: Optional[float] = None,
+                 ctx: Optional[Context] = None, dtype=None):
+        if Ms is not None:
+            if weight is None:
+                raise TypeError("StateEnvs(psi, H, Ms; weight): keyword `weight` is required")
+            if weight <= 0.0:
+                raise ValueError(f"`weight` parameter should be > 0.0 (value passed was `weight={weight}`)")
         self.ctx = ctx or default_context()
         N = len(psi)
         if len(H) != N:
@@ -163,7 +171,9 @@ class StateEnvs:
                 raise ValueError(f"StateEnvs: site indices of MPO and MPS differ at site {k + 1}")
         w = [int(H[0].shape[0])] + [int(W.shape[3]) for W in H]
         if dtype is None:
-            dtype = np.complex128 if any(np.iscomplexobj(a) for a in psi) else np.float64
+            cx = any(np.iscomplexobj(a) for a in psi)
+            cx = cx or any(np.iscomplexobj(a) for M in (Ms or []) for a in M)
+            dtype = np.complex128 if cx else np.float64
         self.dtype = np.dtype(dtype)
         dt = L.TN_C128 if self.dtype == np.complex128 else L.TN_F64
         h = C.c_void_p()
@@ -181,6 +191,19 @@ class StateEnvs:
         for k, A in enumerate(psi, start=1):
             self.set_site(k, A)
         self.H = [np.array(W) for W in H]
+        self.Ms = None
+        if Ms is not None:
+            self.Ms = [[np.array(a) for a in M] for M in Ms]
+            self.ctx.check(self.ctx.lib.tn_env_set_projectors(self.h, len(Ms), float(weight)))
+            for j, M in enumerate(Ms):
+                if len(M) != N:
+                    raise ValueError("StateEnvs: projector MPS length differs from psi")
+                for k, A in enumerate(M, start=1):
+                    if A.shape[1] != d[k - 1]:
+                        raise ValueError(f"StateEnvs: site index of projector {j + 1} differs at site {k}")
+                    A = np.ascontiguousarray(A, dtype=self.dtype)
+                    self.ctx.check(self.ctx.lib.tn_env_set_projector_site(
+                        self.h, j, k, A.ctypes.data_as(C.c_void_p), A.shape[0], A.shape[2]))
 
     def close(self):
         if self.h and self.ctx.h:
@@ -369,6 +392,8 @@ def fullsweep_(sysenv: StateEnvs, solver, nsite_: int, swdata: SweepData, **kwar
     Returns (dE, dS)."""
     outputlevel = kwargs.get("outputlevel", 1)
     noise = kwargs.get("noise", 0.0)
+    if nsite_ != 2:
+        raise NotImplementedError("`fullsweep!()` with nsite=1 is not built on the GPU path")
     if sysenv.orthocenter != 1:
         orthogonalize_(sysenv, 1)
     N = len(sysenv)
@@ -480,16 +505,24 @@ def dmrg_(sysenv: StateEnvs, params: DMRGParams, nsite_: int = 2, **kwargs) -> S
     return swdata
 
 
-def dmrg(psi0, H, params: DMRGParams, nsite_: int = 2, **kwargs):
-    """dmrg(psi0, H, params, nsite) (src/mps/dmrg.jl:231-236) -> (energy, psi)."""
-    sysenv = StateEnvs(psi0, H)
+def dmrg(psi0, H, *args, **kwargs):
+    """dmrg(psi0, H, params, nsite) (src/mps/dmrg.jl:231-236) or, for excited
+    states, dmrg(psi0, H, Ms, params, nsite; weight) (src/mps/dmrg.jl:238-245)
+    -> (energy, psi)."""
+    if args and isinstance(args[0], DMRGParams):
+        Ms, params, rest = None, args[0], args[1:]
+    else:
+        Ms, params, rest = args[0], args[1], args[2:]
+    nsite_ = rest[0] if rest else kwargs.pop("nsite", 2)
+    weight = kwargs.pop("weight", -1.0)
+    sysenv = StateEnvs(psi0, H) if Ms is None else StateEnvs(psi0, H, Ms, weight=weight)
     sw = dmrg_(sysenv, params, nsite_, **kwargs)
     return sw.energy[-1], sysenv.psi
 
 
-def dmrg2(psi0, H, params: DMRGParams, **kwargs):
-    """dmrg2 (src/mps/dmrg.jl:295-297)."""
-    return dmrg(psi0, H, params, 2, **kwargs)
+def dmrg2(psi0, H, *args, **kwargs):
+    """dmrg2 (src/mps/dmrg.jl:295-297): dmrg(..., 2)."""
+    return dmrg(psi0, H, *args, 2, **kwargs)
 
 
 # ---------------------------------------------------------------------------

@@ -127,6 +127,64 @@ def test_matvec_complex_matches_oracle(T):
             assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+def test_projector_matvec_matches_oracle(T, cplx):
+    """H v + w sum_k m_k (m_k^H v) with two penalty MPS of other bond
+    dimensions; positions sweep right and back so the overlap environments
+    are both built and reused."""
+    N, w = 6, 2.5
+    Ws = O.xxz_mpo(N, 1.0, 0.7, np.linspace(-0.4, 0.6, N))
+    A = rand_mps(N, 2, [2, 4, 33, 4, 2], seed=11, cplx=cplx)
+    Ms = [rand_mps(N, 2, [2, 3, 7, 3, 2], seed=12, cplx=cplx),
+          rand_mps(N, 2, [2, 4, 16, 4, 2], seed=13, cplx=cplx)]
+    dt = np.complex128 if cplx else np.float64
+    env = T.StateEnvs(A, Ws, Ms, weight=w)
+    ref = O.Envs(Ws, dtype=dt, Ms=Ms, weight=w)
+    rng = np.random.default_rng(14)
+    for nsite, positions in ((2, [1, 3, 5, 4, 2]), (1, [2, 6, 3])):
+        T.set_nsite_(env, nsite)
+        ref.nsite = nsite
+        for p in positions:
+            print(f"projector cplx={cplx} nsite={nsite} pos={p}", flush=True)
+            T.position_(env, p)
+            ref.position(A, p)
+            dims, n = env.phi_dims()
+            shape = dims if nsite == 2 else (dims[0], dims[1], dims[3])
+            psi = rng.standard_normal(shape)
+            if cplx:
+                psi = psi + 1j * rng.standard_normal(shape)
+            want = ref.product(psi)
+            got = T.product(env, psi).reshape(want.shape)
+            assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), (nsite, p)
+
+
+def test_projector_weight_error(T):
+    psi, Ws = O.neel_mps(4), O.heisenberg_mpo(4)
+    with pytest.raises(ValueError, match="should be > 0.0"):
+        T.StateEnvs(psi, Ws, [psi], weight=-1.0)
+
+
+def test_excited_state_dmrg_n12_reaches_first_excited_level(T):
+    """Both excited-state entry points of the reference's tests
+    (test_MPS_DMRG.jl:71-97, weight 10): StateEnvs(psi0, H, [psi_gr]; weight)
+    + dmrg!, and dmrg(psi0, H, [psi_gr], params, nsite; weight).  The target
+    is the lowest triplet of the N=12 Heisenberg chain by exact
+    diagonalisation, and the result is orthogonal to the ground state."""
+    N, E1 = 12, -4.86114793703639
+    Ws = O.heisenberg_mpo(N)
+    params = T.make_dmrg_params(maxdim=[64], nsweeps=[4], cutoff=1e-14)
+    e0, gs = T.dmrg(O.neel_mps(N), Ws, params, 2, outputlevel=0)
+    assert abs(e0 - (-5.14209063284054)) < 1e-10
+    params = T.make_dmrg_params(maxdim=[64], nsweeps=[6], cutoff=1e-14)
+    env = T.StateEnvs(O.random_product_mps(N, 2, seed=3), Ws, [gs], weight=10.0)
+    sw = T.dmrg_(env, params, 2, outputlevel=0)
+    e1, ex = T.dmrg(O.random_product_mps(N, 2, seed=3), Ws, [gs], params, 2, weight=10.0, outputlevel=0)
+    for e, psi in ((sw.energy[-1], env.psi), (e1, ex)):
+        assert abs(e - E1) < 1e-8
+        assert abs(np.vdot(O.mps_to_dense(gs), O.mps_to_dense(psi))) < 1e-5
+        assert abs(O.mps_energy(psi, Ws) - E1) < 1e-8
+
+
 def _svd_gpu(T, M, maxdim=10 ** 9, mindim=1, cutoff=-1.0):
     import ctypes as C
     ctx = T.default_context()

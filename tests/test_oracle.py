@@ -187,6 +187,45 @@ def test_matvec1_equals_projected_hamiltonian(p):
     assert np.abs(env.product(psi).reshape(-1) - Heff @ psi.reshape(-1)).max() < 1e-12
 
 
+def _complex_mps(N, d, chis, seed):
+    rng = np.random.default_rng(seed)
+    dims = [1] + list(chis) + [1]
+    return [rng.standard_normal((dims[k], d, dims[k + 1]))
+            + 1j * rng.standard_normal((dims[k], d, dims[k + 1])) for k in range(N)]
+
+
+@pytest.mark.parametrize("nsite,p", [(2, 1), (2, 3), (2, 5), (1, 1), (1, 4), (1, 6)])
+def test_penalty_projector_equals_projected_operator(nsite, p):
+    """H + w sum_k |M_k><M_k| projected on the local space of a complex MPS
+    equals the projector-augmented product (projmpo_mps2.jl:94-101 with
+    projmps2.jl:158-224)."""
+    N, d, w = 6, 2, 3.5
+    Ws = O.heisenberg_mpo(N)
+    A = _mixed_canonical(_complex_mps(N, d, [2, 4, 5, 4, 2], seed=40 + p), p, nsite)
+    Ms = [_complex_mps(N, d, [2, 3, 4, 3, 2], seed=50 + p + k) for k in range(2)]
+    env = O.Envs(Ws, dtype=np.complex128, Ms=Ms, weight=w)
+    env.nsite = nsite
+    env.position(A, p)
+    Lm, Rm = _left_iso(A, p), _right_iso(A, p + nsite - 1)
+    P = np.kron(np.kron(Lm, np.eye(d ** nsite)), Rm.T)
+    Hfull = xxz_dense(N).astype(complex)
+    for M in Ms:
+        m = O.mps_to_dense(M).reshape(-1)
+        Hfull = Hfull + w * np.outer(m, m.conj())
+    Heff = P.conj().T @ Hfull @ P
+    rng = np.random.default_rng(p)
+    shape = (Lm.shape[1],) + (d,) * nsite + (Rm.shape[0],)
+    psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    got = env.product(psi).reshape(-1)
+    assert np.abs(got - Heff @ psi.reshape(-1)).max() < 1e-11 * np.abs(Heff).max()
+
+
+def test_penalty_weight_must_be_positive():
+    psi, Ws = O.neel_mps(4), O.heisenberg_mpo(4)
+    with pytest.raises(ValueError, match="should be > 0.0"):
+        O.StateEnvs(psi, Ws, [psi], weight=0.0)
+
+
 # --------------------------------------------------------------------------
 # Krylov solvers
 # --------------------------------------------------------------------------
@@ -309,6 +348,22 @@ def test_dmrg_n12_reaches_ed():
     # the reported energy is the Ritz value of the last update; the state's
     # own energy agrees
     assert abs(O.mps_energy(env.psi, O.heisenberg_mpo(12)) - sw.energy[-1]) < 1e-10
+
+
+def test_excited_state_dmrg_n12_reaches_first_excited_level():
+    """Penalty DMRG with weight 10 against the ground state (the reference's
+    dmrg_ex_1 / dmrg_ex_2 pattern, test_MPS_DMRG.jl:71-97) reaches the
+    lowest triplet, E1 = -4.86114793703639 by exact diagonalisation, and the
+    result is orthogonal to the ground state."""
+    Ws = O.heisenberg_mpo(12)
+    env = O.StateEnvs(O.neel_mps(12), Ws)
+    O.dmrg(env, O.DMRGParams.make([64], [4], cutoff=1e-14))
+    gs = [a.copy() for a in env.psi]
+    ex = O.StateEnvs(O.random_product_mps(12, 2, seed=3), Ws, [gs], weight=10.0)
+    sw = O.dmrg(ex, O.DMRGParams.make([64], [6], cutoff=1e-14))
+    assert abs(sw.energy[-1] - (-4.86114793703639)) < 1e-8
+    ov = np.vdot(O.mps_to_dense(gs), O.mps_to_dense(ex.psi))
+    assert abs(ov) < 1e-5
 
 
 def test_golden_c1_heisenberg_n20():
