@@ -87,6 +87,8 @@ typedef struct tn_update_params {
   int check_convergence;
   int precision;       /* TN_PREC_*: precision of the Krylov matvecs          */
   int refine_fp64;     /* energy from one FP64 matvec of the converged vector */
+  double noise;        /* 0: density-matrix perturbation noise * noiseterm(PH, phi, ortho)
+                          added before the factorisation (update_site.jl:60-62)         */
 } tn_update_params;
 
 typedef struct tn_update_result {
@@ -168,10 +170,15 @@ int tn_krylov_expm(tn_env* env, void* phi_dev, double t_re, double t_im, int kry
 int tn_rayleigh_fp64(tn_env* env, const void* phi_dev, double* e2);
 
 /* ---- bond replacement and truncated SVD -------------------------------- */
-/* replacebond!(psi, b, phi; maxdim, mindim, cutoff, ortho, normalize):
- * eigs_out receives the kept weights P = s^2 (capacity eigs_cap). */
+/* replacebond!(psi, b, phi; maxdim, mindim, cutoff, ortho, normalize,
+ * eigen_perturbation = noise * noiseterm(PH, phi, ortho)) (update_site.jl:59-76;
+ * the noise term is ITensorMPS noiseterm, in-tree mirror
+ * projcouplingmodel.jl:391-492).  noise = 0 is the plain factorisation; a
+ * nonzero noise needs the environment positioned at b with nsite = 2.
+ * eigs_out receives the kept weights P (eigenvalues of the density matrix,
+ * s^2 without noise; capacity eigs_cap). */
 int tn_replacebond(tn_env* env, int b, const void* phi_dev, int ortho_left, long maxdim,
-                   long mindim, double cutoff, int normalize, double* truncerr,
+                   long mindim, double cutoff, double noise, int normalize, double* truncerr,
                    double* eigs_out, long eigs_cap, long* nkept);
 /* Standalone truncated SVD of a real m x n row-major device matrix:
  * M = U S Vt with U (m x k) and Vt (k x n) row-major on the device, S on the

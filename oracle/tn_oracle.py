@@ -649,6 +649,72 @@ def svd_truncate(M: np.ndarray, maxdim: int, mindim: int, cutoff: float):
     return U[:, :n], S[:n], Vh[:n, :], Spectrum(P[:n].copy(), tr.truncerr)
 
 
+def noiseterm_factor(env: "Envs", phi: np.ndarray, ortho: str) -> np.ndarray:
+    """Factor X of the density-matrix perturbation of the two-site update,
+    noiseterm(PH, phi, ortho) = X X^H over the indices that stay on the
+    orthonormal side (ITensorMPS 0.3 noiseterm(::AbstractProjMPO), called at
+    src/mps/update_site.jl:60-62; the in-tree mirror for coupling models is
+    src/mps/projcouplingmodel.jl:391-492):
+      left:  X[(a' s'), (w t b)] = sum L[a, w0, a'] W1[w0, s', s, w] phi[a, s, t, b]
+      right: X[(a s w), (t' b')] = sum phi[a, s, t, b] W2[w, t', t, w2] R[b, w2, b']
+    Rows (left) or columns (right) are the indices of the kept factor; the
+    perturbation in the convention of replacebond below is X X^H (left) and
+    X^T conj(X) (right)."""
+    if env.nsite != 2:
+        raise ValueError("noise term only defined for 2-site ProjMPO")
+    pos = env.lpos + 1
+    a, s, t, c = phi.shape
+    if ortho == "left":
+        T = np.tensordot(env.lproj(), phi, axes=([0], [0]))        # [w0, a', s, t, b]
+        T = np.tensordot(T, env.W[pos - 1], axes=([0, 2], [0, 2]))  # [a', t, b, s', w]
+        return T.transpose(0, 3, 4, 1, 2).reshape(a * s, -1)
+    if ortho == "right":
+        T = np.tensordot(phi, env.W[pos], axes=([2], [2]))          # [a, s, b, w, t', w2]
+        T = np.tensordot(T, env.rproj(), axes=([2, 5], [0, 1]))     # [a, s, w, t', b']
+        return T.reshape(-1, t * c)
+    raise ValueError(f"In noiseterm, got ortho = {ortho}, only supports `left` and `right`")
+
+
+def noiseterm(env: "Envs", phi: np.ndarray, ortho: str) -> np.ndarray:
+    X = noiseterm_factor(env, phi, ortho)
+    return X @ X.conj().T if ortho == "left" else X.T @ X.conj()
+
+
+def replacebond_perturbed(A: List[np.ndarray], b: int, phi: np.ndarray, maxdim: int, mindim: int,
+                          cutoff: float, ortho: str, normalize: bool, drho: np.ndarray) -> Spectrum:
+    """replacebond!(psi, b, phi; eigen_perturbation=drho, ortho): the
+    density-matrix factorisation (ITensors 0.9 factorize_eigen).  The
+    orthonormal factor is the top-n eigenvectors of rho = phi phi^H + drho
+    (left; rho = phi^T conj(phi) + drho on the right), the other factor is its
+    exact projection of phi, normalised; the spectrum is that of rho."""
+    a, s, t, c = phi.shape
+    M = phi.reshape(a * s, t * c)
+    rho = M @ M.conj().T if ortho == "left" else M.T @ M.conj()
+    rho = rho + drho
+    rho = 0.5 * (rho + rho.conj().T)
+    w, V = np.linalg.eigh(rho)
+    w, V = w[::-1], V[:, ::-1]
+    P = np.maximum(w, 0.0)
+    tr = truncate_weights(P, maxdim, mindim, cutoff)
+    n = tr.n
+    U = V[:, :n]
+    if ortho == "left":
+        R = U.conj().T @ M
+        if normalize:
+            R = R / np.linalg.norm(R)
+        A[b] = U.reshape(a, s, n)
+        A[b + 1] = R.reshape(n, t, c)
+    elif ortho == "right":
+        Lf = M @ U.conj()
+        if normalize:
+            Lf = Lf / np.linalg.norm(Lf)
+        A[b] = Lf.reshape(a, s, n)
+        A[b + 1] = U.T.reshape(n, t, c)
+    else:
+        raise ValueError("ortho must be 'left' or 'right'")
+    return Spectrum(P[:n].copy(), tr.truncerr)
+
+
 def replacebond(A: List[np.ndarray], b: int, phi: np.ndarray, maxdim: int, mindim: int,
                 cutoff: float, ortho: str, normalize: bool = True) -> Spectrum:
     """replacebond!(psi, b, phi; ortho) for 0-based bond b (sites b, b+1):
@@ -742,7 +808,7 @@ def _ensure_ortho_first(env: StateEnvs):
 def update_two_site(env: StateEnvs, pos: int, ortho: str, *, solver: str = "eig",
                     time_step=None, maxdim=2 ** 62, mindim=1, cutoff=FLOAT64_THRESHOLD,
                     normalize=True, krylovdim=5, maxiter=2, tol=1e-14,
-                    reverse_step=False):
+                    reverse_step=False, noise=0.0):
     """_update_two_site! (src/mps/update_site.jl:27-90); pos is 1-based."""
     N = len(env)
     if not (0 < pos < N):
@@ -760,7 +826,11 @@ def update_two_site(env: StateEnvs, pos: int, ortho: str, *, solver: str = "eig"
         phi = phi / np.linalg.norm(phi)
     if math.isnan(energy):
         energy = float(np.real(np.vdot(phi, env.PH.product(phi))))
-    spec = replacebond(A, pos - 1, phi, maxdim, mindim, cutoff, ortho, normalize)
+    if abs(noise) > FLOAT64_THRESHOLD:
+        drho = noise * noiseterm(env.PH, phi, ortho)
+        spec = replacebond_perturbed(A, pos - 1, phi, maxdim, mindim, cutoff, ortho, normalize, drho)
+    else:
+        spec = replacebond(A, pos - 1, phi, maxdim, mindim, cutoff, ortho, normalize)
     env.ortho_center = pos + 1 if ortho == "left" else pos
     halfsweep_done = (pos == 1 and ortho == "right") or (pos == N - 1 and ortho == "left")
     if reverse_step and not halfsweep_done:
@@ -804,18 +874,22 @@ def fullsweep(env: StateEnvs, swdata: SweepData, **kw):
 
 def dmrg(env: StateEnvs, params: DMRGParams, *, krylovdim=5, maxiter=2, tol=1e-14,
          energyErrGoal=None, normalize=True) -> SweepData:
-    """dmrg!(sysenv, params, 2) (src/mps/dmrg.jl:148-227), noise = 0 path."""
+    """dmrg!(sysenv, params, 2) (src/mps/dmrg.jl:148-227), with the noise
+    schedule of dmrg.jl:196-205."""
     sw = SweepData()
     for ii in range(len(params.nsweeps)):
         noise = params.noise[ii]
-        if noise != 0.0:
-            raise NotImplementedError("noise > 0 (density-matrix perturbation) is not in the oracle")
-        for _ in range(params.nsweeps[ii]):
+        for jj in range(1, params.nsweeps[ii] + 1):
             enerr, _ = fullsweep(env, sw, maxdim=params.maxdim[ii], cutoff=params.cutoff[ii],
                                  krylovdim=krylovdim, maxiter=maxiter, tol=tol,
-                                 normalize=normalize)
-            if energyErrGoal is not None and abs(enerr) < abs(energyErrGoal):
+                                 normalize=normalize, noise=noise)
+            if energyErrGoal is not None and abs(enerr) < abs(energyErrGoal) and abs(noise) < FLOAT64_THRESHOLD:
                 break
+            if jj == params.disable_noise_after[ii]:
+                noise = 0.0
+            noise /= params.noisedecay[ii]
+            if noise < 100 * FLOAT64_THRESHOLD:
+                noise = 0.0
     return sw
 
 

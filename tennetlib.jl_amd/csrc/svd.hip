@@ -499,7 +499,8 @@ void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bo
   TN_HIP(hipStreamSynchronize(c->stream));
 }
 
-void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out) {
+void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out, const void* X, long kx,
+              double xs) {
   TN_REQUIRE(m > 0 && n > 0, TN_E_INVALID, "svd: empty matrix");
   const long p = left ? m : n;
   const long ppad = ((p + PR - 1) / PR) * PR;
@@ -515,6 +516,14 @@ void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, Gr
     g.K = m; g.A = M; g.lda = n; g.tA = 1; g.cA = cplx; g.B = M; g.ldb = n;
   }
   gemm(c, cplx ? Prec::Z64 : Prec::F64, g);
+  if (X && kx > 0 && xs != 0.0) {  // density-matrix perturbation
+    GemmDesc h;
+    h.M = p; h.N = p; h.K = kx; h.C = G; h.ldc = ppad;
+    h.alpha = xs; h.beta = 1.0;
+    if (left) { h.A = X; h.lda = kx; h.B = X; h.ldb = kx; h.tB = 1; h.cB = cplx; }
+    else      { h.A = X; h.lda = p; h.tA = 1; h.cA = cplx; h.B = X; h.ldb = p; }
+    gemm(c, cplx ? Prec::Z64 : Prec::F64, h);
+  }
   out.p = p;
   out.ppad = ppad;
   out.left = left;

@@ -1027,12 +1027,79 @@ void sorted_weights(const std::vector<double>& w, long r, std::vector<int>& orde
   for (long i = 0; i < r; ++i) P[i] = std::max(0.0, w[order[i]]);
 }
 
-// replacebond!(psi, b, phi; maxdim, mindim, cutoff, ortho, normalize)
-// ortho = "left":  A_b = top-k eigenvectors of phi phi^T, A_{b+1} = A_b^T phi / |.|
-// ortho = "right": A_{b+1} = top-k eigenvectors of phi^T phi (rows), A_b = phi A_{b+1}^T / |.|
+// Factor X of the density-matrix perturbation noiseterm(PH, phi, ortho) of a
+// two-site update at bond b (ITensorMPS 0.3 noiseterm(::AbstractProjMPO),
+// called at src/mps/update_site.jl:60-62; the in-tree mirror for coupling
+// models is src/mps/projcouplingmodel.jl:391-492): the operator applied up to
+// the middle MPO bond w,
+//   left:  X[(a' s'), (w t b)] = sum L[w0][a'][a] W_b[w0, s', s, w] phi[a, s, t, b]
+//   right: X[(w a s), (t' b')] = sum phi[a, s, t, b] W_{b+1}[w, t', t, w2] R[w2][b][b']
+// and the perturbation is X X^H (left) or X^H X (right) in the convention of
+// gram_eig.  Needs the projected operator positioned at b with nsite = 2.
+const void* noise_factor(tn_env* e, int b, const double* phi, bool left, long* kx) {
+  tn_ctx* c = e->ctx;
+  TN_REQUIRE(e->nsite == 2 && e->lpos == b - 1 && e->rpos == b + 2, TN_E_POSITION,
+             "noise term only defined for 2-site ProjMPO positioned at the bond");
+  const bool cx = e->cplx();
+  const long m2 = cx ? 2 : 1;
+  const Prec P = gemm_prec_full(e);
+  const long chil = e->A[b - 1].chil, chir = e->A[b].chir;
+  const long d1 = e->d[b - 1], d2 = e->d[b];
+  const long w0 = e->w[b - 1], wm = e->w[b], w2 = e->w[b + 1];
+  const size_t es = e->esz;
+  if (left) {
+    const EnvT& L = e->LR[b - 1];
+    TN_REQUIRE(L.chi == chil && L.w == w0, TN_E_SHAPE, "noise term: left environment does not match phi");
+    void* X1 = c->workspace(WS_ENV_X, (size_t)(w0 * chil * d1 * d2 * chir) * es);
+    void* X2 = c->workspace(WS_ENV_Y, (size_t)(chil * d1 * wm * d2 * chir) * es);
+    GemmDesc g;  // X1[(w0 a'), (s t b)] = L[(w0 a'), a] phi[a, (s t b)]
+    g.M = w0 * chil; g.N = d1 * d2 * chir; g.K = chil;
+    g.A = L.buf.p; g.lda = chil;
+    g.B = phi; g.ldb = g.N;
+    g.C = X1; g.ldc = g.N;
+    gemm(c, P, g);
+    WStepDesc ws;  // (w0, s) -> (w, s'), output rows (a' s'), columns (w t b)
+    ws.G = (int)w0; ws.Gp = (int)wm; ws.D = (int)d1; ws.Dp = (int)d1;
+    ws.nA = chil; ws.nB = d2 * chir * m2;
+    ws.in = X1; ws.in_sg = chil * d1 * d2 * chir * m2; ws.in_sA = d1 * d2 * chir * m2; ws.in_sS = d2 * chir * m2;
+    ws.out = X2; ws.out_sg = d2 * chir * m2; ws.out_sA = d1 * wm * d2 * chir * m2; ws.out_sS = wm * d2 * chir * m2;
+    ws.op = &op_left(e, b);
+    wstep(c, false, ws);
+    *kx = wm * d2 * chir;
+    return X2;
+  }
+  const EnvT& R = e->LR[b + 2];
+  TN_REQUIRE(R.chi == chir && R.w == w2, TN_E_SHAPE, "noise term: right environment does not match phi");
+  void* X1 = c->workspace(WS_ENV_X, (size_t)(w2 * chil * d1 * d2 * chir) * es);
+  void* X2 = c->workspace(WS_ENV_Y, (size_t)(wm * chil * d1 * d2 * chir) * es);
+  GemmDesc g;  // X1[w2][(a s t), b'] = phi[(a s t), b] R[w2][b][b']
+  g.M = chil * d1 * d2; g.N = chir; g.K = chir;
+  g.A = phi; g.lda = chir; g.sA = 0;
+  g.B = R.buf.p; g.ldb = chir; g.sB = chir * chir;
+  g.C = X1; g.ldc = chir; g.sC = chil * d1 * d2 * chir;
+  g.batch = (int)w2;
+  gemm(c, P, g);
+  WStepDesc ws;  // (w2, t) -> (w, t'), output rows (w a s), columns (t' b')
+  ws.G = (int)w2; ws.Gp = (int)wm; ws.D = (int)d2; ws.Dp = (int)d2;
+  ws.nA = chil * d1; ws.nB = chir * m2;
+  ws.in = X1; ws.in_sg = chil * d1 * d2 * chir * m2; ws.in_sA = d2 * chir * m2; ws.in_sS = chir * m2;
+  ws.out = X2; ws.out_sg = chil * d1 * d2 * chir * m2; ws.out_sA = d2 * chir * m2; ws.out_sS = chir * m2;
+  ws.op = &op_right(e, b + 1);
+  wstep(c, false, ws);
+  *kx = wm * chil * d1;
+  return X2;
+}
+
+// replacebond!(psi, b, phi; maxdim, mindim, cutoff, ortho, normalize, eigen_perturbation)
+// ortho = "left":  A_b = top-k eigenvectors of rho = phi phi^T, A_{b+1} = A_b^T phi / |.|
+// ortho = "right": A_{b+1} = top-k eigenvectors of rho = phi^T phi (rows), A_b = phi A_{b+1}^T / |.|
 // Same factors as U, S V^T (left) or U S, V^T (right) of the truncated SVD.
+// noise != 0 adds noise * noiseterm(PH, phi, ortho) to rho (ITensors 0.9
+// factorize_eigen with eigen_perturbation); the spectrum is then that of the
+// perturbed rho and the second factor is normalised by its own norm.
 void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
-                 bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept) {
+                 bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept,
+                 double noise = 0.0) {
   PhaseTimer pt(e->ctx, PH_SVD);
   TN_REQUIRE(b >= 1 && b < e->N, TN_E_POSITION, "replacebond!: bond out of range");
   tn_ctx* c = e->ctx;
@@ -1042,16 +1109,20 @@ void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxd
   const int d1 = e->d[b - 1], d2 = e->d[b];
   const long m = chil * d1, n = (long)d2 * chir;
   GramEig ge;
-  gram_eig(c, phi, m, n, ortho_left, cx, ge);
+  const bool perturbed = std::fabs(noise) > 1e-15;
+  long kx = 0;
+  const void* X = perturbed ? noise_factor(e, b, phi, ortho_left, &kx) : nullptr;
+  gram_eig(c, phi, m, n, ortho_left, cx, ge, X, kx, perturbed ? noise : 0.0);
   std::vector<int> order;
   std::vector<double> P;
-  sorted_weights(ge.w, std::min(m, n), order, P);
+  // rank(phi phi^H) <= min(m, n); the perturbed density matrix has full rank p
+  sorted_weights(ge.w, perturbed ? ge.p : std::min(m, n), order, P);
   const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
   const long k = tr.n;
   double nrm = 0.0;
   for (long i = 0; i < k; ++i) nrm += P[i];
   nrm = std::sqrt(nrm);
-  const double inv_norm = (normalize && nrm > 0.0) ? 1.0 / nrm : 1.0;
+  const double inv_norm = (normalize && nrm > 0.0 && !perturbed) ? 1.0 / nrm : 1.0;
   int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
   double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
   std::vector<double> ones(k, 1.0);
@@ -1086,6 +1157,16 @@ void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxd
     g.C = Sa.buf.p; g.ldc = k;
     g.alpha = inv_norm;
     gemm(c, P64, g);
+  }
+  if (perturbed && normalize) {  // the second factor is no longer sqrt(sum P) in norm
+    double* F = ortho_left ? Sb.buf.as<double>() : Sa.buf.as<double>();
+    const long nel = ortho_left ? k * n : m * k;
+    double* co = (double*)c->workspace(WS_PROJ_C, 64 * sizeof(double));
+    multi_dot(c, F, nel * w2, 1, F, nel, cx, co);
+    TN_HIP(hipMemcpyAsync(c->pinned, co, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+    const double f2 = c->pinned[0];
+    if (f2 > 0.0) scale_f64(c, F, 1.0 / std::sqrt(f2), nel * w2);
   }
   TN_HIP(hipStreamSynchronize(c->stream));  // host vectors above go out of scope
   Sa.chir = k;
@@ -1211,6 +1292,8 @@ void update_two_site(tn_env* e, int pos, bool ortho_left, const tn_update_params
   TN_REQUIRE(pos > 0 && pos < e->N, TN_E_POSITION, "update_position!: bond out of range");
   TN_REQUIRE(!(p->solver == 0 && p->reverse_step), TN_E_INVALID,
              "update_position!: reverse_step needs a time step (exp_solver)");
+  TN_REQUIRE(!(std::fabs(p->noise) > 1e-15 && p->reverse_step), TN_E_INVALID,
+             "`updatePosition()` :: `noise` cannot be greater than zero for `reverse_step=true` !!");
   TN_REQUIRE((ortho_left && e->ortho == pos) || (!ortho_left && e->ortho == pos + 1), TN_E_POSITION,
              "update_position!: orthocenter is " + std::to_string(e->ortho) + ", expected " +
                  std::to_string(ortho_left ? pos : pos + 1));
@@ -1246,7 +1329,8 @@ void update_two_site(tn_env* e, int pos, bool ortho_left, const tn_update_params
   double terr = 0.0;
   long nk = 0;
   std::vector<double> eigs;
-  replacebond(e, pos, phi, ortho_left, p->maxdim, p->mindim, p->cutoff, p->normalize != 0, &terr, &eigs, &nk);
+  replacebond(e, pos, phi, ortho_left, p->maxdim, p->mindim, p->cutoff, p->normalize != 0, &terr, &eigs, &nk,
+              p->noise);
   const bool done = (pos == 1 && !ortho_left) || (pos == e->N - 1 && ortho_left);
   if (p->reverse_step && !done) {
     const int pos1 = ortho_left ? pos + 1 : pos;
@@ -1655,13 +1739,13 @@ int tn_rayleigh_fp64(tn_env* e, const void* phi, double* e2) {
 }
 
 int tn_replacebond(tn_env* e, int b, const void* phi, int ortho_left, long maxdim, long mindim, double cutoff,
-                   int normalize, double* truncerr, double* eigs_out, long eigs_cap, long* nkept) {
+                   double noise, int normalize, double* truncerr, double* eigs_out, long eigs_cap, long* nkept) {
   return guarded(C(e), [&] {
     check_env(e);
     TN_REQUIRE(phi, TN_E_INVALID, "replacebond!: null phi");
     std::vector<double> eigs;
     replacebond(e, b, (const double*)phi, ortho_left != 0, maxdim, mindim, cutoff, normalize != 0, truncerr,
-                &eigs, nkept);
+                &eigs, nkept, noise);
     if (eigs_out)
       for (long i = 0; i < std::min<long>(eigs_cap, (long)eigs.size()); ++i) eigs_out[i] = eigs[i];
   });

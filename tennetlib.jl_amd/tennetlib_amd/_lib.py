@@ -48,7 +48,7 @@ class UpdateParams(C.Structure):
                 ("normalize", C.c_int), ("maxdim", C.c_long), ("mindim", C.c_long),
                 ("cutoff", C.c_double), ("reverse_step", C.c_int), ("krylovdim", C.c_int),
                 ("maxiter", C.c_int), ("tol", C.c_double), ("check_convergence", C.c_int),
-                ("precision", C.c_int), ("refine_fp64", C.c_int)]
+                ("precision", C.c_int), ("refine_fp64", C.c_int), ("noise", C.c_double)]
 
 
 class UpdateResult(C.Structure):
@@ -100,7 +100,7 @@ SIGNATURES = {
     "tn_krylov_eig": [_P, _P, _I, _I, _D, _I, _PD, C.POINTER(KrylovInfo)],
     "tn_krylov_expm": [_P, _P, _D, _D, _I, _I, _D, _I, C.POINTER(KrylovInfo)],
     "tn_rayleigh_fp64": [_P, _P, _PD],
-    "tn_replacebond": [_P, _I, _P, _I, _L, _L, _D, _I, _PD, _PD, _L, _PL],
+    "tn_replacebond": [_P, _I, _P, _I, _L, _L, _D, _D, _I, _PD, _PD, _L, _PL],
     "tn_svd_trunc": [_P, _P, _L, _L, _L, _L, _D, _P, _PD, _P, _PL, _PD],
     "tn_update_two_site": [_P, _I, _I, C.POINTER(UpdateParams), C.POINTER(UpdateResult)],
     "tn_fullsweep": [_P, C.POINTER(UpdateParams), _PD, _PD, _PL, _PD, _L, _PL],

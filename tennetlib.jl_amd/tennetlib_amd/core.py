@@ -341,8 +341,7 @@ def _params(solver, kwargs) -> L.UpdateParams:
     if noise > 0 and p.reverse_step:
         raise ValueError(f"`updatePosition()` :: `noise={noise}` cannot be greater than zero for "
                          f"`reverse_step=true` !!")
-    if abs(noise) > FLOAT64_THRESHOLD:
-        raise NotImplementedError("noise > 0 (density-matrix perturbation) is not built on the GPU path")
+    p.noise = noise
     if kwargs.get("svd_alg", "divide_and_conquer") not in ("divide_and_conquer", "qr_iteration", "recursive"):
         raise ValueError("unknown svd_alg")
     return p

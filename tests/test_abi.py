@@ -48,7 +48,9 @@ def test_struct_layouts():
     # C layout of tn_update_result: 3 doubles, long, krylov info (3 ints + double), pointer, long
     assert C.sizeof(L.KrylovInfo) == 24
     assert C.sizeof(L.UpdateResult) == 8 * 3 + 8 + 24 + 8 + 8
-    assert C.sizeof(L.UpdateParams) == 96
+    # ..., precision, refine_fp64 (ints at 84, 88), noise (double at 96)
+    assert C.sizeof(L.UpdateParams) == 104
+    assert L.UpdateParams.noise.offset == 96
 
 
 def test_dmrg_params_validation_matches_reference():

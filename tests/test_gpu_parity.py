@@ -185,6 +185,69 @@ def test_excited_state_dmrg_n12_reaches_first_excited_level(T):
         assert abs(O.mps_energy(psi, Ws) - E1) < 1e-8
 
 
+@pytest.mark.parametrize("cplx", [False, True])
+@pytest.mark.parametrize("ortho", ["left", "right"])
+def test_noisy_replacebond_matches_oracle(T, cplx, ortho):
+    """replacebond! with eigen_perturbation = noise * noiseterm(PH, phi, ortho)
+    through the C ABI against the oracle's density-matrix factorisation:
+    kept spectrum, truncation error and the (gauge-free) product of the two
+    new sites."""
+    import ctypes as C
+    N, b, noise, maxdim = 6, 3, 1e-2, 5
+    Ws = O.xxz_mpo(N, 1.0, 0.7, np.linspace(-0.4, 0.6, N))
+    A = rand_mps(N, 2, [2, 4, 9, 6, 2], seed=21, cplx=cplx)
+    env = T.StateEnvs(A, Ws)
+    ctx = env.ctx
+    ref = O.Envs(Ws, dtype=np.complex128 if cplx else np.float64)
+    T.position_(env, b)
+    ref.position(A, b)
+    phi = np.tensordot(A[b - 1], A[b], axes=([2], [0]))
+    drho = noise * O.noiseterm(ref, phi, ortho)
+    Aref = [None, None]
+    sp = O.replacebond_perturbed(Aref, 0, phi, maxdim, 1, 1e-12, ortho, True, drho)
+    dphi = T.DeviceArray.from_numpy(ctx, phi)
+    terr, nk = C.c_double(), C.c_long()
+    eigs = (C.c_double * 64)()
+    print(f"noisy replacebond cplx={cplx} ortho={ortho}", flush=True)
+    ctx.check(ctx.lib.tn_replacebond(env.h, b, dphi.ptr, 1 if ortho == "left" else 0, maxdim, 1, 1e-12, noise, 1,
+                                     C.byref(terr), eigs, 64, C.byref(nk)))
+    assert nk.value == len(sp.eigs) == maxdim
+    assert np.allclose(np.array(eigs[:nk.value]), sp.eigs, rtol=1e-10, atol=1e-14)
+    assert abs(terr.value - sp.truncerr) < 1e-12
+    got = env.psi
+    prod = np.tensordot(got[b - 1], got[b], axes=([2], [0]))
+    want = np.tensordot(Aref[0], Aref[1], axes=([2], [0]))
+    assert np.abs(prod - want).max() < 1e-10 * np.abs(want).max()
+    with pytest.raises(T.TNHIPError, match="noise term only defined"):
+        T.position_(env, 1)
+        ctx.check(ctx.lib.tn_replacebond(env.h, b, dphi.ptr, 1, maxdim, 1, 1e-12, noise, 1,
+                                         C.byref(terr), eigs, 64, C.byref(nk)))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_noisy_dmrg_matches_oracle(T, prec):
+    """The reference's noisy DMRG schedule (test_MPS_DMRG.jl:43-68: noise
+    1e-3, noisedecay 2, disable_noise_after 2) on the N=12 Heisenberg chain:
+    sweep energies follow the oracle and the result reaches ED."""
+    N = 12
+    Ws = O.heisenberg_mpo(N)
+    kw = dict(cutoff=1e-12, noise=1e-3, noisedecay=2.0, disable_noise_after=2)
+    ref = O.StateEnvs(O.neel_mps(N), Ws)
+    sw_ref = O.dmrg(ref, O.DMRGParams.make([64], [6], **kw))
+    env = T.StateEnvs(O.neel_mps(N), Ws)
+    sw = T.dmrg_(env, T.make_dmrg_params(maxdim=[64], nsweeps=[6], **kw), 2, outputlevel=0, precision=prec)
+    # The two noisy sweeps end on Ritz values of a two-iteration Lanczos whose
+    # start vector depends on how the cutoff splits near-degenerate (SU(2))
+    # density-matrix multiplets: 1e-9 there in FP64 (1.2e-10 measured) and
+    # |dE/N| <= 1e-9 in FP32 (1.6e-9 measured in total); the noiseless sweeps
+    # after them agree to 1e-11 (FP64; 1e-13 measured) or 1e-9 (FP32).
+    assert np.allclose(sw.energy[:2], sw_ref.energy[:2], rtol=0, atol=1e-9 if prec == "fp64" else N * 1e-9)
+    tol = 1e-11 if prec == "fp64" else 1e-9
+    assert np.allclose(sw.energy[2:], sw_ref.energy[2:], rtol=0, atol=tol)
+    assert sw.maxchi == sw_ref.maxchi
+    assert abs(sw.energy[-1] - (-5.14209063284054)) < 1e-9
+
+
 def _svd_gpu(T, M, maxdim=10 ** 9, mindim=1, cutoff=-1.0):
     import ctypes as C
     ctx = T.default_context()

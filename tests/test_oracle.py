@@ -220,6 +220,100 @@ def test_penalty_projector_equals_projected_operator(nsite, p):
     assert np.abs(got - Heff @ psi.reshape(-1)).max() < 1e-11 * np.abs(Heff).max()
 
 
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_noiseterm_factor_completes_to_projected_hamiltonian(p):
+    """The noise factor is the operator applied up to the middle MPO bond:
+    finishing the contraction with the other site's W and environment gives
+    P^H H P phi (dense, Kronecker-built H) on a complex MPS, for both sides."""
+    N, d = 6, 2
+    h = [0.2, -0.5, 0.9, 0.1, -0.3, 0.7]
+    Ws = O.xxz_mpo(N, 1.0, 0.7, h)
+    A = _mixed_canonical(_complex_mps(N, d, [2, 4, 5, 4, 2], seed=60 + p), p, 2)
+    env = O.Envs(Ws, dtype=np.complex128)
+    env.nsite = 2
+    env.position(A, p)
+    Lm, Rm = _left_iso(A, p), _right_iso(A, p + 1)
+    P = np.kron(np.kron(Lm, np.eye(d * d)), Rm.T)
+    Heff = P.conj().T @ xxz_dense(N, 1.0, 0.7, h) @ P
+    rng = np.random.default_rng(p)
+    shape = (Lm.shape[1], d, d, Rm.shape[0])
+    phi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    want = (Heff @ phi.reshape(-1)).reshape(shape)
+    a, _, _, c = shape
+    X = O.noiseterm_factor(env, phi, "left").reshape(a, d, Ws[p - 1].shape[3], d, c)
+    got = np.einsum("aswtb,wuty,byc->asuc", X, Ws[p], env.rproj())
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    Y = O.noiseterm_factor(env, phi, "right").reshape(a, d, Ws[p].shape[0], d, c)
+    got = np.einsum("axr,xusw,aswtc->rutc", env.lproj(), Ws[p - 1], Y)
+    assert np.abs(got - want).max() < 1e-12 * np.abs(want).max()
+    for ortho in ("left", "right"):
+        nt = O.noiseterm(env, phi, ortho)
+        assert np.allclose(nt, nt.conj().T, atol=1e-12)
+        assert np.linalg.eigvalsh(nt).min() > -1e-10 * np.abs(nt).max()
+
+
+@pytest.mark.parametrize("ortho", ["left", "right"])
+def test_perturbed_factorisation(ortho):
+    """replacebond! with an eigen_perturbation: (a) a zero perturbation gives
+    the SVD factorisation; (b) without truncation the factors reproduce
+    phi / |phi| with an isometric orthonormal factor; (c) with truncation the
+    orthonormal factor spans eigenvectors of rho = phi phi^H + drho whose
+    eigenvalues are the largest ones (scipy eigvalsh), and the spectrum
+    reported is that of rho."""
+    rng = np.random.default_rng(3 if ortho == "left" else 4)
+    a, s, t, c = 5, 2, 2, 6
+    phi = rng.standard_normal((a, s, t, c)) + 1j * rng.standard_normal((a, s, t, c))
+    m = a * s if ortho == "left" else t * c
+    G = rng.standard_normal((m, 3 * m)) + 1j * rng.standard_normal((m, 3 * m))
+    drho = 1e-2 * (G @ G.conj().T)
+    M = phi.reshape(a * s, t * c)
+
+    def iso_and_product(A):
+        if ortho == "left":
+            Q = A[0].reshape(a * s, -1)
+            return Q, Q @ A[1].reshape(Q.shape[1], -1)
+        Q = A[1].reshape(-1, t * c).T
+        return Q, A[0].reshape(a * s, -1) @ Q.T
+
+    A0, A1 = [None, None], [None, None]
+    sp0 = O.replacebond(A0, 0, phi, 100, 1, 1e-13, ortho)
+    sp1 = O.replacebond_perturbed(A1, 0, phi, 100, 1, 1e-13, ortho, True, np.zeros((m, m)))
+    assert np.allclose(sp0.eigs, sp1.eigs, rtol=1e-12, atol=1e-13)
+    assert np.abs(iso_and_product(A0)[1] - iso_and_product(A1)[1]).max() < 1e-12
+
+    A2 = [None, None]
+    sp = O.replacebond_perturbed(A2, 0, phi, 100, 1, -1.0, ortho, True, drho)
+    Q, prod = iso_and_product(A2)
+    assert Q.shape[1] == min(m, 100)
+    assert np.abs(Q.conj().T @ Q - np.eye(Q.shape[1])).max() < 1e-12
+    assert np.abs(prod - M / np.linalg.norm(M)).max() < 1e-12
+
+    A3 = [None, None]
+    sp = O.replacebond_perturbed(A3, 0, phi, 4, 1, 1e-13, ortho, False, drho)
+    Q, _ = iso_and_product(A3)
+    rho = (M @ M.conj().T if ortho == "left" else M.T @ M.conj()) + drho
+    lam = sla.eigvalsh(rho)[::-1]
+    assert Q.shape[1] == 4
+    assert np.allclose(sp.eigs, lam[:4], rtol=1e-12)
+    assert abs(sp.truncerr - lam[4:].sum() / lam.sum()) < 1e-14
+    Qc = Q if ortho == "left" else Q
+    assert np.abs(rho @ Qc - Qc * lam[:4]).max() < 1e-11 * lam[0]
+
+
+def test_noisy_dmrg_n12_reaches_ed():
+    """The reference's noisy DMRG settings (test_MPS_DMRG.jl:43-68: noise
+    1e-3, noisedecay 2, disable_noise_after 2) followed by noiseless sweeps
+    reach the ED ground energy; the noise visibly changes the first sweeps."""
+    Ws = O.heisenberg_mpo(12)
+    noisy = O.StateEnvs(O.neel_mps(12), Ws)
+    p = O.DMRGParams.make([64], [6], cutoff=1e-14, noise=1e-3, noisedecay=2.0, disable_noise_after=2)
+    sw = O.dmrg(noisy, p)
+    clean = O.StateEnvs(O.neel_mps(12), Ws)
+    sw0 = O.dmrg(clean, O.DMRGParams.make([64], [6], cutoff=1e-14))
+    assert abs(sw.energy[-1] - (-5.14209063284054)) < 1e-10
+    assert abs(sw.energy[0] - sw0.energy[0]) > 1e-9
+
+
 def test_penalty_weight_must_be_positive():
     psi, Ws = O.neel_mps(4), O.heisenberg_mpo(4)
     with pytest.raises(ValueError, match="should be > 0.0"):
