@@ -197,6 +197,52 @@ int tn_update_two_site(tn_env* env, int pos, int ortho_left, const tn_update_par
 int tn_fullsweep(tn_env* env, const tn_update_params* p, double* energy, double* maxtruncerr,
                  long* maxchi, double* mid_eigs, long cap, long* nmid);
 
+/* ---- tree tensor networks (StateEnvsTTN, src/ttn/*.jl) ------------------ */
+/* A tn_ttn holds the node tensors (three legs each, row-major A[a0][a1][a2],
+ * FP64) and the environments of a TTN on the device.  The Hamiltonian enters
+ * as a tree operator: every link carries `nchan` channels, every site leg the
+ * channel operators of its site, and every node a sparse coupling
+ * W[c0][c1][c2] between the channels of its three slots.  Slots 0 and 1 hold
+ * children (nodes or sites), slot 2 the parent (the root: a third child).
+ * Nodes are numbered 0..nnodes-1 by the caller. */
+typedef struct tn_ttn tn_ttn;
+/* TTN / StateEnvsTTN storage (src/ttn/ttn.jl:1-30, state_envs_ttn.jl:1-30) */
+int tn_ttn_create(tn_ctx* ctx, int nnodes, tn_ttn** out);
+int tn_ttn_destroy(tn_ttn* ttn);
+/* link between slot sa of node a and slot sb of node b (graph edge, src/base/graph.jl:100-125) */
+int tn_ttn_set_link(tn_ttn* ttn, int a, int sa, int b, int sb, int nchan);
+/* site leg at slot 0/1: ops[c][bra][ket] = <bra|X_c|ket>, d x d per channel
+ * (the CouplingModel terms of that site, src/base/couplingmodel.jl:153-241) */
+int tn_ttn_set_site(tn_ttn* ttn, int node, int slot, int d, int nchan, const double* ops_host);
+/* the node's tree-operator coupling: nnz entries (c0[k], c1[k], c2[k], w[k]) */
+int tn_ttn_set_coupling(tn_ttn* ttn, int node, int nnz, const int* c0, const int* c1, const int* c2,
+                        const double* w);
+/* ttn[node] = A / ttn[node] (src/ttn/ttn.jl:60-90) */
+int tn_ttn_set_node(tn_ttn* ttn, int node, const long* dims3, const double* A_host);
+int tn_ttn_get_node(tn_ttn* ttn, int node, long* dims3, double* A_host /* nullable */);
+int tn_ttn_node_ptr(tn_ttn* ttn, int node, double** A_dev);
+/* move_linktensors_to_next!(env, psi, node, neighbour at `slot`) (src/ttn/linktensors.jl:63-118) */
+int tn_ttn_update_env(tn_ttn* ttn, int node, int slot);
+/* product(env, psi, v) at the orthogonality center `node` (src/ttn/linktensors.jl:183-221) */
+int tn_ttn_matvec(tn_ttn* ttn, int node, const double* v_dev, double* out_dev);
+/* eig_solver at `node` (src/ttn/update_site_ttn.jl:51-62, src/base/solver.jl:23-43):
+ * Lanczos from the node tensor, which receives the Ritz vector */
+int tn_ttn_eig(tn_ttn* ttn, int node, int krylovdim, int maxiter, double tol, double* energy,
+               tn_krylov_info* info);
+/* moveisometry_to_next!(ttn, node, neighbour at `slot`; maxdim, mindim, cutoff)
+ * (src/ttn/ttn.jl:266-310): QR, or truncating SVD when cutoff > 0 or the bond
+ * exceeds maxdim; the neighbour absorbs the remainder (not normalised) */
+int tn_ttn_move(tn_ttn* ttn, int node, int slot, long maxdim, long mindim, double cutoff,
+                double* truncerr, long* newdim);
+/* normalize!(ttn) at `node` (src/ttn/ttn.jl:204-215); norm receives the old norm */
+int tn_ttn_normalize(tn_ttn* ttn, int node, double* norm);
+int tn_ttn_node_norm(tn_ttn* ttn, int node, double* norm);
+/* subspace_expand! (src/ttn/update_site_ttn.jl:65-109): grow the bond at
+ * `slot` of node and its neighbour by `pad`, appending padA / padB (each the
+ * node's tensor shape with that leg replaced by pad, row-major) */
+int tn_ttn_expand(tn_ttn* ttn, int node, int slot, long pad, const double* padA_host,
+                  const double* padB_host);
+
 #ifdef __cplusplus
 }
 #endif

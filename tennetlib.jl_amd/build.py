@@ -17,7 +17,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 OUT = HERE / "libtnhip.so"
-SOURCES = ["gemm.hip", "gemm_fast.hip", "tensor_ops.hip", "svd.hip", "eig_dc.hip", "tnhip_api.hip"]
+SOURCES = ["gemm.hip", "gemm_fast.hip", "tensor_ops.hip", "svd.hip", "eig_dc.hip", "qr.hip", "tnhip_api.hip", "ttn.hip"]
 
 
 def hipcc() -> str:

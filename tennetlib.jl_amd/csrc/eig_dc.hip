@@ -1426,6 +1426,13 @@ static std::vector<double> dc_tridiag(tn_ctx* c, long p, const std::vector<doubl
   return D;
 }
 
+void wy_tsolve(tn_ctx* c, const double* S, const double* tauv, int j0, int nb, double* W, long n) {
+  TN_REQUIRE(nb >= 1 && nb <= NBW, TN_E_INVALID, "wy_tsolve: block size");
+  hipLaunchKernelGGL(wy_tsolve_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, S, tauv, j0, nb, W,
+                     n);
+  TN_HIP(hipGetLastError());
+}
+
 void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w) {
   TN_REQUIRE(p >= 3 && p <= 16384, TN_E_INVALID, "sym_eig_dc: size out of range");
   const int P = (int)p;

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <functional>
 #include <vector>
 
 #include "../../include/tnhip.h"
@@ -128,6 +129,7 @@ enum WsSlot {
   WS_SVD_X, WS_SVD_Q, WS_SVD_GRAM, WS_SVD_J, WS_SVD_CNT, WS_SVD_PAIRS, WS_SVD_S, WS_SVD_PERM,
   WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP,
   WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_PROJ_X, WS_PROJ_Y, WS_PROJ_C,
+  WS_QR_X, WS_QR_V, WS_QR_Q, WS_QR_R, WS_QR_S, WS_QR_W, WS_TTN_A, WS_TTN_S, WS_TTN_G, WS_TTN_P, WS_TTN_F,
   WS_SLOTS
 };
 
@@ -252,6 +254,27 @@ void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bo
 // real symmetric p x p (3 <= p <= 16384) by Householder tridiagonalisation and
 // divide and conquer (eig_dc.hip); same contract as sym_eig_jacobi (G is
 // overwritten, Vt rows are the eigenvectors, w their eigenvalues)
+// compact-WY helper: W (nb x n, ld n) <- T W for reflectors j0..j0+nb-1 with
+// T^{-1} = diag(1/tau) + striu(S), S = V^T V (nb x nb, ld 64), nb <= 64
+constexpr int WY_NB = 64;
+void wy_tsolve(tn_ctx* c, const double* S, const double* tauv, int j0, int nb, double* W, long n);
+// Householder QR (LAPACK dgeqrf convention: beta = -sign(alpha)|x|,
+// tau = (beta - alpha)/beta, v_0 = 1) of M (m x n) given column by column:
+// X (n x m, row c = column c of M) is overwritten.  With k = min(m, n):
+// Qt (k x m) receives the rows of Q^T (orthonormal), R (k x n) the upper
+// trapezoidal factor, M = Q R.
+void householder_qr(tn_ctx* c, double* X, long m, long n, double* Qt, double* R);
+// permute a row-major 3-leg tensor A[d0][d1][d2] to out[d1][d0][d2]
+void swap01_f64(tn_ctx* c, const double* A, long d0, long d1, long d2, double* out);
+// out = A concatenated with P along axis ax of a 3-leg tensor (dims of A;
+// P has dims with axis ax replaced by p)
+void concat3_f64(tn_ctx* c, const double* A, const long* dims, int ax, const double* P, long p, double* out);
+
+// Lanczos ground state (eigsolve(A, x0, 1, :SR), src/base/solver.jl:36) of the
+// symmetric operator y = A x given by `apply`; phi (n elements, device) is
+// the start vector and receives the Ritz vector; returns the Ritz value
+double krylov_eig_op(tn_ctx* c, bool cx, long n, const std::function<void(const double*, double*)>& apply,
+                     double* phi, int kdim, int maxiter, double tol, tn_krylov_info* info);
 void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w);
 // Hermitian p x p complex128 (interleaved, 2 <= p <= 16384): complex Householder
 // reduction to a real tridiagonal, the same divide and conquer, complex

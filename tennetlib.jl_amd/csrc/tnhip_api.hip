@@ -680,7 +680,6 @@ void sym_eig(int n, std::vector<double> A, std::vector<double>& w, std::vector<d
 // Krylov machinery on device vectors
 // ---------------------------------------------------------------------------
 struct Krylov {
-  tn_env* e;
   tn_ctx* c;
   bool cx;
   long n;       // complex or real elements
@@ -689,7 +688,8 @@ struct Krylov {
   double* base; // slots x nd
   double* w;    // nd
   double* coef; // device scratch
-  Krylov(tn_env* env, long nelem, int nslots) : e(env), c(env->ctx), cx(env->cplx()), n(nelem) {
+  Krylov(tn_env* env, long nelem, int nslots) : Krylov(env->ctx, env->cplx(), nelem, nslots) {}
+  Krylov(tn_ctx* ctx, bool cplx, long nelem, int nslots) : c(ctx), cx(cplx), n(nelem) {
     nd = cx ? 2 * n : n;
     slots = nslots;
     base = (double*)c->workspace(WS_KRY_BASIS, (size_t)slots * nd * sizeof(double));
@@ -806,16 +806,19 @@ void matvec(tn_env* e, const void* v, void* out, int prec) {
 
 void apply_op(tn_env* e, const double* x, double* y, int prec) { matvec(e, x, y, prec); }
 
+}  // namespace
+
 // eigsolve(A, x0, 1, :SR; krylovdim, maxiter, tol, eager=false) (src/base/solver.jl:36)
-double krylov_eig(tn_env* e, double* phi, int kdim, int maxiter, double tol, int prec, tn_krylov_info* info) {
+// for the symmetric operator y = A x given by `apply`
+double tn::krylov_eig_op(tn_ctx* c, bool cx, long n, const std::function<void(const double*, double*)>& apply,
+                         double* phi, int kdim, int maxiter, double tol, tn_krylov_info* info) {
   TN_REQUIRE(kdim >= 1 && kdim <= 60 && maxiter >= 1, TN_E_INVALID, "eig_solver: bad krylovdim/maxiter");
-  const Dom D = domain(e);
-  Krylov K(e, D.n, 2 * kdim + 2);
+  Krylov K(c, cx, n, 2 * kdim + 2);
   tn_krylov_info inf{0, 0, 1, NAN};
   // v0 = x0 / |x0|
   double b0;
   {
-    PhaseTimer pt(e->ctx, PH_KRYLOV);
+    PhaseTimer pt(c, PH_KRYLOV);
     b0 = std::sqrt(K.norm2(phi));
     TN_REQUIRE(std::isfinite(b0) && b0 > 0.0, TN_E_NUMERIC, "eig_solver: zero or non-finite start vector");
     scaled_copy(K.c, K.V(0), phi, 1.0 / b0, K.nd);
@@ -825,10 +828,10 @@ double krylov_eig(tn_env* e, double* phi, int kdim, int maxiter, double tol, int
   int Kn = 1;
   std::vector<cplx> h;
   double r2;
-  apply_op(e, K.V(0), K.w, prec);
+  apply(K.V(0), K.w);
   inf.numops++;
   {
-    PhaseTimer pt(e->ctx, PH_KRYLOV);
+    PhaseTimer pt(c, PH_KRYLOV);
     K.orthogonalize(1, h, r2);
   }
   H(0, 0) = h[0].real();
@@ -847,7 +850,7 @@ double krylov_eig(tn_env* e, double* phi, int kdim, int maxiter, double tol, int
       inf.converged = conv;
       inf.residual = res[0];
       if (conv >= 1 || beta <= tol || inf.numiter >= maxiter) {
-        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        PhaseTimer pt(c, PH_KRYLOV);
         std::vector<cplx> co(Kn);
         for (int i = 0; i < Kn; ++i) co[i] = U[(size_t)i * Kn + 0];
         K.combine(Kn, co, phi);
@@ -860,7 +863,7 @@ double krylov_eig(tn_env* e, double* phi, int kdim, int maxiter, double tol, int
       int keep = (3 * kdim + 2 * conv) / 5;
       keep = std::max(1, std::min(keep, Kn - 1));
       {
-        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        PhaseTimer pt(c, PH_KRYLOV);
         for (int j = 0; j < keep; ++j) {
           std::vector<cplx> co(Kn);
           for (int i = 0; i < Kn; ++i) co[i] = U[(size_t)i * Kn + j];
@@ -877,21 +880,30 @@ double krylov_eig(tn_env* e, double* phi, int kdim, int maxiter, double tol, int
       Kn = keep + 1;
       inf.numiter++;
     } else {
-      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      PhaseTimer pt(c, PH_KRYLOV);
       scaled_copy(K.c, K.V(Kn), K.w, 1.0 / beta, K.nd);
       H(Kn, Kn - 1) = H(Kn - 1, Kn) = beta;
       ++Kn;
     }
-    apply_op(e, K.V(Kn - 1), K.w, prec);
+    apply(K.V(Kn - 1), K.w);
     inf.numops++;
     {
-      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      PhaseTimer pt(c, PH_KRYLOV);
       K.orthogonalize(Kn, h, r2);
     }
     H(Kn - 1, Kn - 1) = h[Kn - 1].real();
     beta = std::sqrt(std::max(0.0, r2));
     TN_REQUIRE(std::isfinite(beta), TN_E_NUMERIC, "eig_solver: non-finite residual");
   }
+}
+
+namespace {
+
+double krylov_eig(tn_env* e, double* phi, int kdim, int maxiter, double tol, int prec, tn_krylov_info* info) {
+  const Dom D = domain(e);
+  return krylov_eig_op(
+      e->ctx, e->cplx(), D.n, [&](const double* x, double* y) { apply_op(e, x, y, prec); }, phi, kdim, maxiter,
+      tol, info);
 }
 
 // exp(t A) x0 by a Lanczos exponential integrator (src/base/solver.jl:80)

@@ -1,0 +1,556 @@
+// C ABI for tree tensor networks (StateEnvsTTN / LinkTensorsTTN,
+// src/ttn/state_envs_ttn.jl, src/ttn/linktensors.jl) on one MI355X.
+//
+// Every node tensor has three legs (slots 0, 1, 2; row-major A[a0][a1][a2]).
+// The Hamiltonian enters as a tree operator: each link carries channels c
+// and each node a sparse coupling W[c0][c1][c2] between the channels of its
+// three legs, so that the projected operator at a node is
+//   (H v)[a0',a1',a2'] = sum_W w E0[c0] E1[c1] E2[c2] v
+// with E_s the environment of slot s (the far side of that leg).  A site leg
+// is a slot whose environment holds the channel operators of that site.
+//
+// Environment layouts (FP64, per slot):
+//   slots 0, 1   E[c][bra][ket]     (the first and the batched middle GEMM)
+//   slot 2       E[c][ket][bra]     (the closing GEMM, K = (c, ket))
+// Node product at node n (D_s the leg dimensions, n_s the channel counts):
+//   A[(c0 a0'), (a1 a2)]  = E0[(c0 a0'), a0] v[a0, (a1 a2)]           GEMM
+//   S[a0'][a1'][c2][a2]  += w E1[c1] A[c0][a0'][a1][a2]  per W entry    GEMM batched over a0'
+//   out[(a0' a1'), a2']   = S[(a0' a1'), (c2 a2)] E2[(c2 a2), a2']       GEMM
+// The environment of slot s toward the neighbour (move_linktensors_to_next!)
+// closes the same chains with the bra copy of v.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "tn_internal.h"
+
+using namespace tn;
+
+struct TTNSlot {
+  int nbr = -1;       // neighbouring node, -1 for a physical site
+  int nbr_slot = -1;  // slot of this link at the neighbour
+  int nchan = 0;
+  DevBuf env;         // environment of the far side, layout by slot
+  long env_dim = 0;   // leg dimension the environment was built for
+  bool env_ok = false;
+};
+
+struct TTNNode {
+  long dims[3] = {0, 0, 0};
+  DevBuf A;
+  TTNSlot slot[3];
+  std::vector<int> c[3];
+  std::vector<double> w;
+  bool has_tensor = false;
+  long size() const { return dims[0] * dims[1] * dims[2]; }
+};
+
+struct tn_ttn {
+  tn_ctx* ctx = nullptr;
+  std::vector<TTNNode> nodes;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guarded_t(tn_ctx* c, F&& f) {
+  try {
+    f();
+    return TN_OK;
+  } catch (const Error& e) {
+    g_err = e.what();
+    if (c) c->last_error = g_err;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    if (c) c->last_error = g_err;
+    return TN_E_NOMEM;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    if (c) c->last_error = g_err;
+    return TN_E_INVALID;
+  }
+}
+
+tn_ctx* CT(tn_ttn* t) { return t ? t->ctx : nullptr; }
+
+TTNNode& node_of(tn_ttn* t, int n) {
+  TN_REQUIRE(t && n >= 0 && n < (int)t->nodes.size(), TN_E_INVALID, "TTN: node index out of range");
+  return t->nodes[n];
+}
+
+void require_env(const TTNNode& N, int s) {
+  const TTNSlot& S = N.slot[s];
+  TN_REQUIRE(S.env_ok && S.env_dim == N.dims[s], TN_E_POSITION,
+             "TTN: environment of slot " + std::to_string(s) + " is not built for the current bond");
+}
+
+void gemm64(tn_ctx* c, GemmDesc g) { gemm(c, Prec::F64, g); }
+
+// S[a0'][a1'][c2][a2] = sum_W w E0[c0] E1[c1] v   (legs 0 and 1 applied, leg 2 open)
+void apply_legs01(tn_ttn* t, const TTNNode& N, const double* v, double* S) {
+  tn_ctx* c = t->ctx;
+  const long D0 = N.dims[0], D1 = N.dims[1], D2 = N.dims[2];
+  const long n0 = N.slot[0].nchan, n2 = N.slot[2].nchan;
+  double* A = (double*)c->workspace(WS_TTN_A, (size_t)(n0 * D0 * D1 * D2) * sizeof(double));
+  {
+    GemmDesc g;
+    g.M = n0 * D0; g.N = D1 * D2; g.K = D0;
+    g.A = N.slot[0].env.p; g.lda = D0;
+    g.B = v; g.ldb = D1 * D2;
+    g.C = A; g.ldc = D1 * D2;
+    gemm64(c, g);
+  }
+  fill_f64(c, S, 0.0, D0 * D1 * n2 * D2);
+  const double* E1 = (const double*)N.slot[1].env.p;
+  for (size_t k = 0; k < N.w.size(); ++k) {
+    GemmDesc g;
+    g.M = D1; g.N = D2; g.K = D1;
+    g.A = E1 + (long)N.c[1][k] * D1 * D1; g.lda = D1; g.sA = 0;
+    g.B = A + (long)N.c[0][k] * D0 * D1 * D2; g.ldb = D2; g.sB = D1 * D2;
+    g.C = S + (long)N.c[2][k] * D2; g.ldc = n2 * D2; g.sC = D1 * n2 * D2;
+    g.batch = (int)D0;
+    g.alpha = N.w[k]; g.beta = 1.0;
+    gemm64(c, g);
+  }
+}
+
+void matvec_node(tn_ttn* t, int n, const double* v, double* out) {
+  PhaseTimer pt(t->ctx, PH_MATVEC);
+  const TTNNode& N = t->nodes[n];
+  for (int s = 0; s < 3; ++s) require_env(N, s);
+  const long D0 = N.dims[0], D1 = N.dims[1], D2 = N.dims[2], n2 = N.slot[2].nchan;
+  double* S = (double*)t->ctx->workspace(WS_TTN_S, (size_t)(D0 * D1 * n2 * D2) * sizeof(double));
+  apply_legs01(t, N, v, S);
+  GemmDesc g;
+  g.M = D0 * D1; g.N = D2; g.K = n2 * D2;
+  g.A = S; g.lda = n2 * D2;
+  g.B = N.slot[2].env.p; g.ldb = D2;
+  g.C = out; g.ldc = D2;
+  gemm64(t->ctx, g);
+}
+
+// G[c2][(a0 a1)][a2'] = v[(a0 a1)][a2] E2[c2][a2][a2']
+double* close_leg2(tn_ttn* t, const TTNNode& N, const double* v) {
+  tn_ctx* c = t->ctx;
+  const long D0 = N.dims[0], D1 = N.dims[1], D2 = N.dims[2], n2 = N.slot[2].nchan;
+  double* G = (double*)c->workspace(WS_TTN_G, (size_t)(n2 * D0 * D1 * D2) * sizeof(double));
+  GemmDesc g;
+  g.M = D0 * D1; g.N = D2; g.K = D2;
+  g.A = v; g.lda = D2; g.sA = 0;
+  g.B = N.slot[2].env.p; g.ldb = D2; g.sB = D2 * D2;
+  g.C = G; g.ldc = D2; g.sC = D0 * D1 * D2;
+  g.batch = (int)n2;
+  gemm64(c, g);
+  return G;
+}
+
+// move_linktensors_to_next! (src/ttn/linktensors.jl:63-118): the environment
+// of the link at slot s of node n, seen from n, written into the neighbour.
+void update_env(tn_ttn* t, int n, int s) {
+  PhaseTimer pt(t->ctx, PH_ENV);
+  tn_ctx* c = t->ctx;
+  TTNNode& N = t->nodes[n];
+  TN_REQUIRE(s >= 0 && s < 3 && N.slot[s].nbr >= 0, TN_E_INVALID, "TTN: slot has no neighbouring node");
+  TN_REQUIRE(N.has_tensor, TN_E_INVALID, "TTN: node tensor not set");
+  for (int o = 0; o < 3; ++o)
+    if (o != s) require_env(N, o);
+  TTNNode& M = t->nodes[N.slot[s].nbr];
+  const int ts = N.slot[s].nbr_slot;
+  TTNSlot& out = M.slot[ts];
+  const long D0 = N.dims[0], D1 = N.dims[1], D2 = N.dims[2];
+  const long Ds = N.dims[s], ns = N.slot[s].nchan;
+  TN_REQUIRE(out.nchan == ns, TN_E_SHAPE, "TTN: channel counts of a link disagree");
+  out.env.ensure((size_t)(ns * Ds * Ds) * sizeof(double));
+  double* F = out.env.as<double>();
+  const double* v = N.A.as<double>();
+  if (s == 2) {
+    const long n2 = ns;
+    double* S = (double*)c->workspace(WS_TTN_S, (size_t)(D0 * D1 * n2 * D2) * sizeof(double));
+    apply_legs01(t, N, v, S);
+    if (ts == 2) {  // F[c][ket][bra] = S^T v
+      GemmDesc g;
+      g.M = n2 * D2; g.N = D2; g.K = D0 * D1;
+      g.A = S; g.lda = n2 * D2; g.tA = 1;
+      g.B = v; g.ldb = D2;
+      g.C = F; g.ldc = D2;
+      gemm64(c, g);
+    } else {  // F[c][bra][ket] = v^T S[c]
+      GemmDesc g;
+      g.M = D2; g.N = D2; g.K = D0 * D1;
+      g.A = v; g.lda = D2; g.tA = 1; g.sA = 0;
+      g.B = S; g.ldb = n2 * D2; g.sB = D2;
+      g.C = F; g.ldc = D2; g.sC = D2 * D2;
+      g.batch = (int)n2;
+      gemm64(c, g);
+    }
+  } else {
+    TN_REQUIRE(ts == 2, TN_E_INVALID, "TTN: a child link must enter the child at its parent slot");
+    double* G = close_leg2(t, N, v);
+    const long n2 = N.slot[2].nchan;
+    (void)n2;
+    double* K = (double*)c->workspace(WS_TTN_A, (size_t)(ns * D0 * D1 * D2) * sizeof(double));
+    fill_f64(c, K, 0.0, ns * D0 * D1 * D2);
+    if (s == 0) {
+      // K[c0][a0][a1'][a2'] += w E1[c1][a1'][a1] G[c2][a0][a1][a2']   (batched over a0)
+      const double* E1 = (const double*)N.slot[1].env.p;
+      for (size_t k = 0; k < N.w.size(); ++k) {
+        GemmDesc g;
+        g.M = D1; g.N = D2; g.K = D1;
+        g.A = E1 + (long)N.c[1][k] * D1 * D1; g.lda = D1; g.sA = 0;
+        g.B = G + (long)N.c[2][k] * D0 * D1 * D2; g.ldb = D2; g.sB = D1 * D2;
+        g.C = K + (long)N.c[0][k] * D0 * D1 * D2; g.ldc = D2; g.sC = D1 * D2;
+        g.batch = (int)D0;
+        g.alpha = N.w[k]; g.beta = 1.0;
+        gemm64(c, g);
+      }
+      // F[c0][a0][a0'] = K[(c0 a0)][(a1' a2')] v[a0'][(a1' a2')]^T
+      GemmDesc g;
+      g.M = ns * D0; g.N = D0; g.K = D1 * D2;
+      g.A = K; g.lda = D1 * D2;
+      g.B = v; g.ldb = D1 * D2; g.tB = 1;
+      g.C = F; g.ldc = D0;
+      gemm64(c, g);
+    } else {
+      // K[c1][a1][a0'][a2'] += w E0[c0][a0'][a0] G[c2][a0][a1][a2']   (batched over a1)
+      const double* E0 = (const double*)N.slot[0].env.p;
+      for (size_t k = 0; k < N.w.size(); ++k) {
+        GemmDesc g;
+        g.M = D0; g.N = D2; g.K = D0;
+        g.A = E0 + (long)N.c[0][k] * D0 * D0; g.lda = D0; g.sA = 0;
+        g.B = G + (long)N.c[2][k] * D0 * D1 * D2; g.ldb = D1 * D2; g.sB = D2;
+        g.C = K + (long)N.c[1][k] * D1 * D0 * D2; g.ldc = D2; g.sC = D0 * D2;
+        g.batch = (int)D1;
+        g.alpha = N.w[k]; g.beta = 1.0;
+        gemm64(c, g);
+      }
+      double* vp = (double*)c->workspace(WS_TTN_P, (size_t)(D0 * D1 * D2) * sizeof(double));
+      swap01_f64(c, v, D0, D1, D2, vp);  // vp[a1'][a0'][a2']
+      GemmDesc g;
+      g.M = ns * D1; g.N = D1; g.K = D0 * D2;
+      g.A = K; g.lda = D0 * D2;
+      g.B = vp; g.ldb = D0 * D2; g.tB = 1;
+      g.C = F; g.ldc = D1;
+      gemm64(c, g);
+    }
+  }
+  out.env_dim = Ds;
+  out.env_ok = true;
+}
+
+// moveisometry_to_next! (src/ttn/ttn.jl:266-310): node n becomes an isometry
+// toward its other legs (QR; truncating SVD when cutoff > 0 or the bond
+// exceeds maxdim), the remainder is absorbed into the neighbour at slot s.
+void move_isometry(tn_ttn* t, int n, int s, long maxdim, long mindim, double cutoff, double* truncerr, long* newdim) {
+  PhaseTimer pt(t->ctx, PH_SVD);
+  tn_ctx* c = t->ctx;
+  TTNNode& N = t->nodes[n];
+  TN_REQUIRE(s >= 0 && s < 3 && N.slot[s].nbr >= 0, TN_E_INVALID, "moveisometry_to_next!: slot has no neighbour");
+  TTNNode& M = t->nodes[N.slot[s].nbr];
+  const int ts = N.slot[s].nbr_slot;
+  const long D0 = N.dims[0], D1 = N.dims[1], D2 = N.dims[2];
+  const long Ds = N.dims[s];
+  const long m = N.size() / Ds;
+  TN_REQUIRE(M.dims[ts] == Ds, TN_E_SHAPE, "moveisometry_to_next!: bond dimensions of the link disagree");
+  double* X = (double*)c->workspace(WS_QR_X, (size_t)(Ds * m) * sizeof(double));
+  const double* A = N.A.as<double>();
+  if (s == 0) copy_f64(c, X, A, Ds * m);
+  else if (s == 1) swap01_f64(c, A, D0, D1, D2, X);
+  else transpose_f64(c, A, D0 * D1, D2, X);
+  const long k0 = std::min(m, Ds);
+  double* Qt = (double*)c->workspace(WS_QR_Q, (size_t)(k0 * m) * sizeof(double));
+  double* R = (double*)c->workspace(WS_QR_R, (size_t)(k0 * Ds) * sizeof(double));
+  householder_qr(c, X, m, Ds, Qt, R);
+  long k = k0;
+  double terr = 0.0;
+  const double* Qn = Qt;
+  const double* Rn = R;
+  if (cutoff > 0.0 || Ds > maxdim) {
+    // SVD of R through its Gram matrix R R^T: the left factor rotates Q
+    GramEig ge;
+    gram_eig(c, R, k0, Ds, true, false, ge);
+    std::vector<int> order(k0);
+    for (long i = 0; i < k0; ++i) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ge.w[x] > ge.w[y]; });
+    std::vector<double> P(k0);
+    for (long i = 0; i < k0; ++i) P[i] = std::max(0.0, ge.w[order[i]]);
+    const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
+    k = tr.n;
+    terr = tr.truncerr;
+    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
+    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
+    std::vector<double> ones(k, 1.0);
+    TN_HIP(hipMemcpyAsync(dperm, order.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(dsc, ones.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    double* Ut = (double*)c->workspace(WS_SVD_GRAM, (size_t)(k * k0) * sizeof(double));
+    gather_rows(c, (const double*)ge.Vt, ge.ppad, dperm, dsc, k, k0, Ut);  // U_k^T (k x k0)
+    double* Q2 = (double*)c->workspace(WS_TTN_F, (size_t)(k * m + k * Ds) * sizeof(double));
+    double* R2 = Q2 + k * m;
+    GemmDesc g;
+    g.M = k; g.N = m; g.K = k0;
+    g.A = Ut; g.lda = k0;
+    g.B = Qt; g.ldb = m;
+    g.C = Q2; g.ldc = m;
+    gemm64(c, g);
+    GemmDesc h;
+    h.M = k; h.N = Ds; h.K = k0;
+    h.A = Ut; h.lda = k0;
+    h.B = R; h.ldb = Ds;
+    h.C = R2; h.ldc = Ds;
+    gemm64(c, h);
+    TN_HIP(hipStreamSynchronize(c->stream));  // host vectors above go out of scope
+    Qn = Q2;
+    Rn = R2;
+  }
+  // node n <- Q (leg s of dimension k)
+  long nd[3] = {D0, D1, D2};
+  nd[s] = k;
+  DevBuf An;
+  An.ensure((size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
+  if (s == 0) copy_f64(c, An.as<double>(), Qn, k * m);
+  else if (s == 1) swap01_f64(c, Qn, k, D0, D2, An.as<double>());
+  else transpose_f64(c, Qn, k, D0 * D1, An.as<double>());
+  // neighbour <- R x_ts neighbour
+  const long B0 = M.dims[0], B1 = M.dims[1], B2 = M.dims[2];
+  long md[3] = {B0, B1, B2};
+  md[ts] = k;
+  DevBuf Bn;
+  Bn.ensure((size_t)(md[0] * md[1] * md[2]) * sizeof(double));
+  const double* B = M.A.as<double>();
+  GemmDesc g;
+  if (ts == 0) {
+    g.M = k; g.N = B1 * B2; g.K = Ds;
+    g.A = Rn; g.lda = Ds;
+    g.B = B; g.ldb = B1 * B2;
+    g.C = Bn.p; g.ldc = B1 * B2;
+  } else if (ts == 2) {
+    g.M = B0 * B1; g.N = k; g.K = Ds;
+    g.A = B; g.lda = Ds;
+    g.B = Rn; g.ldb = Ds; g.tB = 1;
+    g.C = Bn.p; g.ldc = k;
+  } else {
+    g.M = k; g.N = B2; g.K = Ds;
+    g.A = Rn; g.lda = Ds; g.sA = 0;
+    g.B = B; g.ldb = B2; g.sB = Ds * B2;
+    g.C = Bn.p; g.ldc = B2; g.sC = k * B2;
+    g.batch = (int)B0;
+  }
+  gemm64(c, g);
+  TN_HIP(hipStreamSynchronize(c->stream));
+  N.A = std::move(An);
+  M.A = std::move(Bn);
+  for (int i = 0; i < 3; ++i) {
+    N.dims[i] = nd[i];
+    M.dims[i] = md[i];
+  }
+  N.slot[s].env_ok = false;
+  M.slot[ts].env_ok = false;
+  if (truncerr) *truncerr = terr;
+  if (newdim) *newdim = k;
+}
+
+double node_norm2(tn_ttn* t, const TTNNode& N) {
+  tn_ctx* c = t->ctx;
+  double* co = (double*)c->workspace(WS_PROJ_C, 64 * sizeof(double));
+  multi_dot(c, N.A.as<double>(), N.size(), 1, N.A.as<double>(), N.size(), false, co);
+  TN_HIP(hipMemcpyAsync(c->pinned, co, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  TN_HIP(hipStreamSynchronize(c->stream));
+  return c->pinned[0];
+}
+
+}  // namespace
+
+extern "C" {
+
+int tn_ttn_create(tn_ctx* ctx, int nnodes, tn_ttn** out) {
+  return guarded_t(ctx, [&] {
+    TN_REQUIRE(ctx && out && nnodes >= 1, TN_E_INVALID, "tn_ttn_create: bad arguments");
+    tn_ttn* t = new tn_ttn;
+    t->ctx = ctx;
+    t->nodes.resize(nnodes);
+    *out = t;
+  });
+}
+
+int tn_ttn_destroy(tn_ttn* t) {
+  if (!t) return TN_OK;
+  if (t->ctx) (void)hipStreamSynchronize(t->ctx->stream);
+  delete t;
+  return TN_OK;
+}
+
+int tn_ttn_set_link(tn_ttn* t, int a, int sa, int b, int sb, int nchan) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& A = node_of(t, a);
+    TTNNode& B = node_of(t, b);
+    TN_REQUIRE(sa >= 0 && sa < 3 && sb >= 0 && sb < 3 && a != b && nchan >= 1, TN_E_INVALID,
+               "tn_ttn_set_link: bad slots");
+    A.slot[sa].nbr = b;
+    A.slot[sa].nbr_slot = sb;
+    A.slot[sa].nchan = nchan;
+    A.slot[sa].env_ok = false;
+    B.slot[sb].nbr = a;
+    B.slot[sb].nbr_slot = sa;
+    B.slot[sb].nchan = nchan;
+    B.slot[sb].env_ok = false;
+  });
+}
+
+int tn_ttn_set_site(tn_ttn* t, int node, int slot, int d, int nchan, const double* ops) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(slot >= 0 && slot < 2 && d >= 1 && nchan >= 1 && ops, TN_E_INVALID,
+               "tn_ttn_set_site: sites occupy slot 0 or 1");
+    TTNSlot& S = N.slot[slot];
+    S.nbr = -1;
+    S.nbr_slot = -1;
+    S.nchan = nchan;
+    const size_t bytes = (size_t)nchan * d * d * sizeof(double);
+    S.env.ensure(bytes);
+    TN_HIP(hipMemcpyAsync(S.env.p, ops, bytes, hipMemcpyHostToDevice, t->ctx->stream));
+    TN_HIP(hipStreamSynchronize(t->ctx->stream));
+    S.env_dim = d;
+    S.env_ok = true;
+  });
+}
+
+int tn_ttn_set_coupling(tn_ttn* t, int node, int nnz, const int* c0, const int* c1, const int* c2, const double* w) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(nnz >= 1 && c0 && c1 && c2 && w, TN_E_INVALID, "tn_ttn_set_coupling: empty coupling");
+    for (int s = 0; s < 3; ++s) N.c[s].assign(s == 0 ? c0 : s == 1 ? c1 : c2, (s == 0 ? c0 : s == 1 ? c1 : c2) + nnz);
+    N.w.assign(w, w + nnz);
+    for (int k = 0; k < nnz; ++k)
+      for (int s = 0; s < 3; ++s)
+        TN_REQUIRE(N.c[s][k] >= 0 && N.c[s][k] < N.slot[s].nchan, TN_E_SHAPE,
+                   "tn_ttn_set_coupling: channel index out of range (set links and sites first)");
+  });
+}
+
+int tn_ttn_set_node(tn_ttn* t, int node, const long* dims, const double* A) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(dims && A && dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, TN_E_INVALID, "tn_ttn_set_node: bad dims");
+    for (int s = 0; s < 3; ++s) {
+      if (N.dims[s] != dims[s]) N.slot[s].env_ok = N.slot[s].env_ok && N.slot[s].env_dim == dims[s];
+      N.dims[s] = dims[s];
+    }
+    const size_t bytes = (size_t)N.size() * sizeof(double);
+    N.A.ensure(bytes);
+    TN_HIP(hipMemcpyAsync(N.A.p, A, bytes, hipMemcpyHostToDevice, t->ctx->stream));
+    TN_HIP(hipStreamSynchronize(t->ctx->stream));
+    N.has_tensor = true;
+  });
+}
+
+int tn_ttn_get_node(tn_ttn* t, int node, long* dims, double* A) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    if (dims)
+      for (int s = 0; s < 3; ++s) dims[s] = N.dims[s];
+    if (A) {
+      TN_REQUIRE(N.has_tensor, TN_E_INVALID, "tn_ttn_get_node: node tensor not set");
+      TN_HIP(hipMemcpyAsync(A, N.A.p, (size_t)N.size() * sizeof(double), hipMemcpyDeviceToHost, t->ctx->stream));
+      TN_HIP(hipStreamSynchronize(t->ctx->stream));
+    }
+  });
+}
+
+int tn_ttn_node_ptr(tn_ttn* t, int node, double** dev) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(dev && N.has_tensor, TN_E_INVALID, "tn_ttn_node_ptr: node tensor not set");
+    *dev = N.A.as<double>();
+  });
+}
+
+int tn_ttn_update_env(tn_ttn* t, int node, int slot) {
+  return guarded_t(CT(t), [&] {
+    node_of(t, node);
+    update_env(t, node, slot);
+  });
+}
+
+int tn_ttn_matvec(tn_ttn* t, int node, const double* v, double* out) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(v && out, TN_E_INVALID, "tn_ttn_matvec: null vector");
+    (void)N;
+    matvec_node(t, node, v, out);
+  });
+}
+
+int tn_ttn_eig(tn_ttn* t, int node, int krylovdim, int maxiter, double tol, double* energy, tn_krylov_info* info) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(N.has_tensor, TN_E_INVALID, "tn_ttn_eig: node tensor not set");
+    for (int s = 0; s < 3; ++s) require_env(N, s);
+    const double e = krylov_eig_op(
+        t->ctx, false, N.size(), [&](const double* x, double* y) { matvec_node(t, node, x, y); }, N.A.as<double>(),
+        krylovdim, maxiter, tol, info);
+    if (energy) *energy = e;
+  });
+}
+
+int tn_ttn_move(tn_ttn* t, int node, int slot, long maxdim, long mindim, double cutoff, double* truncerr,
+                long* newdim) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(N.has_tensor, TN_E_INVALID, "moveisometry_to_next!: node tensor not set");
+    move_isometry(t, node, slot, maxdim, mindim, cutoff, truncerr, newdim);
+  });
+}
+
+int tn_ttn_normalize(tn_ttn* t, int node, double* norm) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(N.has_tensor, TN_E_INVALID, "normalize!: node tensor not set");
+    const double nr = std::sqrt(node_norm2(t, N));
+    if (nr > 0.0) scale_f64(t->ctx, N.A.as<double>(), 1.0 / nr, N.size());
+    if (norm) *norm = nr;
+  });
+}
+
+int tn_ttn_node_norm(tn_ttn* t, int node, double* norm) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(N.has_tensor && norm, TN_E_INVALID, "tn_ttn_node_norm: node tensor not set");
+    *norm = std::sqrt(node_norm2(t, N));
+  });
+}
+
+int tn_ttn_expand(tn_ttn* t, int node, int slot, long pad, const double* padA, const double* padB) {
+  return guarded_t(CT(t), [&] {
+    TTNNode& N = node_of(t, node);
+    TN_REQUIRE(slot >= 0 && slot < 3 && N.slot[slot].nbr >= 0 && pad >= 0 && padA && padB, TN_E_INVALID,
+               "subspace_expand!: bad arguments");
+    TTNNode& M = t->nodes[N.slot[slot].nbr];
+    const int ts = N.slot[slot].nbr_slot;
+    if (pad == 0) return;
+    tn_ctx* c = t->ctx;
+    auto grow = [&](TTNNode& X, int s, const double* P) {
+      long pd[3] = {X.dims[0], X.dims[1], X.dims[2]};
+      pd[s] = pad;
+      const size_t pb = (size_t)(pd[0] * pd[1] * pd[2]) * sizeof(double);
+      double* Pd = (double*)c->workspace(WS_TTN_P, pb);
+      TN_HIP(hipMemcpyAsync(Pd, P, pb, hipMemcpyHostToDevice, c->stream));
+      long nd[3] = {X.dims[0], X.dims[1], X.dims[2]};
+      nd[s] += pad;
+      DevBuf Y;
+      Y.ensure((size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
+      concat3_f64(c, X.A.as<double>(), X.dims, s, Pd, pad, Y.as<double>());
+      TN_HIP(hipStreamSynchronize(c->stream));
+      X.A = std::move(Y);
+      for (int i = 0; i < 3; ++i) X.dims[i] = nd[i];
+      X.slot[s].env_ok = false;
+    };
+    grow(N, slot, padA);
+    grow(M, ts, padB);
+  });
+}
+
+}  // extern "C"

@@ -93,6 +93,21 @@ SIGNATURES = {
     "tn_env_reset": [_P],
     "tn_env_set_projectors": [_P, _I, _D],
     "tn_env_set_projector_site": [_P, _I, _I, _P, _L, _L],
+    "tn_ttn_create": [_P, _I, C.POINTER(_P)],
+    "tn_ttn_destroy": [_P],
+    "tn_ttn_set_link": [_P, _I, _I, _I, _I, _I],
+    "tn_ttn_set_site": [_P, _I, _I, _I, _I, _PD],
+    "tn_ttn_set_coupling": [_P, _I, _I, _PI, _PI, _PI, _PD],
+    "tn_ttn_set_node": [_P, _I, _PL, _PD],
+    "tn_ttn_get_node": [_P, _I, _PL, _PD],
+    "tn_ttn_node_ptr": [_P, _I, C.POINTER(_PD)],
+    "tn_ttn_update_env": [_P, _I, _I],
+    "tn_ttn_matvec": [_P, _I, _P, _P],
+    "tn_ttn_eig": [_P, _I, _I, _I, _D, _PD, C.POINTER(KrylovInfo)],
+    "tn_ttn_move": [_P, _I, _I, _L, _L, _D, _PD, _PL],
+    "tn_ttn_normalize": [_P, _I, _PD],
+    "tn_ttn_node_norm": [_P, _I, _PD],
+    "tn_ttn_expand": [_P, _I, _I, _L, _PD, _PD],
     "tn_env_phi_dims": [_P, _PL, _PL],
     "tn_env_matvec": [_P, _P, _P, _I],
     "tn_env_contract_phi": [_P, _I, _P],
