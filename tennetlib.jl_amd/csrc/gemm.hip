@@ -539,11 +539,81 @@ GArgs to_args(const GemmDesc& g) {
 
 }  // namespace
 
+namespace {
+// C = alpha * sum_p part[p] + beta * C   (part: P slices of M x N, ld N)
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(const T* __restrict__ part, int P, long M, long N,
+                                                            double alpha, double beta, T* __restrict__ C, long ldc) {
+  const long tot = M * N;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < tot; i += (long)gridDim.x * 256) {
+    double acc = 0.0;
+    for (int q = 0; q < P; ++q) acc += (double)part[(long)q * tot + i];
+    const long r = i / N, col = i - r * N;
+    T* dst = C + r * ldc + col;
+    const double prev = (beta != 0.0) ? beta * (double)*dst : 0.0;
+    *dst = (T)(alpha * acc + prev);
+  }
+}
+}  // namespace
+
+static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g);
+
+// Few output tiles with a long K (K-reductions over whole tensors, Gram
+// matrices of tall matrices) leave most of the chip idle: split K into
+// chunks computed as one batched GEMM into FP64/FP32 partials, then reduce.
+static bool gemm_split_k(tn_ctx* c, Prec p, const GemmDesc& g) {
+  if (!(p == Prec::F64 || p == Prec::F32) || g.batch != 1) return false;
+  const long tiles = ((g.M + 127) / 128) * ((g.N + 127) / 128);
+  if (tiles >= 64 || g.K < 4096) return false;
+  const long P0 = std::min<long>(g.K / 512, 256 / tiles);
+  if (P0 < 2) return false;
+  const long Kc = ((g.K + P0 - 1) / P0 + 15) / 16 * 16;
+  const long P = (g.K + Kc - 1) / Kc;
+  const long full = g.K / Kc;  // chunks of exactly Kc
+  const size_t es = (p == Prec::F64) ? 8 : 4;
+  char* part = (char*)c->workspace(WS_GEMM_SPLIT, (size_t)P * g.M * g.N * es);
+  GemmDesc q = g;
+  q.alpha = 1.0;
+  q.beta = 0.0;
+  q.K = Kc;
+  q.C = part;
+  q.ldc = g.N;
+  q.sC = g.M * g.N;
+  q.sA = g.tA ? Kc * g.lda : Kc;
+  q.sB = g.tB ? Kc : Kc * g.ldb;
+  q.batch = (int)full;
+  gemm_impl(c, p, q);
+  if (full < P) {  // the shorter last chunk
+    GemmDesc r = q;
+    r.batch = 1;
+    r.K = g.K - full * Kc;
+    r.A = (const char*)g.A + (size_t)(full * q.sA) * es;
+    r.B = (const char*)g.B + (size_t)(full * q.sB) * es;
+    r.C = part + (size_t)(full * q.sC) * es;
+    gemm_impl(c, p, r);
+  }
+  const long tot = g.M * g.N;
+  const unsigned grid = (unsigned)std::max<long>(1, std::min<long>(2048, (tot + 255) / 256));
+  if (p == Prec::F64)
+    hipLaunchKernelGGL(splitk_reduce_kernel<double>, dim3(grid), dim3(256), 0, c->stream, (const double*)part, (int)P,
+                       g.M, g.N, g.alpha, g.beta, (double*)g.C, g.ldc);
+  else
+    hipLaunchKernelGGL(splitk_reduce_kernel<float>, dim3(grid), dim3(256), 0, c->stream, (const float*)part, (int)P,
+                       g.M, g.N, g.alpha, g.beta, (float*)g.C, g.ldc);
+  TN_HIP(hipGetLastError());
+  return true;
+}
+
 void gemm(tn_ctx* c, Prec p, const GemmDesc& g) {
   TN_REQUIRE(g.M >= 0 && g.N >= 0 && g.K >= 0 && g.batch >= 1, TN_E_INVALID, "gemm: bad sizes");
   if (g.M == 0 || g.N == 0) return;
   TN_REQUIRE(g.A && g.B && g.C, TN_E_INVALID, "gemm: null operand");
   TN_REQUIRE(g.batch <= 65535, TN_E_INVALID, "gemm: batch too large");
+  if (gemm_split_k(c, p, g)) return;
+  gemm_impl(c, p, g);
+}
+
+static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g) {
   if (gemm_fast_ok(p, g)) {
     gemm_fast(c, p, g);
     return;

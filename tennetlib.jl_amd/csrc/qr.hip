@@ -66,6 +66,92 @@ __global__ __launch_bounds__(256) void qr_step_kernel(double* __restrict__ X, lo
   for (long i = j + threadIdx.x; i < m; i += 256) y[i] -= f * (i == j ? 1.0 : xj[i] * sc);
 }
 
+// Register-resident step for m - j - 1 <= QR_THREADS * ITEMS: each thread
+// loads its ITEMS entries of columns j and c once (all loads in flight),
+// both sums of the step come from one pass (|x|^2 for the reflector and
+// x.y for the update), and column c is written once.
+constexpr int QR_THREADS = 512;
+
+__device__ void qr_block_sum2(double& a, double& b, double* red) {
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_down(a, o, 64);
+    b += __shfl_down(b, o, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[wave] = a;
+    red[16 + wave] = b;
+  }
+  __syncthreads();
+  double sa = 0.0, sb = 0.0;
+#pragma unroll
+  for (int w = 0; w < QR_THREADS / 64; ++w) {
+    sa += red[w];
+    sb += red[16 + w];
+  }
+  a = sa;
+  b = sb;
+}
+
+template <int ITEMS>
+__global__ __launch_bounds__(QR_THREADS) void qr_step_reg_kernel(double* __restrict__ X, long m, int j,
+                                                                 double* __restrict__ Vt, double* __restrict__ tau,
+                                                                 double* __restrict__ beta) {
+  __shared__ double red[32];
+  const double* xj = X + (long)j * m;
+  const long c = (long)j + blockIdx.x;
+  double* y = X + c * m;
+  const bool upd = blockIdx.x != 0;
+  const int tid = threadIdx.x;
+  const long base = (long)j + 1 + tid;
+  const double alpha = xj[j];
+  const double yj = y[j];
+  double xv[ITEMS], yv[ITEMS];
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const long i = base + (long)QR_THREADS * k;
+    xv[k] = i < m ? xj[i] : 0.0;
+    yv[k] = (upd && i < m) ? y[i] : 0.0;
+  }
+  double s = 0.0, d = 0.0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    s = fma(xv[k], xv[k], s);
+    d = fma(xv[k], yv[k], d);
+  }
+  qr_block_sum2(s, d, red);
+  double t = 0.0, b = alpha, sc = 0.0;
+  if (s > 0.0) {
+    const double nrm = sqrt(fma(alpha, alpha, s));
+    b = alpha >= 0.0 ? -nrm : nrm;
+    t = (b - alpha) / b;
+    sc = 1.0 / (alpha - b);
+  }
+  if (!upd) {
+    double* v = Vt + (long)j * m;
+    for (long i = tid; i <= j; i += QR_THREADS) v[i] = i < j ? 0.0 : 1.0;
+#pragma unroll
+    for (int k = 0; k < ITEMS; ++k) {
+      const long i = base + (long)QR_THREADS * k;
+      if (i < m) v[i] = xv[k] * sc;
+    }
+    if (tid == 0) {
+      tau[j] = t;
+      beta[j] = b;
+    }
+    return;
+  }
+  if (t == 0.0) return;
+  const double f = t * fma(sc, d, yj);
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const long i = base + (long)QR_THREADS * k;
+    if (i < m) y[i] = fma(-f * sc, xv[k], yv[k]);
+  }
+  if (tid == 0) y[j] = yj - f;
+}
+
 // Qt (k x m) = [I_k | 0]
 __global__ void qr_eye_kernel(double* __restrict__ Qt, long k, long m) {
   const long tot = k * m;
@@ -124,7 +210,18 @@ void householder_qr(tn_ctx* c, double* X, long m, long n, double* Qt, double* R)
   double* beta = tb + k;
   double* S = tb + 2 * k;
   for (long j = 0; j < k; ++j) {
-    hipLaunchKernelGGL(qr_step_kernel, dim3((unsigned)(n - j)), dim3(256), 0, c->stream, X, m, (int)j, Vt, tau, beta);
+    const long below = m - j - 1;
+    const dim3 grid((unsigned)(n - j));
+    if (below <= QR_THREADS * 1L)
+      hipLaunchKernelGGL(qr_step_reg_kernel<1>, grid, dim3(QR_THREADS), 0, c->stream, X, m, (int)j, Vt, tau, beta);
+    else if (below <= QR_THREADS * 4L)
+      hipLaunchKernelGGL(qr_step_reg_kernel<4>, grid, dim3(QR_THREADS), 0, c->stream, X, m, (int)j, Vt, tau, beta);
+    else if (below <= QR_THREADS * 16L)
+      hipLaunchKernelGGL(qr_step_reg_kernel<16>, grid, dim3(QR_THREADS), 0, c->stream, X, m, (int)j, Vt, tau, beta);
+    else if (below <= QR_THREADS * 32L)
+      hipLaunchKernelGGL(qr_step_reg_kernel<32>, grid, dim3(QR_THREADS), 0, c->stream, X, m, (int)j, Vt, tau, beta);
+    else
+      hipLaunchKernelGGL(qr_step_kernel, grid, dim3(256), 0, c->stream, X, m, (int)j, Vt, tau, beta);
     TN_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(qr_r_kernel, dim3(grid1(k * n)), dim3(256), 0, c->stream, X, m, beta, k, n, R);
