@@ -157,6 +157,50 @@ def tdvp_run(T, N, steps, dt, maxdim, cutoff):
     return out
 
 
+def ttn_instances(T, count, seed=5, total=256):
+    """Many-instance TTN ground states: `total` random-field Heisenberg chains
+    (h_i ~ U[-2, 2], N ~ U{16..64}, numpy default_rng(seed)), each a
+    default binary-tree TTN from default_randomTTN(N, 12) optimised with the
+    reference test's schedule (maxdim 128, 5 sweeps, cutoff 1e-14, noise
+    1e-2, noisedecay 5, disable_noise_after 2).  The first `count` instances
+    run one after another on the library's stream; the projection to all
+    instances scales the wall time by the sum of N."""
+    rng = np.random.default_rng(seed)
+    Ns = [int(n) for n in rng.integers(16, 65, total)]
+    hs = [rng.uniform(-2, 2, n) for n in Ns]
+    params = T.make_optimize_params(maxdim=[128], nsweeps=[5], cutoff=1e-14, noise=1e-2, noisedecay=5,
+                                    disable_noise_after=2)
+    res = []
+    ctx = T.default_context()
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    t0 = time.perf_counter()
+    for i in range(count):
+        N = Ns[i]
+        os_ = T.OpStrings()
+        for j in range(1, N):
+            os_ += (1.0, "Sz", j, "Sz", j + 1)
+            os_ += (0.5, "S+", j, "S-", j + 1)
+            os_ += (0.5, "S-", j, "S+", j + 1)
+        for j in range(1, N + 1):
+            os_ += (float(hs[i][j - 1]), "Sz", j)
+        psi0 = T.default_randomTTN(N, 12, seed=i)
+        env = T.StateEnvsTTN(psi0, T.CouplingModel(os_, N))
+        sw = T.optimize_(env, params, T.default_sweeppath(psi0), outputlevel=0, seed=i)
+        env.close()
+        res.append((sw.energy[-1], sw.maxchi[-1]))
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    dt = time.perf_counter() - t0
+    done_n = sum(Ns[:count])
+    return {"workload": f"{total} random-field Heisenberg chains, N~U{{16..64}}, h~U[-2,2] (seed {seed}); "
+                        f"binary-tree TTN from default_randomTTN(N, 12), maxdim 128, 5 sweeps, noise 1e-2 "
+                        f"(decay 5, off after 2), cutoff 1e-14; FP64",
+            "instances_timed": count, "wall_s": round(dt, 3), "instances_per_s": round(count / dt, 4),
+            "site_sweeps_per_s": round(5 * done_n / dt, 1),
+            "projected_s_for_all": round(dt * sum(Ns) / done_n, 1),
+            "energy_per_site": [round(res[i][0] / Ns[i], 10) for i in range(count)],
+            "maxchi": [res[i][1] for i in range(count)]}
+
+
 def cpu_sweep(snap, N, chi, cutoff, kd, mi):
     """The FP64 numpy oracle running the same timed sweep (same snapshot,
     maxdim, cutoff and Krylov settings) on the host cores."""
@@ -197,6 +241,7 @@ def main():
     ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
     ap.add_argument("--mv-wide", type=int, default=1, help="also time the w = 20 (J1-J2 cylinder) matvec")
     ap.add_argument("--tdvp-steps", type=int, default=40, help="TDVP steps of the XXZ N=100 run (0: skip)")
+    ap.add_argument("--ttn-instances", type=int, default=16, help="TTN instances timed of the 256 (0: skip)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -272,6 +317,11 @@ def main():
         tdvp = tdvp_run(T, 100, args.tdvp_steps, 0.05, 512, 1e-12)
         log(f"tdvp: {tdvp}")
 
+    ttn = None
+    if args.ttn_instances > 0:
+        ttn = ttn_instances(T, args.ttn_instances)
+        log(f"ttn: {ttn['instances_timed']} instances in {ttn['wall_s']} s")
+
     cpu = None
     if not args.skip_cpu:
         t_cpu, cores, e_cpu = cpu_sweep(snap, N, chi, args.cutoff, args.krylovdim, args.maxiter)
@@ -314,6 +364,7 @@ def main():
                      "kernel": f"matvec GEMM ({args.prec}), M=5*{args.mv_chi}, N=4*{args.mv_chi}, "
                                f"K={args.mv_chi}; {F_gemm:.3e} algorithmic flop/launch"},
         "tdvp_xxz_n100": tdvp,
+        "ttn_random_field_batch": ttn,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

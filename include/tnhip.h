@@ -242,6 +242,31 @@ int tn_ttn_node_norm(tn_ttn* ttn, int node, double* norm);
  * node's tensor shape with that leg replaced by pad, row-major) */
 int tn_ttn_expand(tn_ttn* ttn, int node, int slot, long pad, const double* padA_host,
                   const double* padB_host);
+/* orthogonality center of the TTN (ttn.orthocenter, src/ttn/ttn.jl:100-110); -1: none */
+int tn_ttn_set_center(tn_ttn* ttn, int node);
+int tn_ttn_get_center(tn_ttn* ttn, int* node);
+/* seed of the subspace-expansion padding (SplitMix64 -> Box-Muller normals;
+ * the reference draws it with Julia's RNG, update_site_ttn.jl:91,100) */
+int tn_ttn_seed(tn_ttn* ttn, unsigned long long seed);
+/* position!(sysenv, node; maxdim, mindim, cutoff, normalize, node_to_skip)
+ * (src/ttn/state_envs_ttn.jl:120-140): isometry moves along the path (QR, or
+ * SVD when cutoff > 0 or a bond exceeds maxdim), normalisation, then the
+ * environments along the same path except from `skip` (-1: none) */
+int tn_ttn_position(tn_ttn* ttn, int node, long maxdim, long mindim, double cutoff, int normalize, int skip);
+/* update_position!(sysenv, eig_solver, node; ...) (src/ttn/update_site_ttn.jl:42-63) */
+int tn_ttn_update_position(tn_ttn* ttn, int node, long maxdim, long mindim, double cutoff, int normalize,
+                           int krylovdim, int maxiter, double tol, int check_convergence, double* energy);
+/* subspace_expand!(psi, node, neighbour at `slot`, max_expand_dim, noise)
+ * (src/ttn/update_site_ttn.jl:65-109) with padding from the seeded generator */
+int tn_ttn_expand_random(tn_ttn* ttn, int node, int slot, long max_expand_dim, double noise);
+/* fullsweep!(sysenv, sweeppath, eig_solver, swdata; ...) (src/ttn/sweep_ttn.jl:117-274):
+ * path and its reverse; with noise >= 1e-13 the expansion branch toward
+ * `central` (find_eccentric_central_node of the graph).  energy receives the
+ * last update's Ritz value, maxchi the largest bond dimension. */
+int tn_ttn_fullsweep(tn_ttn* ttn, const int* path, int npath, int central, long maxdim, long mindim,
+                     double cutoff, double noise, int expand_dim, int max_expand_dim, int expand_numiter,
+                     int krylovdim, int maxiter, double tol, int normalize, int check_convergence,
+                     double* energy, long* maxchi);
 
 #ifdef __cplusplus
 }

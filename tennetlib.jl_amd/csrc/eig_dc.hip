@@ -1458,8 +1458,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     TN_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const size_t shm = 2 * (size_t)(p - 1) * sizeof(double);
     if (shm > 64 * 1024)
-      TN_HIP(hipFuncSetAttribute((const void*)trd_coop_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shm));
+      raise_lds_limit((const void*)trd_coop_kernel, (size_t)shm);
     int per_cu = 0;
     TN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)trd_coop_kernel, 256, shm));
     const long want = std::min<long>(256, std::max<long>(8, (p - 1 + 7) / 8));
@@ -1484,8 +1483,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   if (!done) {
     const size_t lds_bytes = (size_t)p * sizeof(double);
     if (lds_bytes > 64 * 1024)
-      TN_HIP(hipFuncSetAttribute((const void*)trd_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds_bytes));
+      raise_lds_limit((const void*)trd_reflect_kernel, (size_t)lds_bytes);
     const int j0 = std::max(0, P - TAIL);  // columns j0.. are reduced in LDS
     if (j0 >= 1 && P <= 6000) {
       // fused path: reflector 0, then one launch per column, then the update of step j0 - 1
@@ -1494,12 +1492,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
       double* yB = vB + p;
       double* pB = yB + p;  // >= 512 + p / 4 entries
       const size_t step_lds_max = 3 * (size_t)(P - 1) * sizeof(double);
-      static size_t step_attr = 0;
-      if (step_lds_max > 64 * 1024 && step_lds_max > step_attr) {
-        TN_HIP(hipFuncSetAttribute((const void*)trd_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)step_lds_max));
-        step_attr = step_lds_max;
-      }
+      raise_lds_limit((const void*)trd_step_kernel, step_lds_max);
       const int m = P - 1;
       const int np0 = (m + HR - 1) / HR;
       hipLaunchKernelGGL(trd_reflect_kernel, dim3(np0), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg, P,
@@ -1533,12 +1526,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     }
     const int m0 = P - j0;
     const size_t tshm = ((size_t)m0 * (m0 + 1) + 2 * (size_t)m0) * sizeof(double);
-    static bool tail_attr = false;
-    if (!tail_attr) {
-      TN_HIP(hipFuncSetAttribute((const void*)trd_tail_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(((size_t)TAIL * (TAIL + 1) + 2 * TAIL) * sizeof(double))));
-      tail_attr = true;
-    }
+    raise_lds_limit((const void*)trd_tail_lds_kernel, ((size_t)TAIL * (TAIL + 1) + 2 * TAIL) * sizeof(double));
     hipLaunchKernelGGL(trd_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, j0, dv, ev, tauv);
   }
   TN_HIP(hipGetLastError());
@@ -1620,12 +1608,7 @@ void sym_eig_dc_z(tn_ctx* c, double* Gd, long p, long ldg, double* Vtd, long ldv
   const int j0 = std::max(0, P - TAILZ);
   {
     const size_t lds_max = (size_t)p * sizeof(double2);
-    static size_t attr = 0;
-    if (lds_max > 64 * 1024 && lds_max > attr) {
-      TN_HIP(hipFuncSetAttribute((const void*)trdz_reflect_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds_max));
-      attr = lds_max;
-    }
+    raise_lds_limit((const void*)trdz_reflect_kernel, lds_max);
   }
   for (int j = 0; j < j0; ++j) {
     const int m = P - j - 1;
@@ -1639,12 +1622,7 @@ void sym_eig_dc_z(tn_ctx* c, double* Gd, long p, long ldg, double* Vtd, long ldv
   {
     const int m0 = P - j0;
     const size_t tshm = ((size_t)m0 * (m0 + 1) + 2 * (size_t)m0) * sizeof(double2);
-    static bool tail_attr = false;
-    if (!tail_attr) {
-      TN_HIP(hipFuncSetAttribute((const void*)trdz_tail_lds_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(((size_t)TAILZ * (TAILZ + 1) + 2 * TAILZ) * sizeof(double2))));
-      tail_attr = true;
-    }
+    raise_lds_limit((const void*)trdz_tail_lds_kernel, ((size_t)TAILZ * (TAILZ + 1) + 2 * TAILZ) * sizeof(double2));
     hipLaunchKernelGGL(trdz_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, j0, dv, ev, tauz);
   }
   TN_HIP(hipGetLastError());

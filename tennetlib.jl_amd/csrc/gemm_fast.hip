@@ -15,6 +15,9 @@
 // Row strides must be multiples of the 16-byte vector width and bases
 // 16-byte aligned; the dispatcher in gemm.hip checks this and otherwise
 // uses the generic kernels.
+#include <map>
+#include <mutex>
+
 #include "tn_internal.h"
 
 #include <algorithm>
@@ -439,11 +442,22 @@ __global__ __launch_bounds__(256) void split_transpose_kernel(const T* __restric
 }  // namespace
 
 // ---------------------------------------------------------------------------
-// Raise the dynamic-LDS limit of a kernel above the 64 KiB default (once).
-static void allow_lds(const void* fn, size_t bytes) {
-  if (bytes <= 64 * 1024) return;
-  TN_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+std::recursive_mutex& api_mutex() {
+  static std::recursive_mutex m;
+  return m;
 }
+
+void raise_lds_limit(const void* fn, size_t bytes) {
+  if (bytes <= 64 * 1024) return;
+  static std::mutex mu;
+  static std::map<const void*, size_t> done;  // kernel -> limit set so far
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& cur = done[fn];
+  if (bytes <= cur) return;
+  TN_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  cur = bytes;
+}
+static void allow_lds(const void* fn, size_t bytes) { raise_lds_limit(fn, bytes); }
 
 bool gemm_fast_ok(Prec p, const GemmDesc& g) {
   if (p != Prec::F64 && p != Prec::F32) return false;

@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <string>
 #include <functional>
+#include <mutex>
 #include <vector>
 
 #include "../../include/tnhip.h"
@@ -49,8 +50,14 @@ struct DevBuf {
     return *this;
   }
   ~DevBuf() { release(); }
+  // Kernels queued on any stream may still use the buffer: wait for them
+  // before the memory goes back to the allocator, where another context
+  // (another stream, another host thread) could receive it at once.
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipDeviceSynchronize();
+      (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -254,6 +261,15 @@ void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bo
 // real symmetric p x p (3 <= p <= 16384) by Householder tridiagonalisation and
 // divide and conquer (eig_dc.hip); same contract as sym_eig_jacobi (G is
 // overwritten, Vt rows are the eigenvectors, w their eigenvalues)
+// One library call at a time per process: every C entry point holds this
+// lock for its duration (calls on different contexts from different host
+// threads are serialised; their GPU work still overlaps where it is queued
+// asynchronously).
+std::recursive_mutex& api_mutex();
+// Raise a kernel's dynamic-LDS limit to at least `bytes` (above the 64 KiB
+// default).  Serialised and applied once per kernel and size: changing the
+// attribute while another host thread launches the same kernel is not safe.
+void raise_lds_limit(const void* kernel, size_t bytes);
 // compact-WY helper: W (nb x n, ld n) <- T W for reflectors j0..j0+nb-1 with
 // T^{-1} = diag(1/tau) + striu(S), S = V^T V (nb x nb, ld 64), nb <= 64
 constexpr int WY_NB = 64;

@@ -36,6 +36,7 @@ int set_error(tn_ctx* c, int code, const std::string& msg) {
 
 template <class F>
 int guarded(tn_ctx* c, F&& f) {
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
   try {
     f();
     return TN_OK;
@@ -1407,6 +1408,7 @@ int tn_ctx_create(int device, tn_ctx** out) {
 }
 
 int tn_ctx_destroy(tn_ctx* c) {
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
   if (!c) return TN_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
@@ -1543,6 +1545,7 @@ int tn_env_create(tn_ctx* c, int N, const int* d, const int* w, int dtype, tn_en
 }
 
 int tn_env_destroy(tn_env* e) {
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
   if (!e) return TN_OK;
   (void)hipStreamSynchronize(e->ctx->stream);
   delete e;

@@ -22,6 +22,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -41,6 +43,7 @@ struct TTNSlot {
 struct TTNNode {
   long dims[3] = {0, 0, 0};
   DevBuf A;
+  DevBuf spare;  // grow-only second buffer: new tensors are written here, then swapped with A
   TTNSlot slot[3];
   std::vector<int> c[3];
   std::vector<double> w;
@@ -51,6 +54,8 @@ struct TTNNode {
 struct tn_ttn {
   tn_ctx* ctx = nullptr;
   std::vector<TTNNode> nodes;
+  int center = -1;                       // orthogonality center (-1: none)
+  uint64_t rng = 0x853c49e6748fea9bULL;  // SplitMix64 state of the expansion padding
 };
 
 namespace {
@@ -58,6 +63,7 @@ thread_local std::string g_err;
 
 template <class F>
 int guarded_t(tn_ctx* c, F&& f) {
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
   try {
     f();
     return TN_OK;
@@ -309,7 +315,7 @@ void move_isometry(tn_ttn* t, int n, int s, long maxdim, long mindim, double cut
   // node n <- Q (leg s of dimension k)
   long nd[3] = {D0, D1, D2};
   nd[s] = k;
-  DevBuf An;
+  DevBuf& An = N.spare;
   An.ensure((size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
   if (s == 0) copy_f64(c, An.as<double>(), Qn, k * m);
   else if (s == 1) swap01_f64(c, Qn, k, D0, D2, An.as<double>());
@@ -318,7 +324,7 @@ void move_isometry(tn_ttn* t, int n, int s, long maxdim, long mindim, double cut
   const long B0 = M.dims[0], B1 = M.dims[1], B2 = M.dims[2];
   long md[3] = {B0, B1, B2};
   md[ts] = k;
-  DevBuf Bn;
+  DevBuf& Bn = M.spare;
   Bn.ensure((size_t)(md[0] * md[1] * md[2]) * sizeof(double));
   const double* B = M.A.as<double>();
   GemmDesc g;
@@ -340,9 +346,8 @@ void move_isometry(tn_ttn* t, int n, int s, long maxdim, long mindim, double cut
     g.batch = (int)B0;
   }
   gemm64(c, g);
-  TN_HIP(hipStreamSynchronize(c->stream));
-  N.A = std::move(An);
-  M.A = std::move(Bn);
+  std::swap(N.A, N.spare);
+  std::swap(M.A, M.spare);
   for (int i = 0; i < 3; ++i) {
     N.dims[i] = nd[i];
     M.dims[i] = md[i];
@@ -362,6 +367,136 @@ double node_norm2(tn_ttn* t, const TTNNode& N) {
   return c->pinned[0];
 }
 
+// ---- sweep driver (src/ttn/state_envs_ttn.jl:120-140, update_site_ttn.jl,
+// sweep_ttn.jl:117-274) --------------------------------------------------
+
+// shortest path a -> b in the tree (unique), as node indices
+std::vector<int> tree_path(tn_ttn* t, int a, int b) {
+  const int n = (int)t->nodes.size();
+  std::vector<int> par(n, -2);
+  std::vector<int> q{a};
+  par[a] = -1;
+  for (size_t h = 0; h < q.size() && par[b] == -2; ++h) {
+    const int x = q[h];
+    for (int s = 0; s < 3; ++s) {
+      const int y = t->nodes[x].slot[s].nbr;
+      if (y >= 0 && par[y] == -2) {
+        par[y] = x;
+        q.push_back(y);
+      }
+    }
+  }
+  TN_REQUIRE(par[b] != -2, TN_E_INVALID, "`shortest_path()`: destination is not reachable");
+  std::vector<int> out;
+  for (int x = b; x != -1; x = par[x]) out.push_back(x);
+  std::reverse(out.begin(), out.end());
+  return out;
+}
+
+int slot_to(const TTNNode& N, int m) {
+  for (int s = 0; s < 3; ++s)
+    if (N.slot[s].nbr == m) return s;
+  throw Error(TN_E_INVALID, "TTN: nodes are not neighbours");
+}
+
+// position!(sysenv, node; ...): isometrize! then move_environment! along the path
+void position(tn_ttn* t, int node, long maxdim, long mindim, double cutoff, bool normalize, int skip) {
+  TN_REQUIRE(t->center >= 0, TN_E_POSITION, "`isometrize!()`: TTN does not have a proper orthogonality center !!");
+  const std::vector<int> path = tree_path(t, t->center, node);
+  const double cut = cutoff > 0.0 ? cutoff : -1.0;
+  for (size_t k = 0; k + 1 < path.size(); ++k)
+    move_isometry(t, path[k], slot_to(t->nodes[path[k]], path[k + 1]), maxdim, mindim, cut, nullptr, nullptr);
+  t->center = node;
+  if (normalize) {
+    TTNNode& N = t->nodes[node];
+    const double nr = std::sqrt(node_norm2(t, N));
+    if (nr > 0.0) scale_f64(t->ctx, N.A.as<double>(), 1.0 / nr, N.size());
+  }
+  for (size_t k = 0; k + 1 < path.size(); ++k) {
+    if (path[k] == skip) continue;
+    update_env(t, path[k], slot_to(t->nodes[path[k]], path[k + 1]));
+  }
+}
+
+double eig_at(tn_ttn* t, int node, int kdim, int maxiter, double tol, bool check) {
+  TTNNode& N = t->nodes[node];
+  for (int s = 0; s < 3; ++s) require_env(N, s);
+  tn_krylov_info info{};
+  const double e = krylov_eig_op(
+      t->ctx, false, N.size(), [&](const double* x, double* y) { matvec_node(t, node, x, y); }, N.A.as<double>(), kdim,
+      maxiter, tol, &info);
+  if (check && info.converged < 1) throw Error(TN_E_NOTCONV, "`eig_solver()` not converged !!");
+  return e;
+}
+
+// standard normals: SplitMix64 -> two 53-bit uniforms -> Box-Muller
+double next_u01(uint64_t& st) {
+  st += 0x9E3779B97F4A7C15ULL;
+  uint64_t z = st;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  z ^= z >> 31;
+  return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+void normals(uint64_t& st, std::vector<double>& v) {
+  for (size_t i = 0; i < v.size(); i += 2) {
+    const double u1 = next_u01(st), u2 = next_u01(st);
+    const double r = std::sqrt(-2.0 * std::log(u1));
+    v[i] = r * std::cos(2.0 * M_PI * u2);
+    if (i + 1 < v.size()) v[i + 1] = r * std::sin(2.0 * M_PI * u2);
+  }
+}
+
+// subspace_expand!(psi, node, nextnode, max_expand_dim, noise) (update_site_ttn.jl:65-109)
+void expand_random(tn_ttn* t, int node, int slot, long max_expand_dim, double noise) {
+  TTNNode& A = t->nodes[node];
+  TN_REQUIRE(slot >= 0 && slot < 3 && A.slot[slot].nbr >= 0, TN_E_INVALID, "subspace_expand!: bad slot");
+  TTNNode& B = t->nodes[A.slot[slot].nbr];
+  const int sb = A.slot[slot].nbr_slot;
+  long sa_sum = 0, sb_sum = 0;
+  for (int k = 0; k < 3; ++k) {
+    if (k != slot) sa_sum += A.dims[k];
+    if (k != sb) sb_sum += B.dims[k];
+  }
+  const long p = std::min(std::min(sa_sum, sb_sum), max_expand_dim);
+  if (p <= 0) return;
+  tn_ctx* c = t->ctx;
+  auto grow = [&](TTNNode& X, int s) {
+    long pd[3] = {X.dims[0], X.dims[1], X.dims[2]};
+    pd[s] = p;
+    std::vector<double> P((size_t)(pd[0] * pd[1] * pd[2]));
+    normals(t->rng, P);
+    double pn = 0.0;
+    for (double v : P) pn += v * v;
+    const double xn = std::sqrt(node_norm2(t, X));
+    const double sc = pn > 0.0 ? noise * xn / std::sqrt(pn) : 0.0;
+    for (double& v : P) v *= sc;
+    const size_t pb = P.size() * sizeof(double);
+    double* Pd = (double*)c->workspace(WS_TTN_P, pb);
+    TN_HIP(hipMemcpyAsync(Pd, P.data(), pb, hipMemcpyHostToDevice, c->stream));
+    long nd[3] = {X.dims[0], X.dims[1], X.dims[2]};
+    nd[s] += p;
+    X.spare.ensure((size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
+    concat3_f64(c, X.A.as<double>(), X.dims, s, Pd, p, X.spare.as<double>());
+    TN_HIP(hipStreamSynchronize(c->stream));  // the host padding goes out of scope
+    std::swap(X.A, X.spare);
+    for (int i = 0; i < 3; ++i) X.dims[i] = nd[i];
+    X.slot[s].env_ok = false;
+  };
+  grow(B, sb);  // nextnode's padding first, as the reference
+  grow(A, slot);
+}
+
+long sat_add(long a, long b) { return a > std::numeric_limits<long>::max() - b ? std::numeric_limits<long>::max() : a + b; }
+
+long max_link_dim(tn_ttn* t) {
+  long md = 1;
+  for (const TTNNode& N : t->nodes)
+    for (int s = 0; s < 3; ++s)
+      if (N.slot[s].nbr >= 0) md = std::max(md, N.dims[s]);
+  return md;
+}
+
 }  // namespace
 
 extern "C" {
@@ -377,6 +512,7 @@ int tn_ttn_create(tn_ctx* ctx, int nnodes, tn_ttn** out) {
 }
 
 int tn_ttn_destroy(tn_ttn* t) {
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
   if (!t) return TN_OK;
   if (t->ctx) (void)hipStreamSynchronize(t->ctx->stream);
   delete t;
@@ -540,16 +676,103 @@ int tn_ttn_expand(tn_ttn* t, int node, int slot, long pad, const double* padA, c
       TN_HIP(hipMemcpyAsync(Pd, P, pb, hipMemcpyHostToDevice, c->stream));
       long nd[3] = {X.dims[0], X.dims[1], X.dims[2]};
       nd[s] += pad;
-      DevBuf Y;
-      Y.ensure((size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
-      concat3_f64(c, X.A.as<double>(), X.dims, s, Pd, pad, Y.as<double>());
+      X.spare.ensure((size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
+      concat3_f64(c, X.A.as<double>(), X.dims, s, Pd, pad, X.spare.as<double>());
       TN_HIP(hipStreamSynchronize(c->stream));
-      X.A = std::move(Y);
+      std::swap(X.A, X.spare);
       for (int i = 0; i < 3; ++i) X.dims[i] = nd[i];
       X.slot[s].env_ok = false;
     };
     grow(N, slot, padA);
     grow(M, ts, padB);
+  });
+}
+
+int tn_ttn_set_center(tn_ttn* t, int node) {
+  return guarded_t(CT(t), [&] {
+    TN_REQUIRE(t && node >= -1 && node < (int)t->nodes.size(), TN_E_INVALID, "tn_ttn_set_center: bad node");
+    t->center = node;
+  });
+}
+
+int tn_ttn_get_center(tn_ttn* t, int* node) {
+  return guarded_t(CT(t), [&] {
+    TN_REQUIRE(t && node, TN_E_INVALID, "tn_ttn_get_center: null argument");
+    *node = t->center;
+  });
+}
+
+int tn_ttn_seed(tn_ttn* t, unsigned long long seed) {
+  return guarded_t(CT(t), [&] {
+    TN_REQUIRE(t, TN_E_INVALID, "tn_ttn_seed: null handle");
+    t->rng = (uint64_t)seed;
+  });
+}
+
+int tn_ttn_position(tn_ttn* t, int node, long maxdim, long mindim, double cutoff, int normalize, int skip) {
+  return guarded_t(CT(t), [&] {
+    node_of(t, node);
+    position(t, node, maxdim, mindim, cutoff, normalize != 0, skip);
+  });
+}
+
+int tn_ttn_update_position(tn_ttn* t, int node, long maxdim, long mindim, double cutoff, int normalize,
+                           int krylovdim, int maxiter, double tol, int check_convergence, double* energy) {
+  return guarded_t(CT(t), [&] {
+    node_of(t, node);
+    position(t, node, maxdim, mindim, cutoff, normalize != 0, -1);
+    const double e = eig_at(t, node, krylovdim, maxiter, tol, check_convergence != 0);
+    if (energy) *energy = e;
+  });
+}
+
+int tn_ttn_expand_random(tn_ttn* t, int node, int slot, long max_expand_dim, double noise) {
+  return guarded_t(CT(t), [&] {
+    node_of(t, node);
+    expand_random(t, node, slot, max_expand_dim, noise);
+  });
+}
+
+int tn_ttn_fullsweep(tn_ttn* t, const int* path, int npath, int central, long maxdim, long mindim, double cutoff,
+                     double noise, int expand_dim, int max_expand_dim, int expand_numiter, int krylovdim, int maxiter,
+                     double tol, int normalize, int check_convergence, double* energy, long* maxchi) {
+  return guarded_t(CT(t), [&] {
+    TN_REQUIRE(t && path && npath >= 1, TN_E_INVALID, "fullsweep!: empty sweep path");
+    for (int k = 0; k < npath; ++k) node_of(t, path[k]);
+    TN_REQUIRE(!(expand_dim != 0 && expand_numiter < 2), TN_E_INVALID,
+               "`fullsweep!()`: `expand_numiter=" + std::to_string(expand_numiter) +
+                   "` cannot be less than 2 for `expand_dim=" + std::to_string(expand_dim) + "` !!");
+    std::vector<int> order(path, path + npath);
+    for (int k = npath - 1; k >= 0; --k) order.push_back(path[k]);
+    const bool norm = normalize != 0, chk = check_convergence != 0;
+    double e = NAN;
+    if (std::fabs(noise) < 100 * 1e-15) {
+      for (int node : order) {
+        position(t, node, maxdim, mindim, -1.0, norm, -1);
+        e = eig_at(t, node, krylovdim, maxiter, tol, chk);
+      }
+    } else {
+      node_of(t, central);
+      for (int node : order) {
+        if (node == central) {
+          position(t, node, maxdim, mindim, -1.0, norm, -1);
+          e = eig_at(t, node, krylovdim, maxiter, tol, chk);
+          continue;
+        }
+        const int nxt = tree_path(t, node, central)[1];
+        position(t, node, maxdim, mindim, -1.0, norm, nxt);
+        expand_random(t, node, slot_to(t->nodes[node], nxt), max_expand_dim, noise);
+        t->center = nxt;
+        for (int dummy = 1; dummy <= expand_numiter; ++dummy) {
+          const long md = dummy == expand_numiter ? maxdim : sat_add(maxdim, dummy == 1 ? max_expand_dim : expand_dim);
+          const int nn = dummy % 2 == 1 ? node : nxt;
+          position(t, nn, md, mindim, cutoff, norm, -1);
+          e = eig_at(t, nn, krylovdim, maxiter, tol, chk);
+        }
+      }
+    }
+    if (energy) *energy = e;
+    if (maxchi) *maxchi = max_link_dim(t);
   });
 }
 

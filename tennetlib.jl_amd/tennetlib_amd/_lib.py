@@ -108,6 +108,13 @@ SIGNATURES = {
     "tn_ttn_normalize": [_P, _I, _PD],
     "tn_ttn_node_norm": [_P, _I, _PD],
     "tn_ttn_expand": [_P, _I, _I, _L, _PD, _PD],
+    "tn_ttn_set_center": [_P, _I],
+    "tn_ttn_get_center": [_P, _PI],
+    "tn_ttn_seed": [_P, C.c_ulonglong],
+    "tn_ttn_position": [_P, _I, _L, _L, _D, _I, _I],
+    "tn_ttn_update_position": [_P, _I, _L, _L, _D, _I, _I, _I, _D, _I, _PD],
+    "tn_ttn_expand_random": [_P, _I, _I, _L, _D],
+    "tn_ttn_fullsweep": [_P, _PI, _I, _I, _L, _L, _D, _D, _I, _I, _I, _I, _I, _D, _I, _I, _PD, _PL],
     "tn_env_phi_dims": [_P, _PL, _PL],
     "tn_env_matvec": [_P, _P, _P, _I],
     "tn_env_contract_phi": [_P, _I, _P],

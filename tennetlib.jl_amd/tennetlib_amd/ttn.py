@@ -471,11 +471,12 @@ class StateEnvsTTN:
             c2 = (C.c_int * len(w))(*[x[2] for x in w])
             ww = (C.c_double * len(w))(*[x[3] for x in w])
             self.ctx.check(lib.tn_ttn_set_coupling(self.h, t.index[n], len(w), c0, c1, c2, ww))
-        self.orthocenter = psi.orthocenter
-        dist, _ = _bfs(t.graph, self.orthocenter)
+        dist, _ = _bfs(t.graph, psi.orthocenter)
         for n in sorted(t.graph, key=lambda x: (-dist[x], x)):
-            if n != self.orthocenter:
-                self._env(n, nextnode_in_path(t.graph, n, self.orthocenter))
+            if n != psi.orthocenter:
+                self._env(n, nextnode_in_path(t.graph, n, psi.orthocenter))
+        self.ctx.check(lib.tn_ttn_set_center(self.h, t.index[psi.orthocenter]))
+        self.ctx.check(lib.tn_ttn_seed(self.h, 0))
 
     def close(self):
         if self.h and self.ctx.h:
@@ -490,6 +491,16 @@ class StateEnvsTTN:
 
     def _env(self, a: Node, b: Node):
         self.ctx.check(self.ctx.lib.tn_ttn_update_env(self.h, self.tree.index[a], self.tree.slot_of(a, b)))
+
+    @property
+    def orthocenter(self) -> Node:
+        k = C.c_int()
+        self.ctx.check(self.ctx.lib.tn_ttn_get_center(self.h, C.byref(k)))
+        return self.tree.nodes[k.value] if k.value >= 0 else None
+
+    @orthocenter.setter
+    def orthocenter(self, node: Node):
+        self.ctx.check(self.ctx.lib.tn_ttn_set_center(self.h, self.tree.index[node]))
 
     @property
     def psi(self) -> TTN:
@@ -513,17 +524,9 @@ def position_(sysenv: StateEnvsTTN, node: Node, *, maxdim: int = TYPEMAX_INT, mi
     t = sysenv.tree
     if node not in t.graph:
         raise ValueError("`isometrize!()`: Input position is ill-defined !!")
-    oc = sysenv.orthocenter
-    path = shortest_path(t.graph, oc, node)
-    for a, b in zip(path[:-1], path[1:]):
-        _move(sysenv.ctx, sysenv.h, t, a, b, maxdim, mindim, cutoff)
-    sysenv.orthocenter = node
-    if normalize:
-        sysenv.ctx.check(sysenv.ctx.lib.tn_ttn_normalize(sysenv.h, t.index[node], None))
-    for a, b in zip(path[:-1], path[1:]):
-        if a == node_to_skip:
-            continue
-        sysenv._env(a, b)
+    skip = t.index[node_to_skip] if node_to_skip is not None else -1
+    sysenv.ctx.check(sysenv.ctx.lib.tn_ttn_position(sysenv.h, t.index[node], int(min(maxdim, TYPEMAX_INT)),
+                                                    int(mindim), float(cutoff), int(normalize), skip))
 
 
 def product(sysenv: StateEnvsTTN, v: np.ndarray) -> np.ndarray:
@@ -547,43 +550,27 @@ def update_position_(sysenv: StateEnvsTTN, solver, node: Node, *, time_step=None
     """update_position!(sysenv, solver, node; ...) (update_site_ttn.jl:42-63)."""
     if solver is not eig_solver:
         raise NotImplementedError("TTN updates support `eig_solver` only (as the reference)")
-    position_(sysenv, node, maxdim=maxdim, mindim=mindim, cutoff=cutoff, svd_alg=svd_alg, normalize=normalize)
+    if node not in sysenv.tree.graph:
+        raise ValueError("`isometrize!()`: Input position is ill-defined !!")
     e = C.c_double()
-    info = L.KrylovInfo()
-    sysenv.ctx.check(sysenv.ctx.lib.tn_ttn_eig(
-        sysenv.h, sysenv.tree.index[node], int(kwargs.get("solver_krylovdim", 5)),
-        int(kwargs.get("solver_maxiter", 2)), float(kwargs.get("solver_tol", 1e-14)), C.byref(e), C.byref(info)))
-    if kwargs.get("solver_check_convergence", False) and info.converged < 1:
-        raise RuntimeError("`eig_solver()` not converged !!")
+    sysenv.ctx.check(sysenv.ctx.lib.tn_ttn_update_position(
+        sysenv.h, sysenv.tree.index[node], int(min(maxdim, TYPEMAX_INT)), int(mindim), float(cutoff),
+        int(normalize), int(kwargs.get("solver_krylovdim", 5)), int(kwargs.get("solver_maxiter", 2)),
+        float(kwargs.get("solver_tol", 1e-14)), int(bool(kwargs.get("solver_check_convergence", False))),
+        C.byref(e)))
     return e.value
 
 
-def subspace_expand_(sysenv: StateEnvsTTN, node: Node, nextnode: Node, max_expand_dim: int, noise: float,
-                     rng: np.random.Generator) -> None:
+def subspace_expand_(sysenv: StateEnvsTTN, node: Node, nextnode: Node, max_expand_dim: int,
+                     noise: float) -> None:
     """subspace_expand!(psi, node, nextnode, max_expand_dim, noise)
     (update_site_ttn.jl:65-109), dense: both tensors grow along their common
     bond by p = min(sum of each tensor's other leg dims, max_expand_dim),
-    padded with Gaussian entries scaled to noise * |tensor| (nextnode's
-    padding drawn first)."""
-    t, ctx = sysenv.tree, sysenv.ctx
-    sa, sb = t.slot_of(node, nextnode), t.slot_of(nextnode, node)
-    da, db = sysenv.node_shape(node), sysenv.node_shape(nextnode)
-    p = min(sum(d for k, d in enumerate(da) if k != sa), sum(d for k, d in enumerate(db) if k != sb), max_expand_dim)
-    if p <= 0:
-        return
-
-    def pad(dims, s, n):
-        shape = list(dims)
-        shape[s] = p
-        P = rng.standard_normal(shape)
-        nr = C.c_double()
-        ctx.check(ctx.lib.tn_ttn_node_norm(sysenv.h, t.index[n], C.byref(nr)))
-        return np.ascontiguousarray(P * (noise * nr.value / np.linalg.norm(P)))
-
-    PB = pad(db, sb, nextnode)
-    PA = pad(da, sa, node)
-    ctx.check(ctx.lib.tn_ttn_expand(sysenv.h, t.index[node], sa, p, PA.ctypes.data_as(L._PD),
-                                    PB.ctypes.data_as(L._PD)))
+    padded with Gaussian entries (the StateEnvsTTN's seeded generator)
+    scaled to noise * |tensor|, nextnode's padding first."""
+    t = sysenv.tree
+    sysenv.ctx.check(sysenv.ctx.lib.tn_ttn_expand_random(sysenv.h, t.index[node], t.slot_of(node, nextnode),
+                                                         int(max_expand_dim), float(noise)))
 
 
 @dataclass
@@ -620,13 +607,24 @@ def fullsweep_(sysenv: StateEnvsTTN, sweeppath: Sequence[Node], solver, swdata: 
     if expand_dim != 0 and expand_numiter < 2:
         raise ValueError(f"`fullsweep!()`: `expand_numiter={expand_numiter}` cannot be less than 2 "
                          f"for `expand_dim={expand_dim}` !!")
-    rng = kwargs.get("rng") or np.random.default_rng(0)
     skw = {k: kwargs[k] for k in ("solver_krylovdim", "solver_maxiter", "solver_tol", "solver_check_convergence")
            if k in kwargs}
     energy = float("nan")
     swdata.sweepcount += 1
     path = list(sweeppath)
-    if abs(noise) < 100 * FLOAT64_THRESHOLD:
+    t = sysenv.tree
+    if outputlevel <= 1:  # the whole sweep in the library
+        central = find_eccentric_central_node(t.graph)
+        ids = (C.c_int * len(path))(*[t.index[n] for n in path])
+        e, mc = C.c_double(), C.c_long()
+        sysenv.ctx.check(sysenv.ctx.lib.tn_ttn_fullsweep(
+            sysenv.h, ids, len(path), t.index[central], int(min(maxdim, TYPEMAX_INT)), mindim, cutoff, noise,
+            expand_dim, max_expand_dim, expand_numiter, int(skw.get("solver_krylovdim", 5)),
+            int(skw.get("solver_maxiter", 2)), float(skw.get("solver_tol", 1e-14)), int(normalize),
+            int(bool(skw.get("solver_check_convergence", False))), C.byref(e), C.byref(mc)))
+        energy = e.value
+        swdata.maxchi.append(int(mc.value))
+    elif abs(noise) < 100 * FLOAT64_THRESHOLD:
         for node in path + path[::-1]:
             energy = update_position_(sysenv, solver, node, normalize=normalize, maxdim=maxdim, mindim=mindim,
                                       cutoff=-1.0, **skw)
@@ -643,7 +641,7 @@ def fullsweep_(sysenv: StateEnvsTTN, sweeppath: Sequence[Node], solver, swdata: 
             nxt = nextnode_in_path(sysenv.tree.graph, node, central)
             position_(sysenv, node, normalize=normalize, maxdim=maxdim, mindim=mindim, cutoff=-1.0,
                       node_to_skip=nxt)
-            subspace_expand_(sysenv, node, nxt, max_expand_dim, noise, rng)
+            subspace_expand_(sysenv, node, nxt, max_expand_dim, noise)
             sysenv.orthocenter = nxt
             for dummy in range(1, expand_numiter + 1):
                 md = maxdim if dummy == expand_numiter else (maxdim + max_expand_dim if dummy == 1
@@ -651,7 +649,8 @@ def fullsweep_(sysenv: StateEnvsTTN, sweeppath: Sequence[Node], solver, swdata: 
                 newnode = node if dummy % 2 == 1 else nxt
                 energy = update_position_(sysenv, solver, newnode, normalize=normalize, maxdim=md, mindim=mindim,
                                           cutoff=cutoff, **skw)
-    swdata.maxchi.append(_maxlinkdim(sysenv))
+    if outputlevel > 1:
+        swdata.maxchi.append(_maxlinkdim(sysenv))
     swdata.energy.append(energy)
     enerr = swdata.energy[-1] - swdata.energy[-2] if swdata.sweepcount > 1 else float("nan")
     if outputlevel > 0:
@@ -691,8 +690,9 @@ def make_optimize_params(*, maxdim, nsweeps, cutoff=FLOAT64_THRESHOLD, noise=0.0
 def optimize_(sysenv: StateEnvsTTN, params: OptimizeParamsTTN, sweeppath: Sequence[Node], **kwargs) -> SweepDataTTN:
     """optimize!(sysenv, params, sweeppath; kwargs...) (optimize_ttn.jl:148-223)."""
     enerrgoal = kwargs.pop("energyErrGoal", None)
-    seed = kwargs.pop("seed", 0)
-    kwargs.setdefault("rng", np.random.default_rng(seed))
+    seed = kwargs.pop("seed", None)
+    if seed is not None:
+        sysenv.ctx.check(sysenv.ctx.lib.tn_ttn_seed(sysenv.h, int(seed)))
     sw = SweepDataTTN()
     for ii in range(len(params.nsweeps)):
         noise = params.noise[ii]
