@@ -242,6 +242,7 @@ def main():
     ap.add_argument("--mv-wide", type=int, default=1, help="also time the w = 20 (J1-J2 cylinder) matvec")
     ap.add_argument("--tdvp-steps", type=int, default=40, help="TDVP steps of the XXZ N=100 run (0: skip)")
     ap.add_argument("--ttn-instances", type=int, default=16, help="TTN instances timed of the 256 (0: skip)")
+    ap.add_argument("--ttn-only", action="store_true", help="only the many-instance TTN line")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -267,6 +268,9 @@ def main():
     T.default_context()
 
     # 1. matvec microbenchmark at chi = mv_chi
+    if args.ttn_only:
+        print(json.dumps({"ttn_random_field_batch": ttn_instances(T, max(1, args.ttn_instances))}), flush=True)
+        return
     mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", "bf16x3"])
     mv_wide = None
     if args.mv_wide:

@@ -156,6 +156,38 @@ def test_ttn_random_field_instance_reaches_ed(T):
     assert abs(en - e0) < 1e-9
 
 
+def test_ttn_concurrent_contexts_match_sequential(T):
+    """Regression: two host threads optimising different instances on their
+    own contexts (the library serialises its entry points) reproduce the
+    sequential results bit for bit."""
+    import threading
+    specs = [(16, 3), (20, 4)]
+    params = T.make_optimize_params(maxdim=[64], nsweeps=[3], cutoff=1e-14, noise=1e-2, noisedecay=5,
+                                    disable_noise_after=2)
+
+    def run(N, seed, ctx, out):
+        h = np.random.default_rng(seed).uniform(-2, 2, N)
+        H, _ = _model(T, N, h)
+        psi0 = T.default_randomTTN(N, 12, seed=seed, ctx=ctx)
+        env = T.StateEnvsTTN(psi0, H, ctx=ctx)
+        sw = T.optimize_(env, params, T.default_sweeppath(psi0), outputlevel=0, seed=seed)
+        env.close()
+        out[(N, seed)] = (list(sw.energy), list(sw.maxchi))
+
+    seq, par = {}, {}
+    ctx0 = T.Context(0)
+    for N, seed in specs:
+        run(N, seed, ctx0, seq)
+    ctxs = [T.Context(0) for _ in specs]
+    ths = [threading.Thread(target=run, args=(N, seed, c, par)) for (N, seed), c in zip(specs, ctxs)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    print("ttn concurrent", seq, par, flush=True)
+    assert par == seq
+
+
 def test_ttn_errors(T):
     N = 8
     H, _ = _model(T, N)
