@@ -151,6 +151,16 @@ int tn_env_reset(tn_env* env);
  * projector j (0-based), shape (chil, d_k, chir) of the env's dtype. */
 int tn_env_set_projectors(tn_env* env, int nproj, double weight);
 int tn_env_set_projector_site(tn_env* env, int j, int k, const void* A_host, long chil, long chir);
+/* Noise term of a projected sum of term strings: StateEnvs(psi, H::CouplingModel)
+ * (state_envs.jl:136-175) builds ProjCouplingModel, whose noiseterm
+ * (projcouplingmodel.jl:391-492) sums X_id X_id^H over the term ids already
+ * acting on the kept side of the bond.  When the MPO set with tn_env_set_mpo
+ * is that sum with the ids' channels laid out per bond, skip_left[k] leading
+ * channels of bond k (identity on sites <= k) are left out of the left noise
+ * factor at bond k and skip_right[k] trailing channels (identity on sites > k)
+ * out of the right one; arrays of length N+1, entries 1..N-1 read, each in
+ * [0, w[k]].  Default 0 (ITensorMPS noiseterm of a ProjMPO: all channels). */
+int tn_env_set_noise_channels(tn_env* env, const int* skip_left, const int* skip_right);
 /* shape of the projected-operator domain at the current position:
  * dims[0..3] = (chil, d_p, d_{p+1} or 1, chir); *n = element count */
 int tn_env_phi_dims(tn_env* env, long* dims4, long* n);

@@ -338,6 +338,9 @@ class Envs:
         self.weight = weight
         self.m: List[np.ndarray] = []
 
+    def fresh(self):
+        return Envs(self.W, dtype=self.dtype, Ms=self.Ms, weight=self.weight)
+
     def lproj(self):
         return edge_env(self.dtype) if self.lpos <= 0 else self.LR[self.lpos]
 
@@ -676,6 +679,8 @@ def noiseterm_factor(env: "Envs", phi: np.ndarray, ortho: str) -> np.ndarray:
 
 
 def noiseterm(env: "Envs", phi: np.ndarray, ortho: str) -> np.ndarray:
+    if hasattr(env, "noiseterm"):  # ProjCouplingModel / ProjMPOSum2 (tn_oracle_cm)
+        return env.noiseterm(phi, ortho)
     X = noiseterm_factor(env, phi, ortho)
     return X @ X.conj().T if ortho == "left" else X.T @ X.conj()
 
@@ -791,7 +796,13 @@ class StateEnvs:
         if Ms is not None and (weight is None or weight <= 0.0):
             raise ValueError(f"`weight` parameter should be > 0.0 (value passed was `weight={weight}`)")
         self.psi = [a.copy() for a in psi]
-        self.PH = Envs(Ws, dtype=psi[0].dtype, Ms=Ms, weight=weight or 0.0)
+        from oracle import tn_oracle_cm as CM
+        if isinstance(Ws, CM.CMModel):      # ProjCouplingModel(_MPS), state_envs.jl:136-175
+            self.PH = CM.ProjCM(Ws, Ms=Ms, weight=weight or 0.0)
+        elif len(Ws) and not isinstance(Ws[0], np.ndarray):  # ProjMPOSum2, state_envs.jl:108-134
+            self.PH = CM.ProjSum(Ws, dtype=psi[0].dtype, Ms=Ms, weight=weight or 0.0)
+        else:
+            self.PH = Envs(Ws, dtype=psi[0].dtype, Ms=Ms, weight=weight or 0.0)
         self.ortho_center: Optional[int] = None  # 1-based when known
 
     def __len__(self):
@@ -802,7 +813,7 @@ def _ensure_ortho_first(env: StateEnvs):
     if env.ortho_center != 1:
         env.psi = right_orthogonalize(env.psi)
         env.ortho_center = 1
-        env.PH = Envs(env.PH.W, dtype=env.psi[0].dtype, Ms=env.PH.Ms, weight=env.PH.weight)
+        env.PH = env.PH.fresh()
 
 
 def update_two_site(env: StateEnvs, pos: int, ortho: str, *, solver: str = "eig",

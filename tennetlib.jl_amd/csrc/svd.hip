@@ -500,7 +500,7 @@ void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bo
 }
 
 void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out, const void* X, long kx,
-              double xs) {
+              double xs, long ldx) {
   TN_REQUIRE(m > 0 && n > 0, TN_E_INVALID, "svd: empty matrix");
   const long p = left ? m : n;
   const long ppad = ((p + PR - 1) / PR) * PR;
@@ -520,7 +520,8 @@ void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, Gr
     GemmDesc h;
     h.M = p; h.N = p; h.K = kx; h.C = G; h.ldc = ppad;
     h.alpha = xs; h.beta = 1.0;
-    if (left) { h.A = X; h.lda = kx; h.B = X; h.ldb = kx; h.tB = 1; h.cB = cplx; }
+    const long ld = ldx > 0 ? ldx : kx;
+    if (left) { h.A = X; h.lda = ld; h.B = X; h.ldb = ld; h.tB = 1; h.cB = cplx; }
     else      { h.A = X; h.lda = p; h.tA = 1; h.cA = cplx; h.B = X; h.ldb = p; }
     gemm(c, cplx ? Prec::Z64 : Prec::F64, h);
   }

@@ -250,10 +250,11 @@ struct GramEig {
   std::vector<double> w;    // eigenvalues (unsorted, length p) = weights s^2
   int sweeps = 0;
 };
-// G = M M^H (left) or M^H M (right), plus xs * X X^H (left, X: p x kx) or
-// xs * X^H X (right, X: kx x p) when X is given, then its eigen-decomposition
+// G = M M^H (left) or M^H M (right), plus xs * X X^H (left, X: p x kx with
+// row stride ldx, default kx) or xs * X^H X (right, X: kx x p) when X is
+// given, then its eigen-decomposition
 void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out,
-              const void* X = nullptr, long kx = 0, double xs = 0.0);
+              const void* X = nullptr, long kx = 0, double xs = 0.0, long ldx = 0);
 // eigen-decomposition of a symmetric / Hermitian p x p device matrix G (ld
 // ldg, padding rows/columns zero); Vt (ld = padded size) receives V^T / V^H
 void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bool cplx, std::vector<double>& w,

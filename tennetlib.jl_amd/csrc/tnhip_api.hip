@@ -145,6 +145,10 @@ struct tn_env {
   // penalty projectors (excited states): H v + weight sum_k m_k (m_k^H v)
   std::vector<ProjT> proj;
   double proj_weight = 0.0;
+  // channels of bond k left out of the noise factor: the first noise_skip_l[k]
+  // (identity on sites <= k) for the left factor, the last noise_skip_r[k]
+  // (identity on sites > k) for the right factor
+  std::vector<int> noise_skip_l, noise_skip_r;
 
   bool cplx() const { return dtype == TN_C128; }
 };
@@ -1049,7 +1053,7 @@ void sorted_weights(const std::vector<double>& w, long r, std::vector<int>& orde
 //   right: X[(w a s), (t' b')] = sum phi[a, s, t, b] W_{b+1}[w, t', t, w2] R[w2][b][b']
 // and the perturbation is X X^H (left) or X^H X (right) in the convention of
 // gram_eig.  Needs the projected operator positioned at b with nsite = 2.
-const void* noise_factor(tn_env* e, int b, const double* phi, bool left, long* kx) {
+const void* noise_factor(tn_env* e, int b, const double* phi, bool left, long* kx, long* ldx) {
   tn_ctx* c = e->ctx;
   TN_REQUIRE(e->nsite == 2 && e->lpos == b - 1 && e->rpos == b + 2, TN_E_POSITION,
              "noise term only defined for 2-site ProjMPO positioned at the bond");
@@ -1078,8 +1082,11 @@ const void* noise_factor(tn_env* e, int b, const double* phi, bool left, long* k
     ws.out = X2; ws.out_sg = d2 * chir * m2; ws.out_sA = d1 * wm * d2 * chir * m2; ws.out_sS = wm * d2 * chir * m2;
     ws.op = &op_left(e, b);
     wstep(c, false, ws);
-    *kx = wm * d2 * chir;
-    return X2;
+    // columns (w t b) with w slowest: the kept channels are a column range
+    const long sk = e->noise_skip_l.empty() ? 0 : e->noise_skip_l[b];
+    *ldx = wm * d2 * chir;
+    *kx = (wm - sk) * d2 * chir;
+    return (const char*)X2 + (size_t)(sk * d2 * chir) * es;
   }
   const EnvT& R = e->LR[b + 2];
   TN_REQUIRE(R.chi == chir && R.w == w2, TN_E_SHAPE, "noise term: right environment does not match phi");
@@ -1099,7 +1106,10 @@ const void* noise_factor(tn_env* e, int b, const double* phi, bool left, long* k
   ws.out = X2; ws.out_sg = chil * d1 * d2 * chir * m2; ws.out_sA = d2 * chir * m2; ws.out_sS = chir * m2;
   ws.op = &op_right(e, b + 1);
   wstep(c, false, ws);
-  *kx = wm * chil * d1;
+  // rows (w a s) with w slowest: the kept channels are the leading rows
+  const long sk = e->noise_skip_r.empty() ? 0 : e->noise_skip_r[b];
+  *kx = (wm - sk) * chil * d1;
+  *ldx = 0;
   return X2;
 }
 
@@ -1123,9 +1133,9 @@ void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxd
   const long m = chil * d1, n = (long)d2 * chir;
   GramEig ge;
   const bool perturbed = std::fabs(noise) > 1e-15;
-  long kx = 0;
-  const void* X = perturbed ? noise_factor(e, b, phi, ortho_left, &kx) : nullptr;
-  gram_eig(c, phi, m, n, ortho_left, cx, ge, X, kx, perturbed ? noise : 0.0);
+  long kx = 0, ldx = 0;
+  const void* X = perturbed ? noise_factor(e, b, phi, ortho_left, &kx, &ldx) : nullptr;
+  gram_eig(c, phi, m, n, ortho_left, cx, ge, X, kx, perturbed ? noise : 0.0, ldx);
   std::vector<int> order;
   std::vector<double> P;
   // rank(phi phi^H) <= min(m, n); the perturbed density matrix has full rank p
@@ -1657,6 +1667,22 @@ int tn_env_set_projectors(tn_env* e, int nproj, double weight) {
       P.rpos = e->N + 1;
     }
     e->proj_weight = weight;
+  });
+}
+
+int tn_env_set_noise_channels(tn_env* e, const int* skip_left, const int* skip_right) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(skip_left && skip_right, TN_E_INVALID, "set_noise_channels: null count array");
+    for (int k = 1; k < e->N; ++k)
+      TN_REQUIRE(skip_left[k] >= 0 && skip_right[k] >= 0 && skip_left[k] <= e->w[k] && skip_right[k] <= e->w[k],
+                 TN_E_INVALID, "set_noise_channels: count out of range at bond " + std::to_string(k));
+    e->noise_skip_l.assign(e->N + 1, 0);
+    e->noise_skip_r.assign(e->N + 1, 0);
+    for (int k = 1; k < e->N; ++k) {
+      e->noise_skip_l[k] = skip_left[k];
+      e->noise_skip_r[k] = skip_right[k];
+    }
   });
 }
 

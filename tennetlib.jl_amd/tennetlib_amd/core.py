@@ -30,6 +30,7 @@ from typing import List, Optional, Sequence, Union
 import numpy as np
 
 from . import _lib as L
+from .couplingmodel import CouplingModel, coupling_mpo, mpo_sum_ids
 
 FLOAT64_THRESHOLD = 1e-15  # src/base/global_variables.jl:10
 TYPEMAX_INT = 2 ** 63 - 1
@@ -152,9 +153,12 @@ class StateEnvs:
     (src/mps/state_envs.jl:18-60).  ``StateEnvs(psi, H)`` copies psi.
     ``StateEnvs(psi, H, Ms, weight=w)`` adds the penalty projectors of
     previously found states (ProjMPO_MPS2, state_envs.jl:83-106), for
-    excited states."""
+    excited states.  ``H`` is an MPO (list of W[wl, s', s, wr]), a list of
+    MPOs (ProjMPOSum2, state_envs.jl:108-134) or a CouplingModel
+    (ProjCouplingModel, state_envs.jl:136-175); the last two run as one MPO
+    over their term ids with the reference's product and noise term."""
 
-    def __init__(self, psi: Sequence[np.ndarray], H: Sequence[np.ndarray], Ms=None, weight: Optional[float] = None,
+    def __init__(self, psi: Sequence[np.ndarray], H, Ms=None, weight: Optional[float] = None,
                  ctx: Optional[Context] = None, dtype=None):
         if Ms is not None:
             if weight is None:
@@ -163,9 +167,26 @@ class StateEnvs:
                 raise ValueError(f"`weight` parameter should be > 0.0 (value passed was `weight={weight}`)")
         self.ctx = ctx or default_context()
         N = len(psi)
+        d = [int(a.shape[1]) for a in psi]
+        self.model = H
+        noise_skip = None
+        if isinstance(H, CouplingModel) or (len(H) and not isinstance(H[0], np.ndarray)):
+            # ProjCouplingModel (state_envs.jl:136-175) or ProjMPOSum2 for a
+            # list of MPOs (state_envs.jl:108-134): one MPO over the term ids
+            if isinstance(H, CouplingModel):
+                if len(H) != N or (H.mpos and H.d != d) or (not H.mpos and any(x != 2 for x in d)):
+                    raise ValueError("StateEnvs: the CouplingModel and the MPS have different sites")
+                ids = H.ids()
+            else:
+                for Hk in H:
+                    if len(Hk) != N:
+                        raise ValueError("StateEnvs: MPS and MPO lengths differ")
+                ids = mpo_sum_ids(H)
+            H, skip_l, skip_r = coupling_mpo(ids, N, d)
+            if any(skip_l) or any(skip_r):
+                noise_skip = (skip_l, skip_r)
         if len(H) != N:
             raise ValueError("StateEnvs: MPS and MPO lengths differ")
-        d = [int(a.shape[1]) for a in psi]
         for k, (a, W) in enumerate(zip(psi, H)):
             if W.shape[1] != d[k] or W.shape[2] != d[k]:
                 raise ValueError(f"StateEnvs: site indices of MPO and MPS differ at site {k + 1}")
@@ -191,6 +212,10 @@ class StateEnvs:
         for k, A in enumerate(psi, start=1):
             self.set_site(k, A)
         self.H = [np.array(W) for W in H]
+        if noise_skip is not None:
+            sl = (C.c_int * (N + 1))(*noise_skip[0])
+            sr = (C.c_int * (N + 1))(*noise_skip[1])
+            self.ctx.check(self.ctx.lib.tn_env_set_noise_channels(self.h, sl, sr))
         self.Ms = None
         if Ms is not None:
             self.Ms = [[np.array(a) for a in M] for M in Ms]
@@ -530,8 +555,9 @@ def dmrg2(psi0, H, *args, **kwargs):
 class TDVPEngine:
     """src/mps/tdvp.jl:17-49."""
 
-    def __init__(self, psi, H, ctx: Optional[Context] = None):
-        self.sysenv = StateEnvs(psi, H, ctx=ctx, dtype=np.complex128)
+    def __init__(self, psi, H, Ms=None, weight: Optional[float] = None, ctx: Optional[Context] = None):
+        """TDVPEngine(psi, H) or TDVPEngine(psi, H, Ms; weight) (tdvp.jl:41-63)."""
+        self.sysenv = StateEnvs(psi, H, Ms, weight=weight, ctx=ctx, dtype=np.complex128)
         self.swdata = SweepData()
         self.abstime = 0.0
 

@@ -28,6 +28,7 @@ import numpy as np
 
 from . import _lib as L
 from .core import FLOAT64_THRESHOLD, Context, _live_children, default_context, eig_solver
+from .couplingmodel import SPIN_HALF_OPS, CouplingModel, OpStrings, op_matrix  # noqa: F401
 
 Node = Tuple[int, int]
 TYPEMAX_INT = 2 ** 63 - 1
@@ -315,76 +316,20 @@ def default_sweeppath(psi: TTN) -> List[Node]:
     return path
 
 
-# ---------------------------------------------------------------------------
-# OpStrings / CouplingModel (spin-1/2, 1- and 2-body terms)
-# ---------------------------------------------------------------------------
-
-SPIN_HALF_OPS = {
-    "Id": np.eye(2), "Sz": np.diag([0.5, -0.5]),
-    "S+": np.array([[0.0, 1.0], [0.0, 0.0]]), "S-": np.array([[0.0, 0.0], [1.0, 0.0]]),
-    "Sx": np.array([[0.0, 0.5], [0.5, 0.0]]),
-}
-
-
-class OpStrings:
-    """OpStrings (src/base/opstrings.jl): ``os += (coef, "Sz", j, "Sz", j + 1)``
-    or ``os += (coef, ("Sz", j), ("Sz", j + 1))``."""
-
-    def __init__(self):
-        self.terms: List[Tuple[float, List[Tuple[str, int]]]] = []
-
-    def __iadd__(self, t):
-        coef, rest = float(t[0]), list(t[1:])
-        if rest and isinstance(rest[0], tuple):
-            ops = [(str(o), int(i)) for o, i in rest]
-        else:
-            if len(rest) % 2:
-                raise ValueError("OpStrings: operator names and sites must come in pairs")
-            ops = [(str(rest[k]), int(rest[k + 1])) for k in range(0, len(rest), 2)]
-        self.terms.append((coef, ops))
-        return self
-
-
-class CouplingModel:
-    """CouplingModel(os, sites) (src/base/couplingmodel.jl:153-241) for
-    spin-1/2 sites: 1-body terms per site and 2-body couplings."""
-
-    def __init__(self, os: OpStrings, sites):
-        self.N = sites if isinstance(sites, int) else len(sites)
-        self.onebody = {i: np.zeros((2, 2)) for i in range(1, self.N + 1)}
-        self.twobody: Dict[Tuple[int, str, int, str], float] = {}
-        for coef, ops in os.terms:
-            for o, i in ops:
-                if o not in SPIN_HALF_OPS or not 1 <= i <= self.N:
-                    raise ValueError(f"CouplingModel: unknown operator {o} or site {i}")
-            if len(ops) == 1:
-                (o, i), = ops
-                self.onebody[i] = self.onebody[i] + coef * SPIN_HALF_OPS[o]
-            elif len(ops) == 2:
-                (o1, i), (o2, j) = ops
-                if i == j:
-                    self.onebody[i] = self.onebody[i] + coef * (SPIN_HALF_OPS[o1] @ SPIN_HALF_OPS[o2])
-                    continue
-                if i > j:
-                    (o1, i), (o2, j) = (o2, j), (o1, i)
-                self.twobody[(i, o1, j, o2)] = self.twobody.get((i, o1, j, o2), 0.0) + coef
-            else:
-                raise NotImplementedError("CouplingModel on the GPU path: 1- and 2-body terms only")
-        self.partners: Dict[Tuple[int, str], List[int]] = {}
-        for (i, o1, j, o2) in self.twobody:
-            self.partners.setdefault((i, o1), []).append(j)
-            self.partners.setdefault((j, o2), []).append(i)
-
-
 def _tree_operator(tree: _Tree, H: CouplingModel):
     """Channels of every link and the sparse coupling of every node."""
+    onebody, twobody = H._few_body()
+    partners: Dict[Tuple[int, str], List[int]] = {}
+    for (i, o1, j, o2) in twobody:
+        partners.setdefault((i, o1), []).append(j)
+        partners.setdefault((j, o2), []).append(i)
     chan: Dict[tuple, List[tuple]] = {}
     for it, sites in tree.below.items():
         S = set(sites)
         ch = [("I",), ("H",)]
         for i in sites:
-            for o in sorted(o for (j, o) in H.partners if j == i):
-                if any(k not in S for k in H.partners[(i, o)]):
+            for o in sorted(o for (j, o) in partners if j == i):
+                if any(k not in S for k in partners[(i, o)]):
                     ch.append(("op", i, o))
         chan[it] = ch
     W: Dict[Node, List[tuple]] = {}
@@ -409,7 +354,7 @@ def _tree_operator(tree: _Tree, H: CouplingModel):
             add((Hh[0], I[1], Hh[2]), 1.0)
             add((I[0], Hh[1], Hh[2]), 1.0)
         legs = 3 if root else 2
-        for (i, o1, j, o2), J in H.twobody.items():
+        for (i, o1, j, o2), J in twobody.items():
             a = [k for k in range(legs) if i in S[k]]
             b = [k for k in range(legs) if j in S[k]]
             if a and b and a[0] != b[0]:
@@ -430,7 +375,7 @@ def _tree_operator(tree: _Tree, H: CouplingModel):
         ch = chan[("site", i)]
         F = np.zeros((len(ch), 2, 2))
         for k, c in enumerate(ch):
-            F[k] = np.eye(2) if c[0] == "I" else H.onebody[i] if c[0] == "H" else SPIN_HALF_OPS[c[2]]
+            F[k] = np.eye(2) if c[0] == "I" else onebody[i] if c[0] == "H" else op_matrix(c[2])
         site_ops[i] = F  # [c][bra][ket]
     return chan, W, site_ops
 

@@ -53,8 +53,10 @@ def test_opstrings_and_params_validation():
     with pytest.raises(ValueError, match="pairs"):
         os += (1.0, "Sz", 1, "Sz")
     os += (1.0, "Sz", 1, "Sz", 2, "Sz", 3)
-    with pytest.raises(NotImplementedError):
-        T.CouplingModel(os, 4)
+    g, sn = T.default_graph_sitenodes(5)
+    tree = TT._Tree(5, g, sn, T.find_eccentric_central_node(g, set(sn.values())))
+    with pytest.raises(NotImplementedError):  # 3-body strings: MPS path only
+        TT._tree_operator(tree, T.CouplingModel(os, 5))
     with pytest.raises(ValueError, match="Size mismatch"):
         T.make_optimize_params(maxdim=[16, 32], nsweeps=[1])
     p = T.make_optimize_params(maxdim=[128], nsweeps=[5], cutoff=1e-14, noise=1e-2, noisedecay=5,
