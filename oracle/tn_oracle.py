@@ -305,6 +305,13 @@ def matvec1(L, W, R, psi):
     return T
 
 
+def matvec0(L, R, C):
+    """Zero-site projected product on the bond matrix C[a, b] (the backward
+    step of a one-site update, src/mps/update_site.jl:176-183):
+    out[a', b'] = sum L[a, w, a'] C[a, b] R[b, w, b']."""
+    return np.einsum("awx,ab,bwy->xy", L, C, R)
+
+
 def matvec_flops(chil: int, chir: int, d: int, w: int) -> float:
     """Algorithmic real flops of one two-site matvec (multiply by 4 for
     complex): 2 chil chir d^2 w (chil + chir) + 4 chil chir d^3 w^2."""
@@ -386,8 +393,10 @@ class Envs:
             out = matvec2(L, self.W[pos - 1], self.W[pos], R, v)
         elif self.nsite == 1:
             out = matvec1(L, self.W[pos - 1], R, v)
+        elif self.nsite == 0:
+            out = matvec0(L, R, v)
         else:
-            raise ValueError("nsite must be 1 or 2")
+            raise ValueError("nsite must be 0, 1 or 2")
         for m in self.m:
             out = out + self.weight * np.vdot(m, v) * m
         return out
@@ -417,6 +426,8 @@ def projector_local(A, M, pos, nsite):
     R = overlap_right(A, M, pos + nsite)
     if nsite == 2:
         return np.einsum("ab,bsx,xtd,cd->astc", L.conj(), M[pos - 1], M[pos], R.conj())
+    if nsite == 0:
+        return np.einsum("ab,cb->ac", L.conj(), R.conj())
     return np.einsum("ab,bsd,cd->asc", L.conj(), M[pos - 1], R.conj())
 
 
