@@ -201,6 +201,26 @@ int tn_svd_trunc(tn_ctx* ctx, const double* M_dev, long m, long n, long maxdim, 
 /* update_position!(sysenv, solver, pos, 2, ortho; kwargs...) */
 int tn_update_two_site(tn_env* env, int pos, int ortho_left, const tn_update_params* p,
                        tn_update_result* r);
+/* update_position!(sysenv, solver, pos, 1, ortho; kwargs...) = _update_one_site!
+ * (src/mps/update_site.jl:94-197): the solver on site pos (orthocenter pos), then
+ * the truncated SVD split of that site with S V (left) or U S (right) moved into
+ * the neighbour; with reverse_step the bond matrix is evolved backward by the
+ * zero-site operator first (nsite = 0); with noise > 0 the site and its
+ * neighbour are factorised as a two-site tensor with the noise term.  At the
+ * end of a half sweep (pos = N going left, pos = 1 going right) only the site
+ * is replaced.  tn_env_set_nsite accepts 0 for the zero-site operator. */
+int tn_update_one_site(tn_env* env, int pos, int ortho_left, const tn_update_params* p,
+                       tn_update_result* r);
+/* krylov_extend!(sysenv; extension_krylovdim, extension_applyH_cutoff,
+ * extension_applyH_maxdim, extension_cutoff) (src/mps/sweep.jl:378-534):
+ * global subspace expansion of the environment's MPS with the Krylov vectors
+ * H^k psi (k = 1..krylovdim, ITensorMPS density-matrix MPO-MPS products with
+ * maxdim applyH_maxdim (<= 0: max bond dimension + 2) and cutoff
+ * applyH_cutoff), then _krylov_addbasis! with ext_cutoff; the MPS ends at
+ * orthocenter 1 and every environment is dropped.  Needs an environment with
+ * one MPO and no projectors (else TN_E_UNSUPPORTED). */
+int tn_env_krylov_extend(tn_env* env, int krylovdim, long applyH_maxdim, double applyH_cutoff,
+                         double ext_cutoff);
 /* fullsweep!(sysenv, solver, 2, swdata; kwargs...): energy of the last
  * update, max truncation error, max bond dim; lasteigs of the mid bond
  * N/2 (right half-sweep) in mid_eigs (capacity cap, count in *nmid). */

@@ -499,6 +499,28 @@ void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bo
   TN_HIP(hipStreamSynchronize(c->stream));
 }
 
+// eigen-decomposition of the padded Hermitian matrix G (ld ppad): divide and
+// conquer from p = 65 (TNHIP_EIG_DC_MIN), two-sided Jacobi below
+static void eig_padded(tn_ctx* c, void* G, long p, long ppad, void* Vt, bool cplx, GramEig& out) {
+  out.p = p;
+  out.ppad = ppad;
+  out.cplx = cplx;
+  out.Vt = Vt;
+  static const long dc_min = [] {  // smallest p for the divide-and-conquer path
+    const char* e = std::getenv("TNHIP_EIG_DC_MIN");
+    return e ? std::atol(e) : 65L;
+  }();
+  if (!cplx && p >= std::max(3L, dc_min)) {
+    sym_eig_dc(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
+    out.sweeps = 0;
+  } else if (cplx && p >= std::max(3L, dc_min)) {
+    sym_eig_dc_z(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
+    out.sweeps = 0;
+  } else {
+    sym_eig_jacobi(c, G, p, ppad, Vt, ppad, cplx, out.w, &out.sweeps);
+  }
+}
+
 void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out, const void* X, long kx,
               double xs, long ldx) {
   TN_REQUIRE(m > 0 && n > 0, TN_E_INVALID, "svd: empty matrix");
@@ -525,24 +547,21 @@ void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, Gr
     else      { h.A = X; h.lda = p; h.tA = 1; h.cA = cplx; h.B = X; h.ldb = p; }
     gemm(c, cplx ? Prec::Z64 : Prec::F64, h);
   }
-  out.p = p;
-  out.ppad = ppad;
   out.left = left;
-  out.cplx = cplx;
-  out.Vt = Vt;
-  static const long dc_min = [] {  // smallest p for the divide-and-conquer path
-    const char* e = std::getenv("TNHIP_EIG_DC_MIN");
-    return e ? std::atol(e) : 65L;
-  }();
-  if (!cplx && p >= std::max(3L, dc_min)) {
-    sym_eig_dc(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
-    out.sweeps = 0;
-  } else if (cplx && p >= std::max(3L, dc_min)) {
-    sym_eig_dc_z(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
-    out.sweeps = 0;
-  } else {
-    sym_eig_jacobi(c, G, p, ppad, Vt, ppad, cplx, out.w, &out.sweeps);
-  }
+  eig_padded(c, G, p, ppad, Vt, cplx, out);
+}
+
+void herm_eig(tn_ctx* c, const void* H, long p, long ldh, bool cplx, GramEig& out) {
+  TN_REQUIRE(p > 0, TN_E_INVALID, "eigen: empty matrix");
+  const long ppad = ((p + PR - 1) / PR) * PR;
+  const size_t es = cplx ? 16 : 8;
+  void* G = c->workspace(WS_SVD_X, (size_t)ppad * ppad * es);
+  void* Vt = c->workspace(WS_SVD_Q, (size_t)ppad * ppad * es);
+  fill_f64(c, (double*)G, 0.0, ppad * ppad * (cplx ? 2 : 1));
+  TN_HIP(hipMemcpy2DAsync(G, (size_t)ppad * es, H, (size_t)ldh * es, (size_t)p * es, (size_t)p,
+                          hipMemcpyDeviceToDevice, c->stream));
+  out.left = true;
+  eig_padded(c, G, p, ppad, Vt, cplx, out);
 }
 
 TruncRule truncate_weights(const std::vector<double>& Pin, long maxdim, long mindim, double cutoff) {

@@ -137,7 +137,7 @@ enum WsSlot {
   WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP,
   WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_PROJ_X, WS_PROJ_Y, WS_PROJ_C,
   WS_QR_X, WS_QR_V, WS_QR_Q, WS_QR_R, WS_QR_S, WS_QR_W, WS_TTN_A, WS_TTN_S, WS_TTN_G, WS_TTN_P, WS_TTN_F,
-  WS_GEMM_SPLIT, WS_SLOTS
+  WS_GEMM_SPLIT, WS_BOND, WS_PHI2, WS_SLOTS
 };
 
 // scoped phase timer (HIP events on the context stream)
@@ -255,6 +255,10 @@ struct GramEig {
 // given, then its eigen-decomposition
 void gram_eig(tn_ctx* c, const void* M, long m, long n, bool left, bool cplx, GramEig& out,
               const void* X = nullptr, long kx = 0, double xs = 0.0, long ldx = 0);
+// eigen-decomposition of a Hermitian p x p device matrix H (row stride ldh,
+// left unchanged): out.w the eigenvalues, out.Vt rows the conjugated
+// eigenvectors (ld out.ppad), as gram_eig gives them
+void herm_eig(tn_ctx* c, const void* H, long p, long ldh, bool cplx, GramEig& out);
 // eigen-decomposition of a symmetric / Hermitian p x p device matrix G (ld
 // ldg, padding rows/columns zero); Vt (ld = padded size) receives V^T / V^H
 void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bool cplx, std::vector<double>& w,

@@ -18,6 +18,7 @@
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
 #include <numeric>
 
 #include "tn_internal.h"
@@ -474,8 +475,13 @@ Dom domain(tn_env* e) {
     D.chir = e->A[pos - 1].chir;
     D.d1 = e->d[pos - 1];
     D.d2 = 1;
+  } else if (e->nsite == 0) {  // bond matrix between sites pos-1 and pos
+    D.chil = pos >= 2 ? e->A[pos - 2].chir : 1;
+    D.chir = pos <= e->N ? e->A[pos - 1].chil : 1;
+    D.d1 = 1;
+    D.d2 = 1;
   } else {
-    throw Error(TN_E_UNSUPPORTED, "nsite must be 1 or 2");
+    throw Error(TN_E_UNSUPPORTED, "nsite must be 0, 1 or 2");
   }
   D.n = D.chil * D.d1 * D.d2 * D.chir;
   return D;
@@ -529,11 +535,50 @@ SplitView split_shadow(tn_env* e, int k, bool left) {
   return SplitView{H.as<unsigned short>(), Lo.as<unsigned short>(), ld};
 }
 
+// zero-site product on the bond matrix C[a, b] between sites pos-1 and pos
+// (the backward step of a one-site update, update_site.jl:176-183):
+//   out[a', b'] = sum_w sum_{a,b} L[w][a'][a] C[a][b] R[w][b][b']
+// T[(w a'), b] = L C, permuted to T'[a'][(w b)], then out = T' R[(w b), b'].
+// Always FP64 (complex when the environment is).
+void matvec_zero(tn_env* e, const Dom& D, const void* v, void* out) {
+  tn_ctx* c = e->ctx;
+  const int pos = D.pos;
+  const EnvT& L = e->LR[pos - 1];
+  const EnvT& R = e->LR[pos];
+  const long chil = D.chil, chir = D.chir;
+  const int w = e->w[pos - 1];
+  TN_REQUIRE(L.chi == chil && L.w == w && R.chi == chir && R.w == w, TN_E_SHAPE,
+             "The order of the ProjMPO-ITensor product P*v is not equal to the order of the ITensor v: "
+             "environment and bond-matrix shapes disagree");
+  const bool cx = e->cplx();
+  const Prec P = cx ? Prec::Z64 : Prec::F64;
+  const int m2 = cx ? 2 : 1;
+  double* T1 = (double*)c->workspace(WS_T1, (size_t)w * chil * chir * e->esz);
+  double* T2 = (double*)c->workspace(WS_T2, (size_t)w * chil * chir * e->esz);
+  GemmDesc g1;
+  g1.M = (long)w * chil; g1.N = chir; g1.K = chil;
+  g1.A = L.buf.p; g1.lda = chil;
+  g1.B = v; g1.ldb = chir;
+  g1.C = T1; g1.ldc = chir;
+  gemm(c, P, g1);
+  swap01_f64(c, T1, w, chil, chir * m2, T2);
+  GemmDesc g2;
+  g2.M = chil; g2.N = chir; g2.K = (long)w * chir;
+  g2.A = T2; g2.lda = (long)w * chir;
+  g2.B = R.buf.p; g2.ldb = chir;
+  g2.C = out; g2.ldc = chir;
+  gemm(c, P, g2);
+}
+
 // product(sysenv, v) (ProjMPO.product; mirror src/mps/projmps2.jl:159-173)
 void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   PhaseTimer pt(e->ctx, PH_MATVEC);
   tn_ctx* c = e->ctx;
   const Dom D = domain(e);
+  if (e->nsite == 0) {
+    matvec_zero(e, D, v, out);
+    return;
+  }
   const int pos = D.pos;
   const EnvT& L = e->LR[pos - 1];
   const EnvT& R = e->LR[pos + e->nsite];
@@ -761,6 +806,19 @@ void proj_local(tn_env* e, ProjT& P, const Dom& D) {
   const long cml = P.chiM[pos - 1], cmr = P.chiM[pos + e->nsite];
   const int DD = (e->nsite == 2) ? D.d1 * D.d2 : D.d1;
   TN_REQUIRE(L.chi == D.chil && R.chi == D.chir, TN_E_SHAPE, "projector: environment shapes disagree");
+  if (e->nsite == 0) {  // m[a, b] = sum conj(L[a, c]) conj(R[b, c])
+    P.m.ensure((size_t)D.n * e->esz);
+    GemmDesc g;
+    g.M = D.chil; g.N = D.chir; g.K = cml;
+    g.A = L.buf.p; g.lda = cml; g.cA = cx;
+    g.B = R.buf.p; g.ldb = cmr; g.tB = 1; g.cB = cx;
+    g.C = P.m.p; g.ldc = D.chir;
+    gemm(c, pr, g);
+    P.m_ok = true;
+    P.m_pos = D.pos;
+    P.m_nsite = 0;
+    return;
+  }
   const void* Mphi = P.M[pos - 1].buf.p;
   if (e->nsite == 2) {
     const SiteT& M1 = P.M[pos - 1];
@@ -1384,6 +1442,513 @@ void update_two_site(tn_env* e, int pos, bool ortho_left, const tn_update_params
   if (eigs_out) eigs_out->swap(eigs);
 }
 
+
+// SVD split of the one-site tensor phi of site pos (update_site.jl:154-173).
+// ortho left:  A_pos = U_k (chil d x k), C = S_k V_k^H (k x chir);
+// ortho right: A_pos = V_k^H (k x d chir), C = U_k S_k (chil x k).
+// S is normalised when `normalize`.  C is left in WS_BOND; returns k.
+long split_site(tn_env* e, int pos, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
+                bool normalize, double* truncerr_out, std::vector<double>* eigs_out, double** C_out) {
+  PhaseTimer pt(e->ctx, PH_SVD);
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const int w2 = cx ? 2 : 1;
+  SiteT& S = e->A[pos - 1];
+  const long chil = S.chil, chir = S.chir;
+  const int d = e->d[pos - 1];
+  const long m = ortho_left ? chil * d : chil;
+  const long n = ortho_left ? chir : (long)d * chir;
+  GramEig ge;
+  gram_eig(c, phi, m, n, ortho_left, cx, ge);
+  std::vector<int> order;
+  std::vector<double> P;
+  sorted_weights(ge.w, std::min(m, n), order, P);
+  const TruncRule tr = truncate_weights(P, maxdim, mindim, cutoff);
+  const long k = tr.n;
+  double nrm = 0.0;
+  for (long i = 0; i < k; ++i) nrm += P[i];
+  nrm = std::sqrt(nrm);
+  const double inv_norm = (normalize && nrm > 0.0) ? 1.0 / nrm : 1.0;
+  int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)k * sizeof(int));
+  double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)k * sizeof(double));
+  std::vector<double> ones(k, 1.0);
+  TN_HIP(hipMemcpyAsync(dperm, order.data(), k * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  TN_HIP(hipMemcpyAsync(dsc, ones.data(), k * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const double* Vt = (const double*)ge.Vt;
+  const Prec P64 = gemm_prec_full(e);
+  double* C = (double*)c->workspace(WS_BOND, (size_t)(ortho_left ? k * n : m * k) * e->esz);
+  S.buf.ensure((size_t)m * k * e->esz > (size_t)k * n * e->esz ? (size_t)m * k * e->esz : (size_t)k * n * e->esz);
+  if (ortho_left) {
+    if (cx) gather_rows_T_z(c, Vt, ge.ppad, dperm, dsc, k, m, S.buf.as<double>(), true);
+    else gather_rows_T(c, Vt, ge.ppad, dperm, dsc, k, m, S.buf.as<double>());
+    double* Vk = (double*)c->workspace(WS_SVD_GRAM, (size_t)k * m * e->esz);
+    gather_rows(c, Vt, ge.ppad * w2, dperm, dsc, k, m * w2, Vk);
+    GemmDesc g;  // C = V_k^H phi / |.|
+    g.M = k; g.N = n; g.K = m;
+    g.A = Vk; g.lda = m;
+    g.B = phi; g.ldb = n;
+    g.C = C; g.ldc = n;
+    g.alpha = inv_norm;
+    gemm(c, P64, g);
+    S.chir = k;
+  } else {
+    gather_rows(c, Vt, ge.ppad * w2, dperm, dsc, k, n * w2, S.buf.as<double>());
+    GemmDesc g;  // C = phi V_k / |.|
+    g.M = m; g.N = k; g.K = n;
+    g.A = phi; g.lda = n;
+    g.B = S.buf.p; g.ldb = n; g.tB = 1; g.cB = cx;
+    g.C = C; g.ldc = k;
+    g.alpha = inv_norm;
+    gemm(c, P64, g);
+    S.chil = k;
+  }
+  TN_HIP(hipStreamSynchronize(c->stream));  // host vectors above go out of scope
+  invalidate_site(e, pos);
+  if (truncerr_out) *truncerr_out = tr.truncerr;
+  if (eigs_out) eigs_out->assign(P.begin(), P.begin() + k);
+  *C_out = C;
+  return k;
+}
+
+// _update_one_site! (src/mps/update_site.jl:94-197)
+void update_one_site(tn_env* e, int pos, bool ortho_left, const tn_update_params* p, tn_update_result* r,
+                     std::vector<double>* eigs_out = nullptr) {
+  TN_REQUIRE(p != nullptr, TN_E_INVALID, "null params");
+  TN_REQUIRE(pos >= 1 && pos <= e->N, TN_E_POSITION, "update_position!: site out of range");
+  TN_REQUIRE(!(p->solver == 0 && p->reverse_step), TN_E_INVALID,
+             "update_position!: reverse_step needs a time step (exp_solver)");
+  TN_REQUIRE(!(std::fabs(p->noise) > 1e-15 && p->reverse_step), TN_E_INVALID,
+             "`updatePosition()` :: `noise` cannot be greater than zero for `reverse_step=true` !!");
+  TN_REQUIRE(e->ortho == pos, TN_E_POSITION,
+             "update_position!: orthocenter is " + std::to_string(e->ortho) + ", expected " + std::to_string(pos));
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  e->nsite = 1;
+  SiteT& S = e->A[pos - 1];
+  const long n = S.chil * e->d[pos - 1] * S.chir;
+  const long nd = cx ? 2 * n : n;
+  double* phi = (double*)c->workspace(WS_PHI, (size_t)nd * sizeof(double));
+  copy_f64(c, phi, S.buf.as<double>(), nd);
+  position(e, pos);
+  tn_update_result res{};
+  if (r) { res.eigs = r->eigs; res.eigs_cap = r->eigs_cap; }
+  cplx energy(NAN, 0.0);
+  const cplx t(p->time_step_re, p->time_step_im);
+  if (p->solver == 0) {
+    const double th = krylov_eig(e, phi, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
+    if (p->check_convergence && res.info.converged < 1)
+      throw Error(TN_E_NOTCONV, "`eig_solver()` not converged !!");
+    energy = th;
+    if (p->refine_fp64 && p->precision != TN_PREC_FP64) energy = rayleigh(e, phi);
+  } else {
+    krylov_expm(e, phi, t, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
+    if (p->check_convergence && res.info.converged < 1)
+      throw Error(TN_E_NOTCONV, "`eig_solver()` not converged !!");
+  }
+  if (p->normalize) {
+    Krylov K(e, n, 1);
+    const double nr = std::sqrt(K.norm2(phi));
+    if (nr > 0) scale_f64(c, phi, 1.0 / nr, nd);
+  }
+  if (std::isnan(energy.real())) energy = rayleigh(e, phi);
+  double terr = 0.0;
+  long nk = 0;
+  std::vector<double> eigs;
+  const bool done = (pos == 1 && !ortho_left) || (pos == e->N && ortho_left);
+  if (done) {
+    copy_f64(c, S.buf.as<double>(), phi, nd);
+    invalidate_site(e, pos);
+  } else if (std::fabs(p->noise) > 1e-15) {
+    // two-site density-matrix factorisation of phi and its neighbour
+    const int pos0 = ortho_left ? pos : pos - 1;
+    const SiteT& Sa = e->A[pos0 - 1];
+    const SiteT& Sb = e->A[pos0];
+    const long n2 = Sa.chil * e->d[pos0 - 1] * e->d[pos0] * Sb.chir;
+    double* phi2 = (double*)c->workspace(WS_PHI2, (size_t)n2 * e->esz);
+    GemmDesc g;
+    g.M = Sa.chil * e->d[pos0 - 1]; g.N = (long)e->d[pos0] * Sb.chir; g.K = Sa.chir;
+    g.A = ortho_left ? (const void*)phi : Sa.buf.p; g.lda = Sa.chir;
+    g.B = ortho_left ? Sb.buf.p : (const void*)phi; g.ldb = (long)e->d[pos0] * Sb.chir;
+    g.C = phi2; g.ldc = (long)e->d[pos0] * Sb.chir;
+    gemm(c, gemm_prec_full(e), g);
+    e->nsite = 2;
+    position(e, pos0);
+    replacebond(e, pos0, phi2, ortho_left, p->maxdim, p->mindim, p->cutoff, p->normalize != 0, &terr, &eigs, &nk,
+                p->noise);
+    e->nsite = 1;
+  } else {
+    double* C = nullptr;
+    nk = split_site(e, pos, phi, ortho_left, p->maxdim, p->mindim, p->cutoff, p->normalize != 0, &terr, &eigs, &C);
+    const int nxt = ortho_left ? pos + 1 : pos - 1;
+    SiteT& Nx = e->A[nxt - 1];
+    const long crow = ortho_left ? nk : Nx.chir;     // C is crow x ccol
+    const long ccol = ortho_left ? Nx.chil : nk;
+    if (p->reverse_step) {
+      const int pos1 = ortho_left ? pos + 1 : pos;
+      const long nb = crow * ccol;
+      const long ndb = cx ? 2 * nb : nb;
+      e->nsite = 0;
+      position(e, pos1);
+      krylov_expm(e, C, -t, p->krylovdim, p->maxiter, p->tol, TN_PREC_FP64, nullptr);
+      if (p->normalize) {
+        Krylov K(e, nb, 1);
+        const double nr = std::sqrt(K.norm2(C));
+        if (nr > 0) scale_f64(c, C, 1.0 / nr, ndb);
+      }
+      energy = rayleigh(e, C);
+    }
+    DevBuf newN;
+    GemmDesc g;
+    if (ortho_left) {  // A_{pos+1} <- C A_{pos+1}
+      const long nn = (long)e->d[nxt - 1] * Nx.chir;
+      newN.ensure((size_t)nk * nn * e->esz);
+      g.M = nk; g.N = nn; g.K = Nx.chil;
+      g.A = C; g.lda = Nx.chil;
+      g.B = Nx.buf.p; g.ldb = nn;
+      g.C = newN.p; g.ldc = nn;
+    } else {           // A_{pos-1} <- A_{pos-1} C
+      const long mm = Nx.chil * e->d[nxt - 1];
+      newN.ensure((size_t)mm * nk * e->esz);
+      g.M = mm; g.N = nk; g.K = Nx.chir;
+      g.A = Nx.buf.p; g.lda = Nx.chir;
+      g.B = C; g.ldb = nk;
+      g.C = newN.p; g.ldc = nk;
+    }
+    gemm(c, gemm_prec_full(e), g);
+    TN_HIP(hipStreamSynchronize(c->stream));
+    Nx.buf = std::move(newN);
+    if (ortho_left) Nx.chil = nk;
+    else Nx.chir = nk;
+    invalidate_site(e, nxt);
+    e->ortho = nxt;
+  }
+  res.energy = energy.real();
+  res.energy_im = energy.imag();
+  res.truncerr = terr;
+  res.nkept = nk;
+  if (res.eigs)
+    for (long i = 0; i < std::min<long>(res.eigs_cap, (long)eigs.size()); ++i) res.eigs[i] = eigs[i];
+  if (r) *r = res;
+  if (eigs_out) eigs_out->swap(eigs);
+}
+
+// ---------------------------------------------------------------------------
+// global subspace expansion (krylov_extend!, src/mps/sweep.jl:378-534)
+// ---------------------------------------------------------------------------
+using Sites = std::vector<SiteT>;
+
+struct EnvOwner {
+  tn_env* e = nullptr;
+  ~EnvOwner() {
+    if (e) tn_env_destroy(e);
+  }
+};
+
+void copy_sites(tn_env* e, const Sites& from, Sites& to) {
+  to.resize(from.size());
+  for (size_t k = 0; k < from.size(); ++k) {
+    const long n = from[k].chil * e->d[k] * from[k].chir;
+    to[k].buf.ensure((size_t)n * e->esz);
+    copy_f64(e->ctx, to[k].buf.as<double>(), from[k].buf.as<double>(), e->cplx() ? 2 * n : n);
+    to[k].chil = from[k].chil;
+    to[k].chir = from[k].chir;
+  }
+}
+
+// H psi by ITensorMPS's "densitymatrix" contraction: e2 holds the MPO
+// (H^dag H)[(w w'), s', s, (v v')] = sum_t W[w, t, s, v] conj(W[w', t, s', v'])
+// whose left environments are those of <H psi|H psi>; from the right, the
+// reduced density matrix of H psi on (site j, kept right basis) gives the
+// right-orthonormal site j (its dominant eigenvectors, NDTensors truncation
+// with maxdim / cutoff), the first site keeps the rest.
+void apply_mpo(tn_env* e, tn_env* e2, const Sites& in, long maxdim, double cutoff, Sites& out) {
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const int m2 = cx ? 2 : 1;
+  const Prec P64 = gemm_prec_full(e);
+  const int N = e->N;
+  copy_sites(e, in, e2->A);
+  for (int k = 1; k <= N; ++k) invalidate_site(e2, k);
+  makeL(e2, N - 1);
+  out.resize(N);
+  DevBuf Rb, Pb, Eb, Zb, rb, Qb;
+  // R_N[w_{N-1}][a][t][1] = sum_s W_N[w, t, s, 1] psi_N[a, s]
+  long K = 1;
+  {
+    const long chi = in[N - 1].chil;
+    const int d = e->d[N - 1];
+    Rb.ensure((size_t)e->w[N - 1] * chi * d * K * e->esz);
+    WStepDesc ws;
+    ws.G = e->w[N]; ws.Gp = e->w[N - 1]; ws.D = d; ws.Dp = d;
+    ws.nA = chi; ws.nB = K * m2;
+    ws.in = in[N - 1].buf.p; ws.in_sg = chi * d * K * m2; ws.in_sA = d * K * m2; ws.in_sS = K * m2;
+    ws.out = Rb.p; ws.out_sg = chi * d * K * m2; ws.out_sA = d * K * m2; ws.out_sS = K * m2;
+    ws.op = &op_right(e, N);
+    wstep(c, false, ws);
+  }
+  for (int j = N; j >= 2; --j) {
+    const long chi = in[j - 1].chil;
+    const int W = e->w[j - 1];
+    const int d = e->d[j - 1];
+    const long p = (long)d * K;
+    const long WC = (long)W * chi;
+    const EnvT& E = e2->LR[j - 1];
+    TN_REQUIRE(E.chi == chi && E.w == W * W, TN_E_SHAPE, "apply: environment shapes disagree");
+    Eb.ensure((size_t)WC * WC * e->esz);
+    swap01_f64(c, E.buf.as<double>(), W, WC, chi * m2, Eb.as<double>());  // E'[(w' a')][w][a]
+    Zb.ensure((size_t)WC * p * e->esz);
+    GemmDesc g1;  // Z[(w' a'), q] = E'[(w' a'), (w a)] R[(w a), q]
+    g1.M = WC; g1.N = p; g1.K = WC;
+    g1.A = Eb.p; g1.lda = WC;
+    g1.B = Rb.p; g1.ldb = p;
+    g1.C = Zb.p; g1.ldc = p;
+    gemm(c, P64, g1);
+    rb.ensure((size_t)p * p * e->esz);
+    GemmDesc g2;  // rho[q', q] = sum conj(R[(w' a'), q']) Z[(w' a'), q]
+    g2.M = p; g2.N = p; g2.K = WC;
+    g2.A = Rb.p; g2.lda = p; g2.tA = 1; g2.cA = cx;
+    g2.B = Zb.p; g2.ldb = p;
+    g2.C = rb.p; g2.ldc = p;
+    gemm(c, P64, g2);
+    GramEig ge;
+    herm_eig(c, rb.p, p, p, cx, ge);
+    std::vector<int> order;
+    std::vector<double> P;
+    sorted_weights(ge.w, p, order, P);
+    const TruncRule tr = truncate_weights(P, maxdim, 1, cutoff);
+    const long kn = tr.n;
+    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)kn * sizeof(int));
+    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)kn * sizeof(double));
+    std::vector<double> ones(kn, 1.0);
+    TN_HIP(hipMemcpyAsync(dperm, order.data(), kn * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(dsc, ones.data(), kn * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    SiteT& O = out[j - 1];
+    O.buf.ensure((size_t)kn * p * e->esz);
+    gather_rows(c, (const double*)ge.Vt, ge.ppad * m2, dperm, dsc, kn, p * m2, O.buf.as<double>());
+    O.chil = kn;
+    O.chir = K;
+    Qb.ensure((size_t)WC * kn * e->esz);
+    GemmDesc g3;  // Q[(w a), n] = R[(w a), q] conj(Ut[n, q])
+    g3.M = WC; g3.N = kn; g3.K = p;
+    g3.A = Rb.p; g3.lda = p;
+    g3.B = O.buf.p; g3.ldb = p; g3.tB = 1; g3.cB = cx;
+    g3.C = Qb.p; g3.ldc = kn;
+    gemm(c, P64, g3);
+    const long chi0 = in[j - 2].chil;
+    const int d0 = e->d[j - 2];
+    Pb.ensure((size_t)W * chi0 * d0 * kn * e->esz);
+    GemmDesc g4;  // P[w][(a0 s)][n] = psi_{j-1}[(a0 s), a] Q[w][a][n]
+    g4.M = chi0 * d0; g4.N = kn; g4.K = chi;
+    g4.A = in[j - 2].buf.p; g4.lda = chi; g4.sA = 0;
+    g4.B = Qb.p; g4.ldb = kn; g4.sB = chi * kn;
+    g4.C = Pb.p; g4.ldc = kn; g4.sC = chi0 * d0 * kn;
+    g4.batch = W;
+    gemm(c, P64, g4);
+    Rb.ensure((size_t)e->w[j - 2] * chi0 * d0 * kn * e->esz);
+    WStepDesc ws;  // R_{j-1}[w0][a0][t][n] = sum W_{j-1}[w0, t, s, w] P[w][a0][s][n]
+    ws.G = W; ws.Gp = e->w[j - 2]; ws.D = d0; ws.Dp = d0;
+    ws.nA = chi0; ws.nB = kn * m2;
+    ws.in = Pb.p; ws.in_sg = chi0 * d0 * kn * m2; ws.in_sA = d0 * kn * m2; ws.in_sS = kn * m2;
+    ws.out = Rb.p; ws.out_sg = chi0 * d0 * kn * m2; ws.out_sA = d0 * kn * m2; ws.out_sS = kn * m2;
+    ws.op = &op_right(e, j - 1);
+    wstep(c, false, ws);
+    TN_HIP(hipStreamSynchronize(c->stream));  // host vectors above go out of scope
+    K = kn;
+  }
+  SiteT& O1 = out[0];
+  const long n1 = (long)e->d[0] * K;
+  O1.buf.ensure((size_t)n1 * e->esz);
+  copy_f64(c, O1.buf.as<double>(), Rb.as<double>(), cx ? 2 * n1 : n1);
+  O1.chil = 1;
+  O1.chir = K;
+}
+
+// _krylov_addbasis!(psi, phis, cutoff) (src/mps/sweep.jl:458-534) on the
+// environment's psi; phis are brought to orthocenter N on e2.
+void krylov_addbasis(tn_env* e, tn_env* e2, std::vector<Sites>& phis, double ext_cutoff) {
+  tn_ctx* c = e->ctx;
+  const bool cx = e->cplx();
+  const int m2 = cx ? 2 : 1;
+  const Prec P64 = gemm_prec_full(e);
+  const int N = e->N;
+  orthogonalize(e, N);
+  for (Sites& ph : phis) {
+    for (int k = 0; k < N; ++k) e2->A[k] = std::move(ph[k]);
+    orthogonalize(e2, N);
+    for (int k = 0; k < N; ++k) ph[k] = std::move(e2->A[k]);
+  }
+  DevBuf Bx, G, GP, T1, T2, Tb;
+  for (int j = N; j >= 2; --j) {
+    SiteT& S = e->A[j - 1];
+    const long m = S.chil, n = (long)e->d[j - 1] * S.chir;
+    // B: right singular vectors of psi_j (no truncation)
+    GramEig ge;
+    gram_eig(c, S.buf.p, m, n, false, cx, ge);
+    std::vector<int> order;
+    std::vector<double> P;
+    const long r = std::min(m, n);
+    sorted_weights(ge.w, r, order, P);
+    int* dperm = (int*)c->workspace(WS_SVD_PERM, (size_t)n * sizeof(int));
+    double* dsc = (double*)c->workspace(WS_SVD_SCALE, (size_t)n * sizeof(double));
+    std::vector<double> ones(n, 1.0);
+    TN_HIP(hipMemcpyAsync(dperm, order.data(), r * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(dsc, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    Bx.ensure((size_t)2 * n * n * e->esz);
+    gather_rows(c, (const double*)ge.Vt, ge.ppad * m2, dperm, dsc, r, n * m2, Bx.as<double>());
+    TN_HIP(hipStreamSynchronize(c->stream));
+    // G = sum_phi phi_j^H phi_j / tr  (the conjugate of rho = sum phi^T conj(phi))
+    G.ensure((size_t)n * n * e->esz);
+    double tr = 0.0;
+    for (size_t i = 0; i < phis.size(); ++i) {
+      SiteT& F = phis[i][j - 1];
+      TN_REQUIRE(F.chir == S.chir, TN_E_SHAPE, "krylov_extend!: Krylov vector bonds disagree");
+      GemmDesc g;
+      g.M = n; g.N = n; g.K = F.chil;
+      g.A = F.buf.p; g.lda = n; g.tA = 1; g.cA = cx;
+      g.B = F.buf.p; g.ldb = n;
+      g.C = G.p; g.ldc = n;
+      g.beta = i == 0 ? 0.0 : 1.0;
+      gemm(c, P64, g);
+      Krylov K(c, cx, F.chil * n, 1);
+      tr += K.norm2(F.buf.as<double>());
+    }
+    TN_REQUIRE(tr > 0.0, TN_E_NUMERIC, "krylov_extend!: zero Krylov vector");
+    scale_f64(c, G.as<double>(), 1.0 / tr, n * n * m2);
+    // X = (1 - B^H B) G (1 - B^H B)
+    T1.ensure((size_t)r * n * e->esz);
+    GemmDesc a1;  // T1 = B G
+    a1.M = r; a1.N = n; a1.K = n;
+    a1.A = Bx.p; a1.lda = n;
+    a1.B = G.p; a1.ldb = n;
+    a1.C = T1.p; a1.ldc = n;
+    gemm(c, P64, a1);
+    GP.ensure((size_t)n * n * e->esz);
+    copy_f64(c, GP.as<double>(), G.as<double>(), n * n * m2);
+    GemmDesc a2;  // GP = G - B^H T1
+    a2.M = n; a2.N = n; a2.K = r;
+    a2.A = Bx.p; a2.lda = n; a2.tA = 1; a2.cA = cx;
+    a2.B = T1.p; a2.ldb = n;
+    a2.C = GP.p; a2.ldc = n;
+    a2.alpha = -1.0; a2.beta = 1.0;
+    gemm(c, P64, a2);
+    T2.ensure((size_t)n * r * e->esz);
+    GemmDesc a3;  // T2 = GP B^H
+    a3.M = n; a3.N = r; a3.K = n;
+    a3.A = GP.p; a3.lda = n;
+    a3.B = Bx.p; a3.ldb = n; a3.tB = 1; a3.cB = cx;
+    a3.C = T2.p; a3.ldc = r;
+    gemm(c, P64, a3);
+    copy_f64(c, G.as<double>(), GP.as<double>(), n * n * m2);
+    GemmDesc a4;  // X = GP - T2 B
+    a4.M = n; a4.N = n; a4.K = r;
+    a4.A = T2.p; a4.lda = r;
+    a4.B = Bx.p; a4.ldb = n;
+    a4.C = G.p; a4.ldc = n;
+    a4.alpha = -1.0; a4.beta = 1.0;
+    gemm(c, P64, a4);
+    long kx = r;
+    {
+      Krylov K(c, cx, n * n, 1);
+      const double xn = std::sqrt(K.norm2(G.as<double>()));
+      if (xn > 1e-14) {
+        GramEig gx;
+        herm_eig(c, G.p, n, n, cx, gx);
+        std::vector<int> ox;
+        std::vector<double> Px;
+        sorted_weights(gx.w, n, ox, Px);
+        const TruncRule t = truncate_weights(Px, std::numeric_limits<long>::max(), 1, ext_cutoff);
+        TN_HIP(hipMemcpyAsync(dperm, ox.data(), t.n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        gather_rows(c, (const double*)gx.Vt, gx.ppad * m2, dperm, dsc, t.n, n * m2,
+                    Bx.as<double>() + (size_t)r * n * m2);
+        TN_HIP(hipStreamSynchronize(c->stream));
+        kx = r + t.n;
+      }
+    }
+    // every state: site j-1 absorbs site_j Bx^H, site j becomes Bx
+    auto move_state = [&](SiteT& Aj, SiteT& Ap) {
+      Tb.ensure((size_t)Aj.chil * kx * e->esz);
+      GemmDesc g;
+      g.M = Aj.chil; g.N = kx; g.K = n;
+      g.A = Aj.buf.p; g.lda = n;
+      g.B = Bx.p; g.ldb = n; g.tB = 1; g.cB = cx;
+      g.C = Tb.p; g.ldc = kx;
+      gemm(c, P64, g);
+      const long mm = Ap.chil * e->d[j - 2];
+      DevBuf np;
+      np.ensure((size_t)mm * kx * e->esz);
+      GemmDesc h;
+      h.M = mm; h.N = kx; h.K = Aj.chil;
+      h.A = Ap.buf.p; h.lda = Aj.chil;
+      h.B = Tb.p; h.ldb = kx;
+      h.C = np.p; h.ldc = kx;
+      gemm(c, P64, h);
+      Aj.buf.ensure((size_t)kx * n * e->esz);
+      copy_f64(c, Aj.buf.as<double>(), Bx.as<double>(), kx * n * m2);
+      TN_HIP(hipStreamSynchronize(c->stream));
+      Ap.buf = std::move(np);
+      Ap.chir = kx;
+      Aj.chil = kx;
+    };
+    move_state(e->A[j - 1], e->A[j - 2]);
+    for (Sites& ph : phis) move_state(ph[j - 1], ph[j - 2]);
+    invalidate_site(e, j);
+    invalidate_site(e, j - 1);
+  }
+  orthogonalize(e, 1);
+}
+
+void krylov_extend(tn_env* e, int krylovdim, long applyH_maxdim, double applyH_cutoff, double ext_cutoff) {
+  TN_REQUIRE(krylovdim >= 1, TN_E_INVALID, "krylov_extend!: extension_krylovdim must be >= 1");
+  TN_REQUIRE(e->proj.empty(), TN_E_UNSUPPORTED, "krylov_extend!: the StateEnvs must be created by a single MPO");
+  require_mpo(e);
+  require_sites(e);
+  tn_ctx* c = e->ctx;
+  const int N = e->N;
+  // H^dag H on the host (real MPO)
+  std::vector<int> w2(N + 1);
+  for (int k = 0; k <= N; ++k) w2[k] = e->w[k] * e->w[k];
+  EnvOwner own;
+  int rc = tn_env_create(c, N, e->d.data(), w2.data(), e->dtype, &own.e);
+  if (rc != 0) throw Error(rc, "krylov_extend!: cannot create the H^dag H environment");
+  tn_env* e2 = own.e;
+  for (int k = 1; k <= N; ++k) {
+    const int wl = e->w[k - 1], wr = e->w[k], d = e->d[k - 1];
+    const std::vector<double>& W = e->Wh[k - 1];
+    std::vector<double> HH((size_t)wl * wl * d * d * wr * wr, 0.0);
+    auto Wi = [&](int a, int t, int s, int b) { return W[(((size_t)a * d + t) * d + s) * wr + b]; };
+    for (int a = 0; a < wl; ++a)
+      for (int a2 = 0; a2 < wl; ++a2)
+        for (int sp = 0; sp < d; ++sp)
+          for (int s0 = 0; s0 < d; ++s0)
+            for (int b = 0; b < wr; ++b)
+              for (int b2 = 0; b2 < wr; ++b2) {
+                double acc = 0.0;
+                for (int t = 0; t < d; ++t) acc += Wi(a, t, s0, b) * Wi(a2, t, sp, b2);
+                HH[((((size_t)(a * wl + a2) * d + sp) * d + s0) * wr + b) * wr + b2] = acc;
+              }
+    rc = tn_env_set_mpo(e2, k, HH.data());
+    if (rc != 0) throw Error(rc, "krylov_extend!: cannot set the H^dag H MPO");
+  }
+  long maxlink = 1;
+  for (int k = 0; k < N; ++k) maxlink = std::max(maxlink, e->A[k].chir);
+  if (applyH_maxdim <= 0) applyH_maxdim = maxlink + 2;
+  std::vector<Sites> phis(krylovdim);
+  for (int k = 0; k < krylovdim; ++k) {
+    apply_mpo(e, e2, k == 0 ? e->A : phis[k - 1], applyH_maxdim, applyH_cutoff, phis[k]);
+    SiteT& F = phis[k][0];
+    const long n1 = F.chil * e->d[0] * F.chir;
+    Krylov K(c, e->cplx(), n1, 1);
+    const double nr = std::sqrt(K.norm2(F.buf.as<double>()));
+    TN_REQUIRE(nr > 0.0, TN_E_NUMERIC, "krylov_extend!: H psi vanishes");
+    scale_f64(c, F.buf.as<double>(), 1.0 / nr, e->cplx() ? 2 * n1 : n1);
+  }
+  krylov_addbasis(e, e2, phis, ext_cutoff);
+  for (int k = 1; k <= N; ++k) invalidate_site(e, k);
+  e->lpos = 0;
+  e->rpos = N + 1;
+  e->nsite = 2;
+  e->ortho = 1;
+}
 }  // namespace
 
 // ===========================================================================
@@ -1634,7 +2199,7 @@ int tn_env_orthocenter(tn_env* e, int* k) {
 int tn_env_set_nsite(tn_env* e, int nsite) {
   return guarded(C(e), [&] {
     check_env(e);
-    TN_REQUIRE(nsite == 1 || nsite == 2, TN_E_UNSUPPORTED, "set_nsite!: only nsite 1 and 2 are built");
+    TN_REQUIRE(nsite >= 0 && nsite <= 2, TN_E_UNSUPPORTED, "set_nsite!: only nsite 0, 1 and 2 are built");
     e->nsite = nsite;
   });
 }
@@ -1862,6 +2427,20 @@ int tn_update_two_site(tn_env* e, int pos, int ortho_left, const tn_update_param
   return guarded(C(e), [&] {
     check_env(e);
     update_two_site(e, pos, ortho_left != 0, p, r);
+  });
+}
+
+int tn_env_krylov_extend(tn_env* e, int krylovdim, long applyH_maxdim, double applyH_cutoff, double ext_cutoff) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    krylov_extend(e, krylovdim, applyH_maxdim, applyH_cutoff, ext_cutoff);
+  });
+}
+
+int tn_update_one_site(tn_env* e, int pos, int ortho_left, const tn_update_params* p, tn_update_result* r) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    update_one_site(e, pos, ortho_left != 0, p, r);
   });
 }
 

@@ -9,8 +9,9 @@ from . import core as _core
 from . import ttn as _ttn
 from ._lib import (TNHIPError, TN_PREC_BF16X3, TN_PREC_FP32, TN_PREC_FP64, load)  # noqa: F401
 from .core import (Context, DeviceArray, DMRGParams, StateEnvs, SweepData, TDVPEngine,  # noqa: F401
-                   default_context, dmrg, dmrg2, dmrg_, eig_solver, exp_solver, make_dmrg_params, nsite,
-                   orthogonalize_, set_nsite_, tdvpsweep_)
+                   default_context, dmrg, dmrg1, dmrg2, dmrg_, dynamic_fullsweep_, eig_solver, exp_solver,
+                   krylov_extend_, make_dmrg_params, nsite, orthogonalize_, set_nsite_, tdvpsweep_)
+from .couplingmodel import coupling_mpo, mpo_sum_ids  # noqa: F401
 from .ttn import (TTN, CouplingModel, OpStrings, OptimizeParamsTTN, StateEnvsTTN, SweepDataTTN,  # noqa: F401
                   default_graph_sitenodes, default_randomTTN, default_sweeppath, find_eccentric_central_node,
                   make_optimize_params, maxlinkdim, nextnode_in_path, optimize, optimize_, randomTTN,

@@ -94,6 +94,7 @@ SIGNATURES = {
     "tn_env_set_projectors": [_P, _I, _D],
     "tn_env_set_projector_site": [_P, _I, _I, _P, _L, _L],
     "tn_env_set_noise_channels": [_P, _P, _P],
+    "tn_env_krylov_extend": [_P, _I, _L, _D, _D],
     "tn_ttn_create": [_P, _I, C.POINTER(_P)],
     "tn_ttn_destroy": [_P],
     "tn_ttn_set_link": [_P, _I, _I, _I, _I, _I],
@@ -126,6 +127,7 @@ SIGNATURES = {
     "tn_replacebond": [_P, _I, _P, _I, _L, _L, _D, _D, _I, _PD, _PD, _L, _PL],
     "tn_svd_trunc": [_P, _P, _L, _L, _L, _L, _D, _P, _PD, _P, _PL, _PD],
     "tn_update_two_site": [_P, _I, _I, C.POINTER(UpdateParams), C.POINTER(UpdateResult)],
+    "tn_update_one_site": [_P, _I, _I, C.POINTER(UpdateParams), C.POINTER(UpdateResult)],
     "tn_fullsweep": [_P, C.POINTER(UpdateParams), _PD, _PD, _PL, _PD, _L, _PL],
 }
 _RESTYPE = {"tn_last_error": C.c_char_p, "tn_update_params_default": None}

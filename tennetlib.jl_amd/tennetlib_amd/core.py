@@ -212,6 +212,9 @@ class StateEnvs:
         for k, A in enumerate(psi, start=1):
             self.set_site(k, A)
         self.H = [np.array(W) for W in H]
+        # ProjMPO (one MPO, no penalty projectors): the only kind krylov_extend! takes
+        self.plain_mpo = Ms is None and not (isinstance(self.model, CouplingModel)
+                                             or (len(self.model) and not isinstance(self.model[0], np.ndarray)))
         if noise_skip is not None:
             sl = (C.c_int * (N + 1))(*noise_skip[0])
             sr = (C.c_int * (N + 1))(*noise_skip[1])
@@ -372,17 +375,21 @@ def _params(solver, kwargs) -> L.UpdateParams:
     return p
 
 
+def _update(sysenv: StateEnvs, p, r, pos: int, nsite_: int, ortho: str):
+    fn = sysenv.ctx.lib.tn_update_two_site if nsite_ == 2 else sysenv.ctx.lib.tn_update_one_site
+    sysenv.ctx.check(fn(sysenv.h, pos, int(ortho == "left"), C.byref(p), C.byref(r)))
+
+
 def update_position_(sysenv: StateEnvs, solver, pos: int, nsite_: int, ortho: str, **kwargs):
     """update_position!(sysenv, solver, pos, nsite, ortho; kwargs...)
     (src/mps/update_site.jl:231-277).  Returns (energy, truncerr)."""
-    if nsite_ != 2:
-        raise NotImplementedError("`update_position()` with nsite=1 is not built on the GPU path")
+    if nsite_ not in (1, 2):
+        raise ValueError(f"`update_position()` with `nsite={nsite_}` not implemented !!")
     if ortho not in ("left", "right"):
         raise ValueError("ortho must be \"left\" or \"right\"")
     p = _params(solver, kwargs)
     r = L.UpdateResult()
-    sysenv.ctx.check(sysenv.ctx.lib.tn_update_two_site(sysenv.h, pos, int(ortho == "left"), C.byref(p),
-                                                       C.byref(r)))
+    _update(sysenv, p, r, pos, nsite_, ortho)
     energy = r.energy if kwargs.get("ishermitian", True) else complex(r.energy, r.energy_im)
     return energy, r.truncerr, r
 
@@ -414,10 +421,17 @@ def _entropy(p: np.ndarray) -> float:
 def fullsweep_(sysenv: StateEnvs, solver, nsite_: int, swdata: SweepData, **kwargs):
     """fullsweep!(sysenv, solver, nsite, swdata; kwargs...) (src/mps/sweep.jl:94-196).
     Returns (dE, dS)."""
+    if nsite_ not in (1, 2):
+        raise ValueError(f"`fullsweep!()` with `nsite={nsite_}` not implemented !!")
+    return _sweep(sysenv, solver, swdata, lambda bond: nsite_, kwargs)
+
+
+def _sweep(sysenv: StateEnvs, solver, swdata: SweepData, pick, kwargs):
+    """Left and right half sweeps with the number of sites of every bond's
+    update given by pick(bond) (fullsweep!, src/mps/sweep.jl:94-196;
+    dynamic_fullsweep!, sweep.jl:257-376)."""
     outputlevel = kwargs.get("outputlevel", 1)
     noise = kwargs.get("noise", 0.0)
-    if nsite_ != 2:
-        raise NotImplementedError("`fullsweep!()` with nsite=1 is not built on the GPU path")
     if sysenv.orthocenter != 1:
         orthogonalize_(sysenv, 1)
     N = len(sysenv)
@@ -426,7 +440,6 @@ def fullsweep_(sysenv: StateEnvs, solver, nsite_: int, swdata: SweepData, **kwar
     swdata.sweepcount += 1
     lasteigs: List[np.ndarray] = [None] * (N - 1)
     p = _params(solver, kwargs)
-    ctx = sysenv.ctx
     cap = 1 << 16
     eigbuf = (C.c_double * cap)()
     t0 = time.perf_counter()
@@ -434,18 +447,28 @@ def fullsweep_(sysenv: StateEnvs, solver, nsite_: int, swdata: SweepData, **kwar
     r.eigs = C.cast(eigbuf, C.POINTER(C.c_double))
     r.eigs_cap = cap
     for bond in range(1, N):
-        ctx.check(ctx.lib.tn_update_two_site(sysenv.h, bond, 1, C.byref(p), C.byref(r)))
+        ns = pick(bond)
+        _update(sysenv, p, r, bond, ns, "left")
         energy = r.energy
         maxtruncerr = max(r.truncerr, maxtruncerr)
+        lasteigs[bond - 1] = np.array(eigbuf[:min(r.nkept, cap)])
         if outputlevel > 1:
             print(f"At left sweep {swdata.sweepcount} bond {bond} => Energy {energy}, Err {r.truncerr:.2g}")
+        if ns == 1 and bond == N - 1:
+            _update(sysenv, p, r, N, 1, "left")
+            energy = r.energy
+    swdata.lasteigs = list(lasteigs)
     for bond in range(N - 1, 0, -1):
-        ctx.check(ctx.lib.tn_update_two_site(sysenv.h, bond, 0, C.byref(p), C.byref(r)))
+        ns = pick(bond)
+        _update(sysenv, p, r, bond + 1 if ns == 1 else bond, ns, "right")
         energy = r.energy
         maxtruncerr = max(r.truncerr, maxtruncerr)
         lasteigs[bond - 1] = np.array(eigbuf[:min(r.nkept, cap)])
         if outputlevel > 1:
             print(f"At right sweep {swdata.sweepcount} bond {bond} => Energy {energy}, Err {r.truncerr:.2g}")
+        if ns == 1 and bond == 1:
+            _update(sysenv, p, r, 1, 1, "right")
+            energy = r.energy
     sw_time = time.perf_counter() - t0
     swdata.maxchi.append(max(sysenv.site_dims(k)[1] for k in range(1, N + 1)))
     swdata.energy.append(energy)
@@ -464,6 +487,51 @@ def fullsweep_(sysenv: StateEnvs, solver, nsite_: int, swdata: SweepData, **kwar
         print(f"At sweep {swdata.sweepcount} => dE={enerr}, dS={enterr}, MaxErr={maxtruncerr:.2g}, "
               f"Time={sw_time:.3f}")
     return enerr, enterr
+
+
+def krylov_extend_(sysenv: StateEnvs, **kwargs) -> None:
+    """krylov_extend!(sysenv; kwargs...) (src/mps/sweep.jl:378-452): global
+    subspace expansion with extension_krylovdim (3) Krylov vectors H^k psi
+    from MPO-MPS products (extension_applyH_cutoff 1e-15,
+    extension_applyH_maxdim = maxlinkdim + 2) and basis extension with
+    extension_cutoff (1e-7).  The StateEnvs must hold one MPO."""
+    if not sysenv.plain_mpo:
+        raise ValueError("krylov_extend!: the StateEnvs must be created by a single MPO")
+    kdim = int(kwargs.get("extension_krylovdim", 3))
+    hcut = float(kwargs.get("extension_applyH_cutoff", FLOAT64_THRESHOLD))
+    hmax = int(kwargs.get("extension_applyH_maxdim", 0))
+    ecut = float(kwargs.get("extension_cutoff", 1e-7))
+    t0 = time.perf_counter()
+    sysenv.ctx.check(sysenv.ctx.lib.tn_env_krylov_extend(sysenv.h, kdim, hmax, hcut, ecut))
+    if kwargs.get("outputlevel", 1) > 0:
+        N = len(sysenv)
+        print(f"Global Subspace Expansion: KrylovDim={kdim}, applyH Cutoff={hcut:.2g}, applyH MaxDim={hmax or 'maxlinkdim+2'}")
+        print(f"Global Subspace Expansion: Cutoff={ecut:.2g}, "
+              f"MaxLinkDim={max(sysenv.site_dims(k)[1] for k in range(1, N + 1))}, "
+              f"Time={time.perf_counter() - t0:.3f}")
+
+
+def dynamic_fullsweep_(sysenv: StateEnvs, solver, swdata: SweepData, eigthreshold: float = 1e-12,
+                       extendat: Optional[int] = None, **kwargs):
+    """dynamic_fullsweep!(sysenv, solver, swdata; eigthreshold, extendat,
+    kwargs...) (src/mps/sweep.jl:257-376): two-site updates on bonds whose
+    smallest kept weight of the last sweep is >= eigthreshold and whose
+    dimension is below maxdim, one-site updates elsewhere; the first sweep
+    and every extendat-th one expand the subspace (one-MPO StateEnvs, then a
+    one-site sweep) or run a two-site sweep (other projected operators)."""
+    maxdim = int(min(kwargs.get("maxdim", TYPEMAX_INT), TYPEMAX_INT))
+    if swdata.sweepcount == 0 or (extendat is not None and (swdata.sweepcount + 1) % extendat == 0):
+        if sysenv.plain_mpo:
+            krylov_extend_(sysenv, **kwargs)
+            return fullsweep_(sysenv, solver, 1, swdata, **kwargs)
+        return fullsweep_(sysenv, solver, 2, swdata, **kwargs)
+
+    def pick(bond):
+        ev = swdata.lasteigs[bond - 1]
+        small = len(ev) == 0 or ev[-1] < eigthreshold
+        return 1 if (small or sysenv.site_dims(bond)[1] >= maxdim) else 2
+
+    return _sweep(sysenv, solver, swdata, pick, kwargs)
 
 
 @dataclass
@@ -549,6 +617,11 @@ def dmrg2(psi0, H, *args, **kwargs):
     return dmrg(psi0, H, *args, 2, **kwargs)
 
 
+def dmrg1(psi0, H, *args, **kwargs):
+    """dmrg1 (src/mps/dmrg.jl:314-320): dmrg(..., 1)."""
+    return dmrg(psi0, H, *args, 1, **kwargs)
+
+
 # ---------------------------------------------------------------------------
 # TDVP
 # ---------------------------------------------------------------------------
@@ -562,12 +635,15 @@ class TDVPEngine:
         self.abstime = 0.0
 
 
-def tdvpsweep_(engine: TDVPEngine, time_step, nsite_=2, solver=exp_solver, **kwargs):
+def tdvpsweep_(engine: TDVPEngine, time_step, nsite_="dynamic", solver=exp_solver, **kwargs):
     """tdvpsweep!(engine, time_step; nsite, solver, kwargs...) (src/mps/tdvp.jl:247-277)."""
     if solver is not exp_solver:
         raise ValueError("`tdvpsweep!()`: `solver` must be `exp_solver` !!")
     if nsite_ == "dynamic":
-        raise NotImplementedError("nsite=\"dynamic\" (dynamic_fullsweep!) is not built on the GPU path")
+        dynamic_fullsweep_(engine.sysenv, exp_solver, engine.swdata, time_step=0.5 * time_step,
+                           reverse_step=True, **kwargs)
+        engine.abstime += abs(time_step)
+        return None
     if nsite_ not in (1, 2):
         raise ValueError("`tdvpsweep!()`: `nsite` must be `\"dynamic\"`, `2`, or `1` !!")
     if kwargs.get("extendat", None) is not None:
