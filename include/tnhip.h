@@ -251,6 +251,17 @@ int tn_ttn_set_coupling(tn_ttn* ttn, int node, int nnz, const int* c0, const int
 int tn_ttn_set_node(tn_ttn* ttn, int node, const long* dims3, const double* A_host);
 int tn_ttn_get_node(tn_ttn* ttn, int node, long* dims3, double* A_host /* nullable */);
 int tn_ttn_node_ptr(tn_ttn* ttn, int node, double** A_dev);
+/* Penalty projectors of StateEnvsTTN(psi, H, Ms; weight) (state_envs_ttn.jl:36-70,
+ * EnvCouplingModelProjTTN, environment.jl:42-110; LinkProjTTN, linkproj.jl):
+ * node products add weight * m <m, v> per projector TTN M, with m the
+ * projection of M on the node's local space through the overlap environments
+ * of its links.  set_projectors allocates nproj projectors (weight > 0, else
+ * TN_E_INVALID with the reference's message); set_projector_node uploads node
+ * `node` of projector j (0-based; same tree, any bond dimensions, FP64).  Set
+ * them before the environments are built: every tn_ttn_update_env also moves
+ * the projectors' overlap environments (move_linkproj_to_next!). */
+int tn_ttn_set_projectors(tn_ttn* ttn, int nproj, double weight);
+int tn_ttn_set_projector_node(tn_ttn* ttn, int j, int node, const long* dims, const double* A_host);
 /* move_linktensors_to_next!(env, psi, node, neighbour at `slot`) (src/ttn/linktensors.jl:63-118) */
 int tn_ttn_update_env(tn_ttn* ttn, int node, int slot);
 /* product(env, psi, v) at the orthogonality center `node` (src/ttn/linktensors.jl:183-221) */

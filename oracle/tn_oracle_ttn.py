@@ -458,9 +458,16 @@ class EnvsTTN:
     """Projected operator on a TTN (EnvCouplingModelTTN / LinkTensorsTTN,
     environment.jl:1-40, linktensors.jl:63-221)."""
 
-    def __init__(self, psi: TTNState, H: CouplingTerms):
+    def __init__(self, psi: TTNState, H: CouplingTerms, Ms: Optional[List["TTNState"]] = None,
+                 weight: float = 0.0):
         self.tree = psi.tree
         self.chan, self.W, site = build_tree_operator(psi.tree, H)
+        # penalty projectors (EnvCouplingModelProjTTN, environment.jl:42-110;
+        # LinkProjTTN, linkproj.jl): overlap environments O[(n, s)][x, y]
+        # of the far side of slot s (psi index x, M index y; sites: identity)
+        self.Ms = [M.T for M in (Ms or [])]
+        self.weight = weight
+        self.O: List[Dict[Tuple[Node, int], np.ndarray]] = [dict() for _ in self.Ms]
         self.F: Dict[Tuple[Node, int], np.ndarray] = {}
         for n in self.tree.edges:
             for s, it in enumerate(self.tree.slots[n]):
@@ -492,6 +499,23 @@ class EnvsTTN:
             else:
                 out[c0] += w * np.einsum("by,cz,kbc,lyz->kl", Fa, Fb, v, v, optimize=True)
         self.F[(m, t.slot_of(m, n))] = out
+        for M, O in zip(self.Ms, self.O):
+            # move_linkproj_to_next! (linkproj.jl:48-66): conj(M[n]) x psi[n] x the other links
+            T = v
+            for o in others:
+                if (n, o) in O:
+                    T = np.moveaxis(np.tensordot(T, O[(n, o)], axes=([o], [0])), -1, o)
+            Mn = M[n].conj()
+            O[(m, t.slot_of(m, n))] = np.tensordot(T, Mn, axes=(others, others))
+
+    def local_projection(self, n: Node, j: int) -> np.ndarray:
+        """m[x] = sum_y M[y] prod_s conj(O_s[x_s, y_s]) at node n (contract /
+        product of LinkProjTTN, linkproj.jl:126-165): P v = m <m, v>."""
+        T = self.Ms[j][n]
+        for s in range(3):
+            if (n, s) in self.O[j]:
+                T = np.moveaxis(np.tensordot(T, self.O[j][(n, s)].conj(), axes=([s], [1])), -1, s)
+        return T
 
     def move(self, psi: TTNState, src: Node, dst: Node, skip: Optional[Node] = None):
         """move_linktensors! (linktensors.jl:120-150)."""
@@ -507,15 +531,22 @@ class EnvsTTN:
         for c0, c1, c2, w in self.W[n]:
             out += w * np.einsum("ax,by,cz,abc->xyz", self.F[(n, 0)][c0], self.F[(n, 1)][c1],
                                  self.F[(n, 2)][c2], v, optimize=True)
+        for j in range(len(self.Ms)):
+            m = self.local_projection(n, j)
+            out = out + self.weight * np.vdot(m, v) * m
         return out
 
 
 class StateEnvsTTN:
-    """StateEnvsTTN(psi, H) (state_envs_ttn.jl:20-30): copies psi."""
+    """StateEnvsTTN(psi, H) (state_envs_ttn.jl:20-30) or StateEnvsTTN(psi, H,
+    Ms; weight) (state_envs_ttn.jl:36-70): copies psi."""
 
-    def __init__(self, psi: TTNState, H: CouplingTerms):
+    def __init__(self, psi: TTNState, H: CouplingTerms, Ms: Optional[List[TTNState]] = None,
+                 weight: Optional[float] = None):
+        if Ms is not None and (weight is None or weight <= 0.0):
+            raise ValueError(f"`weight` parameter should be > 0.0 (value passed was `weight={weight}`)")
         self.psi = psi.copy()
-        self.env = EnvsTTN(self.psi, H)
+        self.env = EnvsTTN(self.psi, H, Ms, weight or 0.0)
 
     def position(self, node: Node, skip: Optional[Node] = None, normalize=True, **kw):
         """position!(sysenv, node; ...) (state_envs_ttn.jl:120-140)."""

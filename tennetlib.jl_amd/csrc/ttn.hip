@@ -51,11 +51,27 @@ struct TTNNode {
   long size() const { return dims[0] * dims[1] * dims[2]; }
 };
 
+// penalty projector TTN M (LinkProjTTN, src/ttn/linkproj.jl): its node
+// tensors and, per node and slot, the overlap environment of the far side
+// E[x][y] (x: psi's leg, y: M's leg); physical slots are the identity
+struct TTNProjNode {
+  long dims[3] = {0, 0, 0};
+  DevBuf M;
+  DevBuf env[3];
+  long env_x[3] = {0, 0, 0}, env_y[3] = {0, 0, 0};
+  bool env_ok[3] = {false, false, false};
+  DevBuf m;            // local projection at this node (psi's node shape)
+  long m_epoch = -1;
+};
+
 struct tn_ttn {
   tn_ctx* ctx = nullptr;
   std::vector<TTNNode> nodes;
   int center = -1;                       // orthogonality center (-1: none)
   uint64_t rng = 0x853c49e6748fea9bULL;  // SplitMix64 state of the expansion padding
+  std::vector<std::vector<TTNProjNode>> proj;  // [projector][node]
+  double proj_weight = 0.0;
+  long epoch = 0;  // bumped whenever an overlap environment changes
 };
 
 namespace {
@@ -96,6 +112,8 @@ void require_env(const TTNNode& N, int s) {
 }
 
 void gemm64(tn_ctx* c, GemmDesc g) { gemm(c, Prec::F64, g); }
+
+const double* proj_local(tn_ttn* t, int n, size_t j);
 
 // S[a0'][a1'][c2][a2] = sum_W w E0[c0] E1[c1] v   (legs 0 and 1 applied, leg 2 open)
 void apply_legs01(tn_ttn* t, const TTNNode& N, const double* v, double* S) {
@@ -138,6 +156,15 @@ void matvec_node(tn_ttn* t, int n, const double* v, double* out) {
   g.B = N.slot[2].env.p; g.ldb = D2;
   g.C = out; g.ldc = D2;
   gemm64(t->ctx, g);
+  // product(EnvCouplingModelProjTTN) (environment.jl:104-110): + weight m <m, v>
+  if (!t->proj.empty()) {
+    double* co = (double*)t->ctx->workspace(WS_PROJ_C, 64 * sizeof(double));
+    for (size_t j = 0; j < t->proj.size(); ++j) {
+      const double* m = proj_local(t, n, j);
+      multi_dot(t->ctx, m, N.size(), 1, v, N.size(), false, co);
+      add_scaled_dev(t->ctx, out, m, co, t->proj_weight, N.size(), false);
+    }
+  }
 }
 
 // G[c2][(a0 a1)][a2'] = v[(a0 a1)][a2] E2[c2][a2][a2']
@@ -153,6 +180,127 @@ double* close_leg2(tn_ttn* t, const TTNNode& N, const double* v) {
   g.batch = (int)n2;
   gemm64(c, g);
   return G;
+}
+
+// T with leg `leg` mapped through E (ex x ey): to_y contracts T's leg (dim
+// ex) with E's first index, else T's leg (dim ey) with E's second index.
+void apply_leg(tn_ctx* c, const double* T, const long dims[3], int leg, const double* E, long ex, long ey, bool to_y,
+               double* out) {
+  const long kin = to_y ? ex : ey, kout = to_y ? ey : ex;
+  TN_REQUIRE(dims[leg] == kin, TN_E_SHAPE, "TTN projector: leg and overlap environment disagree");
+  GemmDesc g;
+  if (leg == 0) {  // out[o][x1 x2] = sum_k E'[k][o] T[k][x1 x2]
+    g.M = kout; g.N = dims[1] * dims[2]; g.K = kin;
+    g.A = E; g.lda = ey; g.tA = to_y ? 1 : 0;
+    g.B = T; g.ldb = dims[1] * dims[2];
+    g.C = out; g.ldc = dims[1] * dims[2];
+  } else if (leg == 1) {  // batched over x0
+    g.M = kout; g.N = dims[2]; g.K = kin;
+    g.A = E; g.lda = ey; g.tA = to_y ? 1 : 0; g.sA = 0;
+    g.B = T; g.ldb = dims[2]; g.sB = dims[1] * dims[2];
+    g.C = out; g.ldc = dims[2]; g.sC = kout * dims[2];
+    g.batch = (int)dims[0];
+  } else {  // out[x0 x1][o] = T[x0 x1][k] E'[k][o]
+    g.M = dims[0] * dims[1]; g.N = kout; g.K = kin;
+    g.A = T; g.lda = dims[2];
+    g.B = E; g.ldb = ey; g.tB = to_y ? 0 : 1;
+    g.C = out; g.ldc = kout;
+  }
+  gemm64(c, g);
+}
+
+// move_linkproj_to_next! (src/ttn/linkproj.jl:48-66) for every projector:
+// E[x_s][y_s] = sum psi_n[x] M_n[y] prod_{o != s} E_o[x_o][y_o], written into
+// the neighbour across slot s.
+void proj_update_env(tn_ttn* t, int n, int s) {
+  tn_ctx* c = t->ctx;
+  const TTNNode& N = t->nodes[n];
+  const int nb = N.slot[s].nbr, ts = N.slot[s].nbr_slot;
+  for (auto& P : t->proj) {
+    TTNProjNode& Q = P[n];
+    TN_REQUIRE(Q.M.p != nullptr, TN_E_INVALID, "TTN projector: node tensor not set");
+    long dims[3] = {N.dims[0], N.dims[1], N.dims[2]};
+    const double* T = N.A.as<double>();
+    int cur = 0;
+    for (int o = 0; o < 3; ++o) {
+      if (o == s) continue;
+      if (N.slot[o].nbr < 0) {
+        TN_REQUIRE(Q.dims[o] == N.dims[o], TN_E_SHAPE, "TTN projector: physical dimensions disagree");
+        continue;
+      }
+      TN_REQUIRE(Q.env_ok[o] && Q.env_x[o] == N.dims[o] && Q.env_y[o] == Q.dims[o], TN_E_POSITION,
+                 "TTN projector: overlap environment of slot " + std::to_string(o) + " is not built");
+      long nd[3] = {dims[0], dims[1], dims[2]};
+      nd[o] = Q.dims[o];
+      const size_t need = (size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double);
+      double* out = (double*)c->workspace(cur == 0 ? WS_TTNP_0 : WS_TTNP_1, need);
+      apply_leg(c, T, dims, o, Q.env[o].as<double>(), N.dims[o], Q.dims[o], true, out);
+      T = out;
+      dims[o] = nd[o];
+      cur ^= 1;
+    }
+    TTNProjNode& R = P[nb];
+    const long X = N.dims[s], Y = Q.dims[s];
+    R.env[ts].ensure((size_t)(X * Y) * sizeof(double));
+    double* E = R.env[ts].as<double>();
+    const double* Mt = Q.M.as<double>();
+    GemmDesc g;
+    if (s == 0) {
+      g.M = X; g.N = Y; g.K = Q.dims[1] * Q.dims[2];
+      g.A = T; g.lda = g.K;
+      g.B = Mt; g.ldb = g.K; g.tB = 1;
+    } else if (s == 2) {
+      g.M = X; g.N = Y; g.K = Q.dims[0] * Q.dims[1];
+      g.A = T; g.lda = X; g.tA = 1;
+      g.B = Mt; g.ldb = Y;
+    } else {
+      double* Ts = (double*)c->workspace(WS_TTNP_2, (size_t)(Q.dims[0] * X * Q.dims[2]) * sizeof(double));
+      double* Ms = (double*)c->workspace(WS_TTNP_3, (size_t)(Q.dims[0] * Y * Q.dims[2]) * sizeof(double));
+      swap01_f64(c, T, Q.dims[0], X, Q.dims[2], Ts);
+      swap01_f64(c, Mt, Q.dims[0], Y, Q.dims[2], Ms);
+      g.M = X; g.N = Y; g.K = Q.dims[0] * Q.dims[2];
+      g.A = Ts; g.lda = g.K;
+      g.B = Ms; g.ldb = g.K; g.tB = 1;
+    }
+    g.C = E; g.ldc = Y;
+    gemm64(c, g);
+    R.env_x[ts] = X;
+    R.env_y[ts] = Y;
+    R.env_ok[ts] = true;
+  }
+  ++t->epoch;
+}
+
+// local projection m[x] = sum_y M[y] prod_s E_s[x_s][y_s] at node n (contract /
+// product of LinkProjTTN, linkproj.jl:126-165), cached until an overlap
+// environment changes
+const double* proj_local(tn_ttn* t, int n, size_t j) {
+  tn_ctx* c = t->ctx;
+  const TTNNode& N = t->nodes[n];
+  TTNProjNode& Q = t->proj[j][n];
+  if (Q.m_epoch == t->epoch) return Q.m.as<double>();
+  long dims[3] = {Q.dims[0], Q.dims[1], Q.dims[2]};
+  const double* T = Q.M.as<double>();
+  int cur = 0;
+  for (int s = 0; s < 3; ++s) {
+    if (N.slot[s].nbr < 0) {
+      TN_REQUIRE(Q.dims[s] == N.dims[s], TN_E_SHAPE, "TTN projector: physical dimensions disagree");
+      continue;
+    }
+    TN_REQUIRE(Q.env_ok[s] && Q.env_x[s] == N.dims[s] && Q.env_y[s] == Q.dims[s], TN_E_POSITION,
+               "TTN projector: overlap environment of slot " + std::to_string(s) + " is not built");
+    long nd[3] = {dims[0], dims[1], dims[2]};
+    nd[s] = N.dims[s];
+    double* out = (double*)c->workspace(cur == 0 ? WS_TTNP_0 : WS_TTNP_1, (size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
+    apply_leg(c, T, dims, s, Q.env[s].as<double>(), N.dims[s], Q.dims[s], false, out);
+    T = out;
+    dims[s] = nd[s];
+    cur ^= 1;
+  }
+  Q.m.ensure((size_t)N.size() * sizeof(double));
+  copy_f64(c, Q.m.as<double>(), T, N.size());
+  Q.m_epoch = t->epoch;
+  return Q.m.as<double>();
 }
 
 // move_linktensors_to_next! (src/ttn/linktensors.jl:63-118): the environment
@@ -246,6 +394,7 @@ void update_env(tn_ttn* t, int n, int s) {
   }
   out.env_dim = Ds;
   out.env_ok = true;
+  if (!t->proj.empty()) proj_update_env(t, n, s);
 }
 
 // moveisometry_to_next! (src/ttn/ttn.jl:266-310): node n becomes an isometry
@@ -601,6 +750,38 @@ int tn_ttn_node_ptr(tn_ttn* t, int node, double** dev) {
     TTNNode& N = node_of(t, node);
     TN_REQUIRE(dev && N.has_tensor, TN_E_INVALID, "tn_ttn_node_ptr: node tensor not set");
     *dev = N.A.as<double>();
+  });
+}
+
+int tn_ttn_set_projectors(tn_ttn* t, int nproj, double weight) {
+  return guarded_t(CT(t), [&] {
+    TN_REQUIRE(t != nullptr && nproj >= 0, TN_E_INVALID, "set_projectors: bad arguments");
+    TN_REQUIRE(nproj == 0 || weight > 0.0, TN_E_INVALID,
+               "`weight` parameter should be > 0.0 (value passed was `weight=" + std::to_string(weight) + "`)");
+    t->proj.clear();
+    t->proj.resize(nproj);
+    for (auto& P : t->proj) P.resize(t->nodes.size());
+    t->proj_weight = weight;
+    ++t->epoch;
+  });
+}
+
+int tn_ttn_set_projector_node(tn_ttn* t, int j, int node, const long* dims, const double* A) {
+  return guarded_t(CT(t), [&] {
+    TN_REQUIRE(t != nullptr && j >= 0 && j < (int)t->proj.size(), TN_E_INVALID, "set_projector_node: bad projector");
+    node_of(t, node);
+    TN_REQUIRE(dims && A && dims[0] >= 1 && dims[1] >= 1 && dims[2] >= 1, TN_E_INVALID,
+               "set_projector_node: bad dimensions");
+    TTNProjNode& Q = t->proj[j][node];
+    for (int s = 0; s < 3; ++s) {
+      Q.dims[s] = dims[s];
+      Q.env_ok[s] = false;
+    }
+    const size_t n = (size_t)(dims[0] * dims[1] * dims[2]);
+    Q.M.ensure(n * sizeof(double));
+    TN_HIP(hipMemcpyAsync(Q.M.p, A, n * sizeof(double), hipMemcpyHostToDevice, t->ctx->stream));
+    TN_HIP(hipStreamSynchronize(t->ctx->stream));
+    ++t->epoch;
   });
 }
 

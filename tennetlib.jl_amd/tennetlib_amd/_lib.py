@@ -95,6 +95,8 @@ SIGNATURES = {
     "tn_env_set_projector_site": [_P, _I, _I, _P, _L, _L],
     "tn_env_set_noise_channels": [_P, _P, _P],
     "tn_env_krylov_extend": [_P, _I, _L, _D, _D],
+    "tn_ttn_set_projectors": [_P, _I, _D],
+    "tn_ttn_set_projector_node": [_P, _I, _I, C.POINTER(_L), _PD],
     "tn_ttn_create": [_P, _I, C.POINTER(_P)],
     "tn_ttn_destroy": [_P],
     "tn_ttn_set_link": [_P, _I, _I, _I, _I, _I],

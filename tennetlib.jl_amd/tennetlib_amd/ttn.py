@@ -388,9 +388,22 @@ def _tree_operator(tree: _Tree, H: CouplingModel):
 class StateEnvsTTN:
     """StateEnvsTTN(psi, H::CouplingModel) (state_envs_ttn.jl:20-30): copies
     psi to the GPU and builds the environments toward its orthogonality
-    center (LinkTensorsTTN(psi, M), linktensors.jl:240-262)."""
+    center (LinkTensorsTTN(psi, M), linktensors.jl:240-262).
+    StateEnvsTTN(psi, H, Ms; weight) (state_envs_ttn.jl:36-70) adds the
+    penalty projectors of the TTNs Ms (EnvCouplingModelProjTTN,
+    environment.jl:42-110), for excited states."""
 
-    def __init__(self, psi: TTN, H: CouplingModel, ctx: Optional[Context] = None):
+    def __init__(self, psi: TTN, H: CouplingModel, Ms: Optional[Sequence[TTN]] = None,
+                 weight: Optional[float] = None, ctx: Optional[Context] = None):
+        if Ms is not None:
+            if weight is None:
+                raise TypeError("StateEnvsTTN(psi, H, Ms; weight): keyword `weight` is required")
+            if weight <= 0.0:
+                raise ValueError(f"`weight` parameter should be > 0.0 (value passed was `weight={weight}`)")
+            for M in Ms:
+                if (M.tree.graph != psi.tree.graph or M.tree.sitenodes != psi.tree.sitenodes
+                        or M.tree.root != psi.tree.root):
+                    raise ValueError("StateEnvsTTN: the projector TTNs must share psi's graph")
         if H.N != psi.numsites():
             raise ValueError("StateEnvsTTN: the CouplingModel and the TTN have different sites")
         if psi.orthocenter is None:
@@ -409,6 +422,14 @@ class StateEnvsTTN:
                     F = np.ascontiguousarray(site_ops[it[1]])
                     self.ctx.check(lib.tn_ttn_set_site(self.h, t.index[n], s, 2, F.shape[0], F.ctypes.data_as(L._PD)))
         _upload(self.ctx, self.h, t, psi.tensors)
+        if Ms is not None:
+            self.ctx.check(lib.tn_ttn_set_projectors(self.h, len(Ms), float(weight)))
+            for j, M in enumerate(Ms):
+                for n in t.nodes:
+                    A = np.ascontiguousarray(M.tensors[n], dtype=np.float64)
+                    dims = (C.c_long * 3)(*A.shape)
+                    self.ctx.check(lib.tn_ttn_set_projector_node(self.h, j, t.index[n], dims,
+                                                                 A.ctypes.data_as(L._PD)))
         for n in t.nodes:
             w = W[n]
             c0 = (C.c_int * len(w))(*[x[0] for x in w])
@@ -654,9 +675,16 @@ def optimize_(sysenv: StateEnvsTTN, params: OptimizeParamsTTN, sweeppath: Sequen
     return sw
 
 
-def optimize(psi0: TTN, H: CouplingModel, params: OptimizeParamsTTN, sweeppath: Sequence[Node], **kwargs):
-    """optimize(psi0, H, params, sweeppath; kwargs...) (optimize_ttn.jl:225-260) -> (energy, psi)."""
-    sysenv = StateEnvsTTN(psi0, H)
+def optimize(psi0: TTN, H: CouplingModel, *args, **kwargs):
+    """optimize(psi0, H, params, sweeppath; kwargs...) (optimize_ttn.jl:225-260)
+    or, for excited states, optimize(psi0, H, Ms, params, sweeppath; weight,
+    kwargs...) (optimize_ttn.jl:284-300) -> (energy, psi)."""
+    if args and isinstance(args[0], OptimizeParamsTTN):
+        Ms, (params, sweeppath) = None, args[:2]
+        sysenv = StateEnvsTTN(psi0, H)
+    else:
+        Ms, params, sweeppath = args[:3]
+        sysenv = StateEnvsTTN(psi0, H, Ms, weight=kwargs.pop("weight", -1.0))
     try:
         sw = optimize_(sysenv, params, sweeppath, **kwargs)
         return sw.energy[-1], sysenv.psi

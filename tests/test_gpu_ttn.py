@@ -200,3 +200,67 @@ def test_ttn_errors(T):
     with pytest.raises(ValueError, match="expand_numiter"):
         T.fullsweep_(env, T.default_sweeppath(psi0), T.eig_solver, T.SweepDataTTN(), noise=1e-2, expand_numiter=1)
     env.close()
+
+
+def test_ttn_projector_product_matches_oracle(T):
+    """Node products of StateEnvsTTN(psi, H, Ms; weight) (LinkProjTTN) at every
+    node, two projector TTNs of other bond dimensions."""
+    N, chi = 12, 8
+    rng = np.random.default_rng(41)
+    H, OH = _model(T, N, rng.uniform(-1, 1, N))
+    otree = TO.Tree.build(N)
+    oMs = [TO.random_ttn(otree, chi + 2 + k, seed=50 + k) for k in range(2)]
+    opsi = TO.random_ttn(otree, chi, seed=42)
+    oenv = TO.StateEnvsTTN(opsi, OH, oMs, weight=2.5)
+    for node in TO.default_sweeppath(N, otree.edges):
+        oenv.position(node)
+        print(f"ttn projector product node={node}", flush=True)
+        env = T.StateEnvsTTN(_gpu_ttn(T, oenv.psi), H, [_gpu_ttn(T, M) for M in oMs], weight=2.5)
+        v = rng.standard_normal(oenv.psi.T[node].shape)
+        want = oenv.env.product(node, v)
+        got = T.product(env, v)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), node
+        env.close()
+
+
+def test_ttn_projector_optimize_matches_oracle(T):
+    N = 12
+    H, OH = _model(T, N)
+    otree = TO.Tree.build(N)
+    path = TO.default_sweeppath(N, otree.edges)
+    ogs = TO.StateEnvsTTN(TO.random_ttn(otree, 16, seed=3), OH)
+    TO.optimize(ogs, TO.OptimizeParamsTTN.make([16], [3]), path)
+    opsi = TO.random_ttn(otree, 16, seed=4)
+    oenv = TO.StateEnvsTTN(opsi, OH, [ogs.psi], weight=10.0)
+    sw_ref = TO.optimize(oenv, TO.OptimizeParamsTTN.make([16], [3]), path)
+    env = T.StateEnvsTTN(_gpu_ttn(T, opsi), H, [_gpu_ttn(T, ogs.psi)], weight=10.0)
+    sw = T.optimize_(env, T.make_optimize_params(maxdim=[16], nsweeps=[3]), T.default_sweeppath(env.psi),
+                     outputlevel=0)
+    print("ttn projector optimize", sw.energy, sw_ref.energy, flush=True)
+    assert sw.maxchi == sw_ref.maxchi
+    assert np.allclose(sw.energy, sw_ref.energy, rtol=0, atol=1e-10)
+    env.close()
+
+
+def test_ttn_excited_state_reference_settings(T):
+    """test_TTN.jl:40-64 (dense): ground state from the reference settings,
+    then optimize(psi0, H, [psi_gr], params, sweeppath; weight = 10) with
+    maxdim 128, 10 sweeps, noise 1e-2 (decay 5, off after 6): the lowest
+    triplet E1 = -4.86114793703639 (ED), orthogonal to the ground state."""
+    N = 12
+    H, _ = _model(T, N)
+    psi0 = T.default_randomTTN(N, 12, seed=1)
+    p = T.make_optimize_params(maxdim=[128], nsweeps=[5], cutoff=1e-14, noise=1e-2, noisedecay=5,
+                               disable_noise_after=2)
+    e0, gs = T.optimize(psi0, H, p, T.default_sweeppath(psi0), outputlevel=0)
+    p1 = T.make_optimize_params(maxdim=[128], nsweeps=[10], cutoff=1e-14, noise=1e-2, noisedecay=5,
+                                disable_noise_after=6)
+    e1, ex = T.optimize(psi0, H, [gs], p1, T.default_sweeppath(psi0), weight=10.0, outputlevel=0)
+    print("ttn excited", e0, e1, flush=True)
+    assert abs(e0 - E0_N12) < 1e-9
+    assert abs(e1 - (-4.86114793703639)) < 1e-8
+    otree = TO.Tree.build(N)
+    ov = ttn_to_dense(_as_oracle(gs, otree)) @ ttn_to_dense(_as_oracle(ex, otree))
+    assert abs(ov) < 1e-5
+    with pytest.raises(ValueError, match="should be > 0.0"):
+        T.StateEnvsTTN(psi0, H, [gs], weight=0.0)

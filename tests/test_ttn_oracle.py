@@ -155,3 +155,53 @@ def test_random_field_instance_matches_ed():
     env = TO.StateEnvsTTN(TO.random_ttn(t, 128, seed=4), H)
     sw = TO.optimize(env, TO.OptimizeParamsTTN.make([128], [4]), TO.default_sweeppath(N, t.edges))
     assert abs(sw.energy[-1] - e0) < 1e-9
+
+
+@pytest.mark.parametrize("N,chi", [(7, 4), (12, 6)])
+def test_projector_operator_is_projected_dense_operator(N, chi):
+    """H + w sum_k |M_k><M_k| (LinkProjTTN, linkproj.jl) projected on the
+    tree at every node equals the node product with projectors; the M_k are
+    random TTNs of other bond dimensions on the same tree."""
+    rng = np.random.default_rng(100 + N)
+    H = TO.heisenberg_terms(N, 1.0, rng.uniform(-1, 1, N))
+    t = TO.Tree.build(N)
+    Ms = [TO.random_ttn(t, chi + 1 + k, seed=200 + k) for k in range(2)]
+    w = 3.5
+    Hd = dense_h(H).toarray()
+    for M in Ms:
+        m = ttn_to_dense(M)
+        Hd = Hd + w * np.outer(m, m) / (m @ m)
+    Ms = [M.copy() for M in Ms]
+    for M in Ms:  # unit norm, as optimised states are
+        nrm = np.sqrt(ttn_to_dense(M) @ ttn_to_dense(M))
+        M.T[M.center] = M.T[M.center] / nrm
+    env = TO.StateEnvsTTN(TO.random_ttn(t, chi, seed=N), H, Ms, weight=w)
+    for node in TO.default_sweeppath(N, t.edges):
+        env.position(node)
+        A = env.psi.T[node]
+        x = rng.standard_normal(A.shape)
+        y = rng.standard_normal(A.shape)
+        want = ttn_to_dense(env.psi, node, x) @ (Hd @ ttn_to_dense(env.psi, node, y))
+        got = np.vdot(x, env.env.product(node, y))
+        assert abs(got - want) < 1e-11 * max(1.0, abs(want)), node
+    with pytest.raises(ValueError, match="should be > 0.0"):
+        TO.StateEnvsTTN(env.psi, H, Ms, weight=0.0)
+
+
+def test_excited_state_optimize_reference_settings_reaches_first_excited_level():
+    """test_TTN.jl:40-64 without quantum numbers: the ground state from the
+    reference settings, then optimize(psi0, H, [psi_gr], params; weight = 10)
+    with maxdim 128, 10 sweeps, noise 1e-2 (decay 5, off after 6): the lowest
+    triplet, E1 = -4.86114793703639 (ED), orthogonal to the ground state."""
+    N = 12
+    H = TO.heisenberg_terms(N, 1.0)
+    t = TO.Tree.build(N)
+    path = TO.default_sweeppath(N, t.edges)
+    gs = TO.StateEnvsTTN(TO.random_ttn(t, 12, seed=1), H)
+    TO.optimize(gs, TO.OptimizeParamsTTN.make([128], [5], cutoff=1e-14, noise=1e-2, noisedecay=5,
+                                              disable_noise_after=2), path)
+    ex = TO.StateEnvsTTN(TO.random_ttn(t, 12, seed=2), H, [gs.psi], weight=10.0)
+    sw = TO.optimize(ex, TO.OptimizeParamsTTN.make([128], [10], cutoff=1e-14, noise=1e-2, noisedecay=5,
+                                                   disable_noise_after=6), path)
+    assert abs(sw.energy[-1] - (-4.86114793703639)) < 1e-8
+    assert abs(ttn_to_dense(gs.psi) @ ttn_to_dense(ex.psi)) < 1e-5
