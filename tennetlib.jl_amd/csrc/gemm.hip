@@ -537,6 +537,90 @@ GArgs to_args(const GemmDesc& g) {
   return a;
 }
 
+
+// ------------------------------------------------------------------------
+// Grouped FP64 GEMM: output group g (grid y) and batch z (grid z) accumulate
+// sum over the terms t of group g of w_t A_{ia_t} B_{ib_t}, every term an
+// M x N x K product; the output tile is written once (no read of C).
+// ------------------------------------------------------------------------
+struct GroupedArgs {
+  long M, N, K;
+  const double* A; long lda, sA, tsA;  // A_t = A + ia_t tsA + z sA
+  const double* B; long ldb, sB, tsB;  // B_t = B + ib_t tsB + z sB
+  double* C; long ldc, sC, gsC;        // C_g = C + g gsC + z sC
+  const int* off; const int* ia; const int* ib; const double* w;
+};
+
+template <int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(GroupedArgs ga) {
+  using T = double;
+  using TT = RealTile<T, BM, BN, BK>;
+  constexpr int LA = BM + TT::PAD, LB = BN + TT::PAD;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  __shared__ T As[BK * LA];
+  __shared__ T Bs[BK * LB];
+  const int ntm = (int)((ga.M + BM - 1) / BM), ntn = (int)((ga.N + BN - 1) / BN);
+  int tm, tn;
+  tile_coords(ntm, ntn, tm, tn);
+  const long m0 = (long)tm * BM, n0 = (long)tn * BN;
+  const int g = blockIdx.y;
+  const long z = blockIdx.z;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  GArgs a;
+  a.M = ga.M; a.N = ga.N; a.K = ga.K;
+  a.lda = ga.lda; a.tA = 0; a.cA = 0;
+  a.ldb = ga.ldb; a.tB = 0; a.cB = 0;
+  d4 acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = d4{0, 0, 0, 0};
+  const int t1 = ga.off[g + 1];
+  for (int t = ga.off[g]; t < t1; ++t) {
+    const T* A = ga.A + (long)ga.ia[t] * ga.tsA + z * ga.sA;
+    const T* B = ga.B + (long)ga.ib[t] * ga.tsB + z * ga.sB;
+    const double wt = ga.w[t];
+    T ra[TT::AE], rb[TT::BE];
+    load_tiles<T, BM, BN, BK>(a, A, B, m0, n0, 0, ra, rb);
+    for (long k0 = 0; k0 < a.K; k0 += BK) {
+#pragma unroll
+      for (int i = 0; i < TT::AE; ++i) ra[i] *= wt;
+      __syncthreads();
+      store_tiles<T, BM, BN, BK>(a, As, Bs, ra, rb);
+      __syncthreads();
+      if (k0 + BK < a.K) load_tiles<T, BM, BN, BK>(a, A, B, m0, n0, k0 + BK, ra, rb);
+#pragma unroll
+      for (int kk = 0; kk < BK; kk += 4) {
+        const int kr = kk + (lane >> 4);
+        double fa[FM], fb[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) fa[i] = As[kr * LA + wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) fb[j] = Bs[kr * LB + wn * WTN + j * 16 + (lane & 15)];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+  T* C = ga.C + (long)g * ga.gsC + z * ga.sC;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const long m = m0 + wm * WTM + i * 16 + (lane >> 4) + 4 * r;
+        const long n = n0 + wn * WTN + j * 16 + (lane & 15);
+        if (m < a.M && n < a.N) C[m * ga.ldc + n] = acc[i][j][r];
+      }
+}
+
 }  // namespace
 
 namespace {
@@ -654,6 +738,23 @@ static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g) {
       break;
     }
   }
+  TN_HIP(hipGetLastError());
+}
+
+
+void gemm_grouped_f64(tn_ctx* c, const GroupedDesc& d) {
+  if (d.M <= 0 || d.N <= 0 || d.ngroups <= 0 || d.batch <= 0) return;
+  TN_REQUIRE(d.ngroups <= 65535 && d.batch <= 65535, TN_E_INVALID, "grouped gemm: grid too large");
+  GroupedArgs ga;
+  ga.M = d.M; ga.N = d.N; ga.K = d.K;
+  ga.A = d.A; ga.lda = d.lda; ga.sA = d.sA; ga.tsA = d.tsA;
+  ga.B = d.B; ga.ldb = d.ldb; ga.sB = d.sB; ga.tsB = d.tsB;
+  ga.C = d.C; ga.ldc = d.ldc; ga.sC = d.sC; ga.gsC = d.gsC;
+  ga.off = d.off; ga.ia = d.ia; ga.ib = d.ib; ga.w = d.w;
+  const long tiles = ((d.M + 63) / 64) * ((d.N + 63) / 64);
+  TN_REQUIRE(tiles < (1L << 31), TN_E_INVALID, "grouped gemm: too many tiles");
+  dim3 gr((unsigned)tiles, (unsigned)d.ngroups, (unsigned)d.batch);
+  hipLaunchKernelGGL((gemm_f64_grouped_kernel<64, 64, 16>), gr, dim3(256), 0, c->stream, ga);
   TN_HIP(hipGetLastError());
 }
 

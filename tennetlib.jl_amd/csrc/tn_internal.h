@@ -167,6 +167,20 @@ struct GemmDesc {
   double alpha = 1.0, beta = 0.0;
 };
 void gemm(tn_ctx* c, Prec p, const GemmDesc& g);
+// FP64 grouped GEMM: for every output group g < ngroups and batch z < batch,
+// C_g[z] (M x N, row stride ldc) = sum over terms t in [off[g], off[g+1]) of
+// w[t] * A_{ia[t]}[z] (M x K, lda) * B_{ib[t]}[z] (K x N, ldb), with
+// A_t = A + ia[t] tsA + z sA, B_t = B + ib[t] tsB + z sB, C_g = C + g gsC + z sC
+// (device term arrays); a group without terms is written as zeros.
+struct GroupedDesc {
+  long M = 0, N = 0, K = 0;
+  const double* A = nullptr; long lda = 0, sA = 0, tsA = 0;
+  const double* B = nullptr; long ldb = 0, sB = 0, tsB = 0;
+  double* C = nullptr; long ldc = 0, sC = 0, gsC = 0;
+  const int* off = nullptr; const int* ia = nullptr; const int* ib = nullptr; const double* w = nullptr;
+  int ngroups = 0, batch = 1;
+};
+void gemm_grouped_f64(tn_ctx* c, const GroupedDesc& d);
 // double-buffered FP64/FP32 kernels (gemm_fast.hip); gemm() uses them when
 // gemm_fast_ok() holds (shapes >= one tile, 16-byte aligned operand rows)
 bool gemm_fast_ok(Prec p, const GemmDesc& g);

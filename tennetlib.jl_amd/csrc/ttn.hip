@@ -27,6 +27,8 @@
 #include <string>
 #include <vector>
 
+#include <cstring>
+
 #include "tn_internal.h"
 
 using namespace tn;
@@ -40,6 +42,18 @@ struct TTNSlot {
   bool env_ok = false;
 };
 
+// the node's coupling terms sorted by one output channel for the grouped
+// GEMMs: q = 0 by c2 (A index c1, B index c0), q = 1 by c0 (c1, c2),
+// q = 2 by c1 (c0, c2); device arrays off (ng + 1), ia, ib, w
+struct TermGroups {
+  DevBuf buf;
+  int ng = 0;
+  const int* off = nullptr;
+  const int* ia = nullptr;
+  const int* ib = nullptr;
+  const double* w = nullptr;
+};
+
 struct TTNNode {
   long dims[3] = {0, 0, 0};
   DevBuf A;
@@ -48,6 +62,8 @@ struct TTNNode {
   std::vector<int> c[3];
   std::vector<double> w;
   bool has_tensor = false;
+  TermGroups tg[3];
+  bool tg_ok = false;
   long size() const { return dims[0] * dims[1] * dims[2]; }
 };
 
@@ -115,6 +131,48 @@ void gemm64(tn_ctx* c, GemmDesc g) { gemm(c, Prec::F64, g); }
 
 const double* proj_local(tn_ttn* t, int n, size_t j);
 
+void build_groups(tn_ttn* t, TTNNode& N) {
+  if (N.tg_ok) return;
+  const int nt = (int)N.w.size();
+  const int gkey[3] = {2, 0, 1}, akey[3] = {1, 1, 0}, bkey[3] = {0, 2, 2};
+  for (int q = 0; q < 3; ++q) {
+    const int ng = N.slot[gkey[q]].nchan;
+    std::vector<int> off(ng + 1, 0), ia(nt), ib(nt);
+    std::vector<double> w(nt);
+    for (int k = 0; k < nt; ++k) {
+      for (int x = 0; x < 3; ++x)
+        TN_REQUIRE(N.c[x][k] >= 0 && N.c[x][k] < N.slot[x].nchan, TN_E_INVALID, "TTN: coupling channel out of range");
+      ++off[N.c[gkey[q]][k] + 1];
+    }
+    for (int g = 0; g < ng; ++g) off[g + 1] += off[g];
+    std::vector<int> pos(off.begin(), off.end() - 1);
+    for (int k = 0; k < nt; ++k) {
+      const int at = pos[N.c[gkey[q]][k]]++;
+      ia[at] = N.c[akey[q]][k];
+      ib[at] = N.c[bkey[q]][k];
+      w[at] = N.w[k];
+    }
+    const size_t ni = (size_t)(ng + 1 + 2 * nt);
+    const size_t wofs = ((ni * sizeof(int) + 7) / 8) * 8;
+    std::vector<char> host(wofs + (size_t)nt * sizeof(double) + 8);
+    std::memcpy(host.data(), off.data(), (ng + 1) * sizeof(int));
+    std::memcpy(host.data() + (ng + 1) * sizeof(int), ia.data(), nt * sizeof(int));
+    std::memcpy(host.data() + (ng + 1 + nt) * sizeof(int), ib.data(), nt * sizeof(int));
+    std::memcpy(host.data() + wofs, w.data(), nt * sizeof(double));
+    TermGroups& G = N.tg[q];
+    G.buf.ensure(host.size());
+    TN_HIP(hipMemcpyAsync(G.buf.p, host.data(), host.size(), hipMemcpyHostToDevice, t->ctx->stream));
+    TN_HIP(hipStreamSynchronize(t->ctx->stream));
+    const char* base = (const char*)G.buf.p;
+    G.ng = ng;
+    G.off = (const int*)base;
+    G.ia = G.off + ng + 1;
+    G.ib = G.ia + nt;
+    G.w = (const double*)(base + wofs);
+  }
+  N.tg_ok = true;
+}
+
 // S[a0'][a1'][c2][a2] = sum_W w E0[c0] E1[c1] v   (legs 0 and 1 applied, leg 2 open)
 void apply_legs01(tn_ttn* t, const TTNNode& N, const double* v, double* S) {
   tn_ctx* c = t->ctx;
@@ -129,18 +187,18 @@ void apply_legs01(tn_ttn* t, const TTNNode& N, const double* v, double* S) {
     g.C = A; g.ldc = D1 * D2;
     gemm64(c, g);
   }
-  fill_f64(c, S, 0.0, D0 * D1 * n2 * D2);
-  const double* E1 = (const double*)N.slot[1].env.p;
-  for (size_t k = 0; k < N.w.size(); ++k) {
-    GemmDesc g;
-    g.M = D1; g.N = D2; g.K = D1;
-    g.A = E1 + (long)N.c[1][k] * D1 * D1; g.lda = D1; g.sA = 0;
-    g.B = A + (long)N.c[0][k] * D0 * D1 * D2; g.ldb = D2; g.sB = D1 * D2;
-    g.C = S + (long)N.c[2][k] * D2; g.ldc = n2 * D2; g.sC = D1 * n2 * D2;
-    g.batch = (int)D0;
-    g.alpha = N.w[k]; g.beta = 1.0;
-    gemm64(c, g);
-  }
+  // S[a0'][a1'][c2][a2] = sum_{terms of c2} w E1[c1] A[c0][a0']   (one grouped launch)
+  build_groups(t, const_cast<TTNNode&>(N));
+  const TermGroups& G = N.tg[0];
+  GroupedDesc d;
+  d.M = D1; d.N = D2; d.K = D1;
+  d.A = (const double*)N.slot[1].env.p; d.lda = D1; d.sA = 0; d.tsA = D1 * D1;
+  d.B = A; d.ldb = D2; d.sB = D1 * D2; d.tsB = D0 * D1 * D2;
+  d.C = S; d.ldc = n2 * D2; d.sC = D1 * n2 * D2; d.gsC = D2;
+  d.off = G.off; d.ia = G.ia; d.ib = G.ib; d.w = G.w;
+  d.ngroups = G.ng; d.batch = (int)D0;
+  TN_REQUIRE(G.ng == n2, TN_E_SHAPE, "TTN: channel count of slot 2 changed");
+  gemm_grouped_f64(c, d);
 }
 
 void matvec_node(tn_ttn* t, int n, const double* v, double* out) {
@@ -348,20 +406,19 @@ void update_env(tn_ttn* t, int n, int s) {
     const long n2 = N.slot[2].nchan;
     (void)n2;
     double* K = (double*)c->workspace(WS_TTN_A, (size_t)(ns * D0 * D1 * D2) * sizeof(double));
-    fill_f64(c, K, 0.0, ns * D0 * D1 * D2);
+    build_groups(t, N);
     if (s == 0) {
-      // K[c0][a0][a1'][a2'] += w E1[c1][a1'][a1] G[c2][a0][a1][a2']   (batched over a0)
-      const double* E1 = (const double*)N.slot[1].env.p;
-      for (size_t k = 0; k < N.w.size(); ++k) {
-        GemmDesc g;
-        g.M = D1; g.N = D2; g.K = D1;
-        g.A = E1 + (long)N.c[1][k] * D1 * D1; g.lda = D1; g.sA = 0;
-        g.B = G + (long)N.c[2][k] * D0 * D1 * D2; g.ldb = D2; g.sB = D1 * D2;
-        g.C = K + (long)N.c[0][k] * D0 * D1 * D2; g.ldc = D2; g.sC = D1 * D2;
-        g.batch = (int)D0;
-        g.alpha = N.w[k]; g.beta = 1.0;
-        gemm64(c, g);
-      }
+      // K[c0][a0][a1'][a2'] = sum_{terms of c0} w E1[c1][a1'][a1] G[c2][a0][a1][a2']   (batched over a0)
+      const TermGroups& TG = N.tg[1];
+      TN_REQUIRE(TG.ng == ns, TN_E_SHAPE, "TTN: channel count of slot 0 changed");
+      GroupedDesc d;
+      d.M = D1; d.N = D2; d.K = D1;
+      d.A = (const double*)N.slot[1].env.p; d.lda = D1; d.sA = 0; d.tsA = D1 * D1;
+      d.B = G; d.ldb = D2; d.sB = D1 * D2; d.tsB = D0 * D1 * D2;
+      d.C = K; d.ldc = D2; d.sC = D1 * D2; d.gsC = D0 * D1 * D2;
+      d.off = TG.off; d.ia = TG.ia; d.ib = TG.ib; d.w = TG.w;
+      d.ngroups = TG.ng; d.batch = (int)D0;
+      gemm_grouped_f64(c, d);
       // F[c0][a0][a0'] = K[(c0 a0)][(a1' a2')] v[a0'][(a1' a2')]^T
       GemmDesc g;
       g.M = ns * D0; g.N = D0; g.K = D1 * D2;
@@ -370,18 +427,17 @@ void update_env(tn_ttn* t, int n, int s) {
       g.C = F; g.ldc = D0;
       gemm64(c, g);
     } else {
-      // K[c1][a1][a0'][a2'] += w E0[c0][a0'][a0] G[c2][a0][a1][a2']   (batched over a1)
-      const double* E0 = (const double*)N.slot[0].env.p;
-      for (size_t k = 0; k < N.w.size(); ++k) {
-        GemmDesc g;
-        g.M = D0; g.N = D2; g.K = D0;
-        g.A = E0 + (long)N.c[0][k] * D0 * D0; g.lda = D0; g.sA = 0;
-        g.B = G + (long)N.c[2][k] * D0 * D1 * D2; g.ldb = D1 * D2; g.sB = D2;
-        g.C = K + (long)N.c[1][k] * D1 * D0 * D2; g.ldc = D2; g.sC = D0 * D2;
-        g.batch = (int)D1;
-        g.alpha = N.w[k]; g.beta = 1.0;
-        gemm64(c, g);
-      }
+      // K[c1][a1][a0'][a2'] = sum_{terms of c1} w E0[c0][a0'][a0] G[c2][a0][a1][a2']   (batched over a1)
+      const TermGroups& TG = N.tg[2];
+      TN_REQUIRE(TG.ng == ns, TN_E_SHAPE, "TTN: channel count of slot 1 changed");
+      GroupedDesc d;
+      d.M = D0; d.N = D2; d.K = D0;
+      d.A = (const double*)N.slot[0].env.p; d.lda = D0; d.sA = 0; d.tsA = D0 * D0;
+      d.B = G; d.ldb = D1 * D2; d.sB = D2; d.tsB = D0 * D1 * D2;
+      d.C = K; d.ldc = D2; d.sC = D0 * D2; d.gsC = D1 * D0 * D2;
+      d.off = TG.off; d.ia = TG.ia; d.ib = TG.ib; d.w = TG.w;
+      d.ngroups = TG.ng; d.batch = (int)D1;
+      gemm_grouped_f64(c, d);
       double* vp = (double*)c->workspace(WS_TTN_P, (size_t)(D0 * D1 * D2) * sizeof(double));
       swap01_f64(c, v, D0, D1, D2, vp);  // vp[a1'][a0'][a2']
       GemmDesc g;
@@ -677,10 +733,12 @@ int tn_ttn_set_link(tn_ttn* t, int a, int sa, int b, int sb, int nchan) {
     A.slot[sa].nbr = b;
     A.slot[sa].nbr_slot = sb;
     A.slot[sa].nchan = nchan;
+    A.tg_ok = false;
     A.slot[sa].env_ok = false;
     B.slot[sb].nbr = a;
     B.slot[sb].nbr_slot = sa;
     B.slot[sb].nchan = nchan;
+    B.tg_ok = false;
     B.slot[sb].env_ok = false;
   });
 }
@@ -694,6 +752,7 @@ int tn_ttn_set_site(tn_ttn* t, int node, int slot, int d, int nchan, const doubl
     S.nbr = -1;
     S.nbr_slot = -1;
     S.nchan = nchan;
+    N.tg_ok = false;
     const size_t bytes = (size_t)nchan * d * d * sizeof(double);
     S.env.ensure(bytes);
     TN_HIP(hipMemcpyAsync(S.env.p, ops, bytes, hipMemcpyHostToDevice, t->ctx->stream));
@@ -709,6 +768,7 @@ int tn_ttn_set_coupling(tn_ttn* t, int node, int nnz, const int* c0, const int* 
     TN_REQUIRE(nnz >= 1 && c0 && c1 && c2 && w, TN_E_INVALID, "tn_ttn_set_coupling: empty coupling");
     for (int s = 0; s < 3; ++s) N.c[s].assign(s == 0 ? c0 : s == 1 ? c1 : c2, (s == 0 ? c0 : s == 1 ? c1 : c2) + nnz);
     N.w.assign(w, w + nnz);
+    N.tg_ok = false;
     for (int k = 0; k < nnz; ++k)
       for (int s = 0; s < 3; ++s)
         TN_REQUIRE(N.c[s][k] >= 0 && N.c[s][k] < N.slot[s].nchan, TN_E_SHAPE,
