@@ -698,12 +698,16 @@ void gemm(tn_ctx* c, Prec p, const GemmDesc& g) {
 }
 
 static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g) {
-  if (gemm_fast_ok(p, g)) {
+  // fewer 128 x 128 output tiles than CUs: 64 x 64 tiles spread the work
+  // over four times as many workgroups (mid-size bonds are latency-bound)
+  const long t128 = ((g.M + 127) / 128) * ((g.N + 127) / 128) * (long)g.batch;
+  const bool few_tiles = t128 < 240;
+  if (!few_tiles && gemm_fast_ok(p, g)) {
     gemm_fast(c, p, g);
     return;
   }
   GArgs a = to_args(g);
-  const bool small = (g.M * g.N <= 256L * 256L);
+  const bool small = (g.M * g.N <= 256L * 256L) || few_tiles;
   switch (p) {
     case Prec::F64:
       if (small) {
