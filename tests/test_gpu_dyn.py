@@ -202,3 +202,27 @@ def test_dynamic_tdvp_reference_settings(T, kind):
     got = O.mps_to_dense(eng.sysenv.psi).reshape(-1)
     assert abs(abs(np.vdot(got, vt)) - 1.0) < 1e-8
     assert abs(eng.abstime - 0.1) < 1e-12
+
+
+def test_zero_site_product_with_projectors_matches_oracle(T):
+    """H + w sum_k |M_k><M_k| on the bond matrix (nsite = 0) of a complex
+    MPS: the projectors' local factor is conj(L_M) conj(R_M)^T."""
+    N, w = 6, 2.5
+    Ws = O.xxz_mpo(N, 1.0, 0.7, np.linspace(-0.4, 0.6, N))
+    A = rand_mps(N, [2, 4, 8, 4, 2], seed=181, cplx=True)
+    Ms = [rand_mps(N, [2, 3, 5, 3, 2], seed=182 + k, cplx=True) for k in range(2)]
+    env = T.StateEnvs(A, Ws, Ms, weight=w)
+    ref = O.Envs(Ws, dtype=np.complex128, Ms=Ms, weight=w)
+    T.set_nsite_(env, 0)
+    ref.nsite = 0
+    rng = np.random.default_rng(183)
+    for p in (2, 4, 6, 3):
+        print(f"zero-site projector product pos={p}", flush=True)
+        T.position_(env, p)
+        ref.position(A, p)
+        shape = (A[p - 2].shape[2], A[p - 1].shape[0])
+        v = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+        want = ref.product(v)
+        got = T.product(env, v).reshape(want.shape)
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want), p
+    env.close()
