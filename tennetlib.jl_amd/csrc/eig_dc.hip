@@ -10,7 +10,7 @@
 //     wavefront per leaf); each merge of two neighbouring blocks solves
 //     D + rho z z^T: deflation (small |rho z_i|, or a Givens rotation for
 //     nearly equal d_i) on the host, the secular equation for every
-//     remaining root on the device (one thread per root, bracketed Newton
+//     remaining root on the device (one wavefront per root, bracketed Newton
 //     in tau = lambda - d_origin measured from the nearer pole), the
 //     Gu-Eisenstat z-hat, the eigenvectors z-hat / (d - lambda), and the
 //     block update Q <- Q U as an FP64 MFMA GEMM.  The host keeps the
@@ -45,6 +45,14 @@ __device__ double block_sum(double v, double* red) {
   if (lane == 0) red[wave] = v;
   __syncthreads();
   return (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// sum over the 64 lanes of a wavefront; the butterfly leaves the same bits in
+// every lane (both partners of a stage add the same two operands), so
+// branches on the result are uniform across the wavefront
+__device__ __forceinline__ double wave_allsum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
 }
 
 // ---------------------------------------------------------------------------
@@ -932,13 +940,15 @@ __global__ __launch_bounds__(256) void dc_givens_kernel(double* __restrict__ Q, 
 // and those on the far side are each replaced by one rational term at their
 // nearest pole matching value and slope, and the resulting quadratic is
 // solved for tau; bisection of the sign bracket guards every step.
+// One wavefront per root, the sums over the poles split over its lanes.
 __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restrict__ md,
                                                          const int* __restrict__ root_merge,
                                                          const double* __restrict__ dK,
                                                          const double* __restrict__ zK, int nroots,
                                                          int* __restrict__ org, double* __restrict__ tau) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= nroots) return;
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= nroots) return;  // the whole wavefront leaves
   const MergeDev m = md[root_merge[g]];
   const int i = g - m.koff, k = m.k;
   const double* dd = dK + m.koff;
@@ -948,8 +958,9 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
   double lo, hi, df;  // bracket for tau, far pole relative to the origin (0: none)
   if (i < k - 1) {
     const double mid = 0.5 * (dd[i] + dd[i + 1]);
-    double f = 1.0;
-    for (int q = 0; q < k; ++q) f += rho * zz[q] * zz[q] / (dd[q] - mid);
+    double f = 0.0;
+    for (int q = lane; q < k; q += 64) f += rho * zz[q] * zz[q] / (dd[q] - mid);
+    f = 1.0 + wave_allsum(f);
     if (f >= 0.0) {
       o = i; lo = 0.0; hi = mid - dd[i]; df = dd[i + 1] - dd[i];
     } else {
@@ -960,11 +971,12 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
     lo = 0.0;
     df = 0.0;
     double z2 = 0.0;
-    for (int q = 0; q < k; ++q) z2 += zz[q] * zz[q];
-    hi = rho * z2;
+    for (int q = lane; q < k; q += 64) z2 += zz[q] * zz[q];
+    hi = rho * wave_allsum(z2);
     for (int it = 0; it < 64; ++it) {  // make sure the bracket holds the root
-      double gh = 1.0;
-      for (int q = 0; q < k; ++q) gh += rho * zz[q] * zz[q] / ((dd[q] - dd[o]) - hi);
+      double gh = 0.0;
+      for (int q = lane; q < k; q += 64) gh += rho * zz[q] * zz[q] / ((dd[q] - dd[o]) - hi);
+      gh = 1.0 + wave_allsum(gh);
       if (gh >= 0.0) break;
       hi *= 2.0;
     }
@@ -975,13 +987,18 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
   double t = 0.5 * (lo + hi);
   for (int it = 0; it < 100; ++it) {
     double so = 0.0, dso = 0.0, sf = 0.0, dsf = 0.0, sabs = 0.0;
-    for (int q = 0; q < k; ++q) {
+    for (int q = lane; q < k; q += 64) {
       const double inv = 1.0 / ((dd[q] - dO) - t);
       const double r = rho * zz[q] * zz[q] * inv;
       sabs += fabs(r);
       if ((q <= i) == left_origin) { so += r; dso += r * inv; }
       else { sf += r; dsf += r * inv; }
     }
+    so = wave_allsum(so);
+    dso = wave_allsum(dso);
+    sf = wave_allsum(sf);
+    dsf = wave_allsum(dsf);
+    sabs = wave_allsum(sabs);
     const double gv = 1.0 + so + sf;
     // converged once g is within its own rounding error
     if (fabs(gv) <= 8.0 * EPS * (1.0 + sabs)) break;
@@ -1018,8 +1035,10 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
     t = tn;
     if (done) break;
   }
-  org[g] = o;
-  tau[g] = t;
+  if (lane == 0) {
+    org[g] = o;
+    tau[g] = t;
+  }
 }
 
 // Gu-Eisenstat: z-hat_j so that the computed roots are exact eigenvalues
@@ -1376,8 +1395,8 @@ static std::vector<double> dc_tridiag(tn_ctx* c, long p, const std::vector<doubl
                          d_rot);
     if (nroots > 0) {
       const unsigned gr = (unsigned)((nroots + 255) / 256);
-      hipLaunchKernelGGL(dc_secular_kernel, dim3(gr), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zK, nroots, d_org,
-                         d_tau);
+      hipLaunchKernelGGL(dc_secular_kernel, dim3((unsigned)((nroots + 3) / 4)), dim3(256), 0, c->stream, d_md, d_rm,
+                         d_dK, d_zK, nroots, d_org, d_tau);
       hipLaunchKernelGGL(dc_zhat_kernel, dim3(gr), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zK, d_org, d_tau,
                          nroots, d_zh);
       hipLaunchKernelGGL(dc_evec_kernel, dim3((unsigned)nroots), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zh,
