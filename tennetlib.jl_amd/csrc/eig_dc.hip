@@ -437,8 +437,22 @@ __global__ __launch_bounds__(256) void trd_tail_lds_kernel(double* __restrict__ 
   double* vs = M + (long)m0 * L; // m0
   double* ys = vs + m0;          // m0
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int r = wave; r < m0; r += 4)
-    for (int c = lane; c < m0; c += 64) M[r * L + c] = A[(long)(j0 + r) * ld + (j0 + c)];
+  {  // load the block in batches of 8 elements per thread (loads before stores)
+    const int tot = m0 * m0;
+    for (int base = 0; base < tot; base += 256 * 8) {
+      double t[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + 256 * u + tid;
+        t[u] = (idx < tot) ? A[(long)(j0 + idx / m0) * ld + (j0 + idx % m0)] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int idx = base + 256 * u + tid;
+        if (idx < tot) M[(idx / m0) * L + idx % m0] = t[u];
+      }
+    }
+  }
   __syncthreads();
   for (int jj = 0; jj < m0 - 2; ++jj) {
     const int j = j0 + jj, m = m0 - jj - 1;
@@ -1140,15 +1154,16 @@ __global__ __launch_bounds__(256) void wy_tsolve_kernel(const double* __restrict
   double x[NBW];
 #pragma unroll
   for (int i = 0; i < NBW; ++i) x[i] = (i < nb) ? W[(long)i * n + col] : 0.0;
-  // solve (D^{-1} + U) y = x from the bottom: y_i = tau_i (x_i - sum_{j>i} S_ij y_j)
+  // solve (D^{-1} + U) y = x from the bottom: y_i = tau_i (x_i - sum_{j>i} S_ij y_j),
+  // column-oriented: once y_i is final it is subtracted from every x_j, j < i,
+  // so the dependent chain is one step per row, not one per product
 #pragma unroll
   for (int i = NBW - 1; i >= 0; --i) {
     if (i < nb) {
-      double acc = x[i];
+      const double yi = dinv[i] * x[i];
+      x[i] = yi;
 #pragma unroll
-      for (int j = i + 1; j < NBW; ++j)
-        if (j < nb) acc -= Ss[i][j] * x[j];
-      x[i] = dinv[i] * acc;
+      for (int j = 0; j < i; ++j) x[j] -= Ss[j][i] * yi;
     }
   }
 #pragma unroll

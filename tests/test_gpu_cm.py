@@ -4,10 +4,13 @@ a list of MPOs (ProjMPOSum2) against the oracle's per-id restatement.
 Tolerances: products 1e-12 relative (FP64, summation order only); noisy
 bond replacement: kept spectrum 1e-10 relative, truncation error 1e-12, the
 product of the two new sites 1e-10 relative; sweep energies as in the MPO
-noisy-DMRG parity test (1e-9 over the noisy sweeps, whose two-iteration
-Lanczos Ritz values depend on how the cutoff splits near-degenerate
-density-matrix multiplets; 1e-11 after them); ground energies 1e-9 of exact
-diagonalisation.
+noisy-DMRG parity test: the noisy sweeps' two-iteration Lanczos Ritz values
+depend on how the cutoff splits near-degenerate density-matrix multiplets.
+With every density matrix perturbed by 2e-16 * max|rho| before its eigh, the
+oracle's first-sweep energy moves by up to 6e-12 for the CouplingModel chain
+and by up to 1.9e-9 for the MPO sum (later sweeps < 2e-14): 1e-9 on both
+noisy sweeps of the former, 1e-8 on the first of the latter, 1e-11 after
+them; ground energies 1e-9 of exact diagonalisation.
 """
 
 import ctypes as C
@@ -196,7 +199,8 @@ def test_mpo_sum_noisy_dmrg_matches_oracle(T):
     env = T.StateEnvs(O.neel_mps(N), mpos)
     sw = T.dmrg_(env, T.make_dmrg_params(maxdim=[64], nsweeps=[6], **kw), 2, outputlevel=0)
     print("mpo sum noisy dmrg", sw.energy, sw_ref.energy, flush=True)
-    assert np.allclose(sw.energy[:2], sw_ref.energy[:2], rtol=0, atol=1e-9)
+    assert abs(sw.energy[0] - sw_ref.energy[0]) < 1e-8
+    assert abs(sw.energy[1] - sw_ref.energy[1]) < 1e-9
     assert np.allclose(sw.energy[2:], sw_ref.energy[2:], rtol=0, atol=1e-11)
     e0 = spla.eigsh(O.mpo_to_dense(O.xxz_mpo(N, 1.0, 1.0, h)), k=1, which="SA", tol=1e-12)[0][0]
     assert abs(sw.energy[-1] - e0) < 1e-9

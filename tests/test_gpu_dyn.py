@@ -4,8 +4,11 @@ against the oracle (oracle/tn_oracle_dyn.py).
 
 Tolerances: zero-site products 1e-12 relative (FP64); one-site DMRG sweep
 energies 1e-10 of the oracle's (Ritz values of the same projected problem);
-noisy one-site sweeps 1e-9 (the noise-perturbed factorisation keeps
-near-degenerate density-matrix multiplets whose split follows rounding);
+noisy one-site sweeps 1e-8 on the first sweep and 1e-9 after it (the
+noise-perturbed factorisation keeps near-degenerate density-matrix multiplets
+whose split follows rounding: the oracle itself moves the first sweep's
+energy by up to 4.3e-9 when every density matrix is perturbed by
+2e-16 * max|rho| before its eigh, later sweeps by < 1e-14);
 time evolution: overlap with exact evolution 1e-8, energy conservation 1e-9;
 subspace expansion: bond dimensions equal to the oracle's and the right-basis
 projectors within 1e-8.
@@ -107,7 +110,8 @@ def test_one_site_noisy_dmrg_matches_oracle(T):
         T.fullsweep_(env, T.eig_solver, 1, sw, maxdim=64, cutoff=1e-14, noise=nz, outputlevel=0)
         noise /= 2
     print("dmrg1 noise", sw.energy, sw_ref.energy, flush=True)
-    assert np.allclose(sw.energy, sw_ref.energy, rtol=0, atol=1e-9)
+    assert abs(sw.energy[0] - sw_ref.energy[0]) < 1e-8
+    assert np.allclose(sw.energy[1:], sw_ref.energy[1:], rtol=0, atol=1e-9)
     assert sw.maxchi[-1] > 8
     e0 = spla.eigsh(O.mpo_to_dense(Ws), k=1, which="SA")[0][0]
     assert abs(sw.energy[-1] - e0) < 1e-9

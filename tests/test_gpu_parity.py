@@ -238,10 +238,15 @@ def test_noisy_dmrg_matches_oracle(T, prec):
     sw = T.dmrg_(env, T.make_dmrg_params(maxdim=[64], nsweeps=[6], **kw), 2, outputlevel=0, precision=prec)
     # The two noisy sweeps end on Ritz values of a two-iteration Lanczos whose
     # start vector depends on how the cutoff splits near-degenerate (SU(2))
-    # density-matrix multiplets: 1e-9 there in FP64 (1.2e-10 measured) and
-    # |dE/N| <= 1e-9 in FP32 (1.6e-9 measured in total); the noiseless sweeps
-    # after them agree to 1e-11 (FP64; 1e-13 measured) or 1e-9 (FP32).
-    assert np.allclose(sw.energy[:2], sw_ref.energy[:2], rtol=0, atol=1e-9 if prec == "fp64" else N * 1e-9)
+    # density-matrix multiplets.  The first sweep's energy is fixed only to a
+    # few 1e-9 by the algorithm itself: the oracle, with every density matrix
+    # perturbed by a random symmetric 2e-16 * max|rho| before its eigh, moves it
+    # by up to 1.7e-9 (five perturbations) while later sweeps move by < 2e-14.
+    # Hence 1e-8 on sweep 1 in FP64, 1e-9 on sweep 2, |dE/N| <= 1e-9 in FP32
+    # (1.6e-9 measured in total); the noiseless sweeps after them agree to
+    # 1e-11 (FP64; 1e-13 measured) or 1e-9 (FP32).
+    assert abs(sw.energy[0] - sw_ref.energy[0]) < (1e-8 if prec == "fp64" else N * 1e-9)
+    assert abs(sw.energy[1] - sw_ref.energy[1]) < (1e-9 if prec == "fp64" else N * 1e-9)
     tol = 1e-11 if prec == "fp64" else 1e-9
     assert np.allclose(sw.energy[2:], sw_ref.energy[2:], rtol=0, atol=tol)
     assert sw.maxchi == sw_ref.maxchi
