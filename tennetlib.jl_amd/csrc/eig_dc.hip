@@ -47,6 +47,24 @@ __device__ double block_sum(double v, double* red) {
   return (red[0] + red[1]) + (red[2] + red[3]);
 }
 
+// two block sums with one pair of barriers; each result has the bits block_sum
+// would give
+__device__ void block_sum2(double u, double v, double* red, double& su, double& sv) {
+  for (int o = 32; o > 0; o >>= 1) {
+    u += __shfl_down(u, o, 64);
+    v += __shfl_down(v, o, 64);
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+    red[wave] = u;
+    red[4 + wave] = v;
+  }
+  __syncthreads();
+  su = (red[0] + red[1]) + (red[2] + red[3]);
+  sv = (red[4] + red[5]) + (red[6] + red[7]);
+}
+
 // sum over the 64 lanes of a wavefront; the butterfly leaves the same bits in
 // every lane (both partners of a stage add the same two operands), so
 // branches on the result are uniform across the wavefront
@@ -188,6 +206,7 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
                                                        double* __restrict__ ev, double* __restrict__ tauv) {
   extern __shared__ double lds[];
   __shared__ double red[4];
+  __shared__ double red2[8];
   const int mj = p - j - 1, mn = mj - 1;
   double* vj = lds;
   double* wj = vj + mj;
@@ -214,13 +233,31 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
       rv[u] = (pre && t < mn) ? row0[t] : 0.0;
     }
   }
+  // v_j and y_j (up to 1024 entries) are loaded before the reduction of the
+  // partial sums, so both global round trips overlap
+  constexpr int PV = 4;
+  double pv[PV], py[PV];
+#pragma unroll
+  for (int u = 0; u < PV; ++u) {
+    const int c = tid + 256 * u;
+    pv[u] = (c < mj) ? vj_in[c] : 0.0;
+    py[u] = (c < mj) ? yj[c] : 0.0;
+  }
   const double tj = tauv[j];
   double s = 0.0;
   if (tj != 0.0)
     for (int i = tid; i < npj; i += 256) s += partj[i];
   s = block_sum(s, red);
   const double hs = 0.5 * tj * s;
-  for (int c = tid; c < mj; c += 256) {
+#pragma unroll
+  for (int u = 0; u < PV; ++u) {
+    const int c = tid + 256 * u;
+    if (c < mj) {
+      vj[c] = pv[u];
+      wj[c] = (tj != 0.0) ? py[u] - hs * pv[u] : 0.0;
+    }
+  }
+  for (int c = tid + 256 * PV; c < mj; c += 256) {
     const double v = vj_in[c];
     vj[c] = v;
     wj[c] = (tj != 0.0) ? yj[c] - hs * v : 0.0;
@@ -255,8 +292,8 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
     pa += wj[t + 1] * v;
     pb += vj[t + 1] * v;
   }
-  const double a = block_sum(pa, red);
-  const double b = block_sum(pb, red);
+  double a, b;
+  block_sum2(pa, pb, red2, a, b);
   if (g == 0) {
     for (int t = tid; t < mn; t += 256) {
       A[(long)(j + 2 + t) * ld + (j + 1)] = vn[t];
