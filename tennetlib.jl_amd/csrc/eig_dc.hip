@@ -37,36 +37,6 @@ constexpr int HR = 4;     // rows of y per workgroup in the reflector step (one 
 constexpr int LEAF = 32;  // leaf size of the divide and conquer
 constexpr int NBW = 64;   // reflectors per compact-WY block
 
-// x plus the value of the lane that the DPP control CTRL selects (row-local
-// permutations; every source lane must be active)
-template <int CTRL>
-__device__ __forceinline__ double dpp_add(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
-  return x + __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
-
-__device__ __forceinline__ double read_lane(double x, int l) {
-  const long long b = __double_as_longlong(x);
-  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
-}
-
-// sum over the 64 lanes of a wavefront in registers (no LDS round trips):
-// lane pairs xor 1 and xor 2 inside each quad, then the half-row and row
-// mirrors (both partners hold equal partial sums, so every lane of a row ends
-// with the same bits), then the four row sums read from lanes 0, 16, 32, 48.
-// Every lane of the wavefront must be active.
-__device__ __forceinline__ double wave_sum64(double x) {
-  x = dpp_add<0xB1>(x);   // quad_perm [1, 0, 3, 2]
-  x = dpp_add<0x4E>(x);   // quad_perm [2, 3, 0, 1]
-  x = dpp_add<0x141>(x);  // row_half_mirror
-  x = dpp_add<0x140>(x);  // row_mirror
-  return (read_lane(x, 0) + read_lane(x, 16)) + (read_lane(x, 32) + read_lane(x, 48));
-}
-
 // sum over the 256 threads of a workgroup; every thread gets the same value
 __device__ double block_sum(double v, double* red) {
   v = wave_sum64(v);

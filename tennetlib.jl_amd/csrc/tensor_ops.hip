@@ -95,11 +95,7 @@ __global__ void scaled_copy_kernel(double* __restrict__ y, const double* __restr
 // part[v * nblk + blk] = partial sum of <basis_v, x> over the block's slice.
 // Complex: data are interleaved (re, im); <u, x> = sum conj(u) x -> two parts.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum64(v); }
 
 template <bool CPLX>
 __global__ __launch_bounds__(RED_THREADS) void multidot_stage1(const double* __restrict__ basis, long ld,
