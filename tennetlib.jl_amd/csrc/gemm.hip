@@ -648,8 +648,17 @@ static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g);
 static bool gemm_split_k(tn_ctx* c, Prec p, const GemmDesc& g) {
   if (!(p == Prec::F64 || p == Prec::F32) || g.batch != 1) return false;
   const long tiles = ((g.M + 127) / 128) * ((g.N + 127) / 128);
-  if (tiles >= 64 || g.K < 4096) return false;
-  const long P0 = std::min<long>(g.K / 512, 256 / tiles);
+  long P0;
+  if (tiles < 64 && g.K >= 4096) {
+    P0 = std::min<long>(g.K / 512, 256 / tiles);
+  } else {
+    // mid-size products (the eigensolver's merges and compact-WY updates at
+    // p of a few hundred): fewer than 64 tiles of 64 x 64 and K >= 256 run as
+    // chunks of at least 64 along K
+    const long t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64);
+    if (t64 >= 64 || g.K < 256) return false;
+    P0 = std::min<long>(g.K / 64, 128 / t64);
+  }
   if (P0 < 2) return false;
   const long Kc = ((g.K + P0 - 1) / P0 + 15) / 16 * 16;
   const long P = (g.K + Kc - 1) / Kc;
