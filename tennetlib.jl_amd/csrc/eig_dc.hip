@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void trd_coop_kernel(double* __restrict__ A, l
 // ---- the last columns in LDS: the trailing block A[j0:, j0:] (m0 = p - j0
 // <= TAIL rows) is reduced by one workgroup without leaving LDS; reflectors,
 // d, e and tau are written exactly as by the per-column kernels.
-constexpr int TAIL = 128;
+constexpr int TAIL = 136;  // 136 x 137 doubles + two vectors: 148 KB of LDS
 __global__ __launch_bounds__(256) void trd_tail_lds_kernel(double* __restrict__ A, long ld, int p, int j0,
                                                            double* __restrict__ dv, double* __restrict__ ev,
                                                            double* __restrict__ tauv) {

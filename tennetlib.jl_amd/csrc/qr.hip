@@ -20,7 +20,7 @@ namespace tn {
 namespace {
 
 __device__ double qr_block_sum(double v, double* red) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  v = wave_sum64(v);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) red[wave] = v;
@@ -73,10 +73,8 @@ __global__ __launch_bounds__(256) void qr_step_kernel(double* __restrict__ X, lo
 constexpr int QR_THREADS = 512;
 
 __device__ void qr_block_sum2(double& a, double& b, double* red) {
-  for (int o = 32; o > 0; o >>= 1) {
-    a += __shfl_down(a, o, 64);
-    b += __shfl_down(b, o, 64);
-  }
+  a = wave_sum64(a);
+  b = wave_sum64(b);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   __syncthreads();
   if (lane == 0) {
