@@ -63,12 +63,23 @@ __device__ void block_sum2(double u, double v, double* red, double& su, double& 
   sv = (red[4] + red[5]) + (red[6] + red[7]);
 }
 
-// sum over the 64 lanes of a wavefront; the butterfly leaves the same bits in
-// every lane (both partners of a stage add the same two operands), so
-// branches on the result are uniform across the wavefront
-__device__ __forceinline__ double wave_allsum(double x) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+// x times the value of the lane that the DPP control CTRL selects
+template <int CTRL>
+__device__ __forceinline__ double dpp_mul(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return x * __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+
+// product over the 64 lanes of a wavefront, the same pattern as wave_sum64
+// (every lane gets the same value; every lane must be active)
+__device__ __forceinline__ double wave_prod64(double x) {
+  x = dpp_mul<0xB1>(x);
+  x = dpp_mul<0x4E>(x);
+  x = dpp_mul<0x141>(x);
+  x = dpp_mul<0x140>(x);
+  return (read_lane(x, 0) * read_lane(x, 16)) * (read_lane(x, 32) * read_lane(x, 48));
 }
 
 // ---------------------------------------------------------------------------
@@ -1009,7 +1020,7 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
     const double mid = 0.5 * (dd[i] + dd[i + 1]);
     double f = 0.0;
     for (int q = lane; q < k; q += 64) f += rho * zz[q] * zz[q] / (dd[q] - mid);
-    f = 1.0 + wave_allsum(f);
+    f = 1.0 + wave_sum64(f);
     if (f >= 0.0) {
       o = i; lo = 0.0; hi = mid - dd[i]; df = dd[i + 1] - dd[i];
     } else {
@@ -1021,11 +1032,11 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
     df = 0.0;
     double z2 = 0.0;
     for (int q = lane; q < k; q += 64) z2 += zz[q] * zz[q];
-    hi = rho * wave_allsum(z2);
+    hi = rho * wave_sum64(z2);
     for (int it = 0; it < 64; ++it) {  // make sure the bracket holds the root
       double gh = 0.0;
       for (int q = lane; q < k; q += 64) gh += rho * zz[q] * zz[q] / ((dd[q] - dd[o]) - hi);
-      gh = 1.0 + wave_allsum(gh);
+      gh = 1.0 + wave_sum64(gh);
       if (gh >= 0.0) break;
       hi *= 2.0;
     }
@@ -1043,11 +1054,11 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
       if ((q <= i) == left_origin) { so += r; dso += r * inv; }
       else { sf += r; dsf += r * inv; }
     }
-    so = wave_allsum(so);
-    dso = wave_allsum(dso);
-    sf = wave_allsum(sf);
-    dsf = wave_allsum(dsf);
-    sabs = wave_allsum(sabs);
+    so = wave_sum64(so);
+    dso = wave_sum64(dso);
+    sf = wave_sum64(sf);
+    dsf = wave_sum64(dsf);
+    sabs = wave_sum64(sabs);
     const double gv = 1.0 + so + sf;
     // converged once g is within its own rounding error
     if (fabs(gv) <= 8.0 * EPS * (1.0 + sabs)) break;
@@ -1090,27 +1101,34 @@ __global__ __launch_bounds__(256) void dc_secular_kernel(const MergeDev* __restr
   }
 }
 
-// Gu-Eisenstat: z-hat_j so that the computed roots are exact eigenvalues
+// Gu-Eisenstat: z-hat_j so that the computed roots are exact eigenvalues;
+// one wavefront per j, the product over the roots split over its lanes
 __global__ __launch_bounds__(256) void dc_zhat_kernel(const MergeDev* __restrict__ md,
                                                       const int* __restrict__ root_merge,
                                                       const double* __restrict__ dK,
                                                       const double* __restrict__ zK,
                                                       const int* __restrict__ org, const double* __restrict__ tau,
                                                       int nroots, double* __restrict__ zh) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  if (g >= nroots) return;
+  const int lane = threadIdx.x & 63;
+  const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (g >= nroots) return;  // the whole wavefront leaves
   const MergeDev m = md[root_merge[g]];
   const int j = g - m.koff, k = m.k;
   const double* dd = dK + m.koff;
   const int* oo = org + m.koff;
   const double* tt = tau + m.koff;
   const double dj = dd[j];
-  // lambda_i - d_j = (d_o(i) - d_j) + tau_i
-  double prod = ((dd[oo[k - 1]] - dj) + tt[k - 1]) / m.rho;
-  for (int i = 0; i < j; ++i) prod *= ((dd[oo[i]] - dj) + tt[i]) / (dd[i] - dj);
-  for (int i = j; i < k - 1; ++i) prod *= ((dd[oo[i]] - dj) + tt[i]) / (dd[i + 1] - dj);
-  zh[g] = copysign(sqrt(fmax(prod, 0.0)), zK[m.koff + j]);
+  // lambda_i - d_j = (d_o(i) - d_j) + tau_i; by interlacing every factor
+  // below lies in (0, 1)
+  double prod = 1.0;
+  for (int i = lane; i < k - 1; i += 64) {
+    const double den = (i < j) ? (dd[i] - dj) : (dd[i + 1] - dj);
+    prod *= ((dd[oo[i]] - dj) + tt[i]) / den;
+  }
+  prod = wave_prod64(prod) * (((dd[oo[k - 1]] - dj) + tt[k - 1]) / m.rho);
+  if (lane == 0) zh[g] = copysign(sqrt(fmax(prod, 0.0)), zK[m.koff + j]);
 }
+
 
 // row i of U^T: u_i[j] = zh_j / (d_j - lambda_i), normalised; one workgroup per root
 __global__ __launch_bounds__(256) void dc_evec_kernel(const MergeDev* __restrict__ md,
@@ -1444,11 +1462,10 @@ static std::vector<double> dc_tridiag(tn_ctx* c, long p, const std::vector<doubl
       hipLaunchKernelGGL(dc_givens_kernel, dim3((maxn + 255) / 256, nm), dim3(256), 0, c->stream, Q, ldq, d_md,
                          d_rot);
     if (nroots > 0) {
-      const unsigned gr = (unsigned)((nroots + 255) / 256);
       hipLaunchKernelGGL(dc_secular_kernel, dim3((unsigned)((nroots + 3) / 4)), dim3(256), 0, c->stream, d_md, d_rm,
                          d_dK, d_zK, nroots, d_org, d_tau);
-      hipLaunchKernelGGL(dc_zhat_kernel, dim3(gr), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zK, d_org, d_tau,
-                         nroots, d_zh);
+      hipLaunchKernelGGL(dc_zhat_kernel, dim3((unsigned)((nroots + 3) / 4)), dim3(256), 0, c->stream, d_md, d_rm,
+                         d_dK, d_zK, d_org, d_tau, nroots, d_zh);
       hipLaunchKernelGGL(dc_evec_kernel, dim3((unsigned)nroots), dim3(256), 0, c->stream, d_md, d_rm, d_dK, d_zh,
                          d_org, d_tau, Utb);
     }
