@@ -243,6 +243,7 @@ def main():
     ap.add_argument("--tdvp-steps", type=int, default=40, help="TDVP steps of the XXZ N=100 run (0: skip)")
     ap.add_argument("--ttn-instances", type=int, default=16, help="TTN instances timed of the 256 (0: skip)")
     ap.add_argument("--ttn-only", action="store_true", help="only the many-instance TTN line")
+    ap.add_argument("--tdvp-only", action="store_true", help="only the TDVP XXZ N=100 line")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -270,6 +271,9 @@ def main():
     # 1. matvec microbenchmark at chi = mv_chi
     if args.ttn_only:
         print(json.dumps({"ttn_random_field_batch": ttn_instances(T, max(1, args.ttn_instances))}), flush=True)
+        return
+    if args.tdvp_only:
+        print(json.dumps({"tdvp_xxz_n100": tdvp_run(T, 100, max(1, args.tdvp_steps), 0.05, 512, 1e-12)}), flush=True)
         return
     mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", "bf16x3"])
     mv_wide = None

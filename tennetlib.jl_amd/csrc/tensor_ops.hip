@@ -360,20 +360,25 @@ void scaled_copy(tn_ctx* c, double* y, const double* x, double a, long n) {
 void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double* x, long n, bool cplx,
                double* out_dev) {
   TN_REQUIRE(count >= 1, TN_E_INVALID, "multi_dot: count");
-  const int nblk = (int)std::min<long>(RED_BLOCKS, std::max<long>(1, (n + RED_THREADS - 1) / RED_THREADS));
-  double* part = (double*)c->workspace(WS_KRY_PART, (size_t)2 * MAX_DOT * nblk * sizeof(double));
+  // short vectors (small bonds): one workgroup sums everything and writes
+  // the results itself (part[v * 1 + 0] is out[v]), so no second launch
+  const bool one = n <= 16L * RED_THREADS;
+  const int nblk = one ? 1 : (int)std::min<long>(RED_BLOCKS, std::max<long>(1, (n + RED_THREADS - 1) / RED_THREADS));
+  double* part = one ? nullptr : (double*)c->workspace(WS_KRY_PART, (size_t)2 * MAX_DOT * nblk * sizeof(double));
   for (int v0 = 0; v0 < count; v0 += MAX_DOT) {
     const int cnt = std::min(MAX_DOT, count - v0);
     const int nacc = cplx ? 2 * cnt : cnt;
+    double* o = out_dev + (cplx ? 2 * v0 : v0);
+    double* dst = one ? o : part;
     if (cplx)
       hipLaunchKernelGGL(multidot_stage1<true>, dim3(nblk), dim3(RED_THREADS), 0, c->stream, basis + v0 * ld,
-                         ld, cnt, x, n, part);
+                         ld, cnt, x, n, dst);
     else
       hipLaunchKernelGGL(multidot_stage1<false>, dim3(nblk), dim3(RED_THREADS), 0, c->stream, basis + v0 * ld,
-                         ld, cnt, x, n, part);
+                         ld, cnt, x, n, dst);
     TN_HIP(hipGetLastError());
-    hipLaunchKernelGGL(multidot_stage2, dim3(nacc), dim3(256), 0, c->stream, part, nblk, nacc,
-                       out_dev + (cplx ? 2 * v0 : v0));
+    if (one) continue;
+    hipLaunchKernelGGL(multidot_stage2, dim3(nacc), dim3(256), 0, c->stream, part, nblk, nacc, o);
     TN_HIP(hipGetLastError());
   }
 }
