@@ -100,7 +100,8 @@ __device__ __forceinline__ double wave_sum(double v) { return wave_sum64(v); }
 template <bool CPLX>
 __global__ __launch_bounds__(RED_THREADS) void multidot_stage1(const double* __restrict__ basis, long ld,
                                                                int count, const double* __restrict__ x,
-                                                               long n, double* __restrict__ part) {
+                                                               long n, double* __restrict__ part,
+                                                               double* __restrict__ hout) {
   __shared__ double red[RED_THREADS / 64][2 * MAX_DOT];
   const int nblk = gridDim.x;
   double acc[2 * MAX_DOT];
@@ -138,11 +139,13 @@ __global__ __launch_bounds__(RED_THREADS) void multidot_stage1(const double* __r
     double s = 0.0;
     for (int w = 0; w < RED_THREADS / 64; ++w) s += red[w][v];
     part[(long)v * nblk + blockIdx.x] = s;
+    if (hout) hout[v] = s;  // one-workgroup launch: s is the final value
   }
 }
 
 __global__ __launch_bounds__(256) void multidot_stage2(const double* __restrict__ part, int nblk,
-                                                       int nacc, double* __restrict__ out) {
+                                                       int nacc, double* __restrict__ out,
+                                                       double* __restrict__ hout) {
   __shared__ double red[4];
   const int v = blockIdx.x;
   if (v >= nacc) return;
@@ -151,7 +154,11 @@ __global__ __launch_bounds__(256) void multidot_stage2(const double* __restrict_
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[v] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) {
+    const double r = red[0] + red[1] + red[2] + red[3];
+    out[v] = r;
+    if (hout) hout[v] = r;
+  }
 }
 
 // x -= sum_v coef[v] * basis_v (real or complex coefficients)
@@ -358,7 +365,7 @@ void scaled_copy(tn_ctx* c, double* y, const double* x, double a, long n) {
 }
 
 void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double* x, long n, bool cplx,
-               double* out_dev) {
+               double* out_dev, double* out_host) {
   TN_REQUIRE(count >= 1, TN_E_INVALID, "multi_dot: count");
   // short vectors (small bonds): one workgroup sums everything and writes
   // the results itself (part[v * 1 + 0] is out[v]), so no second launch
@@ -369,16 +376,18 @@ void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double*
     const int cnt = std::min(MAX_DOT, count - v0);
     const int nacc = cplx ? 2 * cnt : cnt;
     double* o = out_dev + (cplx ? 2 * v0 : v0);
+    double* ho = out_host ? out_host + (cplx ? 2 * v0 : v0) : nullptr;
     double* dst = one ? o : part;
+    double* hdst = one ? ho : nullptr;
     if (cplx)
       hipLaunchKernelGGL(multidot_stage1<true>, dim3(nblk), dim3(RED_THREADS), 0, c->stream, basis + v0 * ld,
-                         ld, cnt, x, n, dst);
+                         ld, cnt, x, n, dst, hdst);
     else
       hipLaunchKernelGGL(multidot_stage1<false>, dim3(nblk), dim3(RED_THREADS), 0, c->stream, basis + v0 * ld,
-                         ld, cnt, x, n, dst);
+                         ld, cnt, x, n, dst, hdst);
     TN_HIP(hipGetLastError());
     if (one) continue;
-    hipLaunchKernelGGL(multidot_stage2, dim3(nacc), dim3(256), 0, c->stream, part, nblk, nacc, o);
+    hipLaunchKernelGGL(multidot_stage2, dim3(nacc), dim3(256), 0, c->stream, part, nblk, nacc, o, ho);
     TN_HIP(hipGetLastError());
   }
 }

@@ -92,6 +92,7 @@ struct tn_ctx {
   // scalars, `hstage` (grow-only) for the eigensolver's per-level tables
   static constexpr size_t PINNED_N = 4096;
   double* pinned = nullptr;
+  double* pinned_dev = nullptr;  // the same memory as seen from kernels
   void* hstage = nullptr;
   size_t hstage_bytes = 0;
   void* host_stage(size_t bytes) {
@@ -261,8 +262,11 @@ void fill_f64(tn_ctx* c, double* x, double v, long n);
 void copy_f64(tn_ctx* c, double* dst, const double* src, long n);
 // out[i] = <basis_i, x> for i < count (basis vectors contiguous with stride ld);
 // cplx: complex128 data of n complex elements, out holds (re, im) pairs.
+// out_host (optional): a device-side address of mapped pinned host memory
+// that receives the same values, readable on the host after the stream has
+// been synchronised (no separate copy).
 void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double* x, long n,
-               bool cplx, double* out_dev);
+               bool cplx, double* out_dev, double* out_host = nullptr);
 // x -= sum_i coef[i] * basis_i ; coef on device (real or complex pairs)
 void multi_axpy_sub(tn_ctx* c, const double* basis, long ld, int count, const double* coef_dev,
                     double* x, long n, bool cplx);

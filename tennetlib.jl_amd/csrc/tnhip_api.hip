@@ -749,11 +749,9 @@ struct Krylov {
   double* V(int i) { return base + (long)i * nd; }
   // <x, x>
   double norm2(const double* x) {
-    multi_dot(c, x, nd, 1, x, n, cx, coef);
-    double* h = c->pinned;
-    TN_HIP(hipMemcpyAsync(h, coef, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    multi_dot(c, x, nd, 1, x, n, cx, coef, c->pinned_dev);
     TN_HIP(hipStreamSynchronize(c->stream));
-    return h[0];
+    return c->pinned[0];
   }
   // w <- w - V_{0..K-1} (V^H w), twice; returns the total coefficients
   // (complex pairs when cx) and |w|^2.
@@ -761,13 +759,13 @@ struct Krylov {
     const int str = cx ? 2 : 1;
     // coefficient scratch: pass 1 at [0, 2K), pass 2 at [128, 128+2K),
     // |w|^2 at 256 (K <= 61 basis vectors, complex pairs fit in 128)
-    multi_dot(c, base, nd, K, w, n, cx, coef);
+    // the three results also land in mapped pinned memory (no read-back copy)
+    multi_dot(c, base, nd, K, w, n, cx, coef, c->pinned_dev);
     multi_axpy_sub(c, base, nd, K, coef, w, n, cx);
-    multi_dot(c, base, nd, K, w, n, cx, coef + 128);
+    multi_dot(c, base, nd, K, w, n, cx, coef + 128, c->pinned_dev + 128);
     multi_axpy_sub(c, base, nd, K, coef + 128, w, n, cx);
-    multi_dot(c, w, nd, 1, w, n, cx, coef + 256);
+    multi_dot(c, w, nd, 1, w, n, cx, coef + 256, c->pinned_dev + 256);
     double* hb = c->pinned;
-    TN_HIP(hipMemcpyAsync(hb, coef, 258 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TN_HIP(hipStreamSynchronize(c->stream));
     h.resize(K);
     for (int i = 0; i < K; ++i) {
@@ -1975,7 +1973,8 @@ int tn_ctx_create(int device, tn_ctx** out) {
     double one[2] = {1.0, 0.0};
     TN_HIP(hipMemcpy(c->ones_buf.p, one, 16, hipMemcpyHostToDevice));
     c->ones = c->ones_buf.as<double>();
-    TN_HIP(hipHostMalloc((void**)&c->pinned, tn_ctx::PINNED_N * sizeof(double), hipHostMallocDefault));
+    TN_HIP(hipHostMalloc((void**)&c->pinned, tn_ctx::PINNED_N * sizeof(double), hipHostMallocMapped));
+    TN_HIP(hipHostGetDevicePointer((void**)&c->pinned_dev, c->pinned, 0));
     const char* tv = std::getenv("TNHIP_TIMING");
     c->timing = tv && tv[0] == '1';
     *out = c;
