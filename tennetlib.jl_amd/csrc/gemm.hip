@@ -746,8 +746,16 @@ static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g) {
       }
       break;
     case Prec::Z64: {
-      dim3 gr = grid_for<64, 64>(g);
-      hipLaunchKernelGGL((gemm_z64_kernel<64, 64, 8>), gr, dim3(256), 0, c->stream, a);
+      // small complex products (TDVP at small bond dimension): 32 x 32 tiles
+      // put four times as many workgroups on the MFMA work of each tile
+      const long t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64) * (long)g.batch;
+      if (t64 < 64) {
+        dim3 gr = grid_for<32, 32>(g);
+        hipLaunchKernelGGL((gemm_z64_kernel<32, 32, 8>), gr, dim3(256), 0, c->stream, a);
+      } else {
+        dim3 gr = grid_for<64, 64>(g);
+        hipLaunchKernelGGL((gemm_z64_kernel<64, 64, 8>), gr, dim3(256), 0, c->stream, a);
+      }
       break;
     }
   }
