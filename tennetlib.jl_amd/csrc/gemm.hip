@@ -719,7 +719,11 @@ static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g) {
   const bool small = (g.M * g.N <= 256L * 256L) || few_tiles;
   switch (p) {
     case Prec::F64:
-      if (small) {
+      if (small && ((g.M + 63) / 64) * ((g.N + 63) / 64) * (long)g.batch < 64) {
+        // few 64 x 64 tiles: 32 x 32 tiles, four times the workgroups
+        dim3 gr = grid_for<32, 32>(g);
+        hipLaunchKernelGGL((gemm_f64_kernel<32, 32, 16>), gr, dim3(256), 0, c->stream, a);
+      } else if (small) {
         dim3 gr = grid_for<64, 64>(g);
         hipLaunchKernelGGL((gemm_f64_kernel<64, 64, 16>), gr, dim3(256), 0, c->stream, a);
       } else {
