@@ -10,8 +10,13 @@ whose split follows rounding: the oracle itself moves the first sweep's
 energy by up to 4.3e-9 when every density matrix is perturbed by
 2e-16 * max|rho| before its eigh, later sweeps by < 1e-14);
 time evolution: overlap with exact evolution 1e-8, energy conservation 1e-9;
-subspace expansion: bond dimensions equal to the oracle's and the right-basis
-projectors within 1e-8.
+subspace expansion: bond dimensions equal to the oracle's, the right-basis
+projectors within 1e-8 and the state kept to 2e-9 max|v0|. Perturbing every
+matrix the oracle diagonalises by a random Hermitian 2e-16 * max|.| moves its own
+reconstruction by up to 4.7e-10 max|v0| on the real case (five perturbations;
+complex: 5e-12), so the state is fixed only to that level by the algorithm: the
+basis added from eigenvectors of (1 - B^H B) G (1 - B^H B) near the 1e-7 cutoff
+carries rounding components along B of order eps ||G|| / lambda.
 """
 
 import numpy as np
@@ -163,7 +168,7 @@ def test_krylov_extend_matches_oracle(T, cplx):
     print("gse dims", [a.shape[2] for a in got], [a.shape[2] for a in ref.psi], flush=True)
     assert [a.shape[2] for a in got] == [a.shape[2] for a in ref.psi]
     v0 = O.mps_to_dense(A0).reshape(-1)
-    assert np.abs(O.mps_to_dense(got).reshape(-1) - v0).max() < 1e-10 * np.abs(v0).max()
+    assert np.abs(O.mps_to_dense(got).reshape(-1) - v0).max() < 2e-9 * np.abs(v0).max()
     for Pg, Pr in zip(_right_projectors(got), _right_projectors(ref.psi)):
         assert np.abs(Pg - Pr).max() < 1e-8
     assert env.orthocenter == 1
