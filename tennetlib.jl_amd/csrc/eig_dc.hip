@@ -889,14 +889,20 @@ __global__ __launch_bounds__(64) void dc_leaf_kernel(const double* __restrict__ 
   __syncthreads();
   const double tol_rel = 64.0 * EPS;
   const int pp = tid >> 2, sub = tid & 3;
+  // the tournament runs over the leaf's own size rounded up to even (Pe - 1
+  // rounds of Pe / 2 pairs), not over all P slots: a padding index has no
+  // coupling and never rotates
+  const int Pe = (n + 1) & ~1;
   for (int sweep = 0; sweep < 60; ++sweep) {
     int nsweep = 0;
-    for (int step = 0; step < P - 1; ++step) {
+    for (int step = 0; step < Pe - 1; ++step) {
       int f = 0;
-      if (tid < P / 2) {
-        const int pa = tid, pb = P - 1 - tid;
-        const int i = (pa == 0) ? 0 : 1 + ((pa - 1 + step) % (P - 1));
-        const int jj = 1 + ((pb - 1 + step) % (P - 1));
+      if (tid < P / 2 && tid >= Pe / 2) {
+        pi_[tid] = 0; pj_[tid] = 0; cs[tid] = 1.0; sn[tid] = 0.0; fl[tid] = 0;
+      } else if (tid < P / 2) {
+        const int pa = tid, pb = Pe - 1 - tid;
+        const int i = (pa == 0) ? 0 : 1 + ((pa - 1 + step) % (Pe - 1));
+        const int jj = 1 + ((pb - 1 + step) % (Pe - 1));
         const double x = Gs[i][i], z = Gs[jj][jj], g = Gs[i][jj];
         double c = 1.0, s = 0.0;
         if (fabs(g) > fmax(tol_rel * sqrt(fabs(x)) * sqrt(fabs(z)), tol_abs)) {
@@ -1336,7 +1342,14 @@ static std::vector<double> dc_tridiag(tn_ctx* c, long p, const std::vector<doubl
   for (long i = 0; i + 1 < p; ++i) tnorm = std::max(tnorm, 2.0 * std::fabs(e[i]));
   struct Blk { int a, n; };
   std::vector<Blk> blocks;
-  for (int a = 0; a < P; a += LEAF) blocks.push_back(Blk{a, std::min(LEAF, P - a)});
+  {  // ceil(P / LEAF) leaves of nearly equal size (the slowest leaf sets the time)
+    const int nl = (P + LEAF - 1) / LEAF;
+    for (int b = 0, a = 0; b < nl; ++b) {
+      const int nb = P / nl + (b < P % nl ? 1 : 0);
+      blocks.push_back(Blk{a, nb});
+      a += nb;
+    }
+  }
   std::vector<double> dt = d;  // torn diagonal
   for (size_t b = 1; b < blocks.size(); ++b) {
     const int k = blocks[b].a;
