@@ -166,6 +166,15 @@ int tn_env_set_noise_channels(tn_env* env, const int* skip_left, const int* skip
 int tn_env_phi_dims(tn_env* env, long* dims4, long* n);
 /* product(sysenv, v) at the current position and nsite; v, out on device */
 int tn_env_matvec(tn_env* env, const void* v_dev, void* out_dev, int precision);
+/* Allocation-free FP64 product (no counterpart in the reference): the
+ * product of tn_env_matvec with TN_PREC_FP64 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * It calls no hipMalloc/hipFree; its intermediates live in one fixed 32 MiB
+ * device array per GPU, so (w_l + w_r) * chil * d_p * d_{p+1} * chir elements
+ * plus split-K partial sums must fit (else TN_E_INVALID).  Needs position! at
+ * the current position (nsite 1 or 2) and no penalty projectors (else
+ * TN_E_UNSUPPORTED).  Calls that overlap on one GPU race on that array. */
+int tn_env_matvec_noalloc(tn_env* env, const void* v_dev, void* out_dev);
 /* phi = A_pos * A_{pos+1} (update_site.jl:46) into phi_dev */
 int tn_env_contract_phi(tn_env* env, int pos, void* phi_dev);
 /* orthogonalize!(psi, site): exact (untruncated) gauge move on the device */

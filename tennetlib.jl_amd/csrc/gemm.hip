@@ -645,8 +645,9 @@ static void gemm_impl(tn_ctx* c, Prec p, const GemmDesc& g);
 // Few output tiles with a long K (K-reductions over whole tensors, Gram
 // matrices of tall matrices) leave most of the chip idle: split K into
 // chunks computed as one batched GEMM into FP64/FP32 partials, then reduce.
-static bool gemm_split_k(tn_ctx* c, Prec p, const GemmDesc& g) {
-  if (!(p == Prec::F64 || p == Prec::F32) || g.batch != 1) return false;
+// Chunk length along K of the split (0: no split).
+static long split_chunk(Prec p, const GemmDesc& g) {
+  if (!(p == Prec::F64 || p == Prec::F32) || g.batch != 1) return 0;
   const long tiles = ((g.M + 127) / 128) * ((g.N + 127) / 128);
   long P0;
   if (tiles < 64 && g.K >= 4096) {
@@ -656,15 +657,35 @@ static bool gemm_split_k(tn_ctx* c, Prec p, const GemmDesc& g) {
     // p of a few hundred): fewer than 64 tiles of 64 x 64 and K >= 256 run as
     // chunks of at least 64 along K
     const long t64 = ((g.M + 63) / 64) * ((g.N + 63) / 64);
-    if (t64 >= 64 || g.K < 256) return false;
+    if (t64 >= 64 || g.K < 256) return 0;
     P0 = std::min<long>(g.K / 64, 128 / t64);
   }
-  if (P0 < 2) return false;
-  const long Kc = ((g.K + P0 - 1) / P0 + 15) / 16 * 16;
+  if (P0 < 2) return 0;
+  return ((g.K + P0 - 1) / P0 + 15) / 16 * 16;
+}
+
+size_t gemm_split_bytes(Prec p, const GemmDesc& g) {
+  if (g.M <= 0 || g.N <= 0) return 0;
+  const long Kc = split_chunk(p, g);
+  if (Kc == 0) return 0;
+  const long P = (g.K + Kc - 1) / Kc;
+  return (size_t)P * g.M * g.N * ((p == Prec::F64) ? 8 : 4);
+}
+
+static bool gemm_split_k(tn_ctx* c, Prec p, const GemmDesc& g) {
+  const long Kc = split_chunk(p, g);
+  if (Kc == 0) return false;
   const long P = (g.K + Kc - 1) / Kc;
   const long full = g.K / Kc;  // chunks of exactly Kc
   const size_t es = (p == Prec::F64) ? 8 : 4;
-  char* part = (char*)c->workspace(WS_GEMM_SPLIT, (size_t)P * g.M * g.N * es);
+  const size_t need = (size_t)P * g.M * g.N * es;
+  char* part;
+  if (g.split_ws) {
+    TN_REQUIRE(need <= g.split_cap, TN_E_INVALID, "gemm: split-K buffer too small");
+    part = (char*)g.split_ws;
+  } else {
+    part = (char*)c->workspace(WS_GEMM_SPLIT, need);
+  }
   GemmDesc q = g;
   q.alpha = 1.0;
   q.beta = 0.0;

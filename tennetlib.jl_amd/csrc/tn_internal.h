@@ -121,6 +121,9 @@ struct tn_ctx {
   int open_timers = 0;
   const double* ones = nullptr;  // device scalar 1.0 (edge environments)
   tn::DevBuf ones_buf;
+  // device address of this device's fixed scratch array (allocation-free
+  // product), taken once per device at context creation
+  void* scratch_dev = nullptr;
 
   void* workspace(int slot, size_t bytes) {
     if ((int)ws.size() <= slot) ws.resize(slot + 1);
@@ -199,8 +202,14 @@ struct GemmDesc {
   void* C = nullptr; long ldc = 0; long sC = 0;
   int batch = 1;
   double alpha = 1.0, beta = 0.0;
+  // split-K partial products: a caller-owned device buffer of split_cap bytes
+  // (see gemm_split_bytes) used instead of the context workspace when set
+  void* split_ws = nullptr;
+  size_t split_cap = 0;
 };
 void gemm(tn_ctx* c, Prec p, const GemmDesc& g);
+// bytes of split-K partial products gemm() needs for g (0: no split)
+size_t gemm_split_bytes(Prec p, const GemmDesc& g);
 // FP64 grouped GEMM: for every output group g < ngroups and batch z < batch,
 // C_g[z] (M x N, row stride ldc) = sum over terms t in [off[g], off[g+1]) of
 // w[t] * A_{ia[t]}[z] (M x K, lda) * B_{ib[t]}[z] (K x N, ldb), with

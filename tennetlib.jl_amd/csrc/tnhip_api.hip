@@ -358,6 +358,10 @@ void position(tn_env* e, int pos) {
   makeL(e, pos - 1);
   makeR(e, pos + e->nsite);
   if (!e->proj.empty()) proj_position(e, pos);
+  // the W-step operator of this position is built here, so that products at
+  // this position (including the allocation-free one) upload nothing
+  if (e->nsite == 2) (void)op_two(e, pos);
+  else if (e->nsite == 1) (void)op_left(e, pos);
 }
 
 // overlap environments of a projector (mirror projmps2.jl:77-149):
@@ -867,6 +871,95 @@ void matvec(tn_env* e, const void* v, void* out, int prec) {
 
 void apply_op(tn_env* e, const double* x, double* y, int prec) { matvec(e, x, y, prec); }
 
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// allocation-free FP64 product
+// ---------------------------------------------------------------------------
+// One fixed array per GPU holds the product's intermediates (T1, T2 and the
+// split-K partial sums).  In host code `scratch` is only a symbol: its device
+// address comes from hipGetSymbolAddress, once per device, at context creation.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_addr;  // per device
+
+// the device address of `scratch` on `device`, which must be current
+void* scratch_address(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_addr.size() <= device) g_scratch_addr.resize(device + 1, nullptr);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    TN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TN_REQUIRE(p != nullptr, TN_E_HIP, "hipGetSymbolAddress returned a null scratch address");
+    g_scratch_addr[device] = p;
+  }
+  return g_scratch_addr[device];
+}
+
+inline size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+// product(sysenv, v) in FP64 with every intermediate in the fixed scratch:
+// the same GEMM / W-step / GEMM chain as matvec_mpo, no device allocation,
+// no host-device copy.  Needs position! at the current position (which builds
+// the W-step operator) and no penalty projectors.
+void matvec_noalloc(tn_env* e, const void* v, void* out) {
+  tn_ctx* c = e->ctx;
+  TN_REQUIRE(c->scratch_dev != nullptr, TN_E_HIP, "device scratch address not initialised");
+  TN_REQUIRE(e->proj.empty(), TN_E_UNSUPPORTED,
+             "allocation-free product: environments with penalty projectors are not supported");
+  TN_REQUIRE(e->nsite == 1 || e->nsite == 2, TN_E_UNSUPPORTED, "allocation-free product: nsite must be 1 or 2");
+  TN_REQUIRE(v != nullptr && out != nullptr, TN_E_INVALID, "null vector");
+  const Dom D = domain(e);
+  const int pos = D.pos;
+  const EnvT& L = e->LR[pos - 1];
+  const EnvT& R = e->LR[pos + e->nsite];
+  const long chil = D.chil, chir = D.chir;
+  const int DD = D.d1 * D.d2;
+  const int w0 = e->w[pos - 1];
+  const int w2 = e->w[pos - 1 + e->nsite];
+  TN_REQUIRE(L.chi == chil && L.w == w0 && R.chi == chir && R.w == w2, TN_E_SHAPE,
+             "The order of the ProjMPO-ITensor product P*v is not equal to the order of the ITensor v: "
+             "environment and wave-function shapes disagree");
+  const bool built = (e->nsite == 2) ? e->op2_ok[pos - 1] : e->opL_ok[pos - 1];
+  TN_REQUIRE(built, TN_E_POSITION, "allocation-free product: operator of this position not built (call position!)");
+  const SparseOp& op = (e->nsite == 2) ? e->op2[pos - 1] : e->opL[pos - 1];
+  const bool cx = e->cplx();
+  const Prec P = cx ? Prec::Z64 : Prec::F64;
+  const int m2 = cx ? 2 : 1;
+  const size_t t1 = (size_t)w0 * chil * DD * chir * e->esz;
+  const size_t t2 = (size_t)w2 * chil * DD * chir * e->esz;
+  char* base = (char*)c->scratch_dev;
+  GemmDesc g1;
+  g1.M = (long)w0 * chil; g1.N = (long)DD * chir; g1.K = chil;
+  g1.A = L.buf.p; g1.lda = chil;
+  g1.B = v; g1.ldb = (long)DD * chir;
+  g1.C = base; g1.ldc = (long)DD * chir;
+  GemmDesc g3;
+  g3.M = chil * DD; g3.N = chir; g3.K = (long)w2 * chir;
+  g3.A = base + align256(t1); g3.lda = (long)w2 * chir;
+  g3.B = R.buf.p; g3.ldb = chir;
+  g3.C = out; g3.ldc = chir;
+  const size_t split = std::max(gemm_split_bytes(P, g1), gemm_split_bytes(P, g3));
+  const size_t need = align256(t1) + align256(t2) + align256(split);
+  TN_REQUIRE(need <= NOALLOC_SCRATCH_BYTES, TN_E_INVALID,
+             "allocation-free product needs " + std::to_string(need) + " bytes of scratch, more than the fixed " +
+                 std::to_string(NOALLOC_SCRATCH_BYTES));
+  g1.split_ws = g3.split_ws = base + align256(t1) + align256(t2);
+  g1.split_cap = g3.split_cap = split;
+  gemm(c, P, g1);
+  WStepDesc ws;
+  ws.G = w0; ws.Gp = w2; ws.D = DD; ws.Dp = DD;
+  ws.nA = chil; ws.nB = chir * m2;
+  ws.in = base; ws.in_sg = chil * DD * chir * m2; ws.in_sA = DD * chir * m2; ws.in_sS = chir * m2;
+  ws.out = base + align256(t1); ws.out_sg = chir * m2; ws.out_sA = (long)DD * w2 * chir * m2;
+  ws.out_sS = (long)w2 * chir * m2;
+  ws.op = &op;
+  wstep(c, false, ws);
+  gemm(c, P, g3);
+}
 }  // namespace
 
 // eigsolve(A, x0, 1, :SR; krylovdim, maxiter, tol, eager=false) (src/base/solver.jl:36)
@@ -1975,6 +2068,7 @@ int tn_ctx_create(int device, tn_ctx** out) {
     c->ones = c->ones_buf.as<double>();
     TN_HIP(hipHostMalloc((void**)&c->pinned, tn_ctx::PINNED_N * sizeof(double), hipHostMallocMapped));
     TN_HIP(hipHostGetDevicePointer((void**)&c->pinned_dev, c->pinned, 0));
+    c->scratch_dev = scratch_address(device);
     const char* tv = std::getenv("TNHIP_TIMING");
     c->timing = tv && tv[0] == '1';
     *out = c;
@@ -2299,6 +2393,14 @@ int tn_env_matvec(tn_env* e, const void* v, void* out, int precision) {
     TN_HIP(hipStreamSynchronize(e->ctx->stream));
   });
 }
+int tn_env_matvec_noalloc(tn_env* e, const void* v, void* out) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    matvec_noalloc(e, v, out);
+    TN_HIP(hipStreamSynchronize(e->ctx->stream));
+  });
+}
+
 int tn_env_contract_phi(tn_env* e, int pos, void* phi) {
   return guarded(C(e), [&] {
     check_env(e);

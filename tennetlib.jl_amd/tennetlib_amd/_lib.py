@@ -121,6 +121,7 @@ SIGNATURES = {
     "tn_ttn_fullsweep": [_P, _PI, _I, _I, _L, _L, _D, _D, _I, _I, _I, _I, _I, _D, _I, _I, _PD, _PL],
     "tn_env_phi_dims": [_P, _PL, _PL],
     "tn_env_matvec": [_P, _P, _P, _I],
+    "tn_env_matvec_noalloc": [_P, _P, _P],
     "tn_env_contract_phi": [_P, _I, _P],
     "tn_env_orthogonalize": [_P, _I],
     "tn_krylov_eig": [_P, _P, _I, _I, _D, _I, _PD, C.POINTER(KrylovInfo)],
