@@ -1168,6 +1168,23 @@ cplx rayleigh(tn_env* e, const double* phi) {
   return cplx(h[0], cx ? h[1] : 0.0) / h[4];
 }
 
+// Energy of a reduced-precision eig_solver result phi (Ritz value th):
+// refine_fp64 = 1 takes the Rayleigh quotient of one FP64 product; k >= 2
+// continues with an FP64 Lanczos of dimension k started from phi (one
+// restart cycle, KrylovKit's Ritz-value semantics, solver.jl:36-42), which
+// also moves phi to that FP64 Ritz vector.
+double refine_energy(tn_env* e, double* phi, const tn_update_params* p, tn_krylov_info* info, double th) {
+  if (!p->refine_fp64 || p->precision == TN_PREC_FP64) return th;
+  if (p->refine_fp64 == 1) return rayleigh(e, phi).real();
+  tn_krylov_info ri{};
+  const double th64 = krylov_eig(e, phi, p->refine_fp64, 1, p->tol, TN_PREC_FP64, &ri);
+  if (info) {
+    info->numops += ri.numops;
+    info->residual = ri.residual;
+  }
+  return th64;
+}
+
 // phi = A_pos * A_{pos+1}
 void contract_phi(tn_env* e, int pos, void* phi) {
   TN_REQUIRE(pos >= 1 && pos < e->N, TN_E_POSITION, "contract_phi: bond out of range");
@@ -1485,8 +1502,7 @@ void update_two_site(tn_env* e, int pos, bool ortho_left, const tn_update_params
     const double th = krylov_eig(e, phi, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
     if (p->check_convergence && res.info.converged < 1)
       throw Error(TN_E_NOTCONV, "`eig_solver()` not converged !!");
-    energy = th;
-    if (p->refine_fp64 && p->precision != TN_PREC_FP64) energy = rayleigh(e, phi);
+    energy = refine_energy(e, phi, p, &res.info, th);
   } else {
     krylov_expm(e, phi, t, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
     if (p->check_convergence && res.info.converged < 1)
@@ -1629,8 +1645,7 @@ void update_one_site(tn_env* e, int pos, bool ortho_left, const tn_update_params
     const double th = krylov_eig(e, phi, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
     if (p->check_convergence && res.info.converged < 1)
       throw Error(TN_E_NOTCONV, "`eig_solver()` not converged !!");
-    energy = th;
-    if (p->refine_fp64 && p->precision != TN_PREC_FP64) energy = rayleigh(e, phi);
+    energy = refine_energy(e, phi, p, &res.info, th);
   } else {
     krylov_expm(e, phi, t, p->krylovdim, p->maxiter, p->tol, p->precision, &res.info);
     if (p->check_convergence && res.info.converged < 1)

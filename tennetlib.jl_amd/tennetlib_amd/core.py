@@ -34,6 +34,8 @@ from .couplingmodel import CouplingModel, coupling_mpo, mpo_sum_ids
 
 FLOAT64_THRESHOLD = 1e-15  # src/base/global_variables.jl:10
 TYPEMAX_INT = 2 ** 63 - 1
+# FP64 Lanczos dimension that refines a BF16x3 eig_solver result
+BF16X3_REFINE_KDIM = 2
 
 
 # ---------------------------------------------------------------------------
@@ -364,7 +366,14 @@ def _params(solver, kwargs) -> L.UpdateParams:
     p.tol = float(kwargs.get("solver_tol", p.tol))
     p.check_convergence = int(kwargs.get("solver_check_convergence", False))
     p.precision = _precision_code(kwargs.get("precision", "fp64"))
-    p.refine_fp64 = int(kwargs.get("refine_fp64", True))
+    # energy of a reduced-precision solve: 1 = Rayleigh quotient of one FP64
+    # product (FP32 form), k >= 2 = an FP64 Lanczos of dimension k from the
+    # mixed-precision Ritz vector (BF16x3 form, whose ~1e-5 product error
+    # otherwise stays in the state)
+    ref = kwargs.get("refine_fp64", None)
+    if ref is None:
+        ref = BF16X3_REFINE_KDIM if p.precision == L.TN_PREC_BF16X3 else 1
+    p.refine_fp64 = int(ref)
     noise = float(kwargs.get("noise", 0.0))
     if noise > 0 and p.reverse_step:
         raise ValueError(f"`updatePosition()` :: `noise={noise}` cannot be greater than zero for "

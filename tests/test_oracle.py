@@ -526,3 +526,22 @@ def test_tdvp_xx_chain_matches_free_fermions():
     assert np.abs(_sz_expect(env.psi) - (n - 0.5)).max() < 1e-8
     # <H> conserved (Neel energy of the XX chain is 0)
     assert abs(sw.energy[-1]) < 1e-8
+
+
+def test_entropy_closed_forms():
+    """_entropy (measure.jl:4-20) against closed forms: one weight carries no
+    entropy, k equal weights give ln k (natural log, weights p not
+    amplitudes), zero weights are skipped, and an input whose sum is off by
+    more than 100 * Float64_threshold is renormalised first."""
+    assert O.entropy(np.array([1.0])) == 0.0
+    for k in (2, 3, 7, 64):
+        assert abs(O.entropy(np.full(k, 1.0 / k)) - np.log(k)) < 1e-14
+    assert abs(O.entropy(np.array([0.5, 0.5, 0.0, 0.0])) - np.log(2)) < 1e-15
+    p = np.array([0.7, 0.2, 0.1])
+    want = -(0.7 * np.log(0.7) + 0.2 * np.log(0.2) + 0.1 * np.log(0.1))
+    assert abs(O.entropy(p) - want) < 1e-15
+    # renormalisation branch: 3 p sums to 3, far from 1
+    assert abs(O.entropy(3.0 * p) - want) < 1e-15
+    # within 100 * Float64_threshold of 1: used as given (no renormalisation)
+    q = p * (1.0 + 50 * O.FLOAT64_THRESHOLD)
+    assert O.entropy(q) == float(-sum(x * np.log(x) for x in q))

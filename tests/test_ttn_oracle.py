@@ -77,6 +77,26 @@ def test_default_graph_structure(N):
     assert set(path) == set(edges) and len(path) == len(edges)
 
 
+def test_default_sweeppath_hand_enumerated():
+    """Paths worked out by hand from sweep_ttn.jl:49-65 and ttn_generators.jl:3-94.
+
+    N = 8: 8 sites in pairs on (1,1)..(1,4); nlayers = 3.  Layer 2 runs
+    forward ((3 - 2) odd), layer 1 backward ((3 - 1) even).
+    N = 13: _distribute_site_positions gives the pairs (1,0) (1,1) (1,0) (1,1)
+    (1,1) (1,1) (1,0) (1,1); a pair with a 0 has no layer-1 node (its site
+    hangs on layer 2), so layer 1 holds (1,2) (1,4) (1,5) (1,6) (1,8).  With
+    nlayers = 4, layer 3 runs forward, layer 2 backward, layer 1 forward with
+    the missing nodes skipped."""
+    edges8, sn8 = TO.default_graph_sitenodes(8)
+    assert TO.default_sweeppath(8, edges8) == [(2, 1), (2, 2), (1, 4), (1, 3), (1, 2), (1, 1)]
+    edges13, sn13 = TO.default_graph_sitenodes(13)
+    assert TO.default_sweeppath(13, edges13) == [(3, 1), (3, 2), (2, 4), (2, 3), (2, 2), (2, 1),
+                                                 (1, 2), (1, 4), (1, 5), (1, 6), (1, 8)]
+    # sites on layer 2 where a layer-1 pair is half empty
+    assert sn13[1] == (2, 1) and sn13[2] == (1, 2) and sn13[3] == (1, 2) and sn13[4] == (2, 2)
+    assert sn13[10] == (1, 6) and sn13[11] == (2, 4) and sn13[12] == (1, 8) and sn13[13] == (1, 8)
+
+
 def test_default_graph_docstring_examples():
     # ttn_generators.jl:49-57
     _, sn = TO.default_graph_sitenodes(32)
