@@ -348,124 +348,232 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
   if (tid == 0) partn[g] = dot;
 }
 
-// ---- the same reduction as one cooperative launch (grid co-resident), two
-// grid barriers per column instead of two launches
-//
-// Grid barrier: one counter in device memory, raised with agent-scope
-// release atomics and polled with acquire loads; the spin is bounded and a
-// timeout raises *abort so that every workgroup leaves the kernel.
-__device__ bool grid_barrier(unsigned* ctr, unsigned target, int* abort_flag) {
-  __shared__ int ok;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    ok = 1;
-    __threadfence();
-    __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-    long spins = 0;
-    while (__hip_atomic_load(ctr, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1L << 22)) {
-        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = 0;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
+// ---- blocked reduction for large p (LAPACK dlatrd panels of NBT columns).
+// Within a panel starting at column j0 the trailing block is left as it was
+// at j0; column j's row and its matrix-vector product are corrected with the
+// panel's earlier reflectors v_i and vectors w_i (i = 0..k-1, k = j - j0):
+//   row_j   = A[j, j+1:] - sum_i (v_i[j] w_i + w_i[j] v_i)
+//   y_j     = tau_j (A22 v_j - sum_i (v_i (w_i^T v_j) + w_i (v_i^T v_j)))
+//   w_j     = y_j - (tau_j / 2)(y_j^T v_j) v_j
+// and the panel ends with one MFMA GEMM, A[j1:, j1:] -= V W^T + W V^T.  Each
+// column reads the trailing block once (the product), not twice (product and
+// rank-2 update).  Vectors are stored by global row in the panel buffers
+// P = [V; W] and Q = [W; V] (2 NBT rows of length p; unused rows zero).
+constexpr int NBT = 32;
+constexpr int RCHUNK = 512;  // row entries per workgroup in trd_panel_row_kernel
+
+// Column j: forms w_{k-1} (the previous column's, from y_{j-1} and the
+// partial sums of y_{j-1}^T v_{j-1}), then the current row x (stored in xbuf,
+// entries c = 0..m-1 for columns j+1+c) with partial sums of sum_{c>=1} x_c^2,
+// v_i . x and w_i . x over c >= 1 (i < k); d_j from workgroup 0.
+// Grid: ceil(m / RCHUNK) workgroups.
+__global__ __launch_bounds__(256) void trd_panel_row_kernel(const double* __restrict__ A, long ld, int p, int j, int k,
+                                                            double* __restrict__ P, double* __restrict__ Q,
+                                                            const double* __restrict__ yprev,
+                                                            const double* __restrict__ partB, int nB,
+                                                            const double* __restrict__ tauv,
+                                                            double* __restrict__ xbuf, double* __restrict__ partA,
+                                                            double* __restrict__ dv) {
+  __shared__ double sv[NBT], sw[NBT];  // v_i[j], w_i[j]
+  __shared__ double red[4];
+  __shared__ double accs[4][2 * NBT + 1];
+  const int m = p - j - 1, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // w_{k-1} = y_{j-1} - hs v_{k-1} over rows >= j
+  double hs = 0.0;
+  if (k >= 1) {
+    double s = 0.0;
+    for (int i = tid; i < nB; i += 256) s += partB[i];
+    s = block_sum(s, red);
+    hs = 0.5 * tauv[j - 1] * s;
+  }
+  if (tid < k) {
+    sv[tid] = P[(long)tid * p + j];
+    if (tid < k - 1) sw[tid] = P[(long)(NBT + tid) * p + j];
+  }
+  if (k >= 1 && tid == 0) {
+    const double wj = yprev[0] - hs * P[(long)(k - 1) * p + j];
+    sw[k - 1] = wj;
+    if (blockIdx.x == 0) {
+      P[(long)(NBT + k - 1) * p + j] = wj;
+      Q[(long)(k - 1) * p + j] = wj;
     }
-    __threadfence();
   }
   __syncthreads();
-  return ok != 0;
+  double acc[2 * NBT + 1];
+#pragma unroll
+  for (int q = 0; q < 2 * NBT + 1; ++q) acc[q] = 0.0;
+  const int c0 = blockIdx.x * RCHUNK;
+  for (int c = c0 + tid; c < min(m, c0 + RCHUNK); c += 256) {
+    const long r = j + 1 + c;  // global row / column index
+    if (k >= 1) {
+      const double wn = yprev[c + 1] - hs * P[(long)(k - 1) * p + r];
+      P[(long)(NBT + k - 1) * p + r] = wn;
+      Q[(long)(k - 1) * p + r] = wn;
+    }
+    double x = A[(long)j * ld + r];
+    for (int i = 0; i < k; ++i) x -= sv[i] * P[(long)(NBT + i) * p + r] + sw[i] * P[(long)i * p + r];
+    xbuf[c] = x;
+    if (c >= 1) {
+      acc[0] += x * x;
+#pragma unroll
+      for (int i = 0; i < NBT; ++i) {
+        if (i < k) {
+          acc[1 + i] += P[(long)i * p + r] * x;
+          acc[1 + NBT + i] += P[(long)(NBT + i) * p + r] * x;
+        }
+      }
+    }
+  }
+  const int nq = 1 + 2 * NBT;
+#pragma unroll
+  for (int q = 0; q < nq; ++q) {
+    if (q == 0 || (q - 1) % NBT < k) {
+      const double t = wave_sum64(acc[q]);
+      if (lane == 0) accs[wave][q] = t;
+    }
+  }
+  __syncthreads();
+  for (int q = tid; q < nq; q += 256)
+    if (q == 0 || (q - 1) % NBT < k)
+      partA[(long)blockIdx.x * nq + q] = (accs[0][q] + accs[1][q]) + (accs[2][q] + accs[3][q]);
+  if (blockIdx.x == 0 && tid == 0) {
+    double dj = A[(long)j * ld + j];
+    for (int i = 0; i < k; ++i) dj -= 2.0 * sv[i] * sw[i];
+    dv[j] = dj;
+  }
 }
 
-// dynamic LDS: 2 m0 doubles (v and w of the current column), m0 = p - 1
-__global__ __launch_bounds__(256) void trd_coop_kernel(double* __restrict__ A, long ld, int p,
-                                                       double* __restrict__ y, double* __restrict__ part,
-                                                       double* __restrict__ dv, double* __restrict__ ev,
-                                                       double* __restrict__ tauv, unsigned* __restrict__ bar,
-                                                       int* __restrict__ abort_flag) {
-  extern __shared__ double lds[];
+// Column j: the reflector from the partial sums and xbuf (every workgroup),
+// y_j for this workgroup's rows from one pass over the stored trailing
+// block, partial sums of y_j . v_j; workgroup 0 stores v_j (below the
+// diagonal of column j and in P, Q), e_j and tau_j.  Dynamic LDS: m doubles.
+__global__ __launch_bounds__(256) void trd_panel_symv_kernel(double* __restrict__ A, long ld, int p, int j, int k,
+                                                             double* __restrict__ P, double* __restrict__ Q,
+                                                             const double* __restrict__ xbuf,
+                                                             const double* __restrict__ partA, int nA,
+                                                             double* __restrict__ y, double* __restrict__ partB,
+                                                             double* __restrict__ ev, double* __restrict__ tauv) {
+  extern __shared__ double vs[];
   __shared__ double red[4];
-  double* vs = lds;
-  double* ws = lds + (p - 1);
-  const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  unsigned target = 0;
-  for (int j = 0; j < p - 2; ++j) {
-    const int m = p - j - 1;
-    // phase A: reflector from row j (every workgroup), rows of y = tau G22 v
-    const double* x = A + (long)j * ld + (j + 1);
-    double ss = 0.0;
-    for (int c = tid; c < m; c += 256) {
-      const double t = x[c];
-      vs[c] = t;
-      if (c > 0) ss += t * t;
-    }
-    ss = block_sum(ss, red);
-    const double x0 = vs[0];
-    double tau = 0.0, alpha = x0, scale = 0.0;
-    if (ss > 0.0) {
-      const double nrm = sqrt(x0 * x0 + ss);
-      alpha = (x0 >= 0.0) ? -nrm : nrm;
-      tau = (alpha - x0) / alpha;
-      scale = 1.0 / (x0 - alpha);
-    }
-    __syncthreads();
-    for (int c = tid; c < m; c += 256) vs[c] = (c == 0) ? 1.0 : vs[c] * scale;
-    __syncthreads();
-    if (g == 0) {
-      for (int c = tid; c < m; c += 256) A[(long)(j + 1 + c) * ld + j] = vs[c];
-      if (tid == 0) {
-        dv[j] = A[(long)j * ld + j];
-        ev[j] = alpha;
-        tauv[j] = tau;
-      }
-    }
-    double dot = 0.0;
-    if (tau != 0.0) {
-      for (int r = g * 4 + wave; r < m; r += G * 4) {
-        const double* row = A + (long)(j + 1 + r) * ld + (j + 1);
-        double s = 0.0;
-        for (int c = lane; c < m; c += 64) s += row[c] * vs[c];
-        for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
-        if (lane == 0) {
-          const double yr = tau * s;
-          y[r] = yr;
-          dot += yr * vs[r];
-        }
-      }
-    }
-    dot = block_sum(dot, red);
-    if (tid == 0) part[g] = dot;
-    target += G;
-    if (!grid_barrier(bar, target, abort_flag)) return;
-    // phase B: G22 -= v w^T + w v^T, w = y - (tau/2)(y^T v) v
-    if (tau != 0.0) {
-      double s = 0.0;
-      for (int i = tid; i < G; i += 256) s += part[i];
-      s = block_sum(s, red);
-      const double hs = 0.5 * tau * s;
-      for (int c = tid; c < m; c += 256) ws[c] = y[c] - hs * vs[c];
-      __syncthreads();
-      const int T = (m + 63) / 64;
-      for (int t = g; t < T * T; t += G) {
-        const int tr0 = (t / T) * 64, tc0 = (t % T) * 64;
-        const int c = tc0 + (tid & 63);
-        if (c < m) {
-          const double vc = vs[c], wc = ws[c];
-          for (int rr = tid >> 6; rr < 64; rr += 4) {
-            const int r = tr0 + rr;
-            if (r >= m) break;
-            double* a = A + (long)(j + 1 + r) * ld + (j + 1 + c);
-            *a -= vs[r] * wc + ws[r] * vc;
-          }
-        }
-      }
-    }
-    target += G;
-    if (!grid_barrier(bar, target, abort_flag)) return;
+  __shared__ double sums[2 * NBT + 1];
+  __shared__ double cvw[2 * NBT];  // (v_i . v_j), (w_i . v_j)
+  const int m = p - j - 1, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int G = gridDim.x, g = blockIdx.x;
+  const int nq = 1 + 2 * NBT;
+  for (int q = tid; q < nq; q += 256) {
+    double s = 0.0;
+    if (q == 0 || (q - 1) % NBT < k)
+      for (int b = 0; b < nA; ++b) s += partA[(long)b * nq + q];
+    sums[q] = s;
   }
-  if (g == 0 && tid == 0) {
-    dv[p - 2] = A[(long)(p - 2) * ld + (p - 2)];
-    ev[p - 2] = A[(long)(p - 2) * ld + (p - 1)];
-    dv[p - 1] = A[(long)(p - 1) * ld + (p - 1)];
+  __syncthreads();
+  const double x0 = xbuf[0], ss = sums[0];
+  double tau = 0.0, alpha = x0, scale = 0.0;
+  if (ss > 0.0) {
+    const double nrm = sqrt(x0 * x0 + ss);
+    alpha = (x0 >= 0.0) ? -nrm : nrm;
+    tau = (alpha - x0) / alpha;
+    scale = 1.0 / (x0 - alpha);
+  }
+  for (int c = tid; c < m; c += 256) vs[c] = (c == 0) ? 1.0 : xbuf[c] * scale;
+  if (tid < k) {
+    const long r1 = j + 1;
+    cvw[tid] = sums[1 + tid] * scale + P[(long)tid * p + r1];                    // v_i . v_j
+    cvw[NBT + tid] = sums[1 + NBT + tid] * scale + P[(long)(NBT + tid) * p + r1];  // w_i . v_j
+  }
+  __syncthreads();
+  if (g == 0) {
+    for (int c = tid; c < m; c += 256) {
+      const long r = j + 1 + c;
+      A[r * ld + j] = vs[c];
+      P[(long)k * p + r] = vs[c];
+      Q[(long)(NBT + k) * p + r] = vs[c];
+    }
+    if (tid == 0) {
+      ev[j] = alpha;
+      tauv[j] = tau;
+    }
+  }
+  // two rows per wave, 16-byte loads (a leading odd element is taken alone)
+  double dot = 0.0;
+  for (int rr0 = (g * 4 + wave) * 2; rr0 < m; rr0 += G * 8) {
+    const bool two = rr0 + 1 < m;
+    const long r0 = j + 1 + rr0;
+    const double* row0 = A + r0 * ld + (j + 1);
+    const double* row1 = two ? row0 + ld : row0;
+    double s0 = 0.0, s1 = 0.0;
+    const bool pair1 = two && (((uintptr_t)row1 & 15) == ((uintptr_t)row0 & 15));
+    int c0 = 0;
+    if (((uintptr_t)row0 & 15) != 0) {
+      if (lane == 0) {
+        s0 = row0[0] * vs[0];
+        if (pair1) s1 = row1[0] * vs[0];
+      }
+      c0 = 1;
+    }
+    const int nv = (m - c0) >> 1;
+    const double2* a0 = reinterpret_cast<const double2*>(row0 + c0);
+    const double2* a1 = reinterpret_cast<const double2*>(row1 + c0);
+#pragma unroll 4
+    for (int t = lane; t < nv; t += 64) {
+      const double2 x0 = a0[t];
+      const double va = vs[c0 + 2 * t], vb = vs[c0 + 2 * t + 1];
+      s0 += x0.x * va + x0.y * vb;
+      if (pair1) {
+        const double2 x1 = a1[t];
+        s1 += x1.x * va + x1.y * vb;
+      }
+    }
+    if (two && !pair1)  // ld odd: the second row is not 16-byte aligned with the first
+      for (int c = lane; c < m; c += 64) s1 += row1[c] * vs[c];
+    if (((m - c0) & 1) && lane == 0) {
+      const int c = m - 1;
+      s0 += row0[c] * vs[c];
+      if (pair1) s1 += row1[c] * vs[c];
+    }
+    s0 = wave_sum64(s0);
+    s1 = wave_sum64(s1);
+    // panel corrections sum_i (v_i[r] (w_i . v) + w_i[r] (v_i . v)), one term per lane
+    double q0 = 0.0, q1 = 0.0;
+    if (lane < k) {
+      const double* vi = P + (long)lane * p;
+      const double* wi = P + (long)(NBT + lane) * p;
+      q0 = vi[r0] * cvw[NBT + lane] + wi[r0] * cvw[lane];
+      if (two) q1 = vi[r0 + 1] * cvw[NBT + lane] + wi[r0 + 1] * cvw[lane];
+    }
+    q0 = wave_sum64(q0);
+    q1 = wave_sum64(q1);
+    if (lane == 0) {
+      const double y0 = tau * (s0 - q0);
+      y[rr0] = y0;
+      dot += y0 * vs[rr0];
+      if (two) {
+        const double y1 = tau * (s1 - q1);
+        y[rr0 + 1] = y1;
+        dot += y1 * vs[rr0 + 1];
+      }
+    }
+  }
+  dot = block_sum(dot, red);
+  if (tid == 0) partB[g] = dot;
+}
+
+// w of the panel's last column j (rows >= j + 1), index k in the panel.
+__global__ __launch_bounds__(256) void trd_panel_wlast_kernel(int p, int j, int k, double* __restrict__ P,
+                                                              double* __restrict__ Q, const double* __restrict__ y,
+                                                              const double* __restrict__ partB, int nB,
+                                                              const double* __restrict__ tauv) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nB; i += 256) s += partB[i];
+  s = block_sum(s, red);
+  const double hs = 0.5 * tauv[j] * s;
+  const int m = p - j - 1;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < m; c += gridDim.x * 256) {
+    const long r = j + 1 + c;
+    const double wn = y[c] - hs * P[(long)k * p + r];
+    P[(long)(NBT + k) * p + r] = wn;
+    Q[(long)k * p + r] = wn;
   }
 }
 
@@ -1532,10 +1640,18 @@ void wy_tsolve(tn_ctx* c, const double* S, const double* tauv, int j0, int nb, d
   TN_HIP(hipGetLastError());
 }
 
+// smallest p reduced in panels (below it one fused launch per column wins)
+static int trd_blocked_min() {
+  static const int v = [] {
+    const char* e = std::getenv("TNHIP_TRD_BLOCKED_MIN");
+    return e ? std::atoi(e) : 3072;
+  }();
+  return v;
+}
+
 void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w) {
   TN_REQUIRE(p >= 3 && p <= 16384, TN_E_INVALID, "sym_eig_dc: size out of range");
   const int P = (int)p;
-  double* Gm = G;
   EigStages stage(c, p);
   // ---- 1. tridiagonalisation
   double* vecs = (double*)c->workspace(WS_DC_VEC, (size_t)(12 * p + 2048) * sizeof(double));
@@ -1544,40 +1660,57 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   double* tauv = ev + p;
   double* y = tauv + p;
   double* part = y + p;  // <= ceil(p / HR) entries
-  // one cooperative launch when the grid fits co-resident, else two launches per column
-  static const bool coop_env = [] {
-    const char* e = std::getenv("TNHIP_TRD_COOP");  // measured slower than per-column launches
-    return e && std::atoi(e) == 1;
-  }();
+  // panels of NBT columns (one pass over the trailing block per column) from
+  // p = trd_blocked_min() on; below it one fused launch per column
+  const int jt = std::max(0, P - TAIL);  // columns jt.. are reduced in LDS
   bool done = false;
-  if (coop_env && p <= 8192) {
-    int dev = 0, coop = 0, ncu = 0;
-    TN_HIP(hipGetDevice(&dev));
-    TN_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
-    TN_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t shm = 2 * (size_t)(p - 1) * sizeof(double);
-    if (shm > 64 * 1024)
-      raise_lds_limit((const void*)trd_coop_kernel, (size_t)shm);
-    int per_cu = 0;
-    TN_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)trd_coop_kernel, 256, shm));
-    const long want = std::min<long>(256, std::max<long>(8, (p - 1 + 7) / 8));
-    const long G = std::min<long>(want, (long)per_cu * ncu);
-    if (coop && G >= 1 && per_cu >= 1) {
-      unsigned* bar = (unsigned*)(vecs + 6 * p + 256);  // past the partial sums (<= 2p + 256)
-      int* abort_flag = (int*)(bar + 2);
-      fill_f64(c, (double*)bar, 0.0, 2);
-      int P_ = P;
-      long ld_ = ldg;
-      double* A_ = Gm;
-      void* args[] = {&A_, &ld_, &P_, &y, &part, &dv, &ev, &tauv, &bar, &abort_flag};
-      TN_HIP(hipLaunchCooperativeKernel((const void*)trd_coop_kernel, dim3((unsigned)G), dim3(256), args, shm,
-                                        c->stream));
-      int hab[4] = {0, 0, 0, 0};
-      TN_HIP(hipMemcpyAsync(hab, bar, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      TN_HIP(hipStreamSynchronize(c->stream));
-      TN_REQUIRE(hab[2] == 0, TN_E_HIP, "sym_eig_dc: grid barrier timed out in the tridiagonalisation");
-      done = true;
+  if (jt >= 1 && P >= trd_blocked_min()) {
+    const int nAmax = (P + RCHUNK - 1) / RCHUNK;
+    const int nBmax = (P + 7) / 8;
+    double* Pp = (double*)c->workspace(WS_DC_PANEL, ((size_t)4 * NBT * p + p + (size_t)nAmax * (2 * NBT + 1) +
+                                                     nBmax + 64) * sizeof(double));
+    double* Qp = Pp + (size_t)2 * NBT * p;
+    double* xb = Qp + (size_t)2 * NBT * p;
+    double* partA = xb + p;
+    double* partB = partA + (size_t)nAmax * (2 * NBT + 1);
+    if ((size_t)p * sizeof(double) > 64 * 1024)
+      raise_lds_limit((const void*)trd_panel_symv_kernel, (size_t)p * sizeof(double));
+    for (int j0 = 0; j0 < jt; j0 += NBT) {
+      const int j1 = std::min(jt, j0 + NBT);
+      fill_f64(c, Pp, 0.0, (long)4 * NBT * p);
+      int nBprev = 0;
+      for (int j = j0; j < j1; ++j) {
+        const int k = j - j0, m = P - j - 1;
+        const int nA = (m + RCHUNK - 1) / RCHUNK;
+        hipLaunchKernelGGL(trd_panel_row_kernel, dim3(nA), dim3(256), 0, c->stream, (const double*)G, ldg, P, j, k,
+                           Pp, Qp, (const double*)y, (const double*)partB, nBprev, (const double*)tauv, xb, partA,
+                           dv);
+        const int nB = (m + 7) / 8;  // two rows per wave
+        hipLaunchKernelGGL(trd_panel_symv_kernel, dim3(nB), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg,
+                           P, j, k, Pp, Qp, (const double*)xb, (const double*)partA, nA, y, partB, ev, tauv);
+        nBprev = nB;
+      }
+      const int jl = j1 - 1, ml = P - jl - 1;
+      hipLaunchKernelGGL(trd_panel_wlast_kernel, dim3((unsigned)std::min(64, (ml + 255) / 256)), dim3(256), 0,
+                         c->stream, P, jl, jl - j0, Pp, Qp, (const double*)y, (const double*)partB, nBprev,
+                         (const double*)tauv);
+      TN_HIP(hipGetLastError());
+      const long m1 = p - j1;
+      if (m1 > 0) {  // G[j1:, j1:] -= V W^T + W V^T (K = 2 NBT, unused rows of P, Q zero)
+        GemmDesc g;
+        g.M = m1; g.N = m1; g.K = 2 * NBT;
+        g.A = Pp + j1; g.lda = p; g.tA = 1;
+        g.B = Qp + j1; g.ldb = p;
+        g.C = G + (long)j1 * ldg + j1; g.ldc = ldg;
+        g.alpha = -1.0; g.beta = 1.0;
+        gemm(c, Prec::F64, g);
+      }
     }
+    const int m0 = P - jt;
+    const size_t tshm = ((size_t)m0 * (m0 + 1) + 2 * (size_t)m0) * sizeof(double);
+    raise_lds_limit((const void*)trd_tail_lds_kernel, ((size_t)TAIL * (TAIL + 1) + 2 * TAIL) * sizeof(double));
+    hipLaunchKernelGGL(trd_tail_lds_kernel, dim3(1), dim3(256), tshm, c->stream, G, ldg, P, jt, dv, ev, tauv);
+    done = true;
   }
   if (!done) {
     const size_t lds_bytes = (size_t)p * sizeof(double);

@@ -174,7 +174,7 @@ enum WsSlot {
   WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP,
   WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_PROJ_X, WS_PROJ_Y, WS_PROJ_C,
   WS_QR_X, WS_QR_V, WS_QR_Q, WS_QR_R, WS_QR_S, WS_QR_W, WS_TTN_A, WS_TTN_S, WS_TTN_G, WS_TTN_P, WS_TTN_F,
-  WS_GEMM_SPLIT, WS_BOND, WS_PHI2, WS_TTNP_0, WS_TTNP_1, WS_TTNP_2, WS_TTNP_3, WS_SLOTS
+  WS_GEMM_SPLIT, WS_BOND, WS_PHI2, WS_TTNP_0, WS_TTNP_1, WS_TTNP_2, WS_TTNP_3, WS_DC_PANEL, WS_SLOTS
 };
 
 // scoped phase timer (HIP events on the context stream)
