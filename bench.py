@@ -1,21 +1,35 @@
-"""Benchmark: DMRG sweep time and matvec MFMA utilisation on one MI355X.
+"""Benchmark: chi = 4096 two-site DMRG on one MI355X, mixed vs FP64 GPU forms.
 
-    python bench.py [--steps K] [--warmup W] [--chi 1024] [--N 100] [--prec bf16x3]
+    python bench.py [--steps K] [--warmup W] [--prec fp32|bf16x3]
 
-Workload (Heisenberg chain, open boundaries, d=2, MPO bond dim w=5):
-  * warmup: W two-site DMRG sweeps from the Neel state with maxdim ramped
-    64, 128, ... up to --chi (cutoff 1e-12, eig_solver defaults krylovdim=5,
-    maxiter=2), mixed precision; W = 7 plus the timed sweep gives the
-    8-sweep ramp 64..1024 of the Heisenberg N = 100 workload;
-  * timed: K sweeps at maxdim --chi in the mixed form (matvec --prec, FP64
-    Lanczos / SVD / final Rayleigh quotient) -> `value` (seconds per sweep);
-  * the same K sweeps from the same snapshot in the plain FP64 GPU form,
-    reported beside it with |dE/N| between the two;
-  * matvec microbenchmark at chi=4096 (dense, d=2, w=5) in FP64, FP32 and
-    the mixed form: per-GEMM-launch HIP-event timings on the library's own
-    stream give the roofline of the dominant kernel (matvec GEMM).
-  * cpu_baseline: the FP64 numpy oracle (OpenBLAS, all host threads) running
-    the same timed sweep from the same snapshot.
+Headline workload (`value`, seconds per DMRG sweep): the wide-MPO
+configuration, J1-J2 Heisenberg (J2 = 0.5) on the Ly = 6 x Lx = 12 cylinder
+(snake order, MPO bond dimension up to 20), at maxdim chi = 4096.  A full
+sweep of that chain (142 two-site updates, 94 of them at chi = 4096 on both
+sides) does not fit the run, so a "step" is one bulk two-site update at
+chi = 4096 (eig_solver defaults krylovdim 5, maxiter 2, cutoff 1e-10, the
+update's environment move included), on a six-site bulk window of the
+cylinder (sites 35-40, outer MPO bonds closed with fixed random vectors)
+from a random MPS (seed 7) with every bond at chi: updates sweep the window's
+three bonds left and right.  W untimed updates, then exactly K timed updates
+in the mixed form; the same window from the same state in the plain FP64
+form beside it.  `value` = mean mixed update time x (the algorithmic cost of
+the whole 72-site sweep at maxdim 4096 / that of one bulk update):
+**extrapolated** from measured bulk updates.
+Also reported:
+  * one measured full sweep at chi = 4096: the Heisenberg chain N = 26
+    (w = 5, maxdim 4096, cutoff 0; its three central bonds sit at 4096), both
+    forms from the same converged state, with |dE/N| against exact
+    diagonalisation (E0 = -11.33957965275514, tests/golden/heisenberg_n26_ed.json),
+    truncation errors and 1 - |<psi_fp64|psi>|;
+  * matvec microbenchmark at chi = 4096 (w = 5 and w = 20) in FP64, FP32, BF16x3;
+  * TDVP (XXZ N = 100, 40 steps) and many-instance TTN lines;
+  * `roofline`: the dominant kernel of the timed updates (the matvec GEMM),
+    achieved rate from per-launch HIP events on the library's stream during
+    the instrumented warmup updates;
+  * cpu_baseline: the FP64 numpy oracle on one bulk update of the same window
+    at chi = 1024 (a bounded sample), scaled to the chi = 4096 sweep by the
+    same algorithmic cost model.
 Prints one JSON line.
 """
 
@@ -37,7 +51,10 @@ for p in (ROOT, ROOT / "tennetlib.jl_amd"):
         sys.path.insert(0, str(p))
 
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
+HBM_PEAK_GBS = 8000.0
 METRIC = "DMRG sweep s + matvec %MFMA peak, mixed vs FP64 GPU at |ΔE/N|≤1e-9, χ=4096"
+E0_N26 = -11.33957965275514  # exact (Sz = 0 Lanczos), tests/golden/heisenberg_n26_ed.json
+PH_TRD_SYMV = 5
 
 
 def log(msg):
@@ -47,11 +64,6 @@ def log(msg):
 def heis_mpo(N):
     import tennetlib_amd as T
     return T.heisenberg_mpo(N)
-
-
-def neel(N):
-    import tennetlib_amd as T
-    return T.neel_state(N)
 
 
 def matvec_flops(chil, chir, d, wl, wr=None):
@@ -66,15 +78,26 @@ def phase(ctx, ph):
     return ms.value, n.value
 
 
-def wide_mpo4():
-    """Four bulk sites (35..38) of the J1-J2 Ly = 6 x Lx = 12 cylinder MPO
-    (inner bonds w = 20), outer bonds closed with fixed random vectors."""
+def phase_bytes(ctx, ph):
+    b = C.c_double()
+    ctx.check(ctx.lib.tn_ctx_phase_bytes(ctx.h, ph, C.byref(b)))
+    return b.value
+
+
+def wide_window(n, first=35):
+    """n bulk sites (first..first+n-1, 1-based) of the J1-J2 Ly = 6 x Lx = 12
+    cylinder MPO (inner bonds up to w = 20), outer bonds closed with fixed
+    random vectors (seed 11)."""
     import tennetlib_amd as T
-    Ws = T.j1j2_cylinder_mpo(6, 12, 1.0, 0.5)[35:39]
+    Ws = T.j1j2_cylinder_mpo(6, 12, 1.0, 0.5)[first - 1:first - 1 + n]
     rng = np.random.default_rng(11)
     Ws[0] = np.tensordot(rng.uniform(-1, 1, Ws[0].shape[0]), Ws[0], axes=([0], [0]))[None]
-    Ws[3] = np.tensordot(Ws[3], rng.uniform(-1, 1, Ws[3].shape[3]), axes=([3], [0]))[..., None]
+    Ws[-1] = np.tensordot(Ws[-1], rng.uniform(-1, 1, Ws[-1].shape[3]), axes=([3], [0]))[..., None]
     return [np.ascontiguousarray(w) for w in Ws]
+
+
+def wide_mpo4():
+    return wide_window(4, 36)
 
 
 def matvec_microbench(T, chi, iters, precs, Ws=None):
@@ -119,22 +142,178 @@ def matvec_microbench(T, chi, iters, precs, Ws=None):
     return res, F, Fgemm
 
 
-def sweeps(T, psi, N, chi, nsweeps, prec, cutoff, kd, mi):
-    env = T.StateEnvs(psi, heis_mpo(N))
-    sw = T.SweepData()
-    ctx = env.ctx
+WINDOW = 26        # sites of the bulk window (24..49 of the 72-site cylinder)
+WINDOW_FIRST = 24
+# window path: left-moving updates at positions 12, 13, 14 (chi_l x chi_r =
+# 2048 x 4096, 4096 x 4096, 4096 x 2048 at maxdim 4096), then right-moving 14, 13, 12
+WINDOW_PATH = [(12, "left"), (13, "left"), (14, "left"), (14, "right"), (13, "right"), (12, "right")]
+
+
+def window_dims(chi, n=WINDOW):
+    return [1] + [min(2 ** k, 2 ** (n - k), chi) for k in range(1, n)] + [1]
+
+
+def bulk_updates(T, chi, prec, nwarm, ntimed, cutoff=1e-10, kd=5, mi=2):
+    """nwarm untimed (instrumented) then ntimed timed two-site updates at the
+    centre of a 26-site window of the J1-J2 cylinder (sites 24-49, outer MPO
+    bonds closed with fixed random vectors) from a random MPS (seed 7) whose
+    bond k has dimension min(2^k, 2^(26-k), chi), brought into mixed-canonical
+    form around site 12 on the device.  Returns seconds per update, the phase
+    times of the instrumented updates, their matvec GEMM time and shapes."""
+    ctx = T.default_context()
+    Ws = wide_window(WINDOW, WINDOW_FIRST)
+    w = [Ws[0].shape[0]] + [W.shape[3] for W in Ws]
+    dims = window_dims(chi)
+    rng = np.random.default_rng(7)
+    A = [rng.uniform(-1, 1, (dims[k], 2, dims[k + 1])) / np.sqrt(dims[k] * 2) for k in range(WINDOW)]
+    env = T.StateEnvs(A, Ws, ctx=ctx)
+    del A
+    T.orthogonalize_(env, 12)
+    T.set_nsite_(env, 2)
+    T.position_(env, 12)
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    step = 0
+    rec = []
+    # a bulk update in the middle of a sweep: the FP32 form takes no Rayleigh
+    # quotient there (the sweep reports only its last update's energy); the
+    # BF16x3 form keeps its FP64 Lanczos refinement, which moves the state
+    ref = 0 if prec == "fp32" else None
+
+    def one():
+        nonlocal step
+        pos, ortho = WINDOW_PATH[step % len(WINDOW_PATH)]
+        step += 1
+        chil, chir = env.site_dims(pos)[0], env.site_dims(pos + 1)[1]
+        e, terr, r = T.update_position_(env, T.eig_solver, pos, 2, ortho, maxdim=chi, cutoff=cutoff,
+                                        solver_krylovdim=kd, solver_maxiter=mi, precision=prec, refine_fp64=ref)
+        rec.append(dict(pos=pos, left=ortho == "left", chil=chil, chir=chir, wl=w[pos - 1], wr=w[pos + 1],
+                        numops=r.info.numops, nkept=r.nkept, energy=e))
+
     ctx.enable_timing(True)
+    ctx.timers(reset=True)
+    for _ in range(nwarm):
+        one()
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    tm = ctx.timers(reset=False)
+    g_ms, g_n = phase(ctx, 4)
+    ctx.enable_timing(False)
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    t0 = time.perf_counter()
+    for _ in range(ntimed):
+        one()
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    dt = (time.perf_counter() - t0) / max(ntimed, 1)
+    env.close()
+    return {"s_per_update": dt, "warm_phase_ms": {k: round(v, 1) for k, v in tm.items()},
+            "warm_gemm_ms": g_ms, "warm_gemm_launches": g_n, "warm": rec[:nwarm], "timed": rec[nwarm:]}
+
+
+def gemm_flops(u, d=2):
+    """Algorithmic flops of the matvec GEMM launches of one update record."""
+    F1 = 2.0 * u["wl"] * u["chil"] * d * d * u["chir"] * u["chil"]
+    F2 = 2.0 * u["chil"] * d * d * u["chir"] * u["wr"] * u["chir"]
+    return u["numops"] * (F1 + F2)
+
+
+def sweep_scale(chi, recs, nmv=7):
+    """Algorithmic cost of a whole two-site sweep of the 72-site cylinder at
+    maxdim chi (bond k: min(2^k, 2^(72-k), chi)) over the mean cost of the
+    measured updates."""
+    import tennetlib_amd as T
+    Ws = T.j1j2_cylinder_mpo(6, 12, 1.0, 0.5)
+    N = len(Ws)
+    w = [1] + [W.shape[3] for W in Ws]
+    dims = [1] + [min(2 ** k, 2 ** (N - k), chi) for k in range(1, N)] + [1]
+    tot = 0.0
+    for left in (True, False):
+        for b in range(1, N):
+            tot += update_cost(dims[b - 1], dims[b + 1], 2, max(w[b - 1], w[b + 1]), nmv, left)
+    mean = sum(update_cost(u["chil"], u["chir"], 2, max(u["wl"], u["wr"]), nmv, u["left"]) for u in recs) / len(recs)
+    return tot / mean
+
+
+def random_product(N, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(N):
+        v = rng.standard_normal(2)
+        out.append((v / np.linalg.norm(v)).reshape(1, 2, 1))
+    return out
+
+
+def run_sweeps(T, env, sw, n, chi, prec, cutoff, kd, mi, timing=False):
+    """n fullsweep! calls on env; returns wall seconds per sweep (device
+    synchronised on both sides) and the per-phase device times."""
+    ctx = env.ctx
+    ctx.enable_timing(timing)
     ctx.timers(reset=True)
     ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
     t0 = time.perf_counter()
-    for _ in range(nsweeps):
+    for _ in range(n):
         T.fullsweep_(env, T.eig_solver, 2, sw, maxdim=chi, cutoff=cutoff, solver_krylovdim=kd,
                      solver_maxiter=mi, precision=prec, outputlevel=0)
     ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
-    dt = (time.perf_counter() - t0) / nsweeps
-    tm = ctx.timers(reset=True)
-    ctx.enable_timing(False)
-    return env, sw, dt, tm
+    dt = (time.perf_counter() - t0) / max(n, 1)
+    tm = ctx.timers(reset=False) if timing else None
+    return dt, tm
+
+
+def mps_overlap(A, B):
+    """<A|B> of two MPS (host, numpy/BLAS)."""
+    L = np.ones((1, 1))
+    for a, b in zip(A, B):
+        X = np.tensordot(L, b, axes=([1], [0]))              # (chiA, d, chiB')
+        L = np.tensordot(a.conj(), X, axes=([0, 1], [0, 1]))  # (chiA', chiB')
+    return complex(L.reshape(-1)[0])
+
+
+def exact_dims(N, chi):
+    return [min(2 ** k, 2 ** (N - k), chi) for k in range(1, N)]
+
+
+def update_cost(chil, chir, d, w, nmv, left):
+    """Algorithmic flops of one two-site update: nmv products, one
+    environment update, the density matrix and its eigen-decomposition
+    (4/3 p^3 tridiagonalisation + 2 p^2 k back-transformation, k = p / 2)."""
+    p = chil * d if left else chir * d
+    n = (chil * d) * (d * chir)
+    return (nmv * matvec_flops(chil, chir, d, w) + 4.0 * chil ** 3 * d * w + 2.0 * p * p * (n / p)
+            + (4.0 / 3.0 + 1.0) * p ** 3)
+
+
+def cpu_sample(chi_s, kd, mi):
+    """The FP64 numpy oracle on one update at the centre (position 13) of the
+    same 26-site window at maxdim chi_s (a bounded sample); returns its time
+    and cost-model record."""
+    from oracle import tn_oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except Exception:
+        cores = os.cpu_count() or 1
+    Ws = wide_window(WINDOW, WINDOW_FIRST)
+    w = [Ws[0].shape[0]] + [W.shape[3] for W in Ws]
+    dims = window_dims(chi_s)
+    rng = np.random.default_rng(7)
+    A = [rng.uniform(-1, 1, (dims[k], 2, dims[k + 1])) / np.sqrt(dims[k] * 2) for k in range(WINDOW)]
+    for k in range(12):  # left-orthonormal sites 1..12
+        q, r = np.linalg.qr(A[k].reshape(-1, A[k].shape[2]))
+        A[k] = q.reshape(A[k].shape[0], 2, q.shape[1])
+        A[k + 1] = np.tensordot(r, A[k + 1], axes=([1], [0]))
+    for k in range(WINDOW - 1, 13, -1):  # right-orthonormal sites 15..26
+        m = A[k].reshape(A[k].shape[0], -1)
+        q, r = np.linalg.qr(m.T)
+        A[k] = q.T.reshape(q.shape[1], 2, A[k].shape[2])
+        A[k - 1] = np.tensordot(A[k - 1], r.T, axes=([2], [0]))
+    env = O.StateEnvs(A, Ws)
+    env.ortho_center = 13
+    env.PH.nsite = 2
+    env.PH.position(env.psi, 13)
+    t0 = time.perf_counter()
+    O.update_two_site(env, 13, "left", maxdim=chi_s, cutoff=1e-10, krylovdim=kd, maxiter=mi)
+    t = time.perf_counter() - t0
+    rec = dict(chil=dims[12], chir=dims[14], wl=w[12], wr=w[14], left=True)
+    return t, rec, cores
 
 
 def tdvp_run(T, N, steps, dt, maxdim, cutoff):
@@ -201,20 +380,19 @@ def ttn_instances(T, count, seed=5, total=256):
             "maxchi": [res[i][1] for i in range(count)]}
 
 
-def cpu_sweep(snap, N, chi, cutoff, kd, mi):
-    """The FP64 numpy oracle running the same timed sweep (same snapshot,
-    maxdim, cutoff and Krylov settings) on the host cores."""
-    from oracle import tn_oracle as O
-    try:
-        from threadpoolctl import threadpool_info
-        cores = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
-    except Exception:
-        cores = os.cpu_count() or 1
-    env = O.StateEnvs([np.array(a) for a in snap], O.heisenberg_mpo(N))
-    sw = O.SweepData()
-    t0 = time.perf_counter()
-    O.fullsweep(env, sw, maxdim=chi, cutoff=cutoff, krylovdim=kd, maxiter=mi)
-    return time.perf_counter() - t0, cores, sw.energy[-1]
+def eig_probe(T, p):
+    """One truncated SVD of a random p x p matrix (density matrix of size p),
+    for hardware-counter runs of the eigensolver alone."""
+    ctx = T.default_context()
+    rng = np.random.default_rng(p)
+    M = T.DeviceArray.from_numpy(ctx, rng.standard_normal((p, p)) / p)
+    U = T.DeviceArray(ctx, (p, p))
+    Vt = T.DeviceArray(ctx, (p, p))
+    S = (C.c_double * p)()
+    nk, te = C.c_long(), C.c_double()
+    ctx.check(ctx.lib.tn_svd_trunc(ctx.h, M.ptr, p, p, p // 2, 1, 0.0, U.ptr, S, Vt.ptr, C.byref(nk), C.byref(te)))
+    ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
+    return {"p": p, "kept": nk.value}
 
 
 # HBM bytes per launch of the FP32 matvec GEMM at chi = 4096 (mean of its two
@@ -223,27 +401,80 @@ def cpu_sweep(snap, N, chi, cutoff, kd, mi):
 GEMM_TRAFFIC = {("fp32", 4096): 17.2e9}
 
 
+def n26_sweeps(T, args):
+    """Measured full sweeps at chi = 4096: Heisenberg N = 26 (w = 5, cutoff 0)."""
+    N, chi, cut, kd, mi = 26, args.chi, 0.0, args.krylovdim, args.maxiter
+    target = exact_dims(N, chi)
+    Ws = heis_mpo(N)
+    env = T.StateEnvs(random_product(N, 1), Ws)
+    sw = T.SweepData()
+    t0 = time.perf_counter()
+    # with cutoff 0 the weights at the rounding floor decide the last few
+    # dimensions of each bond: stop once maxdim is reached and the total bond
+    # dimension has settled
+    prev = 0
+    for it in range(12):
+        run_sweeps(T, env, sw, 1, chi, args.prec, cut, kd, mi)
+        tot = sum(env.site_dims(k)[1] for k in range(1, N))
+        log(f"n26 prep sweep {it + 1}: E={sw.energy[-1]:.12f} maxchi={sw.maxchi[-1]} sum(chi)={tot} "
+            f"t={time.perf_counter() - t0:.1f}s")
+        if sw.maxchi[-1] >= max(target) and abs(tot - prev) <= 0.005 * tot:
+            break
+        prev = tot
+    prep_s = time.perf_counter() - t0
+    snap = env.psi
+    env.close()
+    out = {"workload": f"two-site DMRG sweeps, Heisenberg OBC N={N}, d=2, w=5, maxdim {chi}, cutoff 0, "
+                       f"krylovdim {kd}, maxiter {mi}; bonds up to {target}", "state_prep_s": round(prep_s, 1)}
+    psis = {}
+    for form in ("fp64", args.prec):
+        env = T.StateEnvs(snap, Ws)
+        swf = T.SweepData()
+        _, tm = run_sweeps(T, env, swf, 1, chi, form, cut, kd, mi, timing=True)
+        e1, terr = swf.energy[-1], swf.maxtruncerr[-1]
+        if not args.skip_overlap:
+            psis[form] = env.psi
+        dt, _ = run_sweeps(T, env, swf, args.n26_sweeps, chi, form, cut, kd, mi, timing=False)
+        chis = sum(env.site_dims(k)[1] for k in range(1, N))
+        out[form] = {"s_per_sweep": round(dt, 4), "sweeps_timed": args.n26_sweeps, "energy_first": e1,
+                     "energy": swf.energy[-1], "truncerr_first": terr, "sum_chi": chis,
+                     "dE_per_site_vs_exact": abs(swf.energy[-1] - E0_N26) / N,
+                     "first_sweep_phase_ms": {k: round(v, 1) for k, v in tm.items()}}
+        log(f"n26 {form}: {dt:.3f} s/sweep E={swf.energy[-1]:.14f} sum(chi)={chis} first-sweep phases={tm}")
+        env.close()
+    out["speedup_vs_fp64"] = round(out["fp64"]["s_per_sweep"] / out[args.prec]["s_per_sweep"], 3)
+    out["dE_per_site_first_sweep_vs_fp64"] = abs(out[args.prec]["energy_first"] - out["fp64"]["energy_first"]) / N
+    if len(psis) == 2:
+        a, b = psis["fp64"], psis[args.prec]
+        ov = mps_overlap(a, b)
+        out["one_minus_overlap_first_sweep"] = 1.0 - abs(ov) / np.sqrt(mps_overlap(a, a).real *
+                                                                         mps_overlap(b, b).real)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=7)
-    ap.add_argument("--N", type=int, default=100)
-    ap.add_argument("--chi", type=int, default=1024)
-    ap.add_argument("--mv-chi", type=int, default=4096)
-    ap.add_argument("--mv-iters", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=6, help="timed bulk updates (mixed form)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed (instrumented) bulk updates before them")
+    ap.add_argument("--chi", type=int, default=4096)
     ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3"])
-    ap.add_argument("--cutoff", type=float, default=1e-12)
     ap.add_argument("--krylovdim", type=int, default=5)
     ap.add_argument("--maxiter", type=int, default=2)
-    ap.add_argument("--skip-fp64", action="store_true")
-    ap.add_argument("--skip-cpu", action="store_true")
-    ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
+    ap.add_argument("--fp64-steps", type=int, default=6, help="timed bulk updates in the FP64 form")
+    ap.add_argument("--n26-sweeps", type=int, default=1, help="timed N=26 sweeps per form (0: skip the line)")
+    ap.add_argument("--mv-chi", type=int, default=4096)
+    ap.add_argument("--mv-iters", type=int, default=5)
     ap.add_argument("--mv-wide", type=int, default=1, help="also time the w = 20 (J1-J2 cylinder) matvec")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-overlap", action="store_true")
+    ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
     ap.add_argument("--tdvp-steps", type=int, default=40, help="TDVP steps of the XXZ N=100 run (0: skip)")
-    ap.add_argument("--ttn-instances", type=int, default=16, help="TTN instances timed of the 256 (0: skip)")
+    ap.add_argument("--ttn-instances", type=int, default=8, help="TTN instances timed of the 256 (0: skip)")
     ap.add_argument("--ttn-only", action="store_true", help="only the many-instance TTN line")
     ap.add_argument("--tdvp-only", action="store_true", help="only the TDVP XXZ N=100 line")
+    ap.add_argument("--eig-only", type=int, default=0, metavar="P",
+                    help="only one density-matrix eigen-decomposition of size P (counter runs)")
     ap.add_argument("--copy-ref", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -268,57 +499,48 @@ def main():
     import tennetlib_amd as T
     T.default_context()
 
-    # 1. matvec microbenchmark at chi = mv_chi
+    if args.eig_only:
+        print(json.dumps({"eig_probe": eig_probe(T, args.eig_only)}), flush=True)
+        return
     if args.ttn_only:
         print(json.dumps({"ttn_random_field_batch": ttn_instances(T, max(1, args.ttn_instances))}), flush=True)
         return
     if args.tdvp_only:
         print(json.dumps({"tdvp_xxz_n100": tdvp_run(T, 100, max(1, args.tdvp_steps), 0.05, 512, 1e-12)}), flush=True)
         return
+
+    # 1. matvec microbenchmark at chi = mv_chi
     mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", "bf16x3"])
     mv_wide = None
     if args.mv_wide:
-        # the wide-MPO matvec (J1-J2 cylinder bulk, w = 20) at the same chi
-        mv_wide, F_wide, _ = matvec_microbench(T, args.mv_chi, max(2, args.mv_iters // 2), ["fp64", "fp32"],
-                                               Ws=wide_mpo4())
+        mv_wide, F_wide, _ = matvec_microbench(T, args.mv_chi, max(2, args.mv_iters // 2),
+                                               ["fp64", "fp32", "bf16x3"], Ws=wide_mpo4())
     if args.mv_only:
         print(json.dumps({"matvec": mv, "matvec_wide": mv_wide, "chi": args.mv_chi}), flush=True)
         return
 
-    # 2. DMRG: warmup ramp, then timed sweeps in the mixed and FP64 forms
-    N, chi = args.N, args.chi
-    ramp = []
-    m = 64
-    while len(ramp) < args.warmup:
-        ramp.append(min(m, chi))
-        m *= 2
-    env = T.StateEnvs(neel(N), heis_mpo(N))
-    sw = T.SweepData()
-    t0 = time.perf_counter()
-    for md in ramp:
-        T.fullsweep_(env, T.eig_solver, 2, sw, maxdim=md, cutoff=args.cutoff, solver_krylovdim=args.krylovdim,
-                     solver_maxiter=args.maxiter, precision=args.prec, outputlevel=0)
-        log(f"warmup sweep maxdim={md}: E={sw.energy[-1]:.12f} maxchi={sw.maxchi[-1]} "
-            f"t={time.perf_counter() - t0:.1f}s")
-    snap = env.psi
-    env.close()
+    # 2. headline: bulk updates at chi on the J1-J2 window, mixed then FP64
+    chi, kd, mi = args.chi, args.krylovdim, args.maxiter
+    mixed = bulk_updates(T, chi, args.prec, max(1, args.warmup), args.steps, kd=kd, mi=mi)
+    scale = sweep_scale(chi, mixed["timed"])
+    value = mixed["s_per_update"] * scale
+    log(f"bulk updates chi={chi} {args.prec}: {mixed['s_per_update']:.3f} s/update -> {value:.1f} s/sweep "
+        f"(x{scale:.2f}); dims {[(u['chil'], u['chir'], u['nkept'], u['numops']) for u in mixed['timed']]}; "
+        f"warmup phases {mixed['warm_phase_ms']}")
+    fp64 = bulk_updates(T, chi, "fp64", 1, args.fp64_steps, kd=kd, mi=mi)
+    scale64 = sweep_scale(chi, fp64["timed"])
+    sweep64 = fp64["s_per_update"] * scale64
+    log(f"bulk updates chi={chi} fp64: {fp64['s_per_update']:.3f} s/update -> {sweep64:.1f} s/sweep; "
+        f"dims {[(u['chil'], u['chir'], u['nkept'], u['numops']) for u in fp64['timed']]}; "
+        f"warmup phases {fp64['warm_phase_ms']}")
+    # dominant kernel of the updates: the matvec GEMM launches of the
+    # instrumented warmup updates
+    flops = sum(gemm_flops(u) for u in mixed["warm"])
+    gemm_tf = flops / (mixed["warm_gemm_ms"] * 1e-3) / 1e12 if mixed["warm_gemm_ms"] > 0 else None
+    gemm_frac = gemm_tf / PEAK["fp32"] if gemm_tf else None
 
-    env_m, sw_m, dt_m, tm_m = sweeps(T, snap, N, chi, args.steps, args.prec, args.cutoff, args.krylovdim,
-                                     args.maxiter)
-    log(f"mixed ({args.prec}) sweep: {dt_m:.3f} s, E={sw_m.energy[-1]:.14f}, maxchi={sw_m.maxchi[-1]}, "
-        f"phases(ms)={ {k: round(v, 1) for k, v in tm_m.items()} }")
-    E_m = sw_m.energy[-1]
-    chi_m = sw_m.maxchi[-1]
-    env_m.close()
-    fp64 = None
-    if not args.skip_fp64:
-        env_d, sw_d, dt_d, tm_d = sweeps(T, snap, N, chi, args.steps, "fp64", args.cutoff, args.krylovdim,
-                                         args.maxiter)
-        log(f"fp64 sweep: {dt_d:.3f} s, E={sw_d.energy[-1]:.14f}, maxchi={sw_d.maxchi[-1]}, "
-            f"phases(ms)={ {k: round(v, 1) for k, v in tm_d.items()} }")
-        fp64 = dict(sweep_s=dt_d, energy=sw_d.energy[-1], maxchi=sw_d.maxchi[-1],
-                    phase_ms={k: round(v, 2) for k, v in tm_d.items()})
-        env_d.close()
+    # 3. measured full sweeps at chi = 4096 (N = 26 chain)
+    n26 = n26_sweeps(T, args) if args.n26_sweeps > 0 else None
 
     tdvp = None
     if args.tdvp_steps > 0:
@@ -332,45 +554,52 @@ def main():
 
     cpu = None
     if not args.skip_cpu:
-        t_cpu, cores, e_cpu = cpu_sweep(snap, N, chi, args.cutoff, args.krylovdim, args.maxiter)
-        cpu = {"value": round(t_cpu, 3), "unit": "s/sweep", "cores": cores, "kind": "port",
-               "sample": f"numpy FP64 oracle (restatement, not the Julia reference), the same sweep: "
-                         f"{2 * (N - 1)} two-site updates from the same snapshot at maxdim {chi}, "
-                         f"cutoff {args.cutoff:g}; energy {e_cpu:.12f}"}
-        log(f"cpu baseline: {t_cpu:.2f} s/sweep on {cores} threads, E={e_cpu:.14f}")
+        chi_s = 1024
+        t_up, rec, cores = cpu_sample(chi_s, kd, mi)
+        cscale = sweep_scale(chi, [rec])
+        cpu = {"value": round(t_up * cscale, 1), "unit": "s/sweep", "cores": cores, "kind": "port",
+               "sample": f"numpy FP64 oracle (restatement, not the Julia reference): one two-site update at the "
+                         f"centre of the same J1-J2 window at chi={chi_s} (w=20), {t_up:.2f} s, scaled by "
+                         f"{cscale:.1f} = algorithmic cost of the 72-site chi={chi} sweep / that update's "
+                         f"(extrapolated)"}
+        log(f"cpu baseline: {t_up:.2f} s for one chi={chi_s} update -> {t_up * cscale:.0f} s/sweep (scaled)")
 
-    pk = PEAK["fp32"]
-    g = mv[args.prec]
     line = {
         "metric": METRIC,
-        "value": round(dt_m, 4),
+        "value": round(value, 3),
         "unit": "s/sweep",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(dt_m * 1e3, 2),
+        "ms_per_step": round(mixed["s_per_update"] * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": f"{args.prec} matvec (MFMA), fp64 Krylov / factorisation / energy refinement",
-        "data": "synthetic (Neel start, Heisenberg MPO built in-tree)",
-        "config": {"workload": f"two-site DMRG sweep, Heisenberg OBC N={N}, d=2, w=5, maxdim {chi} "
-                               f"(ramp {ramp}), cutoff {args.cutoff:g}, krylovdim {args.krylovdim}, "
-                               f"maxiter {args.maxiter}; matvec microbench chi={args.mv_chi}",
-                   "maxchi": chi_m, "parallelism": "single GPU"},
-        "fp64_gpu": fp64,
-        "speedup_vs_fp64": (round(fp64["sweep_s"] / dt_m, 3) if fp64 else None),
-        "dE_per_site_vs_fp64": (abs(E_m - fp64["energy"]) / N if fp64 else None),
-        "energy_mixed": E_m,
+        "dtype": f"{args.prec} matvec (MFMA) + FP64 Rayleigh-quotient energy; fp64 environments, Krylov "
+                 f"vectors, factorisation",
+        "data": "synthetic (random MPS, seed 7; J1-J2 cylinder MPO built in-tree)",
+        "config": {"workload": f"J1-J2 (J2=0.5) Ly=6 x Lx=12 cylinder, two-site DMRG at maxdim {chi} (cutoff 1e-10, "
+                               f"krylovdim {kd}, maxiter {mi}): a step is one update at the centre of a 26-site "
+                               f"window (chi_l x chi_r up to {chi} x {chi}, w up to 20, environment move included); "
+                               f"value = mean step x {scale:.2f} (algorithmic cost of the 142-update sweep / mean "
+                               f"cost of the timed steps), extrapolated",
+                   "parallelism": "single GPU", "extrapolated": True},
+        "fp64_gpu": {"s_per_update": round(fp64["s_per_update"], 4), "updates_timed": args.fp64_steps,
+                     "sweep_s": round(sweep64, 3), "warm_phase_ms": fp64["warm_phase_ms"]},
+        "mixed_warm_phase_ms": mixed["warm_phase_ms"],
+        "speedup_vs_fp64": round(sweep64 / value, 3),
+        "n26_full_sweeps_chi4096": n26,
         "matvec_chi4096": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv.items()},
-        "matvec_pct_fp32_peak": round(100 * mv[args.prec]["matvec_tflops"] / pk, 2),
+        "matvec_pct_fp32_peak": {k: round(100 * v["matvec_tflops"] / PEAK["fp32"], 2) for k, v in mv.items()},
         "matvec_chi4096_w20": ({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv_wide.items()}
                                if mv_wide else None),
-        "roofline": {"bound": "mfma", "achieved": round(g["gemm_tflops"], 2), "peak": pk, "unit": "TFLOP/s",
-                     "frac": round(g["gemm_tflops"] / pk, 4),
-                     "traffic": GEMM_TRAFFIC.get((args.prec, args.mv_chi)),
-                     "kernel": f"matvec GEMM ({args.prec}), M=5*{args.mv_chi}, N=4*{args.mv_chi}, "
-                               f"K={args.mv_chi}; {F_gemm:.3e} algorithmic flop/launch"},
+        "roofline": {"bound": "mfma", "achieved": (round(gemm_tf, 2) if gemm_tf else None), "peak": PEAK["fp32"],
+                     "unit": "TFLOP/s", "frac": (round(gemm_frac, 4) if gemm_frac else None),
+                     "traffic": None,
+                     "kernel": f"matvec GEMM ({args.prec}) of the bulk updates: M=w*{chi}, N=4*{chi}, K={chi} and "
+                               f"M=4*{chi}, N={chi}, K=w*{chi} (w=17..20); {mixed['warm_gemm_launches']} launches, "
+                               f"{mixed['warm_gemm_ms']:.1f} ms, {flops:.3e} algorithmic flop in the instrumented "
+                               f"warmup updates"},
         "tdvp_xxz_n100": tdvp,
         "ttn_random_field_batch": ttn,
         "cpu_baseline": cpu,

@@ -118,6 +118,10 @@ int tn_ctx_enable_timing(tn_ctx* ctx, int on);
 /* accumulated device ms and event count of one phase: 0 matvec, 1 env update,
  * 2 Krylov vector ops, 3 SVD/replacebond, 4 each GEMM launch of a matvec */
 int tn_ctx_phase_stats(tn_ctx* ctx, int phase, double* ms, long* count);
+/* phase 5 times each launch of the tridiagonalisation's product kernel
+ * (large density matrices); tn_ctx_phase_bytes returns the algorithmic bytes
+ * of the timed launches of a phase (8 m^2 per launch for phase 5) */
+int tn_ctx_phase_bytes(tn_ctx* ctx, int phase, double* bytes);
 void tn_update_params_default(tn_update_params* p, int solver);
 
 /* ---- StateEnvs{ProjMPO}: MPS + cached environments -------------------- */

@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
 // rank-2 update).  Vectors are stored by global row in the panel buffers
 // P = [V; W] and Q = [W; V] (2 NBT rows of length p; unused rows zero).
 constexpr int NBT = 32;
-constexpr int RCHUNK = 512;  // row entries per workgroup in trd_panel_row_kernel
+constexpr int RCHUNK = 256;  // row entries per workgroup in trd_panel_row_kernel (one per thread)
 
 // Column j: forms w_{k-1} (the previous column's, from y_{j-1} and the
 // partial sums of y_{j-1}^T v_{j-1}), then the current row x (stored in xbuf,
@@ -1686,8 +1686,12 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
                            Pp, Qp, (const double*)y, (const double*)partB, nBprev, (const double*)tauv, xb, partA,
                            dv);
         const int nB = (m + 7) / 8;  // two rows per wave
-        hipLaunchKernelGGL(trd_panel_symv_kernel, dim3(nB), dim3(256), (size_t)m * sizeof(double), c->stream, G, ldg,
-                           P, j, k, Pp, Qp, (const double*)xb, (const double*)partA, nA, y, partB, ev, tauv);
+        {
+          PhaseTimer pt(c, PH_TRD_SYMV);
+          if (c->timing) c->phase_bytes[PH_TRD_SYMV] += 8.0 * (double)m * (double)m;  // one read of G22
+          hipLaunchKernelGGL(trd_panel_symv_kernel, dim3(nB), dim3(256), (size_t)m * sizeof(double), c->stream, G,
+                             ldg, P, j, k, Pp, Qp, (const double*)xb, (const double*)partA, nA, y, partB, ev, tauv);
+        }
         nBprev = nB;
       }
       const int jl = j1 - 1, ml = P - jl - 1;

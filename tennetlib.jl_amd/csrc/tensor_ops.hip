@@ -70,6 +70,45 @@ __global__ __launch_bounds__(256) void wstep_kernel(int G, int Gp, int D, int Dp
   }
 }
 
+// The same W step with VEC consecutive B per thread (16-byte loads and
+// stores: 64 VEC columns per workgroup, so every row segment it touches is
+// 1 KiB contiguous instead of 256 B); shapes and strides multiples of VEC.
+// Dynamic LDS: K x 64 VEC elements.
+template <typename T, int VEC>
+__global__ __launch_bounds__(256) void wstep_vec_kernel(int G, int Gp, int D, int Dp, long nB,
+                                                        const T* __restrict__ in, long in_sg, long in_sA,
+                                                        long in_sS, T* __restrict__ out, long out_sg,
+                                                        long out_sA, long out_sS,
+                                                        const int* __restrict__ rowptr,
+                                                        const int* __restrict__ colidx,
+                                                        const double* __restrict__ vals) {
+  typedef T vec_t __attribute__((ext_vector_type(VEC)));
+  extern __shared__ unsigned char smem_raw[];
+  vec_t* slab = reinterpret_cast<vec_t*>(smem_raw);  // [K][64] vectors
+  const int K = G * D;
+  const int tb = threadIdx.x & 63;
+  const int ty = threadIdx.x >> 6;
+  const long A = blockIdx.y;
+  const long B = ((long)blockIdx.x * 64 + tb) * VEC;
+  const bool valid = B < nB;
+  for (int i = ty; i < K; i += 4) {
+    const int g = i / D, S = i % D;
+    vec_t v = (vec_t)(T(0));
+    if (valid) v = *reinterpret_cast<const vec_t*>(in + g * in_sg + A * in_sA + S * in_sS + B);
+    slab[i * 64 + tb] = v;
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int rows = Gp * Dp;
+  for (int o = ty; o < rows; o += 4) {
+    const int b = rowptr[o], e = rowptr[o + 1];
+    vec_t acc = (vec_t)(T(0));
+    for (int z = b; z < e; ++z) acc += (T)vals[z] * slab[colidx[z] * 64 + tb];
+    const int gp = o / Dp, Sp = o % Dp;
+    *reinterpret_cast<vec_t*>(out + gp * out_sg + A * out_sA + Sp * out_sS + B) = acc;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // conversions / fills
 // ---------------------------------------------------------------------------
@@ -311,12 +350,37 @@ static void wstep_launch(tn_ctx* c, bool fp32, bool split, const WStepDesc& w, u
   if (w.nA == 0 || w.nB == 0) return;
   TN_REQUIRE(w.nA <= 65535L, TN_E_INVALID, "wstep: nA exceeds grid.y");
   const int K = w.G * w.D;
-  const size_t shm = (size_t)K * 64 * (fp32 ? sizeof(float) : sizeof(double));
-  TN_REQUIRE(shm <= 64 * 1024, TN_E_UNSUPPORTED, "wstep: MPO slab exceeds 64 KiB of LDS");
-  dim3 grid((unsigned)((w.nB + 63) / 64), (unsigned)w.nA);
   const int* rp = w.op->rowptr.as<int>();
   const int* ci = w.op->colidx.as<int>();
   const double* vv = w.op->vals.as<double>();
+  if (!split) {
+    // 16-byte vectors along B when every offset allows them and the slab fits
+    const int vec = fp32 ? 4 : 2;
+    const size_t es = fp32 ? sizeof(float) : sizeof(double);
+    const size_t vshm = (size_t)K * 64 * vec * es;
+    auto ok = [&](long x) { return x % vec == 0; };
+    if (w.nB >= 64 * vec && ok(w.nB) && ok(w.in_sg) && ok(w.in_sA) && ok(w.in_sS) && ok(w.out_sg) &&
+        ok(w.out_sA) && ok(w.out_sS) && (uintptr_t)w.in % 16 == 0 && (uintptr_t)w.out % 16 == 0 &&
+        vshm <= 160 * 1024) {
+      dim3 grid((unsigned)((w.nB / vec + 63) / 64), (unsigned)w.nA);
+      if (fp32) {
+        if (vshm > 64 * 1024) raise_lds_limit((const void*)wstep_vec_kernel<float, 4>, vshm);
+        hipLaunchKernelGGL((wstep_vec_kernel<float, 4>), grid, dim3(256), vshm, c->stream, w.G, w.Gp, w.D, w.Dp,
+                           w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg, w.out_sA,
+                           w.out_sS, rp, ci, vv);
+      } else {
+        if (vshm > 64 * 1024) raise_lds_limit((const void*)wstep_vec_kernel<double, 2>, vshm);
+        hipLaunchKernelGGL((wstep_vec_kernel<double, 2>), grid, dim3(256), vshm, c->stream, w.G, w.Gp, w.D, w.Dp,
+                           w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg,
+                           w.out_sA, w.out_sS, rp, ci, vv);
+      }
+      TN_HIP(hipGetLastError());
+      return;
+    }
+  }
+  const size_t shm = (size_t)K * 64 * (fp32 ? sizeof(float) : sizeof(double));
+  TN_REQUIRE(shm <= 64 * 1024, TN_E_UNSUPPORTED, "wstep: MPO slab exceeds 64 KiB of LDS");
+  dim3 grid((unsigned)((w.nB + 63) / 64), (unsigned)w.nA);
   if (split)
     hipLaunchKernelGGL((wstep_kernel<float, true>), grid, dim3(256), shm, c->stream, w.G, w.Gp, w.D, w.Dp, w.nA,
                        w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)nullptr, w.out_sg, w.out_sA,

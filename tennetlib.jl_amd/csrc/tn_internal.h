@@ -78,7 +78,9 @@ struct DevBuf {
 
 // timed phases; PH_MV_GEMM brackets each GEMM launch of a matvec on its own
 // (nested inside PH_MATVEC), the others do not nest
-enum Phase { PH_MATVEC = 0, PH_ENV = 1, PH_KRYLOV = 2, PH_SVD = 3, PH_MV_GEMM = 4, PH_COUNT = 5 };
+// PH_TRD_SYMV brackets each launch of the tridiagonalisation's product
+// kernel (nested inside PH_SVD) and counts its algorithmic bytes
+enum Phase { PH_MATVEC = 0, PH_ENV = 1, PH_KRYLOV = 2, PH_SVD = 3, PH_MV_GEMM = 4, PH_TRD_SYMV = 5, PH_COUNT = 6 };
 
 }  // namespace tn
 
@@ -113,8 +115,9 @@ struct tn_ctx {
   }
   // per-phase timing
   bool timing = false;
-  double phase_ms[tn::PH_COUNT] = {0, 0, 0, 0, 0};
-  long phase_n[tn::PH_COUNT] = {0, 0, 0, 0, 0};
+  double phase_ms[tn::PH_COUNT] = {0, 0, 0, 0, 0, 0};
+  long phase_n[tn::PH_COUNT] = {0, 0, 0, 0, 0, 0};
+  double phase_bytes[tn::PH_COUNT] = {0, 0, 0, 0, 0, 0};  // algorithmic bytes of timed launches
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pool;
   std::vector<int> ev_phase;
   size_t ev_used = 0;

@@ -2156,7 +2156,7 @@ int tn_ctx_timers(tn_ctx* c, double* ms4, int reset) {
     if (ms4)
       for (int i = 0; i < 4; ++i) ms4[i] = c->phase_ms[i];
     if (reset)
-      for (int i = 0; i < PH_COUNT; ++i) { c->phase_ms[i] = 0.0; c->phase_n[i] = 0; }
+      for (int i = 0; i < PH_COUNT; ++i) { c->phase_ms[i] = 0.0; c->phase_n[i] = 0; c->phase_bytes[i] = 0.0; }
   });
 }
 int tn_ctx_enable_timing(tn_ctx* c, int on) {
@@ -2174,6 +2174,14 @@ int tn_ctx_phase_stats(tn_ctx* c, int phase, double* ms, long* count) {
     flush_timers(c);
     if (ms) *ms = c->phase_ms[phase];
     if (count) *count = c->phase_n[phase];
+  });
+}
+
+int tn_ctx_phase_bytes(tn_ctx* c, int phase, double* bytes) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c && bytes, TN_E_INVALID, "null argument");
+    TN_REQUIRE(phase >= 0 && phase < PH_COUNT, TN_E_INVALID, "phase_bytes: unknown phase");
+    *bytes = c->phase_bytes[phase];
   });
 }
 
@@ -2570,14 +2578,18 @@ int tn_fullsweep(tn_env* e, const tn_update_params* p, double* energy, double* m
     double en = NAN, mte = 0.0;
     std::vector<double> mid;
     tn_update_result r{};
+    // the sweep reports the energy of its last update; a Rayleigh-quotient
+    // refinement (refine_fp64 = 1) changes only that number, so it runs there
+    tn_update_params pm = *p;
+    if (p->refine_fp64 == 1 && p->precision != TN_PREC_FP64) pm.refine_fp64 = 0;
     for (int b = 1; b <= N - 1; ++b) {
-      update_two_site(e, b, true, p, &r);
+      update_two_site(e, b, true, &pm, &r);
       en = r.energy;
       mte = std::max(mte, r.truncerr);
     }
     for (int b = N - 1; b >= 1; --b) {
       std::vector<double> eigs;
-      update_two_site(e, b, false, p, &r, &eigs);
+      update_two_site(e, b, false, b == 1 ? p : &pm, &r, &eigs);
       en = r.energy;
       mte = std::max(mte, r.truncerr);
       if (b == N / 2) mid.swap(eigs);  // lasteigs[N/2] (sweep.jl:141,158)

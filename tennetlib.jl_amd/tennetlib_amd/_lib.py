@@ -78,6 +78,7 @@ SIGNATURES = {
     "tn_ctx_timers": [_P, _PD, _I],
     "tn_ctx_enable_timing": [_P, _I],
     "tn_ctx_phase_stats": [_P, _I, _PD, _PL],
+    "tn_ctx_phase_bytes": [_P, _I, _PD],
     "tn_update_params_default": [C.POINTER(UpdateParams), _I],
     "tn_env_create": [_P, _I, _PI, _PI, _I, C.POINTER(_P)],
     "tn_env_destroy": [_P],

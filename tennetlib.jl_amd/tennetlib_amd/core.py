@@ -449,6 +449,14 @@ def _sweep(sysenv: StateEnvs, solver, swdata: SweepData, pick, kwargs):
     swdata.sweepcount += 1
     lasteigs: List[np.ndarray] = [None] * (N - 1)
     p = _params(solver, kwargs)
+    # The sweep reports the energy of its last update (sweep.jl:137-157).  A
+    # Rayleigh-quotient refinement (refine_fp64 = 1) changes only that number,
+    # not the state, so with outputlevel <= 1 it runs for the last update only;
+    # a Lanczos refinement (>= 2) moves the state and runs at every update.
+    p_mid = p
+    if p.refine_fp64 == 1 and p.precision != L.TN_PREC_FP64 and outputlevel <= 1:
+        p_mid = L.UpdateParams.from_buffer_copy(p)
+        p_mid.refine_fp64 = 0
     cap = 1 << 16
     eigbuf = (C.c_double * cap)()
     t0 = time.perf_counter()
@@ -457,19 +465,20 @@ def _sweep(sysenv: StateEnvs, solver, swdata: SweepData, pick, kwargs):
     r.eigs_cap = cap
     for bond in range(1, N):
         ns = pick(bond)
-        _update(sysenv, p, r, bond, ns, "left")
+        _update(sysenv, p_mid, r, bond, ns, "left")
         energy = r.energy
         maxtruncerr = max(r.truncerr, maxtruncerr)
         lasteigs[bond - 1] = np.array(eigbuf[:min(r.nkept, cap)])
         if outputlevel > 1:
             print(f"At left sweep {swdata.sweepcount} bond {bond} => Energy {energy}, Err {r.truncerr:.2g}")
         if ns == 1 and bond == N - 1:
-            _update(sysenv, p, r, N, 1, "left")
+            _update(sysenv, p_mid, r, N, 1, "left")
             energy = r.energy
     swdata.lasteigs = list(lasteigs)
     for bond in range(N - 1, 0, -1):
         ns = pick(bond)
-        _update(sysenv, p, r, bond + 1 if ns == 1 else bond, ns, "right")
+        last = bond == 1 and ns == 2
+        _update(sysenv, p if last else p_mid, r, bond + 1 if ns == 1 else bond, ns, "right")
         energy = r.energy
         maxtruncerr = max(r.truncerr, maxtruncerr)
         lasteigs[bond - 1] = np.array(eigbuf[:min(r.nkept, cap)])
