@@ -165,6 +165,20 @@ int tn_env_set_projector_site(tn_env* env, int j, int k, const void* A_host, lon
  * out of the right one; arrays of length N+1, entries 1..N-1 read, each in
  * [0, w[k]].  Default 0 (ITensorMPS noiseterm of a ProjMPO: all channels). */
 int tn_env_set_noise_channels(tn_env* env, const int* skip_left, const int* skip_right);
+/* Quantum-number (U(1) or U(1)xU(1)) conservation: ConserveQNs of the
+ * reference's QN-conserving runs (test/test_MPS_DMRG.jl:10,29; NDTensors
+ * BlockSparse factorisation, upstream).  nq charges per QN (1 or 2; 0 turns
+ * it off); site_qn: for every site k the QNs of its d_k basis states (nq ints
+ * each, sites concatenated); link_qn: for every link 0..N (link k carries the
+ * QN of sites 1..k) the QNs of its elements, links concatenated.  The MPS must
+ * be block-sparse with respect to them and right-orthonormal with orthocenter
+ * 1 (a product state is).  replacebond! then factorises block by block with
+ * NDTensors' block truncation and updates the link QNs; one-site updates,
+ * noise, projectors and complex tensors are not built in this mode
+ * (TN_E_UNSUPPORTED). */
+int tn_env_set_qn(tn_env* env, int nq, const int* site_qn, const int* link_qn);
+/* QNs of link k (0..N): *n elements, nq ints each into out (capacity cap ints) */
+int tn_env_get_link_qn(tn_env* env, int k, int* out, long cap, long* n);
 /* shape of the projected-operator domain at the current position:
  * dims[0..3] = (chil, d_p, d_{p+1} or 1, chir); *n = element count */
 int tn_env_phi_dims(tn_env* env, long* dims4, long* n);

@@ -514,20 +514,9 @@ void gemm_bf16x3_nt(tn_ctx* c, long M, long N, long K, const unsigned short* Ah,
              "gemm_bf16x3_nt: row stride too large for 32-bit tile offsets");
   SArgs a{M, N, K, Ah, Al, lda, Bh, Bl, ldb, C, ldc};
   const long ntm = (M + SBM - 1) / SBM, ntn = (N + SBN - 1) / SBN;
-  // K tile: 32 by default; TNHIP_BF16_BK=64 selects the one-workgroup-per-CU form
-  static const int bk = [] {
-    const char* e = std::getenv("TNHIP_BF16_BK");
-    return (e && std::atoi(e) == 64) ? 64 : 32;
-  }();
-  if (bk == 64) {
-    const size_t shm = 2 * (size_t)SCfg<64>::STAGE * sizeof(unsigned short);
-    allow_lds((const void*)gemm_bf16x3_nt_kernel<64>, shm);
-    hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<64>, dim3((unsigned)(ntm * ntn)), dim3(256), shm, c->stream, a);
-  } else {
-    const size_t shm = 2 * (size_t)SCfg<32>::STAGE * sizeof(unsigned short);
-    allow_lds((const void*)gemm_bf16x3_nt_kernel<32>, shm);
-    hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<32>, dim3((unsigned)(ntm * ntn)), dim3(256), shm, c->stream, a);
-  }
+  const size_t shm = 2 * (size_t)SCfg<32>::STAGE * sizeof(unsigned short);
+  allow_lds((const void*)gemm_bf16x3_nt_kernel<32>, shm);
+  hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<32>, dim3((unsigned)(ntm * ntn)), dim3(256), shm, c->stream, a);
   TN_HIP(hipGetLastError());
 }
 

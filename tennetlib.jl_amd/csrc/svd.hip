@@ -500,16 +500,13 @@ void sym_eig_jacobi(tn_ctx* c, void* G, long p, long ldg, void* Vt, long ldv, bo
 }
 
 // eigen-decomposition of the padded Hermitian matrix G (ld ppad): divide and
-// conquer from p = 16 (TNHIP_EIG_DC_MIN), two-sided Jacobi below
+// conquer from p = 16, two-sided Jacobi below
 static void eig_padded(tn_ctx* c, void* G, long p, long ppad, void* Vt, bool cplx, GramEig& out) {
   out.p = p;
   out.ppad = ppad;
   out.cplx = cplx;
   out.Vt = Vt;
-  static const long dc_min = [] {  // smallest p for the divide-and-conquer path
-    const char* e = std::getenv("TNHIP_EIG_DC_MIN");
-    return e ? std::atol(e) : 16L;
-  }();
+  constexpr long dc_min = 16;  // smallest p for the divide-and-conquer path
   if (!cplx && p >= std::max(3L, dc_min)) {
     sym_eig_dc(c, (double*)G, p, ppad, (double*)Vt, ppad, out.w);
     out.sweeps = 0;

@@ -177,7 +177,8 @@ enum WsSlot {
   WS_SVD_SCALE, WS_PHI, WS_MISC, WS_ENV_X, WS_ENV_Y, WS_KRY_TMP,
   WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_PROJ_X, WS_PROJ_Y, WS_PROJ_C,
   WS_QR_X, WS_QR_V, WS_QR_Q, WS_QR_R, WS_QR_S, WS_QR_W, WS_TTN_A, WS_TTN_S, WS_TTN_G, WS_TTN_P, WS_TTN_F,
-  WS_GEMM_SPLIT, WS_BOND, WS_PHI2, WS_TTNP_0, WS_TTNP_1, WS_TTNP_2, WS_TTNP_3, WS_DC_PANEL, WS_SLOTS
+  WS_GEMM_SPLIT, WS_BOND, WS_PHI2, WS_TTNP_0, WS_TTNP_1, WS_TTNP_2, WS_TTNP_3, WS_DC_PANEL,
+  WS_QN_M, WS_QN_V, WS_QN_W, WS_QN_IDX, WS_QN_T, WS_SLOTS
 };
 
 // scoped phase timer (HIP events on the context stream)
@@ -367,6 +368,16 @@ void sym_eig_dc_z(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, 
 // complex: out[j][k] = conj?(src[perm[k]][j]) * scale[k]  (src rows of complex)
 void gather_rows_T_z(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
                      long kcount, long L, double* out, bool conj);
+
+// QN blocks of the factorisation (qn.hip).  qn_small_eig: nblk blocks of at
+// most 32 rows of M (ld n); blk_dev[4b..4b+3] = {first row, size p, offset
+// of the eigenvalues in w, offset of the eigenvector rows in V}; one launch.
+void qn_small_eig(tn_ctx* c, const double* M, long n, const int* blk_dev, int nblk, double* w, double* V);
+// qn_scatter: column j of out (rows x k, ld ldo; its transpose when trans)
+// receives eigenvector col[4j+2] (stored at V + col[4j] as rows of length
+// col[4j+3]) on the rows rows[col[4j+1] ..]; out must be zero elsewhere
+void qn_scatter(tn_ctx* c, const double* V, const int* col_dev, const int* rows_dev, int k, long ldo, bool trans,
+                double* out);
 
 // truncation rule of NDTensors truncate!!
 struct TruncRule {

@@ -19,6 +19,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <limits>
+#include <array>
+#include <map>
 #include <numeric>
 
 #include "tn_internal.h"
@@ -150,6 +152,11 @@ struct tn_env {
   // (identity on sites <= k) for the left factor, the last noise_skip_r[k]
   // (identity on sites > k) for the right factor
   std::vector<int> noise_skip_l, noise_skip_r;
+  // quantum numbers (U(1) or U(1)xU(1), nq charges; nq = 0: none): per site
+  // the QN of each basis state, per link 0..N the QN of each link element
+  // (link k carries the QN of sites 1..k)
+  int nq = 0;
+  std::vector<std::vector<std::array<int, 2>>> site_qn, link_qn;
 
   bool cplx() const { return dtype == TN_C128; }
 };
@@ -1286,11 +1293,19 @@ const void* noise_factor(tn_env* e, int b, const double* phi, bool left, long* k
 // noise != 0 adds noise * noiseterm(PH, phi, ortho) to rho (ITensors 0.9
 // factorize_eigen with eigen_perturbation); the spectrum is then that of the
 // perturbed rho and the second factor is normalised by its own norm.
+void replacebond_qn(tn_env* e, int b, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
+                    bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept);
+
 void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
                  bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept,
                  double noise = 0.0) {
   PhaseTimer pt(e->ctx, PH_SVD);
   TN_REQUIRE(b >= 1 && b < e->N, TN_E_POSITION, "replacebond!: bond out of range");
+  if (e->nq > 0) {
+    TN_REQUIRE(std::fabs(noise) <= 1e-15, TN_E_UNSUPPORTED, "QN-conserving factorisation: noise is not built");
+    replacebond_qn(e, b, phi, ortho_left, maxdim, mindim, cutoff, normalize, truncerr_out, eigs_out, nkept);
+    return;
+  }
   tn_ctx* c = e->ctx;
   const bool cx = e->cplx();
   const int w2 = cx ? 2 : 1;  // doubles per element
@@ -1368,11 +1383,191 @@ void replacebond(tn_env* e, int b, const double* phi, bool ortho_left, long maxd
   if (nkept) *nkept = k;
 }
 
+// replacebond! of a QN-conserving two-site tensor (NDTensors BlockSparse
+// semantics, upstream ITensors/NDTensors; called at update_site.jl:64-76):
+// the density matrix of the kept side is block diagonal in the QN of its
+// combined (link, site) index.  Rows of M (ortho left; columns, through the
+// transpose, for ortho right) are gathered sector by sector; blocks up to 32
+// rows are formed and diagonalised in one launch (one wavefront each), larger
+// ones one by one (gram_eig: divide and conquer).  All weights go through
+// truncate!!; each block keeps its weights above docut (whole multiplets split
+// across blocks are dropped together, exact zeros go).  The new link lists
+// the kept sectors in ascending QN order, weights descending within each.
+void replacebond_qn(tn_env* e, int b, const double* phi, bool ortho_left, long maxdim, long mindim, double cutoff,
+                    bool normalize, double* truncerr_out, std::vector<double>* eigs_out, long* nkept) {
+  tn_ctx* c = e->ctx;
+  TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "QN-conserving factorisation: real tensors only");
+  const long chil = e->A[b - 1].chil, chir = e->A[b].chir;
+  const int d1 = e->d[b - 1], d2 = e->d[b];
+  const long m = chil * d1, n = (long)d2 * chir;
+  using Q = std::array<int, 2>;
+  const auto& ql = e->link_qn[b - 1];
+  const auto& qr = e->link_qn[b + 1];
+  const auto& q1 = e->site_qn[b - 1];
+  const auto& q2 = e->site_qn[b];
+  TN_REQUIRE((long)ql.size() == chil && (long)qr.size() == chir, TN_E_SHAPE,
+             "QN-conserving factorisation: link QNs do not match the bond dimensions");
+  // QN of every row of G (G = M for ortho left, M^T for ortho right)
+  const long p = ortho_left ? m : n, L = ortho_left ? n : m;
+  std::vector<Q> qrow(p);
+  if (ortho_left) {
+    for (long a = 0; a < chil; ++a)
+      for (int s = 0; s < d1; ++s) qrow[a * d1 + s] = Q{ql[a][0] + q1[s][0], ql[a][1] + q1[s][1]};
+  } else {
+    for (int t = 0; t < d2; ++t)
+      for (long k = 0; k < chir; ++k) qrow[t * chir + k] = Q{qr[k][0] - q2[t][0], qr[k][1] - q2[t][1]};
+  }
+  std::map<Q, std::vector<int>> rows;
+  for (long r = 0; r < p; ++r) rows[qrow[r]].push_back((int)r);
+  const int ns = (int)rows.size();
+  std::vector<Q> sq;
+  std::vector<int> perm, roff, psz;
+  perm.reserve(p);
+  for (auto& kv : rows) {
+    sq.push_back(kv.first);
+    roff.push_back((int)perm.size());
+    psz.push_back((int)kv.second.size());
+    perm.insert(perm.end(), kv.second.begin(), kv.second.end());
+  }
+  // eigenvalue / eigenvector slots of every block
+  std::vector<long> woff(ns), voff(ns);
+  long wtot = 0, vtot = 0;
+  for (int i = 0; i < ns; ++i) {
+    woff[i] = wtot;
+    voff[i] = vtot;
+    wtot += psz[i];
+    vtot += (long)psz[i] * psz[i];
+  }
+  TN_REQUIRE(vtot < (1L << 31), TN_E_UNSUPPORTED, "QN-conserving factorisation: blocks too large");
+  // G gathered by sector: Gp[i] = G[perm[i]] (p x L)
+  double* Gp = (double*)c->workspace(WS_QN_M, (size_t)p * L * sizeof(double));
+  int* ibuf = (int*)c->workspace(WS_QN_IDX, (size_t)(p + 4 * std::max<long>(ns, p) + 16) * sizeof(int));
+  int* dperm = ibuf;
+  int* dmeta = ibuf + p;
+  double* dones = (double*)c->workspace(WS_SVD_SCALE, (size_t)p * sizeof(double));
+  fill_f64(c, dones, 1.0, p);
+  TN_HIP(hipMemcpyAsync(dperm, perm.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  const double* Gsrc = phi;
+  if (!ortho_left) {
+    double* Mt = (double*)c->workspace(WS_QN_T, (size_t)m * n * sizeof(double));
+    transpose_f64(c, phi, m, n, Mt);
+    Gsrc = Mt;
+  }
+  gather_rows(c, Gsrc, L, dperm, dones, p, L, Gp);
+  double* Wd = (double*)c->workspace(WS_QN_W, (size_t)wtot * sizeof(double));
+  double* Vd = (double*)c->workspace(WS_QN_V, (size_t)std::max<long>(vtot, 1) * sizeof(double));
+  // small blocks: one launch
+  std::vector<int> meta;
+  int nsmall = 0;
+  for (int i = 0; i < ns; ++i)
+    if (psz[i] <= 32) {
+      meta.insert(meta.end(), {roff[i], psz[i], (int)woff[i], (int)voff[i]});
+      ++nsmall;
+    }
+  if (nsmall) {
+    TN_HIP(hipMemcpyAsync(dmeta, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    qn_small_eig(c, Gp, L, dmeta, nsmall, Wd, Vd);
+  }
+  std::vector<std::vector<double>> w(ns);
+  // large blocks: the dense eigensolver, one block at a time
+  for (int i = 0; i < ns; ++i) {
+    if (psz[i] <= 32) continue;
+    GramEig ge;
+    gram_eig(c, Gp + (long)roff[i] * L, psz[i], L, true, false, ge);
+    w[i] = ge.w;
+    TN_HIP(hipMemcpy2DAsync(Vd + voff[i], (size_t)psz[i] * sizeof(double), ge.Vt, (size_t)ge.ppad * sizeof(double),
+                            (size_t)psz[i] * sizeof(double), (size_t)psz[i], hipMemcpyDeviceToDevice, c->stream));
+  }
+  if (nsmall) {
+    std::vector<double> hw(wtot);
+    TN_HIP(hipMemcpyAsync(hw.data(), Wd, wtot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TN_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < ns; ++i)
+      if (psz[i] <= 32) w[i].assign(hw.begin() + woff[i], hw.begin() + woff[i] + psz[i]);
+  }
+  // weights of every block in descending order, then NDTensors' block rule
+  std::vector<std::vector<int>> ord(ns);
+  std::vector<double> allP;
+  allP.reserve(p);
+  for (int i = 0; i < ns; ++i) {
+    ord[i].resize(psz[i]);
+    std::iota(ord[i].begin(), ord[i].end(), 0);
+    std::stable_sort(ord[i].begin(), ord[i].end(), [&](int x, int y) { return w[i][x] > w[i][y]; });
+    for (int x : ord[i]) allP.push_back(std::max(w[i][x], 0.0));
+  }
+  std::sort(allP.begin(), allP.end(), std::greater<double>());
+  const TruncRule tr = truncate_weights(allP, maxdim, mindim, cutoff);
+  std::vector<int> col;      // kept columns: {V offset, row offset, eigenvector, block size}
+  std::vector<double> kept;
+  std::vector<Q> newq;
+  for (int i = 0; i < ns; ++i) {
+    for (int j = 0; j < psz[i]; ++j) {
+      const double x = w[i][ord[i][j]];
+      if (!(x > tr.docut)) break;
+      col.insert(col.end(), {(int)voff[i], roff[i], ord[i][j], psz[i]});
+      kept.push_back(x);
+      newq.push_back(sq[i]);
+    }
+  }
+  const long k = (long)kept.size();
+  TN_REQUIRE(k >= 1, TN_E_NUMERIC, "QN-conserving factorisation: no weight above the truncation threshold");
+  double nrm = 0.0;
+  for (double x : kept) nrm += x;
+  nrm = std::sqrt(nrm);
+  const double inv_norm = (normalize && nrm > 0.0) ? 1.0 / nrm : 1.0;
+  int* dcol = (int*)c->workspace(WS_SVD_PERM, (size_t)4 * k * sizeof(int));
+  TN_HIP(hipMemcpyAsync(dcol, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  SiteT& Sa = e->A[b - 1];
+  SiteT& Sb = e->A[b];
+  Sa.buf.ensure((size_t)m * k * e->esz);
+  Sb.buf.ensure((size_t)k * n * e->esz);
+  if (ortho_left) {
+    // A_b = U (m x k, zero outside each column's sector), A_{b+1} = U^T phi / |.|
+    fill_f64(c, Sa.buf.as<double>(), 0.0, m * k);
+    qn_scatter(c, Vd, dcol, dperm, (int)k, k, false, Sa.buf.as<double>());
+    GemmDesc g;
+    g.M = k; g.N = n; g.K = m;
+    g.A = Sa.buf.p; g.lda = k; g.tA = 1;
+    g.B = phi; g.ldb = n;
+    g.C = Sb.buf.p; g.ldc = n;
+    g.alpha = inv_norm;
+    gemm(c, Prec::F64, g);
+  } else {
+    // A_{b+1} = U^T (k x n), A_b = phi U / |.|
+    fill_f64(c, Sb.buf.as<double>(), 0.0, k * n);
+    qn_scatter(c, Vd, dcol, dperm, (int)k, n, true, Sb.buf.as<double>());
+    GemmDesc g;
+    g.M = m; g.N = k; g.K = n;
+    g.A = phi; g.lda = n;
+    g.B = Sb.buf.p; g.ldb = n; g.tB = 1;
+    g.C = Sa.buf.p; g.ldc = k;
+    g.alpha = inv_norm;
+    gemm(c, Prec::F64, g);
+  }
+  TN_HIP(hipStreamSynchronize(c->stream));  // host index arrays above go out of scope
+  Sa.chir = k;
+  Sb.chil = k;
+  e->link_qn[b] = newq;
+  invalidate_site(e, b);
+  invalidate_site(e, b + 1);
+  e->ortho = ortho_left ? b + 1 : b;
+  if (truncerr_out) *truncerr_out = tr.truncerr;
+  if (eigs_out) {
+    std::sort(kept.begin(), kept.end(), std::greater<double>());
+    eigs_out->swap(kept);
+  }
+  if (nkept) *nkept = k;
+}
+
 // orthogonalize!(psi, site): exact gauge move.  Each step keeps the full row
 // (column) space of the site matrix: the top min(m, n) eigenvectors of its
 // Gram matrix become the orthonormal factor, the exact remainder moves on.
 void orthogonalize(tn_env* e, int site) {
   TN_REQUIRE(site >= 1 && site <= e->N, TN_E_POSITION, "orthogonalize!: site out of range");
+  TN_REQUIRE(e->nq == 0 || e->ortho == site, TN_E_UNSUPPORTED,
+             "QN-conserving environment: the orthocenter moves only through two-site updates "
+             "(start from a right-orthonormal state with orthocenter 1)");
+  if (e->nq > 0) return;
   require_sites(e);
   tn_ctx* c = e->ctx;
   const bool cx = e->cplx();
@@ -1620,6 +1815,7 @@ long split_site(tn_env* e, int pos, const double* phi, bool ortho_left, long max
 // _update_one_site! (src/mps/update_site.jl:94-197)
 void update_one_site(tn_env* e, int pos, bool ortho_left, const tn_update_params* p, tn_update_result* r,
                      std::vector<double>* eigs_out = nullptr) {
+  TN_REQUIRE(e->nq == 0, TN_E_UNSUPPORTED, "QN-conserving environment: one-site updates are not built");
   TN_REQUIRE(p != nullptr, TN_E_INVALID, "null params");
   TN_REQUIRE(pos >= 1 && pos <= e->N, TN_E_POSITION, "update_position!: site out of range");
   TN_REQUIRE(!(p->solver == 0 && p->reverse_step), TN_E_INVALID,
@@ -2004,6 +2200,7 @@ void krylov_addbasis(tn_env* e, tn_env* e2, std::vector<Sites>& phis, double ext
 }
 
 void krylov_extend(tn_env* e, int krylovdim, long applyH_maxdim, double applyH_cutoff, double ext_cutoff) {
+  TN_REQUIRE(e->nq == 0, TN_E_UNSUPPORTED, "QN-conserving environment: subspace expansion is not built");
   TN_REQUIRE(krylovdim >= 1, TN_E_INVALID, "krylov_extend!: extension_krylovdim must be >= 1");
   TN_REQUIRE(e->proj.empty(), TN_E_UNSUPPORTED, "krylov_extend!: the StateEnvs must be created by a single MPO");
   require_mpo(e);
@@ -2108,7 +2305,14 @@ int tn_ctx_destroy(tn_ctx* c) {
   return TN_OK;
 }
 
-const char* tn_last_error(const tn_ctx* c) { return c ? c->last_error.c_str() : g_thread_error.c_str(); }
+// a copy per calling thread, taken under the library lock: the context's
+// message may be replaced by a call on another thread right after
+const char* tn_last_error(const tn_ctx* c) {
+  thread_local std::string copy;
+  std::lock_guard<std::recursive_mutex> lock(api_mutex());
+  copy = c ? c->last_error : g_thread_error;
+  return copy.c_str();
+}
 
 int tn_ctx_synchronize(tn_ctx* c) {
   return guarded(c, [&] {
@@ -2416,6 +2620,49 @@ int tn_env_matvec(tn_env* e, const void* v, void* out, int precision) {
     TN_HIP(hipStreamSynchronize(e->ctx->stream));
   });
 }
+int tn_env_set_qn(tn_env* e, int nq, const int* site_qn, const int* link_qn) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(nq >= 0 && nq <= 2, TN_E_INVALID, "set_qn: nq must be 0, 1 or 2");
+    if (nq == 0) {
+      e->nq = 0;
+      e->site_qn.clear();
+      e->link_qn.clear();
+      return;
+    }
+    TN_REQUIRE(site_qn && link_qn, TN_E_INVALID, "set_qn: null QN array");
+    require_sites(e);
+    TN_REQUIRE(!e->cplx(), TN_E_UNSUPPORTED, "set_qn: QN conservation is built for real tensors");
+    TN_REQUIRE(e->proj.empty(), TN_E_UNSUPPORTED, "set_qn: QN conservation with penalty projectors is not built");
+    std::vector<std::vector<std::array<int, 2>>> sq(e->N), lq(e->N + 1);
+    long o = 0;
+    for (int k = 0; k < e->N; ++k)
+      for (int s2 = 0; s2 < e->d[k]; ++s2, ++o) sq[k].push_back({site_qn[o * nq], nq > 1 ? site_qn[o * nq + 1] : 0});
+    o = 0;
+    for (int k = 0; k <= e->N; ++k) {
+      const long chi = (k == 0) ? e->A[0].chil : e->A[k - 1].chir;
+      for (long a = 0; a < chi; ++a, ++o) lq[k].push_back({link_qn[o * nq], nq > 1 ? link_qn[o * nq + 1] : 0});
+    }
+    for (const auto& q : lq[0]) TN_REQUIRE(q[0] == 0 && q[1] == 0, TN_E_INVALID, "set_qn: link 0 must carry QN 0");
+    e->nq = nq;
+    e->site_qn.swap(sq);
+    e->link_qn.swap(lq);
+  });
+}
+
+int tn_env_get_link_qn(tn_env* e, int k, int* out, long cap, long* n) {
+  return guarded(C(e), [&] {
+    check_env(e);
+    TN_REQUIRE(e->nq > 0, TN_E_INVALID, "get_link_qn: no QNs set");
+    TN_REQUIRE(k >= 0 && k <= e->N, TN_E_INVALID, "get_link_qn: link out of range");
+    const auto& q = e->link_qn[k];
+    if (n) *n = (long)q.size();
+    if (out)
+      for (long a = 0; a < (long)q.size() && (a + 1) * e->nq <= cap; ++a)
+        for (int j = 0; j < e->nq; ++j) out[a * e->nq + j] = q[a][j];
+  });
+}
+
 int tn_env_matvec_noalloc(tn_env* e, const void* v, void* out) {
   return guarded(C(e), [&] {
     check_env(e);

@@ -32,5 +32,6 @@ product = _dispatch(_core.product, _ttn.product)
 update_position_ = _dispatch(_core.update_position_, _ttn.update_position_)
 fullsweep_ = _dispatch(_core.fullsweep_, _ttn.fullsweep_)
 getpsi = _dispatch(_core.getpsi, _ttn.getpsi)
-from .models import (cylinder_couplings, heisenberg_mpo, j1j2_cylinder_mpo, neel_state, product_state,  # noqa: F401
-                     spin_coupling_mpo, xxz_mpo)
+from .models import (cylinder_couplings, heisenberg_mpo, hubbard_mpo, hubbard_occupation_state,  # noqa: F401
+                     hubbard_qns, j1j2_cylinder_mpo, neel_state, product_state, spin_coupling_mpo, spin_half_qns,
+                     xxz_mpo)

@@ -123,6 +123,8 @@ SIGNATURES = {
     "tn_env_phi_dims": [_P, _PL, _PL],
     "tn_env_matvec": [_P, _P, _P, _I],
     "tn_env_matvec_noalloc": [_P, _P, _P],
+    "tn_env_set_qn": [_P, _I, _PI, _PI],
+    "tn_env_get_link_qn": [_P, _I, _PI, _L, _PL],
     "tn_env_contract_phi": [_P, _I, _P],
     "tn_env_orthogonalize": [_P, _I],
     "tn_krylov_eig": [_P, _P, _I, _I, _D, _I, _PD, C.POINTER(KrylovInfo)],

@@ -161,7 +161,7 @@ class StateEnvs:
     over their term ids with the reference's product and noise term."""
 
     def __init__(self, psi: Sequence[np.ndarray], H, Ms=None, weight: Optional[float] = None,
-                 ctx: Optional[Context] = None, dtype=None):
+                 ctx: Optional[Context] = None, dtype=None, qns=None):
         if Ms is not None:
             if weight is None:
                 raise TypeError("StateEnvs(psi, H, Ms; weight): keyword `weight` is required")
@@ -222,6 +222,9 @@ class StateEnvs:
             sr = (C.c_int * (N + 1))(*noise_skip[1])
             self.ctx.check(self.ctx.lib.tn_env_set_noise_channels(self.h, sl, sr))
         self.Ms = None
+        self.nq = 0
+        if qns is not None:
+            self._set_qns(psi, qns)
         if Ms is not None:
             self.Ms = [[np.array(a) for a in M] for M in Ms]
             self.ctx.check(self.ctx.lib.tn_env_set_projectors(self.h, len(Ms), float(weight)))
@@ -277,6 +280,51 @@ class StateEnvs:
 
     def set_orthocenter(self, k: int):
         self.ctx.check(self.ctx.lib.tn_env_set_orthocenter(self.h, k))
+
+    def _set_qns(self, psi, qns):
+        """QN conservation (ConserveQNs, test/test_MPS_DMRG.jl:10,29): qns[k]
+        lists the QN (tuple of 1 or 2 ints) of every basis state of site k.
+        Link QNs are read off psi, which must be block-sparse (every link
+        element a definite QN) and right-orthonormal from site 1 (product
+        states are)."""
+        N = self.N
+        if len(qns) != N or any(len(q) != self.d[k] for k, q in enumerate(qns)):
+            raise ValueError("StateEnvs: qns must list one QN per basis state of every site")
+        nq = len(qns[0][0])
+        if nq not in (1, 2) or any(len(x) != nq for q in qns for x in q):
+            raise ValueError("StateEnvs: QNs must all have 1 or 2 charges")
+        links = [[(0,) * nq]]
+        for k, a in enumerate(psi):
+            a = np.asarray(a)
+            qr = [None] * a.shape[2]
+            for i, s_, j in zip(*np.nonzero(a)):
+                q = tuple(x + y for x, y in zip(links[-1][i], qns[k][s_]))
+                if qr[j] is None:
+                    qr[j] = q
+                elif qr[j] != q:
+                    raise ValueError(f"StateEnvs: link {k + 1} element {j} has no definite QN")
+            links.append([q if q is not None else (0,) * nq for q in qr])
+        sq = np.array([x for q in qns for x in q], dtype=np.int32).reshape(-1)
+        lq = np.array([x for q in links for x in q], dtype=np.int32).reshape(-1)
+        self.ctx.check(self.ctx.lib.tn_env_set_qn(self.h, nq, sq.ctypes.data_as(L._PI), lq.ctypes.data_as(L._PI)))
+        self.nq = nq
+        if all(np.asarray(a).shape[0] == 1 and np.asarray(a).shape[2] == 1 for a in psi):
+            self.set_orthocenter(1)  # a product state is orthonormal about any site
+
+    def link_qns(self, k: int):
+        """QNs of the elements of link k (0..N) of a QN-conserving environment."""
+        n = C.c_long()
+        self.ctx.check(self.ctx.lib.tn_env_get_link_qn(self.h, k, None, 0, C.byref(n)))
+        buf = (C.c_int * max(1, n.value * self.nq))()
+        self.ctx.check(self.ctx.lib.tn_env_get_link_qn(self.h, k, buf, n.value * self.nq, C.byref(n)))
+        return [tuple(buf[a * self.nq:(a + 1) * self.nq]) for a in range(n.value)]
+
+    def sectors(self, k: int) -> dict:
+        """{QN: dimension} of link k."""
+        out = {}
+        for q in self.link_qns(k):
+            out[q] = out.get(q, 0) + 1
+        return out
 
     def phi_dims(self):
         dims = (C.c_long * 4)()

@@ -45,6 +45,67 @@ def heisenberg_mpo(N: int, J: float = 1.0) -> List[np.ndarray]:
     return xxz_mpo(N, J=J, Delta=1.0)
 
 
+def hubbard_site_ops():
+    """Spinful-fermion site in the basis |0>, |up>, |dn>, |updn> = c+_up c+_dn |0>:
+    (I, F, c_up, c_dn) with F = (-1)^n the Jordan-Wigner parity; Op[s', s]."""
+    I = np.eye(4)
+    F = np.diag([1.0, -1.0, -1.0, 1.0])
+    cup = np.zeros((4, 4))
+    cup[0, 1] = 1.0   # |up> -> |0>
+    cup[2, 3] = 1.0   # |updn> -> |dn>
+    cdn = np.zeros((4, 4))
+    cdn[0, 2] = 1.0   # |dn> -> |0>
+    cdn[1, 3] = -1.0  # c_dn c+_up c+_dn |0> = -c+_up |0>
+    return I, F, cup, cdn
+
+
+def hubbard_mpo(N: int, t: float = 1.0, U: float = 4.0) -> List[np.ndarray]:
+    """1D Fermi-Hubbard chain H = -t sum_{i,s} (c+_{i,s} c_{i+1,s} + h.c.) +
+    U sum_i n_{i,up} n_{i,dn}, open boundaries, as a w = 6 MPO with
+    Jordan-Wigner strings (the config of the QN block-sparse benchmark:
+    N = 64, t = 1, U = 4).  Channel 5 is "nothing placed yet", 0 "done";
+    channels 1-4 carry an open hopping term: c+_s F or c_s F was placed at
+    site i (the string F is attached on the operator's own site, so the
+    partner at i+1 needs none), the partner c_s or c+_s closes it."""
+    I, F, cup, cdn = hubbard_site_ops()
+    nup, ndn = cup.T @ cup, cdn.T @ cdn
+    out = []
+    for k in range(N):
+        W = np.zeros((6, 4, 4, 6))
+        W[0, :, :, 0] = I
+        W[1, :, :, 0] = cup
+        W[2, :, :, 0] = cup.T
+        W[3, :, :, 0] = cdn
+        W[4, :, :, 0] = cdn.T
+        W[5, :, :, 0] = U * (nup @ ndn)
+        W[5, :, :, 1] = -t * (cup.T @ F)
+        W[5, :, :, 2] = t * (cup @ F)
+        W[5, :, :, 3] = -t * (cdn.T @ F)
+        W[5, :, :, 4] = t * (cdn @ F)
+        W[5, :, :, 5] = I
+        if k == 0:
+            W = W[5:6]
+        if k == N - 1:
+            W = W[..., 0:1]
+        out.append(np.ascontiguousarray(W))
+    return out
+
+
+def spin_half_qns():
+    """QNs of |up>, |down>: (2 Sz,) = (1,), (-1,)."""
+    return [(1,), (-1,)]
+
+
+def hubbard_qns():
+    """QNs of |0>, |up>, |dn>, |updn>: (N_up, N_dn)."""
+    return [(0, 0), (1, 0), (0, 1), (1, 1)]
+
+
+def hubbard_occupation_state(occ: Sequence[int]) -> List[np.ndarray]:
+    """Product state with site k in basis state occ[k] (0 empty, 1 up, 2 dn, 3 both)."""
+    return product_state([np.eye(4)[o] for o in occ])
+
+
 def product_state(vectors: Sequence[np.ndarray], dtype=np.float64) -> List[np.ndarray]:
     return [np.asarray(v, dtype=dtype).reshape(1, -1, 1).copy() for v in vectors]
 

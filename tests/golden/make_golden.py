@@ -163,7 +163,24 @@ def ed_n26():
     (HERE / "heisenberg_n26_ed.json").write_text(json.dumps(out, indent=1))
 
 
-FIXTURES = {"c1": c1, "matvec": matvec, "c2": c2, "tdvp": tdvp_xxz_n100, "j1j2": j1j2_6x12, "ed26": ed_n26}
+def hubbard_qn(N=32, maxdim=(32, 64, 128, 128), U=4.0, name="hubbard_qn_n32"):
+    """QN-conserving two-site DMRG of the Hubbard chain (t = 1, U = 4, half
+    filling, sector (N/2, N/2)) from the occupation product state up, dn,
+    up, dn, ...: per-sweep energies, bond dimensions and the QN sectors of
+    every link (oracle/tn_oracle_qn.py)."""
+    from oracle import tn_oracle_qn as Q
+    occ = [1, 2] * (N // 2)
+    psi = O.product_mps([np.eye(4)[o] for o in occ])
+    st = Q.QNState(psi, O.hubbard_mpo(N, 1.0, U), [Q.hubbard_qns()] * N)
+    sw = Q.dmrg(st, list(maxdim), 1e-12)
+    out = dict(model="hubbard", N=N, t=1.0, U=U, sector=[N // 2, N // 2], start="up,dn,...", maxdim=list(maxdim),
+               cutoff=1e-12, krylovdim=5, maxiter=2, energy=sw.energy, maxchi=sw.maxchi, maxtruncerr=sw.maxtruncerr,
+               sectors=[{f"{q[0]},{q[1]}": n for q, n in sorted(st.sectors(k).items())} for k in range(N + 1)])
+    (HERE / f"{name}.json").write_text(json.dumps(out, indent=1))
+
+
+FIXTURES = {"c1": c1, "matvec": matvec, "c2": c2, "tdvp": tdvp_xxz_n100, "j1j2": j1j2_6x12, "ed26": ed_n26,
+            "hubbard_qn": hubbard_qn}
 
 if __name__ == "__main__":
     for name in (sys.argv[1:] or FIXTURES):

@@ -62,3 +62,51 @@ def test_chain_builder_matches_fsm_heisenberg():
     Ws = M.spin_coupling_mpo(N, J)
     assert max(w.shape[3] for w in Ws[:-1]) == 5  # compressed to the FSM bond dimension
     assert np.abs(O.mpo_to_dense(Ws) - O.mpo_to_dense(O.heisenberg_mpo(N))).max() < 1e-12
+
+
+def _fermion_ops(N):
+    """c_{i,s} on N sites (modes ordered 1up, 1dn, 2up, ...) by Jordan-Wigner
+    from the site operators; checked to satisfy {c_a, c_b^+} = delta_ab."""
+    from tennetlib_amd import models as M
+    I, F, cup, cdn = M.hubbard_site_ops()
+    ops = {}
+    for i in range(N):
+        for s, c in (("up", cup), ("dn", cdn)):
+            mats = [F] * i + [c] + [I] * (N - i - 1)
+            op = mats[0]
+            for m in mats[1:]:
+                op = np.kron(op, m)
+            ops[(i, s)] = op
+    return ops
+
+
+def test_hubbard_mpo_is_second_quantised_hamiltonian():
+    """The package's Jordan-Wigner MPO equals -t sum (c+_i c_{i+1} + h.c.) +
+    U sum n_up n_dn built from fermion operators whose anticommutators are
+    checked; at U = 0 the sector ground energies are free-fermion sums."""
+    import itertools
+    from oracle import tn_oracle as O
+    from tennetlib_amd import models as M
+    N, t, U = 4, 0.7, 3.1
+    c = _fermion_ops(N)
+    keys = list(c)
+    dim = 4 ** N
+    for a, b in itertools.product(keys, keys):
+        ac = c[a] @ c[b].T + c[b].T @ c[a]
+        assert np.abs(ac - (np.eye(dim) if a == b else 0)).max() < 1e-12
+        assert np.abs(c[a] @ c[b] + c[b] @ c[a]).max() < 1e-12
+    H = np.zeros((dim, dim))
+    for i in range(N - 1):
+        for s in ("up", "dn"):
+            hop = c[(i, s)].T @ c[(i + 1, s)]
+            H -= t * (hop + hop.T)
+    for i in range(N):
+        H += U * (c[(i, "up")].T @ c[(i, "up")]) @ (c[(i, "dn")].T @ c[(i, "dn")])
+    assert np.abs(O.mpo_to_dense(M.hubbard_mpo(N, t, U)) - H).max() < 1e-12
+    # U = 0: ground energy of the (2, 2) sector
+    H0 = O.mpo_to_dense(M.hubbard_mpo(N, 1.0, 0.0))
+    qs = [tuple(np.sum([M.hubbard_qns()[x] for x in idx], axis=0)) for idx in itertools.product(range(4), repeat=N)]
+    sel = [k for k, q in enumerate(qs) if q == (2, 2)]
+    e = np.linalg.eigvalsh(H0[np.ix_(sel, sel)])[0]
+    lv = sorted(-2.0 * np.cos(np.pi * k / (N + 1)) for k in range(1, N + 1))
+    assert abs(e - 2 * (lv[0] + lv[1])) < 1e-12
