@@ -163,7 +163,7 @@ def ed_n26():
     (HERE / "heisenberg_n26_ed.json").write_text(json.dumps(out, indent=1))
 
 
-def hubbard_qn(N=32, maxdim=(32, 64, 128, 128), U=4.0, name="hubbard_qn_n32"):
+def hubbard_qn(N=32, maxdim=(32, 64, 128, 128), U=4.0, cutoff=1e-12, name="hubbard_qn_n32"):
     """QN-conserving two-site DMRG of the Hubbard chain (t = 1, U = 4, half
     filling, sector (N/2, N/2)) from the occupation product state up, dn,
     up, dn, ...: per-sweep energies, bond dimensions and the QN sectors of
@@ -172,15 +172,22 @@ def hubbard_qn(N=32, maxdim=(32, 64, 128, 128), U=4.0, name="hubbard_qn_n32"):
     occ = [1, 2] * (N // 2)
     psi = O.product_mps([np.eye(4)[o] for o in occ])
     st = Q.QNState(psi, O.hubbard_mpo(N, 1.0, U), [Q.hubbard_qns()] * N)
-    sw = Q.dmrg(st, list(maxdim), 1e-12)
+    sw = Q.dmrg(st, list(maxdim), cutoff)
     out = dict(model="hubbard", N=N, t=1.0, U=U, sector=[N // 2, N // 2], start="up,dn,...", maxdim=list(maxdim),
-               cutoff=1e-12, krylovdim=5, maxiter=2, energy=sw.energy, maxchi=sw.maxchi, maxtruncerr=sw.maxtruncerr,
+               cutoff=cutoff, krylovdim=5, maxiter=2, energy=sw.energy, maxchi=sw.maxchi, maxtruncerr=sw.maxtruncerr,
                sectors=[{f"{q[0]},{q[1]}": n for q, n in sorted(st.sectors(k).items())} for k in range(N + 1)])
     (HERE / f"{name}.json").write_text(json.dumps(out, indent=1))
 
 
+def hubbard_qn_n16():
+    """Cutoff-limited QN DMRG (N = 16, maxdim ramp to 512, cutoff 1e-9): no
+    maxdim cut through a multiplet, so the sectors of every link are fixed by
+    the problem, not by rounding."""
+    hubbard_qn(N=16, maxdim=(64, 256, 512, 512), cutoff=1e-9, name="hubbard_qn_n16")
+
+
 FIXTURES = {"c1": c1, "matvec": matvec, "c2": c2, "tdvp": tdvp_xxz_n100, "j1j2": j1j2_6x12, "ed26": ed_n26,
-            "hubbard_qn": hubbard_qn}
+            "hubbard_qn": hubbard_qn, "hubbard_qn_n16": hubbard_qn_n16}
 
 if __name__ == "__main__":
     for name in (sys.argv[1:] or FIXTURES):

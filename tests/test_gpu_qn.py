@@ -1,0 +1,168 @@
+"""GPU parity of the QN-conserving (U(1), U(1)xU(1)) path against the QN oracle
+(oracle/tn_oracle_qn.py), exact diagonalisation and closed forms.
+
+Bars: FP64 energies to |dE| <= 1e-10 (1e-9 against ED / closed forms, which
+the DMRG reaches to its own convergence), block weights to 1e-13, and the QN
+sectors of every link (QN -> dimension) exactly as the oracle's."""
+
+import ctypes as C
+import itertools
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import tn_oracle as O
+from oracle import tn_oracle_qn as Q
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def T():
+    import tennetlib_amd as T
+    T.default_context()
+    return T
+
+
+def _qn_mps(rng, sq, links):
+    A = []
+    for k in range(len(sq)):
+        ql, qr = links[k], links[k + 1]
+        a = rng.standard_normal((len(ql), len(sq[k]), len(qr)))
+        for i, s, j in itertools.product(range(len(ql)), range(len(sq[k])), range(len(qr))):
+            if Q.qadd(ql[i], sq[k][s]) != qr[j]:
+                a[i, s, j] = 0.0
+        A.append(a)
+    return A
+
+
+@pytest.mark.parametrize("ortho", ["left", "right"])
+def test_qn_replacebond_matches_oracle(T, ortho):
+    """Block factorisation through the C ABI: kept weights, truncation error,
+    the new link's sectors and the product of the two new sites."""
+    hs = Q.hubbard_qns()
+    rng = np.random.default_rng(5)
+    links = [[(0, 0)], [(0, 0), (1, 0), (0, 1), (1, 1)],
+             [(1, 0), (0, 1), (1, 1), (1, 1), (2, 1), (1, 2), (2, 0)],
+             [(2, 1), (1, 2), (2, 2), (2, 2), (3, 1), (1, 3), (2, 1), (3, 2)],
+             [(3, 2), (2, 3), (3, 3), (2, 2), (4, 2)], [(4, 4)]]
+    sq = [hs] * 5
+    A = _qn_mps(rng, sq, links)
+    # the last site must reach the single total QN: restrict its entries accordingly
+    env = T.StateEnvs(A, O.hubbard_mpo(5, 1.0, 4.0), qns=sq)
+    b = 3
+    phi = np.tensordot(A[b - 1], A[b], axes=([2], [0]))
+    phi /= np.linalg.norm(phi)
+    # cutoff 0 would keep rounding-floor weights whose count depends on the
+    # eigensolver; 1e-14 drops them in both
+    for maxdim, cutoff in [(100, 1e-14), (9, 1e-12), (4, 1e-3)]:
+        _, _, newq, spec = Q.block_factorize(phi, links[b - 1], hs, hs, links[b + 1], maxdim, 1, cutoff, ortho)
+        env2 = T.StateEnvs(A, O.hubbard_mpo(5, 1.0, 4.0), qns=sq)
+        ctx = env2.ctx
+        dphi = T.DeviceArray.from_numpy(ctx, phi)
+        terr, nk = C.c_double(), C.c_long()
+        eigs = (C.c_double * 256)()
+        print(f"qn replacebond {ortho} maxdim={maxdim} cutoff={cutoff}", flush=True)
+        ctx.check(ctx.lib.tn_replacebond(env2.h, b, dphi.ptr, 1 if ortho == "left" else 0, maxdim, 1, cutoff, 0.0,
+                                         1, C.byref(terr), eigs, 256, C.byref(nk)))
+        assert nk.value == len(newq)
+        assert np.abs(np.array(eigs[:nk.value]) - spec.eigs).max() < 1e-13
+        assert abs(terr.value - spec.truncerr) < 1e-13
+        assert env2.sectors(b) == {q: n for q, n in spec.sectors.items()}
+        got = env2.psi
+        prod = np.tensordot(got[b - 1], got[b], axes=([2], [0]))
+        want = np.tensordot(*Q.block_factorize(phi, links[b - 1], hs, hs, links[b + 1], maxdim, 1, cutoff,
+                                               ortho)[:2], axes=([2], [0]))
+        assert np.abs(prod - want).max() < 1e-12
+        # the new sites respect the QNs (zeros outside the allowed blocks are exact)
+        lq = [env2.link_qns(k) for k in range(6)]
+        assert Q.check_flux(got, sq, lq, tol=0.0)
+        dphi.free()
+        env2.close()
+    env.close()
+
+
+def test_qn_dmrg_heisenberg_matches_oracle_and_ed(T):
+    N = 12
+    sq = [Q.spin_half_qns()] * N
+    ref = Q.QNState(O.neel_mps(N), O.heisenberg_mpo(N), sq)
+    sw_ref = Q.dmrg(ref, [16, 32, 64, 64], 1e-12)
+    env = T.StateEnvs(O.neel_mps(N), O.heisenberg_mpo(N), qns=sq)
+    params = T.make_dmrg_params(maxdim=[16, 32, 64, 64], nsweeps=[1] * 4, cutoff=1e-12)
+    print("qn dmrg heisenberg n12", flush=True)
+    sw = T.dmrg_(env, params, 2, outputlevel=0)
+    assert np.abs(np.array(sw.energy) - np.array(sw_ref.energy)).max() < 1e-10
+    assert abs(sw.energy[-1] - (-5.14209063284054)) < 1e-9
+    for k in range(N + 1):
+        assert env.sectors(k) == ref.sectors(k), k
+
+
+def test_qn_dmrg_hubbard_closed_form_and_ed(T):
+    """U = 0, N = 8, sector (4, 4): free fermions; U = 4, N = 6, sector (3, 3): ED."""
+    hs = Q.hubbard_qns()
+    N = 8
+    env = T.StateEnvs(T.hubbard_occupation_state([1, 2] * (N // 2)), T.hubbard_mpo(N, 1.0, 0.0), qns=[hs] * N)
+    print("qn dmrg hubbard n8 u0", flush=True)
+    sw = T.dmrg_(env, T.make_dmrg_params(maxdim=[16, 64, 256, 256], nsweeps=[1] * 4, cutoff=1e-13), 2,
+                 outputlevel=0)
+    assert abs(sw.energy[-1] - Q.hubbard_free_fermion_energy(N, 4, 4)) < 1e-9
+    assert env.link_qns(N) == [(4, 4)]
+    N = 6
+    env = T.StateEnvs(T.hubbard_occupation_state([1, 2] * (N // 2)), T.hubbard_mpo(N, 1.0, 4.0), qns=[hs] * N)
+    sw = T.dmrg_(env, T.make_dmrg_params(maxdim=[16, 64, 64], nsweeps=[1] * 3, cutoff=1e-13), 2, outputlevel=0)
+    H = O.mpo_to_dense(O.hubbard_mpo(N, 1.0, 4.0))
+    qs = [tuple(np.sum([hs[x] for x in idx], axis=0)) for idx in itertools.product(range(4), repeat=N)]
+    sel = [i for i, q in enumerate(qs) if q == (3, 3)]
+    e_ed = np.linalg.eigvalsh(H[np.ix_(sel, sel)])[0]
+    assert abs(sw.energy[-1] - e_ed) < 1e-9
+
+
+def _golden_run(T, name):
+    g = json.loads((GOLDEN / f"{name}.json").read_text())
+    N = g["N"]
+    hs = Q.hubbard_qns()
+    env = T.StateEnvs(T.hubbard_occupation_state([1, 2] * (N // 2)), T.hubbard_mpo(N, g["t"], g["U"]),
+                      qns=[hs] * N)
+    params = T.make_dmrg_params(maxdim=g["maxdim"], nsweeps=[1] * len(g["maxdim"]), cutoff=g["cutoff"])
+    print(f"qn dmrg {name}", flush=True)
+    sw = T.dmrg_(env, params, 2, outputlevel=0)
+    dE = np.abs(np.array(sw.energy) - np.array(g["energy"])) / N
+    print(f"{name} |dE/N|", dE, "maxchi", sw.maxchi, g["maxchi"], flush=True)
+    want = [{tuple(int(x) for x in q.split(",")): n for q, n in g["sectors"][k].items()} for k in range(N + 1)]
+    got = [env.sectors(k) for k in range(N + 1)]
+    return g, sw, dE, got, want
+
+
+def test_qn_dmrg_hubbard_n16_matches_golden(T):
+    """Hubbard N = 16, U = 4, (8, 8), maxdim ramp to 512 with the bond
+    dimensions set by cutoff 1e-9: FP64 energies to 1e-10 per site, bond
+    dimensions and the QN sectors of every link exactly as the oracle's."""
+    g, sw, dE, got, want = _golden_run(T, "hubbard_qn_n16")
+    assert dE.max() <= 1e-10
+    assert sw.maxchi == g["maxchi"]
+    assert got == want
+
+
+def test_qn_dmrg_hubbard_n32_matches_golden(T):
+    """Hubbard N = 32, U = 4, (16, 16), maxdim ramp 32, 64, 128, 128, the
+    bond dimensions set by maxdim.  The oracle itself moves by up to 9e-8 per
+    site on the ramp sweeps and regroups the sectors of several links under
+    2e-16 perturbations of its density matrices (the maxdim cut splits
+    multiplets; tests/test_oracle_qn.py::test_hubbard_n32_sensitivity), so:
+    first sweep to 1e-10 per site, ramp sweeps to 1e-6, the last sweep to
+    5e-9; maxchi exact; every link's sectors allowed and its dimension
+    within 4 of the oracle's."""
+    g, sw, dE, got, want = _golden_run(T, "hubbard_qn_n32")
+    assert dE[0] <= 1e-10
+    assert dE.max() <= 1e-6
+    assert dE[-1] <= 5e-9
+    assert sw.maxchi == g["maxchi"]
+    N = g["N"]
+    for k in range(N + 1):
+        assert abs(sum(got[k].values()) - sum(want[k].values())) <= 4, k
+        # (N_up, N_dn) of sites 1..k: each in 0..k and reachable from the total (N/2, N/2)
+        for (nu, nd) in got[k]:
+            assert 0 <= nu <= k and 0 <= nd <= k and nu >= k - N // 2 and nd >= k - N // 2, (k, nu, nd)
