@@ -10,6 +10,8 @@
 // prefetch; a workgroup is 4 wavefronts (2 x 2) of 64 lanes.
 #include "tn_internal.h"
 
+#include <type_traits>
+
 namespace tn {
 namespace {
 
@@ -621,6 +623,125 @@ __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(GroupedArgs ga) {
       }
 }
 
+// ------------------------------------------------------------------------
+// Block-sparse GEMM over uneven blocks (QN sectors): one launch covers a list
+// of output tiles; tile t belongs to task tiles[t].x, whose output block
+// C + c_off (M x N, ld ldc) is the sum over the task's K segments of
+// A_s (M x K_s, ld lda) * B_s (K_s x N, ld ldb), A_s = A + a_off, B_s = B +
+// b_off.  A task without segments writes zeros.  The tile list is laid out
+// XCD-major: workgroup id i runs list entry (i % 8) * per + i / 8, so the
+// contiguous run of tiles one XCD runs shares its tasks' A and B panels in
+// that XCD's L2; padding entries (task < 0) exit at once.
+// ------------------------------------------------------------------------
+struct BsArgs {
+  const void* A; long lda;
+  const void* B; long ldb;
+  void* C; long ldc;
+  const BsTask* tasks;
+  const BsSeg* segs;
+  const int4* tiles;
+  long per;
+  double alpha;
+};
+
+template <typename T, int BM, int BN, int BK>
+__global__ __launch_bounds__(256) void gemm_bs_kernel(BsArgs a) {
+  using TT = RealTile<T, BM, BN, BK>;
+  constexpr bool F64 = sizeof(T) == 8;
+  constexpr int LA = BM + TT::PAD, LB = BN + TT::PAD;
+  constexpr int WTM = BM / 2, WTN = BN / 2;
+  constexpr int MB = F64 ? 16 : 32;  // MFMA block edge
+  constexpr int FM = WTM / MB, FN = WTN / MB;
+  typedef typename std::conditional<F64, d4, f16v>::type Acc;
+  __shared__ T As[BK * LA];
+  __shared__ T Bs[BK * LB];
+  const long id = (long)(blockIdx.x % 8) * a.per + blockIdx.x / 8;
+  const int4 tl = a.tiles[id];
+  if (tl.x < 0) return;
+  const BsTask tk = a.tasks[tl.x];
+  const long m0 = tl.y, n0 = tl.z;
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  GArgs g;
+  g.M = tk.M; g.N = tk.N; g.K = 0;
+  g.lda = a.lda; g.tA = 0; g.cA = 0;
+  g.ldb = a.ldb; g.tB = 0; g.cB = 0;
+  Acc acc[FM][FN];
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) acc[i][j] = Acc{};
+  for (int sgi = tk.seg0; sgi < tk.seg0 + tk.nseg; ++sgi) {
+    const BsSeg sg = a.segs[sgi];
+    g.K = sg.K;
+    const T* A = reinterpret_cast<const T*>(a.A) + sg.a_off;
+    const T* B = reinterpret_cast<const T*>(a.B) + sg.b_off;
+    T ra[TT::AE], rb[TT::BE];
+    load_tiles<T, BM, BN, BK>(g, A, B, m0, n0, 0, ra, rb);
+    for (long k0 = 0; k0 < g.K; k0 += BK) {
+      __syncthreads();
+      store_tiles<T, BM, BN, BK>(g, As, Bs, ra, rb);
+      __syncthreads();
+      if (k0 + BK < g.K) load_tiles<T, BM, BN, BK>(g, A, B, m0, n0, k0 + BK, ra, rb);
+      if constexpr (F64) {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+          const int kr = kk + (lane >> 4);
+          double fa[FM], fb[FN];
+#pragma unroll
+          for (int i = 0; i < FM; ++i) fa[i] = As[kr * LA + wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+          for (int j = 0; j < FN; ++j) fb[j] = Bs[kr * LB + wn * WTN + j * 16 + (lane & 15)];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+      } else {
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 2) {
+          const int kr = kk + (lane >> 5);
+          float fa[FM], fb[FN];
+#pragma unroll
+          for (int i = 0; i < FM; ++i) fa[i] = As[kr * LA + wm * WTM + i * 32 + (lane & 31)];
+#pragma unroll
+          for (int j = 0; j < FN; ++j) fb[j] = Bs[kr * LB + wn * WTN + j * 32 + (lane & 31)];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+      }
+    }
+  }
+  T* C = reinterpret_cast<T*>(a.C) + tk.c_off;
+#pragma unroll
+  for (int i = 0; i < FM; ++i)
+#pragma unroll
+    for (int j = 0; j < FN; ++j) {
+      if constexpr (F64) {
+        // register r of lane: row lane/16 + 4r, column lane%16 of the 16x16 block
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long m = m0 + wm * WTM + i * 16 + (lane >> 4) + 4 * r;
+          const long n = n0 + wn * WTN + j * 16 + (lane & 15);
+          if (m < g.M && n < g.N) C[m * a.ldc + n] = a.alpha * acc[i][j][r];
+        }
+      } else {
+        // register r of lane: row 8(r/4) + 4(lane/32) + r%4, column lane%32
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const long m = m0 + wm * WTM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const long n = n0 + wn * WTN + j * 32 + (lane & 31);
+          if (m < g.M && n < g.N) C[m * a.ldc + n] = (float)a.alpha * acc[i][j][r];
+        }
+      }
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -801,6 +922,26 @@ void gemm_grouped_f64(tn_ctx* c, const GroupedDesc& d) {
   TN_REQUIRE(tiles < (1L << 31), TN_E_INVALID, "grouped gemm: too many tiles");
   dim3 gr((unsigned)tiles, (unsigned)d.ngroups, (unsigned)d.batch);
   hipLaunchKernelGGL((gemm_f64_grouped_kernel<64, 64, 16>), gr, dim3(256), 0, c->stream, ga);
+  TN_HIP(hipGetLastError());
+}
+
+void gemm_blocksparse(tn_ctx* c, Prec p, const BsDesc& d) {
+  if (d.ntiles <= 0) return;
+  TN_REQUIRE(p == Prec::F64 || p == Prec::F32, TN_E_INVALID, "block-sparse gemm: FP64 or FP32 only");
+  TN_REQUIRE(d.ntiles % 8 == 0 && d.ntiles < (1L << 31), TN_E_INVALID, "block-sparse gemm: bad tile list");
+  BsArgs a;
+  a.A = d.A; a.lda = d.lda;
+  a.B = d.B; a.ldb = d.ldb;
+  a.C = d.C; a.ldc = d.ldc;
+  a.tasks = d.tasks; a.segs = d.segs; a.tiles = d.tiles;
+  a.per = d.ntiles / 8;
+  a.alpha = d.alpha;
+  if (p == Prec::F64)
+    hipLaunchKernelGGL((gemm_bs_kernel<double, BS_TILE, BS_TILE, 16>), dim3((unsigned)d.ntiles), dim3(256), 0,
+                       c->stream, a);
+  else
+    hipLaunchKernelGGL((gemm_bs_kernel<float, BS_TILE, BS_TILE, 16>), dim3((unsigned)d.ntiles), dim3(256), 0,
+                       c->stream, a);
   TN_HIP(hipGetLastError());
 }
 

@@ -228,6 +228,32 @@ struct GroupedDesc {
   int ngroups = 0, batch = 1;
 };
 void gemm_grouped_f64(tn_ctx* c, const GroupedDesc& d);
+// Block-sparse GEMM over uneven blocks (gemm.hip): task i writes the block
+// C + c_off (M x N, ld ldc) = alpha * sum over its segments s of
+// (A + a_off_s)(M x K_s, ld lda) * (B + b_off_s)(K_s x N, ld ldb); a task
+// without segments writes zeros.  tiles (device) lists {task, m0, n0, 0} of
+// the BS_TILE x BS_TILE output tiles in XCD-major order: entry (i % 8) *
+// (ntiles / 8) + i / 8 runs as workgroup i; ntiles is a multiple of 8 and
+// padding entries have task -1.  Offsets are in elements of the precision.
+constexpr int BS_TILE = 64;
+struct BsTask {
+  long c_off;
+  int M, N, seg0, nseg;
+};
+struct BsSeg {
+  long a_off, b_off, K;
+};
+struct BsDesc {
+  const void* A = nullptr; long lda = 0;
+  const void* B = nullptr; long ldb = 0;
+  void* C = nullptr; long ldc = 0;
+  const BsTask* tasks = nullptr;
+  const BsSeg* segs = nullptr;
+  const int4* tiles = nullptr;
+  long ntiles = 0;
+  double alpha = 1.0;
+};
+void gemm_blocksparse(tn_ctx* c, Prec p, const BsDesc& d);
 // double-buffered FP64/FP32 kernels (gemm_fast.hip); gemm() uses them when
 // gemm_fast_ok() holds (shapes >= one tile, 16-byte aligned operand rows)
 bool gemm_fast_ok(Prec p, const GemmDesc& g);

@@ -122,6 +122,25 @@ struct ProjT {
   int m_pos = 0, m_nsite = 0;
 };
 
+// Block-sparse two-site product of a QN-conserving environment at one
+// position: the task lists of the two GEMMs (uploaded once per position and
+// link sectors), see qn_plan.
+struct QnMvPlan {
+  long key[4] = {-1, -1, -1, -1};  // pos, versions of links pos-1 and pos+1, MPO version
+  bool ok = false;                 // false: links not sector-contiguous, use the dense product
+  std::vector<BsTask> t1, t3;
+  std::vector<BsSeg> s1, s3;
+  std::vector<int4> tl1, tl3;
+  DevBuf dev;
+  const BsTask* dt1 = nullptr;
+  const BsTask* dt3 = nullptr;
+  const BsSeg* ds1 = nullptr;
+  const BsSeg* ds3 = nullptr;
+  const int4* dtl1 = nullptr;
+  const int4* dtl3 = nullptr;
+  double flops = 0.0;  // block-sparse GEMM flops of one product
+};
+
 struct tn_env {
   tn_ctx* ctx = nullptr;
   int N = 0;
@@ -157,6 +176,15 @@ struct tn_env {
   // (link k carries the QN of sites 1..k)
   int nq = 0;
   std::vector<std::vector<std::array<int, 2>>> site_qn, link_qn;
+  std::vector<long> link_ver;  // bumped whenever link_qn[k] changes
+  long mpo_ver = 0;            // bumped by every set_mpo
+  // QN flux of every MPO link element: bra QN - ket QN of the environments
+  // on that channel (mpo_flux[k][w], k = 0..N); flux_known[k][w] = 0 for
+  // channels no nonzero path from the left edge reaches
+  long flux_ver = -1;
+  std::vector<std::vector<std::array<int, 2>>> mpo_flux;
+  std::vector<std::vector<char>> flux_known;
+  QnMvPlan qplan;
 
   bool cplx() const { return dtype == TN_C128; }
 };
@@ -581,6 +609,287 @@ void matvec_zero(tn_env* e, const Dom& D, const void* v, void* out) {
   gemm(c, P, g2);
 }
 
+// ---------------------------------------------------------------------------
+// QN-conserving products: block-sparse GEMMs over the link sectors
+// ---------------------------------------------------------------------------
+using QN2 = std::array<int, 2>;
+inline QN2 qplus(const QN2& a, const QN2& b) { return QN2{a[0] + b[0], a[1] + b[1]}; }
+inline QN2 qminus(const QN2& a, const QN2& b) { return QN2{a[0] - b[0], a[1] - b[1]}; }
+
+// flux of every MPO channel, propagated from the left edge (flux 0): a
+// nonzero W_k[wl, s', s, wr] carries flux(wr) = flux(wl) + q(s') - q(s).
+// An entry that contradicts it, or a right edge with flux != 0, means the
+// MPO does not conserve the QNs.
+void infer_mpo_flux(tn_env* e) {
+  if (e->flux_ver == e->mpo_ver) return;
+  const int N = e->N;
+  e->mpo_flux.assign(N + 1, {});
+  e->flux_known.assign(N + 1, {});
+  for (int k = 0; k <= N; ++k) {
+    e->mpo_flux[k].assign(e->w[k], QN2{0, 0});
+    e->flux_known[k].assign(e->w[k], 0);
+  }
+  e->flux_known[0][0] = 1;
+  for (int k = 1; k <= N; ++k) {
+    const int wl = e->w[k - 1], wr = e->w[k], d = e->d[k - 1];
+    const auto& W = e->Wh[k - 1];
+    const auto& q = e->site_qn[k - 1];
+    for (int a = 0; a < wl; ++a) {
+      if (!e->flux_known[k - 1][a]) continue;
+      for (int sp = 0; sp < d; ++sp)
+        for (int s = 0; s < d; ++s)
+          for (int b = 0; b < wr; ++b) {
+            if (W[((size_t)(a * d + sp) * d + s) * wr + b] == 0.0) continue;
+            const QN2 f = qplus(e->mpo_flux[k - 1][a], qminus(q[sp], q[s]));
+            if (!e->flux_known[k][b]) {
+              e->flux_known[k][b] = 1;
+              e->mpo_flux[k][b] = f;
+            } else {
+              TN_REQUIRE(e->mpo_flux[k][b] == f, TN_E_INVALID,
+                         "QN conservation: the MPO tensor of site " + std::to_string(k) +
+                             " does not conserve the QNs");
+            }
+          }
+    }
+  }
+  TN_REQUIRE(!e->flux_known[N][0] || e->mpo_flux[N][0] == (QN2{0, 0}), TN_E_INVALID,
+             "QN conservation: the MPO changes the total QN");
+  e->flux_ver = e->mpo_ver;
+}
+
+struct SectorRange {
+  QN2 q;
+  long off, n;
+};
+// sectors of a link as contiguous ranges; false if some QN occurs in two runs
+bool link_sectors(const std::vector<QN2>& lq, std::vector<SectorRange>& out, std::map<QN2, int>& index) {
+  out.clear();
+  index.clear();
+  for (long a = 0; a < (long)lq.size(); ++a) {
+    if (!out.empty() && out.back().q == lq[a]) {
+      ++out.back().n;
+      continue;
+    }
+    if (index.count(lq[a])) return false;
+    index[lq[a]] = (int)out.size();
+    out.push_back({lq[a], a, 1});
+  }
+  return true;
+}
+
+// append the BS_TILE x BS_TILE tiles of task t to tl
+void push_tiles(std::vector<int4>& tl, int t, int M, int N) {
+  for (int m0 = 0; m0 < M; m0 += BS_TILE)
+    for (int n0 = 0; n0 < N; n0 += BS_TILE) tl.push_back(int4{t, m0, n0, 0});
+}
+void pad_tiles(std::vector<int4>& tl) {
+  while (tl.size() % 8) tl.push_back(int4{-1, 0, 0, 0});
+}
+
+// Task lists of the two-site product at position pos (nsite 2) with
+// T1[(w a'), (S b)] = Lt[(w a'), a] phi[a, (S b)]  and
+// out[(a' S'), b'] = T2[(a' S'), (w2 b)] Rt[(w2 b), b']:
+//  * GEMM 1: one task per (w, sector Qa' of link pos-1, S) whose ket sector
+//    Qa = Qa' - flux(w) exists, output block columns b in Qb = Qa + q(S)
+//    (tasks without Qa write zeros, so every T1 block the W step can carry
+//    into a block GEMM 2 reads is written);
+//  * GEMM 2: one task per (Qa', S') with Qb' = Qa' + q(S') on link pos+1,
+//    one K segment per channel w2 (ket sector Qb = Qb' - flux(w2)).
+// Blocks of T1 and T2 outside these are never read; out is zero-filled.
+const QnMvPlan* qn_plan(tn_env* e, int pos) {
+  QnMvPlan& P = e->qplan;
+  const long key[4] = {pos, e->link_ver[pos - 1], e->link_ver[pos + 1], e->mpo_ver};
+  if (std::equal(key, key + 4, P.key)) return P.ok ? &P : nullptr;
+  P.ok = false;
+  infer_mpo_flux(e);
+  std::copy(key, key + 4, P.key);
+  std::vector<SectorRange> sa, sb;
+  std::map<QN2, int> ia, ib;
+  if (!link_sectors(e->link_qn[pos - 1], sa, ia) || !link_sectors(e->link_qn[pos + 1], sb, ib)) return nullptr;
+  const long chil = (long)e->link_qn[pos - 1].size(), chir = (long)e->link_qn[pos + 1].size();
+  const int d1 = e->d[pos - 1], d2 = e->d[pos], DD = d1 * d2;
+  const int w0 = e->w[pos - 1], w2 = e->w[pos + 1];
+  std::vector<QN2> qS(DD);
+  for (int s1 = 0; s1 < d1; ++s1)
+    for (int s2 = 0; s2 < d2; ++s2) qS[s1 * d2 + s2] = qplus(e->site_qn[pos - 1][s1], e->site_qn[pos][s2]);
+  P.t1.clear(); P.t3.clear(); P.s1.clear(); P.s3.clear(); P.tl1.clear(); P.tl3.clear();
+  double flops = 0.0;
+  const long ld1 = (long)DD * chir;
+  auto task1 = [&](int w, const SectorRange& ap, int S, const SectorRange& bq, const SectorRange* aq) {
+    BsTask t;
+    t.c_off = ((long)w * chil + ap.off) * ld1 + (long)S * chir + bq.off;
+    t.M = (int)ap.n;
+    t.N = (int)bq.n;
+    t.seg0 = (int)P.s1.size();
+    t.nseg = 0;
+    if (aq) {
+      BsSeg g;
+      g.a_off = ((long)w * chil + ap.off) * chil + aq->off;
+      g.b_off = aq->off * ld1 + (long)S * chir + bq.off;
+      g.K = aq->n;
+      P.s1.push_back(g);
+      t.nseg = 1;
+      flops += 2.0 * t.M * t.N * g.K;
+    }
+    push_tiles(P.tl1, (int)P.t1.size(), t.M, t.N);
+    P.t1.push_back(t);
+  };
+  for (int w = 0; w < w0; ++w) {
+    for (const SectorRange& ap : sa) {
+      if (!e->flux_known[pos - 1][w]) {
+        // no nonzero environment on this channel: zero its T1 rows everywhere
+        for (int S = 0; S < DD; ++S)
+          for (const SectorRange& bq : sb) task1(w, ap, S, bq, nullptr);
+        continue;
+      }
+      const QN2 qa = qminus(ap.q, e->mpo_flux[pos - 1][w]);
+      auto ita = ia.find(qa);
+      const SectorRange* aq = (ita == ia.end()) ? nullptr : &sa[ita->second];
+      for (int S = 0; S < DD; ++S) {
+        auto itb = ib.find(qplus(qa, qS[S]));
+        if (itb != ib.end()) task1(w, ap, S, sb[itb->second], aq);
+      }
+    }
+  }
+  for (const SectorRange& ap : sa) {
+    for (int S = 0; S < DD; ++S) {
+      auto it = ib.find(qplus(ap.q, qS[S]));
+      if (it == ib.end()) continue;
+      const SectorRange& bp = sb[it->second];
+      BsTask t;
+      t.c_off = (ap.off * DD + S) * chir + bp.off;
+      t.M = (int)ap.n;
+      t.N = (int)bp.n;
+      t.seg0 = (int)P.s3.size();
+      t.nseg = 0;
+      for (int v = 0; v < w2; ++v) {
+        if (!e->flux_known[pos + 1][v]) continue;  // T2 is zero on such a channel
+        auto jt = ib.find(qminus(bp.q, e->mpo_flux[pos + 1][v]));
+        if (jt == ib.end()) continue;
+        const SectorRange& kb = sb[jt->second];
+        BsSeg g;
+        g.a_off = (ap.off * DD + S) * (long)w2 * chir + (long)v * chir + kb.off;
+        g.b_off = ((long)v * chir + kb.off) * chir + bp.off;
+        g.K = kb.n;
+        P.s3.push_back(g);
+        ++t.nseg;
+        flops += 2.0 * t.M * t.N * g.K;
+      }
+      push_tiles(P.tl3, (int)P.t3.size(), t.M, t.N);
+      P.t3.push_back(t);
+    }
+  }
+  pad_tiles(P.tl1);
+  pad_tiles(P.tl3);
+  // every block the kernels touch lies inside its buffer
+  auto inside = [](long off, long rows, long cols, long ld, long size) {
+    return rows <= 0 || cols <= 0 || (off >= 0 && off + (rows - 1) * ld + cols <= size);
+  };
+  const long nL = (long)w0 * chil * chil, nphi = chil * DD * chir, nT1 = (long)w0 * chil * DD * chir;
+  const long nT2 = (long)w2 * chil * DD * chir, nR = (long)w2 * chir * chir;
+  for (const BsTask& t : P.t1) {
+    TN_REQUIRE(inside(t.c_off, t.M, t.N, ld1, nT1), TN_E_INVALID, "QN product: block outside T1");
+    for (int i = t.seg0; i < t.seg0 + t.nseg; ++i)
+      TN_REQUIRE(inside(P.s1[i].a_off, t.M, P.s1[i].K, chil, nL) && inside(P.s1[i].b_off, P.s1[i].K, t.N, ld1, nphi),
+                 TN_E_INVALID, "QN product: block outside L or phi");
+  }
+  for (const BsTask& t : P.t3) {
+    TN_REQUIRE(inside(t.c_off, t.M, t.N, ld1, nphi), TN_E_INVALID, "QN product: block outside the output");
+    for (int i = t.seg0; i < t.seg0 + t.nseg; ++i)
+      TN_REQUIRE(inside(P.s3[i].a_off, t.M, P.s3[i].K, (long)DD * w2 * chir, nT2) &&
+                     inside(P.s3[i].b_off, P.s3[i].K, t.N, chir, nR),
+                 TN_E_INVALID, "QN product: block outside T2 or R");
+  }
+  // one upload: tasks, segments, tiles of both GEMMs (16-byte aligned parts)
+  auto al = [](size_t x) { return (x + 15) / 16 * 16; };
+  const size_t b_t1 = al(P.t1.size() * sizeof(BsTask)), b_t3 = al(P.t3.size() * sizeof(BsTask));
+  const size_t b_s1 = al(P.s1.size() * sizeof(BsSeg)), b_s3 = al(P.s3.size() * sizeof(BsSeg));
+  const size_t b_l1 = al(P.tl1.size() * sizeof(int4)), b_l3 = al(P.tl3.size() * sizeof(int4));
+  const size_t total = b_t1 + b_t3 + b_s1 + b_s3 + b_l1 + b_l3 + 16;
+  std::vector<char> host(total, 0);
+  size_t o = 0;
+  auto put = [&](const void* src, size_t n, size_t span) {
+    if (n) std::memcpy(host.data() + o, src, n);
+    const size_t at = o;
+    o += span;
+    return at;
+  };
+  const size_t o_t1 = put(P.t1.data(), P.t1.size() * sizeof(BsTask), b_t1);
+  const size_t o_t3 = put(P.t3.data(), P.t3.size() * sizeof(BsTask), b_t3);
+  const size_t o_s1 = put(P.s1.data(), P.s1.size() * sizeof(BsSeg), b_s1);
+  const size_t o_s3 = put(P.s3.data(), P.s3.size() * sizeof(BsSeg), b_s3);
+  const size_t o_l1 = put(P.tl1.data(), P.tl1.size() * sizeof(int4), b_l1);
+  const size_t o_l3 = put(P.tl3.data(), P.tl3.size() * sizeof(int4), b_l3);
+  P.dev.ensure(total);
+  TN_HIP(hipMemcpyAsync(P.dev.p, host.data(), total, hipMemcpyHostToDevice, e->ctx->stream));
+  TN_HIP(hipStreamSynchronize(e->ctx->stream));  // host staging goes out of scope
+  char* base = (char*)P.dev.p;
+  P.dt1 = (const BsTask*)(base + o_t1);
+  P.dt3 = (const BsTask*)(base + o_t3);
+  P.ds1 = (const BsSeg*)(base + o_s1);
+  P.ds3 = (const BsSeg*)(base + o_s3);
+  P.dtl1 = (const int4*)(base + o_l1);
+  P.dtl3 = (const int4*)(base + o_l3);
+  P.flops = flops;
+  P.ok = true;
+  return &P;
+}
+
+// two-site product of a QN-conserving environment on the sector blocks only
+// (FP64, or FP32 on the FP32 environment shadows); same result as the dense
+// product, whose blocks outside the sectors are zero
+void matvec_qn(tn_env* e, const QnMvPlan& qp, const Dom& D, const SparseOp& op, const void* v, void* out,
+               bool f32) {
+  tn_ctx* c = e->ctx;
+  const int pos = D.pos;
+  const long chil = D.chil, chir = D.chir;
+  const int DD = D.d1 * D.d2;
+  const int w0 = e->w[pos - 1], w2 = e->w[pos + 1];
+  const size_t es = f32 ? 4 : 8;
+  void* T1 = c->workspace(WS_T1, (size_t)w0 * chil * DD * chir * es);
+  void* T2 = c->workspace(WS_T2, (size_t)w2 * chil * DD * chir * es);
+  const void* vin = v;
+  const void* Lp = e->LR[pos - 1].buf.p;
+  const void* Rp = e->LR[pos + 2].buf.p;
+  void* outp = out;
+  if (f32) {
+    float* v32 = (float*)c->workspace(WS_PSI32, (size_t)D.n * 4);
+    cvt_f64_to_f32(c, (const double*)v, v32, D.n);
+    vin = v32;
+    Lp = shadow(e, pos - 1, true);
+    Rp = shadow(e, pos + 2, false);
+    outp = c->workspace(WS_OUT32, (size_t)(D.n + 1) / 2 * 8);
+  }
+  fill_f64(c, (double*)outp, 0.0, f32 ? (D.n + 1) / 2 : D.n);
+  const Prec P = f32 ? Prec::F32 : Prec::F64;
+  BsDesc g1;
+  g1.A = Lp; g1.lda = chil;
+  g1.B = vin; g1.ldb = (long)DD * chir;
+  g1.C = T1; g1.ldc = (long)DD * chir;
+  g1.tasks = qp.dt1; g1.segs = qp.ds1; g1.tiles = qp.dtl1; g1.ntiles = (long)qp.tl1.size();
+  {
+    PhaseTimer pg(c, PH_MV_GEMM);
+    gemm_blocksparse(c, P, g1);
+  }
+  WStepDesc ws;
+  ws.G = w0; ws.Gp = w2; ws.D = DD; ws.Dp = DD;
+  ws.nA = chil; ws.nB = chir;
+  ws.in = T1; ws.in_sg = chil * DD * chir; ws.in_sA = DD * chir; ws.in_sS = chir;
+  ws.out = T2; ws.out_sg = chir; ws.out_sA = (long)DD * w2 * chir; ws.out_sS = (long)w2 * chir;
+  ws.op = &op;
+  wstep(c, f32, ws);
+  BsDesc g3;
+  g3.A = T2; g3.lda = (long)DD * w2 * chir;
+  g3.B = Rp; g3.ldb = chir;
+  g3.C = outp; g3.ldc = (long)DD * chir;
+  g3.tasks = qp.dt3; g3.segs = qp.ds3; g3.tiles = qp.dtl3; g3.ntiles = (long)qp.tl3.size();
+  {
+    PhaseTimer pg(c, PH_MV_GEMM);
+    gemm_blocksparse(c, P, g3);
+  }
+  if (f32) cvt_f32_to_f64(c, (const float*)outp, (double*)out, D.n);
+}
+
 // product(sysenv, v) (ProjMPO.product; mirror src/mps/projmps2.jl:159-173)
 void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   PhaseTimer pt(e->ctx, PH_MATVEC);
@@ -602,6 +911,14 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
              "environment and wave-function shapes disagree");
   const SparseOp& op = (e->nsite == 2) ? op_two(e, pos) : op_left(e, pos);
   const bool cx = e->cplx();
+  if (e->nq > 0 && e->nsite == 2 && !cx && (prec == TN_PREC_FP64 || prec == TN_PREC_FP32) &&
+      (long)e->link_qn[pos - 1].size() == chil && (long)e->link_qn[pos + 1].size() == chir) {
+    const QnMvPlan* qp = qn_plan(e, pos);
+    if (qp) {
+      matvec_qn(e, *qp, D, op, v, out, prec == TN_PREC_FP32);
+      return;
+    }
+  }
   if (cx && prec != TN_PREC_FP64)
     throw Error(TN_E_UNSUPPORTED, "mixed-precision matvec is implemented for real tensors only");
   if (prec == TN_PREC_BF16X3) {
@@ -1548,6 +1865,7 @@ void replacebond_qn(tn_env* e, int b, const double* phi, bool ortho_left, long m
   Sa.chir = k;
   Sb.chil = k;
   e->link_qn[b] = newq;
+  ++e->link_ver[b];
   invalidate_site(e, b);
   invalidate_site(e, b + 1);
   e->ortho = ortho_left ? b + 1 : b;
@@ -2454,6 +2772,7 @@ int tn_env_set_mpo(tn_env* e, int k, const double* W) {
     const size_t n = (size_t)e->w[k - 1] * e->d[k - 1] * e->d[k - 1] * e->w[k];
     e->Wh[k - 1].assign(W, W + n);
     e->W_set[k - 1] = 1;
+    ++e->mpo_ver;
     e->opL_ok[k - 1] = e->opR_ok[k - 1] = 0;
     if (k >= 2) e->op2_ok[k - 2] = 0;
     e->op2_ok[k - 1] = 0;
@@ -2647,6 +2966,12 @@ int tn_env_set_qn(tn_env* e, int nq, const int* site_qn, const int* link_qn) {
     e->nq = nq;
     e->site_qn.swap(sq);
     e->link_qn.swap(lq);
+    e->link_ver.resize(e->N + 1, 0);
+    for (auto& v : e->link_ver) ++v;
+    e->flux_ver = -1;
+    bool all = true;
+    for (int k = 0; k < e->N; ++k) all = all && e->W_set[k];
+    if (all) infer_mpo_flux(e);  // rejects an MPO that does not conserve the QNs
   });
 }
 
