@@ -635,7 +635,7 @@ __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(GroupedArgs ga) {
 // ------------------------------------------------------------------------
 struct BsArgs {
   const void* A; long lda;
-  const void* B; long ldb;
+  const void* B; long ldb; int tB;
   void* C; long ldc;
   const BsTask* tasks;
   const BsSeg* segs;
@@ -666,7 +666,7 @@ __global__ __launch_bounds__(256) void gemm_bs_kernel(BsArgs a) {
   GArgs g;
   g.M = tk.M; g.N = tk.N; g.K = 0;
   g.lda = a.lda; g.tA = 0; g.cA = 0;
-  g.ldb = a.ldb; g.tB = 0; g.cB = 0;
+  g.ldb = a.ldb; g.tB = a.tB; g.cB = 0;
   Acc acc[FM][FN];
 #pragma unroll
   for (int i = 0; i < FM; ++i)
@@ -931,7 +931,7 @@ void gemm_blocksparse(tn_ctx* c, Prec p, const BsDesc& d) {
   TN_REQUIRE(d.ntiles % 8 == 0 && d.ntiles < (1L << 31), TN_E_INVALID, "block-sparse gemm: bad tile list");
   BsArgs a;
   a.A = d.A; a.lda = d.lda;
-  a.B = d.B; a.ldb = d.ldb;
+  a.B = d.B; a.ldb = d.ldb; a.tB = d.tB;
   a.C = d.C; a.ldc = d.ldc;
   a.tasks = d.tasks; a.segs = d.segs; a.tiles = d.tiles;
   a.per = d.ntiles / 8;

@@ -1,13 +1,12 @@
 // Quantum-number (U(1), U(1)xU(1)) block kernels of the truncating
 // factorisation (replacebond! of a QN-conserving two-site tensor):
 //
-//  * qn_small_eig_kernel: every QN sector of the combined (link, site) index
-//    whose density-matrix block is at most 32 x 32 gets one wavefront, in one
-//    launch: the block rho_q = M_q M_q^T is formed from the sector's gathered
-//    rows of M (staged through LDS 64 columns at a time), then diagonalised by
-//    cyclic Jacobi rotations in LDS (relative accuracy for these positive
-//    semi-definite blocks); eigenvalues (unsorted) and eigenvectors (as rows)
-//    go to the sector's slots of the block outputs.
+//  * qn_jacobi_kernel: every QN sector of the combined (link, site) index
+//    whose density-matrix block rho_q is at most 96 x 96 gets one workgroup,
+//    in one launch per size class (32, 64, 96): rho_q, formed beforehand by
+//    one block-sparse GEMM launch, is diagonalised in LDS by parallel-order
+//    cyclic Jacobi; eigenvalues (unsorted) and eigenvectors (as rows) go to
+//    the sector's slots of the block outputs.
 //  * qn_scatter_kernel: the kept eigenvectors of every sector become the
 //    columns of the orthonormal factor (rows of its transpose), each written
 //    to the rows of M that belong to its sector; everything else stays zero.
@@ -19,100 +18,111 @@
 namespace tn {
 namespace {
 
-constexpr int QB = 32;  // largest block handled by qn_small_eig_kernel
 
-// one wavefront per block; blk[b] = {row offset in M, size p, output offset of
-// the eigenvalues, output offset of the eigenvector rows}
-__global__ __launch_bounds__(64) void qn_small_eig_kernel(const double* __restrict__ M, long n,
-                                                          const int* __restrict__ blk, double* __restrict__ w,
-                                                          double* __restrict__ V, int max_sweeps) {
-  __shared__ double A[QB][QB + 1];
-  __shared__ double U[QB][QB + 1];
-  __shared__ double stage[QB][65];
+// One workgroup per block.  blk[b] = {row offset of the block in the stacked
+// density matrices R (ld QN_RHO_LD), size p <= PB, output offset of the
+// eigenvalues, output offset of the eigenvector rows}.  The block and its
+// eigenvector matrix U sit in LDS; cyclic Jacobi in the round-robin
+// (parallel) order: each round applies p/2 disjoint rotations at once,
+// A <- J^T A J and U <- U J, and a sweep is p - 1 rounds (p rounded up to
+// even; pairs with the padding index do nothing).  A rotation sets its pair's
+// off-diagonal entries to zero; pairs with |a_ij| <= 1e-3 eps max_i a_ii are
+// left alone, and the sweeps end when a whole sweep rotates nothing (every
+// off-diagonal entry below that bound: eigenvalues to ~p 1e-3 eps |A|).
+constexpr int JT = 1024;  // threads of qn_jacobi_kernel: 16 waves hide the LDS latency of one block per CU
+
+template <int PB>
+__global__ __launch_bounds__(JT) void qn_jacobi_kernel(const double* __restrict__ R, const int* __restrict__ blk,
+                                                        double* __restrict__ w, double* __restrict__ V,
+                                                        int max_sweeps) {
+  constexpr int LD = PB + 1;
+  extern __shared__ double lds[];
+  double* A = lds;             // PB x LD
+  double* U = lds + PB * LD;   // PB x LD
+  __shared__ double rc[PB / 2], rs[PB / 2];
+  __shared__ int ri[PB / 2], rj[PB / 2];
+  __shared__ double red[JT / 64];
+  __shared__ int nrot;
   const int b = blockIdx.x;
   const int r0 = blk[4 * b + 0], p = blk[4 * b + 1], wo = blk[4 * b + 2], vo = blk[4 * b + 3];
-  const int lane = threadIdx.x;
-  // rho = M_q M_q^T: pairs (i, j), i <= j, spread over the lanes
-  const int npair = p * (p + 1) / 2;
-  double acc[9];
-#pragma unroll
-  for (int u = 0; u < 9; ++u) acc[u] = 0.0;
-  for (long c0 = 0; c0 < n; c0 += 64) {
-    const int cw = (int)min((long)64, n - c0);
-    for (int i = 0; i < p; ++i) stage[i][lane] = (lane < cw) ? M[(long)(r0 + i) * n + c0 + lane] : 0.0;
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 9; ++u) {
-      const int t = lane + 64 * u;
-      if (t < npair) {
-        // t -> (i, j) with i <= j, row-major over the upper triangle
-        int i = 0, rem = t;
-        while (rem >= p - i) { rem -= p - i; ++i; }
-        const int j = i + rem;
-        double s = 0.0;
-        for (int cc = 0; cc < 64; ++cc) s += stage[i][cc] * stage[j][cc];
-        acc[u] += s;
-      }
-    }
-    __syncthreads();
+  const int tid = threadIdx.x;
+  double dmax = 0.0;
+  for (int t = tid; t < PB * PB; t += JT) {
+    const int i = t / PB, j = t % PB;
+    if (i >= p || j >= p) continue;
+    const double x = R[(long)(r0 + i) * QN_RHO_LD + j];
+    A[i * LD + j] = x;
+    U[i * LD + j] = (i == j) ? 1.0 : 0.0;
+    if (i == j) dmax = fmax(dmax, fabs(x));
   }
-#pragma unroll
-  for (int u = 0; u < 9; ++u) {
-    const int t = lane + 64 * u;
-    if (t < npair) {
-      int i = 0, rem = t;
-      while (rem >= p - i) { rem -= p - i; ++i; }
-      const int j = i + rem;
-      A[i][j] = acc[u];
-      A[j][i] = acc[u];
-    }
-  }
-  for (int t = lane; t < QB * QB; t += 64) U[t / QB][t % QB] = (t / QB == t % QB) ? 1.0 : 0.0;
+  for (int o = 32; o > 0; o >>= 1) dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = dmax;
   __syncthreads();
-  // cyclic Jacobi: rotations (pq) one at a time, applied by the lanes
+  for (int k = 0; k < JT / 64; ++k) dmax = fmax(dmax, red[k]);
+  const double tol = 1e-3 * 2.220446049250313e-16 * dmax;
+  const int n = p + (p & 1);  // players of the round-robin, padded to even
+  const int np = n / 2;
   for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-    double off = 0.0, dia = 0.0;
-    for (int t = lane; t < p * p; t += 64) {
-      const int i = t / p, j = t % p;
-      const double x = A[i][j] * A[i][j];
-      if (i == j) dia += x; else off += x;
-    }
-    off = wave_sum64(off);
-    dia = wave_sum64(dia);
-    if (off <= 1e-32 * dia || off == 0.0) break;
-    for (int pp = 0; pp < p - 1; ++pp) {
-      for (int qq = pp + 1; qq < p; ++qq) {
-        const double apq = A[pp][qq];
-        if (apq == 0.0) continue;  // uniform: every lane reads the same entry
-        const double app = A[pp][pp], aqq = A[qq][qq];
-        const double zeta = (aqq - app) / (2.0 * apq);
-        const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / sqrt(1.0 + tt * tt), sn = cs * tt;
-        __syncthreads();
-        if (lane < p) {  // columns pp, qq of A
-          const double akp = A[lane][pp], akq = A[lane][qq];
-          A[lane][pp] = cs * akp - sn * akq;
-          A[lane][qq] = sn * akp + cs * akq;
-        } else if (lane >= 32 && lane - 32 < p) {  // columns pp, qq of U
-          const int k = lane - 32;
-          const double ukp = U[k][pp], ukq = U[k][qq];
-          U[k][pp] = cs * ukp - sn * ukq;
-          U[k][qq] = sn * ukp + cs * ukq;
+    if (tid == 0) nrot = 0;
+    __syncthreads();
+    for (int round = 0; round < n - 1; ++round) {
+      if (tid < np) {
+        // circle method: player n-1 fixed, the others rotate
+        int i, j;
+        if (tid == 0) { i = round; j = n - 1; }
+        else { i = (round + tid) % (n - 1); j = (round - tid + n - 1) % (n - 1); }
+        double c = 1.0, sn = 0.0;
+        if (i < p && j < p) {
+          const double apq = A[i * LD + j];
+          if (fabs(apq) > tol) {
+            const double zeta = (A[j * LD + j] - A[i * LD + i]) / (2.0 * apq);
+            const double az = fabs(zeta);
+            const double t = (az > 1e150) ? 0.5 / zeta
+                                           : (zeta >= 0.0 ? 1.0 : -1.0) / (az + sqrt(1.0 + zeta * zeta));
+            c = 1.0 / sqrt(1.0 + t * t);
+            sn = c * t;
+            atomicAdd(&nrot, 1);
+          } else {
+            i = j = PB;  // nothing to do for this pair
+          }
         }
-        __syncthreads();
-        if (lane < p) {  // rows pp, qq of A
-          const double apk = A[pp][lane], aqk = A[qq][lane];
-          A[pp][lane] = cs * apk - sn * aqk;
-          A[qq][lane] = sn * apk + cs * aqk;
-        }
-        __syncthreads();
+        ri[tid] = i; rj[tid] = j; rc[tid] = c; rs[tid] = sn;
       }
+      __syncthreads();
+      // columns i, j of A and U (every row)
+      for (int t = tid; t < np * PB * 2; t += JT) {
+        int k = t / PB;
+        const int r = t % PB;
+        double* X = A;
+        if (k >= np) { k -= np; X = U; }
+        const int i = ri[k], j = rj[k];
+        if (i >= p || j >= p || r >= p) continue;
+        const double c = rc[k], sn = rs[k];
+        const double xi = X[r * LD + i], xj = X[r * LD + j];
+        X[r * LD + i] = c * xi - sn * xj;
+        X[r * LD + j] = sn * xi + c * xj;
+      }
+      __syncthreads();
+      // rows i, j of A (every column); the pair's own entries become zero
+      for (int t = tid; t < np * PB; t += JT) {
+        const int k = t / PB, r = t % PB;
+        const int i = ri[k], j = rj[k];
+        if (i >= p || j >= p || r >= p) continue;
+        const double c = rc[k], sn = rs[k];
+        const double xi = A[i * LD + r], xj = A[j * LD + r];
+        A[i * LD + r] = (r == j) ? 0.0 : c * xi - sn * xj;
+        A[j * LD + r] = (r == i) ? 0.0 : sn * xi + c * xj;
+      }
+      __syncthreads();
     }
+    const int nr = nrot;  // read after the last round's barrier
+    __syncthreads();      // every thread has read it before thread 0 resets it
+    if (nr == 0) break;
   }
-  for (int i = lane; i < p; i += 64) w[wo + i] = A[i][i];
-  for (int t = lane; t < p * p; t += 64) {
+  for (int i = tid; i < p; i += JT) w[wo + i] = A[i * LD + i];
+  for (int t = tid; t < p * p; t += JT) {
     const int i = t / p, r = t % p;  // eigenvector i (column i of U), component r
-    V[vo + (long)i * p + r] = U[r][i];
+    V[vo + (long)i * p + r] = U[r * LD + i];
   }
 }
 
@@ -136,9 +146,17 @@ __global__ __launch_bounds__(256) void qn_scatter_kernel(const double* __restric
 
 }  // namespace
 
-void qn_small_eig(tn_ctx* c, const double* M, long n, const int* blk_dev, int nblk, double* w, double* V) {
+void qn_block_eig(tn_ctx* c, const double* R, const int* blk_dev, int nblk, int pmax, double* w, double* V) {
   if (nblk <= 0) return;
-  hipLaunchKernelGGL(qn_small_eig_kernel, dim3((unsigned)nblk), dim3(64), 0, c->stream, M, n, blk_dev, w, V, 60);
+  TN_REQUIRE(pmax >= 1 && pmax <= QN_JACOBI_MAX, TN_E_INVALID, "QN block eigensolver: block too large");
+  auto launch = [&](auto kern, int pb) {
+    const size_t shm = (size_t)2 * pb * (pb + 1) * sizeof(double);
+    if (shm > 64 * 1024) raise_lds_limit((const void*)kern, shm);
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(JT), shm, c->stream, R, blk_dev, w, V, 40);
+  };
+  if (pmax <= 32) launch(qn_jacobi_kernel<32>, 32);
+  else if (pmax <= 64) launch(qn_jacobi_kernel<64>, 64);
+  else launch(qn_jacobi_kernel<96>, 96);
   TN_HIP(hipGetLastError());
 }
 

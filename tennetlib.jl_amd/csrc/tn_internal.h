@@ -178,7 +178,7 @@ enum WsSlot {
   WS_DC_VEC, WS_DC_Q, WS_DC_SMALL, WS_DC_TMP, WS_DC_UT, WS_DC_LEVEL, WS_DC_W, WS_PROJ_X, WS_PROJ_Y, WS_PROJ_C,
   WS_QR_X, WS_QR_V, WS_QR_Q, WS_QR_R, WS_QR_S, WS_QR_W, WS_TTN_A, WS_TTN_S, WS_TTN_G, WS_TTN_P, WS_TTN_F,
   WS_GEMM_SPLIT, WS_BOND, WS_PHI2, WS_TTNP_0, WS_TTNP_1, WS_TTNP_2, WS_TTNP_3, WS_DC_PANEL,
-  WS_QN_M, WS_QN_V, WS_QN_W, WS_QN_IDX, WS_QN_T, WS_SLOTS
+  WS_QN_M, WS_QN_V, WS_QN_W, WS_QN_IDX, WS_QN_T, WS_QN_RHO, WS_QN_PLAN, WS_SLOTS
 };
 
 // scoped phase timer (HIP events on the context stream)
@@ -246,6 +246,7 @@ struct BsSeg {
 struct BsDesc {
   const void* A = nullptr; long lda = 0;
   const void* B = nullptr; long ldb = 0;
+  int tB = 0;  // 1: segment operand B_s is stored transposed (N x K_s, ld ldb)
   void* C = nullptr; long ldc = 0;
   const BsTask* tasks = nullptr;
   const BsSeg* segs = nullptr;
@@ -395,10 +396,13 @@ void sym_eig_dc_z(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, 
 void gather_rows_T_z(tn_ctx* c, const double* src, long ld_src, const int* perm, const double* scale,
                      long kcount, long L, double* out, bool conj);
 
-// QN blocks of the factorisation (qn.hip).  qn_small_eig: nblk blocks of at
-// most 32 rows of M (ld n); blk_dev[4b..4b+3] = {first row, size p, offset
-// of the eigenvalues in w, offset of the eigenvector rows in V}; one launch.
-void qn_small_eig(tn_ctx* c, const double* M, long n, const int* blk_dev, int nblk, double* w, double* V);
+// QN blocks of the factorisation (qn.hip).  qn_block_eig: nblk density-matrix
+// blocks, each at most pmax <= QN_JACOBI_MAX rows, stacked in R (ld
+// QN_RHO_LD); blk_dev[4b..4b+3] = {row offset in R, size p, offset of the p
+// eigenvalues in w, offset of the p eigenvectors (rows of length p) in V}.
+constexpr int QN_JACOBI_MAX = 96;
+constexpr int QN_RHO_LD = 96;
+void qn_block_eig(tn_ctx* c, const double* R, const int* blk_dev, int nblk, int pmax, double* w, double* V);
 // qn_scatter: column j of out (rows x k, ld ldo; its transpose when trans)
 // receives eigenvector col[4j+2] (stored at V + col[4j] as rows of length
 // col[4j+3]) on the rows rows[col[4j+1] ..]; out must be zero elsewhere

@@ -1760,7 +1760,6 @@ void replacebond_qn(tn_env* e, int b, const double* phi, bool ortho_left, long m
   double* Gp = (double*)c->workspace(WS_QN_M, (size_t)p * L * sizeof(double));
   int* ibuf = (int*)c->workspace(WS_QN_IDX, (size_t)(p + 4 * std::max<long>(ns, p) + 16) * sizeof(int));
   int* dperm = ibuf;
-  int* dmeta = ibuf + p;
   double* dones = (double*)c->workspace(WS_SVD_SCALE, (size_t)p * sizeof(double));
   fill_f64(c, dones, 1.0, p);
   TN_HIP(hipMemcpyAsync(dperm, perm.data(), p * sizeof(int), hipMemcpyHostToDevice, c->stream));
@@ -1773,34 +1772,93 @@ void replacebond_qn(tn_env* e, int b, const double* phi, bool ortho_left, long m
   gather_rows(c, Gsrc, L, dperm, dones, p, L, Gp);
   double* Wd = (double*)c->workspace(WS_QN_W, (size_t)wtot * sizeof(double));
   double* Vd = (double*)c->workspace(WS_QN_V, (size_t)std::max<long>(vtot, 1) * sizeof(double));
-  // small blocks: one launch
-  std::vector<int> meta;
-  int nsmall = 0;
+  // Blocks of a size class (up to 32, 64, 96 rows) go to one batched Jacobi
+  // launch when the block-size histogram says it is cheaper than the dense
+  // eigensolver block by block: measured on MI355X, a launch of class c
+  // takes about qn_batch_ms[c] (set by its largest block, all blocks run
+  // side by side) and one block of p rows through the dense eigensolver
+  // about 0.35 + 0.005 p ms (launch-latency bound below 136 rows).  The
+  // batched blocks' rho_q = G_q G_q^T come from one block-sparse GEMM
+  // launch into a stack (ld QN_RHO_LD); the rest use the dense eigensolver.
+  static constexpr double qn_batch_ms[3] = {0.25, 1.2, 4.0};
+  double serial_ms[3] = {0.0, 0.0, 0.0};
   for (int i = 0; i < ns; ++i)
-    if (psz[i] <= 32) {
-      meta.insert(meta.end(), {roff[i], psz[i], (int)woff[i], (int)voff[i]});
-      ++nsmall;
+    if (psz[i] <= QN_JACOBI_MAX) serial_ms[(psz[i] + 31) / 32 - 1] += (psz[i] < 3 ? 0.05 : 0.35 + 0.005 * psz[i]);
+  bool batch_cls[3];
+  for (int cl = 0; cl < 3; ++cl) batch_cls[cl] = serial_ms[cl] > qn_batch_ms[cl];
+  std::vector<long> soff(ns, -1);
+  long srows = 0;
+  for (int i = 0; i < ns; ++i)
+    if (psz[i] <= QN_JACOBI_MAX && batch_cls[(psz[i] + 31) / 32 - 1]) {
+      soff[i] = srows;
+      srows += psz[i];
     }
-  if (nsmall) {
-    TN_HIP(hipMemcpyAsync(dmeta, meta.data(), meta.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    qn_small_eig(c, Gp, L, dmeta, nsmall, Wd, Vd);
+  std::vector<BsTask> gt;
+  std::vector<BsSeg> gs;
+  std::vector<int4> gtl;
+  std::vector<int> meta;
+  std::vector<char> host;  // staging of the block plan, alive until the stream is synchronised
+  int cls_n[3] = {0, 0, 0};
+  if (srows > 0) {
+    double* Rst = (double*)c->workspace(WS_QN_RHO, (size_t)srows * QN_RHO_LD * sizeof(double));
+    for (int i = 0; i < ns; ++i) {
+      if (soff[i] < 0) continue;
+      BsTask t;
+      t.c_off = soff[i] * QN_RHO_LD;
+      t.M = t.N = psz[i];
+      t.seg0 = (int)gs.size();
+      t.nseg = 1;
+      gs.push_back(BsSeg{(long)roff[i] * L, (long)roff[i] * L, L});
+      push_tiles(gtl, (int)gt.size(), t.M, t.N);
+      gt.push_back(t);
+    }
+    pad_tiles(gtl);
+    // size classes 32, 64, 96: metadata of class 0, then 1, then 2
+    for (int cl = 0; cl < 3; ++cl)
+      for (int i = 0; i < ns; ++i) {
+        if (soff[i] < 0 || (psz[i] + 31) / 32 - 1 != cl) continue;
+        meta.insert(meta.end(), {(int)soff[i], psz[i], (int)woff[i], (int)voff[i]});
+        ++cls_n[cl];
+      }
+    auto al = [](size_t x) { return (x + 15) / 16 * 16; };
+    const size_t b_t = al(gt.size() * sizeof(BsTask)), b_s = al(gs.size() * sizeof(BsSeg));
+    const size_t b_l = al(gtl.size() * sizeof(int4)), b_m = al(meta.size() * sizeof(int));
+    host.assign(b_t + b_s + b_l + b_m, 0);
+    std::memcpy(host.data(), gt.data(), gt.size() * sizeof(BsTask));
+    std::memcpy(host.data() + b_t, gs.data(), gs.size() * sizeof(BsSeg));
+    std::memcpy(host.data() + b_t + b_s, gtl.data(), gtl.size() * sizeof(int4));
+    std::memcpy(host.data() + b_t + b_s + b_l, meta.data(), meta.size() * sizeof(int));
+    char* dplan = (char*)c->workspace(WS_QN_PLAN, host.size());
+    TN_HIP(hipMemcpyAsync(dplan, host.data(), host.size(), hipMemcpyHostToDevice, c->stream));
+    BsDesc g;
+    g.A = Gp; g.lda = L;
+    g.B = Gp; g.ldb = L; g.tB = 1;
+    g.C = Rst; g.ldc = QN_RHO_LD;
+    g.tasks = (const BsTask*)dplan;
+    g.segs = (const BsSeg*)(dplan + b_t);
+    g.tiles = (const int4*)(dplan + b_t + b_s);
+    g.ntiles = (long)gtl.size();
+    gemm_blocksparse(c, Prec::F64, g);
+    const int* dm = (const int*)(dplan + b_t + b_s + b_l);
+    for (int cl = 0, o = 0; cl < 3; o += cls_n[cl], ++cl)
+      qn_block_eig(c, Rst, dm + 4 * o, cls_n[cl], 32 * (cl + 1), Wd, Vd);
   }
   std::vector<std::vector<double>> w(ns);
   // large blocks: the dense eigensolver, one block at a time
   for (int i = 0; i < ns; ++i) {
-    if (psz[i] <= 32) continue;
+    if (soff[i] >= 0) continue;
     GramEig ge;
     gram_eig(c, Gp + (long)roff[i] * L, psz[i], L, true, false, ge);
     w[i] = ge.w;
     TN_HIP(hipMemcpy2DAsync(Vd + voff[i], (size_t)psz[i] * sizeof(double), ge.Vt, (size_t)ge.ppad * sizeof(double),
                             (size_t)psz[i] * sizeof(double), (size_t)psz[i], hipMemcpyDeviceToDevice, c->stream));
   }
-  if (nsmall) {
+  if (srows > 0) {
     std::vector<double> hw(wtot);
     TN_HIP(hipMemcpyAsync(hw.data(), Wd, wtot * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TN_HIP(hipStreamSynchronize(c->stream));
     for (int i = 0; i < ns; ++i)
-      if (psz[i] <= 32) w[i].assign(hw.begin() + woff[i], hw.begin() + woff[i] + psz[i]);
+      if (soff[i] >= 0) w[i].assign(hw.begin() + woff[i], hw.begin() + woff[i] + psz[i]);
   }
   // weights of every block in descending order, then NDTensors' block rule
   std::vector<std::vector<int>> ord(ns);

@@ -85,6 +85,60 @@ def test_qn_replacebond_matches_oracle(T, ortho):
     env.close()
 
 
+@pytest.mark.parametrize("ortho", ["left", "right"])
+def test_qn_replacebond_block_size_classes(T, ortho):
+    """U(1) blocks of 10 to 120 rows.  ortho left: 2, 6 and 6 blocks in the
+    size classes up to 32, 64 and 96 rows, enough for each class to take the
+    batched Jacobi launch; ortho right: blocks of 60 and 120 rows through the
+    dense eigensolver.  Kept weights to 1e-13 of the oracle's (numpy eigh per
+    block), the new site orthonormal to 1e-13, the product of the two new
+    sites to 1e-12."""
+    sizes_l = dict(zip(range(-12, 13, 2), [10, 40, 40, 40, 20, 40, 40, 40, 20, 40, 40, 40, 10]))
+    sizes_r = {q: 60 for q in range(-14, 15, 2)}
+    ql = [(q,) for q in sorted(sizes_l) for _ in range(sizes_l[q])]
+    qr = [(q,) for q in sorted(sizes_r) for _ in range(sizes_r[q])]
+    qs = Q.spin_half_qns()
+    rng = np.random.default_rng(21)
+    mask = _sector_mask(ql, qs, qs, qr)
+    phi = rng.standard_normal(mask.shape) * mask
+    # graded weights, so that truncation cuts inside blocks
+    phi *= np.exp(-0.02 * np.arange(len(ql)))[:, None, None, None]
+    phi /= np.linalg.norm(phi)
+    N, b, chim = 4, 2, 64
+    dims = [1, len(ql), chim, len(qr), 1]
+    A = [rng.standard_normal((dims[k], 2, dims[k + 1])) for k in range(N)]
+    for maxdim, cutoff in [(10 ** 6, 1e-14), (300, 1e-12), (100, 1e-6)]:
+        env = T.StateEnvs(A, O.heisenberg_mpo(N))
+        ctx = env.ctx
+        sq = np.array([x for _ in range(N) for q in qs for x in q], dtype=np.int32)
+        links = [[(0,)], ql, [(0,)] * chim, qr, [(0,)]]
+        lq = np.array([x for lk in links for q in lk for x in q], dtype=np.int32)
+        ctx.check(ctx.lib.tn_env_set_qn(env.h, 1, sq.ctypes.data_as(C.POINTER(C.c_int)),
+                                        lq.ctypes.data_as(C.POINTER(C.c_int))))
+        env.nq = 1
+        Ab, Ab1, newq, spec = Q.block_factorize(phi, ql, qs, qs, qr, maxdim, 1, cutoff, ortho)
+        dphi = T.DeviceArray.from_numpy(ctx, phi)
+        terr, nk = C.c_double(), C.c_long()
+        eigs = (C.c_double * 4096)()
+        print(f"qn replacebond classes {ortho} maxdim={maxdim} cutoff={cutoff}", flush=True)
+        ctx.check(ctx.lib.tn_replacebond(env.h, b, dphi.ptr, 1 if ortho == "left" else 0, maxdim, 1, cutoff, 0.0,
+                                         1, C.byref(terr), eigs, 4096, C.byref(nk)))
+        assert nk.value == len(newq)
+        assert np.abs(np.array(eigs[:nk.value]) - spec.eigs).max() < 1e-13
+        assert env.sectors(b) == spec.sectors
+        got = [env.get_site(b), env.get_site(b + 1)]
+        if ortho == "left":
+            U = got[0].reshape(-1, nk.value)
+            assert np.abs(U.T @ U - np.eye(nk.value)).max() < 1e-13
+        else:
+            U = got[1].reshape(nk.value, -1)
+            assert np.abs(U @ U.T - np.eye(nk.value)).max() < 1e-13
+        prod = np.tensordot(got[0], got[1], axes=([2], [0]))
+        assert np.abs(prod - np.tensordot(Ab, Ab1, axes=([2], [0]))).max() < 1e-12
+        dphi.free()
+        env.close()
+
+
 def test_qn_dmrg_heisenberg_matches_oracle_and_ed(T):
     N = 12
     sq = [Q.spin_half_qns()] * N
@@ -151,13 +205,15 @@ def test_qn_dmrg_hubbard_n32_matches_golden(T):
     bond dimensions set by maxdim.  The oracle itself moves by up to 9e-8 per
     site on the ramp sweeps and regroups the sectors of several links under
     2e-16 perturbations of its density matrices (the maxdim cut splits
-    multiplets; tests/test_oracle_qn.py::test_hubbard_n32_sensitivity), so:
-    first sweep to 1e-10 per site, ramp sweeps to 1e-6, the last sweep to
-    5e-9; maxchi exact; every link's sectors allowed and its dimension
-    within 4 of the oracle's."""
+    multiplets; tests/test_oracle_qn.py::test_hubbard_n32_sensitivity), and
+    by 7e-7 under 2e-14 perturbations (the backward error of an eigensolver
+    of another kind, such as the Jacobi of small blocks here), so: first
+    sweep to 1e-10 per site, ramp sweeps to 1e-5, the last sweep to 5e-9;
+    maxchi exact; every link's sectors allowed and its dimension within 4 of
+    the oracle's."""
     g, sw, dE, got, want = _golden_run(T, "hubbard_qn_n32")
     assert dE[0] <= 1e-10
-    assert dE.max() <= 1e-6
+    assert dE.max() <= 1e-5
     assert dE[-1] <= 5e-9
     assert sw.maxchi == g["maxchi"]
     N = g["N"]
