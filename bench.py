@@ -336,6 +336,42 @@ def tdvp_run(T, N, steps, dt, maxdim, cutoff):
     return out
 
 
+def hubbard_qn_run(T, N, ramp, cutoff, kd, mi, prec):
+    """QN-conserving two-site DMRG of the Hubbard chain (t=1, U=4, half
+    filling (N/2, N/2), U(1)xU(1)) from up,dn,up,dn,...: one sweep per maxdim
+    of the ramp in FP64, then one more sweep at the last maxdim in each of
+    FP64 and the mixed form (the same bond dimensions; sweeps run one after
+    the other on the same state).  Sweep times are wall-clock with the device
+    synchronised on both sides."""
+    from tennetlib_amd.models import hubbard_qns
+    env = T.StateEnvs(T.hubbard_occupation_state([1, 2] * (N // 2)), T.hubbard_mpo(N, 1.0, 4.0),
+                      qns=[hubbard_qns()] * N)
+    ctx = env.ctx
+    sw = T.SweepData()
+    ramp_s = []
+    t_all = time.perf_counter()
+    for md in ramp:
+        dt, tm = run_sweeps(T, env, sw, 1, md, "fp64", cutoff, kd, mi, timing=True)
+        ramp_s.append(round(dt, 3))
+        log(f"hubbard qn N={N} maxdim {md}: {dt:.2f} s E={sw.energy[-1]:.12f} maxchi={sw.maxchi[-1]} "
+            f"phases={ {k: round(v) for k, v in tm.items()} }")
+    out = {"workload": f"1D Hubbard N={N}, t=1, U=4, (N_up, N_dn)=({N // 2}, {N // 2}) U(1)xU(1), two-site DMRG "
+                       f"ramp maxdim {ramp} (one sweep each, cutoff {cutoff:g}, krylovdim {kd}, maxiter {mi}), "
+                       f"then one sweep at maxdim {ramp[-1]} per form",
+           "ramp_s_per_sweep": ramp_s}
+    for form in ("fp64", prec):
+        dt, tm = run_sweeps(T, env, sw, 1, ramp[-1], form, cutoff, kd, mi, timing=True)
+        mid = env.sectors(N // 2)
+        out[form] = {"s_per_sweep": round(dt, 3), "energy": sw.energy[-1], "maxchi": sw.maxchi[-1],
+                     "maxtruncerr": sw.maxtruncerr[-1], "phase_ms": {k: round(v, 1) for k, v in tm.items()},
+                     "mid_bond_sectors": len(mid), "mid_bond_largest_sector": max(mid.values())}
+        log(f"hubbard qn N={N} {form}: {dt:.2f} s/sweep E={sw.energy[-1]:.12f} maxchi={sw.maxchi[-1]}")
+    out["speedup_vs_fp64"] = round(out["fp64"]["s_per_sweep"] / out[prec]["s_per_sweep"], 3)
+    out["total_s"] = round(time.perf_counter() - t_all, 1)
+    env.close()
+    return out
+
+
 def ttn_instances(T, count, seed=5, total=256):
     """Many-instance TTN ground states: `total` random-field Heisenberg chains
     (h_i ~ U[-2, 2], N ~ U{16..64}, numpy default_rng(seed)), each a
@@ -452,6 +488,11 @@ def n26_sweeps(T, args):
     return out
 
 
+def hubbard_qn_line(T, args):
+    ramp = [m for m in (64, 128, 256, 512, 1024, 2048, 4096) if m < args.hubbard_maxdim] + [args.hubbard_maxdim]
+    return hubbard_qn_run(T, 64, ramp, 1e-14, args.krylovdim, args.maxiter, args.prec)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -472,6 +513,9 @@ def main():
     ap.add_argument("--tdvp-steps", type=int, default=40, help="TDVP steps of the XXZ N=100 run (0: skip)")
     ap.add_argument("--ttn-instances", type=int, default=8, help="TTN instances timed of the 256 (0: skip)")
     ap.add_argument("--ttn-only", action="store_true", help="only the many-instance TTN line")
+    ap.add_argument("--hubbard-maxdim", type=int, default=2048,
+                    help="last maxdim of the QN Hubbard N=64 ramp (0: skip the line)")
+    ap.add_argument("--hubbard-only", action="store_true", help="only the QN Hubbard N=64 line")
     ap.add_argument("--tdvp-only", action="store_true", help="only the TDVP XXZ N=100 line")
     ap.add_argument("--eig-only", type=int, default=0, metavar="P",
                     help="only one density-matrix eigen-decomposition of size P (counter runs)")
@@ -504,6 +548,9 @@ def main():
         return
     if args.ttn_only:
         print(json.dumps({"ttn_random_field_batch": ttn_instances(T, max(1, args.ttn_instances))}), flush=True)
+        return
+    if args.hubbard_only:
+        print(json.dumps({"hubbard_qn_n64": hubbard_qn_line(T, args)}), flush=True)
         return
     if args.tdvp_only:
         print(json.dumps({"tdvp_xxz_n100": tdvp_run(T, 100, max(1, args.tdvp_steps), 0.05, 512, 1e-12)}), flush=True)
@@ -551,6 +598,8 @@ def main():
     if args.ttn_instances > 0:
         ttn = ttn_instances(T, args.ttn_instances)
         log(f"ttn: {ttn['instances_timed']} instances in {ttn['wall_s']} s")
+
+    hub = hubbard_qn_line(T, args) if args.hubbard_maxdim > 0 else None
 
     cpu = None
     if not args.skip_cpu:
@@ -602,6 +651,7 @@ def main():
                                f"warmup updates"},
         "tdvp_xxz_n100": tdvp,
         "ttn_random_field_batch": ttn,
+        "hubbard_qn_n64": hub,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -126,6 +126,11 @@ def test_qn_replacebond_block_size_classes(T, ortho):
         assert nk.value == len(newq)
         assert np.abs(np.array(eigs[:nk.value]) - spec.eigs).max() < 1e-13
         assert env.sectors(b) == spec.sectors
+        if maxdim > 10 ** 5:
+            # the union of the block spectra is the dense spectrum of phi
+            P = np.linalg.svd(phi.reshape(len(ql) * 2, -1), compute_uv=False) ** 2
+            P = P[P > 1e-14]
+            assert np.abs(np.array(eigs[:len(P)]) - P).max() < 1e-13
         got = [env.get_site(b), env.get_site(b + 1)]
         if ortho == "left":
             U = got[0].reshape(-1, nk.value)
@@ -262,3 +267,22 @@ def test_qn_blocksparse_product_matches_dense(T, model):
         got32 = T.product(env, phi, "fp32")
         assert np.abs(got32 - ref).max() <= 2e-6 * scale, pos
         assert not np.any(got32[~mask])
+
+
+def test_qn_dmrg_hubbard_n64_u0_closed_form(T):
+    """Hubbard N = 64, U = 0, (32, 32): QN DMRG ramp to maxdim 1024 against
+    the free-fermion ground energy -80.768626322437; the DMRG energy is
+    variational (not below it) and within 2e-6 of it (truncation error of
+    maxdim 1024, about 1e-8 per bond)."""
+    N = 64
+    e0 = Q.hubbard_free_fermion_energy(N, N // 2, N // 2)
+    assert abs(e0 - (-80.76862632243697)) < 1e-12
+    env = T.StateEnvs(T.hubbard_occupation_state([1, 2] * (N // 2)), T.hubbard_mpo(N, 1.0, 0.0),
+                      qns=[Q.hubbard_qns()] * N)
+    print("qn dmrg hubbard n64 u0", flush=True)
+    sw = T.dmrg_(env, T.make_dmrg_params(maxdim=[64, 128, 256, 512, 1024], nsweeps=[1] * 5, cutoff=1e-14), 2,
+                 outputlevel=0)
+    print("n64 u0 energies", sw.energy, "maxchi", sw.maxchi, flush=True)
+    assert sw.energy[-1] > e0 - 1e-10
+    assert sw.energy[-1] - e0 < 2e-6
+    assert env.link_qns(N) == [(N // 2, N // 2)]
