@@ -87,6 +87,33 @@ __device__ __forceinline__ uint4 ld16(Rsrc r, unsigned off) {
 __device__ __forceinline__ unsigned voff(int rr, long c, long ld, long cols, int esz) {
   return c < cols ? (unsigned)((rr * ld + c) * esz) : OOB;
 }
+// the VN elements at (tile row rr, columns c .. c + VN - 1) as one 16-byte
+// value: AL = 1, one vector load (the row start and c are vector-aligned and
+// the row width is whole vectors); AL = 0, element loads, each bounded by the
+// row width on its own, for operands whose rows are not 16-byte aligned
+template <typename T, int AL>
+__device__ __forceinline__ uint4 ldvec(Rsrc r, int rr, long c, long ld, long cols) {
+  constexpr int ES = (int)sizeof(T);
+  if constexpr (AL) {
+    return ld16(r, voff(rr, c, ld, cols, ES));
+  } else if constexpr (ES == 8) {
+    uint4 v;
+    const unsigned long long x0 = __builtin_bit_cast(
+        unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, voff(rr, c, ld, cols, ES), 0, 0));
+    const unsigned long long x1 = __builtin_bit_cast(
+        unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(r, voff(rr, c + 1, ld, cols, ES), 0, 0));
+    v.x = (unsigned)x0; v.y = (unsigned)(x0 >> 32);
+    v.z = (unsigned)x1; v.w = (unsigned)(x1 >> 32);
+    return v;
+  } else {
+    uint4 v;
+    v.x = __builtin_amdgcn_raw_buffer_load_b32(r, voff(rr, c, ld, cols, ES), 0, 0);
+    v.y = __builtin_amdgcn_raw_buffer_load_b32(r, voff(rr, c + 1, ld, cols, ES), 0, 0);
+    v.z = __builtin_amdgcn_raw_buffer_load_b32(r, voff(rr, c + 2, ld, cols, ES), 0, 0);
+    v.w = __builtin_amdgcn_raw_buffer_load_b32(r, voff(rr, c + 3, ld, cols, ES), 0, 0);
+    return v;
+  }
+}
 
 template <typename T, int TA, int TB>
 struct FastCfg {
@@ -102,7 +129,7 @@ struct FastCfg {
   static constexpr int A_ELEMS = A_ROWS * A_LD, B_ELEMS = B_ROWS * B_LD;
 };
 
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int AL>
 __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
   using CF = FastCfg<T, TA, TB>;
   constexpr int BM = CF::BM, BN = CF::BN, BK = CF::BK, VN = CF::VN;
@@ -132,13 +159,13 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
     for (int i = 0; i < CF::A_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::A_COLS / VN), cc = (idx % (CF::A_COLS / VN)) * VN;
-      ra[i] = ld16(ra_r, TA ? voff(rr, m0 + cc, a.lda, a.M, ES) : voff(rr, k0 + cc, a.lda, a.K, ES));
+      ra[i] = TA ? ldvec<T, AL>(ra_r, rr, m0 + cc, a.lda, a.M) : ldvec<T, AL>(ra_r, rr, k0 + cc, a.lda, a.K);
     }
 #pragma unroll
     for (int i = 0; i < CF::B_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::B_COLS / VN), cc = (idx % (CF::B_COLS / VN)) * VN;
-      rb[i] = ld16(rb_r, TB ? voff(rr, k0 + cc, a.ldb, a.K, ES) : voff(rr, n0 + cc, a.ldb, a.N, ES));
+      rb[i] = TB ? ldvec<T, AL>(rb_r, rr, k0 + cc, a.ldb, a.K) : ldvec<T, AL>(rb_r, rr, n0 + cc, a.ldb, a.N);
     }
   };
   auto sstore = [&](int st) {
@@ -459,19 +486,23 @@ void raise_lds_limit(const void* fn, size_t bytes) {
 }
 static void allow_lds(const void* fn, size_t bytes) { raise_lds_limit(fn, bytes); }
 
+// operands whose rows are 16-byte aligned with a valid width of whole
+// vectors take the vector loads; any other operand takes element loads
+static bool fast_aligned(Prec p, const GemmDesc& g) {
+  const int vn = (p == Prec::F64) ? 2 : 4;
+  auto al = [&](const void* ptr, long ld, long s, long cols) {
+    return ((uintptr_t)ptr % 16 == 0) && (ld % vn == 0) && (s % vn == 0) && (cols % vn == 0);
+  };
+  return al(g.A, g.lda, g.sA, g.tA ? g.M : g.K) && al(g.B, g.ldb, g.sB, g.tB ? g.K : g.N);
+}
+
 bool gemm_fast_ok(Prec p, const GemmDesc& g) {
   if (p != Prec::F64 && p != Prec::F32) return false;
   if (g.cA || g.cB) return false;
   if (g.M < 128 || g.N < 128 || g.K < 16) return false;
-  const int vn = (p == Prec::F64) ? 2 : 4;
   const size_t es = (p == Prec::F64) ? 8 : 4;
-  // 16-byte aligned rows whose valid width is whole vectors; a 128-row tile
-  // must span less than 2^31 bytes (32-bit buffer offsets)
-  auto al = [&](const void* ptr, long ld, long s, long cols) {
-    return ((uintptr_t)ptr % 16 == 0) && (ld % vn == 0) && (s % vn == 0) && (cols % vn == 0) &&
-           (size_t)ld * es * 130 < ((size_t)1 << 31);
-  };
-  return al(g.A, g.lda, g.sA, g.tA ? g.M : g.K) && al(g.B, g.ldb, g.sB, g.tB ? g.K : g.N);
+  // a 128-row tile must span less than 2^31 bytes (32-bit buffer offsets)
+  return (size_t)g.lda * es * 130 < ((size_t)1 << 31) && (size_t)g.ldb * es * 130 < ((size_t)1 << 31);
 }
 
 void gemm_fast(tn_ctx* c, Prec p, const GemmDesc& g) {
@@ -483,12 +514,18 @@ void gemm_fast(tn_ctx* c, Prec p, const GemmDesc& g) {
   a.alpha = g.alpha; a.beta = g.beta;
   const long ntm = (g.M + 127) / 128, ntn = (g.N + 127) / 128;
   dim3 grid((unsigned)(ntm * ntn), 1, (unsigned)g.batch);
-#define TN_LAUNCH_FAST(T, TA, TB)                                                                    \
+  const bool aligned = fast_aligned(p, g);
+#define TN_LAUNCH_FAST1(T, TA, TB, AL)                                                               \
   do {                                                                                               \
     using CF = FastCfg<T, TA, TB>;                                                                   \
     const size_t shm = 2 * (size_t)(CF::A_ELEMS + CF::B_ELEMS) * sizeof(T);                          \
-    allow_lds((const void*)gemm_fast_kernel<T, TA, TB>, shm);                                        \
-    hipLaunchKernelGGL((gemm_fast_kernel<T, TA, TB>), grid, dim3(256), shm, c->stream, a);           \
+    allow_lds((const void*)gemm_fast_kernel<T, TA, TB, AL>, shm);                                    \
+    hipLaunchKernelGGL((gemm_fast_kernel<T, TA, TB, AL>), grid, dim3(256), shm, c->stream, a);       \
+  } while (0)
+#define TN_LAUNCH_FAST(T, TA, TB)                 \
+  do {                                            \
+    if (aligned) TN_LAUNCH_FAST1(T, TA, TB, 1);   \
+    else TN_LAUNCH_FAST1(T, TA, TB, 0);           \
   } while (0)
   if (p == Prec::F64) {
     if (!g.tA && !g.tB) TN_LAUNCH_FAST(double, 0, 0);
@@ -502,6 +539,7 @@ void gemm_fast(tn_ctx* c, Prec p, const GemmDesc& g) {
     else TN_LAUNCH_FAST(float, 1, 1);
   }
 #undef TN_LAUNCH_FAST
+#undef TN_LAUNCH_FAST1
   TN_HIP(hipGetLastError());
 }
 

@@ -46,6 +46,9 @@ CASES = [
     # even bond dimensions >= 128: 16-byte aligned rows take the double-buffered
     # kernels, including the transposed operands of the environment updates
     ("xxz", 6, [2, 4, 132, 136, 4], [2, 3, 4]),
+    # odd bond dimensions >= 128: rows not 16-byte aligned take the same
+    # kernels with element-granular loads
+    ("xxz", 6, [2, 4, 131, 137, 4], [2, 3, 4]),
 ]
 
 
@@ -650,22 +653,23 @@ def test_dmrg_j1j2_cylinder_6x12_matches_golden(T):
     assert all(abs(a - b) <= max(2, 0.1 * b) for a, b in zip(links, g["linkdims"]))
 
 
+@pytest.mark.parametrize("cap", [256, 255])
 @pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
-def test_matvec_j1j2_cylinder_6x12_full_size(T, prec, tol):
+def test_matvec_j1j2_cylinder_6x12_full_size(T, prec, tol, cap):
     """The wide-MPO product at the J1-J2 6 x 12 cylinder's full size: a random
-    MPS with bond dimensions min(2^k, 2^(N-k), 256), environments built over
+    MPS with bond dimensions min(2^k, 2^(N-k), cap), environments built over
     all 72 sites, two-site products at bulk positions (MPO bonds 17 and 20)
-    against the oracle."""
+    against the oracle (cap 255: odd, unaligned rows)."""
     Ly, Lx = 6, 12
     N = Ly * Lx
     Ws = T.j1j2_cylinder_mpo(Ly, Lx, 1.0, 0.5)
-    chis = [min(2 ** k, 2 ** (N - k), 256) for k in range(1, N)]
+    chis = [min(2 ** k, 2 ** (N - k), cap) for k in range(1, N)]
     A = rand_mps(N, 2, chis, seed=72)
     env = T.StateEnvs(A, Ws)
     ref = O.Envs(Ws)
     rng = np.random.default_rng(73)
     for pos in (30, 36, 41):
-        print(f"j1j2 full-size matvec {prec} pos={pos}", flush=True)
+        print(f"j1j2 full-size matvec {prec} cap={cap} pos={pos}", flush=True)
         T.position_(env, pos)
         ref.position(A, pos)
         dims, n = env.phi_dims()
