@@ -197,15 +197,22 @@ def bulk_updates(T, chi, prec, nwarm, ntimed, cutoff=1e-10, kd=5, mi=2):
     tm = ctx.timers(reset=False)
     g_ms, g_n = phase(ctx, 4)
     ctx.enable_timing(False)
+    # the timed updates record HIP events around each matvec GEMM launch only
+    # (no synchronisation inside the timed region)
+    ctx.timers(reset=True)
+    ctx.check(ctx.lib.tn_ctx_enable_timing_mask(ctx.h, 1 << 4))
     ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
     t0 = time.perf_counter()
     for _ in range(ntimed):
         one()
     ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
     dt = (time.perf_counter() - t0) / max(ntimed, 1)
+    t_ms, t_n = phase(ctx, 4)
+    ctx.enable_timing(False)
     env.close()
     return {"s_per_update": dt, "warm_phase_ms": {k: round(v, 1) for k, v in tm.items()},
-            "warm_gemm_ms": g_ms, "warm_gemm_launches": g_n, "warm": rec[:nwarm], "timed": rec[nwarm:]}
+            "warm_gemm_ms": g_ms, "warm_gemm_launches": g_n, "timed_gemm_ms": t_ms, "timed_gemm_launches": t_n,
+            "warm": rec[:nwarm], "timed": rec[nwarm:]}
 
 
 def gemm_flops(u, d=2):
@@ -286,11 +293,14 @@ def cpu_sample(chi_s, kd, mi):
     same 26-site window at maxdim chi_s (a bounded sample); returns its time
     and cost-model record."""
     from oracle import tn_oracle as O
+    # threads the BLAS runs (its pool size), capped by the CPUs this process may run on
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
     try:
         from threadpoolctl import threadpool_info
-        cores = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+        blas = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
     except Exception:
-        cores = os.cpu_count() or 1
+        blas = avail
+    cores = max(1, min(blas, avail))
     Ws = wide_window(WINDOW, WINDOW_FIRST)
     w = [Ws[0].shape[0]] + [W.shape[3] for W in Ws]
     dims = window_dims(chi_s)
@@ -557,9 +567,10 @@ def main():
         return
 
     # 1. matvec microbenchmark at chi = mv_chi
-    mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", "bf16x3"])
-    mv_wide = None
-    if args.mv_wide:
+    mv, mv_wide = None, None
+    if args.mv_iters > 0:
+        mv, F_mv, F_gemm = matvec_microbench(T, args.mv_chi, args.mv_iters, ["fp64", "fp32", "bf16x3"])
+    if args.mv_wide and args.mv_iters > 0:
         mv_wide, F_wide, _ = matvec_microbench(T, args.mv_chi, max(2, args.mv_iters // 2),
                                                ["fp64", "fp32", "bf16x3"], Ws=wide_mpo4())
     if args.mv_only:
@@ -574,16 +585,18 @@ def main():
     log(f"bulk updates chi={chi} {args.prec}: {mixed['s_per_update']:.3f} s/update -> {value:.1f} s/sweep "
         f"(x{scale:.2f}); dims {[(u['chil'], u['chir'], u['nkept'], u['numops']) for u in mixed['timed']]}; "
         f"warmup phases {mixed['warm_phase_ms']}")
-    fp64 = bulk_updates(T, chi, "fp64", 1, args.fp64_steps, kd=kd, mi=mi)
-    scale64 = sweep_scale(chi, fp64["timed"])
-    sweep64 = fp64["s_per_update"] * scale64
-    log(f"bulk updates chi={chi} fp64: {fp64['s_per_update']:.3f} s/update -> {sweep64:.1f} s/sweep; "
-        f"dims {[(u['chil'], u['chir'], u['nkept'], u['numops']) for u in fp64['timed']]}; "
-        f"warmup phases {fp64['warm_phase_ms']}")
-    # dominant kernel of the updates: the matvec GEMM launches of the
-    # instrumented warmup updates
-    flops = sum(gemm_flops(u) for u in mixed["warm"])
-    gemm_tf = flops / (mixed["warm_gemm_ms"] * 1e-3) / 1e12 if mixed["warm_gemm_ms"] > 0 else None
+    fp64, sweep64 = None, None
+    if args.fp64_steps > 0:
+        fp64 = bulk_updates(T, chi, "fp64", 1, args.fp64_steps, kd=kd, mi=mi)
+        scale64 = sweep_scale(chi, fp64["timed"])
+        sweep64 = fp64["s_per_update"] * scale64
+        log(f"bulk updates chi={chi} fp64: {fp64['s_per_update']:.3f} s/update -> {sweep64:.1f} s/sweep; "
+            f"dims {[(u['chil'], u['chir'], u['nkept'], u['numops']) for u in fp64['timed']]}; "
+            f"warmup phases {fp64['warm_phase_ms']}")
+    # dominant kernel of the updates: the matvec GEMM launches of the timed
+    # updates (HIP events on the library stream around each launch)
+    flops = sum(gemm_flops(u) for u in mixed["timed"])
+    gemm_tf = flops / (mixed["timed_gemm_ms"] * 1e-3) / 1e12 if mixed["timed_gemm_ms"] > 0 else None
     gemm_frac = gemm_tf / PEAK["fp32"] if gemm_tf else None
 
     # 3. measured full sweeps at chi = 4096 (N = 26 chain)
@@ -633,22 +646,24 @@ def main():
                                f"value = mean step x {scale:.2f} (algorithmic cost of the 142-update sweep / mean "
                                f"cost of the timed steps), extrapolated",
                    "parallelism": "single GPU", "extrapolated": True},
-        "fp64_gpu": {"s_per_update": round(fp64["s_per_update"], 4), "updates_timed": args.fp64_steps,
-                     "sweep_s": round(sweep64, 3), "warm_phase_ms": fp64["warm_phase_ms"]},
+        "fp64_gpu": ({"s_per_update": round(fp64["s_per_update"], 4), "updates_timed": args.fp64_steps,
+                      "sweep_s": round(sweep64, 3), "warm_phase_ms": fp64["warm_phase_ms"]} if fp64 else None),
         "mixed_warm_phase_ms": mixed["warm_phase_ms"],
-        "speedup_vs_fp64": round(sweep64 / value, 3),
+        "speedup_vs_fp64": round(sweep64 / value, 3) if sweep64 else None,
         "n26_full_sweeps_chi4096": n26,
-        "matvec_chi4096": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv.items()},
-        "matvec_pct_fp32_peak": {k: round(100 * v["matvec_tflops"] / PEAK["fp32"], 2) for k, v in mv.items()},
+        "matvec_chi4096": ({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv.items()} if mv else None),
+        "matvec_pct_fp32_peak": ({k: round(100 * v["matvec_tflops"] / PEAK["fp32"], 2) for k, v in mv.items()}
+                                 if mv else None),
         "matvec_chi4096_w20": ({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv_wide.items()}
                                if mv_wide else None),
         "roofline": {"bound": "mfma", "achieved": (round(gemm_tf, 2) if gemm_tf else None), "peak": PEAK["fp32"],
                      "unit": "TFLOP/s", "frac": (round(gemm_frac, 4) if gemm_frac else None),
                      "traffic": None,
-                     "kernel": f"matvec GEMM ({args.prec}) of the bulk updates: M=w*{chi}, N=4*{chi}, K={chi} and "
-                               f"M=4*{chi}, N={chi}, K=w*{chi} (w=17..20); {mixed['warm_gemm_launches']} launches, "
-                               f"{mixed['warm_gemm_ms']:.1f} ms, {flops:.3e} algorithmic flop in the instrumented "
-                               f"warmup updates"},
+                     "kernel": f"matvec GEMM ({args.prec}) of the bulk updates: M=w*chi_l, N=4*chi_r, K=chi_l and "
+                               f"M=4*chi_l, N=chi_r, K=w*chi_r (w=17..20, chi up to {chi}); "
+                               f"{mixed['timed_gemm_launches']} launches, {mixed['timed_gemm_ms']:.1f} ms "
+                               f"({mixed['timed_gemm_ms'] / max(1, mixed['timed_gemm_launches']):.2f} ms per launch), "
+                               f"{flops:.3e} algorithmic flop in the timed updates"},
         "tdvp_xxz_n100": tdvp,
         "ttn_random_field_batch": ttn,
         "hubbard_qn_n64": hub,

@@ -86,7 +86,10 @@ typedef struct tn_update_params {
   double tol;          /* 1e-14 (eig) / 1e-12 (exp)                           */
   int check_convergence;
   int precision;       /* TN_PREC_*: precision of the Krylov matvecs          */
-  int refine_fp64;     /* energy from one FP64 matvec of the converged vector */
+  int refine_fp64;     /* mixed precision only.  0: none; 1: the reported energy is the
+                          Rayleigh quotient of one FP64 product of the converged vector;
+                          k >= 2: one FP64 Lanczos run of Krylov dimension k started from
+                          the converged vector (moves the state; the BF16x3 default is 2) */
   double noise;        /* 0: density-matrix perturbation noise * noiseterm(PH, phi, ortho)
                           added before the factorisation (update_site.jl:60-62)         */
 } tn_update_params;
@@ -115,6 +118,11 @@ int tn_memcpy_d2h(tn_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes
 int tn_ctx_timers(tn_ctx* ctx, double* ms4, int reset);
 /* turn per-phase event timing on (1) or off (0); off by default */
 int tn_ctx_enable_timing(tn_ctx* ctx, int on);
+/* time only the phases whose bit is set in mask (bit p: phase p of
+ * tn_ctx_phase_stats); 0 turns timing off.  Events are recorded on the
+ * context's stream without synchronising, so timing one phase (e.g. bit 4,
+ * the matvec GEMM launches) leaves a timed region's wall clock as it is */
+int tn_ctx_enable_timing_mask(tn_ctx* ctx, unsigned mask);
 /* accumulated device ms and event count of one phase: 0 matvec, 1 env update,
  * 2 Krylov vector ops, 3 SVD/replacebond, 4 each GEMM launch of a matvec */
 int tn_ctx_phase_stats(tn_ctx* ctx, int phase, double* ms, long* count);
@@ -172,10 +180,13 @@ int tn_env_set_noise_channels(tn_env* env, const int* skip_left, const int* skip
  * each, sites concatenated); link_qn: for every link 0..N (link k carries the
  * QN of sites 1..k) the QNs of its elements, links concatenated.  The MPS must
  * be block-sparse with respect to them and right-orthonormal with orthocenter
- * 1 (a product state is).  replacebond! then factorises block by block with
- * NDTensors' block truncation and updates the link QNs; one-site updates,
- * noise, projectors and complex tensors are not built in this mode
- * (TN_E_UNSUPPORTED). */
+ * 1 (a product state is).  The MPO must conserve the QNs (every nonzero
+ * W[wl, s', s, wr] with flux(wr) = flux(wl) + q(s') - q(s), total flux 0;
+ * else TN_E_INVALID).  replacebond! then factorises block by block with
+ * NDTensors' block truncation and updates the link QNs (kept elements grouped
+ * by QN); the two-site FP64 and FP32 products run as block-sparse GEMMs over
+ * the link sectors.  One-site updates, noise, projectors and complex tensors
+ * are not built in this mode (TN_E_UNSUPPORTED). */
 int tn_env_set_qn(tn_env* env, int nq, const int* site_qn, const int* link_qn);
 /* QNs of link k (0..N): *n elements, nq ints each into out (capacity cap ints) */
 int tn_env_get_link_qn(tn_env* env, int k, int* out, long cap, long* n);

@@ -113,8 +113,9 @@ struct tn_ctx {
     hstage_bytes = nb;
     return hstage;
   }
-  // per-phase timing
+  // per-phase timing: phases whose bit is set in timing_mask
   bool timing = false;
+  unsigned timing_mask = 0;
   double phase_ms[tn::PH_COUNT] = {0, 0, 0, 0, 0, 0};
   long phase_n[tn::PH_COUNT] = {0, 0, 0, 0, 0, 0};
   double phase_bytes[tn::PH_COUNT] = {0, 0, 0, 0, 0, 0};  // algorithmic bytes of timed launches

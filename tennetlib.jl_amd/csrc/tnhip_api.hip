@@ -58,7 +58,7 @@ int guarded(tn_ctx* c, F&& f) {
 // ---------------------------------------------------------------------------
 namespace tn {
 PhaseTimer::PhaseTimer(tn_ctx* ctx, int phase) : c(ctx), ph(phase), idx((size_t)-1) {
-  if (!c->timing) return;
+  if (!c->timing || !((c->timing_mask >> phase) & 1u)) return;
   // flush only when no timer is open: an enclosing timer's stop event is
   // not recorded yet and must keep its slot
   if (c->ev_used >= 4096 && c->open_timers == 0) flush_timers(c);
@@ -2744,6 +2744,15 @@ int tn_ctx_enable_timing(tn_ctx* c, int on) {
     TN_REQUIRE(c, TN_E_INVALID, "null context");
     flush_timers(c);
     c->timing = on != 0;
+    c->timing_mask = on != 0 ? ~0u : 0u;
+  });
+}
+int tn_ctx_enable_timing_mask(tn_ctx* c, unsigned mask) {
+  return guarded(c, [&] {
+    TN_REQUIRE(c, TN_E_INVALID, "null context");
+    flush_timers(c);
+    c->timing = mask != 0;
+    c->timing_mask = mask;
   });
 }
 

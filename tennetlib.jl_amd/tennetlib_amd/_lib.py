@@ -77,6 +77,7 @@ SIGNATURES = {
     "tn_memcpy_d2h": [_P, _P, _P, C.c_size_t],
     "tn_ctx_timers": [_P, _PD, _I],
     "tn_ctx_enable_timing": [_P, _I],
+    "tn_ctx_enable_timing_mask": [_P, C.c_uint],
     "tn_ctx_phase_stats": [_P, _I, _PD, _PL],
     "tn_ctx_phase_bytes": [_P, _I, _PD],
     "tn_update_params_default": [C.POINTER(UpdateParams), _I],
