@@ -109,6 +109,118 @@ __global__ __launch_bounds__(256) void wstep_vec_kernel(int G, int Gp, int D, in
   }
 }
 
+// The W step of a dense operator (a compressed MPO pair: every output row
+// mixes most of the (w, S) inputs) as an MFMA product per workgroup:
+// out[(g', S'), cols] = Op[(g', S'), (g, S)] * slab[(g, S), cols] for one A
+// index and WM_NB columns.  Op (Mp x Kp, zero-padded) and the input slab
+// (Kp x WM_NB) sit in LDS; each wave owns 32 output columns and all row
+// blocks (FP32: v_mfma_f32_32x32x2f32, FP64: v_mfma_f64_16x16x4f64).
+constexpr int WM_NB = 128;
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// SPLIT (T = float): the result goes out as bf16 hi/lo pairs (ohi, olo).
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(256) void wstep_mfma_kernel(int G, int Gp, int D, int Dp, long nB,
+                                                         const T* __restrict__ in, long in_sg, long in_sA,
+                                                         long in_sS, T* __restrict__ out, long out_sg,
+                                                         long out_sA, long out_sS, const T* __restrict__ opd,
+                                                         int Mp, int Kp, unsigned short* __restrict__ ohi,
+                                                         unsigned short* __restrict__ olo) {
+  constexpr bool F64 = sizeof(T) == 8;
+  extern __shared__ unsigned char smem_raw[];
+  const int M = Gp * Dp, K = G * D;
+  const int lda = Kp + 1;
+  T* Aop = reinterpret_cast<T*>(smem_raw);  // Mp x lda
+  T* slab = Aop + Mp * lda;                 // Kp x WM_NB
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const long Ai = blockIdx.y;
+  const long B0 = (long)blockIdx.x * WM_NB;
+  for (int t = tid; t < Mp * Kp; t += 256) {
+    const int m = t / Kp, k = t % Kp;
+    Aop[m * lda + k] = (m < M && k < K) ? opd[m * K + k] : T(0);
+  }
+  for (int t = tid; t < Kp * WM_NB; t += 256) {
+    const int i = t / WM_NB, col = t % WM_NB;
+    T v = T(0);
+    if (i < K && B0 + col < nB) {
+      const int g = i / D, S = i % D;
+      v = in[g * in_sg + Ai * in_sA + S * in_sS + B0 + col];
+    }
+    slab[i * WM_NB + col] = v;
+  }
+  __syncthreads();
+  if constexpr (!F64) {
+    const int nmb = Mp / 32;  // <= 4
+    f16v acc[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[i] = f16v{};
+    for (int k0 = 0; k0 < Kp; k0 += 2) {
+      const int kr = k0 + (lane >> 5);
+      const float bv = slab[kr * WM_NB + wave * 32 + (lane & 31)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (i < nmb) acc[i] = __builtin_amdgcn_mfma_f32_32x32x2f32(Aop[(i * 32 + (lane & 31)) * lda + kr], bv,
+                                                                   acc[i], 0, 0, 0);
+    }
+    // register r of lane: row 8(r/4) + 4(lane/32) + r%4, column lane%32
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (i >= nmb) continue;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        const long col = B0 + wave * 32 + (lane & 31);
+        if (row < M && col < nB) {
+          const int gp = row / Dp, Sp = row % Dp;
+          const long off = gp * out_sg + Ai * out_sA + Sp * out_sS + col;
+          if constexpr (SPLIT) {
+            const float x = acc[i][r];
+            const unsigned short h = f2bf_rne(x);
+            ohi[off] = h;
+            olo[off] = f2bf_rne(x - __uint_as_float(((unsigned int)h) << 16));
+          } else {
+            out[off] = acc[i][r];
+          }
+        }
+      }
+    }
+  } else {
+    const int nmb = Mp / 16;  // <= 8
+    d4 acc[8][2];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i][0] = acc[i][1] = d4{0, 0, 0, 0};
+    for (int k0 = 0; k0 < Kp; k0 += 4) {
+      const int kr = k0 + (lane >> 4);
+      const double b0 = slab[kr * WM_NB + wave * 32 + (lane & 15)];
+      const double b1 = slab[kr * WM_NB + wave * 32 + 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        if (i >= nmb) continue;
+        const double av = Aop[(i * 16 + (lane & 15)) * lda + kr];
+        acc[i][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b0, acc[i][0], 0, 0, 0);
+        acc[i][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b1, acc[i][1], 0, 0, 0);
+      }
+    }
+    // register r of lane: row lane/16 + 4r, column lane%16
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (i >= nmb) continue;
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = i * 16 + (lane >> 4) + 4 * r;
+          const long col = B0 + wave * 32 + j * 16 + (lane & 15);
+          if (row < M && col < nB) {
+            const int gp = row / Dp, Sp = row % Dp;
+            out[gp * out_sg + Ai * out_sA + Sp * out_sS + col] = acc[i][j][r];
+          }
+        }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // conversions / fills
 // ---------------------------------------------------------------------------
@@ -341,6 +453,14 @@ void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>
     TN_HIP(hipMemcpy(op.colidx.p, ci.data(), ci.size() * sizeof(int), hipMemcpyHostToDevice));
     TN_HIP(hipMemcpy(op.vals.p, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice));
   }
+  op.dense = rows <= 128 && cols <= 128 && op.nnz >= 8 * rows;
+  if (op.dense) {
+    std::vector<float> f(dense.begin(), dense.begin() + (size_t)rows * cols);
+    op.d64.ensure((size_t)rows * cols * sizeof(double));
+    op.d32.ensure((size_t)rows * cols * sizeof(float));
+    TN_HIP(hipMemcpy(op.d64.p, dense.data(), (size_t)rows * cols * sizeof(double), hipMemcpyHostToDevice));
+    TN_HIP(hipMemcpy(op.d32.p, f.data(), (size_t)rows * cols * sizeof(float), hipMemcpyHostToDevice));
+  }
 }
 
 static void wstep_launch(tn_ctx* c, bool fp32, bool split, const WStepDesc& w, unsigned short* hi,
@@ -353,6 +473,37 @@ static void wstep_launch(tn_ctx* c, bool fp32, bool split, const WStepDesc& w, u
   const int* rp = w.op->rowptr.as<int>();
   const int* ci = w.op->colidx.as<int>();
   const double* vv = w.op->vals.as<double>();
+  if (w.op->dense) {
+    // dense operator: MFMA product per workgroup when Op and the slab fit in LDS
+    const int mb = fp32 ? 32 : 16, ks = fp32 ? 2 : 4;
+    const int M = w.Gp * w.Dp;
+    const int Mp = (M + mb - 1) / mb * mb, Kp = (K + ks - 1) / ks * ks;
+    const size_t es = fp32 ? sizeof(float) : sizeof(double);
+    const size_t mshm = ((size_t)Mp * (Kp + 1) + (size_t)Kp * WM_NB) * es;
+    if (mshm <= 160 * 1024 && Mp <= 128) {
+      dim3 grid((unsigned)((w.nB + WM_NB - 1) / WM_NB), (unsigned)w.nA);
+      if (split) {
+        if (mshm > 64 * 1024) raise_lds_limit((const void*)wstep_mfma_kernel<float, true>, mshm);
+        hipLaunchKernelGGL((wstep_mfma_kernel<float, true>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
+                           w.Dp, w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)nullptr, w.out_sg,
+                           w.out_sA, w.out_sS, w.op->d32.as<float>(), Mp, Kp, hi, lo);
+      } else if (fp32) {
+        if (mshm > 64 * 1024) raise_lds_limit((const void*)wstep_mfma_kernel<float, false>, mshm);
+        hipLaunchKernelGGL((wstep_mfma_kernel<float, false>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
+                           w.Dp, w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg,
+                           w.out_sA, w.out_sS, w.op->d32.as<float>(), Mp, Kp, (unsigned short*)nullptr,
+                           (unsigned short*)nullptr);
+      } else {
+        if (mshm > 64 * 1024) raise_lds_limit((const void*)wstep_mfma_kernel<double, false>, mshm);
+        hipLaunchKernelGGL((wstep_mfma_kernel<double, false>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
+                           w.Dp, w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg,
+                           w.out_sA, w.out_sS, w.op->d64.as<double>(), Mp, Kp, (unsigned short*)nullptr,
+                           (unsigned short*)nullptr);
+      }
+      TN_HIP(hipGetLastError());
+      return;
+    }
+  }
   if (!split) {
     // 16-byte vectors along B when every offset allows them and the slab fits
     const int vec = fp32 ? 4 : 2;

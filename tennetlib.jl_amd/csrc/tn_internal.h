@@ -280,6 +280,10 @@ void split_bf16(tn_ctx* c, const void* src, bool src_f64, long rows, long cols, 
 struct SparseOp {
   int rows = 0, cols = 0, nnz = 0;
   DevBuf rowptr, colidx, vals;  // int, int, double
+  // row-major dense copies (FP64, FP32) for the MFMA W step, kept when the
+  // operator has at least 8 entries per row and at most 128 rows and columns
+  bool dense = false;
+  DevBuf d64, d32;
 };
 struct WStepDesc {
   int G = 0, Gp = 0, D = 0, Dp = 0;

@@ -654,7 +654,7 @@ def test_dmrg_j1j2_cylinder_6x12_matches_golden(T):
 
 
 @pytest.mark.parametrize("cap", [256, 255])
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 1e-4)])
 def test_matvec_j1j2_cylinder_6x12_full_size(T, prec, tol, cap):
     """The wide-MPO product at the J1-J2 6 x 12 cylinder's full size: a random
     MPS with bond dimensions min(2^k, 2^(N-k), cap), environments built over
