@@ -240,6 +240,14 @@ __global__ void scaled_copy_kernel(double* __restrict__ y, const double* __restr
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = a * x[i];
 }
+// y = x / sqrt(max(0, *r2)) with r2 read from device memory (the squared
+// norm a Krylov step left there), so the next basis vector needs no host trip
+__global__ void scale_rnorm_kernel(double* __restrict__ y, const double* __restrict__ x,
+                                   const double* __restrict__ r2, long n) {
+  const double a = 1.0 / sqrt(fmax(0.0, *r2));
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
 
 // ---------------------------------------------------------------------------
 // Krylov: multi-dot (first stage + fixed-order second stage)
@@ -334,6 +342,80 @@ __global__ __launch_bounds__(256) void multi_axpy_kernel(const double* __restric
       if (sub) { x[2 * i] -= ar; x[2 * i + 1] -= ai; }
       else { x[2 * i] = ar; x[2 * i + 1] = ai; }
     }
+  }
+}
+
+// A whole Krylov orthogonalisation step for a short vector in one
+// workgroup: two passes of w <- w - V (V^H w) over the K basis vectors, then
+// |w|^2.  The pass coefficients go to coef[0..) and coef[128..) (complex
+// pairs when cplx) and |w|^2 to coef[256], each also to hout (mapped host
+// memory) at the same offsets.  Dots: one wave per basis vector (DPP sums).
+constexpr int KS_THREADS = 1024;
+template <bool CPLX>
+__global__ __launch_bounds__(KS_THREADS) void krylov_ortho_small_kernel(const double* __restrict__ basis, long ld,
+                                                                        int K, double* __restrict__ w, long n,
+                                                                        double* __restrict__ coef,
+                                                                        double* __restrict__ hout) {
+  __shared__ double cf[128];
+  __shared__ double red[KS_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  constexpr int NW = KS_THREADS / 64;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = wave; i < K; i += NW) {
+      const double* v = basis + (long)i * ld;
+      double re = 0.0, im = 0.0;
+      if (!CPLX) {
+        for (long j = lane; j < n; j += 64) re = fma(v[j], w[j], re);
+      } else {
+        for (long j = lane; j < n; j += 64) {
+          const double vr = v[2 * j], vi = v[2 * j + 1], wr = w[2 * j], wi = w[2 * j + 1];
+          re = fma(vr, wr, fma(vi, wi, re));
+          im = fma(vr, wi, fma(-vi, wr, im));
+        }
+      }
+      re = wave_sum64(re);
+      if (CPLX) im = wave_sum64(im);
+      if (lane == 0) {
+        if (CPLX) { cf[2 * i] = re; cf[2 * i + 1] = im; }
+        else cf[i] = re;
+      }
+    }
+    __syncthreads();
+    for (long j = tid; j < n; j += KS_THREADS) {
+      if (!CPLX) {
+        double acc = 0.0;
+        for (int v = 0; v < K; ++v) acc = fma(cf[v], basis[v * ld + j], acc);
+        w[j] -= acc;
+      } else {
+        double ar = 0.0, ai = 0.0;
+        for (int v = 0; v < K; ++v) {
+          const double cr = cf[2 * v], ci = cf[2 * v + 1];
+          const double br = basis[v * ld + 2 * j], bi = basis[v * ld + 2 * j + 1];
+          ar = fma(cr, br, fma(-ci, bi, ar));
+          ai = fma(cr, bi, fma(ci, br, ai));
+        }
+        w[2 * j] -= ar;
+        w[2 * j + 1] -= ai;
+      }
+    }
+    const int nc = CPLX ? 2 * K : K;
+    for (int i = tid; i < nc; i += KS_THREADS) {
+      coef[128 * pass + i] = cf[i];
+      hout[128 * pass + i] = cf[i];
+    }
+    __syncthreads();  // w complete and cf read before the next pass rewrites cf
+  }
+  const long nd = CPLX ? 2 * n : n;
+  double s = 0.0;
+  for (long j = tid; j < nd; j += KS_THREADS) s = fma(w[j], w[j], s);
+  s = wave_sum64(s);
+  if (lane == 0) red[wave] = s;
+  __syncthreads();
+  if (tid == 0) {
+    double t = 0.0;
+    for (int i = 0; i < NW; ++i) t += red[i];
+    coef[256] = t;
+    hout[256] = t;
   }
 }
 
@@ -578,6 +660,11 @@ void scaled_copy(tn_ctx* c, double* y, const double* x, double a, long n) {
   hipLaunchKernelGGL(scaled_copy_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, y, x, a, n);
   TN_HIP(hipGetLastError());
 }
+void scale_rnorm(tn_ctx* c, double* y, const double* x, const double* r2_dev, long n) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(scale_rnorm_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, y, x, r2_dev, n);
+  TN_HIP(hipGetLastError());
+}
 
 void multi_dot(tn_ctx* c, const double* basis, long ld, int count, const double* x, long n, bool cplx,
                double* out_dev, double* out_host) {
@@ -632,6 +719,19 @@ void add_scaled_dev(tn_ctx* c, double* y, const double* x, const double* coef_de
   hipLaunchKernelGGL(add_scaled_dev_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, y, x, coef_dev, scale, n,
                      cplx ? 1 : 0);
   TN_HIP(hipGetLastError());
+}
+
+bool krylov_ortho_small(tn_ctx* c, const double* basis, long ld, int K, double* w, long n, bool cplx,
+                        double* coef_dev, double* hout) {
+  if (n > 4L * KS_THREADS || K < 1 || K > 61) return false;
+  if (cplx)
+    hipLaunchKernelGGL(krylov_ortho_small_kernel<true>, dim3(1), dim3(KS_THREADS), 0, c->stream, basis, ld, K, w,
+                       n, coef_dev, hout);
+  else
+    hipLaunchKernelGGL(krylov_ortho_small_kernel<false>, dim3(1), dim3(KS_THREADS), 0, c->stream, basis, ld, K, w,
+                       n, coef_dev, hout);
+  TN_HIP(hipGetLastError());
+  return true;
 }
 
 void multi_axpy_sub(tn_ctx* c, const double* basis, long ld, int count, const double* coef_dev, double* x,

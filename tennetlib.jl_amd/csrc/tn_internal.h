@@ -95,6 +95,8 @@ struct tn_ctx {
   static constexpr size_t PINNED_N = 4096;
   double* pinned = nullptr;
   double* pinned_dev = nullptr;  // the same memory as seen from kernels
+  // completion events of the Krylov steps in flight (two slots)
+  hipEvent_t kry_ev[2] = {nullptr, nullptr};
   void* hstage = nullptr;
   size_t hstage_bytes = 0;
   void* host_stage(size_t bytes) {
@@ -322,6 +324,14 @@ void lincomb(tn_ctx* c, const double* basis, long ld, int count, const double* c
 void scale_f64(tn_ctx* c, double* x, double s, long n);
 // y = a*x (a real), n doubles
 void scaled_copy(tn_ctx* c, double* y, const double* x, double a, long n);
+// one Krylov orthogonalisation step (two passes against K basis vectors, then
+// |w|^2) in a single launch for short vectors; coefficients at coef_dev[0..),
+// [128..), |w|^2 at [256], mirrored to hout.  Returns false (nothing queued)
+// when n or K is outside its range
+bool krylov_ortho_small(tn_ctx* c, const double* basis, long ld, int K, double* w, long n, bool cplx,
+                        double* coef_dev, double* hout);
+// y = x / sqrt(max(0, *r2_dev)) (r2_dev: a device scalar)
+void scale_rnorm(tn_ctx* c, double* y, const double* x, const double* r2_dev, long n);
 // row sum of squares of a (rows x L) row-major matrix into out[rows]
 void row_norms2(tn_ctx* c, const double* X, long rows, long L, long ld, double* out);
 // y += scale * coef_dev[0] * x (complex coefficient pair when cplx)

@@ -1075,34 +1075,55 @@ struct Krylov {
     coef = (double*)c->workspace(WS_KRY_COEF, 1024 * sizeof(double));
   }
   double* V(int i) { return base + (long)i * nd; }
-  // <x, x>
+  // <x, x> (result in its own pinned area, clear of the two step slots)
   double norm2(const double* x) {
-    multi_dot(c, x, nd, 1, x, n, cx, coef, c->pinned_dev);
+    multi_dot(c, x, nd, 1, x, n, cx, coef + 384, c->pinned_dev + 1536);
     TN_HIP(hipStreamSynchronize(c->stream));
-    return c->pinned[0];
+    return c->pinned[1536];
   }
-  // w <- w - V_{0..K-1} (V^H w), twice; returns the total coefficients
-  // (complex pairs when cx) and |w|^2.
-  void orthogonalize(int K, std::vector<cplx>& h, double& nrm2) {
-    const int str = cx ? 2 : 1;
+  // Krylov steps run one ahead of the host: step results (both passes'
+  // coefficients and |w|^2) land in pinned slot s (512 doubles each), and
+  // kry_ev[s] marks their completion, so the host reads step k while step
+  // k + 1 is already queued.
+  void ensure_events() {
+    for (hipEvent_t& ev : c->kry_ev)
+      if (!ev) TN_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  }
+  // w <- w - V_{0..K-1} (V^H w), twice (queued; results in slot s)
+  void orthogonalize_async(int K, int s) {
     // coefficient scratch: pass 1 at [0, 2K), pass 2 at [128, 128+2K),
     // |w|^2 at 256 (K <= 61 basis vectors, complex pairs fit in 128)
-    // the three results also land in mapped pinned memory (no read-back copy)
-    multi_dot(c, base, nd, K, w, n, cx, coef, c->pinned_dev);
+    double* hd = c->pinned_dev + 512 * s;
+    if (krylov_ortho_small(c, base, nd, K, w, n, cx, coef, hd)) {
+      ensure_events();
+      TN_HIP(hipEventRecord(c->kry_ev[s], c->stream));
+      return;
+    }
+    multi_dot(c, base, nd, K, w, n, cx, coef, hd);
     multi_axpy_sub(c, base, nd, K, coef, w, n, cx);
-    multi_dot(c, base, nd, K, w, n, cx, coef + 128, c->pinned_dev + 128);
+    multi_dot(c, base, nd, K, w, n, cx, coef + 128, hd + 128);
     multi_axpy_sub(c, base, nd, K, coef + 128, w, n, cx);
-    multi_dot(c, w, nd, 1, w, n, cx, coef + 256, c->pinned_dev + 256);
-    double* hb = c->pinned;
-    TN_HIP(hipStreamSynchronize(c->stream));
+    multi_dot(c, w, nd, 1, w, n, cx, coef + 256, hd + 256);
+    ensure_events();
+    TN_HIP(hipEventRecord(c->kry_ev[s], c->stream));
+  }
+  // the total coefficients (complex pairs when cx) and |w|^2 of slot s
+  void orthogonalize_wait(int K, int s, std::vector<cplx>& h, double& nrm2) {
+    TN_HIP(hipEventSynchronize(c->kry_ev[s]));
+    const double* hb = c->pinned + 512 * s;
     h.resize(K);
     for (int i = 0; i < K; ++i) {
       if (cx) h[i] = cplx(hb[2 * i] + hb[128 + 2 * i], hb[2 * i + 1] + hb[128 + 2 * i + 1]);
       else h[i] = cplx(hb[i] + hb[128 + i], 0.0);
     }
-    (void)str;
     nrm2 = hb[256];
   }
+  void orthogonalize(int K, std::vector<cplx>& h, double& nrm2) {
+    orthogonalize_async(K, 0);
+    orthogonalize_wait(K, 0, h, nrm2);
+  }
+  // V(i) = w / |w| with |w|^2 of the last queued step (device side)
+  void next_from_w(int i) { scale_rnorm(c, V(i), w, coef + 256, nd); }
   // out = sum_i coefs[i] V_i
   void combine(int K, const std::vector<cplx>& co, double* out) {
     // the pinned upload area may still feed an earlier copy: drain the stream first
@@ -1303,19 +1324,37 @@ double tn::krylov_eig_op(tn_ctx* c, bool cx, long n, const std::function<void(co
   }
   std::vector<double> Hm((size_t)(kdim + 1) * (kdim + 1), 0.0);
   auto H = [&](int i, int j) -> double& { return Hm[(size_t)i * (kdim + 1) + j]; };
-  int Kn = 1;
+  // Kn basis vectors; the step that made V(Kn-1)'s product is queued in slot sl
+  int Kn = 1, sl = 0;
   std::vector<cplx> h;
   double r2;
   apply(K.V(0), K.w);
-  inf.numops++;
   {
     PhaseTimer pt(c, PH_KRYLOV);
-    K.orthogonalize(1, h, r2);
+    K.orthogonalize_async(1, sl);
   }
-  H(0, 0) = h[0].real();
-  double beta = std::sqrt(std::max(0.0, r2));
   std::vector<double> theta, U;
   while (true) {
+    // queue the next step before reading this one (it is discarded if this
+    // step ends the cycle on an invariant subspace)
+    const bool ahead = Kn < kdim;
+    if (ahead) {
+      {
+        PhaseTimer pt(c, PH_KRYLOV);
+        K.next_from_w(Kn);
+      }
+      apply(K.V(Kn), K.w);
+      PhaseTimer pt(c, PH_KRYLOV);
+      K.orthogonalize_async(Kn + 1, sl ^ 1);
+    }
+    {
+      PhaseTimer pt(c, PH_KRYLOV);
+      K.orthogonalize_wait(Kn, sl, h, r2);
+    }
+    inf.numops++;
+    H(Kn - 1, Kn - 1) = h[Kn - 1].real();
+    const double beta = std::sqrt(std::max(0.0, r2));
+    TN_REQUIRE(std::isfinite(beta), TN_E_NUMERIC, "eig_solver: non-finite residual");
     if (Kn == kdim || beta <= tol) {
       std::vector<double> Hs((size_t)Kn * Kn);
       for (int i = 0; i < Kn; ++i)
@@ -1338,6 +1377,7 @@ double tn::krylov_eig_op(tn_ctx* c, bool cx, long n, const std::function<void(co
         return theta[0];
       }
       // thick restart: keep (3*kdim + 2*conv)/5 Ritz vectors + residual direction
+      // (here Kn == kdim: no step was queued ahead, w is this step's)
       int keep = (3 * kdim + 2 * conv) / 5;
       keep = std::max(1, std::min(keep, Kn - 1));
       {
@@ -1357,21 +1397,15 @@ double tn::krylov_eig_op(tn_ctx* c, bool cx, long n, const std::function<void(co
       }
       Kn = keep + 1;
       inf.numiter++;
-    } else {
+      apply(K.V(Kn - 1), K.w);
       PhaseTimer pt(c, PH_KRYLOV);
-      scaled_copy(K.c, K.V(Kn), K.w, 1.0 / beta, K.nd);
+      K.orthogonalize_async(Kn, sl);
+    } else {
+      // the step queued ahead made V(Kn) = w / beta and its product
       H(Kn, Kn - 1) = H(Kn - 1, Kn) = beta;
       ++Kn;
+      sl ^= 1;
     }
-    apply(K.V(Kn - 1), K.w);
-    inf.numops++;
-    {
-      PhaseTimer pt(c, PH_KRYLOV);
-      K.orthogonalize(Kn, h, r2);
-    }
-    H(Kn - 1, Kn - 1) = h[Kn - 1].real();
-    beta = std::sqrt(std::max(0.0, r2));
-    TN_REQUIRE(std::isfinite(beta), TN_E_NUMERIC, "eig_solver: non-finite residual");
   }
 }
 
@@ -1411,14 +1445,30 @@ void krylov_expm(tn_env* e, double* phi, cplx t, int kdim, int maxiter, double t
     }
     std::vector<double> Hm((size_t)(kdim + 1) * (kdim + 1), 0.0);
     auto H = [&](int i, int j) -> double& { return Hm[(size_t)i * (kdim + 1) + j]; };
-    int Kn = 0;
+    // Kn basis vectors done; the step of V(Kn) is queued in slot sl, and the
+    // one after it is queued ahead while this one is read (discarded if this
+    // step converges)
+    int Kn = 0, sl = 0;
+    apply_op(e, K.V(0), K.w, prec);
+    {
+      PhaseTimer pt(e->ctx, PH_KRYLOV);
+      K.orthogonalize_async(1, sl);
+    }
     while (true) {
-      apply_op(e, K.V(Kn), K.w, prec);
-      inf.numops++;
+      if (Kn + 1 < kdim) {
+        {
+          PhaseTimer pt(e->ctx, PH_KRYLOV);
+          K.next_from_w(Kn + 1);
+        }
+        apply_op(e, K.V(Kn + 1), K.w, prec);
+        PhaseTimer pt(e->ctx, PH_KRYLOV);
+        K.orthogonalize_async(Kn + 2, sl ^ 1);
+      }
       {
         PhaseTimer pt(e->ctx, PH_KRYLOV);
-        K.orthogonalize(Kn + 1, h, r2);
+        K.orthogonalize_wait(Kn + 1, sl, h, r2);
       }
+      inf.numops++;
       H(Kn, Kn) = h[Kn].real();
       ++Kn;
       const double beta = std::sqrt(std::max(0.0, r2));
@@ -1445,7 +1495,7 @@ void krylov_expm(tn_env* e, double* phi, cplx t, int kdim, int maxiter, double t
       if (err <= tol * remaining || beta <= 1e-14 * hmax) {
         PhaseTimer pt(e->ctx, PH_KRYLOV);
         for (auto& v : y) v *= nx;
-        K.combine(Kn, y, phi);
+        K.combine(Kn, y, phi);  // waits for the step queued ahead, then drops it
         remaining = 0.0;
         break;
       }
@@ -1466,9 +1516,9 @@ void krylov_expm(tn_env* e, double* phi, cplx t, int kdim, int maxiter, double t
         last = frac;
         break;
       }
-      PhaseTimer pt(e->ctx, PH_KRYLOV);
-      scaled_copy(K.c, K.V(Kn), K.w, 1.0 / beta, K.nd);
+      // the step queued ahead made V(Kn) = w / beta and its product
       H(Kn, Kn - 1) = H(Kn - 1, Kn) = beta;
+      sl ^= 1;
     }
   }
   inf.converged = remaining == 0.0 ? 1 : 0;
@@ -2672,6 +2722,8 @@ int tn_ctx_destroy(tn_ctx* c) {
     (void)hipEventDestroy(pr.first);
     (void)hipEventDestroy(pr.second);
   }
+  for (hipEvent_t& ev : c->kry_ev)
+    if (ev) (void)hipEventDestroy(ev);
   c->ws.clear();
   c->ones_buf.release();
   if (c->pinned) (void)hipHostFree(c->pinned);

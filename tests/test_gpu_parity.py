@@ -594,7 +594,14 @@ def test_dmrg_c2_heisenberg_n100_matches_golden(T, prec, tol):
     assert abs(sw.entropy[-1] - g["entropy"][-1]) < 10 * wtol
     if prec == "fp64":
         assert sw.maxchi[4:] == g["maxchi"][4:]
-        assert [a.shape[2] for a in env.psi[:-1]] == g["linkdims"]
+        # a kept weight near ~1e-13 carries an absolute error of ~1e-17 from
+        # either eigensolver, so one lying within ~1e-4 relative of the 1e-12
+        # cutoff's threshold is kept or dropped on rounding: a change of
+        # summation order in the Krylov vector kernels moved one bond of 99
+        # by one (168 against 169)
+        links = [a.shape[2] for a in env.psi[:-1]]
+        assert max(abs(a - b) for a, b in zip(links, g["linkdims"])) <= 1
+        assert sum(a != b for a, b in zip(links, g["linkdims"])) <= 3
         assert len(mid) == len(g["mid_eigs"])
     else:
         # FP32 rounding of the products moves the smallest kept weights (~1e-13)
