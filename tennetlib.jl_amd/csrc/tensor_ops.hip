@@ -222,6 +222,44 @@ __global__ __launch_bounds__(256) void wstep_mfma_kernel(int G, int Gp, int D, i
 }
 
 // ---------------------------------------------------------------------------
+// complex <-> real FP32 layouts of the complex mixed-precision product: a
+// complex product C = A B runs as one real GEMM [Cr | Ci] = [Ar | Ai] *
+// [[Br, Bi], [-Bi, Br]] (four real products' flops, FP32 MFMA)
+// ---------------------------------------------------------------------------
+// dst (rows x 2 cols): [re | im] of the interleaved complex src (rows x cols)
+__global__ void cplx_split_rows_kernel(const double* __restrict__ src, long rows, long cols,
+                                       float* __restrict__ dst) {
+  const long n = rows * cols;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / cols, c = i % cols;
+    dst[r * 2 * cols + c] = (float)src[2 * i];
+    dst[r * 2 * cols + cols + c] = (float)src[2 * i + 1];
+  }
+}
+// dst (2 rows x 2 cols) = [[re, im], [-im, re]] of the interleaved complex src
+__global__ void cplx_block_kernel(const double* __restrict__ src, long rows, long cols, float* __restrict__ dst) {
+  const long n = rows * cols;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / cols, c = i % cols;
+    const float re = (float)src[2 * i], im = (float)src[2 * i + 1];
+    dst[r * 2 * cols + c] = re;
+    dst[r * 2 * cols + cols + c] = im;
+    dst[(rows + r) * 2 * cols + c] = -im;
+    dst[(rows + r) * 2 * cols + cols + c] = re;
+  }
+}
+// interleaved complex dst (rows x cols) from src (rows x 2 cols) = [re | im]
+__global__ void cplx_join_rows_kernel(const float* __restrict__ src, long rows, long cols,
+                                      double* __restrict__ dst) {
+  const long n = rows * cols;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / cols, c = i % cols;
+    dst[2 * i] = (double)src[r * 2 * cols + c];
+    dst[2 * i + 1] = (double)src[r * 2 * cols + cols + c];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // conversions / fills
 // ---------------------------------------------------------------------------
 __global__ void cvt_d2f_kernel(const double* __restrict__ x, float* __restrict__ y, long n) {
@@ -643,6 +681,23 @@ void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n) {
 void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n) {
   if (n <= 0) return;
   hipLaunchKernelGGL(cvt_f2d_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, x, y, n);
+  TN_HIP(hipGetLastError());
+}
+void cplx_split_rows_f32(tn_ctx* c, const double* src, long rows, long cols, float* dst) {
+  if (rows * cols <= 0) return;
+  hipLaunchKernelGGL(cplx_split_rows_kernel, dim3(grid1d(rows * cols)), dim3(256), 0, c->stream, src, rows, cols,
+                     dst);
+  TN_HIP(hipGetLastError());
+}
+void cplx_block_f32(tn_ctx* c, const double* src, long rows, long cols, float* dst) {
+  if (rows * cols <= 0) return;
+  hipLaunchKernelGGL(cplx_block_kernel, dim3(grid1d(rows * cols)), dim3(256), 0, c->stream, src, rows, cols, dst);
+  TN_HIP(hipGetLastError());
+}
+void cplx_join_rows_f64(tn_ctx* c, const float* src, long rows, long cols, double* dst) {
+  if (rows * cols <= 0) return;
+  hipLaunchKernelGGL(cplx_join_rows_kernel, dim3(grid1d(rows * cols)), dim3(256), 0, c->stream, src, rows, cols,
+                     dst);
   TN_HIP(hipGetLastError());
 }
 void fill_f64(tn_ctx* c, double* x, double v, long n) {

@@ -306,6 +306,12 @@ void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>
 void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n);
 void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n);
 void fill_f64(tn_ctx* c, double* x, double v, long n);
+// FP32 layouts of interleaved complex FP64 matrices (rows x cols complex):
+// split rows: dst (rows x 2 cols) = [re | im]; block: dst (2 rows x 2 cols) =
+// [[re, im], [-im, re]]; join rows: interleaved dst from [re | im] rows
+void cplx_split_rows_f32(tn_ctx* c, const double* src, long rows, long cols, float* dst);
+void cplx_block_f32(tn_ctx* c, const double* src, long rows, long cols, float* dst);
+void cplx_join_rows_f64(tn_ctx* c, const float* src, long rows, long cols, double* dst);
 void copy_f64(tn_ctx* c, double* dst, const double* src, long n);
 // out[i] = <basis_i, x> for i < count (basis vectors contiguous with stride ld);
 // cplx: complex128 data of n complex elements, out holds (re, im) pairs.

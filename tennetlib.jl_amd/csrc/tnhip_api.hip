@@ -543,6 +543,24 @@ const float* shadow(tn_env* e, int k, bool left) {
 
 inline long pad8(long x) { return ((x + 7) / 8) * 8; }
 
+// FP32 shadows of complex environments for the complex mixed-precision
+// product: left Lt[(w a'), a] as rows [re | im] (w chi x 2 chi); right
+// Rt[(w2 b), b'] as the real block [[re, im], [-im, re]] (2 w2 chi x 2 chi)
+const float* cshadow(tn_env* e, int k, bool left) {
+  DevBuf& B = left ? e->L32 : e->R32;
+  long& key = left ? e->shadow_key_l : e->shadow_key_r;
+  const EnvT& E = e->LR[k];
+  const long want = e->LR_version[k] * 4096 + k;
+  const long rows = (long)E.w * E.chi, cols = E.chi;
+  if (key != want) {
+    B.ensure((size_t)rows * cols * (left ? 2 : 4) * sizeof(float));
+    if (left) cplx_split_rows_f32(e->ctx, E.buf.as<double>(), rows, cols, B.as<float>());
+    else cplx_block_f32(e->ctx, E.buf.as<double>(), rows, cols, B.as<float>());
+    key = want;
+  }
+  return B.as<float>();
+}
+
 // bf16 hi/lo shadows of the environments for the BF16x3 matvec:
 //   left  Lt[(w a'), a]   -> rows w*chi, k = a contiguous, ld = pad8(chi)
 //   right Rt[(w2 b), b']  -> transposed: rows b', k = (w2 b) contiguous,
@@ -890,6 +908,55 @@ void matvec_qn(tn_env* e, const QnMvPlan& qp, const Dom& D, const SparseOp& op, 
   if (f32) cvt_f32_to_f64(c, (const float*)outp, (double*)out, D.n);
 }
 
+// complex product in FP32 (nsite 1 or 2): the complex GEMMs as real FP32
+// GEMMs on [re | im] rows times [[re, im], [-im, re]] blocks, the W step on
+// the real and imaginary planes; FP64 in and out
+void matvec_cplx32(tn_env* e, const Dom& D, const SparseOp& op, const void* v, void* out) {
+  tn_ctx* c = e->ctx;
+  const int pos = D.pos;
+  const long chil = D.chil, chir = D.chir;
+  const int DD = D.d1 * D.d2;
+  const int w0 = e->w[pos - 1];
+  const int w2 = e->w[pos - 1 + e->nsite];
+  const float* Ls = cshadow(e, pos - 1, true);
+  const float* Rs = cshadow(e, pos + e->nsite, false);
+  const long N1 = (long)DD * chir;
+  float* Bp = (float*)c->workspace(WS_PSI32, (size_t)4 * chil * N1 * sizeof(float));
+  cplx_block_f32(c, (const double*)v, chil, N1, Bp);  // [[re, im], [-im, re]] of psi
+  float* T1 = (float*)c->workspace(WS_T1, (size_t)w0 * chil * 2 * N1 * sizeof(float));
+  float* T2 = (float*)c->workspace(WS_T2, (size_t)chil * DD * 2 * w2 * chir * sizeof(float));
+  float* O32 = (float*)c->workspace(WS_OUT32, (size_t)chil * DD * 2 * chir * sizeof(float));
+  GemmDesc g1;  // [T1r | T1i][(w a'), .] = [Lr | Li] * psi block
+  g1.M = (long)w0 * chil; g1.N = 2 * N1; g1.K = 2 * chil;
+  g1.A = Ls; g1.lda = 2 * chil;
+  g1.B = Bp; g1.ldb = 2 * N1;
+  g1.C = T1; g1.ldc = 2 * N1;
+  {
+    PhaseTimer pg(c, PH_MV_GEMM);
+    gemm(c, Prec::F32, g1);
+  }
+  for (int part = 0; part < 2; ++part) {  // the W step is real: each plane alone
+    WStepDesc ws;
+    ws.G = w0; ws.Gp = w2; ws.D = DD; ws.Dp = DD;
+    ws.nA = chil; ws.nB = chir;
+    ws.in = T1 + part * N1; ws.in_sg = chil * 2 * N1; ws.in_sA = 2 * N1; ws.in_sS = chir;
+    ws.out = T2 + part * (long)w2 * chir; ws.out_sg = chir; ws.out_sA = (long)DD * 2 * w2 * chir;
+    ws.out_sS = 2L * w2 * chir;
+    ws.op = &op;
+    wstep(c, true, ws);
+  }
+  GemmDesc g3;  // [outr | outi][(a' S'), .] = [T2r | T2i] * R block
+  g3.M = chil * DD; g3.N = 2 * chir; g3.K = 2L * w2 * chir;
+  g3.A = T2; g3.lda = 2L * w2 * chir;
+  g3.B = Rs; g3.ldb = 2 * chir;
+  g3.C = O32; g3.ldc = 2 * chir;
+  {
+    PhaseTimer pg(c, PH_MV_GEMM);
+    gemm(c, Prec::F32, g3);
+  }
+  cplx_join_rows_f64(c, O32, chil * DD, chir, (double*)out);
+}
+
 // product(sysenv, v) (ProjMPO.product; mirror src/mps/projmps2.jl:159-173)
 void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   PhaseTimer pt(e->ctx, PH_MATVEC);
@@ -919,8 +986,12 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
       return;
     }
   }
+  if (cx && prec == TN_PREC_FP32) {
+    matvec_cplx32(e, D, op, v, out);
+    return;
+  }
   if (cx && prec != TN_PREC_FP64)
-    throw Error(TN_E_UNSUPPORTED, "mixed-precision matvec is implemented for real tensors only");
+    throw Error(TN_E_UNSUPPORTED, "BF16x3 matvec is implemented for real tensors only");
   if (prec == TN_PREC_BF16X3) {
     // T1 = Lt * psi              (bf16x3, A = Lt split, B^T = psi^T split)
     // T2 = W-step(T1) written as bf16 hi/lo rows (a' S') x (w2 b)

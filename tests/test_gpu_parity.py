@@ -483,6 +483,59 @@ def test_tdvp_exact_xx_chain_free_fermions(T):
     assert abs(eng.swdata.energy[-1]) < 1e-8  # <H> of the XX chain from Neel is 0 and conserved
 
 
+@pytest.mark.parametrize("model,chis", [("xxz", [2, 4, 33, 4, 2]), ("xxz", [2, 4, 150, 131, 4]),
+                                         ("j1j2", [2, 4, 8, 40, 8, 4, 2])])
+def test_matvec_complex_fp32_matches_oracle(T, model, chis):
+    """The complex product in FP32 (real FP32 MFMA GEMMs on [re | im] rows and
+    [[re, im], [-im, re]] blocks, the W step on each plane; sparse and dense
+    pair operators, aligned and unaligned rows) against the oracle's complex
+    product: 2e-6 relative, and really reduced precision."""
+    N = len(chis) + 1
+    if model == "xxz":
+        Ws = O.xxz_mpo(N, 1.0, 0.5)
+    else:
+        Ws = T.j1j2_cylinder_mpo(4, N // 4, 1.0, 0.5)
+    A = rand_mps(N, 2, chis, seed=17, cplx=True)
+    env = T.StateEnvs(A, Ws)
+    ref = O.Envs(Ws, dtype=np.complex128)
+    rng = np.random.default_rng(18)
+    for nsite in (2, 1):
+        T.set_nsite_(env, nsite)
+        ref.nsite = nsite
+        for p in range(1, N - nsite + 2):
+            print(f"complex fp32 {model} nsite={nsite} pos={p}", flush=True)
+            T.position_(env, p)
+            ref.position(A, p)
+            dims, n = env.phi_dims()
+            shape = dims if nsite == 2 else (dims[0], dims[1], dims[3])
+            psi = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+            want = ref.product(psi)
+            got = T.product(env, psi, precision="fp32").reshape(want.shape)
+            err = np.linalg.norm(got - want) / np.linalg.norm(want)
+            assert err <= 2e-6, (p, err)
+            assert err > 1e-12
+
+
+def test_tdvp_fp32_matches_fp64(T):
+    """Two-site TDVP (XXZ, Delta = 0.5, N = 20, from Neel, maxdim 64) with the
+    FP32 complex product in the Krylov exponential against the FP64 form:
+    energies to 1e-6 (the FP32 product's ~1e-7 relative error per step) and
+    the S_z profile to 1e-5 after 10 steps."""
+    N, dt, steps = 20, 0.05, 10
+    Ws = O.xxz_mpo(N, 1.0, 0.5)
+    A = [a.astype(complex) for a in O.neel_mps(N)]
+    e64 = T.TDVPEngine(A, Ws)
+    e32 = T.TDVPEngine(A, Ws)
+    for it in range(steps):
+        print(f"tdvp fp32 step {it}", flush=True)
+        T.tdvpsweep_(e64, -1j * dt, 2, maxdim=64, cutoff=1e-12, outputlevel=0)
+        T.tdvpsweep_(e32, -1j * dt, 2, maxdim=64, cutoff=1e-12, outputlevel=0, precision="fp32")
+    assert np.abs(np.array(e32.swdata.energy) - np.array(e64.swdata.energy)).max() < 1e-6
+    d = np.abs(_sz_profile(e32.sysenv.psi) - _sz_profile(e64.sysenv.psi)).max()
+    print("tdvp fp32 vs fp64 sz", d, flush=True)
+    assert d < 1e-5
+
+
 def test_tdvp_truncated_xxz_matches_oracle(T):
     """XXZ (Delta = 0.5) from Neel with truncation: GPU and oracle TDVP agree
     on the energy (conserved, -(N-1) Delta / 4) and the Sz profile."""
