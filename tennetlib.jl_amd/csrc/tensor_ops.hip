@@ -266,6 +266,15 @@ __global__ void cvt_d2f_kernel(const double* __restrict__ x, float* __restrict__
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = (float)x[i];
 }
+// rows x cols block with row strides lds (source) and ldd (destination)
+__global__ void cvt_d2f_2d_kernel(const double* __restrict__ x, long rows, long cols, long lds,
+                                  float* __restrict__ y, long ldd) {
+  const long n = rows * cols;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / cols, c = i % cols;
+    y[r * ldd + c] = (float)x[r * lds + c];
+  }
+}
 __global__ void cvt_f2d_kernel(const float* __restrict__ x, double* __restrict__ y, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = (double)x[i];
@@ -676,6 +685,16 @@ void wstep_split(tn_ctx* c, const WStepDesc& w, unsigned short* hi, unsigned sho
 void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n) {
   if (n <= 0) return;
   hipLaunchKernelGGL(cvt_d2f_kernel, dim3(grid1d(n)), dim3(256), 0, c->stream, x, y, n);
+  TN_HIP(hipGetLastError());
+}
+void cvt_f64_to_f32_2d(tn_ctx* c, const double* x, long rows, long cols, long lds, float* y, long ldd) {
+  if (rows * cols <= 0) return;
+  if (lds == cols && ldd == cols) {
+    cvt_f64_to_f32(c, x, y, rows * cols);
+    return;
+  }
+  hipLaunchKernelGGL(cvt_d2f_2d_kernel, dim3(grid1d(rows * cols)), dim3(256), 0, c->stream, x, rows, cols, lds, y,
+                     ldd);
   TN_HIP(hipGetLastError());
 }
 void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n) {

@@ -305,6 +305,8 @@ void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>
 // ---------------------------------------------------------------------------
 void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n);
 void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n);
+// rows x cols FP64 block (row stride lds) to FP32 with row stride ldd
+void cvt_f64_to_f32_2d(tn_ctx* c, const double* x, long rows, long cols, long lds, float* y, long ldd);
 void fill_f64(tn_ctx* c, double* x, double v, long n);
 // FP32 layouts of interleaved complex FP64 matrices (rows x cols complex):
 // split rows: dst (rows x 2 cols) = [re | im]; block: dst (2 rows x 2 cols) =

@@ -526,18 +526,23 @@ Dom domain(tn_env* e) {
   return D;
 }
 
-// FP32 shadow of an environment (mixed-precision matvecs)
-const float* shadow(tn_env* e, int k, bool left) {
+inline long pad4(long x) { return ((x + 3) / 4) * 4; }
+
+// FP32 shadow of an environment (mixed-precision matvecs): rows (w, chi) of
+// chi entries, with row stride *ld = chi, or chi rounded up to 4 when pad
+// (16-byte aligned rows for the vector-load GEMM)
+const float* shadow(tn_env* e, int k, bool left, bool pad, long* ld) {
   DevBuf& B = left ? e->L32 : e->R32;
   long& key = left ? e->shadow_key_l : e->shadow_key_r;
   const EnvT& E = e->LR[k];
-  const long want = e->LR_version[k] * 4096 + k;
-  const long n = (long)E.w * E.chi * E.chi;
+  const long want = (e->LR_version[k] * 8192 + k) * 2 + (pad ? 1 : 0);
+  const long rows = (long)E.w * E.chi, ldd = pad ? pad4(E.chi) : E.chi;
   if (key != want) {
-    B.ensure((size_t)n * sizeof(float));
-    cvt_f64_to_f32(e->ctx, E.buf.as<double>(), B.as<float>(), n);
+    B.ensure((size_t)rows * ldd * sizeof(float));
+    cvt_f64_to_f32_2d(e->ctx, E.buf.as<double>(), rows, E.chi, E.chi, B.as<float>(), ldd);
     key = want;
   }
+  *ld = ldd;
   return B.as<float>();
 }
 
@@ -874,8 +879,11 @@ void matvec_qn(tn_env* e, const QnMvPlan& qp, const Dom& D, const SparseOp& op, 
     float* v32 = (float*)c->workspace(WS_PSI32, (size_t)D.n * 4);
     cvt_f64_to_f32(c, (const double*)v, v32, D.n);
     vin = v32;
-    Lp = shadow(e, pos - 1, true);
-    Rp = shadow(e, pos + 2, false);
+    long ldl, ldr;  // unpadded: the block offsets assume row strides chi_l and chi_r
+    Lp = shadow(e, pos - 1, true, false, &ldl);
+    Rp = shadow(e, pos + 2, false, false, &ldr);
+    (void)ldl;
+    (void)ldr;
     outp = c->workspace(WS_OUT32, (size_t)(D.n + 1) / 2 * 8);
   }
   fill_f64(c, (double*)outp, 0.0, f32 ? (D.n + 1) / 2 : D.n);
@@ -1034,24 +1042,29 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   else throw Error(TN_E_INVALID, "unknown precision");
   const size_t es = f32 ? 4 : e->esz;
   const int m2 = cx ? 2 : 1;
-  void* T1 = c->workspace(WS_T1, (size_t)w0 * chil * DD * chir * es);
-  void* T2 = c->workspace(WS_T2, (size_t)w2 * chil * DD * chir * es);
+  // FP32 operands are copies of ours: their rows are padded to 16 bytes so the
+  // GEMMs take vector loads whatever the bond dimensions
+  long lda1 = chil, ldb1 = (long)DD * chir, ld2 = (long)w2 * chir, ldb3 = chir;
   const void* vin = v;
   const void* Lp = L.buf.p;
   const void* Rp = R.buf.p;
   void* outp = out;
   if (f32) {
-    float* v32 = (float*)c->workspace(WS_PSI32, (size_t)D.n * 4);
-    cvt_f64_to_f32(c, (const double*)v, v32, D.n);
+    ldb1 = pad4(ldb1);
+    ld2 = pad4(ld2);
+    float* v32 = (float*)c->workspace(WS_PSI32, (size_t)chil * ldb1 * 4);
+    cvt_f64_to_f32_2d(c, (const double*)v, chil, (long)DD * chir, (long)DD * chir, v32, ldb1);
     vin = v32;
-    Lp = shadow(e, pos - 1, true);
-    Rp = shadow(e, pos + e->nsite, false);
+    Lp = shadow(e, pos - 1, true, true, &lda1);
+    Rp = shadow(e, pos + e->nsite, false, true, &ldb3);
     outp = c->workspace(WS_OUT32, (size_t)D.n * 4);
   }
+  void* T1 = c->workspace(WS_T1, (size_t)w0 * chil * DD * chir * es);
+  void* T2 = c->workspace(WS_T2, (size_t)chil * DD * ld2 * es);
   GemmDesc g1;
   g1.M = (long)w0 * chil; g1.N = DD * chir; g1.K = chil;
-  g1.A = Lp; g1.lda = chil;
-  g1.B = vin; g1.ldb = DD * chir;
+  g1.A = Lp; g1.lda = lda1;
+  g1.B = vin; g1.ldb = ldb1;
   g1.C = T1; g1.ldc = DD * chir;
   {
     PhaseTimer pg(c, PH_MV_GEMM);
@@ -1061,13 +1074,13 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   ws.G = w0; ws.Gp = w2; ws.D = DD; ws.Dp = DD;
   ws.nA = chil; ws.nB = chir * m2;
   ws.in = T1; ws.in_sg = chil * DD * chir * m2; ws.in_sA = DD * chir * m2; ws.in_sS = chir * m2;
-  ws.out = T2; ws.out_sg = chir * m2; ws.out_sA = (long)DD * w2 * chir * m2; ws.out_sS = (long)w2 * chir * m2;
+  ws.out = T2; ws.out_sg = chir * m2; ws.out_sA = (long)DD * ld2 * m2; ws.out_sS = ld2 * m2;
   ws.op = &op;
   wstep(c, f32, ws);
   GemmDesc g3;
   g3.M = chil * DD; g3.N = chir; g3.K = (long)w2 * chir;
-  g3.A = T2; g3.lda = (long)w2 * chir;
-  g3.B = Rp; g3.ldb = chir;
+  g3.A = T2; g3.lda = ld2;
+  g3.B = Rp; g3.ldb = ldb3;
   g3.C = outp; g3.ldc = chir;
   {
     PhaseTimer pg(c, PH_MV_GEMM);
