@@ -33,8 +33,8 @@ typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 struct FArgs {
   long M, N, K;
-  const void* A; long lda; long sA;
-  const void* B; long ldb; long sB;
+  const void* A; long lda; long sA; long wA;  // wA, wB: readable row widths
+  const void* B; long ldb; long sB; long wB;
   void* C; long ldc; long sC;
   double alpha, beta;
 };
@@ -153,19 +153,19 @@ __global__ __launch_bounds__(256) void gemm_fast_kernel(FArgs a) {
   // tile k0 of A: rows m (TA = 0, k contiguous) or rows k (TA = 1, m contiguous);
   // a k0 past the end loads zeros (the unconditional prefetch of the last step)
   auto gload = [&](long k0) {
-    const Rsrc ra_r = TA ? tile_rsrc(A, k0, a.K, BK, a.lda, a.M, ES) : tile_rsrc(A, m0, a.M, BM, a.lda, a.K, ES);
-    const Rsrc rb_r = TB ? tile_rsrc(B, n0, a.N, BN, a.ldb, a.K, ES) : tile_rsrc(B, k0, a.K, BK, a.ldb, a.N, ES);
+    const Rsrc ra_r = TA ? tile_rsrc(A, k0, a.K, BK, a.lda, a.wA, ES) : tile_rsrc(A, m0, a.M, BM, a.lda, a.wA, ES);
+    const Rsrc rb_r = TB ? tile_rsrc(B, n0, a.N, BN, a.ldb, a.wB, ES) : tile_rsrc(B, k0, a.K, BK, a.ldb, a.wB, ES);
 #pragma unroll
     for (int i = 0; i < CF::A_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::A_COLS / VN), cc = (idx % (CF::A_COLS / VN)) * VN;
-      ra[i] = TA ? ldvec<T, AL>(ra_r, rr, m0 + cc, a.lda, a.M) : ldvec<T, AL>(ra_r, rr, k0 + cc, a.lda, a.K);
+      ra[i] = TA ? ldvec<T, AL>(ra_r, rr, m0 + cc, a.lda, a.wA) : ldvec<T, AL>(ra_r, rr, k0 + cc, a.lda, a.wA);
     }
 #pragma unroll
     for (int i = 0; i < CF::B_VECS; ++i) {
       const int idx = tid + i * 256;
       const int rr = idx / (CF::B_COLS / VN), cc = (idx % (CF::B_COLS / VN)) * VN;
-      rb[i] = TB ? ldvec<T, AL>(rb_r, rr, k0 + cc, a.ldb, a.K) : ldvec<T, AL>(rb_r, rr, n0 + cc, a.ldb, a.N);
+      rb[i] = TB ? ldvec<T, AL>(rb_r, rr, k0 + cc, a.ldb, a.wB) : ldvec<T, AL>(rb_r, rr, n0 + cc, a.ldb, a.wB);
     }
   };
   auto sstore = [&](int st) {
@@ -488,12 +488,24 @@ static void allow_lds(const void* fn, size_t bytes) { raise_lds_limit(fn, bytes)
 
 // operands whose rows are 16-byte aligned with a valid width of whole
 // vectors take the vector loads; any other operand takes element loads
+// readable row widths of A and B: the valid width, rounded up to whole
+// vectors for an operand the caller marks zero-padded
+static long width_a(Prec p, const GemmDesc& g) {
+  const int vn = (p == Prec::F64) ? 2 : 4;
+  if (g.tA) return g.M;
+  return g.zpA ? (g.K + vn - 1) / vn * vn : g.K;
+}
+static long width_b(Prec p, const GemmDesc& g) {
+  const int vn = (p == Prec::F64) ? 2 : 4;
+  if (g.tB) return g.K;
+  return g.zpB ? (g.N + vn - 1) / vn * vn : g.N;
+}
 static bool fast_aligned(Prec p, const GemmDesc& g) {
   const int vn = (p == Prec::F64) ? 2 : 4;
   auto al = [&](const void* ptr, long ld, long s, long cols) {
-    return ((uintptr_t)ptr % 16 == 0) && (ld % vn == 0) && (s % vn == 0) && (cols % vn == 0);
+    return ((uintptr_t)ptr % 16 == 0) && (ld % vn == 0) && (s % vn == 0) && (cols % vn == 0) && cols <= ld;
   };
-  return al(g.A, g.lda, g.sA, g.tA ? g.M : g.K) && al(g.B, g.ldb, g.sB, g.tB ? g.K : g.N);
+  return al(g.A, g.lda, g.sA, width_a(p, g)) && al(g.B, g.ldb, g.sB, width_b(p, g));
 }
 
 bool gemm_fast_ok(Prec p, const GemmDesc& g) {
@@ -508,13 +520,16 @@ bool gemm_fast_ok(Prec p, const GemmDesc& g) {
 void gemm_fast(tn_ctx* c, Prec p, const GemmDesc& g) {
   FArgs a;
   a.M = g.M; a.N = g.N; a.K = g.K;
+  const bool aligned = fast_aligned(p, g);
   a.A = g.A; a.lda = g.lda; a.sA = g.sA;
   a.B = g.B; a.ldb = g.ldb; a.sB = g.sB;
+  // the vector kernel reads the padded widths; the element kernel the valid ones
+  a.wA = aligned ? width_a(p, g) : (g.tA ? g.M : g.K);
+  a.wB = aligned ? width_b(p, g) : (g.tB ? g.K : g.N);
   a.C = g.C; a.ldc = g.ldc; a.sC = g.sC;
   a.alpha = g.alpha; a.beta = g.beta;
   const long ntm = (g.M + 127) / 128, ntn = (g.N + 127) / 128;
   dim3 grid((unsigned)(ntm * ntn), 1, (unsigned)g.batch);
-  const bool aligned = fast_aligned(p, g);
 #define TN_LAUNCH_FAST1(T, TA, TB, AL)                                                               \
   do {                                                                                               \
     using CF = FastCfg<T, TA, TB>;                                                                   \

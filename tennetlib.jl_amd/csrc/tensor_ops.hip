@@ -266,14 +266,21 @@ __global__ void cvt_d2f_kernel(const double* __restrict__ x, float* __restrict__
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = (float)x[i];
 }
-// rows x cols block with row strides lds (source) and ldd (destination)
+// rows x cols block with row strides lds (source) and ldd (destination);
+// destination columns cols .. ldd - 1 are set to zero
 __global__ void cvt_d2f_2d_kernel(const double* __restrict__ x, long rows, long cols, long lds,
                                   float* __restrict__ y, long ldd) {
-  const long n = rows * cols;
+  const long n = rows * ldd;
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-    const long r = i / cols, c = i % cols;
-    y[r * ldd + c] = (float)x[r * lds + c];
+    const long r = i / ldd, c = i % ldd;
+    y[i] = c < cols ? (float)x[r * lds + c] : 0.0f;
   }
+}
+// columns c0 .. c1 - 1 of a rows x ld FP32 matrix set to zero
+__global__ void zero_cols_f32_kernel(float* __restrict__ y, long rows, long c0, long c1, long ld) {
+  const long w = c1 - c0, n = rows * w;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[(i / w) * ld + c0 + i % w] = 0.0f;
 }
 __global__ void cvt_f2d_kernel(const float* __restrict__ x, double* __restrict__ y, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -693,8 +700,14 @@ void cvt_f64_to_f32_2d(tn_ctx* c, const double* x, long rows, long cols, long ld
     cvt_f64_to_f32(c, x, y, rows * cols);
     return;
   }
-  hipLaunchKernelGGL(cvt_d2f_2d_kernel, dim3(grid1d(rows * cols)), dim3(256), 0, c->stream, x, rows, cols, lds, y,
+  hipLaunchKernelGGL(cvt_d2f_2d_kernel, dim3(grid1d(rows * ldd)), dim3(256), 0, c->stream, x, rows, cols, lds, y,
                      ldd);
+  TN_HIP(hipGetLastError());
+}
+void zero_cols_f32(tn_ctx* c, float* y, long rows, long c0, long c1, long ld) {
+  if (rows <= 0 || c1 <= c0) return;
+  hipLaunchKernelGGL(zero_cols_f32_kernel, dim3(grid1d(rows * (c1 - c0))), dim3(256), 0, c->stream, y, rows, c0, c1,
+                     ld);
   TN_HIP(hipGetLastError());
 }
 void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n) {

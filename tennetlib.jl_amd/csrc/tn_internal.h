@@ -213,6 +213,12 @@ struct GemmDesc {
   // (see gemm_split_bytes) used instead of the context workspace when set
   void* split_ws = nullptr;
   size_t split_cap = 0;
+  // zpA / zpB: the rows of A (not transposed) / B (not transposed) hold finite
+  // values from their valid width (K / N) up to the next 16-byte boundary, so
+  // the fast kernel may read whole vectors there (A's extra columns meet B's
+  // rows past K, which load as zeros; B's extra columns land in discarded
+  // output columns)
+  int zpA = 0, zpB = 0;
 };
 void gemm(tn_ctx* c, Prec p, const GemmDesc& g);
 // bytes of split-K partial products gemm() needs for g (0: no split)
@@ -306,7 +312,10 @@ void build_sparse_op(SparseOp& op, int rows, int cols, const std::vector<double>
 void cvt_f64_to_f32(tn_ctx* c, const double* x, float* y, long n);
 void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n);
 // rows x cols FP64 block (row stride lds) to FP32 with row stride ldd
+// (columns cols .. ldd - 1 of the destination set to zero)
 void cvt_f64_to_f32_2d(tn_ctx* c, const double* x, long rows, long cols, long lds, float* y, long ldd);
+// columns c0 .. c1 - 1 of a rows x ld FP32 matrix set to zero
+void zero_cols_f32(tn_ctx* c, float* y, long rows, long c0, long c1, long ld);
 void fill_f64(tn_ctx* c, double* x, double v, long n);
 // FP32 layouts of interleaved complex FP64 matrices (rows x cols complex):
 // split rows: dst (rows x 2 cols) = [re | im]; block: dst (2 rows x 2 cols) =

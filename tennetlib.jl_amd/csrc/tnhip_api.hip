@@ -1063,8 +1063,8 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   void* T2 = c->workspace(WS_T2, (size_t)chil * DD * ld2 * es);
   GemmDesc g1;
   g1.M = (long)w0 * chil; g1.N = DD * chir; g1.K = chil;
-  g1.A = Lp; g1.lda = lda1;
-  g1.B = vin; g1.ldb = ldb1;
+  g1.A = Lp; g1.lda = lda1; g1.zpA = f32;
+  g1.B = vin; g1.ldb = ldb1; g1.zpB = f32;
   g1.C = T1; g1.ldc = DD * chir;
   {
     PhaseTimer pg(c, PH_MV_GEMM);
@@ -1077,10 +1077,11 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   ws.out = T2; ws.out_sg = chir * m2; ws.out_sA = (long)DD * ld2 * m2; ws.out_sS = ld2 * m2;
   ws.op = &op;
   wstep(c, f32, ws);
+  if (f32 && ld2 > (long)w2 * chir) zero_cols_f32(c, (float*)T2, chil * DD, (long)w2 * chir, ld2, ld2);
   GemmDesc g3;
   g3.M = chil * DD; g3.N = chir; g3.K = (long)w2 * chir;
-  g3.A = T2; g3.lda = ld2;
-  g3.B = Rp; g3.ldb = ldb3;
+  g3.A = T2; g3.lda = ld2; g3.zpA = f32;
+  g3.B = Rp; g3.ldb = ldb3; g3.zpB = f32;
   g3.C = outp; g3.ldc = chir;
   {
     PhaseTimer pg(c, PH_MV_GEMM);
