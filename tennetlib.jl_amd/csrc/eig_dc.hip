@@ -1669,8 +1669,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     double* xb = Qp + (size_t)2 * NBT * p;
     double* partA = xb + p;
     double* partB = partA + (size_t)nAmax * (2 * NBT + 1);
-    if ((size_t)p * sizeof(double) > 64 * 1024)
-      raise_lds_limit((const void*)trd_panel_symv_kernel, (size_t)p * sizeof(double));
+    raise_lds_limit((const void*)trd_panel_symv_kernel, (size_t)p * sizeof(double));
     for (int j0 = 0; j0 < jt; j0 += NBT) {
       const int j1 = std::min(jt, j0 + NBT);
       fill_f64(c, Pp, 0.0, (long)4 * NBT * p);
@@ -1714,8 +1713,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   }
   if (!done) {
     const size_t lds_bytes = (size_t)p * sizeof(double);
-    if (lds_bytes > 64 * 1024)
-      raise_lds_limit((const void*)trd_reflect_kernel, (size_t)lds_bytes);
+    raise_lds_limit((const void*)trd_reflect_kernel, (size_t)lds_bytes);
     const int j0 = std::max(0, P - TAIL);  // columns j0.. are reduced in LDS
     if (j0 >= 1 && P <= 6000) {
       // fused path: reflector 0, then one launch per column, then the update of step j0 - 1

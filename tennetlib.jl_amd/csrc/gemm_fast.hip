@@ -474,15 +474,26 @@ std::recursive_mutex& api_mutex() {
   return m;
 }
 
+// Every launch of a kernel with dynamic LDS goes through this first.  The
+// first call for a kernel sets its dynamic-LDS limit once, to everything the
+// 160 KB LDS leaves beside its static LDS, and it is never changed again, so
+// no attribute change can meet a launch of the same kernel on another thread.
 void raise_lds_limit(const void* fn, size_t bytes) {
-  if (bytes <= 64 * 1024) return;
+  constexpr size_t LDS_BYTES = 160 * 1024;
   static std::mutex mu;
-  static std::map<const void*, size_t> done;  // kernel -> limit set so far
+  static std::map<const void*, size_t> done;  // kernel -> dynamic-LDS limit
   std::lock_guard<std::mutex> lock(mu);
-  size_t& cur = done[fn];
-  if (bytes <= cur) return;
-  TN_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-  cur = bytes;
+  auto it = done.find(fn);
+  if (it == done.end()) {
+    hipFuncAttributes attr;
+    TN_HIP(hipFuncGetAttributes(&attr, fn));
+    const size_t lim = LDS_BYTES - std::min(LDS_BYTES, (size_t)attr.sharedSizeBytes);
+    TN_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim));
+    it = done.emplace(fn, lim).first;
+  }
+  TN_REQUIRE(bytes <= it->second, TN_E_UNSUPPORTED,
+             "kernel needs " + std::to_string(bytes) + " bytes of dynamic LDS, more than the " +
+                 std::to_string(it->second) + " available");
 }
 static void allow_lds(const void* fn, size_t bytes) { raise_lds_limit(fn, bytes); }
 

@@ -151,7 +151,7 @@ void qn_block_eig(tn_ctx* c, const double* R, const int* blk_dev, int nblk, int 
   TN_REQUIRE(pmax >= 1 && pmax <= QN_JACOBI_MAX, TN_E_INVALID, "QN block eigensolver: block too large");
   auto launch = [&](auto kern, int pb) {
     const size_t shm = (size_t)2 * pb * (pb + 1) * sizeof(double);
-    if (shm > 64 * 1024) raise_lds_limit((const void*)kern, shm);
+    raise_lds_limit((const void*)kern, shm);
     hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(JT), shm, c->stream, R, blk_dev, w, V, 40);
   };
   if (pmax <= 32) launch(qn_jacobi_kernel<32>, 32);

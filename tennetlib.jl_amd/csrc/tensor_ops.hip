@@ -619,18 +619,18 @@ static void wstep_launch(tn_ctx* c, bool fp32, bool split, const WStepDesc& w, u
     if (mshm <= 160 * 1024 && Mp <= 128) {
       dim3 grid((unsigned)((w.nB + WM_NB - 1) / WM_NB), (unsigned)w.nA);
       if (split) {
-        if (mshm > 64 * 1024) raise_lds_limit((const void*)wstep_mfma_kernel<float, true>, mshm);
+        raise_lds_limit((const void*)wstep_mfma_kernel<float, true>, mshm);
         hipLaunchKernelGGL((wstep_mfma_kernel<float, true>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
                            w.Dp, w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)nullptr, w.out_sg,
                            w.out_sA, w.out_sS, w.op->d32.as<float>(), Mp, Kp, hi, lo);
       } else if (fp32) {
-        if (mshm > 64 * 1024) raise_lds_limit((const void*)wstep_mfma_kernel<float, false>, mshm);
+        raise_lds_limit((const void*)wstep_mfma_kernel<float, false>, mshm);
         hipLaunchKernelGGL((wstep_mfma_kernel<float, false>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
                            w.Dp, w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg,
                            w.out_sA, w.out_sS, w.op->d32.as<float>(), Mp, Kp, (unsigned short*)nullptr,
                            (unsigned short*)nullptr);
       } else {
-        if (mshm > 64 * 1024) raise_lds_limit((const void*)wstep_mfma_kernel<double, false>, mshm);
+        raise_lds_limit((const void*)wstep_mfma_kernel<double, false>, mshm);
         hipLaunchKernelGGL((wstep_mfma_kernel<double, false>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
                            w.Dp, w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg,
                            w.out_sA, w.out_sS, w.op->d64.as<double>(), Mp, Kp, (unsigned short*)nullptr,
@@ -651,12 +651,12 @@ static void wstep_launch(tn_ctx* c, bool fp32, bool split, const WStepDesc& w, u
         vshm <= 160 * 1024) {
       dim3 grid((unsigned)((w.nB / vec + 63) / 64), (unsigned)w.nA);
       if (fp32) {
-        if (vshm > 64 * 1024) raise_lds_limit((const void*)wstep_vec_kernel<float, 4>, vshm);
+        raise_lds_limit((const void*)wstep_vec_kernel<float, 4>, vshm);
         hipLaunchKernelGGL((wstep_vec_kernel<float, 4>), grid, dim3(256), vshm, c->stream, w.G, w.Gp, w.D, w.Dp,
                            w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg, w.out_sA,
                            w.out_sS, rp, ci, vv);
       } else {
-        if (vshm > 64 * 1024) raise_lds_limit((const void*)wstep_vec_kernel<double, 2>, vshm);
+        raise_lds_limit((const void*)wstep_vec_kernel<double, 2>, vshm);
         hipLaunchKernelGGL((wstep_vec_kernel<double, 2>), grid, dim3(256), vshm, c->stream, w.G, w.Gp, w.D, w.Dp,
                            w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg,
                            w.out_sA, w.out_sS, rp, ci, vv);
