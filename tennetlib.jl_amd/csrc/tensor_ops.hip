@@ -276,6 +276,20 @@ __global__ void cvt_d2f_2d_kernel(const double* __restrict__ x, long rows, long 
     y[i] = c < cols ? (float)x[r * lds + c] : 0.0f;
   }
 }
+// rows x cols FP64 block copied to row stride ldd, columns cols .. ldd - 1 zero
+__global__ void pad_rows_f64_kernel(const double* __restrict__ x, long rows, long cols, long lds,
+                                    double* __restrict__ y, long ldd) {
+  const long n = rows * ldd;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long r = i / ldd, c = i % ldd;
+    y[i] = c < cols ? x[r * lds + c] : 0.0;
+  }
+}
+__global__ void zero_cols_f64_kernel(double* __restrict__ y, long rows, long c0, long c1, long ld) {
+  const long w = c1 - c0, n = rows * w;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[(i / w) * ld + c0 + i % w] = 0.0;
+}
 // columns c0 .. c1 - 1 of a rows x ld FP32 matrix set to zero
 __global__ void zero_cols_f32_kernel(float* __restrict__ y, long rows, long c0, long c1, long ld) {
   const long w = c1 - c0, n = rows * w;
@@ -702,6 +716,18 @@ void cvt_f64_to_f32_2d(tn_ctx* c, const double* x, long rows, long cols, long ld
   }
   hipLaunchKernelGGL(cvt_d2f_2d_kernel, dim3(grid1d(rows * ldd)), dim3(256), 0, c->stream, x, rows, cols, lds, y,
                      ldd);
+  TN_HIP(hipGetLastError());
+}
+void pad_rows_f64(tn_ctx* c, const double* x, long rows, long cols, long lds, double* y, long ldd) {
+  if (rows * ldd <= 0) return;
+  hipLaunchKernelGGL(pad_rows_f64_kernel, dim3(grid1d(rows * ldd)), dim3(256), 0, c->stream, x, rows, cols, lds, y,
+                     ldd);
+  TN_HIP(hipGetLastError());
+}
+void zero_cols_f64(tn_ctx* c, double* y, long rows, long c0, long c1, long ld) {
+  if (rows <= 0 || c1 <= c0) return;
+  hipLaunchKernelGGL(zero_cols_f64_kernel, dim3(grid1d(rows * (c1 - c0))), dim3(256), 0, c->stream, y, rows, c0, c1,
+                     ld);
   TN_HIP(hipGetLastError());
 }
 void zero_cols_f32(tn_ctx* c, float* y, long rows, long c0, long c1, long ld) {

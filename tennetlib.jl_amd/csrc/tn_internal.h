@@ -316,6 +316,9 @@ void cvt_f32_to_f64(tn_ctx* c, const float* x, double* y, long n);
 void cvt_f64_to_f32_2d(tn_ctx* c, const double* x, long rows, long cols, long lds, float* y, long ldd);
 // columns c0 .. c1 - 1 of a rows x ld FP32 matrix set to zero
 void zero_cols_f32(tn_ctx* c, float* y, long rows, long c0, long c1, long ld);
+// FP64: rows x cols block to row stride ldd with zero padding; zero columns
+void pad_rows_f64(tn_ctx* c, const double* x, long rows, long cols, long lds, double* y, long ldd);
+void zero_cols_f64(tn_ctx* c, double* y, long rows, long c0, long c1, long ld);
 void fill_f64(tn_ctx* c, double* x, double v, long n);
 // FP32 layouts of interleaved complex FP64 matrices (rows x cols complex):
 // split rows: dst (rows x 2 cols) = [re | im]; block: dst (2 rows x 2 cols) =

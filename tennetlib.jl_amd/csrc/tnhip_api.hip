@@ -159,6 +159,9 @@ struct tn_env {
   // FP32 shadows of the environments used by mixed-precision matvecs
   DevBuf L32, R32;
   long shadow_key_l = -1, shadow_key_r = -1;
+  // FP64 copies with even row strides of environments of odd bond dimension
+  DevBuf L64, R64;
+  long pad64_key_l = -1, pad64_key_r = -1;
   // bf16 hi/lo shadows (BF16x3 matvec)
   DevBuf Lh, Ll, Rh, Rl;
   long split_key_l = -1, split_key_r = -1;
@@ -547,6 +550,27 @@ const float* shadow(tn_env* e, int k, bool left, bool pad, long* ld) {
 }
 
 inline long pad8(long x) { return ((x + 7) / 8) * 8; }
+
+// A real FP64 environment with 16-byte aligned rows: itself when chi is
+// even, else a copy with row stride chi + 1 and a zero last column
+const double* pad64(tn_env* e, int k, bool left, long* ld) {
+  const EnvT& E = e->LR[k];
+  if (E.chi % 2 == 0) {
+    *ld = E.chi;
+    return E.buf.as<double>();
+  }
+  DevBuf& B = left ? e->L64 : e->R64;
+  long& key = left ? e->pad64_key_l : e->pad64_key_r;
+  const long want = e->LR_version[k] * 8192 + k;
+  const long rows = (long)E.w * E.chi, ldd = E.chi + 1;
+  if (key != want) {
+    B.ensure((size_t)rows * ldd * sizeof(double));
+    pad_rows_f64(e->ctx, E.buf.as<double>(), rows, E.chi, E.chi, B.as<double>(), ldd);
+    key = want;
+  }
+  *ld = ldd;
+  return B.as<double>();
+}
 
 // FP32 shadows of complex environments for the complex mixed-precision
 // product: left Lt[(w a'), a] as rows [re | im] (w chi x 2 chi); right
@@ -1058,13 +1082,19 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
     Lp = shadow(e, pos - 1, true, true, &lda1);
     Rp = shadow(e, pos + e->nsite, false, true, &ldb3);
     outp = c->workspace(WS_OUT32, (size_t)D.n * 4);
+  } else if (!cx) {
+    // FP64: even row strides for the environments and T2 as well
+    ld2 += ld2 & 1;
+    Lp = pad64(e, pos - 1, true, &lda1);
+    Rp = pad64(e, pos + e->nsite, false, &ldb3);
   }
+  const bool zp = f32 || !cx;
   void* T1 = c->workspace(WS_T1, (size_t)w0 * chil * DD * chir * es);
   void* T2 = c->workspace(WS_T2, (size_t)chil * DD * ld2 * es);
   GemmDesc g1;
   g1.M = (long)w0 * chil; g1.N = DD * chir; g1.K = chil;
-  g1.A = Lp; g1.lda = lda1; g1.zpA = f32;
-  g1.B = vin; g1.ldb = ldb1; g1.zpB = f32;
+  g1.A = Lp; g1.lda = lda1; g1.zpA = zp;
+  g1.B = vin; g1.ldb = ldb1; g1.zpB = zp;
   g1.C = T1; g1.ldc = DD * chir;
   {
     PhaseTimer pg(c, PH_MV_GEMM);
@@ -1077,11 +1107,14 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
   ws.out = T2; ws.out_sg = chir * m2; ws.out_sA = (long)DD * ld2 * m2; ws.out_sS = ld2 * m2;
   ws.op = &op;
   wstep(c, f32, ws);
-  if (f32 && ld2 > (long)w2 * chir) zero_cols_f32(c, (float*)T2, chil * DD, (long)w2 * chir, ld2, ld2);
+  if (ld2 > (long)w2 * chir) {
+    if (f32) zero_cols_f32(c, (float*)T2, chil * DD, (long)w2 * chir, ld2, ld2);
+    else zero_cols_f64(c, (double*)T2, chil * DD, (long)w2 * chir, ld2, ld2);
+  }
   GemmDesc g3;
   g3.M = chil * DD; g3.N = chir; g3.K = (long)w2 * chir;
-  g3.A = T2; g3.lda = ld2; g3.zpA = f32;
-  g3.B = Rp; g3.ldb = ldb3; g3.zpB = f32;
+  g3.A = T2; g3.lda = ld2; g3.zpA = zp;
+  g3.B = Rp; g3.ldb = ldb3; g3.zpB = zp;
   g3.C = outp; g3.ldc = chir;
   {
     PhaseTimer pg(c, PH_MV_GEMM);
