@@ -153,7 +153,7 @@ def window_dims(chi, n=WINDOW):
     return [1] + [min(2 ** k, 2 ** (n - k), chi) for k in range(1, n)] + [1]
 
 
-def bulk_updates(T, chi, prec, nwarm, ntimed, cutoff=1e-10, kd=5, mi=2):
+def bulk_updates(T, chi, prec, nwarm, ntimed, cutoff=1e-10, kd=5, mi=2, refine_precision="fp64"):
     """nwarm untimed (instrumented) then ntimed timed two-site updates at the
     centre of a 26-site window of the J1-J2 cylinder (sites 24-49, outer MPO
     bonds closed with fixed random vectors) from a random MPS (seed 7) whose
@@ -176,7 +176,8 @@ def bulk_updates(T, chi, prec, nwarm, ntimed, cutoff=1e-10, kd=5, mi=2):
     rec = []
     # a bulk update in the middle of a sweep: the FP32 form takes no Rayleigh
     # quotient there (the sweep reports only its last update's energy); the
-    # BF16x3 form keeps its FP64 Lanczos refinement, which moves the state
+    # BF16x3 form keeps its Lanczos refinement (FP64 or FP32 products), which
+    # moves the state
     ref = 0 if prec == "fp32" else None
 
     def one():
@@ -185,9 +186,10 @@ def bulk_updates(T, chi, prec, nwarm, ntimed, cutoff=1e-10, kd=5, mi=2):
         step += 1
         chil, chir = env.site_dims(pos)[0], env.site_dims(pos + 1)[1]
         e, terr, r = T.update_position_(env, T.eig_solver, pos, 2, ortho, maxdim=chi, cutoff=cutoff,
-                                        solver_krylovdim=kd, solver_maxiter=mi, precision=prec, refine_fp64=ref)
+                                        solver_krylovdim=kd, solver_maxiter=mi, precision=prec, refine_fp64=ref,
+                                        refine_precision=refine_precision)
         rec.append(dict(pos=pos, left=ortho == "left", chil=chil, chir=chir, wl=w[pos - 1], wr=w[pos + 1],
-                        numops=r.info.numops, nkept=r.nkept, energy=e))
+                        numops=r.info.numops, nkept=r.nkept, energy=e, residual=r.info.residual))
 
     ctx.enable_timing(True)
     ctx.timers(reset=True)
@@ -510,6 +512,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2, help="untimed (instrumented) bulk updates before them")
     ap.add_argument("--chi", type=int, default=4096)
     ap.add_argument("--prec", default="fp32", choices=["fp32", "bf16x3"])
+    ap.add_argument("--refine-precision", default="fp64", choices=["fp64", "fp32"],
+                    help="products of the BF16x3 form's Lanczos refinement")
     ap.add_argument("--krylovdim", type=int, default=5)
     ap.add_argument("--maxiter", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=6, help="timed bulk updates in the FP64 form")
@@ -579,12 +583,13 @@ def main():
 
     # 2. headline: bulk updates at chi on the J1-J2 window, mixed then FP64
     chi, kd, mi = args.chi, args.krylovdim, args.maxiter
-    mixed = bulk_updates(T, chi, args.prec, max(1, args.warmup), args.steps, kd=kd, mi=mi)
+    mixed = bulk_updates(T, chi, args.prec, max(1, args.warmup), args.steps, kd=kd, mi=mi,
+                         refine_precision=args.refine_precision)
     scale = sweep_scale(chi, mixed["timed"])
     value = mixed["s_per_update"] * scale
     log(f"bulk updates chi={chi} {args.prec}: {mixed['s_per_update']:.3f} s/update -> {value:.1f} s/sweep "
         f"(x{scale:.2f}); dims {[(u['chil'], u['chir'], u['nkept'], u['numops']) for u in mixed['timed']]}; "
-        f"warmup phases {mixed['warm_phase_ms']}")
+        f"warmup phases {mixed['warm_phase_ms']}; residuals {[float('%.2e' % u['residual']) for u in mixed['timed']]}")
     fp64, sweep64 = None, None
     if args.fp64_steps > 0:
         fp64 = bulk_updates(T, chi, "fp64", 1, args.fp64_steps, kd=kd, mi=mi)

@@ -92,6 +92,9 @@ typedef struct tn_update_params {
                           the converged vector (moves the state; the BF16x3 default is 2) */
   double noise;        /* 0: density-matrix perturbation noise * noiseterm(PH, phi, ortho)
                           added before the factorisation (update_site.jl:60-62)         */
+  int refine_precision; /* products of a refine_fp64 >= 2 Lanczos run: TN_PREC_FP64
+                           (default) or TN_PREC_FP32 (enough to take a BF16x3 state to
+                           the FP32 form's accuracy at a fraction of an FP64 product)  */
 } tn_update_params;
 
 typedef struct tn_update_result {

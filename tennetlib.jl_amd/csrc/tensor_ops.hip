@@ -296,6 +296,11 @@ __global__ void zero_cols_f32_kernel(float* __restrict__ y, long rows, long c0, 
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[(i / w) * ld + c0 + i % w] = 0.0f;
 }
+__global__ void zero_cols_u16_kernel(unsigned short* __restrict__ y, long rows, long c0, long c1, long ld) {
+  const long w = c1 - c0, n = rows * w;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    y[(i / w) * ld + c0 + i % w] = 0;
+}
 __global__ void cvt_f2d_kernel(const float* __restrict__ x, double* __restrict__ y, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
     y[i] = (double)x[i];
@@ -733,6 +738,12 @@ void zero_cols_f64(tn_ctx* c, double* y, long rows, long c0, long c1, long ld) {
 void zero_cols_f32(tn_ctx* c, float* y, long rows, long c0, long c1, long ld) {
   if (rows <= 0 || c1 <= c0) return;
   hipLaunchKernelGGL(zero_cols_f32_kernel, dim3(grid1d(rows * (c1 - c0))), dim3(256), 0, c->stream, y, rows, c0, c1,
+                     ld);
+  TN_HIP(hipGetLastError());
+}
+void zero_cols_u16(tn_ctx* c, unsigned short* y, long rows, long c0, long c1, long ld) {
+  if (rows <= 0 || c1 <= c0) return;
+  hipLaunchKernelGGL(zero_cols_u16_kernel, dim3(grid1d(rows * (c1 - c0))), dim3(256), 0, c->stream, y, rows, c0, c1,
                      ld);
   TN_HIP(hipGetLastError());
 }

@@ -319,6 +319,8 @@ void zero_cols_f32(tn_ctx* c, float* y, long rows, long c0, long c1, long ld);
 // FP64: rows x cols block to row stride ldd with zero padding; zero columns
 void pad_rows_f64(tn_ctx* c, const double* x, long rows, long cols, long lds, double* y, long ldd);
 void zero_cols_f64(tn_ctx* c, double* y, long rows, long c0, long c1, long ld);
+// y[r * ld + j] = 0 for r < rows, c0 <= j < c1 (bf16 hi/lo planes)
+void zero_cols_u16(tn_ctx* c, unsigned short* y, long rows, long c0, long c1, long ld);
 void fill_f64(tn_ctx* c, double* x, double v, long n);
 // FP32 layouts of interleaved complex FP64 matrices (rows x cols complex):
 // split rows: dst (rows x 2 cols) = [re | im]; block: dst (2 rows x 2 cols) =

@@ -1051,6 +1051,10 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
     ws.out = nullptr; ws.out_sg = chir; ws.out_sA = (long)DD * ld3; ws.out_sS = ld3;
     ws.op = &op;
     wstep_split(c, ws, th, tl);
+    // the W step writes columns below K3 only; the product reads rows up to
+    // pad8(K3), and stale bytes there (possibly Inf/NaN patterns) times the
+    // zero padding of Rt would poison the sums
+    if (ld3 > K3) zero_cols_u16(c, th, 2 * M3, K3, ld3, ld3);  // th and tl are contiguous
     float* o32 = (float*)c->workspace(WS_OUT32, (size_t)D.n * 4);
     {
       PhaseTimer pg(c, PH_MV_GEMM);
@@ -1471,8 +1475,11 @@ double tn::krylov_eig_op(tn_ctx* c, bool cx, long n, const std::function<void(co
     }
     inf.numops++;
     H(Kn - 1, Kn - 1) = h[Kn - 1].real();
+    // a NaN would pass std::max(0.0, .) as zero and end the cycle as if on
+    // an invariant subspace
+    TN_REQUIRE(std::isfinite(r2) && std::isfinite(h[Kn - 1].real()), TN_E_NUMERIC,
+               "eig_solver: non-finite product or residual");
     const double beta = std::sqrt(std::max(0.0, r2));
-    TN_REQUIRE(std::isfinite(beta), TN_E_NUMERIC, "eig_solver: non-finite residual");
     if (Kn == kdim || beta <= tol) {
       std::vector<double> Hs((size_t)Kn * Kn);
       for (int i = 0; i < Kn; ++i)
@@ -1588,6 +1595,8 @@ void krylov_expm(tn_env* e, double* phi, cplx t, int kdim, int maxiter, double t
       }
       inf.numops++;
       H(Kn, Kn) = h[Kn].real();
+      TN_REQUIRE(std::isfinite(r2) && std::isfinite(h[Kn].real()), TN_E_NUMERIC,
+                 "exp_solver: non-finite product or residual");
       ++Kn;
       const double beta = std::sqrt(std::max(0.0, r2));
       std::vector<double> Hs((size_t)Kn * Kn), th, U;
@@ -1669,7 +1678,8 @@ double refine_energy(tn_env* e, double* phi, const tn_update_params* p, tn_krylo
   if (!p->refine_fp64 || p->precision == TN_PREC_FP64) return th;
   if (p->refine_fp64 == 1) return rayleigh(e, phi).real();
   tn_krylov_info ri{};
-  const double th64 = krylov_eig(e, phi, p->refine_fp64, 1, p->tol, TN_PREC_FP64, &ri);
+  const int rp = p->refine_precision == TN_PREC_FP32 ? TN_PREC_FP32 : TN_PREC_FP64;
+  const double th64 = krylov_eig(e, phi, p->refine_fp64, 1, p->tol, rp, &ri);
   if (info) {
     info->numops += ri.numops;
     info->residual = ri.residual;
@@ -2959,6 +2969,7 @@ void tn_update_params_default(tn_update_params* p, int solver) {
   p->check_convergence = 0;
   p->precision = TN_PREC_FP64;
   p->refine_fp64 = 1;
+  p->refine_precision = TN_PREC_FP64;
 }
 
 int tn_env_create(tn_ctx* c, int N, const int* d, const int* w, int dtype, tn_env** out) {
@@ -3389,8 +3400,11 @@ int tn_fullsweep(tn_env* e, const tn_update_params* p, double* energy, double* m
     tn_update_result r{};
     // the sweep reports the energy of its last update; a Rayleigh-quotient
     // refinement (refine_fp64 = 1) changes only that number, so it runs there
-    tn_update_params pm = *p;
+    // (a Lanczos refinement in FP32 moves the state at every update; the last
+    // update refines in FP64, so the reported energy is an FP64 Ritz value)
+    tn_update_params pm = *p, pl = *p;
     if (p->refine_fp64 == 1 && p->precision != TN_PREC_FP64) pm.refine_fp64 = 0;
+    if (p->refine_fp64 >= 2) pl.refine_precision = TN_PREC_FP64;
     for (int b = 1; b <= N - 1; ++b) {
       update_two_site(e, b, true, &pm, &r);
       en = r.energy;
@@ -3398,7 +3412,7 @@ int tn_fullsweep(tn_env* e, const tn_update_params* p, double* energy, double* m
     }
     for (int b = N - 1; b >= 1; --b) {
       std::vector<double> eigs;
-      update_two_site(e, b, false, b == 1 ? p : &pm, &r, &eigs);
+      update_two_site(e, b, false, b == 1 ? &pl : &pm, &r, &eigs);
       en = r.energy;
       mte = std::max(mte, r.truncerr);
       if (b == N / 2) mid.swap(eigs);  // lasteigs[N/2] (sweep.jl:141,158)

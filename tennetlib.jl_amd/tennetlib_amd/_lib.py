@@ -48,7 +48,8 @@ class UpdateParams(C.Structure):
                 ("normalize", C.c_int), ("maxdim", C.c_long), ("mindim", C.c_long),
                 ("cutoff", C.c_double), ("reverse_step", C.c_int), ("krylovdim", C.c_int),
                 ("maxiter", C.c_int), ("tol", C.c_double), ("check_convergence", C.c_int),
-                ("precision", C.c_int), ("refine_fp64", C.c_int), ("noise", C.c_double)]
+                ("precision", C.c_int), ("refine_fp64", C.c_int), ("noise", C.c_double),
+                ("refine_precision", C.c_int)]
 
 
 class UpdateResult(C.Structure):

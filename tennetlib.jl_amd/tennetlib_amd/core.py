@@ -422,6 +422,10 @@ def _params(solver, kwargs) -> L.UpdateParams:
     if ref is None:
         ref = BF16X3_REFINE_KDIM if p.precision == L.TN_PREC_BF16X3 else 1
     p.refine_fp64 = int(ref)
+    # products of that Lanczos refinement: FP64 (default) or FP32
+    p.refine_precision = _precision_code(kwargs.get("refine_precision", "fp64"))
+    if p.refine_precision not in (L.TN_PREC_FP64, L.TN_PREC_FP32):
+        raise ValueError("refine_precision must be 'fp64' or 'fp32'")
     noise = float(kwargs.get("noise", 0.0))
     if noise > 0 and p.reverse_step:
         raise ValueError(f"`updatePosition()` :: `noise={noise}` cannot be greater than zero for "
@@ -501,10 +505,16 @@ def _sweep(sysenv: StateEnvs, solver, swdata: SweepData, pick, kwargs):
     # Rayleigh-quotient refinement (refine_fp64 = 1) changes only that number,
     # not the state, so with outputlevel <= 1 it runs for the last update only;
     # a Lanczos refinement (>= 2) moves the state and runs at every update.
+    # An FP32 Lanczos refinement moves the state at every update; the last
+    # update refines in FP64 so that the reported energy is an FP64 Ritz value.
     p_mid = p
     if p.refine_fp64 == 1 and p.precision != L.TN_PREC_FP64 and outputlevel <= 1:
         p_mid = L.UpdateParams.from_buffer_copy(p)
         p_mid.refine_fp64 = 0
+    p_last = p
+    if p.refine_fp64 >= 2 and p.refine_precision != L.TN_PREC_FP64:
+        p_last = L.UpdateParams.from_buffer_copy(p)
+        p_last.refine_precision = L.TN_PREC_FP64
     cap = 1 << 16
     eigbuf = (C.c_double * cap)()
     t0 = time.perf_counter()
@@ -526,14 +536,14 @@ def _sweep(sysenv: StateEnvs, solver, swdata: SweepData, pick, kwargs):
     for bond in range(N - 1, 0, -1):
         ns = pick(bond)
         last = bond == 1 and ns == 2
-        _update(sysenv, p if last else p_mid, r, bond + 1 if ns == 1 else bond, ns, "right")
+        _update(sysenv, p_last if last else p_mid, r, bond + 1 if ns == 1 else bond, ns, "right")
         energy = r.energy
         maxtruncerr = max(r.truncerr, maxtruncerr)
         lasteigs[bond - 1] = np.array(eigbuf[:min(r.nkept, cap)])
         if outputlevel > 1:
             print(f"At right sweep {swdata.sweepcount} bond {bond} => Energy {energy}, Err {r.truncerr:.2g}")
         if ns == 1 and bond == 1:
-            _update(sysenv, p, r, 1, 1, "right")
+            _update(sysenv, p_last, r, 1, 1, "right")
             energy = r.energy
     sw_time = time.perf_counter() - t0
     swdata.maxchi.append(max(sysenv.site_dims(k)[1] for k in range(1, N + 1)))

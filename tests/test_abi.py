@@ -48,9 +48,10 @@ def test_struct_layouts():
     # C layout of tn_update_result: 3 doubles, long, krylov info (3 ints + double), pointer, long
     assert C.sizeof(L.KrylovInfo) == 24
     assert C.sizeof(L.UpdateResult) == 8 * 3 + 8 + 24 + 8 + 8
-    # ..., precision, refine_fp64 (ints at 84, 88), noise (double at 96)
-    assert C.sizeof(L.UpdateParams) == 104
+    # ..., precision, refine_fp64 (ints at 84, 88), noise (double at 96), refine_precision (int at 104)
+    assert C.sizeof(L.UpdateParams) == 112
     assert L.UpdateParams.noise.offset == 96
+    assert L.UpdateParams.refine_precision.offset == 104
 
 
 def test_dmrg_params_validation_matches_reference():
