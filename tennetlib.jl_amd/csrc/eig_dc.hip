@@ -577,6 +577,224 @@ __global__ __launch_bounds__(256) void trd_panel_wlast_kernel(int p, int j, int 
   }
 }
 
+// ---- the same column step for large trailing blocks, reading only the lower
+// triangle of the symmetric trailing block (half the bytes of a full pass).
+// The block is cut into SYT x SYT tiles on the global grid (multiples of SYT,
+// so 16-byte row loads stay aligned); a lower tile (I > J) contributes
+// A_IJ v_J to the rows of block I and A_IJ^T v_I to the rows of block J, a
+// diagonal tile (read whole) A_II v_I.  Each contribution goes to its own slot
+// S[b][t] (row block b, partner block t), and the reduction sums the slots in
+// a fixed order, so the result does not depend on scheduling.
+constexpr int SYT = 64;
+
+// tau, alpha and the reflector scale of column j from the sum of squares ss
+// of x_1.. and x_0 (every kernel that needs them computes them this way)
+__device__ __forceinline__ void trd_reflector(double x0, double ss, double& tau, double& alpha, double& scale) {
+  tau = 0.0;
+  alpha = x0;
+  scale = 0.0;
+  if (ss > 0.0) {
+    const double nrm = sqrt(x0 * x0 + ss);
+    alpha = (x0 >= 0.0) ? -nrm : nrm;
+    tau = (alpha - x0) / alpha;
+    scale = 1.0 / (x0 - alpha);
+  }
+}
+
+// sum of squares of x_1.. from the row kernel's partial sums, reduced by the
+// calling wave (every lane gets the same bits; both kernels below use it, so
+// they agree on tau and the reflector scale)
+__device__ __forceinline__ double trd_ss(const double* __restrict__ partA, int nA) {
+  const int lane = threadIdx.x & 63;
+  double s = 0.0;
+  for (int b = lane; b < nA; b += 64) s += partA[(long)b * (1 + 2 * NBT)];
+  return wave_sum64(s);
+}
+
+// Grid: nt (nt + 1) / 2 workgroups, one per lower tile (I >= J) of the tile
+// range [a0, a0 + nt) that covers rows and columns j + 1 .. p - 1.
+// S: nt x nt slots of SYT doubles.  Every wave issues its tile loads first and
+// forms the reflector entries it needs itself (no barrier before the product).
+__global__ __launch_bounds__(256) void trd_symv_tile_kernel(const double* __restrict__ A, long ld, int p, int j,
+                                                            const double* __restrict__ xbuf,
+                                                            const double* __restrict__ partA, int nA, int a0, int nt,
+                                                            double* __restrict__ S) {
+  __shared__ double rs[SYT];
+  __shared__ double cs[8][SYT];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x;
+  int I = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= b) ++I;
+  while (I * (I + 1) / 2 > b) --I;
+  const int J = b - I * (I + 1) / 2;
+  const long r1 = j + 1;  // first row and column of the trailing block
+  const long R0 = (long)(a0 + I) * SYT, C0 = (long)(a0 + J) * SYT;
+  // half-wave h of wave `wave` takes rows wave * 16 + 2 it + h; lane hl the
+  // column pair 2 hl, 2 hl + 1.  Rows go in batches of four: the batch's
+  // loads are issued before the previous batch is reduced.
+  const int hl = lane & 31, h = lane >> 5;
+  const long c = C0 + 2 * hl;
+  const bool ok0 = c >= r1 && c < p, ok1 = c + 1 >= r1 && c + 1 < p;
+  // the tile's columns all inside the block and its rows 16-byte aligned:
+  // one vector load per lane and row
+  const bool tvec = C0 >= r1 && C0 + SYT <= p && (ld & 1) == 0 && ((uintptr_t)A & 15) == 0;
+  const long rb = R0 + wave * 16 + h;  // row of it = 0
+  const double* base = A + rb * ld + c;
+  constexpr int NIT = SYT / 8, NB = 4;
+  double2 a[NB];
+  double xr[NB];
+  auto load = [&](int it0) {
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const long r = rb + 2 * (it0 + u);
+      a[u] = make_double2(0.0, 0.0);
+      xr[u] = 0.0;
+      if (r >= r1 && r < p) {
+        const double* q = base + (long)(2 * (it0 + u)) * ld;
+        if (tvec) {
+          a[u] = *reinterpret_cast<const double2*>(q);
+        } else {
+          if (ok0) a[u].x = q[0];
+          if (ok1) a[u].y = q[1];
+        }
+        if (r > r1) xr[u] = xbuf[r - r1];
+      }
+    }
+  };
+  load(0);
+  const double xc0 = (ok0 && c > r1) ? xbuf[c - r1] : 0.0;
+  const double xc1 = (ok1 && c + 1 > r1) ? xbuf[c + 1 - r1] : 0.0;
+  double tau, alpha, scale;
+  trd_reflector(xbuf[0], trd_ss(partA, nA), tau, alpha, scale);
+  // v at global index g: 1 at r1, x scaled above it, 0 outside the block
+  const double v0 = !ok0 ? 0.0 : (c == r1 ? 1.0 : xc0 * scale);
+  const double v1 = !ok1 ? 0.0 : (c + 1 == r1 ? 1.0 : xc1 * scale);
+  double cs0 = 0.0, cs1 = 0.0;
+#pragma unroll
+  for (int it0 = 0; it0 < NIT; it0 += NB) {
+    double2 b[NB];
+    double xb[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      b[u] = a[u];
+      xb[u] = xr[u];
+    }
+    if (it0 + NB < NIT) load(it0 + NB);
+#pragma unroll
+    for (int u = 0; u < NB; ++u) {
+      const int rl = wave * 16 + 2 * (it0 + u) + h;
+      const long r = R0 + rl;
+      double s = b[u].x * v0 + b[u].y * v1;
+      s += __shfl_xor(s, 16);
+      s += __shfl_xor(s, 8);
+      s += __shfl_xor(s, 4);
+      s += __shfl_xor(s, 2);
+      s += __shfl_xor(s, 1);
+      if (hl == 0) rs[rl] = s;
+      const double w = (r < r1 || r >= p) ? 0.0 : (r == r1 ? 1.0 : xb[u] * scale);
+      cs0 += b[u].x * w;
+      cs1 += b[u].y * w;
+    }
+  }
+  cs[wave * 2 + h][2 * hl] = cs0;
+  cs[wave * 2 + h][2 * hl + 1] = cs1;
+  __syncthreads();
+  if (tid < SYT) {
+    S[((long)I * nt + J) * SYT + tid] = rs[tid];
+  } else if (I != J && tid < 2 * SYT) {
+    const int cl = tid - SYT;
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) t += cs[q][cl];
+    S[((long)J * nt + I) * SYT + cl] = t;
+  }
+}
+
+// Rows of column j's product from the tile slots, the panel corrections, y_j
+// and the partial sums of y_j . v_j (one per workgroup); every workgroup
+// stores v_j for its rows (below the diagonal of column j and in P, Q),
+// workgroup 0 e_j and tau_j.  Eight lanes per row, each summing every eighth
+// slot and correction term, then combined in a fixed order.
+// Grid: ceil(m / 32) workgroups; nA <= 64.
+__global__ __launch_bounds__(256) void trd_symv_reduce_kernel(double* __restrict__ A, long ld, int p, int j, int k,
+                                                              double* __restrict__ P, double* __restrict__ Q,
+                                                              const double* __restrict__ xbuf,
+                                                              const double* __restrict__ partA, int nA, int a0,
+                                                              int nt, const double* __restrict__ S,
+                                                              double* __restrict__ y, double* __restrict__ partB,
+                                                              double* __restrict__ ev, double* __restrict__ tauv) {
+  __shared__ double red[4];
+  __shared__ double sums[2 * NBT + 1];
+  __shared__ double cvw[2 * NBT];  // (v_i . v_j), (w_i . v_j)
+  const int m = p - j - 1, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nq = 1 + 2 * NBT;
+  // sums of (v_i . x) and (w_i . x) over the row kernel's workgroups: wave w
+  // reduces q = 1 + w, 5 + w, ... (one lane per workgroup partial)
+  {
+    double vals[(2 * NBT + 3) / 4];
+#pragma unroll
+    for (int u = 0; u < (2 * NBT + 3) / 4; ++u) {
+      const int q = 1 + wave + 4 * u;
+      vals[u] = (lane < nA && q < nq && (q - 1) % NBT < k) ? partA[(long)lane * nq + q] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < (2 * NBT + 3) / 4; ++u) {
+      const double t = wave_sum64(vals[u]);
+      const int q = 1 + wave + 4 * u;
+      if (lane == 0 && q < nq) sums[q] = t;
+    }
+  }
+  double tau, alpha, scale;
+  trd_reflector(xbuf[0], trd_ss(partA, nA), tau, alpha, scale);
+  __syncthreads();
+  if (tid < k) {
+    const long r1 = j + 1;
+    cvw[tid] = sums[1 + tid] * scale + P[(long)tid * p + r1];
+    cvw[NBT + tid] = sums[1 + NBT + tid] * scale + P[(long)(NBT + tid) * p + r1];
+  }
+  if (blockIdx.x == 0 && tid == 0) {
+    ev[j] = alpha;
+    tauv[j] = tau;
+  }
+  __syncthreads();
+  const int g = tid & 7;
+  const int c = blockIdx.x * 32 + (tid >> 3);
+  const long r = j + 1 + c;
+  double s = 0.0, qv = 0.0, vcur = 0.0;
+  if (c < m) {
+    vcur = (c == 0) ? 1.0 : xbuf[c] * scale;
+    const double* Sb = S + (r / SYT - a0) * nt * SYT + (r % SYT);
+    for (int t0 = 0; t0 < nt; t0 += 128) {
+      double u[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int t = t0 + g + 8 * q;
+        u[q] = t < nt ? Sb[(long)t * SYT] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) s += u[q];
+    }
+    for (int i = g; i < k; i += 8) qv += P[(long)i * p + r] * cvw[NBT + i] + P[(long)(NBT + i) * p + r] * cvw[i];
+  }
+  s += __shfl_xor(s, 4);
+  s += __shfl_xor(s, 2);
+  s += __shfl_xor(s, 1);
+  qv += __shfl_xor(qv, 4);
+  qv += __shfl_xor(qv, 2);
+  qv += __shfl_xor(qv, 1);
+  double dot = 0.0;
+  if (c < m && g == 0) {
+    A[r * ld + j] = vcur;
+    P[(long)k * p + r] = vcur;
+    Q[(long)(NBT + k) * p + r] = vcur;
+    const double yv = tau * (s - qv);
+    y[c] = yv;
+    dot = yv * vcur;
+  }
+  dot = block_sum(dot, red);
+  if (tid == 0) partB[blockIdx.x] = dot;
+}
+
 // ---- the last columns in LDS: the trailing block A[j0:, j0:] (m0 = p - j0
 // <= TAIL rows) is reduced by one workgroup without leaving LDS; reflectors,
 // d, e and tau are written exactly as by the per-column kernels.
@@ -1644,6 +1862,9 @@ void wy_tsolve(tn_ctx* c, const double* S, const double* tauv, int j0, int nb, d
 // faster (measured: p = 2048 takes 45 ms fused against 57 ms in panels,
 // p = 4096 245 ms against 171 ms)
 constexpr int TRD_BLOCKED_MIN = 3072;
+// smallest trailing block whose product reads only the lower triangle (two
+// launches per column instead of one)
+constexpr int TRD_TILE_MIN = 1024;
 
 void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, std::vector<double>& w) {
   TN_REQUIRE(p >= 3 && p <= 16384, TN_E_INVALID, "sym_eig_dc: size out of range");
@@ -1663,12 +1884,14 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
   if (jt >= 1 && P >= TRD_BLOCKED_MIN) {
     const int nAmax = (P + RCHUNK - 1) / RCHUNK;
     const int nBmax = (P + 7) / 8;
+    const long ntmax = (p + SYT - 1) / SYT + 1;
     double* Pp = (double*)c->workspace(WS_DC_PANEL, ((size_t)4 * NBT * p + p + (size_t)nAmax * (2 * NBT + 1) +
-                                                     nBmax + 64) * sizeof(double));
+                                                     nBmax + 64 + (size_t)ntmax * ntmax * SYT) * sizeof(double));
     double* Qp = Pp + (size_t)2 * NBT * p;
     double* xb = Qp + (size_t)2 * NBT * p;
     double* partA = xb + p;
     double* partB = partA + (size_t)nAmax * (2 * NBT + 1);
+    double* Sy = partB + nBmax + 64;  // tile slots of the lower-triangle product
     raise_lds_limit((const void*)trd_panel_symv_kernel, (size_t)p * sizeof(double));
     for (int j0 = 0; j0 < jt; j0 += NBT) {
       const int j1 = std::min(jt, j0 + NBT);
@@ -1680,14 +1903,28 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
         hipLaunchKernelGGL(trd_panel_row_kernel, dim3(nA), dim3(256), 0, c->stream, (const double*)G, ldg, P, j, k,
                            Pp, Qp, (const double*)y, (const double*)partB, nBprev, (const double*)tauv, xb, partA,
                            dv);
-        const int nB = (m + 7) / 8;  // two rows per wave
-        {
+        if (m >= TRD_TILE_MIN) {
+          // lower-triangle tiles, then the fixed-order reduction
+          const int a0 = (j + 1) / SYT;
+          const int nt = (P + SYT - 1) / SYT - a0;
+          const int nB = (m + 31) / 32;  // eight lanes per row
+          TN_REQUIRE(nA <= 64 && nB <= nBmax && nt <= ntmax, TN_E_INVALID, "tridiagonalisation: tile plan");
+          PhaseTimer pt(c, PH_TRD_SYMV);
+          if (c->timing) c->phase_bytes[PH_TRD_SYMV] += 4.0 * (double)m * (double)(m + SYT);  // lower half of G22
+          hipLaunchKernelGGL(trd_symv_tile_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, c->stream,
+                             (const double*)G, ldg, P, j, (const double*)xb, (const double*)partA, nA, a0, nt, Sy);
+          hipLaunchKernelGGL(trd_symv_reduce_kernel, dim3(nB), dim3(256), 0, c->stream, G, ldg, P, j, k, Pp, Qp,
+                             (const double*)xb, (const double*)partA, nA, a0, nt, (const double*)Sy, y, partB, ev,
+                             tauv);
+          nBprev = nB;
+        } else {
+          const int nB = (m + 7) / 8;  // two rows per wave
           PhaseTimer pt(c, PH_TRD_SYMV);
           if (c->timing) c->phase_bytes[PH_TRD_SYMV] += 8.0 * (double)m * (double)m;  // one read of G22
           hipLaunchKernelGGL(trd_panel_symv_kernel, dim3(nB), dim3(256), (size_t)m * sizeof(double), c->stream, G,
                              ldg, P, j, k, Pp, Qp, (const double*)xb, (const double*)partA, nA, y, partB, ev, tauv);
+          nBprev = nB;
         }
-        nBprev = nB;
       }
       const int jl = j1 - 1, ml = P - jl - 1;
       hipLaunchKernelGGL(trd_panel_wlast_kernel, dim3((unsigned)std::min(64, (ml + 255) / 256)), dim3(256), 0,

@@ -344,7 +344,9 @@ def _spectrum(case, p, rng):
                                       ("uniform", 65, 80), ("graded", 130, 97),
                                       # panel-blocked tridiagonalisation (p >= 1024), incl. a partial last panel
                                       ("multiplets", 1500, 1601), ("lowrank", 2200, 2100), ("graded", 2100, 2200),
-                                      ("uniform", 4096, 4096), ("graded", 8192, 8192)])
+                                      ("uniform", 4096, 4096), ("graded", 8192, 8192),
+                                      # odd Gram size: rows of the lower-triangle product not 16-byte aligned
+                                      ("multiplets", 3101, 3073)])
 def test_density_eigensolver_spectra(T, case, m, n):
     """Divide-and-conquer range (Gram size > 64): weights to eps * sum(P),
     orthonormal Gram-side factor, exact reconstruction, for degenerate,
