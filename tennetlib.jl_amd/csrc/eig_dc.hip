@@ -366,6 +366,9 @@ constexpr int RCHUNK = 256;  // row entries per workgroup in trd_panel_row_kerne
 // partial sums of y_{j-1}^T v_{j-1}), then the current row x (stored in xbuf,
 // entries c = 0..m-1 for columns j+1+c) with partial sums of sum_{c>=1} x_c^2,
 // v_i . x and w_i . x over c >= 1 (i < k); d_j from workgroup 0.
+// One entry per thread: its panel entries are loaded first, and the values
+// every thread shares (y_{j-1}^T v_{j-1}, v_i[j], w_i[j]) are read by each
+// wave itself, so nothing waits on a barrier before the row is formed.
 // Grid: ceil(m / RCHUNK) workgroups.
 __global__ __launch_bounds__(256) void trd_panel_row_kernel(const double* __restrict__ A, long ld, int p, int j, int k,
                                                             double* __restrict__ P, double* __restrict__ Q,
@@ -374,61 +377,66 @@ __global__ __launch_bounds__(256) void trd_panel_row_kernel(const double* __rest
                                                             const double* __restrict__ tauv,
                                                             double* __restrict__ xbuf, double* __restrict__ partA,
                                                             double* __restrict__ dv) {
-  __shared__ double sv[NBT], sw[NBT];  // v_i[j], w_i[j]
-  __shared__ double red[4];
   __shared__ double accs[4][2 * NBT + 1];
   const int m = p - j - 1, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // w_{k-1} = y_{j-1} - hs v_{k-1} over rows >= j
+  const int c = blockIdx.x * RCHUNK + tid;
+  const bool act = c < m;
+  const long r = j + 1 + c;  // global row / column index
+  // this entry's loads
+  double V[NBT], W[NBT];
+#pragma unroll
+  for (int i = 0; i < NBT; ++i) {
+    V[i] = (act && i < k) ? P[(long)i * p + r] : 0.0;
+    W[i] = (act && i < k - 1) ? P[(long)(NBT + i) * p + r] : 0.0;
+  }
+  const double a = act ? A[(long)j * ld + r] : 0.0;
+  const double yp = (act && k >= 1) ? yprev[c + 1] : 0.0;
+  // hs = (tau_{j-1} / 2) y_{j-1}^T v_{j-1}, reduced by every wave alike
   double hs = 0.0;
   if (k >= 1) {
     double s = 0.0;
-    for (int i = tid; i < nB; i += 256) s += partB[i];
-    s = block_sum(s, red);
-    hs = 0.5 * tauv[j - 1] * s;
+    for (int i = lane; i < nB; i += 64) s += partB[i];
+    hs = 0.5 * tauv[j - 1] * wave_sum64(s);
   }
-  if (tid < k) {
-    sv[tid] = P[(long)tid * p + j];
-    if (tid < k - 1) sw[tid] = P[(long)(NBT + tid) * p + j];
-  }
-  if (k >= 1 && tid == 0) {
-    const double wj = yprev[0] - hs * P[(long)(k - 1) * p + j];
-    sw[k - 1] = wj;
-    if (blockIdx.x == 0) {
+  // w_{k-1} = y_{j-1} - hs v_{k-1} over rows >= j
+  double wj = 0.0;
+  if (k >= 1) {
+    wj = yprev[0] - hs * P[(long)(k - 1) * p + j];
+    double vk1 = 0.0;  // V[k - 1], selected without a runtime index (keeps V in registers)
+#pragma unroll
+    for (int i = 0; i < NBT; ++i)
+      if (i == k - 1) vk1 = V[i];
+    const double wn = yp - hs * vk1;
+#pragma unroll
+    for (int i = 0; i < NBT; ++i)
+      if (i == k - 1) W[i] = wn;
+    if (act) {
+      P[(long)(NBT + k - 1) * p + r] = wn;
+      Q[(long)(k - 1) * p + r] = wn;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
       P[(long)(NBT + k - 1) * p + j] = wj;
       Q[(long)(k - 1) * p + j] = wj;
     }
   }
-  __syncthreads();
-  double acc[2 * NBT + 1];
+  // x = A[j, r] - sum_i (v_i[j] w_i[r] + w_i[j] v_i[r])
+  double x = a;
 #pragma unroll
-  for (int q = 0; q < 2 * NBT + 1; ++q) acc[q] = 0.0;
-  const int c0 = blockIdx.x * RCHUNK;
-  for (int c = c0 + tid; c < min(m, c0 + RCHUNK); c += 256) {
-    const long r = j + 1 + c;  // global row / column index
-    if (k >= 1) {
-      const double wn = yprev[c + 1] - hs * P[(long)(k - 1) * p + r];
-      P[(long)(NBT + k - 1) * p + r] = wn;
-      Q[(long)(k - 1) * p + r] = wn;
-    }
-    double x = A[(long)j * ld + r];
-    for (int i = 0; i < k; ++i) x -= sv[i] * P[(long)(NBT + i) * p + r] + sw[i] * P[(long)i * p + r];
-    xbuf[c] = x;
-    if (c >= 1) {
-      acc[0] += x * x;
-#pragma unroll
-      for (int i = 0; i < NBT; ++i) {
-        if (i < k) {
-          acc[1 + i] += P[(long)i * p + r] * x;
-          acc[1 + NBT + i] += P[(long)(NBT + i) * p + r] * x;
-        }
-      }
+  for (int i = 0; i < NBT; ++i) {
+    if (i < k) {
+      const double svi = P[(long)i * p + j];
+      const double swi = (i == k - 1) ? wj : P[(long)(NBT + i) * p + j];
+      x -= svi * W[i] + swi * V[i];
     }
   }
+  if (act) xbuf[c] = x;
+  const double xs = (act && c >= 1) ? x : 0.0;
   const int nq = 1 + 2 * NBT;
 #pragma unroll
   for (int q = 0; q < nq; ++q) {
     if (q == 0 || (q - 1) % NBT < k) {
-      const double t = wave_sum64(acc[q]);
+      const double f = q == 0 ? xs * xs : (q <= NBT ? V[q - 1] * xs : W[q - 1 - NBT] * xs);
+      const double t = wave_sum64(f);
       if (lane == 0) accs[wave][q] = t;
     }
   }
@@ -438,7 +446,10 @@ __global__ __launch_bounds__(256) void trd_panel_row_kernel(const double* __rest
       partA[(long)blockIdx.x * nq + q] = (accs[0][q] + accs[1][q]) + (accs[2][q] + accs[3][q]);
   if (blockIdx.x == 0 && tid == 0) {
     double dj = A[(long)j * ld + j];
-    for (int i = 0; i < k; ++i) dj -= 2.0 * sv[i] * sw[i];
+    for (int i = 0; i < k; ++i) {
+      const double swi = (i == k - 1) ? wj : P[(long)(NBT + i) * p + j];
+      dj -= 2.0 * P[(long)i * p + j] * swi;
+    }
     dv[j] = dj;
   }
 }
@@ -728,19 +739,39 @@ __global__ __launch_bounds__(256) void trd_symv_reduce_kernel(double* __restrict
   __shared__ double cvw[2 * NBT];  // (v_i . v_j), (w_i . v_j)
   const int m = p - j - 1, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nq = 1 + 2 * NBT;
+  const int g = tid & 7;
+  const int c = blockIdx.x * 32 + (tid >> 3);
+  const long r = j + 1 + c;
+  // loads that need no reduction result go first: this row's tile slots (the
+  // first 128 partners), its panel entries and x
+  constexpr int NPQ = NBT / 8;
+  double u[16], pv[NPQ], pw[NPQ], xc = 0.0;
+  const double* Sb = S + (r / SYT - a0) * nt * SYT + (r % SYT);
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int t = g + 8 * q;
+    u[q] = (c < m && t < nt) ? Sb[(long)t * SYT] : 0.0;
+  }
+#pragma unroll
+  for (int q = 0; q < NPQ; ++q) {
+    const int i = g + 8 * q;
+    pv[q] = (c < m && i < k) ? P[(long)i * p + r] : 0.0;
+    pw[q] = (c < m && i < k) ? P[(long)(NBT + i) * p + r] : 0.0;
+  }
+  if (c < m && c > 0) xc = xbuf[c];
   // sums of (v_i . x) and (w_i . x) over the row kernel's workgroups: wave w
   // reduces q = 1 + w, 5 + w, ... (one lane per workgroup partial)
   {
     double vals[(2 * NBT + 3) / 4];
 #pragma unroll
-    for (int u = 0; u < (2 * NBT + 3) / 4; ++u) {
-      const int q = 1 + wave + 4 * u;
-      vals[u] = (lane < nA && q < nq && (q - 1) % NBT < k) ? partA[(long)lane * nq + q] : 0.0;
+    for (int uu = 0; uu < (2 * NBT + 3) / 4; ++uu) {
+      const int q = 1 + wave + 4 * uu;
+      vals[uu] = (lane < nA && q < nq && (q - 1) % NBT < k) ? partA[(long)lane * nq + q] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < (2 * NBT + 3) / 4; ++u) {
-      const double t = wave_sum64(vals[u]);
-      const int q = 1 + wave + 4 * u;
+    for (int uu = 0; uu < (2 * NBT + 3) / 4; ++uu) {
+      const double t = wave_sum64(vals[uu]);
+      const int q = 1 + wave + 4 * uu;
       if (lane == 0 && q < nq) sums[q] = t;
     }
   }
@@ -757,15 +788,12 @@ __global__ __launch_bounds__(256) void trd_symv_reduce_kernel(double* __restrict
     tauv[j] = tau;
   }
   __syncthreads();
-  const int g = tid & 7;
-  const int c = blockIdx.x * 32 + (tid >> 3);
-  const long r = j + 1 + c;
   double s = 0.0, qv = 0.0, vcur = 0.0;
   if (c < m) {
-    vcur = (c == 0) ? 1.0 : xbuf[c] * scale;
-    const double* Sb = S + (r / SYT - a0) * nt * SYT + (r % SYT);
-    for (int t0 = 0; t0 < nt; t0 += 128) {
-      double u[16];
+    vcur = (c == 0) ? 1.0 : xc * scale;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += u[q];
+    for (int t0 = 128; t0 < nt; t0 += 128) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) {
         const int t = t0 + g + 8 * q;
@@ -774,7 +802,11 @@ __global__ __launch_bounds__(256) void trd_symv_reduce_kernel(double* __restrict
 #pragma unroll
       for (int q = 0; q < 16; ++q) s += u[q];
     }
-    for (int i = g; i < k; i += 8) qv += P[(long)i * p + r] * cvw[NBT + i] + P[(long)(NBT + i) * p + r] * cvw[i];
+#pragma unroll
+    for (int q = 0; q < NPQ; ++q) {
+      const int i = g + 8 * q;
+      if (i < k) qv += pv[q] * cvw[NBT + i] + pw[q] * cvw[i];
+    }
   }
   s += __shfl_xor(s, 4);
   s += __shfl_xor(s, 2);
