@@ -517,6 +517,8 @@ def main():
     ap.add_argument("--krylovdim", type=int, default=5)
     ap.add_argument("--maxiter", type=int, default=2)
     ap.add_argument("--fp64-steps", type=int, default=6, help="timed bulk updates in the FP64 form")
+    ap.add_argument("--bf16x3-steps", type=int, default=6,
+                    help="timed bulk updates in the BF16x3 form with FP32 refinement (0: skip the line)")
     ap.add_argument("--n26-sweeps", type=int, default=1, help="timed N=26 sweeps per form (0: skip the line)")
     ap.add_argument("--mv-chi", type=int, default=4096)
     ap.add_argument("--mv-iters", type=int, default=5)
@@ -598,6 +600,19 @@ def main():
         log(f"bulk updates chi={chi} fp64: {fp64['s_per_update']:.3f} s/update -> {sweep64:.1f} s/sweep; "
             f"dims {[(u['chil'], u['chir'], u['nkept'], u['numops']) for u in fp64['timed']]}; "
             f"warmup phases {fp64['warm_phase_ms']}")
+    # the BF16x3 form (three bf16 MFMA products per FP32 product, Lanczos
+    # refinement of dimension 2 with FP32 products) at the same positions
+    bf = None
+    if args.bf16x3_steps > 0 and args.prec == "fp32":
+        bfr = bulk_updates(T, chi, "bf16x3", 1, args.bf16x3_steps, kd=kd, mi=mi, refine_precision="fp32")
+        bscale = sweep_scale(chi, bfr["timed"])
+        bf = {"s_per_update": round(bfr["s_per_update"], 4), "updates_timed": args.bf16x3_steps,
+              "sweep_s": round(bfr["s_per_update"] * bscale, 3),
+              "speedup_vs_fp64": round(sweep64 / (bfr["s_per_update"] * bscale), 3) if sweep64 else None,
+              "refine": "Lanczos of dimension 2 from the BF16x3 Ritz vector, FP32 products (the last update "
+                        "of a sweep refines in FP64)"}
+        log(f"bulk updates chi={chi} bf16x3 (fp32 refinement): {bfr['s_per_update']:.3f} s/update -> "
+            f"{bf['sweep_s']:.1f} s/sweep; numops {[u['numops'] for u in bfr['timed']]}")
     # dominant kernel of the updates: the matvec GEMM launches of the timed
     # updates (HIP events on the library stream around each launch)
     flops = sum(gemm_flops(u) for u in mixed["timed"])
@@ -655,6 +670,7 @@ def main():
                       "sweep_s": round(sweep64, 3), "warm_phase_ms": fp64["warm_phase_ms"]} if fp64 else None),
         "mixed_warm_phase_ms": mixed["warm_phase_ms"],
         "speedup_vs_fp64": round(sweep64 / value, 3) if sweep64 else None,
+        "bf16x3_fp32_refine": bf,
         "n26_full_sweeps_chi4096": n26,
         "matvec_chi4096": ({k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in mv.items()} if mv else None),
         "matvec_pct_fp32_peak": ({k: round(100 * v["matvec_tflops"] / PEAK["fp32"], 2) for k, v in mv.items()}

@@ -389,7 +389,7 @@ __global__ __launch_bounds__(256) void trd_panel_row_kernel(const double* __rest
     V[i] = (act && i < k) ? P[(long)i * p + r] : 0.0;
     W[i] = (act && i < k - 1) ? P[(long)(NBT + i) * p + r] : 0.0;
   }
-  const double a = act ? A[(long)j * ld + r] : 0.0;
+  const double a = act ? A[r * ld + j] : 0.0;  // column j below the diagonal (the lower triangle is current)
   const double yp = (act && k >= 1) ? yprev[c + 1] : 0.0;
   // hs = (tau_{j-1} / 2) y_{j-1}^T v_{j-1}, reduced by every wave alike
   double hs = 0.0;
@@ -825,6 +825,97 @@ __global__ __launch_bounds__(256) void trd_symv_reduce_kernel(double* __restrict
   }
   dot = block_sum(dot, red);
   if (tid == 0) partB[blockIdx.x] = dot;
+}
+
+// ---- the panel's rank-64 update of the trailing block, lower triangle only:
+// A[r, c] -= sum_i (v_i[r] w_i[c] + w_i[r] v_i[c]) for the 64 x 64 tiles on
+// the global grid with I >= J (diagonal tiles whole), rows and columns from
+// j1 on.  P = [V; W] and Q = [W; V] (2 NBT rows of length p), so the sum is
+// P^T Q over K = 2 NBT.  One workgroup per tile: both K x 64 panels in LDS,
+// FP64 MFMA 16x16x4 (each wave a 32 x 32 quarter), then one read-modify-write
+// of the tile.  Grid: nt (nt + 1) / 2 tiles of the range [a0, a0 + nt).
+typedef double trd_d4 __attribute__((ext_vector_type(4)));
+constexpr int SYK = 2 * NBT;
+__global__ __launch_bounds__(256) void trd_syr2k_lower_kernel(double* __restrict__ A, long ld, int p, int j1,
+                                                              const double* __restrict__ P,
+                                                              const double* __restrict__ Q, int a0) {
+  constexpr int KH = SYK / 2;  // the K range in two halves (33 KB of LDS, four workgroups per CU)
+  __shared__ double Pr[KH][SYT + 1];
+  __shared__ double Qc[KH][SYT + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x;
+  int I = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= b) ++I;
+  while (I * (I + 1) / 2 > b) --I;
+  const int J = b - I * (I + 1) / 2;
+  const long R0 = (long)(a0 + I) * SYT, C0 = (long)(a0 + J) * SYT;
+  const int wm = wave >> 1, wn = wave & 1;
+  trd_d4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj) acc[i][jj] = trd_d4{0.0, 0.0, 0.0, 0.0};
+  for (int h0 = 0; h0 < SYK; h0 += KH) {
+    // panels: Pr[kk][x] = P[h0 + kk][R0 + x], Qc[kk][y] = Q[h0 + kk][C0 + y] (zero outside [j1, p))
+    if (h0 > 0) __syncthreads();
+    for (int e = tid; e < KH * SYT; e += 256) {
+      const int kk = e / SYT, x = e % SYT;
+      const long r = R0 + x, c = C0 + x;
+      Pr[kk][x] = (r >= j1 && r < p) ? P[(long)(h0 + kk) * p + r] : 0.0;
+      Qc[kk][x] = (c >= j1 && c < p) ? Q[(long)(h0 + kk) * p + c] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KH; kk += 4) {
+      const int kr = kk + (lane >> 4);
+      double fa[2], fb[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) fa[i] = Pr[kr][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int jj = 0; jj < 2; ++jj) fb[jj] = Qc[kr][wn * 32 + jj * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int jj = 0; jj < 2; ++jj)
+          acc[i][jj] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[jj], acc[i][jj], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const long r = R0 + wm * 32 + i * 16 + (lane >> 4) + 4 * q;
+        const long c = C0 + wn * 32 + jj * 16 + (lane & 15);
+        if (r >= j1 && r < p && c >= j1 && c < p) A[r * ld + c] -= acc[i][jj][q];
+      }
+}
+
+// A[c, r] = A[r, c] for j0 <= c < r < p: the upper triangle of the trailing
+// block rebuilt from the lower one, once, before kernels that read whole rows
+// take over.  Grid: nt (nt + 1) / 2 tiles; diagonal tiles mirror in place.
+__global__ __launch_bounds__(256) void trd_mirror_lower_kernel(double* __restrict__ A, long ld, int p, int j0,
+                                                               int a0) {
+  __shared__ double T[SYT][SYT + 1];
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x;
+  int I = (int)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= b) ++I;
+  while (I * (I + 1) / 2 > b) --I;
+  const int J = b - I * (I + 1) / 2;
+  const long R0 = (long)(a0 + I) * SYT, C0 = (long)(a0 + J) * SYT;
+  for (int e = tid; e < SYT * SYT; e += 256) {
+    const int x = e / SYT, y = e % SYT;
+    const long r = R0 + x, c = C0 + y;
+    T[x][y] = (r >= j0 && r < p && c >= j0 && c < p) ? A[r * ld + c] : 0.0;
+  }
+  __syncthreads();
+  for (int e = tid; e < SYT * SYT; e += 256) {
+    const int y = e / SYT, x = e % SYT;  // write row C0 + y, column R0 + x
+    const long r = R0 + x, c = C0 + y;
+    if (r >= j0 && r < p && c >= j0 && c < p && r > c) A[c * ld + r] = T[x][y];
+  }
 }
 
 // ---- the last columns in LDS: the trailing block A[j0:, j0:] (m0 = p - j0
@@ -1925,6 +2016,9 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     double* partB = partA + (size_t)nAmax * (2 * NBT + 1);
     double* Sy = partB + nBmax + 64;  // tile slots of the lower-triangle product
     raise_lds_limit((const void*)trd_panel_symv_kernel, (size_t)p * sizeof(double));
+    // panel updates touch only the lower triangle while the next panel's
+    // columns all take the lower-triangle product
+    bool lower_only = P - NBT >= TRD_TILE_MIN;
     for (int j0 = 0; j0 < jt; j0 += NBT) {
       const int j1 = std::min(jt, j0 + NBT);
       fill_f64(c, Pp, 0.0, (long)4 * NBT * p);
@@ -1964,7 +2058,22 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
                          (const double*)tauv);
       TN_HIP(hipGetLastError());
       const long m1 = p - j1;
-      if (m1 > 0) {  // G[j1:, j1:] -= V W^T + W V^T (K = 2 NBT, unused rows of P, Q zero)
+      if (m1 > 0 && lower_only) {
+        // G[j1:, j1:] -= V W^T + W V^T on the lower tiles; while every column
+        // of the next panel reads only the lower triangle that is all that is
+        // needed, and before the first one that reads whole rows (or the LDS
+        // tail) the upper triangle is rebuilt from it once
+        const int a0 = j1 / SYT;
+        const int nt = (P + SYT - 1) / SYT - a0;
+        hipLaunchKernelGGL(trd_syr2k_lower_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, c->stream, G,
+                           ldg, P, j1, (const double*)Pp, (const double*)Qp, a0);
+        if (P - j1 - NBT < TRD_TILE_MIN) {
+          hipLaunchKernelGGL(trd_mirror_lower_kernel, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, c->stream,
+                             G, ldg, P, j1, a0);
+          lower_only = false;
+        }
+        TN_HIP(hipGetLastError());
+      } else if (m1 > 0) {  // G[j1:, j1:] -= V W^T + W V^T (K = 2 NBT, unused rows of P, Q zero)
         GemmDesc g;
         g.M = m1; g.N = m1; g.K = 2 * NBT;
         g.A = Pp + j1; g.lda = p; g.tA = 1;
