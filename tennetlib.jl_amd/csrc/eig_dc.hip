@@ -361,6 +361,8 @@ __global__ __launch_bounds__(256) void trd_step_kernel(double* __restrict__ A, l
 // P = [V; W] and Q = [W; V] (2 NBT rows of length p; unused rows zero).
 constexpr int NBT = 32;
 constexpr int RCHUNK = 256;  // row entries per workgroup in trd_panel_row_kernel (one per thread)
+// dynamic LDS of trd_panel_row_kernel: four waves' products, (2 NBT + 1) rows of 65
+constexpr size_t TRD_ROW_LDS = (size_t)4 * (2 * NBT + 1) * 65 * sizeof(double);
 
 // Column j: forms w_{k-1} (the previous column's, from y_{j-1} and the
 // partial sums of y_{j-1}^T v_{j-1}), then the current row x (stored in xbuf,
@@ -378,6 +380,7 @@ __global__ __launch_bounds__(256) void trd_panel_row_kernel(const double* __rest
                                                             double* __restrict__ xbuf, double* __restrict__ partA,
                                                             double* __restrict__ dv) {
   __shared__ double accs[4][2 * NBT + 1];
+  extern __shared__ double tr[];  // per wave (2 NBT + 1) x 65: the products, transposed for the sums
   const int m = p - j - 1, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int c = blockIdx.x * RCHUNK + tid;
   const bool act = c < m;
@@ -432,12 +435,30 @@ __global__ __launch_bounds__(256) void trd_panel_row_kernel(const double* __rest
   if (act) xbuf[c] = x;
   const double xs = (act && c >= 1) ? x : 0.0;
   const int nq = 1 + 2 * NBT;
+  // the wave's 64 x nq products go to LDS as rows q of 65 (padded), and lane
+  // q sums row q (lane 0 also row 64): one LDS pass instead of nq cross-lane
+  // reductions
+  double* T = tr + (long)wave * nq * 65;
 #pragma unroll
   for (int q = 0; q < nq; ++q) {
     if (q == 0 || (q - 1) % NBT < k) {
       const double f = q == 0 ? xs * xs : (q <= NBT ? V[q - 1] * xs : W[q - 1 - NBT] * xs);
-      const double t = wave_sum64(f);
-      if (lane == 0) accs[wave][q] = t;
+      T[q * 65 + lane] = f;
+    }
+  }
+  __syncthreads();
+  for (int q = lane; q < nq; q += 64) {
+    if (q == 0 || (q - 1) % NBT < k) {
+      const double* row = T + q * 65;
+      double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll
+      for (int t = 0; t < 64; t += 4) {
+        s0 += row[t];
+        s1 += row[t + 1];
+        s2 += row[t + 2];
+        s3 += row[t + 3];
+      }
+      accs[wave][q] = (s0 + s1) + (s2 + s3);
     }
   }
   __syncthreads();
@@ -2016,6 +2037,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
     double* partB = partA + (size_t)nAmax * (2 * NBT + 1);
     double* Sy = partB + nBmax + 64;  // tile slots of the lower-triangle product
     raise_lds_limit((const void*)trd_panel_symv_kernel, (size_t)p * sizeof(double));
+    raise_lds_limit((const void*)trd_panel_row_kernel, TRD_ROW_LDS);
     // panel updates touch only the lower triangle while the next panel's
     // columns all take the lower-triangle product
     bool lower_only = P - NBT >= TRD_TILE_MIN;
@@ -2026,7 +2048,7 @@ void sym_eig_dc(tn_ctx* c, double* G, long p, long ldg, double* Vt, long ldv, st
       for (int j = j0; j < j1; ++j) {
         const int k = j - j0, m = P - j - 1;
         const int nA = (m + RCHUNK - 1) / RCHUNK;
-        hipLaunchKernelGGL(trd_panel_row_kernel, dim3(nA), dim3(256), 0, c->stream, (const double*)G, ldg, P, j, k,
+        hipLaunchKernelGGL(trd_panel_row_kernel, dim3(nA), dim3(256), TRD_ROW_LDS, c->stream, (const double*)G, ldg, P, j, k,
                            Pp, Qp, (const double*)y, (const double*)partB, nBprev, (const double*)tauv, xb, partA,
                            dv);
         if (m >= TRD_TILE_MIN) {
