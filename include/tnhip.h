@@ -268,7 +268,7 @@ int tn_env_krylov_extend(tn_env* env, int krylovdim, long applyH_maxdim, double 
 int tn_fullsweep(tn_env* env, const tn_update_params* p, double* energy, double* maxtruncerr,
                  long* maxchi, double* mid_eigs, long cap, long* nmid);
 
-/* ---- tree tensor networks (StateEnvsTTN, src/ttn/*.jl) ------------------ */
+/* ---- tree tensor networks (StateEnvsTTN, the src/ttn/ files) ------------ */
 /* A tn_ttn holds the node tensors (three legs each, row-major A[a0][a1][a2],
  * FP64) and the environments of a TTN on the device.  The Hamiltonian enters
  * as a tree operator: every link carries `nchan` channels, every site leg the
