@@ -250,7 +250,7 @@ def random_product(N, seed):
     return out
 
 
-def run_sweeps(T, env, sw, n, chi, prec, cutoff, kd, mi, timing=False):
+def run_sweeps(T, env, sw, n, chi, prec, cutoff, kd, mi, timing=False, refine_precision="fp64"):
     """n fullsweep! calls on env; returns wall seconds per sweep (device
     synchronised on both sides) and the per-phase device times."""
     ctx = env.ctx
@@ -260,7 +260,7 @@ def run_sweeps(T, env, sw, n, chi, prec, cutoff, kd, mi, timing=False):
     t0 = time.perf_counter()
     for _ in range(n):
         T.fullsweep_(env, T.eig_solver, 2, sw, maxdim=chi, cutoff=cutoff, solver_krylovdim=kd,
-                     solver_maxiter=mi, precision=prec, outputlevel=0)
+                     solver_maxiter=mi, precision=prec, refine_precision=refine_precision, outputlevel=0)
     ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
     dt = (time.perf_counter() - t0) / max(n, 1)
     tm = ctx.timers(reset=False) if timing else None
@@ -369,16 +369,21 @@ def hubbard_qn_run(T, N, ramp, cutoff, kd, mi, prec):
             f"phases={ {k: round(v) for k, v in tm.items()} }")
     out = {"workload": f"1D Hubbard N={N}, t=1, U=4, (N_up, N_dn)=({N // 2}, {N // 2}) U(1)xU(1), two-site DMRG "
                        f"ramp maxdim {ramp} (one sweep each, cutoff {cutoff:g}, krylovdim {kd}, maxiter {mi}), "
-                       f"then one sweep at maxdim {ramp[-1]} per form",
+                       f"then one sweep at maxdim {ramp[-1]} per form (bf16x3: Lanczos refinement of "
+                       f"dimension 2 with FP32 products, the last update in FP64)",
            "ramp_s_per_sweep": ramp_s}
-    for form in ("fp64", prec):
-        dt, tm = run_sweeps(T, env, sw, 1, ramp[-1], form, cutoff, kd, mi, timing=True)
+    forms = ["fp64", prec] + (["bf16x3"] if prec != "bf16x3" else [])
+    for form in forms:
+        dt, tm = run_sweeps(T, env, sw, 1, ramp[-1], form, cutoff, kd, mi, timing=True,
+                            refine_precision="fp32" if form == "bf16x3" else "fp64")
         mid = env.sectors(N // 2)
         out[form] = {"s_per_sweep": round(dt, 3), "energy": sw.energy[-1], "maxchi": sw.maxchi[-1],
                      "maxtruncerr": sw.maxtruncerr[-1], "phase_ms": {k: round(v, 1) for k, v in tm.items()},
                      "mid_bond_sectors": len(mid), "mid_bond_largest_sector": max(mid.values())}
         log(f"hubbard qn N={N} {form}: {dt:.2f} s/sweep E={sw.energy[-1]:.12f} maxchi={sw.maxchi[-1]}")
     out["speedup_vs_fp64"] = round(out["fp64"]["s_per_sweep"] / out[prec]["s_per_sweep"], 3)
+    if "bf16x3" in out:
+        out["bf16x3_speedup_vs_fp64"] = round(out["fp64"]["s_per_sweep"] / out["bf16x3"]["s_per_sweep"], 3)
     out["total_s"] = round(time.perf_counter() - t_all, 1)
     env.close()
     return out

@@ -644,10 +644,14 @@ struct BsArgs {
   double alpha;
 };
 
-template <typename T, int BM, int BN, int BK>
+// SPLIT (FP32 operands only): each FP32 fragment is split into bf16 hi/lo
+// parts in registers and the product accumulates lo*hi + hi*lo + hi*hi with
+// the bf16 MFMA 32x32x16 (BK = 16: one step per staged K chunk).
+template <typename T, int BM, int BN, int BK, bool SPLIT = false>
 __global__ __launch_bounds__(256) void gemm_bs_kernel(BsArgs a) {
   using TT = RealTile<T, BM, BN, BK>;
   constexpr bool F64 = sizeof(T) == 8;
+  static_assert(!SPLIT || (!F64 && BK == 16), "the split product takes FP32 operands in K chunks of 16");
   constexpr int LA = BM + TT::PAD, LB = BN + TT::PAD;
   constexpr int WTM = BM / 2, WTN = BN / 2;
   constexpr int MB = F64 ? 16 : 32;  // MFMA block edge
@@ -699,6 +703,42 @@ __global__ __launch_bounds__(256) void gemm_bs_kernel(BsArgs a) {
             for (int j = 0; j < FN; ++j)
               acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[i], fb[j], acc[i][j], 0, 0, 0);
         }
+      } else if constexpr (SPLIT) {
+        // lane: row / column lane % 32, the 8 k values 8 (lane / 32) .. + 7
+        const int kb = 8 * (lane >> 5);
+        bf8 ah[FM], al[FM], bh[FN], bl[FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+          us8 h, l;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const float x = As[(kb + q) * LA + wm * WTM + i * 32 + (lane & 31)];
+            h[q] = f2bf_rne(x);
+            l[q] = f2bf_rne(x - bf2f(h[q]));
+          }
+          ah[i] = __builtin_bit_cast(bf8, h);
+          al[i] = __builtin_bit_cast(bf8, l);
+        }
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+          us8 h, l;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const float x = Bs[(kb + q) * LB + wn * WTN + j * 32 + (lane & 31)];
+            h[q] = f2bf_rne(x);
+            l[q] = f2bf_rne(x - bf2f(h[q]));
+          }
+          bh[j] = __builtin_bit_cast(bf8, h);
+          bl[j] = __builtin_bit_cast(bf8, l);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+          for (int j = 0; j < FN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          }
       } else {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 2) {
@@ -927,7 +967,8 @@ void gemm_grouped_f64(tn_ctx* c, const GroupedDesc& d) {
 
 void gemm_blocksparse(tn_ctx* c, Prec p, const BsDesc& d) {
   if (d.ntiles <= 0) return;
-  TN_REQUIRE(p == Prec::F64 || p == Prec::F32, TN_E_INVALID, "block-sparse gemm: FP64 or FP32 only");
+  TN_REQUIRE(p == Prec::F64 || p == Prec::F32 || p == Prec::BF16X3, TN_E_INVALID,
+             "block-sparse gemm: FP64, FP32 or BF16x3 (FP32 operands) only");
   TN_REQUIRE(d.ntiles % 8 == 0 && d.ntiles < (1L << 31), TN_E_INVALID, "block-sparse gemm: bad tile list");
   BsArgs a;
   a.A = d.A; a.lda = d.lda;
@@ -939,8 +980,11 @@ void gemm_blocksparse(tn_ctx* c, Prec p, const BsDesc& d) {
   if (p == Prec::F64)
     hipLaunchKernelGGL((gemm_bs_kernel<double, BS_TILE, BS_TILE, 16>), dim3((unsigned)d.ntiles), dim3(256), 0,
                        c->stream, a);
-  else
+  else if (p == Prec::F32)
     hipLaunchKernelGGL((gemm_bs_kernel<float, BS_TILE, BS_TILE, 16>), dim3((unsigned)d.ntiles), dim3(256), 0,
+                       c->stream, a);
+  else
+    hipLaunchKernelGGL((gemm_bs_kernel<float, BS_TILE, BS_TILE, 16, true>), dim3((unsigned)d.ntiles), dim3(256), 0,
                        c->stream, a);
   TN_HIP(hipGetLastError());
 }

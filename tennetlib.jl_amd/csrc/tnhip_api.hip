@@ -886,8 +886,11 @@ const QnMvPlan* qn_plan(tn_env* e, int pos) {
 // (FP64, or FP32 on the FP32 environment shadows); same result as the dense
 // product, whose blocks outside the sectors are zero
 void matvec_qn(tn_env* e, const QnMvPlan& qp, const Dom& D, const SparseOp& op, const void* v, void* out,
-               bool f32) {
+               int prec) {
   tn_ctx* c = e->ctx;
+  // FP32 and BF16x3 take FP32 copies of psi and the environments; BF16x3
+  // splits the FP32 fragments into bf16 hi/lo parts inside the block product
+  const bool f32 = prec != TN_PREC_FP64;
   const int pos = D.pos;
   const long chil = D.chil, chir = D.chir;
   const int DD = D.d1 * D.d2;
@@ -911,7 +914,7 @@ void matvec_qn(tn_env* e, const QnMvPlan& qp, const Dom& D, const SparseOp& op, 
     outp = c->workspace(WS_OUT32, (size_t)(D.n + 1) / 2 * 8);
   }
   fill_f64(c, (double*)outp, 0.0, f32 ? (D.n + 1) / 2 : D.n);
-  const Prec P = f32 ? Prec::F32 : Prec::F64;
+  const Prec P = prec == TN_PREC_FP64 ? Prec::F64 : (prec == TN_PREC_FP32 ? Prec::F32 : Prec::BF16X3);
   BsDesc g1;
   g1.A = Lp; g1.lda = chil;
   g1.B = vin; g1.ldb = (long)DD * chir;
@@ -1010,11 +1013,12 @@ void matvec_mpo(tn_env* e, const void* v, void* out, int prec) {
              "environment and wave-function shapes disagree");
   const SparseOp& op = (e->nsite == 2) ? op_two(e, pos) : op_left(e, pos);
   const bool cx = e->cplx();
-  if (e->nq > 0 && e->nsite == 2 && !cx && (prec == TN_PREC_FP64 || prec == TN_PREC_FP32) &&
+  if (e->nq > 0 && e->nsite == 2 && !cx &&
+      (prec == TN_PREC_FP64 || prec == TN_PREC_FP32 || prec == TN_PREC_BF16X3) &&
       (long)e->link_qn[pos - 1].size() == chil && (long)e->link_qn[pos + 1].size() == chir) {
     const QnMvPlan* qp = qn_plan(e, pos);
     if (qp) {
-      matvec_qn(e, *qp, D, op, v, out, prec == TN_PREC_FP32);
+      matvec_qn(e, *qp, D, op, v, out, prec);
       return;
     }
   }

@@ -240,8 +240,9 @@ def _sector_mask(ql, qs1, qs2, qr):
 def test_qn_blocksparse_product_matches_dense(T, model):
     """The QN product (block-sparse GEMMs over the link sectors, uneven blocks)
     against the dense product of the same MPS and MPO without QNs, at every
-    position: FP64 to 1e-12 and FP32 to 2e-6 of max|H phi|; entries outside
-    the sectors exactly zero."""
+    position: FP64 to 1e-12, FP32 to 2e-6 and BF16x3 (the FP32 fragments
+    split into bf16 hi/lo parts in the block product) to 1e-4 of max|H phi|;
+    entries outside the sectors exactly zero."""
     if model == "hubbard":
         N, sq, Ws = 12, [Q.hubbard_qns()] * 12, T.hubbard_mpo(12, 1.0, 4.0)
         psi0 = T.hubbard_occupation_state([1, 2] * 6)
@@ -267,6 +268,40 @@ def test_qn_blocksparse_product_matches_dense(T, model):
         got32 = T.product(env, phi, "fp32")
         assert np.abs(got32 - ref).max() <= 2e-6 * scale, pos
         assert not np.any(got32[~mask])
+        gotb = T.product(env, phi, "bf16x3")
+        errb = np.abs(gotb - ref).max()
+        assert errb <= 1e-4 * scale, pos
+        assert errb > 1e-12 * scale  # really split into bf16 parts
+        assert not np.any(gotb[~mask])
+
+
+@pytest.mark.parametrize("refp", ["fp64", "fp32"])
+def test_qn_dmrg_hubbard_bf16x3_energy(T, refp):
+    """QN Hubbard N = 16, U = 4, (8, 8), the cutoff-limited ramp of the golden
+    run, with the BF16x3 block product and its Lanczos refinement (FP64 or
+    FP32 products): the last two sweeps' energies within 1e-9 per site of
+    the FP64 form (measured 2e-10 and 4e-10 with FP64 refinement); the
+    refinement is one Krylov step more than the FP64 solve, so the
+    unconverged first sweeps follow another path (measured 2.1e-5 and 1e-6,
+    held to 1e-4); every link's sectors allowed."""
+    g = json.loads((GOLDEN / "hubbard_qn_n16.json").read_text())
+    N = g["N"]
+    hs = Q.hubbard_qns()
+    runs = {}
+    for prec in ("fp64", "bf16x3"):
+        env = T.StateEnvs(T.hubbard_occupation_state([1, 2] * (N // 2)), T.hubbard_mpo(N, g["t"], g["U"]),
+                          qns=[hs] * N)
+        params = T.make_dmrg_params(maxdim=g["maxdim"], nsweeps=[1] * len(g["maxdim"]), cutoff=g["cutoff"])
+        print(f"qn hubbard n16 {prec} refine {refp}", flush=True)
+        sw = T.dmrg_(env, params, 2, outputlevel=0, precision=prec, refine_precision=refp)
+        runs[prec] = (np.array(sw.energy), [env.sectors(k) for k in range(N + 1)])
+    dE = np.abs(runs["bf16x3"][0] - runs["fp64"][0]) / N
+    print("qn hubbard n16 bf16x3 |dE/N|", dE, flush=True)
+    assert dE[-2:].max() <= 1e-9
+    assert dE.max() <= 1e-4
+    for k in range(N + 1):
+        for (nu, nd) in runs["bf16x3"][1][k]:
+            assert 0 <= nu <= k and 0 <= nd <= k and nu >= k - N // 2 and nd >= k - N // 2, (k, nu, nd)
 
 
 def test_qn_dmrg_hubbard_n64_u0_closed_form(T):
