@@ -896,7 +896,12 @@ void matvec_qn(tn_env* e, const QnMvPlan& qp, const Dom& D, const SparseOp& op, 
   const int DD = D.d1 * D.d2;
   const int w0 = e->w[pos - 1], w2 = e->w[pos + 1];
   const size_t es = f32 ? 4 : 8;
-  void* T1 = c->workspace(WS_T1, (size_t)w0 * chil * DD * chir * es);
+  // the block GEMM writes only T1's allowed blocks, while a dense operator's
+  // W step (an MFMA product) multiplies every entry of T1: the rest must hold
+  // zeros, not stale workspace bytes (0 * NaN is NaN)
+  const size_t t1w = ((size_t)w0 * chil * DD * chir * es + 7) / 8;
+  void* T1 = c->workspace(WS_T1, t1w * 8);
+  fill_f64(c, (double*)T1, 0.0, (long)t1w);
   void* T2 = c->workspace(WS_T2, (size_t)w2 * chil * DD * chir * es);
   const void* vin = v;
   const void* Lp = e->LR[pos - 1].buf.p;

@@ -120,7 +120,7 @@ def neel_state(N: int, d: int = 2, dtype=np.float64) -> List[np.ndarray]:
 
 
 def spin_coupling_mpo(N: int, couplings, h: Optional[Sequence[float]] = None, compress: bool = True,
-                      tol: float = 1e-13) -> List[np.ndarray]:
+                      tol: float = 1e-13, conserve_sz: bool = False) -> List[np.ndarray]:
     """H = sum_{(i,j)} J_ij S_i . S_j + sum_i h_i Sz_i for arbitrary pairs
     (0-based sites, i != j), as an MPO.
 
@@ -131,7 +131,13 @@ def spin_coupling_mpo(N: int, couplings, h: Optional[Sequence[float]] = None, co
     MPO input (e.g. its J1-J2 cylinder models).  With `compress`, each bond
     is then reduced by an SVD sweep pair (relative tolerance `tol` on the
     singular values of the bond matrices), which keeps H exact to
-    ~tol * |H| and gives the minimal bond dimension of the sum."""
+    ~tol * |H| and gives the minimal bond dimension of the sum.  With
+    `conserve_sz` the compression keeps every bond channel of definite S_z
+    flux (each bond matrix is block-diagonal in the flux, and the QR and SVD
+    run block by block), so the MPO can drive an S_z-conserving state
+    (`StateEnvs(..., qns=[spin_half_qns()] * N)`); the bond dimension is the
+    same, since the minimal rank of a block-diagonal matrix is the sum of its
+    blocks' ranks."""
     I = np.eye(2)
     Sz = np.diag([0.5, -0.5])
     Sp = np.array([[0.0, 1.0], [0.0, 0.0]])
@@ -179,8 +185,84 @@ def spin_coupling_mpo(N: int, couplings, h: Optional[Sequence[float]] = None, co
             W = W[..., 0:1]
         Ws.append(np.ascontiguousarray(W))
     if compress and N > 1:
-        Ws = _compress_mpo(Ws, tol)
+        Ws = _compress_mpo_flux(Ws, tol, [1, -1]) if conserve_sz else _compress_mpo(Ws, tol)
     return Ws
+
+
+def _mpo_channel_flux(Ws: List[np.ndarray], q: Sequence[int]) -> List[np.ndarray]:
+    """Flux of every bond channel of an MPO whose entries W[a, s, t, b] change
+    the flux by q[s] - q[t]: flux[k + 1][b] = flux[k][a] + q[s] - q[t] for
+    every nonzero entry (flux of the left edge 0).  Raises if a channel would
+    carry two fluxes."""
+    fl = [np.zeros(Ws[0].shape[0], dtype=np.int64)]
+    for k, W in enumerate(Ws):
+        nxt = {}
+        for a, s, t, b in zip(*np.nonzero(W)):
+            f = int(fl[-1][a]) + q[s] - q[t]
+            if nxt.setdefault(int(b), f) != f:
+                raise ValueError(f"MPO channel {b} of bond {k + 1} has no definite flux")
+        fl.append(np.array([nxt.get(b, 0) for b in range(W.shape[3])], dtype=np.int64))
+    return fl
+
+
+def _compress_mpo_flux(Ws: List[np.ndarray], tol: float, q: Sequence[int]) -> List[np.ndarray]:
+    """_compress_mpo block by block in the channel flux: the bond matrices are
+    block-diagonal once rows and columns are grouped by flux, so the QR of a
+    column block and the SVD of a row block leave every new channel with one
+    flux; singular values are cut at tol times the bond's largest."""
+    Ws = [w.copy() for w in Ws]
+    N = len(Ws)
+    q = list(q)
+    fl = _mpo_channel_flux(Ws, q)
+    dq = np.subtract.outer(np.array(q), np.array(q))  # q[s] - q[t]
+    for k in range(N - 1):  # left-orthogonalise with a QR per flux block of the right channels
+        wl, d1, d2, wr = Ws[k].shape
+        rowf = (fl[k][:, None, None] + dq[None, :, :]).reshape(-1)
+        M = Ws[k].reshape(wl * d1 * d2, wr)
+        Qs, Rs, nf = [], [], []
+        for f in np.unique(fl[k + 1]):
+            cols = np.nonzero(fl[k + 1] == f)[0]
+            rows = np.nonzero(rowf == f)[0]
+            Qb, Rb = np.linalg.qr(M[np.ix_(rows, cols)])
+            Qf = np.zeros((M.shape[0], Qb.shape[1]))
+            Qf[rows] = Qb
+            Rf = np.zeros((Qb.shape[1], wr))
+            Rf[:, cols] = Rb
+            Qs.append(Qf)
+            Rs.append(Rf)
+            nf += [int(f)] * Qb.shape[1]
+        Q, R = np.hstack(Qs), np.vstack(Rs)
+        Ws[k] = Q.reshape(wl, d1, d2, -1)
+        Ws[k + 1] = np.tensordot(R, Ws[k + 1], axes=([1], [0]))
+        fl[k + 1] = np.array(nf, dtype=np.int64)
+    for k in range(N - 1, 0, -1):  # right-to-left truncated SVD per flux block of the left channels
+        wl, d1, d2, wr = Ws[k].shape
+        colf = (fl[k + 1][None, None, :] - dq[:, :, None]).reshape(-1)
+        M = Ws[k].reshape(wl, d1 * d2 * wr)
+        blocks = []
+        for f in np.unique(fl[k]):
+            rows = np.nonzero(fl[k] == f)[0]
+            cols = np.nonzero(colf == f)[0]
+            U, sv, Vt = np.linalg.svd(M[np.ix_(rows, cols)], full_matrices=False)
+            blocks.append((int(f), rows, cols, U, sv, Vt))
+        smax = max(b[4][0] for b in blocks if b[4].size)
+        Vs, Us, nf = [], [], []
+        for f, rows, cols, U, sv, Vt in blocks:
+            keep = int(np.sum(sv > tol * smax))
+            if keep == 0:
+                continue
+            Vf = np.zeros((keep, M.shape[1]))
+            Vf[:, cols] = Vt[:keep]
+            Uf = np.zeros((wl, keep))
+            Uf[rows] = U[:, :keep] * sv[:keep]
+            Vs.append(Vf)
+            Us.append(Uf)
+            nf += [f] * keep
+        V, Usv = np.vstack(Vs), np.hstack(Us)
+        Ws[k] = V.reshape(-1, d1, d2, wr)
+        Ws[k - 1] = np.tensordot(Ws[k - 1], Usv, axes=([3], [0]))
+        fl[k] = np.array(nf, dtype=np.int64)
+    return [np.ascontiguousarray(w) for w in Ws]
 
 
 def _compress_mpo(Ws: List[np.ndarray], tol: float) -> List[np.ndarray]:
@@ -225,7 +307,9 @@ def cylinder_couplings(Ly: int, Lx: int, J1: float = 1.0, J2: float = 0.0, snake
 
 
 def j1j2_cylinder_mpo(Ly: int, Lx: int, J1: float = 1.0, J2: float = 0.5, snake: bool = True,
-                      compress: bool = True) -> List[np.ndarray]:
+                      compress: bool = True, conserve_sz: bool = False) -> List[np.ndarray]:
     """The J1-J2 Heisenberg cylinder of the wide-MPO benchmark (Ly = 6, Lx = 12,
-    J2 = 0.5 there)."""
-    return spin_coupling_mpo(Ly * Lx, cylinder_couplings(Ly, Lx, J1, J2, snake), compress=compress)
+    J2 = 0.5 there); `conserve_sz` keeps S_z flux per bond channel through the
+    compression."""
+    return spin_coupling_mpo(Ly * Lx, cylinder_couplings(Ly, Lx, J1, J2, snake), compress=compress,
+                             conserve_sz=conserve_sz)

@@ -321,3 +321,80 @@ def test_qn_dmrg_hubbard_n64_u0_closed_form(T):
     assert sw.energy[-1] > e0 - 1e-10
     assert sw.energy[-1] - e0 < 2e-6
     assert env.link_qns(N) == [(N // 2, N // 2)]
+
+def _sz0_ground_energy(N, J):
+    """Lowest eigenvalue of H = sum J_ij S_i . S_j in the S_z = 0 sector
+    (sparse Lanczos on the C(N, N/2) basis states)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    states = [x for x in range(2 ** N) if bin(x).count("1") == N // 2]
+    index = {x: i for i, x in enumerate(states)}
+    rows, cols, vals = [], [], []
+    for i, x in enumerate(states):
+        diag = 0.0
+        for (a, b), v in J.items():
+            ba, bb = (x >> (N - 1 - a)) & 1, (x >> (N - 1 - b)) & 1
+            diag += v * (0.25 if ba == bb else -0.25)
+            if ba != bb:  # S+ S- + S- S+ flips the pair
+                y = x ^ (1 << (N - 1 - a)) ^ (1 << (N - 1 - b))
+                rows.append(index[y])
+                cols.append(i)
+                vals.append(0.5 * v)
+        rows.append(i)
+        cols.append(i)
+        vals.append(diag)
+    H = sp.csr_matrix((vals, (rows, cols)), shape=(len(states), len(states)))
+    return spl.eigsh(H, k=1, which="SA")[0][0]
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp32", 1e-9)])
+def test_qn_dmrg_j1j2_cylinder_sz0_reaches_ed(T, prec, tol):
+    """J1-J2 Heisenberg (J2 = 0.5) on a 4 x 4 cylinder with conserved S_z: the
+    S_z-conserving compressed MPO (w = 14), Neel start, QN two-site DMRG to
+    full bond dimension reaches the S_z = 0 exact-diagonalisation energy, the
+    last link carries total S_z = 0 only, and every link's sectors are
+    reachable."""
+    Ly, Lx = 4, 4
+    N = Ly * Lx
+    J = T.cylinder_couplings(Ly, Lx, 1.0, 0.5)
+    e0 = _sz0_ground_energy(N, J)
+    Ws = T.j1j2_cylinder_mpo(Ly, Lx, 1.0, 0.5, conserve_sz=True)
+    assert max(w.shape[3] for w in Ws[:-1]) == 14
+    sq = [T.spin_half_qns()] * N
+    env = T.StateEnvs(T.neel_state(N), Ws, qns=sq)
+    params = T.make_dmrg_params(maxdim=[32, 64, 128, 256, 256, 256, 256], nsweeps=[1] * 7, cutoff=1e-12)
+    print(f"qn j1j2 4x4 {prec}", flush=True)
+    sw = T.dmrg_(env, params, 2, outputlevel=0, precision=prec)
+    print("qn j1j2 4x4 energies", sw.energy, "ED", e0, "maxchi", sw.maxchi, flush=True)
+    assert abs(sw.energy[-1] - e0) / N < tol
+    assert set(env.sectors(N)) == {(0,)}
+    for k in range(1, N):
+        for (m,) in env.sectors(k):
+            assert abs(m) <= min(k, N - k) and (m - k) % 2 == 0, (k, m)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 1e-4)])
+def test_qn_product_dense_operator_ignores_stale_workspace(T, prec, tol):
+    """With a dense compressed operator (the S_z-conserving J1-J2 MPO) the W
+    step multiplies every entry of T1, of which the block GEMM writes only the
+    allowed blocks: the rest must be zeros.  A dense FP64 product of a NaN
+    vector on the same context first fills the shared workspace with NaN; the
+    QN product that follows still matches the dense product."""
+    Ly, Lx = 4, 4
+    N = Ly * Lx
+    Ws = T.j1j2_cylinder_mpo(Ly, Lx, 1.0, 0.5, conserve_sz=True)
+    sq = [T.spin_half_qns()] * N
+    env = T.StateEnvs(T.neel_state(N), Ws, qns=sq)
+    T.dmrg_(env, T.make_dmrg_params(maxdim=[16, 32], nsweeps=[1, 1], cutoff=1e-10), 2, outputlevel=0)
+    dense = T.StateEnvs(env.psi, Ws)
+    pos = N // 2
+    T.position_(env, pos)
+    T.position_(dense, pos)
+    mask = _sector_mask(env.link_qns(pos - 1), sq[pos - 1], sq[pos], env.link_qns(pos + 1))
+    phi = np.random.default_rng(4).standard_normal(mask.shape) * mask
+    ref = T.product(dense, phi, "fp64")
+    poisoned = T.product(dense, np.full(mask.shape, np.nan), "fp64")
+    assert np.isnan(poisoned).all()
+    got = T.product(env, phi, prec)
+    assert np.isfinite(got).all()
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()

@@ -56,6 +56,46 @@ def test_j1j2_cylinder_mpo(snake):
         assert np.abs(O.mpo_to_dense(Ws) - Hd).max() < 1e-12
 
 
+def _channel_flux_ok(Ws):
+    """Every bond channel carries one 2 S_z flux: flux(b) = flux(a) + q(s) - q(t)."""
+    q = [1, -1]
+    fl = {0: 0}
+    for W in Ws:
+        nxt = {}
+        for a, s, t, b in zip(*np.nonzero(np.abs(W) > 0)):
+            f = fl[int(a)] + q[s] - q[t]
+            if nxt.setdefault(int(b), f) != f:
+                return False
+        fl = nxt
+    return fl == {0: 0}
+
+
+def test_flux_preserving_compression():
+    """The S_z-conserving compression gives the same Hamiltonian and the same
+    bond dimensions as the plain one, with every channel of definite flux; the
+    plain SVD compression mixes the S+, S- and S_z channels."""
+    rng = np.random.default_rng(5)
+    N = 9
+    J = {}
+    for _ in range(14):
+        i, j = rng.choice(N, 2, replace=False)
+        J[(int(min(i, j)), int(max(i, j)))] = float(rng.normal())
+    h = rng.normal(size=N)
+    Hd = _dense_from_couplings(N, J, h)
+    Wp = M.spin_coupling_mpo(N, J, h=h)
+    Wq = M.spin_coupling_mpo(N, J, h=h, conserve_sz=True)
+    assert np.abs(O.mpo_to_dense(Wq) - Hd).max() < 1e-12
+    assert [w.shape for w in Wq] == [w.shape for w in Wp]
+    assert _channel_flux_ok(Wq)
+    assert not _channel_flux_ok(Wp)
+    assert _channel_flux_ok(M.spin_coupling_mpo(N, J, h=h, compress=False))
+    Ly, Lx = 3, 3
+    Wc = M.j1j2_cylinder_mpo(Ly, Lx, 1.0, 0.5, conserve_sz=True)
+    assert _channel_flux_ok(Wc)
+    assert np.abs(O.mpo_to_dense(Wc) - _dense_from_couplings(Ly * Lx, M.cylinder_couplings(Ly, Lx, 1.0, 0.5))
+                  ).max() < 1e-12
+
+
 def test_chain_builder_matches_fsm_heisenberg():
     N = 8
     J = {(i, i + 1): 1.0 for i in range(N - 1)}
