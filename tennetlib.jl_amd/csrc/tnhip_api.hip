@@ -898,10 +898,12 @@ void matvec_qn(tn_env* e, const QnMvPlan& qp, const Dom& D, const SparseOp& op, 
   const size_t es = f32 ? 4 : 8;
   // the block GEMM writes only T1's allowed blocks, while a dense operator's
   // W step (an MFMA product) multiplies every entry of T1: the rest must hold
-  // zeros, not stale workspace bytes (0 * NaN is NaN)
+  // zeros, not stale workspace bytes (0 * NaN is NaN).  A sparse operator's W
+  // step reads, for every allowed T2 entry, only allowed T1 entries (its
+  // nonzero couplings conserve the flux), so it needs no clearing.
   const size_t t1w = ((size_t)w0 * chil * DD * chir * es + 7) / 8;
   void* T1 = c->workspace(WS_T1, t1w * 8);
-  fill_f64(c, (double*)T1, 0.0, (long)t1w);
+  if (op.dense) fill_f64(c, (double*)T1, 0.0, (long)t1w);
   void* T2 = c->workspace(WS_T2, (size_t)w2 * chil * DD * chir * es);
   const void* vin = v;
   const void* Lp = e->LR[pos - 1].buf.p;
