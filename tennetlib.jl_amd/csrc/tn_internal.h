@@ -61,7 +61,12 @@ struct DevBuf {
     p = nullptr;
     bytes = 0;
   }
-  // grow-only; contents are NOT preserved
+  // grow-only; contents are NOT preserved.  A new buffer starts zeroed:
+  // padding and channels that no kernel writes (and that are only ever
+  // multiplied by zero) then hold zeros, not whatever the allocator returns
+  // (memory another context freed, which may encode NaN).  The fill is
+  // complete before the buffer is returned; allocation happens outside any
+  // captured region.
   void ensure(size_t nbytes) {
     if (nbytes <= bytes && p) return;
     release();
@@ -70,6 +75,15 @@ struct DevBuf {
     if (e != hipSuccess) {
       p = nullptr;
       throw Error(TN_E_NOMEM, "hipMalloc(" + std::to_string(nb) + ") failed: " + hipGetErrorString(e));
+    }
+    e = hipMemset(p, 0, nb);
+    // the fill runs on the null stream, which does not order with the
+    // library's non-blocking streams: wait for it before any kernel writes
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      (void)hipFree(p);
+      p = nullptr;
+      throw Error(TN_E_HIP, std::string("hipMemset of a new buffer failed: ") + hipGetErrorString(e));
     }
     bytes = nb;
   }

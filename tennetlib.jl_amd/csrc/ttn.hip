@@ -643,12 +643,12 @@ double next_u01(uint64_t& st) {
   z ^= z >> 31;
   return ((z >> 11) + 0.5) * (1.0 / 9007199254740992.0);
 }
-void normals(uint64_t& st, std::vector<double>& v) {
-  for (size_t i = 0; i < v.size(); i += 2) {
+void normals(uint64_t& st, double* v, size_t n) {
+  for (size_t i = 0; i < n; i += 2) {
     const double u1 = next_u01(st), u2 = next_u01(st);
     const double r = std::sqrt(-2.0 * std::log(u1));
     v[i] = r * std::cos(2.0 * M_PI * u2);
-    if (i + 1 < v.size()) v[i + 1] = r * std::sin(2.0 * M_PI * u2);
+    if (i + 1 < n) v[i + 1] = r * std::sin(2.0 * M_PI * u2);
   }
 }
 
@@ -669,21 +669,24 @@ void expand_random(tn_ttn* t, int node, int slot, long max_expand_dim, double no
   auto grow = [&](TTNNode& X, int s) {
     long pd[3] = {X.dims[0], X.dims[1], X.dims[2]};
     pd[s] = p;
-    std::vector<double> P((size_t)(pd[0] * pd[1] * pd[2]));
-    normals(t->rng, P);
+    const size_t np = (size_t)(pd[0] * pd[1] * pd[2]);
+    const double xn = std::sqrt(node_norm2(t, X));  // synchronises the stream
+    // the padding is drawn into the context's pinned staging buffer (free:
+    // the stream is idle) and uploaded from there, not from pageable memory
+    double* P = (double*)c->host_stage(np * sizeof(double));
+    normals(t->rng, P, np);
     double pn = 0.0;
-    for (double v : P) pn += v * v;
-    const double xn = std::sqrt(node_norm2(t, X));
+    for (size_t i = 0; i < np; ++i) pn += P[i] * P[i];
     const double sc = pn > 0.0 ? noise * xn / std::sqrt(pn) : 0.0;
-    for (double& v : P) v *= sc;
-    const size_t pb = P.size() * sizeof(double);
+    for (size_t i = 0; i < np; ++i) P[i] *= sc;
+    const size_t pb = np * sizeof(double);
     double* Pd = (double*)c->workspace(WS_TTN_P, pb);
-    TN_HIP(hipMemcpyAsync(Pd, P.data(), pb, hipMemcpyHostToDevice, c->stream));
+    TN_HIP(hipMemcpyAsync(Pd, P, pb, hipMemcpyHostToDevice, c->stream));
     long nd[3] = {X.dims[0], X.dims[1], X.dims[2]};
     nd[s] += p;
     X.spare.ensure((size_t)(nd[0] * nd[1] * nd[2]) * sizeof(double));
     concat3_f64(c, X.A.as<double>(), X.dims, s, Pd, p, X.spare.as<double>());
-    TN_HIP(hipStreamSynchronize(c->stream));  // the host padding goes out of scope
+    TN_HIP(hipStreamSynchronize(c->stream));  // the staging buffer is reused
     std::swap(X.A, X.spare);
     for (int i = 0; i < 3; ++i) X.dims[i] = nd[i];
     X.slot[s].env_ok = false;
