@@ -126,8 +126,10 @@ __global__ __launch_bounds__(256) void wstep_mfma_kernel(int G, int Gp, int D, i
                                                          long in_sS, T* __restrict__ out, long out_sg,
                                                          long out_sA, long out_sS, const T* __restrict__ opd,
                                                          int Mp, int Kp, unsigned short* __restrict__ ohi,
-                                                         unsigned short* __restrict__ olo) {
+                                                         unsigned short* __restrict__ olo, int vecin) {
   constexpr bool F64 = sizeof(T) == 8;
+  constexpr int VN = 16 / (int)sizeof(T);  // elements per 16-byte vector
+  typedef T vec_t __attribute__((ext_vector_type(VN)));
   extern __shared__ unsigned char smem_raw[];
   const int M = Gp * Dp, K = G * D;
   const int lda = Kp + 1;
@@ -140,14 +142,26 @@ __global__ __launch_bounds__(256) void wstep_mfma_kernel(int G, int Gp, int D, i
     const int m = t / Kp, k = t % Kp;
     Aop[m * lda + k] = (m < M && k < K) ? opd[m * K + k] : T(0);
   }
-  for (int t = tid; t < Kp * WM_NB; t += 256) {
-    const int i = t / WM_NB, col = t % WM_NB;
-    T v = T(0);
-    if (i < K && B0 + col < nB) {
-      const int g = i / D, S = i % D;
-      v = in[g * in_sg + Ai * in_sA + S * in_sS + B0 + col];
+  if (vecin) {  // 16-byte loads: strides, nB and the input base are whole vectors
+    for (int t = tid; t < Kp * (WM_NB / VN); t += 256) {
+      const int i = t / (WM_NB / VN), col = (t % (WM_NB / VN)) * VN;
+      vec_t v = (vec_t)(T(0));
+      if (i < K && B0 + col < nB) {
+        const int g = i / D, S = i % D;
+        v = *reinterpret_cast<const vec_t*>(in + g * in_sg + Ai * in_sA + S * in_sS + B0 + col);
+      }
+      *reinterpret_cast<vec_t*>(slab + i * WM_NB + col) = v;
     }
-    slab[i * WM_NB + col] = v;
+  } else {
+    for (int t = tid; t < Kp * WM_NB; t += 256) {
+      const int i = t / WM_NB, col = t % WM_NB;
+      T v = T(0);
+      if (i < K && B0 + col < nB) {
+        const int g = i / D, S = i % D;
+        v = in[g * in_sg + Ai * in_sA + S * in_sS + B0 + col];
+      }
+      slab[i * WM_NB + col] = v;
+    }
   }
   __syncthreads();
   if constexpr (!F64) {
@@ -636,24 +650,29 @@ static void wstep_launch(tn_ctx* c, bool fp32, bool split, const WStepDesc& w, u
     const size_t es = fp32 ? sizeof(float) : sizeof(double);
     const size_t mshm = ((size_t)Mp * (Kp + 1) + (size_t)Kp * WM_NB) * es;
     if (mshm <= 160 * 1024 && Mp <= 128) {
-      dim3 grid((unsigned)((w.nB + WM_NB - 1) / WM_NB), (unsigned)w.nA);
+      const long nbt = (w.nB + WM_NB - 1) / WM_NB;
+      const int vn = fp32 ? 4 : 2;
+      auto whole = [&](long x) { return x % vn == 0; };
+      const int vecin = whole(w.nB) && whole(w.in_sg) && whole(w.in_sA) && whole(w.in_sS) &&
+                        (uintptr_t)w.in % 16 == 0;
+      dim3 grid((unsigned)nbt, (unsigned)w.nA);
       if (split) {
         raise_lds_limit((const void*)wstep_mfma_kernel<float, true>, mshm);
         hipLaunchKernelGGL((wstep_mfma_kernel<float, true>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
                            w.Dp, w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)nullptr, w.out_sg,
-                           w.out_sA, w.out_sS, w.op->d32.as<float>(), Mp, Kp, hi, lo);
+                           w.out_sA, w.out_sS, w.op->d32.as<float>(), Mp, Kp, hi, lo, vecin);
       } else if (fp32) {
         raise_lds_limit((const void*)wstep_mfma_kernel<float, false>, mshm);
         hipLaunchKernelGGL((wstep_mfma_kernel<float, false>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
                            w.Dp, w.nB, (const float*)w.in, w.in_sg, w.in_sA, w.in_sS, (float*)w.out, w.out_sg,
                            w.out_sA, w.out_sS, w.op->d32.as<float>(), Mp, Kp, (unsigned short*)nullptr,
-                           (unsigned short*)nullptr);
+                           (unsigned short*)nullptr, vecin);
       } else {
         raise_lds_limit((const void*)wstep_mfma_kernel<double, false>, mshm);
         hipLaunchKernelGGL((wstep_mfma_kernel<double, false>), grid, dim3(256), mshm, c->stream, w.G, w.Gp, w.D,
                            w.Dp, w.nB, (const double*)w.in, w.in_sg, w.in_sA, w.in_sS, (double*)w.out, w.out_sg,
                            w.out_sA, w.out_sS, w.op->d64.as<double>(), Mp, Kp, (unsigned short*)nullptr,
-                           (unsigned short*)nullptr);
+                           (unsigned short*)nullptr, vecin);
       }
       TN_HIP(hipGetLastError());
       return;
