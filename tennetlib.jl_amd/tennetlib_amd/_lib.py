@@ -3,6 +3,9 @@
 The shared library is loaded from the package directory (built in-tree by
 ``tennetlib.jl_amd/build.py``).  There is no CPU fallback: if the library is
 missing or the device is not a gfx950 GPU, calls raise ``TNHIPError``.
+
+``TNHIP_LIB=<path>`` loads another build of the same ABI instead, such as the
+host-instrumented ``libtnhip_hardened.so`` (``build.py --hardened``).
 """
 
 from __future__ import annotations
@@ -12,7 +15,7 @@ import os
 from pathlib import Path
 
 PKG_ROOT = Path(__file__).resolve().parent.parent
-LIB_PATH = PKG_ROOT / "libtnhip.so"
+LIB_PATH = Path(os.environ["TNHIP_LIB"]).resolve() if os.environ.get("TNHIP_LIB") else PKG_ROOT / "libtnhip.so"
 
 TN_OK = 0
 TN_E_INVALID = -1
