@@ -73,3 +73,20 @@ def test_update_position_rejects_noise_with_reverse_step():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(L.TNHIPError, match="no CPU fallback"):
         L.load(tmp_path / "libtnhip.so")
+
+
+def test_tnhip_lib_selects_another_build(tmp_path):
+    """TNHIP_LIB points the binding at another build of the same ABI (such as the
+    host-instrumented one); a path that does not exist fails loudly on load."""
+    import os
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, sys.argv[1]); from tennetlib_amd import _lib as L\n"
+            "print(L.LIB_PATH)\n"
+            "try:\n    L.load()\nexcept L.TNHIPError as e:\n    print('refused', 'no CPU fallback' in str(e))\n")
+    want = tmp_path / "libtnhip_other.so"
+    env = dict(os.environ, TNHIP_LIB=str(want))
+    out = subprocess.run([sys.executable, "-c", code, str(ROOT / "tennetlib.jl_amd")], env=env,
+                         capture_output=True, text=True, check=True).stdout.split("\n")
+    assert out[0] == str(want.resolve())
+    assert out[1] == "refused True"
