@@ -389,48 +389,56 @@ def hubbard_qn_run(T, N, ramp, cutoff, kd, mi, prec):
     return out
 
 
-def ttn_instances(T, count, seed=5, total=256):
+def ttn_instances(T, count, seed=5, total=256, batch_count=0, workers=4):
     """Many-instance TTN ground states: `total` random-field Heisenberg chains
     (h_i ~ U[-2, 2], N ~ U{16..64}, numpy default_rng(seed)), each a
     default binary-tree TTN from default_randomTTN(N, 12) optimised with the
     reference test's schedule (maxdim 128, 5 sweeps, cutoff 1e-14, noise
     1e-2, noisedecay 5, disable_noise_after 2).  The first `count` instances
-    run one after another on the library's stream; the projection to all
-    instances scales the wall time by the sum of N."""
+    run one after another on the library's stream; then the first
+    `batch_count` run instance-parallel over `workers` processes sharing the
+    GPU (optimize_instances, worker start-up included in the wall time), and
+    the instances both runs share are compared.  Projections to all
+    instances scale the wall time by the sum of N."""
+    from tennetlib_amd.batch import ground_state, optimize_instances
     rng = np.random.default_rng(seed)
     Ns = [int(n) for n in rng.integers(16, 65, total)]
     hs = [rng.uniform(-2, 2, n) for n in Ns]
-    params = T.make_optimize_params(maxdim=[128], nsweeps=[5], cutoff=1e-14, noise=1e-2, noisedecay=5,
-                                    disable_noise_after=2)
+    specs = [(Ns[i], hs[i], i) for i in range(total)]
+    pkw = dict(maxdim=[128], nsweeps=[5], cutoff=1e-14, noise=1e-2, noisedecay=5, disable_noise_after=2)
     res = []
     ctx = T.default_context()
     ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
     t0 = time.perf_counter()
     for i in range(count):
-        N = Ns[i]
-        os_ = T.OpStrings()
-        for j in range(1, N):
-            os_ += (1.0, "Sz", j, "Sz", j + 1)
-            os_ += (0.5, "S+", j, "S-", j + 1)
-            os_ += (0.5, "S-", j, "S+", j + 1)
-        for j in range(1, N + 1):
-            os_ += (float(hs[i][j - 1]), "Sz", j)
-        psi0 = T.default_randomTTN(N, 12, seed=i)
-        env = T.StateEnvsTTN(psi0, T.CouplingModel(os_, N))
-        sw = T.optimize_(env, params, T.default_sweeppath(psi0), outputlevel=0, seed=i)
-        env.close()
-        res.append((sw.energy[-1], sw.maxchi[-1]))
+        res.append(ground_state(*specs[i], pkw))
     ctx.check(ctx.lib.tn_ctx_synchronize(ctx.h))
     dt = time.perf_counter() - t0
     done_n = sum(Ns[:count])
-    return {"workload": f"{total} random-field Heisenberg chains, N~U{{16..64}}, h~U[-2,2] (seed {seed}); "
-                        f"binary-tree TTN from default_randomTTN(N, 12), maxdim 128, 5 sweeps, noise 1e-2 "
-                        f"(decay 5, off after 2), cutoff 1e-14; FP64",
-            "instances_timed": count, "wall_s": round(dt, 3), "instances_per_s": round(count / dt, 4),
-            "site_sweeps_per_s": round(5 * done_n / dt, 1),
-            "projected_s_for_all": round(dt * sum(Ns) / done_n, 1),
-            "energy_per_site": [round(res[i][0] / Ns[i], 10) for i in range(count)],
-            "maxchi": [res[i][1] for i in range(count)]}
+    out = {"workload": f"{total} random-field Heisenberg chains, N~U{{16..64}}, h~U[-2,2] (seed {seed}); "
+                       f"binary-tree TTN from default_randomTTN(N, 12), maxdim 128, 5 sweeps, noise 1e-2 "
+                       f"(decay 5, off after 2), cutoff 1e-14; FP64",
+           "instances_timed": count, "wall_s": round(dt, 3), "instances_per_s": round(count / dt, 4),
+           "site_sweeps_per_s": round(5 * done_n / dt, 1),
+           "projected_s_for_all": round(dt * sum(Ns) / done_n, 1),
+           "energy_per_site": [round(res[i][0][-1] / Ns[i], 10) for i in range(count)],
+           "maxchi": [res[i][1][-1] for i in range(count)]}
+    if batch_count > 0:
+        t0 = time.perf_counter()
+        rb = optimize_instances(specs[:batch_count], pkw, workers=workers)
+        dtb = time.perf_counter() - t0
+        nb = sum(Ns[:batch_count])
+        both = min(count, batch_count)
+        out["batched"] = {
+            "workers": min(workers, batch_count), "instances": batch_count, "wall_s": round(dtb, 3),
+            "instances_per_s": round(batch_count / dtb, 4), "site_sweeps_per_s": round(5 * nb / dtb, 1),
+            "projected_s_for_all": round(dtb * sum(Ns) / nb, 1),
+            "speedup_vs_sequential": round((5 * nb / dtb) / (5 * done_n / dt), 3),
+            "compared_instances": both,
+            "max_abs_dE_per_site_vs_sequential": max(
+                (abs(rb[i][0][-1] - res[i][0][-1]) / Ns[i] for i in range(both)), default=None),
+            "maxchi_equal": all(rb[i][1] == res[i][1] for i in range(both))}
+    return out
 
 
 def eig_probe(T, p):
@@ -533,6 +541,9 @@ def main():
     ap.add_argument("--mv-only", action="store_true", help="matvec microbenchmark only")
     ap.add_argument("--tdvp-steps", type=int, default=40, help="TDVP steps of the XXZ N=100 run (0: skip)")
     ap.add_argument("--ttn-instances", type=int, default=8, help="TTN instances timed of the 256 (0: skip)")
+    ap.add_argument("--ttn-batch", type=int, default=32,
+                    help="TTN instances run instance-parallel over --ttn-workers processes (0: skip)")
+    ap.add_argument("--ttn-workers", type=int, default=4, help="worker processes of the TTN batch")
     ap.add_argument("--ttn-only", action="store_true", help="only the many-instance TTN line")
     ap.add_argument("--hubbard-maxdim", type=int, default=2048,
                     help="last maxdim of the QN Hubbard N=64 ramp (0: skip the line)")
@@ -568,7 +579,9 @@ def main():
         print(json.dumps({"eig_probe": eig_probe(T, args.eig_only)}), flush=True)
         return
     if args.ttn_only:
-        print(json.dumps({"ttn_random_field_batch": ttn_instances(T, max(1, args.ttn_instances))}), flush=True)
+        print(json.dumps({"ttn_random_field_batch": ttn_instances(T, max(1, args.ttn_instances),
+                                                                  batch_count=args.ttn_batch,
+                                                                  workers=args.ttn_workers)}), flush=True)
         return
     if args.hubbard_only:
         print(json.dumps({"hubbard_qn_n64": hubbard_qn_line(T, args)}), flush=True)
@@ -634,8 +647,10 @@ def main():
 
     ttn = None
     if args.ttn_instances > 0:
-        ttn = ttn_instances(T, args.ttn_instances)
+        ttn = ttn_instances(T, args.ttn_instances, batch_count=args.ttn_batch, workers=args.ttn_workers)
         log(f"ttn: {ttn['instances_timed']} instances in {ttn['wall_s']} s")
+        if "batched" in ttn:
+            log(f"ttn batched: {ttn['batched']}")
 
     hub = hubbard_qn_line(T, args) if args.hubbard_maxdim > 0 else None
 

@@ -12,6 +12,7 @@ from .core import (Context, DeviceArray, DMRGParams, StateEnvs, SweepData, TDVPE
                    default_context, dmrg, dmrg1, dmrg2, dmrg_, dynamic_fullsweep_, eig_solver, exp_solver,
                    krylov_extend_, make_dmrg_params, nsite, orthogonalize_, set_nsite_, tdvpsweep_)
 from .couplingmodel import coupling_mpo, mpo_sum_ids  # noqa: F401
+from .batch import optimize_instances  # noqa: F401
 from .ttn import (TTN, CouplingModel, OpStrings, OptimizeParamsTTN, StateEnvsTTN, SweepDataTTN,  # noqa: F401
                   default_graph_sitenodes, default_randomTTN, default_sweeppath, find_eccentric_central_node,
                   make_optimize_params, maxlinkdim, nextnode_in_path, optimize, optimize_, randomTTN,

@@ -264,3 +264,38 @@ def test_ttn_excited_state_reference_settings(T):
     assert abs(ov) < 1e-5
     with pytest.raises(ValueError, match="should be > 0.0"):
         T.StateEnvsTTN(psi0, H, [gs], weight=0.0)
+
+
+def test_ttn_instances_in_worker_processes_match_sequential(T):
+    """Instance-parallel runs (optimize_instances: spawned worker processes
+    sharing the GPU, each with its own library context) against the same
+    instances run one after another in this process, for the noiseless and
+    the noisy schedule (noise 1e-2, decay 5, off after 2; cutoff 1e-14; 5
+    sweeps), maxdim 64.  Noiseless (4 sweeps at the initial bond dimension
+    12): the same energies to 1e-12 per site and the same bond dimensions.
+    Noisy: energies to 1e-9 per site (the cutoff may split a rounding-level
+    spectrum differently, so bond dimensions are not compared); the noisy
+    N = 12 instance meets exact diagonalisation to 1e-9."""
+    from tennetlib_amd.batch import ground_state, optimize_instances
+    rng = np.random.default_rng(11)
+    specs = [(N, rng.uniform(-2, 2, N), s) for s, N in enumerate([12, 16, 20, 24])]
+    quiet = dict(maxdim=[64], nsweeps=[4], cutoff=0.0)
+    noisy = dict(maxdim=[64], nsweeps=[5], cutoff=1e-14, noise=1e-2, noisedecay=5, disable_noise_after=2)
+    runs = {}
+    for name, pkw, tol in (("noiseless", quiet, 1e-12), ("noisy", noisy, 1e-9)):
+        print(f"ttn instances {name}: sequential", flush=True)
+        seq = [ground_state(N, h, s, pkw) for N, h, s in specs]
+        print(f"ttn instances {name}: 2 worker processes", flush=True)
+        par = optimize_instances(specs, pkw, workers=2)
+        dE = [abs(a[0][-1] - b[0][-1]) / N for a, b, (N, _, _) in zip(seq, par, specs)]
+        exact = all(a == b for a, b in zip(seq, par))
+        print(f"ttn instances {name}: max |dE/N| {max(dE):.3e}, identical {exact}", flush=True)
+        assert max(dE) <= tol
+        if name == "noiseless":
+            assert [a[1] for a in seq] == [b[1] for b in par]
+        runs[name] = par
+    # the noisy schedule grows the bonds from 12; at N = 12 bond dimension 64
+    # holds the ground state exactly: exact diagonalisation
+    N0, h0, _ = specs[0]
+    e0 = spla.eigsh(dense_h(TO.heisenberg_terms(N0, 1.0, h0)), k=1, which="SA", tol=1e-12)[0][0]
+    assert abs(runs["noisy"][0][0][-1] - e0) <= 1e-9

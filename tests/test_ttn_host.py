@@ -62,3 +62,19 @@ def test_opstrings_and_params_validation():
     p = T.make_optimize_params(maxdim=[128], nsweeps=[5], cutoff=1e-14, noise=1e-2, noisedecay=5,
                                disable_noise_after=2)
     assert p.noise == [1e-2] and p.noisedecay == [5.0] and p.disable_noise_after == [2]
+
+
+def test_optimize_instances_host_paths():
+    """The instance-parallel runner: an empty batch returns nothing, and an
+    instance that fails its host-side validation (here a schedule whose maxdim
+    and nsweeps lengths differ, make_optimize_params' size check) raises the
+    same error in the caller, both in-process and from a spawned worker, after
+    the pool has shut down."""
+    from tennetlib_amd.batch import optimize_instances
+    assert optimize_instances([], dict(maxdim=[16], nsweeps=[1])) == []
+    bad = dict(maxdim=[16, 32], nsweeps=[1])
+    spec = [(8, np.zeros(8), 0)]
+    with pytest.raises(ValueError, match="Size mismatch"):
+        optimize_instances(spec, bad, workers=1)
+    with pytest.raises(ValueError, match="Size mismatch"):
+        optimize_instances(spec * 2, bad, workers=2)
