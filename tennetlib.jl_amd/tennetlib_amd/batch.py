@@ -23,9 +23,9 @@ from typing import Dict, List, Sequence, Tuple
 import numpy as np
 
 # processes on the one GPU, each holding a context.  Measured on one MI355X
-# with 32 random-field chains (N 16..64, maxdim 128): 4 workers 3.49
-# instances/s, 6 workers 2.35, 8 workers 1.42, one process 0.94; past four
-# processes their hardware queues oversubscribe the GPU.
+# with 32 random-field chains (N 16..64, maxdim 128), one run per setting on
+# separate machines: 3 workers 1.48 instances/s, 4 workers 2.29 and 3.49,
+# 5 workers 1.86, 6 workers 2.35, 8 workers 1.42 and 1.53; one process 0.94.
 DEFAULT_WORKERS = 4
 MAX_WORKERS = 8
 
@@ -61,6 +61,16 @@ def ground_state(N: int, h: Sequence[float], seed: int, params: Dict) -> Tuple[L
     return [float(e) for e in sw.energy], [int(c) for c in sw.maxchi]
 
 
+def _init_worker():
+    """One BLAS thread per worker: the workers' host threads issue the GPU work,
+    and several processes' BLAS pools on the same cores would compete with them."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:
+        return
+    threadpool_limits(limits=1)
+
+
 def _run(task):
     i, N, h, seed, params = task
     return i, ground_state(N, h, seed, params)
@@ -85,7 +95,7 @@ def optimize_instances(specs: Sequence[Tuple[int, Sequence[float], int]], params
     nw = min(int(workers), MAX_WORKERS, len(tasks))
     order = sorted(tasks, key=lambda t: -t[1])  # largest N first: the slowest instances start early
     out: List = [None] * len(tasks)
-    pool = mp.get_context("spawn").Pool(nw)
+    pool = mp.get_context("spawn").Pool(nw, initializer=_init_worker)
     try:
         for i, res in pool.imap_unordered(_run, order, chunksize=1):
             out[i] = res
