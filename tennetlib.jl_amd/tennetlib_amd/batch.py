@@ -18,6 +18,7 @@ generators), not on which worker ran it.
 from __future__ import annotations
 
 import multiprocessing as mp
+from concurrent.futures import FIRST_EXCEPTION, ProcessPoolExecutor, wait
 from typing import Dict, List, Sequence, Tuple
 
 import numpy as np
@@ -71,38 +72,50 @@ def _init_worker():
     threadpool_limits(limits=1)
 
 
-def _run(task):
-    i, N, h, seed, params = task
-    return i, ground_state(N, h, seed, params)
+def _call(i, fn, args):
+    return i, fn(*args)
+
+
+def map_instances(fn, args_list: Sequence[tuple], workers: int = DEFAULT_WORKERS, size=None) -> List:
+    """fn(*args) for every tuple of ``args_list``, over ``workers`` spawned
+    processes (at most MAX_WORKERS; <= 1: in this process, in order).  ``fn``
+    must be importable by name (a module-level function).  ``size(args)``, if
+    given, orders the submissions largest first, so the slowest instances start
+    early.  Results come back in the order of ``args_list``.  The first failure
+    (an instance's exception, or BrokenProcessPool if a worker process died)
+    is raised here once the pool has shut down; instances not yet started are
+    cancelled."""
+    args_list = [tuple(a) for a in args_list]
+    if not args_list:
+        return []
+    if workers <= 1:
+        return [fn(*a) for a in args_list]
+    nw = min(int(workers), MAX_WORKERS, len(args_list))
+    idx = list(range(len(args_list)))
+    if size is not None:
+        idx.sort(key=lambda i: -size(args_list[i]))
+    out: List = [None] * len(args_list)
+    with ProcessPoolExecutor(max_workers=nw, mp_context=mp.get_context("spawn"), initializer=_init_worker) as ex:
+        futs = [ex.submit(_call, i, fn, args_list[i]) for i in idx]
+        done, pending = wait(futs, return_when=FIRST_EXCEPTION)
+        for f in pending:
+            f.cancel()
+        for f in futs:
+            if f.done() and not f.cancelled() and f.exception() is not None:
+                raise f.exception()
+        for f in done:
+            i, res = f.result()
+            out[i] = res
+    return out
 
 
 def optimize_instances(specs: Sequence[Tuple[int, Sequence[float], int]], params: Dict,
                        workers: int = DEFAULT_WORKERS) -> List[Tuple[List[float], List[int]]]:
-    """Ground states of independent random-field chains.
+    """Ground states of independent random-field chains (ground_state per
+    instance, map_instances over the workers, largest N first).
 
     ``specs``: (N, h, seed) per instance; ``params``: keyword arguments of
     make_optimize_params.  ``workers`` <= 1 runs them in this process on the
-    default context; otherwise a pool of ``workers`` spawned processes (at most
-    MAX_WORKERS) takes the instances largest first, one at a time, as workers
-    free up.  Results come back in the order of ``specs``.  A failure in any
-    instance is raised here after the pool has been shut down."""
-    tasks = [(i, int(N), np.asarray(h, dtype=np.float64), int(seed), dict(params))
-             for i, (N, h, seed) in enumerate(specs)]
-    if not tasks:
-        return []
-    if workers <= 1:
-        return [ground_state(N, h, seed, p) for _, N, h, seed, p in tasks]
-    nw = min(int(workers), MAX_WORKERS, len(tasks))
-    order = sorted(tasks, key=lambda t: -t[1])  # largest N first: the slowest instances start early
-    out: List = [None] * len(tasks)
-    pool = mp.get_context("spawn").Pool(nw, initializer=_init_worker)
-    try:
-        for i, res in pool.imap_unordered(_run, order, chunksize=1):
-            out[i] = res
-        pool.close()
-    except BaseException:
-        pool.terminate()
-        raise
-    finally:
-        pool.join()
-    return out
+    default context.  Results come back in the order of ``specs``."""
+    args = [(int(N), np.asarray(h, dtype=np.float64), int(seed), dict(params)) for N, h, seed in specs]
+    return map_instances(ground_state, args, workers, size=lambda a: a[0])

@@ -78,3 +78,19 @@ def test_optimize_instances_host_paths():
         optimize_instances(spec, bad, workers=1)
     with pytest.raises(ValueError, match="Size mismatch"):
         optimize_instances(spec * 2, bad, workers=2)
+
+
+def test_map_instances_order_and_worker_death():
+    """map_instances returns results in input order whatever order the workers
+    finish in (submissions sorted by size), and a worker process that dies
+    surfaces as BrokenProcessPool in the caller instead of a hang."""
+    import operator
+    import os
+    from concurrent.futures.process import BrokenProcessPool
+    from tennetlib_amd.batch import map_instances
+    args = [(i, 3) for i in range(7)]
+    want = [i * 3 for i in range(7)]
+    assert map_instances(operator.mul, args, workers=1) == want
+    assert map_instances(operator.mul, args, workers=3, size=lambda a: a[0]) == want
+    with pytest.raises(BrokenProcessPool):
+        map_instances(os._exit, [(3,), (3,)], workers=2)
